@@ -1,0 +1,188 @@
+/*
+ * seisfd.h — C-ABI of the MI355X acoustic finite-difference engine.
+ *
+ * Every entry point is plain C: integers, floats and host pointers, no C++
+ * or torch types, so Fortran (iso_c_binding), C and Python (ctypes) callers
+ * bind to it directly.  All arrays are column-major with depth (z) fastest,
+ * i.e. the layout of the reference's `vel(nz,nx)` and `trace(nt,ntr)`.
+ *
+ * Return codes: 0 on success, a negative SFD_E* code on failure (the message
+ * is available from sfd_last_error()).  Nothing aborts the process; the
+ * reference calls `erexit` instead (libs/libwave/ReadSouVelRho.f90:54,385,675).
+ *
+ * Threading: one handle per host thread.  A handle owns its device buffers,
+ * its HIP stream and its captured graphs; the caller owns every host buffer.
+ */
+#ifndef SEISFD_H
+#define SEISFD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFD_OK            0
+#define SFD_EINVAL       -1   /* bad argument / geometry                      */
+#define SFD_ENOMEM       -2   /* device or host allocation failed             */
+#define SFD_EHIP         -3   /* HIP runtime error                             */
+#define SFD_EUNSUPPORTED -4   /* option outside the implemented subset        */
+#define SFD_ESTATE       -5   /* call order (e.g. no model set)               */
+#define SFD_EUNSTABLE    -6   /* Courant check failed (ReadSouVelRho.f90:629-675) */
+#define SFD_ENOTFINITE   -7   /* NaN/Inf met in a misfit or wavefield guard   */
+
+/* Snapshot storage for RTM / FWI source wavefields. */
+#define SFD_SNAP_FP32     0   /* reference behaviour: FP32 copies           */
+#define SFD_SNAP_FP16S    1   /* FP16 with one max|S| scale per snapshot     */
+
+/* Receiver extraction semantics (libs/libwave/Extraction.f90). */
+#define SFD_EXTRACT_SINC_V2    0  /* Extraction_array_sinc: gain v^2 (:109-134)   */
+#define SFD_EXTRACT_NEAREST    1  /* Extraction_array_simple_afwi_noomp (:71-106) */
+#define SFD_EXTRACT_SINC_LSRTM 2  /* Extraction_array_LSRTM_sinc, gain 1/(dz dx) (:217-246) */
+#define SFD_EXTRACT_SINC_UNIT  3  /* Extraction_array_sinc_afwi: gain 1 (:137-162) */
+
+/*
+ * Grid / run parameters.  Mirrors the fields of the reference's
+ * GeneralParam (libs/libwave/GeneralParamTypes.f90:9-48) that the 2D
+ * constant-density path reads, plus the model header (n1,n2,o1,o2,d1,d2).
+ */
+typedef struct sfd_grid {
+    int32_t nz, nx;          /* model samples: depth (fast axis), lateral     */
+    float   oz, ox;          /* model origin (m)                              */
+    float   dz, dx;          /* spacing (m)                                   */
+    float   dt;              /* time step (s)                                 */
+    int32_t ntaper;          /* absorbing pad per side (cells), >= 10         */
+    int32_t lsinc;           /* sinc interpolator length (7)                  */
+    int32_t snapi;           /* snapshot interval (steps)                     */
+    float   aperture_x;      /* migration aperture (m), ExtractPadModel.f90:55 */
+    int32_t rec_type;        /* 0 only (no receiver ghost)                    */
+    int32_t shot_type;       /* 0 only (no source ghost)                      */
+    int32_t surf_type;       /* 0 only (absorbing top, Boundary.f90:1623)      */
+} sfd_grid;
+
+/*
+ * One shot: a source, its wavelet and its receiver spread.  Replaces the
+ * reference's sourcevec/datavec TraceSpace arrays
+ * (libs/libwave/DataSpaceTypes.f90:110-117): coord(1)=z, coord(2)=x.
+ */
+typedef struct sfd_shot {
+    float        sz, sx;     /* source depth / lateral position (m)           */
+    int32_t      nt;         /* record length (samples at dt)                 */
+    int32_t      ntr;        /* receivers                                     */
+    const float *wavelet;    /* nt samples                                    */
+    const float *rz;         /* ntr receiver depths (m)                       */
+    const float *rx;         /* ntr receiver lateral positions (m)            */
+} sfd_shot;
+
+/* FWI gradient post-processing options (compute_function_gradient.f90:332-343). */
+typedef struct sfd_fwi_opts {
+    float   illupow;         /* illumination power, Inversion_types.f90:60    */
+    int32_t rect_z, rect_x;  /* triangle smoothing half-lengths (cells)       */
+    int32_t do_z, do_x;      /* smoothing switches (Smoothing.f90:57-61)      */
+    int32_t snap_mode;       /* SFD_SNAP_*                                    */
+} sfd_fwi_opts;
+
+typedef struct sfd_ctx *sfd_handle;
+
+/* ---- lifetime (replaces propagator_acoustic's per-call allocation,
+ *      libs/libwave/Propagator.f90:144-160,261-262) ------------------------ */
+int  sfd_create(sfd_handle *h, const sfd_grid *g);
+int  sfd_destroy(sfd_handle h);
+const char *sfd_last_error(sfd_handle h);
+int  sfd_device_info(char *name, int len, int *cu_count, size_t *hbm_bytes);
+
+/*
+ * Model: unpadded vel(nz,nx) in m/s, column-major.  Windowing and
+ * edge-replication padding are done on the device per shot, as
+ * read_window_vel / copy_window_vel_gath + model_pad do
+ * (libs/libwave/ExtractPadModel.f90:87-285,395-507;
+ *  libs/libwave/ReadSouVelRho.f90:545-596).  Courant check as vel_check
+ * (ReadSouVelRho.f90:603-675).  vel_bwd may be NULL (the reference's optional
+ * `vel2` used by the RTM backward pass, to_disk_to_memory_RTM.f90:438).
+ */
+int  sfd_set_model(sfd_handle h, const float *vel, const float *vel_bwd);
+
+/*
+ * Forward modeling of nshots shots (Acoustic_modeling_sep.f90:80-88 for one
+ * shot; to_disk_to_memory_AMOD.f90:57-65 with extract=NEAREST).  Traces are
+ * written per shot, each shot's block (nt, ntr) time-fastest, blocks
+ * concatenated in shot order.  Source injection is Injection_sinc
+ * (Injection.f90:151-194,402-456).
+ */
+int  sfd_forward(sfd_handle h, const sfd_shot *shots, int nshots,
+                 float *traces_out, int extract_kind);
+
+/*
+ * RTM of one or more shots (to_disk_to_memory_RTM.f90:358-492): forward with
+ * snapshots every snapi steps, reference-style backward with the adjoint
+ * stencil (FDderivatives.f90:847-891) injecting traces_in with
+ * Injection_sinc, zero-lag imaging and illumination
+ * (Imaging.f90:228-275).  image/illum are full-model (nz,nx) and accumulated
+ * with += through the nearest-index scatter of mod_copy_image
+ * (ExtractPadModel.f90:662-695).
+ */
+int  sfd_rtm(sfd_handle h, const sfd_shot *shots, int nshots,
+             const float *traces_in, float *image, float *illum, int snap_mode);
+
+/*
+ * Born modeling (to_disk_to_memory_BORN.f90:123-220): pass 1 forward with
+ * snapshots (source second-differenced as readsou does with Born=.true.,
+ * ReadSouVelRho.f90:318-326), pass 2 scattering from refl(nz,nx) via
+ * Injection_Born (Imaging.f90:747-788), extraction with gain 1/(dz dx).
+ */
+int  sfd_born(sfd_handle h, const sfd_shot *shots, int nshots,
+              const float *refl, float *traces_out);
+
+/* LSRTM imaging (to_disk_to_memory_RTM.f90:494-579, Imaging.f90:698-745). */
+int  sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots,
+               const float *traces_in, float *image, float *illum);
+
+/*
+ * FWI misfit and gradient for a batch of shots, the body of compute_fct_gdt
+ * (progs/afwi/compute_function_gradient.f90:208-350): per shot AMOD
+ * (nearest extraction), residual w(dobs-dmod) and FP64 L2 misfit
+ * (Objective_Function_Residual_Adjsource.f90:17-40, libs/libinv/Norm.f90:28),
+ * AGRAD (simple injection, FWI imaging Imaging.f90:322-422), scatter-add,
+ * then illumination normalisation, mask and triangle smoothing
+ * (Smoothing.f90:87-227).  mute and vpmask may be NULL (all ones).
+ * grad is the reference's `grad` (its sign convention: l_bfgs passes -grad
+ * to LBFGS, Inversion.f90:128).  resid_out may be NULL.
+ */
+int  sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots,
+                      const float *dobs, const float *mute, const float *vpmask,
+                      const sfd_fwi_opts *opt, double *misfit, float *grad,
+                      float *illum_out, float *resid_out);
+
+/*
+ * Per-phase timing of the last call (the reference's verbose timers,
+ * Propagator.f90:165-258): ms[0]=total GPU time of the propagation kernels,
+ * ms[1]=forward pass, ms[2]=backward pass, ms[3]=host-side setup.
+ * pts[0]= FD point-updates executed (sum over shots of nz_FD*nx_FD*nt).
+ */
+int  sfd_last_timing(sfd_handle h, double *ms, double *pts);
+
+/*
+ * Step kernel tile configuration (tile_z x tile_x cells per workgroup).
+ * Chosen by the autotuner when tile_z <= 0.
+ */
+int  sfd_set_tile(sfd_handle h, int tile_z, int tile_x);
+
+/* ---- L-BFGS (libs/libinv/lbfgs.f:49-558 LBFGS + MCSRCH + MCSTEP), reverse
+ *      communication exactly as the reference's IFLAG protocol.  Vectors
+ *      live on the device; the two-loop recursion and the line-search dot
+ *      products are FP64 device reductions. ------------------------------- */
+typedef struct sfd_lbfgs *sfd_lbfgs_handle;
+int  sfd_lbfgs_create(sfd_lbfgs_handle *lb, int n, int m, double xmin, double xmax,
+                      const float *mask, int boundt, int freeze_soft,
+                      const double *xinit);
+/* One call of LBFGS(N,M,X,F,G,...): x (in/out), f, g as the reference passes
+ * them (g is the gradient of f); iflag/info as in lbfgs.f. */
+int  sfd_lbfgs_step(sfd_lbfgs_handle lb, double *x, double f, const double *g,
+                    int *iflag, int *info);
+int  sfd_lbfgs_destroy(sfd_lbfgs_handle lb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEISFD_H */

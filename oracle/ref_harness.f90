@@ -1,0 +1,205 @@
+! ref_harness.f90 — TEST INFRASTRUCTURE ONLY.
+!
+! C-callable harness around the reference's own SEP-free Fortran kernels
+! (compiled in place from /root/reference by oracle/Makefile into
+! oracle/_ref/libseisref.so).  It composes those kernels in the order of
+! propagator_acoustic (libs/libwave/Propagator.f90:142-243), which itself
+! cannot be compiled here: it uses Imaging_mod, which needs SEPlib
+! (libs/libwave/Imaging.f90:8).  The harness adds no arithmetic of its own:
+! every floating-point operation on the wavefield is a call into the
+! reference's modules.  Used only to pin oracle/seisoracle.c.
+module ref_harness
+  use iso_c_binding
+  use GeneralParam_types
+  use DataSpace_types
+  use ModelSpace_types
+  use FD_types
+  use FDcoefs_assign
+  use FD_derivatives
+  use FDswaptime_mod
+  use Interpolate_mod
+  use Boundary_mod
+  use Injection_mod
+  use Extraction_mod
+  use Taper_mod
+  implicit none
+contains
+
+  subroutine ref_mksinc(sinc, lsinc, shift) bind(C, name='ref_mksinc')
+    integer(c_int), value :: lsinc
+    real(c_float), value  :: shift
+    real(c_float)         :: sinc(lsinc)
+    call mksinc(sinc, lsinc, shift)
+  end subroutine ref_mksinc
+
+  subroutine ref_higdon_coeff(nu, coeff) bind(C, name='ref_higdon_coeff')
+    real(c_float), value :: nu
+    real(c_float)        :: coeff(8)
+    call Higdon_coeff(nu, 1.0, 0.7, 0.5, coeff)
+  end subroutine ref_higdon_coeff
+
+  subroutine ref_taper(n, t) bind(C, name='ref_taper')
+    integer(c_int), value :: n
+    real(c_float)         :: t(n)
+    type(ModelSpace) :: m
+    m%nxw = n
+    m%nyw = 1
+    call compute_taper(m)
+    t = m%taperx
+    call compute_taper_close(m)
+  end subroutine ref_taper
+
+  ! One propagation in Propagator.f90 order.
+  !   stencil: 0 forward_grid (FDderivatives.f90:715), 1 adjoint_grid (:847)
+  !   inj    : 0 none, 1 Injection_sinc, 2 Injection_simple_noomp, 3 Injection_LSRTM_sinc
+  !   ext    : -1 none, 0 Extraction_array_sinc, 1 Extraction_array_simple_afwi_noomp,
+  !            2 Extraction_array_LSRTM_sinc, 3 Extraction_array_sinc_afwi
+  !   tdir   : +1 it=1..nt, -1 it=nt..1
+  !   wave   : (nz,nxw,nsnap) captured u2 every snapi steps (Extraction_wavefield)
+  subroutine ref_propagate(nz, nxw, ntaper, dz, dx, dt, oz, oxw, lsinc, snapi, vel_pad, &
+       &                   stencil, inj, nsou, souz, soux, soutr, ext, ndat, datz, datx, dattr, &
+       &                   nt, tdir, nsnap, wave) bind(C, name='ref_propagate')
+    integer(c_int), value :: nz, nxw, ntaper, lsinc, snapi, stencil, inj, nsou, ext, ndat, nt, tdir, nsnap
+    real(c_float), value  :: dz, dx, dt, oz, oxw
+    real(c_float)         :: vel_pad(*), souz(*), soux(*), soutr(nt, *), datz(*), datx(*), dattr(nt, *)
+    real(c_float)         :: wave(nz, nxw, *)
+
+    type(GeneralParam)         :: genpar
+    type(FDbounds)             :: bounds
+    type(ModelSpace)           :: model
+    type(ModelSpace_elevation) :: elev
+    type(HigdonParam)          :: hig
+    type(UnscaledFDcoefs)      :: fdcoefs
+    type(ScaledFDcoefs), target :: scaled
+    type(USpace)               :: grid
+    type(WaveSpace)            :: wfld
+    type(TraceSpace), allocatable :: sou(:), dat(:)
+    integer :: i, j, it, n1, n2
+
+    genpar%twoD = .true.
+    genpar%withRho = .false.
+    genpar%Born = .false.
+    genpar%LSRTM = .false.
+    genpar%verbose = .false.
+    genpar%optim = .true.
+    genpar%dt = dt
+    genpar%nt = nt
+    genpar%delta = (/ dz, dx, 1.0 /)
+    genpar%omodel = (/ oz, oxw, 0.0 /)
+    genpar%lsinc = lsinc
+    genpar%nbound = 0
+    genpar%ntaper = ntaper
+    genpar%snapi = snapi
+    genpar%ntsnap = nsnap
+    genpar%rec_type = 0
+    genpar%shot_type = 0
+    genpar%surf_type = 0
+    genpar%coefpower = 2
+    if (tdir > 0) then
+       genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
+    else
+       genpar%tmin = nt; genpar%tmax = 1; genpar%tstep = -1
+    end if
+
+    bounds%nmin1 = -ntaper + 1
+    bounds%nmax1 = nz + ntaper
+    bounds%nmin2 = -ntaper + 1
+    bounds%nmax2 = nxw + ntaper
+    bounds%nmin3 = 1
+    bounds%nmax3 = 1
+
+    model%nz = nz
+    model%nxw = nxw
+    model%nyw = 1
+    model%dz = dz
+    model%dx = dx
+    model%dy = 1.0
+    n1 = bounds%nmax1 - bounds%nmin1 + 9
+    n2 = bounds%nmax2 - bounds%nmin2 + 9
+    allocate(model%vel(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, 1:1))
+    do j = 1, n2
+       do i = 1, n1
+          model%vel(bounds%nmin1-5+i, bounds%nmin2-5+j, 1) = vel_pad(i + (j-1)*n1)
+       end do
+    end do
+
+    allocate(sou(max(nsou,1)), dat(max(ndat,1)))
+    do i = 1, nsou
+       sou(i)%coord = (/ souz(i), soux(i), 0.0 /)
+       allocate(sou(i)%trace(nt, 1))
+       sou(i)%trace(:, 1) = soutr(:, i)
+    end do
+    do i = 1, ndat
+       dat(i)%coord = (/ datz(i), datx(i), 0.0 /)
+       allocate(dat(i)%trace(nt, 1))
+       dat(i)%trace(:, 1) = dattr(:, i)
+    end do
+    if (nsnap > 0) then
+       allocate(wfld%wave(nz, nxw, 1, nsnap, 1))
+       wfld%wave = 0.
+    end if
+
+    ! Propagator.f90:144-166
+    call allocateUSpace(grid, genpar, bounds)
+    call zeroUSpace(grid)
+    allocate(hig%gx(16, bounds%nmin1:bounds%nmax1, bounds%nmin3:bounds%nmax3))
+    allocate(hig%gy(16, bounds%nmin1:bounds%nmax1, bounds%nmin2:bounds%nmax2))
+    allocate(hig%gz(16, bounds%nmin2:bounds%nmax2, bounds%nmin3:bounds%nmax3))
+    call FD_acoustic_init_coefs(fdcoefs)
+    call FD_types_assign_scaled_coefs(fdcoefs, scaled, genpar)
+    call FD_derivatives_coef_init(scaled)
+    call Higdon(genpar%dt, model, bounds, hig)
+    if (nsou > 0) call ModelSpace_compute_array_xyz_positions(genpar, sou(1:nsou))
+    if (ndat > 0) call ModelSpace_compute_array_xyz_positions(genpar, dat(1:ndat))
+    call ModelSpace_elevation_parameters(elev, bounds, genpar)
+    wfld%counter = 0
+
+    ! Propagator.f90:168-243
+    do it = genpar%tmin, genpar%tmax, genpar%tstep
+       if (ndat > 0) then
+          select case (ext)
+          case (0)
+             call Extraction_array_sinc(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          case (1)
+             call Extraction_array_simple_afwi_noomp(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          case (2)
+             call Extraction_array_LSRTM_sinc(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          case (3)
+             call Extraction_array_sinc_afwi(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          end select
+       end if
+       if (nsnap > 0) call Extraction_wavefield(bounds, model, elev, grid%u2, genpar, it, dat=wfld)
+       call Boundary_set_free_surface_grid(bounds, model, elev, grid, genpar)
+       if (stencil == 0) then
+          call FD_2nd_2D_derivatives_scalar_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       else
+          call FD_2nd_2D_derivatives_scalar_adjoint_grid(genpar, bounds, grid%u2, grid%u3, model)
+       end if
+       if (nsou > 0) then
+          select case (inj)
+          case (1)
+             call Injection_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+          case (2)
+             call Injection_simple_noomp(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+          case (3)
+             call Injection_LSRTM_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+          end select
+       end if
+       call FD_2nd_time_derivative_grid(genpar, bounds, grid)
+       call Boundary_3d(genpar, bounds, grid, model, hig)
+       call FDswaptime_pointer(grid)
+    end do
+
+    do i = 1, ndat
+       dattr(:, i) = dat(i)%trace(:, 1)
+    end do
+    if (nsnap > 0) then
+       do i = 1, nsnap
+          wave(:, :, i) = wfld%wave(:, :, 1, i, 1)
+       end do
+    end if
+    call deallocateHigdonParam(hig)
+    call deallocateUSpace(grid)
+  end subroutine ref_propagate
+
+end module ref_harness

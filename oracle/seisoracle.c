@@ -1,0 +1,77 @@
+/*
+ * seisoracle.c — CPU oracle for the acoustic FD engine.
+ *
+ * TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg, always as the checker, never as the thing
+ * measured or shipped.  The product (seismic_amd/) never links it.
+ *
+ * It restates the reference (esiwgnahz/seismic, libs/libwave + progs/) in
+ * plain C: the same pass order (stencil, injection, 5-level update, Higdon,
+ * pointer rotation), the same loop order and the same left-to-right
+ * arithmetic.  seisoracle_impl.h is compiled twice:
+ *   orc_f_*  REAL=float   (FP32, the reference's precision)
+ *   orc_d_*  REAL=double  (FP64 oracle for image/gradient error statements)
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no -ffast-math).
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   - kernels and composed forward/backward propagation: bitwise or to a few
+ *     ulps against the reference's own Fortran kernels compiled by flang into
+ *     oracle/_ref (oracle/ref_harness.f90), composed in Propagator.f90 order;
+ *   - FD coefficients: polynomial exactness (degree <= 9), closed form;
+ *   - mksinc: shift-0 delta, unit sum; Higdon_coeff: closed formula;
+ *   - imaging / FWI gradient / Born: brute-force sums over captured fields;
+ *   - triangle smoothing: impulse -> triangle, constant preservation;
+ *   - L-BFGS (oracle/lbfgs_oracle.c): trajectory equal to lbfgs.f via _ref.
+ * Parity unpinned: window/pad helpers (shot_window, window_pad, copy_image)
+ * restate ExtractPadModel.f90/ReadSouVelRho.f90, which need SEPlib and cannot
+ * be built here; they are checked against closed-form expectations only.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#include "seisfd.h"
+
+typedef struct {
+    int nmin1, nmax1, nmin2, nmax2; /* FD region, Fortran indices          */
+    int NZ, NX;                     /* allocated incl. 4-cell operator halo */
+} ORC_BOUNDS;
+
+/* index of Fortran (i,j) in an array allocated (nmin1-4:nmax1+4, nmin2-4:nmax2+4) */
+#define ORC_IDX(b, i, j) ((size_t)((i) - ((b)->nmin1 - 4)) + (size_t)((j) - ((b)->nmin2 - 4)) * (size_t)(b)->NZ)
+
+/* read_window_vel bounds, ExtractPadModel.f90:228-239 (shot/surf type 0) */
+void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
+{
+    b->nmin1 = -ntaper + 1;
+    b->nmax1 = nz + ntaper;
+    b->nmin2 = -ntaper + 1;
+    b->nmax2 = nxw + ntaper;
+    b->NZ = b->nmax1 - b->nmin1 + 9;
+    b->NX = b->nmax2 - b->nmin2 + 9;
+}
+
+#define REAL float
+#define SFX f
+#define ROUND roundf
+#define FABS fabsf
+#define SIN sinf
+#define COS cosf
+#define POW powf
+#include "seisoracle_impl.h"
+#undef REAL
+#undef SFX
+#undef ROUND
+#undef FABS
+#undef SIN
+#undef COS
+#undef POW
+
+#define REAL double
+#define SFX d
+#define ROUND round
+#define FABS fabs
+#define SIN sin
+#define COS cos
+#define POW pow
+#include "seisoracle_impl.h"

@@ -1,0 +1,106 @@
+"""ctypes mirror of include/seisfd.h (the engine's C-ABI).
+
+The structures below are byte-for-byte the C structs; the loader finds the
+in-tree built library ``seismic_amd/lib/libseisfd.so`` and fails loudly when
+it is missing (there is no CPU fallback for any engine entry point).
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libseisfd.so")
+
+SFD_OK = 0
+SFD_EINVAL = -1
+SFD_ENOMEM = -2
+SFD_EHIP = -3
+SFD_EUNSUPPORTED = -4
+SFD_ESTATE = -5
+SFD_EUNSTABLE = -6
+SFD_ENOTFINITE = -7
+
+SFD_SNAP_FP32 = 0
+SFD_SNAP_FP16S = 1
+
+SFD_EXTRACT_SINC_V2 = 0
+SFD_EXTRACT_NEAREST = 1
+SFD_EXTRACT_SINC_LSRTM = 2
+SFD_EXTRACT_SINC_UNIT = 3
+
+
+class sfd_grid(C.Structure):
+    _fields_ = [
+        ("nz", C.c_int32), ("nx", C.c_int32),
+        ("oz", C.c_float), ("ox", C.c_float),
+        ("dz", C.c_float), ("dx", C.c_float),
+        ("dt", C.c_float),
+        ("ntaper", C.c_int32), ("lsinc", C.c_int32), ("snapi", C.c_int32),
+        ("aperture_x", C.c_float),
+        ("rec_type", C.c_int32), ("shot_type", C.c_int32), ("surf_type", C.c_int32),
+    ]
+
+
+class sfd_shot(C.Structure):
+    _fields_ = [
+        ("sz", C.c_float), ("sx", C.c_float),
+        ("nt", C.c_int32), ("ntr", C.c_int32),
+        ("wavelet", C.POINTER(C.c_float)),
+        ("rz", C.POINTER(C.c_float)),
+        ("rx", C.POINTER(C.c_float)),
+    ]
+
+
+class sfd_fwi_opts(C.Structure):
+    _fields_ = [
+        ("illupow", C.c_float),
+        ("rect_z", C.c_int32), ("rect_x", C.c_int32),
+        ("do_z", C.c_int32), ("do_x", C.c_int32),
+        ("snap_mode", C.c_int32),
+    ]
+
+
+# Every symbol declared in include/seisfd.h, in declaration order.
+EXPORTS = [
+    "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
+    "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_born", "sfd_lsrtm",
+    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile",
+    "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
+]
+
+_lib = None
+
+
+def load():
+    """Load libseisfd.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libseisfd.so not built at %s; run __graft_entry__.build()" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    lib.sfd_create.argtypes = [P(C.c_void_p), P(sfd_grid)]
+    lib.sfd_destroy.argtypes = [C.c_void_p]
+    lib.sfd_last_error.argtypes = [C.c_void_p]
+    lib.sfd_last_error.restype = C.c_char_p
+    lib.sfd_device_info.argtypes = [C.c_char_p, C.c_int, P(C.c_int), P(C.c_size_t)]
+    lib.sfd_set_model.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sfd_forward.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_int]
+    lib.sfd_rtm.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,
+                            C.c_void_p, C.c_int]
+    lib.sfd_born.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p]
+    lib.sfd_lsrtm.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,
+                              C.c_void_p]
+    lib.sfd_fwi_gradient.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, P(sfd_fwi_opts), P(C.c_double), C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
+    lib.sfd_last_timing.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+    lib.sfd_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.sfd_lbfgs_create.argtypes = [P(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.sfd_lbfgs_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                   P(C.c_int), P(C.c_int)]
+    lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib

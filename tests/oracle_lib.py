@@ -1,0 +1,337 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle.so) and to the
+reference's own Fortran kernels (oracle/_ref/libseisref.so).
+
+Test infrastructure: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker only.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from seismic_amd.abi import sfd_grid, sfd_shot, sfd_fwi_opts
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libseisref.so")
+
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+
+
+class ORC_BOUNDS(C.Structure):
+    _fields_ = [("nmin1", C.c_int), ("nmax1", C.c_int), ("nmin2", C.c_int),
+                ("nmax2", C.c_int), ("NZ", C.c_int), ("NX", C.c_int)]
+
+
+def _ensure_built():
+    if not os.path.exists(ORACLE_SO):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+_orc = None
+_ref = None
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        _ensure_built()
+        _orc = C.CDLL(ORACLE_SO)
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        # Lazy binding: Extraction.f90's disk-spill routine references SEPlib's
+        # srite_, which is never called on the in-memory path exercised here.
+        _ref = C.CDLL(REF_SO, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    return _ref
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture=0.0):
+    g = sfd_grid()
+    g.nz, g.nx = nz, nx
+    g.oz, g.ox, g.dz, g.dx, g.dt = oz, ox, dz, dx, dt
+    g.ntaper, g.lsinc, g.snapi = ntaper, lsinc, snapi
+    g.aperture_x = aperture
+    g.rec_type = g.shot_type = g.surf_type = 0
+    return g
+
+
+class Shot:
+    """Keeps the numpy arrays behind an sfd_shot alive."""
+
+    def __init__(self, sz, sx, wavelet, rz, rx):
+        self.wavelet = np.ascontiguousarray(wavelet, dtype=np.float32)
+        self.rz = np.ascontiguousarray(rz, dtype=np.float32)
+        self.rx = np.ascontiguousarray(rx, dtype=np.float32)
+        self.sz, self.sx = float(sz), float(sx)
+        self.nt = len(self.wavelet)
+        self.ntr = len(self.rz)
+
+    def c(self):
+        s = sfd_shot()
+        s.sz, s.sx = self.sz, self.sx
+        s.nt, s.ntr = self.nt, self.ntr
+        s.wavelet = self.wavelet.ctypes.data_as(_FP)
+        s.rz = self.rz.ctypes.data_as(_FP)
+        s.rx = self.rx.ctypes.data_as(_FP)
+        return s
+
+
+def _dt(prec):
+    return np.float32 if prec == "f" else np.float64
+
+
+def shot_window(g, shot):
+    L = oracle()
+    oxw = C.c_float()
+    nxw = C.c_int()
+    s = shot.c()
+    L.orc_f_shot_window(C.byref(g), C.byref(s), C.byref(oxw), C.byref(nxw))
+    return oxw.value, nxw.value
+
+
+def bounds(nz, nxw, ntaper):
+    b = ORC_BOUNDS()
+    oracle().orc_bounds(C.byref(b), nz, nxw, ntaper)
+    return b
+
+
+def padded_vel(g, vel, shot, prec="f"):
+    """Window + edge-replication pad of the full model for one shot."""
+    L = oracle()
+    oxw, nxw = shot_window(g, shot)
+    b = bounds(g.nz, nxw, g.ntaper)
+    v = np.ascontiguousarray(vel, dtype=_dt(prec))
+    out = np.zeros(b.NZ * b.NX, dtype=_dt(prec))
+    fn = getattr(L, "orc_%s_window_pad" % prec)
+    fn.argtypes = [C.c_void_p, C.c_void_p, (C.c_float if prec == "f" else C.c_double), C.c_int,
+                   C.c_void_p, C.c_void_p]
+    fn(C.byref(g), ptr(v), oxw, nxw, C.byref(b), ptr(out))
+    return out.reshape(b.NX, b.NZ).T, oxw, nxw, b
+
+
+def modeling(g, vel, shot, extract_kind=0, prec="f", snaps=False):
+    L = oracle()
+    T = _dt(prec)
+    v = np.ascontiguousarray(vel, dtype=T)
+    w = np.ascontiguousarray(shot.wavelet, dtype=T)
+    tr = np.zeros((shot.ntr, shot.nt), dtype=T)
+    snap = None
+    sp = None
+    if snaps:
+        oxw, nxw = shot_window(g, shot)
+        snap = np.zeros((shot.nt // g.snapi, nxw, g.nz), dtype=T)
+        sp = ptr(snap)
+    s = shot.c()
+    fn = getattr(L, "orc_%s_modeling" % prec)
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    rc = fn(C.byref(g), ptr(v), C.byref(s), ptr(w), ptr(tr), extract_kind, sp)
+    assert rc == 0
+    return (tr, snap) if snaps else tr
+
+
+def rtm(g, vel, shot, traces_in, prec="f", lsrtm=False, vel_bwd=None, capture=False):
+    """Returns (image, illum) full-model (nz,nx) as column-major -> array [nx, nz]^T."""
+    L = oracle()
+    T = _dt(prec)
+    v = np.ascontiguousarray(vel, dtype=T)
+    vb = None if vel_bwd is None else np.ascontiguousarray(vel_bwd, dtype=T)
+    w = np.ascontiguousarray(shot.wavelet, dtype=T)
+    d = np.ascontiguousarray(traces_in, dtype=T)
+    img = np.zeros(g.nz * g.nx, dtype=T)
+    ill = np.zeros(g.nz * g.nx, dtype=T)
+    oxw, nxw = shot_window(g, shot)
+    wimg = np.zeros(g.nz * nxw, dtype=T)
+    will = np.zeros(g.nz * nxw, dtype=T)
+    cap = None
+    if capture:
+        cap = np.zeros((shot.nt // g.snapi, nxw, g.nz), dtype=T)
+    s = shot.c()
+    fn = getattr(L, "orc_%s_rtm" % prec)
+    fn.argtypes = [C.c_void_p] * 8 + [C.c_int] + [C.c_void_p] * 3
+    rc = fn(C.byref(g), ptr(v), None if vb is None else ptr(vb), C.byref(s), ptr(w), ptr(d),
+            ptr(img), ptr(ill), int(lsrtm), None if cap is None else ptr(cap), ptr(wimg), ptr(will))
+    assert rc == 0
+    out = dict(image=img.reshape(g.nx, g.nz).T, illum=ill.reshape(g.nx, g.nz).T,
+               win_image=wimg.reshape(nxw, g.nz).T, win_illum=will.reshape(nxw, g.nz).T,
+               oxw=oxw, nxw=nxw)
+    if capture:
+        out["capture"] = cap
+    return out
+
+
+def born(g, vel, refl, shot, prec="f"):
+    L = oracle()
+    T = _dt(prec)
+    v = np.ascontiguousarray(vel, dtype=T)
+    m = np.ascontiguousarray(refl, dtype=T)
+    w = np.ascontiguousarray(shot.wavelet, dtype=T)
+    tr = np.zeros((shot.ntr, shot.nt), dtype=T)
+    s = shot.c()
+    fn = getattr(L, "orc_%s_born" % prec)
+    fn.argtypes = [C.c_void_p] * 6
+    rc = fn(C.byref(g), ptr(v), ptr(m), C.byref(s), ptr(w), ptr(tr))
+    assert rc == 0
+    return tr
+
+
+def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1,
+                 do_z=0, do_x=0, prec="f"):
+    """compute_fct_gdt restatement.  dobs: list of (ntr, nt) arrays per shot."""
+    L = oracle()
+    T = _dt(prec)
+    v = np.ascontiguousarray(vel, dtype=T)
+    cs = (sfd_shot * len(shots))(*[s.c() for s in shots])
+    ob = np.ascontiguousarray(np.concatenate([np.asarray(d, dtype=T).ravel() for d in dobs]))
+    mu = None if mute is None else np.ascontiguousarray(
+        np.concatenate([np.asarray(m, dtype=T).ravel() for m in mute]))
+    vm = None if vpmask is None else np.ascontiguousarray(vpmask, dtype=T)
+    o = sfd_fwi_opts()
+    o.illupow, o.rect_z, o.rect_x, o.do_z, o.do_x, o.snap_mode = illupow, rect_z, rect_x, do_z, do_x, 0
+    f = C.c_double()
+    grad = np.zeros(g.nz * g.nx, dtype=T)
+    illu = np.zeros(g.nz * g.nx, dtype=T)
+    res = np.zeros_like(ob)
+    w = np.ascontiguousarray(shots[0].wavelet, dtype=T)
+    fn = getattr(L, "orc_%s_fwi_gradient" % prec)
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_void_p]
+    rc = fn(C.byref(g), ptr(v), len(shots), cs, ptr(w), ptr(ob), None if mu is None else ptr(mu),
+            None if vm is None else ptr(vm), C.byref(o), C.byref(f), ptr(grad), ptr(illu), ptr(res))
+    assert rc == 0
+    return dict(f=f.value, grad=grad.reshape(g.nx, g.nz).T, illum=illu.reshape(g.nx, g.nz).T,
+                resid=res)
+
+
+def mksinc(lsinc, shift, prec="f"):
+    T = _dt(prec)
+    out = np.zeros(lsinc, dtype=T)
+    fn = getattr(oracle(), "orc_%s_mksinc" % prec)
+    fn.argtypes = [C.c_void_p, C.c_int, (C.c_float if prec == "f" else C.c_double)]
+    fn(ptr(out), lsinc, shift)
+    return out
+
+
+def higdon_coeff(nu, prec="f"):
+    T = _dt(prec)
+    out = np.zeros(8, dtype=T)
+    fn = getattr(oracle(), "orc_%s_higdon_coeff" % prec)
+    R = C.c_float if prec == "f" else C.c_double
+    fn.argtypes = [R, R, R, R, C.c_void_p]
+    fn(nu, 1.0, 0.7, 0.5, ptr(out))
+    return out
+
+
+def taper(n, prec="f"):
+    T = _dt(prec)
+    out = np.zeros(n, dtype=T)
+    fn = getattr(oracle(), "orc_%s_compute_taper" % prec)
+    fn.argtypes = [C.c_void_p, C.c_int]
+    fn(ptr(out), n)
+    return out
+
+
+def triangle2(u, rect_z, rect_x, doz=1, dox=1, prec="f"):
+    T = _dt(prec)
+    a = np.asfortranarray(np.array(u, dtype=T))
+    fn = getattr(oracle(), "orc_%s_triangle2" % prec)
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    fn(a.shape[0], a.shape[1], rect_z, rect_x, doz, dox, a.ctypes.data_as(C.c_void_p))
+    return np.array(a)
+
+
+# ---- reference (flang build) ----
+def ref_mksinc(lsinc, shift):
+    out = np.zeros(lsinc, dtype=np.float32)
+    fn = ref().ref_mksinc
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_float]
+    fn(ptr(out), lsinc, shift)
+    return out
+
+
+def ref_higdon_coeff(nu):
+    out = np.zeros(8, dtype=np.float32)
+    fn = ref().ref_higdon_coeff
+    fn.argtypes = [C.c_float, C.c_void_p]
+    fn(nu, ptr(out))
+    return out
+
+
+def ref_taper(n):
+    out = np.zeros(n, dtype=np.float32)
+    fn = ref().ref_taper
+    fn.argtypes = [C.c_int, C.c_void_p]
+    fn(n, ptr(out))
+    return out
+
+
+def ref_propagate(g, vel_pad, oxw, nxw, stencil, inj, sou_z, sou_x, sou_tr, ext, dat_z, dat_x,
+                  nt, tdir, nsnap):
+    """Runs the reference kernels in Propagator.f90 order.  vel_pad: [NZ, NX]."""
+    fn = ref().ref_propagate
+    fn.argtypes = ([C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 2 + [C.c_void_p] + [C.c_int] * 3
+                   + [C.c_void_p] * 3 + [C.c_int] * 2 + [C.c_void_p] * 3 + [C.c_int] * 3 + [C.c_void_p])
+    vp = np.asfortranarray(vel_pad, dtype=np.float32)
+    sz = np.ascontiguousarray(sou_z, dtype=np.float32)
+    sx = np.ascontiguousarray(sou_x, dtype=np.float32)
+    st = np.ascontiguousarray(sou_tr, dtype=np.float32).reshape(len(sz), nt) if len(sz) else \
+        np.zeros((1, nt), np.float32)
+    dz_ = np.ascontiguousarray(dat_z, dtype=np.float32)
+    dx_ = np.ascontiguousarray(dat_x, dtype=np.float32)
+    dtr = np.zeros((max(len(dz_), 1), nt), dtype=np.float32)
+    wave = np.zeros((max(nsnap, 1), nxw, g.nz), dtype=np.float32)
+    fn(g.nz, nxw, g.ntaper, g.dz, g.dx, g.dt, g.oz, oxw, g.lsinc, g.snapi, vp.ctypes.data_as(C.c_void_p),
+       stencil, inj, len(sz), ptr(sz), ptr(sx), ptr(st), ext, len(dz_), ptr(dz_), ptr(dx_), ptr(dtr),
+       nt, tdir, nsnap, ptr(wave))
+    return dtr[:len(dz_)], wave[:nsnap]
+
+
+# ---- L-BFGS oracle ----
+class LBFGS:
+    """orc_lbfgs state machine (restatement of lbfgs.f LBFGS)."""
+
+    def __init__(self, n, m=5, xmin=-1e30, xmax=1e30, mask=None, boundt=0, freeze_soft=1, xinit=None):
+        L = oracle()
+        L.orc_lbfgs_new.restype = C.c_void_p
+        L.orc_lbfgs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_void_p,
+                                C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int), C.c_double, C.c_double, C.c_void_p, C.c_int,
+                                C.c_int, C.c_void_p]
+        L.orc_lbfgs_free.argtypes = [C.c_void_p]
+        self.L = L
+        self.S = L.orc_lbfgs_new()
+        self.n, self.m = n, m
+        self.diag = np.zeros(n)
+        self.w = np.zeros(n * (2 * m + 1) + 2 * m)
+        self.iflag = C.c_int(0)
+        self.myinfo = C.c_int(0)
+        self.xmin, self.xmax = xmin, xmax
+        self.mask = np.ones(n, np.float32) if mask is None else np.asarray(mask, np.float32)
+        self.boundt, self.type = boundt, freeze_soft
+        self.xinit = np.zeros(n) if xinit is None else np.asarray(xinit, np.float64)
+
+    def step(self, x, f, g, eps=1e-20, xtol=np.finfo(np.float64).eps):
+        gg = np.ascontiguousarray(g, np.float64)
+        self.L.orc_lbfgs(self.S, self.n, self.m, ptr(x), f, ptr(gg), ptr(self.diag), eps, xtol,
+                         ptr(self.w), C.byref(self.iflag), C.byref(self.myinfo), self.xmin, self.xmax,
+                         ptr(self.mask), self.boundt, self.type, ptr(self.xinit))
+        return self.iflag.value, self.myinfo.value
+
+    def __del__(self):
+        try:
+            self.L.orc_lbfgs_free(self.S)
+        except Exception:
+            pass

@@ -1,0 +1,83 @@
+"""Pins the C oracle against the reference's own Fortran kernels (flang build
+in oracle/_ref).  Skipped where the reference tree is absent."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from models import ricker, smooth_random
+
+pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference tree)")
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.25, -0.25, 0.5, -0.5, 0.1234, -0.4999])
+def test_mksinc_bitwise(shift):
+    assert np.array_equal(O.mksinc(7, shift), O.ref_mksinc(7, shift))
+
+
+@pytest.mark.parametrize("nu", [0.05, 0.2, 0.375, 0.45, 0.6])
+def test_higdon_coeff_bitwise(nu):
+    assert np.array_equal(O.higdon_coeff(nu), O.ref_higdon_coeff(nu))
+
+
+@pytest.mark.parametrize("n", [4, 7, 20, 21, 401])
+def test_taper_bitwise(n):
+    assert np.array_equal(O.taper(n), O.ref_taper(n))
+
+
+def _case(seed=1, nz=36, nx=48, nt=240):
+    g = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12, snapi=4)
+    vel = smooth_random(nz, nx, seed=seed)
+    w = ricker(25.0, nt, 1e-3)
+    rx = np.linspace(13.0, 10.0 * (nx - 1) - 17.0, 21).astype(np.float32)
+    rz = np.full_like(rx, 23.0)
+    shot = O.Shot(37.0, 241.0, w, rz, rx)
+    return g, vel, shot
+
+
+@pytest.mark.parametrize("ext", [0, 1, 2, 3])
+def test_forward_gather_matches_reference(ext):
+    g, vel, shot = _case()
+    vpad, oxw, nxw, b = O.padded_vel(g, vel, shot)
+    ref_tr, ref_w = O.ref_propagate(g, vpad, oxw, nxw, 0, 1, [shot.sz], [shot.sx], shot.wavelet[None, :],
+                                    ext, shot.rz, shot.rx, shot.nt, 1, shot.nt // g.snapi)
+    tr, snaps = O.modeling(g, vel, shot, extract_kind=ext, snaps=True)
+    # bitwise: same kernels, same order, same FP32 operations
+    assert np.array_equal(tr, ref_tr), rel(tr, ref_tr)
+    assert np.array_equal(snaps, ref_w), rel(snaps, ref_w)
+
+
+@pytest.mark.parametrize("inj,stencil", [(1, 1), (3, 1), (2, 0)])
+def test_backward_fields_match_reference(inj, stencil):
+    """Adjoint-style (RTM/LSRTM) and forward-stencil (AGRAD) backward passes with
+    receivers as sources: the u2 fields at every imaging step."""
+    g, vel, shot = _case(seed=2)
+    rng = np.random.default_rng(5)
+    data = rng.standard_normal((shot.ntr, shot.nt)).astype(np.float32)
+    vpad, oxw, nxw, b = O.padded_vel(g, vel, shot)
+    _, ref_w = O.ref_propagate(g, vpad, oxw, nxw, stencil, inj, shot.rz, shot.rx, data, -1, [], [],
+                               shot.nt, -1, shot.nt // g.snapi)
+    # oracle: run the same backward pass through the restated kernels
+    if inj in (1, 3) and stencil == 1:
+        out = O.rtm(g, vel, shot, data, lsrtm=(inj == 3), capture=True)
+        cap = out["capture"]
+    else:
+        cap = _agrad_capture(g, vel, shot, data)
+    assert np.array_equal(cap, ref_w), rel(cap, ref_w)
+
+
+def _agrad_capture(g, vel, shot, data):
+    # AGRAD pass: capture via the reference-order helper exposed for tests
+    import ctypes as C
+    L = O.oracle()
+    fn = L.orc_f_test_agrad_capture
+    fn.argtypes = [C.c_void_p] * 5
+    oxw, nxw = O.shot_window(g, shot)
+    cap = np.zeros((shot.nt // g.snapi, nxw, g.nz), np.float32)
+    s = shot.c()
+    d = np.ascontiguousarray(data, np.float32)
+    assert fn(C.byref(g), O.ptr(np.ascontiguousarray(vel)), C.byref(s), O.ptr(d), O.ptr(cap)) == 0
+    return cap
