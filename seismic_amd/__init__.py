@@ -1,0 +1,10 @@
+"""seismic_amd — MI355X-native acoustic finite-difference engine.
+
+A drop-in for the libwave propagator of esiwgnahz/seismic and the modeling,
+Born, RTM/LSRTM and FWI-gradient drivers above it, behind a C-ABI
+(include/seisfd.h, libseisfd.so).  Python here is test/bench plumbing over
+that ABI (ctypes); Fortran callers use seismic_amd/fortran/seisfd_mod.f90.
+"""
+from .abi import (SFD_SNAP_FP32, SFD_SNAP_FP16S, SFD_EXTRACT_SINC_V2, SFD_EXTRACT_NEAREST,  # noqa: F401
+                  SFD_EXTRACT_SINC_LSRTM, SFD_EXTRACT_SINC_UNIT)
+from .engine import Engine, Shot, make_grid, device_info, SeisFDError  # noqa: F401

@@ -1,0 +1,115 @@
+// engine.h — host-side state of one seisfd handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/seisfd.h"
+#include "fd_kernels.h"
+
+struct EngineError {
+    int code;
+    std::string msg;
+    EngineError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// Grow-only device buffer reused across calls (no allocation in steady state).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n);
+    ~DevBuf();
+};
+
+// Host-side plan of one shot: window, padded-grid geometry, tiles, and the
+// injection / extraction tables of the step kernel.
+struct ShotPlan {
+    float oxw = 0;
+    int nxw = 0, nt = 0, ntr = 0, ntsnap = 0;
+    int NZ = 0, NX = 0, LDZ = 0, zoff = 0, zsh = 0, xsh = 0, ntz = 0, ntx = 0, tile0 = 0, i0 = 0;
+    int n_inj_traces = 1;
+    std::vector<int> colsrc, colmap_raw;
+    std::vector<float> taper;
+    std::vector<int> inj_tile, inj_cell, inj_cptr, inj_tr;
+    std::vector<float> inj_a, inj_wz, inj_wx;
+    std::vector<int> injb_tile, injb_cell, injb_cptr, injb_tr;
+    std::vector<float> injb_a, injb_wz, injb_wx;
+    std::vector<int> ext_tile, ext_rec, ext_pos;
+    std::vector<float> ext_w;
+    int ext_kind = 0;
+    void take_backward(const ShotPlan &o);
+};
+
+struct InjSet {
+    const int *tile = nullptr, *cell = nullptr, *cptr = nullptr, *tr = nullptr;
+    const float *a = nullptr, *wz = nullptr, *wx = nullptr;
+};
+
+struct BatchNeeds {
+    bool backward_injection = false;
+    bool snapshots = false;
+    bool snap_fp16 = false;
+    bool imaging = false;
+    bool born = false;
+};
+
+struct Batch {
+    DevBuf mem;
+    int ns = 0, nblocks = 0;
+    std::vector<sfd::ShotDev> shots_h;
+    std::vector<InjSet> injb;
+    std::vector<const int *> colsrc_d;
+    std::vector<int> blk_shot_h;
+    sfd::ShotDev *shots = nullptr;
+    const int *blk_shot = nullptr;
+};
+
+void second_difference(std::vector<float> &w, float dt);
+void sfd_host_mksinc(float *sinc, int lsinc, float shift);
+
+struct Engine {
+    sfd_grid g;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::vector<float> vel_h, velb_h;
+    float *d_vel = nullptr, *d_velb = nullptr;
+    bool have_model = false;
+    int tz = 64, tx = 32;
+    bool autotune_pending = false;
+    int xcd_remap = 1;
+    int graph_chunk = 200;
+    bool use_graphs = true;
+    double timing[4] = {0, 0, 0, 0};
+    double pts = 0;
+    Batch fwd_batch;
+    DevBuf scratch, fwi_mem;
+
+    explicit Engine(const sfd_grid &grid);
+    ~Engine();
+    void set_model(const float *vel, const float *vel_bwd);
+    void forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind);
+    void rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
+             bool lsrtm);
+    void born(const sfd_shot *shots, int ns, const float *refl, float *traces_out);
+    void fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
+                      const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
+
+    static bool tile_supported(int TZ, int TX);
+    void check_shots(const sfd_shot *shots, int nshots) const;
+    float padded_vel(const std::vector<float> &vh, const ShotPlan &p, int icz, int icx) const;
+    void plan_shot(const sfd_shot &s, ShotPlan &p) const;
+    void plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz, const float *sx,
+                        int kind) const;
+    void plan_extraction(ShotPlan &p, const std::vector<float> &vh, const sfd_shot &s, int kind) const;
+    void build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeeds &need);
+    void upload_shots(Batch &B);
+    void prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float *d_model);
+    void zero_wavefields(Batch &B, std::vector<ShotPlan> &plans);
+    sfd::StepArgs base_args(const Batch &B) const;
+    void launch_step(const Batch &B, sfd::StepArgs A, bool adj);
+    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, bool fp16_scale);
+    double count_pts(const std::vector<ShotPlan> &plans) const;
+    void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);
+    void gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image, float *illum);
+};

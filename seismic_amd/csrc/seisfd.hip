@@ -1,0 +1,963 @@
+// seisfd.hip — host side of the MI355X acoustic FD engine and its C-ABI
+// (include/seisfd.h).
+//
+// The reference's per-shot Fortran drivers (progs/modeling, progs/rtm,
+// progs/afwi) become batched calls: every shot of a call is windowed, padded
+// and planned on the host once, then all shots advance together, one fused
+// step kernel launch per time step (fd_kernels.h), captured into HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/seisfd.h"
+#include "fd_kernels.h"
+#include "engine.h"
+
+using namespace sfd;
+
+// ---------------------------------------------------------------------------
+// Reference arithmetic needed on the host (FP32, same operation order).
+
+static inline int nint_f(float x) { return (int)roundf(x); }   // Fortran nint
+
+// ModelSpaceTypes.f90:330-337 find_i_d
+static inline void find_i_d(int *ic, float *dc, float c, float o, float d) {
+    *ic = nint_f((c - o) / d) + 1;
+    *dc = c - (o + (float)(*ic - 1) * d);
+}
+
+// Interpolate.f90:15-42 mksinc
+void sfd_host_mksinc(float *sinc, int lsinc, float shift) {
+    const float pi = 3.141592654f;
+    float fsum = 0.f, fact, pos;
+    for (int n = 1; n <= lsinc; n++) {
+        if (n <= (int)((float)(lsinc / 2 + 1) + shift))
+            fact = ((float)(n - 1) - shift) / (float)(lsinc / 2);
+        else
+            fact = ((float)(lsinc - n) + shift) / (float)(lsinc / 2);
+        if (fact <= 0.f) fact = 0.f;
+        pos = pi * ((float)(n - (lsinc / 2 + 1)) - shift);
+        if (fabsf(pos) > 0.005f)
+            sinc[n - 1] = fact * sinf(pos) / pos;
+        else
+            sinc[n - 1] = fact * (1.f - pos * pos / 6.f);
+        fsum = fsum + sinc[n - 1];
+    }
+    for (int n = 1; n <= lsinc; n++) sinc[n - 1] = sinc[n - 1] / fsum;
+}
+
+// Taper.f90:14-44 compute_taper / cos_sin_taper
+static void host_taper(float *t, int n) {
+    const float pi = 3.14159265359f;
+    const int nt = std::min(n / 2, 10);
+    const int m = std::min(nt, n);
+    for (int i = 0; i < n; i++) t[i] = 1.f;
+    for (int i = 1; i <= m; i++) {
+        const float s = sinf(pi * (float)(i - 1) / (float)(2 * (m - 1)));
+        t[i - 1] = s * s;
+    }
+    const int b = std::max(1, n - nt);
+    for (int i = b; i <= n; i++) {
+        const float c = cosf(pi * (float)(i - b) / (float)(2 * nt));
+        t[i - 1] = c * c;
+    }
+}
+
+// ExtractPadModel.f90:19-85 extract_coord_source_receiver_patch + nxw (:183)
+static void shot_window(const sfd_grid &g, const sfd_shot &s, float *oxw, int *nxw) {
+    float sx = s.sx, gx1 = s.rx[0], gxn = s.rx[0];
+    for (int i = 1; i < s.ntr; i++) {
+        gx1 = std::min(gx1, s.rx[i]);
+        gxn = std::max(gxn, s.rx[i]);
+    }
+    const float minx = std::min(sx, gx1), maxx = std::max(sx, gxn);
+    const float mx1 = (gx1 + sx) / 2, mxn = (gxn + sx) / 2;
+    const float cx = mx1 + fabsf(mxn - mx1) / 2;
+    *oxw = std::min(minx, cx - g.aperture_x / 2);
+    const float endx = std::max(maxx, cx + g.aperture_x / 2);
+    *nxw = nint_f((endx - *oxw) / g.dx) + 1;
+}
+
+// ---------------------------------------------------------------------------
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            char b_[256];                                                              \
+            snprintf(b_, sizeof b_, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                              \
+            throw EngineError(SFD_EHIP, b_);                                           \
+        }                                                                              \
+    } while (0)
+
+void DevBuf::reserve(size_t n) {
+    if (n <= cap) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        p = nullptr;
+        throw EngineError(SFD_ENOMEM, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    cap = n;
+}
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+
+// Bump allocator over one device buffer: plan offsets, then allocate once.
+struct Layout {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = (off + 255) & ~(size_t)255;
+        off = o + bytes;
+        return o;
+    }
+};
+
+static void check_grid(const sfd_grid &g) {
+    if (g.nz < 2 || g.nx < 1 || !(g.dz > 0) || !(g.dx > 0) || !(g.dt > 0))
+        throw EngineError(SFD_EINVAL, "bad grid dimensions or spacing");
+    if (g.ntaper < 10) throw EngineError(SFD_EINVAL, "ntaper must be >= 10 (Higdon bands and injection must not overlap)");
+    if (g.lsinc != 7) throw EngineError(SFD_EUNSUPPORTED, "only lsinc=7 is implemented");
+    if (g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
+    if (g.rec_type != 0 || g.shot_type != 0 || g.surf_type != 0)
+        throw EngineError(SFD_EUNSUPPORTED, "ghost sources/receivers and free surface are not implemented");
+}
+
+// ReadSouVelRho.f90:603-675 vel_check (2D: Courant <= 0.45)
+static void vel_check(const sfd_grid &g, const float *v, size_t n) {
+    float vmax = v[0];
+    for (size_t i = 1; i < n; i++) vmax = std::max(vmax, v[i]);
+    const float stab = vmax * g.dt / std::min(g.dz, g.dx);
+    if (stab > 0.45f) {
+        char b[160];
+        snprintf(b, sizeof b, "Courant number %.6g > 0.45: unstable (largest stable dt %.6g)", stab,
+                 0.45f / stab * g.dt);
+        throw EngineError(SFD_EUNSTABLE, b);
+    }
+}
+
+Engine::Engine(const sfd_grid &grid) : g(grid) {
+    check_grid(g);
+    HIPCHK(hipStreamCreate(&stream));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventCreate(&ev2));
+}
+
+Engine::~Engine() {
+    if (stream) (void)hipStreamDestroy(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (ev2) (void)hipEventDestroy(ev2);
+    if (d_vel) (void)hipFree(d_vel);
+    if (d_velb) (void)hipFree(d_velb);
+}
+
+void Engine::set_model(const float *vel, const float *vel_bwd) {
+    const size_t n = (size_t)g.nz * g.nx;
+    if (!vel) throw EngineError(SFD_EINVAL, "vel is NULL");
+    for (size_t i = 0; i < n; i++)
+        if (!(vel[i] > 0.f) || !std::isfinite(vel[i])) throw EngineError(SFD_EINVAL, "velocity must be finite and > 0");
+    vel_check(g, vel, n);
+    vel_h.assign(vel, vel + n);
+    if (!d_vel) HIPCHK(hipMalloc(&d_vel, n * sizeof(float)));
+    HIPCHK(hipMemcpy(d_vel, vel, n * sizeof(float), hipMemcpyHostToDevice));
+    if (vel_bwd) {
+        vel_check(g, vel_bwd, n);
+        velb_h.assign(vel_bwd, vel_bwd + n);
+        if (!d_velb) HIPCHK(hipMalloc(&d_velb, n * sizeof(float)));
+        HIPCHK(hipMemcpy(d_velb, vel_bwd, n * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        velb_h.clear();
+        if (d_velb) (void)hipFree(d_velb);
+        d_velb = nullptr;
+    }
+    have_model = true;
+}
+
+// Padded velocity at a Fortran window cell (icz, icx) (1-based, may lie in
+// the pad): the edge-replicated value model_pad would hold there.
+float Engine::padded_vel(const std::vector<float> &vh, const ShotPlan &p, int icz, int icx) const {
+    int wz = std::min(std::max(icz, 1), g.nz) - 1;
+    int wx = std::min(std::max(icx, 1), p.nxw) - 1;
+    return vh[(size_t)p.colsrc[wx] * g.nz + wz];
+}
+
+void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
+    if (s.nt < 1 || s.ntr < 1 || !s.rz || !s.rx) throw EngineError(SFD_EINVAL, "shot needs nt >= 1 and >= 1 receiver");
+    shot_window(g, s, &p.oxw, &p.nxw);
+    if (p.nxw < 1) throw EngineError(SFD_EINVAL, "empty shot window");
+    p.nt = s.nt;
+    p.ntr = s.ntr;
+    p.ntsnap = s.nt / g.snapi;
+    p.NZ = g.nz + 2 * g.ntaper + 8;
+    p.NX = p.nxw + 2 * g.ntaper + 8;
+    p.i0 = 4 + g.ntaper;
+    const int rz_ = (p.NZ - 1) % tz;
+    p.zsh = rz_ >= 9 ? 0 : 9 - rz_;
+    const int rx_ = (p.NX - 1) % tx;
+    p.xsh = rx_ >= 9 ? 0 : 9 - rx_;
+    p.ntz = (p.NZ + p.zsh + tz - 1) / tz;
+    p.ntx = (p.NX + p.xsh + tx - 1) / tx;
+    p.zoff = p.zsh;
+    p.LDZ = ((p.zoff + p.NZ + 31) / 32) * 32;
+    // nearest-neighbour window columns (ExtractPadModel.f90:516-522)
+    p.colsrc.resize(p.nxw);
+    for (int j = 1; j <= p.nxw; j++) {
+        const float cxx = (float)(j - 1) * g.dx + p.oxw;
+        int ix = nint_f((cxx - g.ox) / g.dx) + 1;
+        p.colmap_raw.push_back(ix);
+        ix = std::min(std::max(ix, 1), g.nx);
+        p.colsrc[j - 1] = ix - 1;
+    }
+    p.taper.resize(p.nxw);
+    host_taper(p.taper.data(), p.nxw);
+}
+
+// Tile of a padded cell, and its local cell index.
+static inline void cell_tile(const ShotPlan &p, int TZ, int TX, int iz, int ix, int *t, int *local) {
+    const int tzi = (iz + p.zsh) / TZ, txi = (ix + p.xsh) / TX;
+    const int z0 = tzi * TZ - p.zsh, x0 = txi * TX - p.xsh;
+    *t = tzi + p.ntz * txi;
+    *local = (iz - z0) + TZ * (ix - x0);
+}
+
+// Injection plan: kind 1 = Injection_sinc (gain v^2, Injection.f90:151-194),
+// 2 = Injection_simple_noomp (v^2, :63-104), 3 = Injection_LSRTM_sinc (1/(dz dx), :239-272).
+void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz_, const float *sx_,
+                            int kind) const {
+    struct C { int iz, ix, tr; float a, wz, wx; };
+    std::vector<C> cs;
+    cs.reserve((size_t)nsrc * 49);
+    float wzs[7], wxs[7];
+    for (int k = 0; k < nsrc; k++) {
+        int icz, icx;
+        float dcz, dcx;
+        find_i_d(&icz, &dcz, sz_[k], g.oz, g.dz);
+        find_i_d(&icx, &dcx, sx_[k], p.oxw, g.dx);
+        const int iz = icz - 1 + p.i0, ix = icx - 1 + p.i0;
+        if (iz - 3 < 4 || iz + 3 > p.NZ - 5 || ix - 3 < 4 || ix + 3 > p.NX - 5)
+            throw EngineError(SFD_EINVAL, "source/receiver footprint leaves the FD region");
+        float a;
+        if (kind == 3) {
+            a = 1.f / (g.dz * g.dx);
+        } else {
+            const float v = padded_vel(vh, p, icz, icx);
+            a = v * v;
+        }
+        if (kind == 2) {
+            cs.push_back({iz, ix, k, a, 1.f, 1.f});
+            continue;
+        }
+        sfd_host_mksinc(wzs, 7, dcz / g.dz);
+        sfd_host_mksinc(wxs, 7, dcx / g.dx);
+        for (int j = -3; j <= 3; j++)
+            for (int i = -3; i <= 3; i++) cs.push_back({iz + i, ix + j, k, a, wzs[3 + i], wxs[3 + j]});
+    }
+    // group by cell, keeping trace order within a cell (stable sort)
+    const int ntile = p.ntz * p.ntx;
+    std::vector<std::pair<long, int>> key(cs.size());
+    for (size_t i = 0; i < cs.size(); i++) {
+        int t, loc;
+        cell_tile(p, tz, tx, cs[i].iz, cs[i].ix, &t, &loc);
+        key[i] = {(long)t * (long)(tz * tx) + loc, (int)i};
+    }
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<long, int> &a, const std::pair<long, int> &b) {
+        return a.first < b.first;
+    });
+    p.inj_tile.assign(ntile + 1, 0);
+    p.inj_cell.clear(); p.inj_cptr.clear(); p.inj_a.clear(); p.inj_wz.clear(); p.inj_wx.clear(); p.inj_tr.clear();
+    long prev = -1;
+    for (size_t i = 0; i < key.size(); i++) {
+        const C &c = cs[key[i].second];
+        if (key[i].first != prev) {
+            const int t = (int)(key[i].first / (tz * tx));
+            p.inj_cell.push_back((int)(key[i].first % (tz * tx)));
+            p.inj_cptr.push_back((int)p.inj_a.size());
+            p.inj_tile[t + 1]++;
+            prev = key[i].first;
+        }
+        p.inj_a.push_back(c.a);
+        p.inj_wz.push_back(c.wz);
+        p.inj_wx.push_back(c.wx);
+        p.inj_tr.push_back(c.tr);
+    }
+    p.inj_cptr.push_back((int)p.inj_a.size());
+    for (int t = 0; t < ntile; t++) p.inj_tile[t + 1] += p.inj_tile[t];
+    for (size_t c = 0; c + 1 < p.inj_cptr.size(); c++)
+        if (p.inj_cptr[c + 1] - p.inj_cptr[c] > 255) throw EngineError(SFD_EUNSUPPORTED, "more than 255 injections in one cell");
+    if (p.inj_a.size() >= (1u << 23)) throw EngineError(SFD_EUNSUPPORTED, "injection plan too large");
+}
+
+// Extraction plan (Extraction.f90:71-246).
+void Engine::plan_extraction(ShotPlan &p, const std::vector<float> &vh, const sfd_shot &s, int kind) const {
+    const int ntile = p.ntz * p.ntx;
+    struct R { int t, loc, r; float w[16]; };
+    std::vector<R> rs;
+    for (int r = 0; r < s.ntr; r++) {
+        int icz, icx;
+        float dcz, dcx;
+        find_i_d(&icz, &dcz, s.rz[r], g.oz, g.dz);
+        find_i_d(&icx, &dcx, s.rx[r], p.oxw, g.dx);
+        const int iz = icz - 1 + p.i0, ix = icx - 1 + p.i0;
+        if (iz - 3 < 0 || iz + 3 >= p.NZ || ix - 3 < 0 || ix + 3 >= p.NX)
+            throw EngineError(SFD_EINVAL, "receiver outside the padded grid");
+        R e;
+        memset(&e, 0, sizeof e);
+        cell_tile(p, tz, tx, iz, ix, &e.t, &e.loc);
+        e.r = r;
+        if (kind == EXT_SINC_V2) {
+            const float v = padded_vel(vh, p, icz, icx);
+            e.w[0] = v * v;
+        } else if (kind == EXT_SINC_LSRTM) {
+            e.w[0] = 1.f / (g.dz * g.dx);
+        } else {
+            e.w[0] = 1.f;
+        }
+        if (kind != EXT_NEAREST) {
+            sfd_host_mksinc(e.w + 1, 7, dcz / g.dz);
+            sfd_host_mksinc(e.w + 8, 7, dcx / g.dx);
+        }
+        rs.push_back(e);
+    }
+    std::stable_sort(rs.begin(), rs.end(), [](const R &a, const R &b) { return a.t < b.t; });
+    p.ext_tile.assign(ntile + 1, 0);
+    p.ext_rec.clear(); p.ext_pos.clear(); p.ext_w.clear();
+    for (const R &e : rs) {
+        p.ext_tile[e.t + 1]++;
+        p.ext_rec.push_back(e.r);
+        p.ext_pos.push_back((e.loc % tz) | ((e.loc / tz) << 16));
+        p.ext_w.insert(p.ext_w.end(), e.w, e.w + 16);
+    }
+    for (int t = 0; t < ntile; t++) p.ext_tile[t + 1] += p.ext_tile[t];
+    p.ext_kind = kind;
+}
+
+// ---------------------------------------------------------------------------
+// Batch: device memory of all shots of one call.
+
+template <class T>
+static size_t put(Layout &L, std::vector<std::pair<size_t, const void *>> &up,
+                  std::vector<size_t> &upn, const std::vector<T> &v) {
+    const size_t o = L.take(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (!v.empty()) {
+        up.push_back({o, v.data()});
+        upn.push_back(v.size() * sizeof(T));
+    }
+    return o;
+}
+
+void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeeds &need) {
+    const int ns = (int)plans.size();
+    Layout L;
+    std::vector<std::pair<size_t, const void *>> up;
+    std::vector<size_t> upn;
+    struct Off {
+        size_t u[4], v2, vel, hz, hx, colsrc, taper;
+        size_t inj_tile, inj_cell, inj_cptr, inj_a, inj_wz, inj_wx, inj_tr, inj_traces;
+        size_t injb_tile, injb_cell, injb_cptr, injb_a, injb_wz, injb_wx, injb_tr;
+        size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
+        size_t snap, snap_scale, snap_max, image, illum, born_m;
+    };
+    std::vector<Off> off(ns);
+    int tile0 = 0;
+    B.blk_shot_h.clear();
+    B.colsrc_d.clear();
+    for (int s = 0; s < ns; s++) {
+        ShotPlan &p = plans[s];
+        Off &o = off[s];
+        const size_t nf = (size_t)p.LDZ * p.NX;
+        for (int k = 0; k < 4; k++) o.u[k] = L.take(nf * sizeof(float));
+        o.v2 = L.take(nf * sizeof(float));
+        o.vel = L.take(nf * sizeof(float));
+        o.hz = L.take((size_t)16 * (p.NX - 8) * sizeof(float));
+        o.hx = L.take((size_t)16 * (p.NZ - 8) * sizeof(float));
+        o.colsrc = put(L, up, upn, p.colsrc);
+        o.taper = put(L, up, upn, p.taper);
+        o.inj_tile = put(L, up, upn, p.inj_tile);
+        o.inj_cell = put(L, up, upn, p.inj_cell);
+        o.inj_cptr = put(L, up, upn, p.inj_cptr);
+        o.inj_a = put(L, up, upn, p.inj_a);
+        o.inj_wz = put(L, up, upn, p.inj_wz);
+        o.inj_wx = put(L, up, upn, p.inj_wx);
+        o.inj_tr = put(L, up, upn, p.inj_tr);
+        o.inj_traces = L.take((size_t)p.nt * std::max(p.n_inj_traces, 1) * sizeof(float));
+        if (need.backward_injection) {
+            o.injb_tile = put(L, up, upn, p.injb_tile);
+            o.injb_cell = put(L, up, upn, p.injb_cell);
+            o.injb_cptr = put(L, up, upn, p.injb_cptr);
+            o.injb_a = put(L, up, upn, p.injb_a);
+            o.injb_wz = put(L, up, upn, p.injb_wz);
+            o.injb_wx = put(L, up, upn, p.injb_wx);
+            o.injb_tr = put(L, up, upn, p.injb_tr);
+        }
+        o.ext_tile = put(L, up, upn, p.ext_tile);
+        o.ext_rec = put(L, up, upn, p.ext_rec);
+        o.ext_pos = put(L, up, upn, p.ext_pos);
+        o.ext_w = put(L, up, upn, p.ext_w);
+        o.traces = L.take((size_t)p.nt * p.ntr * sizeof(float));
+        const size_t npix = (size_t)g.nz * p.nxw;
+        o.snap = L.take(need.snapshots ? (size_t)std::max(p.ntsnap, 1) * npix * (need.snap_fp16 ? 2 : 4) : 16);
+        o.snap_scale = L.take((size_t)std::max(p.ntsnap, 1) * sizeof(float));
+        o.snap_max = L.take(sizeof(unsigned));
+        o.image = L.take(need.imaging ? npix * sizeof(float) : 16);
+        o.illum = L.take(need.imaging ? npix * sizeof(float) : 16);
+        o.born_m = L.take(need.born ? npix * sizeof(float) : 16);
+        p.tile0 = tile0;
+        const int nt_ = p.ntz * p.ntx;
+        for (int k = 0; k < nt_; k++) B.blk_shot_h.push_back(s);
+        tile0 += nt_;
+    }
+    B.nblocks = tile0;
+    const size_t o_blk = put(L, up, upn, B.blk_shot_h);
+    const size_t o_shots = L.take(sizeof(ShotDev) * ns);
+    B.mem.reserve(L.off);
+    char *base = (char *)B.mem.p;
+    // zero everything once (wavefields, accumulators, traces), then upload plans
+    HIPCHK(hipMemsetAsync(base, 0, L.off, stream));
+    for (size_t i = 0; i < up.size(); i++)
+        HIPCHK(hipMemcpyAsync(base + up[i].first, up[i].second, upn[i], hipMemcpyHostToDevice, stream));
+    B.shots_h.assign(ns, ShotDev());
+    B.injb.assign(ns, InjSet());
+    for (int s = 0; s < ns; s++) {
+        ShotPlan &p = plans[s];
+        Off &o = off[s];
+        ShotDev &S = B.shots_h[s];
+        memset(&S, 0, sizeof S);
+        for (int k = 0; k < 4; k++) S.u[k] = (float *)(base + o.u[k]);
+        S.v2 = (const float *)(base + o.v2);
+        S.vel = (const float *)(base + o.vel);
+        S.hz = (const float *)(base + o.hz);
+        S.hx = (const float *)(base + o.hx);
+        S.NZ = p.NZ; S.NX = p.NX; S.LDZ = p.LDZ; S.zoff = p.zoff;
+        S.zsh = p.zsh; S.xsh = p.xsh; S.ntz = p.ntz; S.ntx = p.ntx; S.tile0 = p.tile0;
+        S.nz = g.nz; S.nxw = p.nxw; S.i0 = p.i0; S.nt = p.nt; S.ntsnap = p.ntsnap;
+        S.inj_tile = (const int *)(base + o.inj_tile);
+        S.inj_cell = (const int *)(base + o.inj_cell);
+        S.inj_cptr = (const int *)(base + o.inj_cptr);
+        S.inj_a = (const float *)(base + o.inj_a);
+        S.inj_wz = (const float *)(base + o.inj_wz);
+        S.inj_wx = (const float *)(base + o.inj_wx);
+        S.inj_tr = (const int *)(base + o.inj_tr);
+        S.inj_traces = (const float *)(base + o.inj_traces);
+        if (need.backward_injection) {
+            InjSet &I = B.injb[s];
+            I.tile = (const int *)(base + o.injb_tile);
+            I.cell = (const int *)(base + o.injb_cell);
+            I.cptr = (const int *)(base + o.injb_cptr);
+            I.a = (const float *)(base + o.injb_a);
+            I.wz = (const float *)(base + o.injb_wz);
+            I.wx = (const float *)(base + o.injb_wx);
+            I.tr = (const int *)(base + o.injb_tr);
+        }
+        S.ext_tile = p.ext_tile.empty() ? nullptr : (const int *)(base + o.ext_tile);
+        S.ext_rec = (const int *)(base + o.ext_rec);
+        S.ext_pos = (const int *)(base + o.ext_pos);
+        S.ext_w = (const float *)(base + o.ext_w);
+        S.traces = (float *)(base + o.traces);
+        S.ext_kind = p.ext_kind;
+        S.snap = base + o.snap;
+        S.snap_scale = need.snap_fp16 ? (float *)(base + o.snap_scale) : nullptr;
+        S.snap_max = (unsigned *)(base + o.snap_max);
+        S.image = (float *)(base + o.image);
+        S.illum = (float *)(base + o.illum);
+        S.born_m = (const float *)(base + o.born_m);
+        S.taper = (const float *)(base + o.taper);
+        S.dt2snap = (g.dt * (float)g.snapi) * (g.dt * (float)g.snapi);
+        B.colsrc_d.push_back((const int *)(base + o.colsrc));
+    }
+    B.blk_shot = (const int *)(base + o_blk);
+    B.shots = (ShotDev *)(base + o_shots);
+    B.ns = ns;
+    upload_shots(B);
+}
+
+void Engine::upload_shots(Batch &B) {
+    HIPCHK(hipMemcpyAsync((void *)B.shots, B.shots_h.data(), sizeof(ShotDev) * B.ns, hipMemcpyHostToDevice, stream));
+}
+
+// Window + pad every shot's model and build its Higdon tables.
+void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float *d_model) {
+    for (int s = 0; s < B.ns; s++) {
+        const ShotPlan &p = plans[s];
+        ShotDev &S = B.shots_h[s];
+        dim3 blk(256), grd((p.NZ + 255) / 256, p.NX);
+        hipLaunchKernelGGL(window_pad_kernel, grd, blk, 0, stream, d_model, g.nz, B.colsrc_d[s], p.nxw, g.ntaper,
+                           p.NZ, p.NX, p.LDZ, p.zoff, (float *)S.vel, (float *)S.v2);
+        const int nk = std::max(p.NX, p.NZ);
+        hipLaunchKernelGGL(higdon_tables_kernel, dim3((nk + 255) / 256), dim3(256), 0, stream, S.vel, p.NZ, p.NX,
+                           p.LDZ, p.zoff, g.dt, g.dz, g.dx, (float *)S.hz, (float *)S.hx);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void Engine::zero_wavefields(Batch &B, std::vector<ShotPlan> &plans) {
+    for (int s = 0; s < B.ns; s++) {
+        const size_t nf = (size_t)plans[s].LDZ * plans[s].NX * sizeof(float);
+        for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(B.shots_h[s].u[k], 0, nf, stream));
+    }
+}
+
+StepArgs Engine::base_args(const Batch &B) const {
+    StepArgs A;
+    memset(&A, 0, sizeof A);
+    A.shots = B.shots;
+    A.blk_shot = B.blk_shot;
+    A.nblocks = B.nblocks;
+    A.xcd_remap = xcd_remap;
+    A.snapi = g.snapi;
+    // FDcoefs.f90:25-36 scaled as FDtypes.f90:367-387 (coefpower 2)
+    const float c0 = -205.f / 72.f, c1 = 8.f / 5, c2 = -1.f / 5.f, c3 = 8.f / 315.f, c4 = -1.f / 560.f;
+    const float dzi = 1.f / g.dz, dxi = 1.f / g.dx;
+    const float dz2 = dzi * dzi, dx2 = dxi * dxi;
+    const float cc[5] = {c0, c1, c2, c3, c4};
+    for (int k = 0; k < 5; k++) {
+        A.c[k] = cc[k] * dz2;
+        A.c[5 + k] = cc[k] * dx2;
+    }
+    const float dt2 = g.dt * g.dt;
+    A.c12dt2 = 12.f * dt2;
+    A.div11 = 1.f / 11.f;
+    return A;
+}
+
+typedef void (*StepFn)(StepArgs);
+
+static StepFn pick_kernel(int TZ, int TX, bool adj) {
+#define K(a, b)                                                                             \
+    if (TZ == a && TX == b) return adj ? (StepFn)fd_step_kernel<a, b, true> : (StepFn)fd_step_kernel<a, b, false>;
+    K(64, 32) K(64, 64) K(32, 32) K(32, 64) K(128, 32) K(64, 128)
+#undef K
+    return nullptr;
+}
+
+bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false) != nullptr; }
+
+void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
+    StepFn f = pick_kernel(tz, tx, adj);
+    if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
+    int grid = B.nblocks;
+    if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), 0, stream, A);
+}
+
+// Run nsteps of the lockstep time loop.  Steps are recorded into HIP graphs
+// of `chunk` launches and replayed (the per-step arguments differ only in n).
+void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, bool fp16_scale) {
+    const int chunk = graph_chunk;
+    for (int n0 = 0; n0 < nsteps; n0 += chunk) {
+        const int n1 = std::min(nsteps, n0 + chunk);
+        bool use_graph = chunk > 1 && use_graphs;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (use_graph) HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int n = n0; n < n1; n++) {
+            A.n = n;
+            A.rot = n & 3;
+            launch_step(B, A, adj);
+            if (fp16_scale) {
+                // forward time: after step it=n+1, publish the max for snapshot (n+2)/snapi
+                const int itn = n + 2;
+                if (itn % g.snapi == 0)
+                    hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
+                                       itn / g.snapi);
+            }
+        }
+        if (use_graph) {
+            HIPCHK(hipStreamEndCapture(stream, &graph));
+            HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphLaunch(exec, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+        }
+    }
+    HIPCHK(hipGetLastError());
+}
+
+static int max_nt(const std::vector<ShotPlan> &plans) {
+    int m = 0;
+    for (const ShotPlan &p : plans) m = std::max(m, p.nt);
+    return m;
+}
+
+double Engine::count_pts(const std::vector<ShotPlan> &plans) const {
+    double pts = 0;
+    for (const ShotPlan &p : plans) pts += (double)(p.NZ - 8) * (double)(p.NX - 8) * (double)p.nt;
+    return pts;
+}
+
+// Upload the traces a shot injects (nt per trace, time fastest).
+void Engine::upload_inj_traces(Batch &B, int s, const float *tr, size_t count) {
+    HIPCHK(hipMemcpyAsync((void *)B.shots_h[s].inj_traces, tr, count * sizeof(float), hipMemcpyHostToDevice, stream));
+}
+
+void Engine::check_shots(const sfd_shot *shots, int nshots) const {
+    if (!have_model) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model first");
+    if (!shots || nshots < 1) throw EngineError(SFD_EINVAL, "no shots");
+    for (int s = 0; s < nshots; s++)
+        if (!shots[s].wavelet || shots[s].nt < 1 || shots[s].ntr < 1) throw EngineError(SFD_EINVAL, "bad shot descriptor");
+}
+
+// ---- forward modeling (Acoustic_modeling_sep.f90 / AMOD) -------------------
+void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind) {
+    check_shots(shots, ns);
+    if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract kind");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<ShotPlan> plans(ns);
+    for (int s = 0; s < ns; s++) {
+        plan_shot(shots[s], plans[s]);
+        plan_injection(plans[s], vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
+        plans[s].n_inj_traces = 1;
+        plan_extraction(plans[s], vel_h, shots[s], extract_kind);
+    }
+    BatchNeeds need;
+    build_batch(fwd_batch, plans, need);
+    for (int s = 0; s < ns; s++) upload_inj_traces(fwd_batch, s, shots[s].wavelet, shots[s].nt);
+    prepare_models(fwd_batch, plans, d_vel);
+    StepArgs A = base_args(fwd_batch);
+    A.dir = 1;
+    A.inject = 1;
+    A.extract = 1;
+    auto t1 = std::chrono::steady_clock::now();
+    HIPCHK(hipEventRecord(ev0, stream));
+    run_loop(fwd_batch, A, false, max_nt(plans), false);
+    HIPCHK(hipEventRecord(ev1, stream));
+    HIPCHK(hipEventSynchronize(ev1));
+    size_t off = 0;
+    for (int s = 0; s < ns; s++) {
+        const size_t n = (size_t)plans[s].nt * plans[s].ntr;
+        HIPCHK(hipMemcpy(traces_out + off, fwd_batch.shots_h[s].traces, n * sizeof(float), hipMemcpyDeviceToHost));
+        off += n;
+    }
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    timing[0] = ms; timing[1] = ms; timing[2] = 0;
+    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    pts = count_pts(plans);
+}
+
+// Scatter window images to the full model (mod_copy_image) and add into host arrays.
+void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image, float *illum) {
+    // inverse map: model column -> (shot, window column), shot order
+    std::vector<std::vector<int>> inv(g.nx);
+    for (int s = 0; s < B.ns; s++)
+        for (int j = 0; j < plans[s].nxw; j++) {
+            const int ix = plans[s].colmap_raw[j];
+            if (ix < 1 || ix > g.nx) continue;   // ExtractPadModel.f90:686
+            inv[ix - 1].push_back((s << 20) | j);
+        }
+    std::vector<int> ptr(g.nx + 1, 0), idx;
+    for (int ix = 0; ix < g.nx; ix++) {
+        ptr[ix + 1] = ptr[ix] + (int)inv[ix].size();
+        idx.insert(idx.end(), inv[ix].begin(), inv[ix].end());
+    }
+    const size_t N = (size_t)g.nz * g.nx;
+    Layout L;
+    const size_t o_ptr = L.take(ptr.size() * 4), o_idx = L.take(std::max<size_t>(idx.size(), 1) * 4);
+    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_out = L.take(N * 4);
+    scratch.reserve(L.off);
+    char *base = (char *)scratch.p;
+    HIPCHK(hipMemcpyAsync(base + o_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, stream));
+    if (!idx.empty()) HIPCHK(hipMemcpyAsync(base + o_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, stream));
+    std::vector<float> h(N);
+    for (int pass = 0; pass < 2; pass++) {
+        float *dst = pass == 0 ? image : illum;
+        if (!dst) continue;
+        std::vector<CopyImg> ci(B.ns);
+        for (int s = 0; s < B.ns; s++) ci[s] = {pass == 0 ? B.shots_h[s].image : B.shots_h[s].illum, nullptr, plans[s].nxw};
+        HIPCHK(hipMemcpyAsync(base + o_ci, ci.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemsetAsync(base + o_out, 0, N * 4, stream));
+        hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
+                           (const CopyImg *)(base + o_ci), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
+                           (const int *)(base + o_idx), (float *)(base + o_out));
+        HIPCHK(hipMemcpyAsync(h.data(), base + o_out, N * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + h[q];
+    }
+}
+
+// ---- RTM / LSRTM (to_disk_to_memory_RTM.f90:358-579) -------------------------
+void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
+                 bool lsrtm) {
+    check_shots(shots, ns);
+    if (!traces_in) throw EngineError(SFD_EINVAL, "traces_in is NULL");
+    if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<ShotPlan> plans(ns);
+    for (int s = 0; s < ns; s++) {
+        ShotPlan &p = plans[s];
+        plan_shot(shots[s], p);
+        plan_injection(p, vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
+        // backward injection: receivers as sources (Injection_sinc / Injection_LSRTM_sinc)
+        ShotPlan tmp = p;
+        plan_injection(tmp, velb_h.empty() ? vel_h : velb_h, shots[s].ntr, shots[s].rz, shots[s].rx, lsrtm ? 3 : 1);
+        p.take_backward(tmp);
+        p.n_inj_traces = std::max(1, shots[s].ntr);
+    }
+    BatchNeeds need;
+    need.backward_injection = true;
+    need.snapshots = true;
+    need.snap_fp16 = snap_mode == SFD_SNAP_FP16S;
+    need.imaging = true;
+    Batch &B = fwd_batch;
+    build_batch(B, plans, need);
+    // forward source (second-differenced for LSRTM, ReadSouVelRho.f90:318-326)
+    for (int s = 0; s < ns; s++) {
+        std::vector<float> w(shots[s].wavelet, shots[s].wavelet + shots[s].nt);
+        if (lsrtm) second_difference(w, g.dt);
+        upload_inj_traces(B, s, w.data(), w.size());
+    }
+    prepare_models(B, plans, d_vel);
+    auto t1 = std::chrono::steady_clock::now();
+    StepArgs A = base_args(B);
+    A.dir = 1;
+    A.inject = 1;
+    A.store_snap = 1;
+    A.snap_fp16 = need.snap_fp16;
+    HIPCHK(hipEventRecord(ev0, stream));
+    run_loop(B, A, false, max_nt(plans), need.snap_fp16);
+    HIPCHK(hipEventRecord(ev1, stream));
+    // backward
+    zero_wavefields(B, plans);
+    if (d_velb) prepare_models(B, plans, d_velb);
+    size_t off = 0;
+    for (int s = 0; s < ns; s++) {
+        ShotDev &S = B.shots_h[s];
+        const InjSet &I = B.injb[s];
+        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_a = I.a;
+        S.inj_wz = I.wz; S.inj_wx = I.wx; S.inj_tr = I.tr;
+        const size_t n = (size_t)shots[s].nt * shots[s].ntr;
+        upload_inj_traces(B, s, traces_in + off, n);
+        off += n;
+        std::vector<float> z((size_t)g.nz * plans[s].nxw, 0.f);
+    }
+    upload_shots(B);
+    StepArgs Ab = base_args(B);
+    Ab.dir = -1;
+    Ab.inject = 1;
+    Ab.snap_fp16 = need.snap_fp16;
+    Ab.imaging = lsrtm ? IMG_LSRTM : IMG_RTM;
+    run_loop(B, Ab, true, max_nt(plans), false);
+    HIPCHK(hipEventRecord(ev2, stream));
+    HIPCHK(hipEventSynchronize(ev2));
+    float ms1 = 0, ms2 = 0;
+    HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+    HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+    timing[0] = ms1 + ms2; timing[1] = ms1; timing[2] = ms2;
+    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    pts = 2 * count_pts(plans);
+    gather_images(B, plans, image, illum);
+}
+
+// ---- Born modeling (to_disk_to_memory_BORN.f90:123-220) ----------------------
+void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *traces_out) {
+    check_shots(shots, ns);
+    if (!refl || !traces_out) throw EngineError(SFD_EINVAL, "NULL refl/traces_out");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<ShotPlan> plans(ns);
+    for (int s = 0; s < ns; s++) {
+        plan_shot(shots[s], plans[s]);
+        plan_injection(plans[s], vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
+        plans[s].n_inj_traces = 1;
+        plan_extraction(plans[s], vel_h, shots[s], EXT_SINC_LSRTM);
+    }
+    BatchNeeds need;
+    need.snapshots = true;
+    need.born = true;
+    Batch &B = fwd_batch;
+    build_batch(B, plans, need);
+    for (int s = 0; s < ns; s++) {
+        std::vector<float> w(shots[s].wavelet, shots[s].wavelet + shots[s].nt);
+        second_difference(w, g.dt);   // readsou with Born=.true.
+        upload_inj_traces(B, s, w.data(), w.size());
+        // reflectivity window (readref + mod_window_pad interior)
+        std::vector<float> m((size_t)g.nz * plans[s].nxw);
+        for (int j = 0; j < plans[s].nxw; j++)
+            for (int i = 0; i < g.nz; i++) m[(size_t)j * g.nz + i] = refl[(size_t)plans[s].colsrc[j] * g.nz + i];
+        HIPCHK(hipMemcpyAsync((void *)B.shots_h[s].born_m, m.data(), m.size() * 4, hipMemcpyHostToDevice, stream));
+    }
+    prepare_models(B, plans, d_vel);
+    auto t1 = std::chrono::steady_clock::now();
+    StepArgs A = base_args(B);
+    A.dir = 1;
+    A.inject = 1;
+    A.store_snap = 1;
+    HIPCHK(hipEventRecord(ev0, stream));
+    run_loop(B, A, false, max_nt(plans), false);
+    HIPCHK(hipEventRecord(ev1, stream));
+    zero_wavefields(B, plans);
+    StepArgs A2 = base_args(B);
+    A2.dir = 1;
+    A2.extract = 1;
+    A2.imaging = IMG_BORN;
+    run_loop(B, A2, false, max_nt(plans), false);
+    HIPCHK(hipEventRecord(ev2, stream));
+    HIPCHK(hipEventSynchronize(ev2));
+    size_t off = 0;
+    for (int s = 0; s < ns; s++) {
+        const size_t n = (size_t)plans[s].nt * plans[s].ntr;
+        HIPCHK(hipMemcpy(traces_out + off, B.shots_h[s].traces, n * sizeof(float), hipMemcpyDeviceToHost));
+        off += n;
+    }
+    float ms1 = 0, ms2 = 0;
+    HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+    HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+    timing[0] = ms1 + ms2; timing[1] = ms1; timing[2] = ms2;
+    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    pts = 2 * count_pts(plans);
+}
+
+void Engine::fwi_gradient(const sfd_shot *, int, const float *, const float *, const float *, const sfd_fwi_opts *,
+                          double *, float *, float *, float *) {
+    throw EngineError(SFD_EUNSUPPORTED, "fwi_gradient not built yet");
+}
+
+void ShotPlan::take_backward(const ShotPlan &o) {
+    injb_tile = o.inj_tile; injb_cell = o.inj_cell; injb_cptr = o.inj_cptr;
+    injb_a = o.inj_a; injb_wz = o.inj_wz; injb_wx = o.inj_wx; injb_tr = o.inj_tr;
+}
+
+void second_difference(std::vector<float> &w, float dt) {
+    // ReadSouVelRho.f90:318-326
+    std::vector<float> o(w);
+    const int nt = (int)w.size();
+    for (int it = 0; it < nt; it++) w[it] = 0.f;
+    for (int it = 2; it <= nt - 1; it++) w[it - 1] = w[it - 1] + (-2.f * o[it - 1] + o[it - 2] + o[it]) / (dt * dt);
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+
+struct sfd_ctx {
+    Engine *e = nullptr;
+    std::string err;
+};
+
+static std::string g_create_err;
+
+template <class F>
+static int guarded(sfd_handle h, F &&f) {
+    if (!h || !h->e) return SFD_EINVAL;
+    try {
+        f(*h->e);
+        h->err.clear();
+        return SFD_OK;
+    } catch (const EngineError &e) {
+        h->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return SFD_EINVAL;
+    }
+}
+
+extern "C" {
+
+int sfd_create(sfd_handle *h, const sfd_grid *g) {
+    if (!h || !g) return SFD_EINVAL;
+    *h = nullptr;
+    try {
+        sfd_ctx *c = new sfd_ctx();
+        c->e = new Engine(*g);
+        *h = c;
+        return SFD_OK;
+    } catch (const EngineError &e) {
+        g_create_err = e.msg;
+        return e.code;
+    }
+}
+
+int sfd_destroy(sfd_handle h) {
+    if (!h) return SFD_EINVAL;
+    delete h->e;
+    delete h;
+    return SFD_OK;
+}
+
+const char *sfd_last_error(sfd_handle h) {
+    if (!h) return g_create_err.c_str();
+    return h->err.c_str();
+}
+
+int sfd_device_info(char *name, int len, int *cu_count, size_t *hbm_bytes) {
+    hipDeviceProp_t p;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return SFD_EHIP;
+    if (name && len > 0) {
+        snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+    return SFD_OK;
+}
+
+int sfd_set_model(sfd_handle h, const float *vel, const float *vel_bwd) {
+    return guarded(h, [&](Engine &e) { e.set_model(vel, vel_bwd); });
+}
+
+int sfd_forward(sfd_handle h, const sfd_shot *shots, int nshots, float *traces_out, int extract_kind) {
+    return guarded(h, [&](Engine &e) {
+        if (!traces_out) throw EngineError(SFD_EINVAL, "traces_out is NULL");
+        e.forward(shots, nshots, traces_out, extract_kind);
+    });
+}
+
+int sfd_rtm(sfd_handle h, const sfd_shot *shots, int nshots, const float *traces_in, float *image, float *illum,
+            int snap_mode) {
+    return guarded(h, [&](Engine &e) { e.rtm(shots, nshots, traces_in, image, illum, snap_mode, false); });
+}
+
+int sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots, const float *traces_in, float *image, float *illum) {
+    return guarded(h, [&](Engine &e) { e.rtm(shots, nshots, traces_in, image, illum, SFD_SNAP_FP32, true); });
+}
+
+int sfd_born(sfd_handle h, const sfd_shot *shots, int nshots, const float *refl, float *traces_out) {
+    return guarded(h, [&](Engine &e) { e.born(shots, nshots, refl, traces_out); });
+}
+
+int sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs, const float *mute,
+                     const float *vpmask, const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out,
+                     float *resid_out) {
+    return guarded(h, [&](Engine &e) {
+        e.fwi_gradient(shots, nshots, dobs, mute, vpmask, opt, misfit, grad, illum_out, resid_out);
+    });
+}
+
+int sfd_last_timing(sfd_handle h, double *ms, double *pts) {
+    return guarded(h, [&](Engine &e) {
+        if (ms)
+            for (int k = 0; k < 4; k++) ms[k] = e.timing[k];
+        if (pts) pts[0] = e.pts;
+    });
+}
+
+int sfd_set_tile(sfd_handle h, int tile_z, int tile_x) {
+    return guarded(h, [&](Engine &e) {
+        if (tile_z <= 0) {
+            e.autotune_pending = true;
+            return;
+        }
+        if (!Engine::tile_supported(tile_z, tile_x)) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
+        e.tz = tile_z;
+        e.tx = tile_x;
+        e.autotune_pending = false;
+    });
+}
+
+int sfd_lbfgs_create(sfd_lbfgs_handle *, int, int, double, double, const float *, int, int, const double *) {
+    return SFD_EUNSUPPORTED;
+}
+int sfd_lbfgs_step(sfd_lbfgs_handle, double *, double, const double *, int *, int *) { return SFD_EUNSUPPORTED; }
+int sfd_lbfgs_destroy(sfd_lbfgs_handle) { return SFD_EUNSUPPORTED; }
+
+}  // extern "C"

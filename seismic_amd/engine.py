@@ -1,0 +1,193 @@
+"""Host-side mirror of the reference drivers over the seisfd C-ABI.
+
+Each function keeps the name and argument meaning of the reference routine it
+replaces (Fortran drivers under progs/ and libs/libwave), on numpy arrays in
+the reference's column-major layout (vel[nz, nx] with z fastest on disk is
+``np.asfortranarray`` here; traces are (ntr, nt) row-major = (nt, ntr)
+column-major).  All compute runs in libseisfd.so on the GPU; there is no CPU
+fallback: a missing library or device raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import sfd_grid, sfd_shot, sfd_fwi_opts
+
+_FP = C.POINTER(C.c_float)
+
+
+class SeisFDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("seisfd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture_x=0.0):
+    """GeneralParam subset (ReadParam.f90:18-36 defaults: ntaper=20, snapi=4, lsinc=7)."""
+    g = sfd_grid()
+    g.nz, g.nx = int(nz), int(nx)
+    g.oz, g.ox, g.dz, g.dx, g.dt = oz, ox, dz, dx, dt
+    g.ntaper, g.lsinc, g.snapi = ntaper, lsinc, snapi
+    g.aperture_x = aperture_x
+    g.rec_type = g.shot_type = g.surf_type = 0
+    return g
+
+
+class Shot:
+    """One source with its receiver spread (sourcevec/datavec TraceSpace)."""
+
+    def __init__(self, sz, sx, wavelet, rz, rx):
+        self.wavelet = np.ascontiguousarray(wavelet, dtype=np.float32)
+        self.rz = np.ascontiguousarray(rz, dtype=np.float32)
+        self.rx = np.ascontiguousarray(rx, dtype=np.float32)
+        self.sz, self.sx = float(sz), float(sx)
+        self.nt = int(self.wavelet.size)
+        self.ntr = int(self.rz.size)
+        if self.rx.size != self.ntr:
+            raise ValueError("rz and rx differ in length")
+
+    def c(self):
+        s = sfd_shot()
+        s.sz, s.sx, s.nt, s.ntr = self.sz, self.sx, self.nt, self.ntr
+        s.wavelet = self.wavelet.ctypes.data_as(_FP)
+        s.rz = self.rz.ctypes.data_as(_FP)
+        s.rx = self.rx.ctypes.data_as(_FP)
+        return s
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _model_buf(v, nz, nx):
+    """(nz, nx) array -> contiguous column-major float32 buffer."""
+    v = np.asarray(v, dtype=np.float32)
+    if v.shape != (nz, nx):
+        raise ValueError("model shape %s != (%d, %d)" % (v.shape, nz, nx))
+    return np.ascontiguousarray(v.T)
+
+
+class Engine:
+    """One seisfd handle (one GPU stream, its device buffers and graphs)."""
+
+    def __init__(self, grid):
+        self.lib = abi.load()
+        self.g = grid
+        h = C.c_void_p()
+        rc = self.lib.sfd_create(C.byref(h), C.byref(grid))
+        if rc != 0:
+            raise SeisFDError(rc, self.lib.sfd_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SeisFDError(rc, self.lib.sfd_last_error(self.h).decode())
+
+    def _shots(self, shots):
+        arr = (sfd_shot * len(shots))(*[s.c() for s in shots])
+        return arr
+
+    def set_model(self, vel, vel_bwd=None):
+        self._vel = _model_buf(vel, self.g.nz, self.g.nx)
+        self._velb = None if vel_bwd is None else _model_buf(vel_bwd, self.g.nz, self.g.nx)
+        self._chk(self.lib.sfd_set_model(self.h, _ptr(self._vel), _ptr(self._velb)))
+
+    def set_tile(self, tz, tx):
+        self._chk(self.lib.sfd_set_tile(self.h, tz, tx))
+
+    # Acoustic_modeling_sep.f90 / AMOD_to_memory
+    def forward(self, shots, extract_kind=abi.SFD_EXTRACT_SINC_V2):
+        tot = sum(s.nt * s.ntr for s in shots)
+        out = np.zeros(tot, dtype=np.float32)
+        self._chk(self.lib.sfd_forward(self.h, self._shots(shots), len(shots), _ptr(out), extract_kind))
+        return self._split(out, shots)
+
+    @staticmethod
+    def _split(flat, shots):
+        res, o = [], 0
+        for s in shots:
+            n = s.nt * s.ntr
+            res.append(flat[o:o + n].reshape(s.ntr, s.nt))
+            o += n
+        return res
+
+    @staticmethod
+    def _cat(traces, shots):
+        return np.ascontiguousarray(np.concatenate(
+            [np.asarray(t, dtype=np.float32).reshape(s.ntr, s.nt).ravel() for t, s in zip(traces, shots)]))
+
+    # RTM_to_memory
+    def rtm(self, shots, traces, snap_mode=abi.SFD_SNAP_FP32, image=None, illum=None):
+        d = self._cat(traces, shots)
+        nz, nx = self.g.nz, self.g.nx
+        img = np.zeros(nz * nx, np.float32) if image is None else np.ascontiguousarray(np.asarray(image, np.float32).T).ravel()
+        ill = np.zeros(nz * nx, np.float32) if illum is None else np.ascontiguousarray(np.asarray(illum, np.float32).T).ravel()
+        self._chk(self.lib.sfd_rtm(self.h, self._shots(shots), len(shots), _ptr(d), _ptr(img), _ptr(ill), snap_mode))
+        return img.reshape(nx, nz).T, ill.reshape(nx, nz).T
+
+    # LSRTM_to_memory
+    def lsrtm(self, shots, traces):
+        d = self._cat(traces, shots)
+        nz, nx = self.g.nz, self.g.nx
+        img = np.zeros(nz * nx, np.float32)
+        ill = np.zeros(nz * nx, np.float32)
+        self._chk(self.lib.sfd_lsrtm(self.h, self._shots(shots), len(shots), _ptr(d), _ptr(img), _ptr(ill)))
+        return img.reshape(nx, nz).T, ill.reshape(nx, nz).T
+
+    # BORN_to_memory
+    def born(self, shots, refl):
+        m = _model_buf(refl, self.g.nz, self.g.nx)
+        tot = sum(s.nt * s.ntr for s in shots)
+        out = np.zeros(tot, dtype=np.float32)
+        self._chk(self.lib.sfd_born(self.h, self._shots(shots), len(shots), _ptr(m), _ptr(out)))
+        return self._split(out, shots)
+
+    # compute_fct_gdt
+    def fwi_gradient(self, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
+                     do_x=0, snap_mode=abi.SFD_SNAP_FP32):
+        nz, nx = self.g.nz, self.g.nx
+        ob = self._cat(dobs, shots)
+        mu = None if mute is None else self._cat(mute, shots)
+        vm = None if vpmask is None else _model_buf(vpmask, nz, nx)
+        o = sfd_fwi_opts()
+        o.illupow, o.rect_z, o.rect_x, o.do_z, o.do_x, o.snap_mode = illupow, rect_z, rect_x, do_z, do_x, snap_mode
+        f = C.c_double()
+        grad = np.zeros(nz * nx, np.float32)
+        illu = np.zeros(nz * nx, np.float32)
+        res = np.zeros_like(ob)
+        self._chk(self.lib.sfd_fwi_gradient(self.h, self._shots(shots), len(shots), _ptr(ob), _ptr(mu), _ptr(vm),
+                                            C.byref(o), C.byref(f), _ptr(grad), _ptr(illu), _ptr(res)))
+        return f.value, grad.reshape(nx, nz).T, illu.reshape(nx, nz).T, self._split(res, shots)
+
+    def last_timing(self):
+        ms = (C.c_double * 4)()
+        pts = (C.c_double * 1)()
+        self._chk(self.lib.sfd_last_timing(self.h, ms, pts))
+        return dict(total_ms=ms[0], fwd_ms=ms[1], bwd_ms=ms[2], setup_ms=ms[3], pts=pts[0])
+
+
+def device_info():
+    lib = abi.load()
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    mem = C.c_size_t()
+    rc = lib.sfd_device_info(name, 256, C.byref(cu), C.byref(mem))
+    if rc != 0:
+        raise SeisFDError(rc, "no HIP device")
+    return dict(name=name.value.decode(), cus=cu.value, hbm_bytes=mem.value)

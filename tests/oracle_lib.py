@@ -59,6 +59,11 @@ def ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def colmajor(a, dtype):
+    """(nz, nx) array -> contiguous column-major (z fastest) buffer."""
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).T)
+
+
 def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture=0.0):
     g = sfd_grid()
     g.nz, g.nx = nz, nx
@@ -114,7 +119,7 @@ def padded_vel(g, vel, shot, prec="f"):
     L = oracle()
     oxw, nxw = shot_window(g, shot)
     b = bounds(g.nz, nxw, g.ntaper)
-    v = np.ascontiguousarray(vel, dtype=_dt(prec))
+    v = colmajor(vel, _dt(prec))
     out = np.zeros(b.NZ * b.NX, dtype=_dt(prec))
     fn = getattr(L, "orc_%s_window_pad" % prec)
     fn.argtypes = [C.c_void_p, C.c_void_p, (C.c_float if prec == "f" else C.c_double), C.c_int,
@@ -126,7 +131,7 @@ def padded_vel(g, vel, shot, prec="f"):
 def modeling(g, vel, shot, extract_kind=0, prec="f", snaps=False):
     L = oracle()
     T = _dt(prec)
-    v = np.ascontiguousarray(vel, dtype=T)
+    v = colmajor(vel, T)
     w = np.ascontiguousarray(shot.wavelet, dtype=T)
     tr = np.zeros((shot.ntr, shot.nt), dtype=T)
     snap = None
@@ -147,8 +152,8 @@ def rtm(g, vel, shot, traces_in, prec="f", lsrtm=False, vel_bwd=None, capture=Fa
     """Returns (image, illum) full-model (nz,nx) as column-major -> array [nx, nz]^T."""
     L = oracle()
     T = _dt(prec)
-    v = np.ascontiguousarray(vel, dtype=T)
-    vb = None if vel_bwd is None else np.ascontiguousarray(vel_bwd, dtype=T)
+    v = colmajor(vel, T)
+    vb = None if vel_bwd is None else colmajor(vel_bwd, T)
     w = np.ascontiguousarray(shot.wavelet, dtype=T)
     d = np.ascontiguousarray(traces_in, dtype=T)
     img = np.zeros(g.nz * g.nx, dtype=T)
@@ -176,8 +181,8 @@ def rtm(g, vel, shot, traces_in, prec="f", lsrtm=False, vel_bwd=None, capture=Fa
 def born(g, vel, refl, shot, prec="f"):
     L = oracle()
     T = _dt(prec)
-    v = np.ascontiguousarray(vel, dtype=T)
-    m = np.ascontiguousarray(refl, dtype=T)
+    v = colmajor(vel, T)
+    m = colmajor(refl, T)
     w = np.ascontiguousarray(shot.wavelet, dtype=T)
     tr = np.zeros((shot.ntr, shot.nt), dtype=T)
     s = shot.c()
@@ -193,12 +198,12 @@ def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_
     """compute_fct_gdt restatement.  dobs: list of (ntr, nt) arrays per shot."""
     L = oracle()
     T = _dt(prec)
-    v = np.ascontiguousarray(vel, dtype=T)
+    v = colmajor(vel, T)
     cs = (sfd_shot * len(shots))(*[s.c() for s in shots])
     ob = np.ascontiguousarray(np.concatenate([np.asarray(d, dtype=T).ravel() for d in dobs]))
     mu = None if mute is None else np.ascontiguousarray(
         np.concatenate([np.asarray(m, dtype=T).ravel() for m in mute]))
-    vm = None if vpmask is None else np.ascontiguousarray(vpmask, dtype=T)
+    vm = None if vpmask is None else colmajor(vpmask, T)
     o = sfd_fwi_opts()
     o.illupow, o.rect_z, o.rect_x, o.do_z, o.do_x, o.snap_mode = illupow, rect_z, rect_x, do_z, do_x, 0
     f = C.c_double()

@@ -1,0 +1,214 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the CPU oracle on
+the same seeded inputs.  Bar: relative L2 <= 1e-5 in FP32 (north-star
+tolerance); the engine is built to reproduce the reference's FP32 rounding, so
+most cases are also reported (and asserted where stated) bitwise."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from models import ricker, layered_anomaly, smooth_random
+
+sa = pytest.importorskip("seismic_amd")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm((np.asarray(a, np.float64) - b).ravel()) /
+                 max(np.linalg.norm(np.asarray(b, np.float64).ravel()), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def engine(nz, nx, dz, dx, dt, ntaper, snapi=4, vel=None, vel_bwd=None):
+    e = sa.Engine(sa.make_grid(nz, nx, dz, dx, dt, ntaper=ntaper, snapi=snapi))
+    if vel is not None:
+        e.set_model(vel, vel_bwd)
+    return e
+
+
+def both_shots(sz, sx, w, rz, rx):
+    return sa.Shot(sz, sx, w, rz, rx), O.Shot(sz, sx, w, rz, rx)
+
+
+def test_config1_forward_gather():
+    """Config 1: 201x401 at 10 m, three layers + Gaussian anomaly, 15 Hz Ricker
+    at 20 m depth mid-x, 401 receivers at 20 m, dt=1 ms, nt=2000, 40-pt pad."""
+    say("case: config1 forward 201x401 nt=2000")
+    nz, nx, nt = 201, 401, 2000
+    vel = layered_anomaly(nz, nx, 10.0, 10.0, seed=1)
+    w = ricker(15.0, nt, 1e-3)
+    rx = np.arange(nx, dtype=np.float32) * 10.0
+    rz = np.full_like(rx, 20.0)
+    gs, os_ = both_shots(20.0, 2000.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 40, vel=vel)
+    gpu = e.forward([gs])[0]
+    ref = O.modeling(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=40), vel, os_, extract_kind=0)
+    err = rel(gpu, ref)
+    say("config1 gather rel L2 = %.3e bitwise=%s" % (err, np.array_equal(gpu, ref)))
+    assert err <= TOL
+    assert np.array_equal(gpu, ref)
+
+
+@pytest.mark.parametrize("ext", [0, 1, 2, 3])
+def test_forward_offgrid_all_extractions(ext):
+    say("case: forward off-grid ext=%d" % ext)
+    nz, nx, nt = 47, 83, 400
+    vel = smooth_random(nz, nx, seed=ext)
+    w = ricker(20.0, nt, 1e-3)
+    rx = np.linspace(13.3, 10.0 * (nx - 1) - 7.7, 37).astype(np.float32)
+    rz = np.full_like(rx, 31.7)
+    gs, os_ = both_shots(44.4, 333.3, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    gpu = e.forward([gs], extract_kind=ext)[0]
+    ref = O.modeling(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel, os_, extract_kind=ext)
+    err = rel(gpu, ref)
+    say("ext=%d rel L2 = %.3e bitwise=%s" % (ext, err, np.array_equal(gpu, ref)))
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("tile", [(64, 32), (32, 32), (64, 64), (128, 32), (32, 64), (64, 128)])
+def test_tile_shapes_bitwise(tile):
+    """Every compiled tile shape gives the same gather (grid sizes chosen so
+    tile-origin shifts and partial tiles occur)."""
+    say("case: tile shape %s" % (tile,))
+    nz, nx, nt = 61, 129, 250
+    vel = smooth_random(nz, nx, seed=7)
+    w = ricker(25.0, nt, 1e-3)
+    rx = np.linspace(5.0, 10.0 * (nx - 1) - 5.0, 50).astype(np.float32)
+    rz = np.full_like(rx, 15.0)
+    gs, os_ = both_shots(25.0, 640.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 15, vel=vel)
+    e.set_tile(*tile)
+    gpu = e.forward([gs])[0]
+    ref = O.modeling(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=15), vel, os_, extract_kind=0)
+    say("tile %s rel L2 = %.3e bitwise=%s" % (tile, rel(gpu, ref), np.array_equal(gpu, ref)))
+    assert rel(gpu, ref) <= TOL
+
+
+def test_batched_ragged_shots():
+    """Several shots with uneven receiver spreads and record lengths in one call."""
+    say("case: batched ragged 5 shots")
+    nz, nx = 50, 160
+    vel = smooth_random(nz, nx, seed=11)
+    rng = np.random.default_rng(4)
+    og = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=14)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 14, vel=vel)
+    gshots, oshots = [], []
+    for k in range(5):
+        nt = int(rng.integers(150, 400))
+        sx = float(rng.uniform(200, 1400))
+        ntr = int(rng.integers(10, 60))
+        rx = np.clip(sx + (np.arange(ntr) - ntr / 2) * 10.0, 0, 10.0 * (nx - 1)).astype(np.float32)
+        rz = np.full_like(rx, 20.0)
+        w = ricker(20.0 + k, nt, 1e-3)
+        a, b = both_shots(20.0, sx, w, rz, rx)
+        gshots.append(a)
+        oshots.append(b)
+    outs = e.forward(gshots)
+    for k, (o, s) in enumerate(zip(outs, oshots)):
+        ref = O.modeling(og, vel, s, extract_kind=0)
+        say("shot %d nt=%d ntr=%d rel=%.3e" % (k, s.nt, s.ntr, rel(o, ref)))
+        assert rel(o, ref) <= TOL
+
+
+def _rtm_case(seed=5, nz=48, nx=72, nt=320, snapi=4):
+    vel = smooth_random(nz, nx, seed=seed)
+    w = ricker(20.0, nt, 1e-3)
+    rx = np.linspace(10.0, 10.0 * (nx - 2), 30).astype(np.float32)
+    rz = np.full_like(rx, 12.0)
+    rng = np.random.default_rng(seed)
+    data = (rng.standard_normal((len(rx), nt)) * 1e-2).astype(np.float32)
+    return vel, w, rx, rz, data
+
+
+@pytest.mark.parametrize("lsrtm", [False, True])
+def test_rtm_image_matches_oracle(lsrtm):
+    say("case: %s small" % ("lsrtm" if lsrtm else "rtm"))
+    nz, nx, nt = 48, 72, 320
+    vel, w, rx, rz, data = _rtm_case(nz=nz, nx=nx, nt=nt)
+    gs, os_ = both_shots(18.0, 355.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    if lsrtm:
+        img, ill = e.lsrtm([gs], [data])
+    else:
+        img, ill = e.rtm([gs], [data])
+    out = O.rtm(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel, os_, data, lsrtm=lsrtm)
+    ei, el = rel(img, out["image"]), rel(ill, out["illum"])
+    say("image rel=%.3e illum rel=%.3e bitwise=%s/%s" % (ei, el, np.array_equal(img, out["image"]),
+                                                         np.array_equal(ill, out["illum"])))
+    assert ei <= TOL and el <= TOL
+
+
+def test_rtm_vel_bwd():
+    """Optional second velocity for the backward pass (3DRTM.f90 vel2)."""
+    say("case: rtm with vel2")
+    nz, nx, nt = 40, 64, 240
+    vel, w, rx, rz, data = _rtm_case(nz=nz, nx=nx, nt=nt, seed=8)
+    vel2 = (vel * 1.05).astype(np.float32)
+    gs, os_ = both_shots(18.0, 300.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel, vel_bwd=vel2)
+    img, ill = e.rtm([gs], [data])
+    out = O.rtm(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel, os_, data, vel_bwd=vel2)
+    assert rel(img, out["image"]) <= TOL and rel(ill, out["illum"]) <= TOL
+
+
+def test_rtm_fp16_snapshots_vs_fp64():
+    """FP16 max-scaled snapshots: image error against the FP64 oracle, next to
+    the FP32-snapshot error."""
+    say("case: rtm fp16 snapshots")
+    nz, nx, nt = 48, 72, 320
+    vel, w, rx, rz, data = _rtm_case(nz=nz, nx=nx, nt=nt, seed=9)
+    gs, os_ = both_shots(18.0, 355.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    i32, _ = e.rtm([gs], [data], snap_mode=sa.SFD_SNAP_FP32)
+    i16, _ = e.rtm([gs], [data], snap_mode=sa.SFD_SNAP_FP16S)
+    og = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12)
+    o64 = O.rtm(og, vel, os_, data, prec="d")
+    o32 = O.rtm(og, vel, os_, data, prec="f")
+    e32, e16, eo = rel(i32, o64["image"]), rel(i16, o64["image"]), rel(o32["image"], o64["image"])
+    say("image vs FP64: fp32-snap %.3e  fp16-snap %.3e  (fp32 oracle itself %.3e)" % (e32, e16, eo))
+    # FP32 storage reproduces the FP32 oracle; its distance to FP64 is FP32's own
+    assert rel(i32, o32["image"]) <= TOL
+    assert e32 <= 2 * eo + 1e-7
+    # FP16 max-scaled storage: within 1e-3 of the FP64 image
+    assert e16 <= 1e-3
+
+
+def test_born_matches_oracle():
+    say("case: born small")
+    nz, nx, nt = 44, 70, 300
+    vel = smooth_random(nz, nx, seed=13)
+    refl = np.zeros((nz, nx), np.float32)
+    refl[25, 10:60] = 0.1
+    refl[33, 20:40] = -0.05
+    w = ricker(20.0, nt, 1e-3)
+    rx = np.linspace(10.0, 10.0 * (nx - 2), 25).astype(np.float32)
+    rz = np.full_like(rx, 12.0)
+    gs, os_ = both_shots(18.0, 345.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    gpu = e.born([gs], np.asfortranarray(refl))[0]
+    ref = O.born(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel, np.asfortranarray(refl), os_)
+    say("born rel=%.3e bitwise=%s" % (rel(gpu, ref), np.array_equal(gpu, ref)))
+    assert rel(gpu, ref) <= TOL
+
+
+def test_error_codes():
+    say("case: error codes")
+    g = sa.make_grid(40, 40, 10.0, 10.0, 1e-3, ntaper=5)
+    with pytest.raises(sa.SeisFDError) as ei:
+        sa.Engine(g)
+    assert ei.value.code == -1
+    e = engine(40, 40, 10.0, 10.0, 1e-3, 12)
+    w = ricker(20.0, 50, 1e-3)
+    s = sa.Shot(20.0, 200.0, w, [20.0], [100.0])
+    with pytest.raises(sa.SeisFDError) as ei:
+        e.forward([s])
+    assert ei.value.code == -5   # no model yet
+    vel = np.full((40, 40), 5000.0, np.float32)   # Courant 5.0 > 0.45
+    with pytest.raises(sa.SeisFDError) as ei:
+        e.set_model(vel)
+    assert ei.value.code == -6

@@ -79,5 +79,5 @@ def _agrad_capture(g, vel, shot, data):
     cap = np.zeros((shot.nt // g.snapi, nxw, g.nz), np.float32)
     s = shot.c()
     d = np.ascontiguousarray(data, np.float32)
-    assert fn(C.byref(g), O.ptr(np.ascontiguousarray(vel)), C.byref(s), O.ptr(d), O.ptr(cap)) == 0
+    assert fn(C.byref(g), O.ptr(O.colmajor(vel, np.float32)), C.byref(s), O.ptr(d), O.ptr(cap)) == 0
     return cap
