@@ -27,6 +27,11 @@ SFD_EXTRACT_NEAREST = 1
 SFD_EXTRACT_SINC_LSRTM = 2
 SFD_EXTRACT_SINC_UNIT = 3
 
+SFD_OPT_GRAPHS = 1
+SFD_OPT_GRAPH_CHUNK = 2
+SFD_OPT_XCD_REMAP = 3
+SFD_OPT_KERNEL_EVENTS = 4
+
 
 class sfd_grid(C.Structure):
     _fields_ = [
@@ -63,7 +68,7 @@ class sfd_fwi_opts(C.Structure):
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_born", "sfd_lsrtm",
-    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile",
+    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_set_option", "sfd_kernel_stats",
     "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
 ]
 
@@ -97,6 +102,8 @@ def load():
                                      C.c_void_p, C.c_void_p]
     lib.sfd_last_timing.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.sfd_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.sfd_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.sfd_kernel_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int)]
     lib.sfd_lbfgs_create.argtypes = [P(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     lib.sfd_lbfgs_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,

@@ -80,6 +80,10 @@ struct Engine {
     int xcd_remap = 1;
     int graph_chunk = 200;
     bool use_graphs = true;
+    int event_every = 8;
+    std::vector<hipEvent_t> kev;          // sampled launch events (pairs)
+    double kstat_us[2] = {0, 0};
+    int kstat_n[2] = {0, 0};
     double timing[4] = {0, 0, 0, 0};
     double pts = 0;
     Batch fwd_batch;
@@ -109,7 +113,9 @@ struct Engine {
     sfd::StepArgs base_args(const Batch &B) const;
     void launch_step(const Batch &B, sfd::StepArgs A, bool adj);
     void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, bool fp16_scale);
+    void reset_kstats();
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);
     void gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image, float *illum);
+    void scatter_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill);
 };

@@ -17,6 +17,7 @@
 
 #include "../../include/seisfd.h"
 #include "fd_kernels.h"
+#include "fwi_kernels.h"
 #include "engine.h"
 
 using namespace sfd;
@@ -154,6 +155,7 @@ Engine::Engine(const sfd_grid &grid) : g(grid) {
 }
 
 Engine::~Engine() {
+    for (auto &e : kev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -551,19 +553,34 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 }
 
 // Run nsteps of the lockstep time loop.  Steps are recorded into HIP graphs
-// of `chunk` launches and replayed (the per-step arguments differ only in n).
+// of `graph_chunk` launches and replayed (the per-step arguments differ only
+// in n).  Every event_every-th launch is bracketed by HIP events on the engine
+// stream; their elapsed times give the step kernel's duration.
 void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, bool fp16_scale) {
-    const int chunk = graph_chunk;
+    const int chunk = std::max(1, graph_chunk);
+    const int dir = A.dir > 0 ? 0 : 1;
+    const int maxpairs = 64;
+    if (event_every > 0 && kev.empty()) {
+        kev.resize(2 * maxpairs);
+        for (auto &e : kev) HIPCHK(hipEventCreate(&e));
+    }
     for (int n0 = 0; n0 < nsteps; n0 += chunk) {
         const int n1 = std::min(nsteps, n0 + chunk);
-        bool use_graph = chunk > 1 && use_graphs;
+        const bool use_graph = use_graphs && (n1 - n0) > 1;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        int npairs = 0;
         if (use_graph) HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int n = n0; n < n1; n++) {
             A.n = n;
             A.rot = n & 3;
+            const bool sample = event_every > 0 && n % event_every == 0 && npairs < maxpairs;
+            if (sample) HIPCHK(hipEventRecord(kev[2 * npairs], stream));
             launch_step(B, A, adj);
+            if (sample) {
+                HIPCHK(hipEventRecord(kev[2 * npairs + 1], stream));
+                npairs++;
+            }
             if (fp16_scale) {
                 // forward time: after step it=n+1, publish the max for snapshot (n+2)/snapi
                 const int itn = n + 2;
@@ -576,12 +593,28 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, bool fp1
             HIPCHK(hipStreamEndCapture(stream, &graph));
             HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             HIPCHK(hipGraphLaunch(exec, stream));
+        }
+        if (npairs > 0) {
+            HIPCHK(hipStreamSynchronize(stream));
+            for (int k = 0; k < npairs; k++) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, kev[2 * k], kev[2 * k + 1]));
+                kstat_us[dir] += 1000.0 * ms;
+                kstat_n[dir]++;
+            }
+        }
+        if (use_graph) {
             HIPCHK(hipStreamSynchronize(stream));
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
         }
     }
     HIPCHK(hipGetLastError());
+}
+
+void Engine::reset_kstats() {
+    kstat_us[0] = kstat_us[1] = 0;
+    kstat_n[0] = kstat_n[1] = 0;
 }
 
 static int max_nt(const std::vector<ShotPlan> &plans) {
@@ -611,6 +644,7 @@ void Engine::check_shots(const sfd_shot *shots, int nshots) const {
 // ---- forward modeling (Acoustic_modeling_sep.f90 / AMOD) -------------------
 void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind) {
     check_shots(shots, ns);
+    reset_kstats();
     if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract kind");
     auto t0 = std::chrono::steady_clock::now();
     std::vector<ShotPlan> plans(ns);
@@ -690,6 +724,7 @@ void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image,
 void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
                  bool lsrtm) {
     check_shots(shots, ns);
+    reset_kstats();
     if (!traces_in) throw EngineError(SFD_EINVAL, "traces_in is NULL");
     if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
     auto t0 = std::chrono::steady_clock::now();
@@ -762,6 +797,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
 // ---- Born modeling (to_disk_to_memory_BORN.f90:123-220) ----------------------
 void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *traces_out) {
     check_shots(shots, ns);
+    reset_kstats();
     if (!refl || !traces_out) throw EngineError(SFD_EINVAL, "NULL refl/traces_out");
     auto t0 = std::chrono::steady_clock::now();
     std::vector<ShotPlan> plans(ns);
@@ -817,9 +853,204 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
     pts = 2 * count_pts(plans);
 }
 
-void Engine::fwi_gradient(const sfd_shot *, int, const float *, const float *, const float *, const sfd_fwi_opts *,
-                          double *, float *, float *, float *) {
-    throw EngineError(SFD_EUNSUPPORTED, "fwi_gradient not built yet");
+// ---- FWI misfit + gradient (compute_function_gradient.f90:208-350) ---------
+void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
+                          const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out) {
+    check_shots(shots, ns);
+    reset_kstats();
+    if (!dobs || !opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL dobs/opt/misfit/grad");
+    if (opt->snap_mode != SFD_SNAP_FP32 && opt->snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
+    if ((opt->do_z && opt->rect_z < 1) || (opt->do_x && opt->rect_x < 1)) throw EngineError(SFD_EINVAL, "bad smoothing length");
+    if ((opt->do_z && opt->rect_z > g.nz) || (opt->do_x && opt->rect_x > g.nx))
+        throw EngineError(SFD_EINVAL, "smoothing box longer than the model (Smoothing.f90:202 boxconv)");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<ShotPlan> plans(ns);
+    long ndata = 0;
+    int ntraces = 0;
+    for (int s = 0; s < ns; s++) {
+        ShotPlan &p = plans[s];
+        plan_shot(shots[s], p);
+        plan_injection(p, vel_h, 1, &shots[s].sz, &shots[s].sx, 1);        // Injection_sinc (AMOD)
+        plan_extraction(p, vel_h, shots[s], EXT_NEAREST);                     // Extraction_array_simple_afwi
+        ShotPlan tmp = p;
+        plan_injection(tmp, vel_h, shots[s].ntr, shots[s].rz, shots[s].rx, 2);  // Injection_simple (AGRAD)
+        p.take_backward(tmp);
+        p.n_inj_traces = std::max(1, shots[s].ntr);
+        ndata += (long)shots[s].nt * shots[s].ntr;
+        ntraces += shots[s].ntr;
+    }
+    BatchNeeds need;
+    need.backward_injection = true;
+    need.snapshots = true;
+    need.snap_fp16 = opt->snap_mode == SFD_SNAP_FP16S;
+    need.imaging = true;
+    Batch &B = fwd_batch;
+    build_batch(B, plans, need);
+    for (int s = 0; s < ns; s++) upload_inj_traces(B, s, shots[s].wavelet, shots[s].nt);
+    prepare_models(B, plans, d_vel);
+
+    // batch-level FWI buffers
+    const size_t N = (size_t)g.nz * g.nx;
+    std::vector<TraceRef> refs;
+    refs.reserve(ntraces);
+    {
+        long off = 0;
+        for (int s = 0; s < ns; s++)
+            for (int r = 0; r < shots[s].ntr; r++) {
+                refs.push_back({s, r, shots[s].nt, off});
+                off += shots[s].nt;
+            }
+    }
+    std::vector<float *> adjp(ns);
+    for (int s = 0; s < ns; s++) adjp[s] = (float *)B.shots_h[s].inj_traces;
+    const int rblocks = 1024;
+    Layout L;
+    const size_t o_dobs = L.take(ndata * 4), o_mute = L.take(mute ? ndata * 4 : 16), o_res = L.take(ndata * 4);
+    const size_t o_refs = L.take(sizeof(TraceRef) * ntraces), o_adj = L.take(sizeof(float *) * ns);
+    const size_t o_ft = L.take(sizeof(double) * ntraces);
+    const size_t o_grad = L.take(N * 4), o_illu = L.take(N * 4), o_tmp = L.take(N * 4);
+    const size_t o_mask = L.take(vpmask ? N * 4 : 16);
+    const size_t o_bmax = L.take(rblocks * 4), o_bsum = L.take(rblocks * 8);
+    const int maxn = std::max(g.nz, g.nx);
+    const int maxr = std::max(opt->rect_z, opt->rect_x) + 1;
+    const int nlines = std::max(g.nz, g.nx);
+    const size_t o_pp = L.take((size_t)(maxn + maxr) * nlines * 4), o_qq = L.take((size_t)(maxn + 2 * maxr) * nlines * 4);
+    fwi_mem.reserve(L.off);
+    char *fb = (char *)fwi_mem.p;
+    HIPCHK(hipMemcpyAsync(fb + o_dobs, dobs, ndata * 4, hipMemcpyHostToDevice, stream));
+    if (mute) HIPCHK(hipMemcpyAsync(fb + o_mute, mute, ndata * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(fb + o_refs, refs.data(), sizeof(TraceRef) * ntraces, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(fb + o_adj, adjp.data(), sizeof(float *) * ns, hipMemcpyHostToDevice, stream));
+    if (vpmask) HIPCHK(hipMemcpyAsync(fb + o_mask, vpmask, N * 4, hipMemcpyHostToDevice, stream));
+    auto t1 = std::chrono::steady_clock::now();
+
+    // AMOD: forward modeling with nearest extraction and snapshots
+    StepArgs A = base_args(B);
+    A.dir = 1;
+    A.inject = 1;
+    A.extract = 1;
+    A.store_snap = 1;
+    A.snap_fp16 = need.snap_fp16;
+    HIPCHK(hipEventRecord(ev0, stream));
+    run_loop(B, A, false, max_nt(plans), need.snap_fp16);
+    // residual, FP64 misfit per trace, adjoint source into the injection buffers
+    hipLaunchKernelGGL(residual_kernel, dim3(ntraces), dim3(256), 0, stream, (const TraceRef *)(fb + o_refs),
+                       B.shots, (const float *)(fb + o_dobs), mute ? (const float *)(fb + o_mute) : nullptr,
+                       (float *)(fb + o_res), (float *const *)(fb + o_adj), (double *)(fb + o_ft));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev1, stream));
+    // AGRAD: reverse time, forward stencil, simple injection, FWI imaging condition
+    zero_wavefields(B, plans);
+    for (int s = 0; s < ns; s++) {
+        ShotDev &S = B.shots_h[s];
+        const InjSet &I = B.injb[s];
+        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_a = I.a;
+        S.inj_wz = I.wz; S.inj_wx = I.wx; S.inj_tr = I.tr;
+    }
+    upload_shots(B);
+    StepArgs Ab = base_args(B);
+    Ab.dir = -1;
+    Ab.inject = 1;
+    Ab.snap_fp16 = need.snap_fp16;
+    Ab.imaging = IMG_FWI;
+    run_loop(B, Ab, false, max_nt(plans), false);
+    HIPCHK(hipEventRecord(ev2, stream));
+
+    // mod_copy_image into the full model, then the post-processing chain
+    float *d_grad = (float *)(fb + o_grad), *d_illu = (float *)(fb + o_illu), *d_tmp = (float *)(fb + o_tmp);
+    scatter_images(B, plans, d_grad, d_illu);
+    std::vector<double> ft(ntraces);
+    HIPCHK(hipMemcpyAsync(ft.data(), fb + o_ft, sizeof(double) * ntraces, hipMemcpyDeviceToHost, stream));
+    double scaling = 0;
+    for (int s = 0; s < ns; s++) scaling += 2.0 * (double)shots[s].nt * (double)shots[s].ntr;
+    hipLaunchKernelGGL(fwi_scale_kernel, dim3(rblocks), dim3(256), 0, stream, d_grad, d_illu, N, opt->illupow,
+                       (float)scaling, (float *)(fb + o_bmax), (double *)(fb + o_bsum));
+    std::vector<float> bmax(rblocks);
+    std::vector<double> bsum(rblocks);
+    HIPCHK(hipMemcpyAsync(bmax.data(), fb + o_bmax, rblocks * 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(bsum.data(), fb + o_bsum, rblocks * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double f = 0;
+    for (int k = 0; k < ntraces; k++) f = f + ft[k];
+    f = f / scaling;
+    float mx = bmax[0];
+    double ss = 0;
+    const int used = (int)std::min<size_t>(rblocks, (N + 255) / 256);
+    for (int k = 0; k < used; k++) {
+        mx = std::max(mx, bmax[k]);
+        ss = ss + bsum[k];
+    }
+    const double rn = sqrt(ss / (double)N);
+    if (!std::isfinite(f) || !std::isfinite(rn)) throw EngineError(SFD_ENOTFINITE, "misfit or illumination is not finite");
+    const float *d_mask = vpmask ? (const float *)(fb + o_mask) : nullptr;
+    hipLaunchKernelGGL(fwi_normalize_kernel, dim3(rblocks), dim3(256), 0, stream, d_grad, d_illu, N, mx, rn, d_mask);
+    float *pp = (float *)(fb + o_pp), *qq = (float *)(fb + o_qq);
+    if (opt->do_x) {   // triangle along x for every depth row (Smoothing.f90:109-116)
+        hipLaunchKernelGGL(triangle_kernel, dim3((g.nz + 127) / 128), dim3(128), 0, stream, d_grad, d_tmp, g.nz, g.nx,
+                           (size_t)1, (size_t)g.nz, opt->rect_x, pp, qq);
+        std::swap(d_grad, d_tmp);
+    }
+    if (opt->do_z) {   // then along z for every column (:118-125)
+        hipLaunchKernelGGL(triangle_kernel, dim3((g.nx + 127) / 128), dim3(128), 0, stream, d_grad, d_tmp, g.nx, g.nz,
+                           (size_t)g.nz, (size_t)1, opt->rect_z, pp, qq);
+        std::swap(d_grad, d_tmp);
+    }
+    if (d_mask) hipLaunchKernelGGL(mask_kernel, dim3(rblocks), dim3(256), 0, stream, d_grad, N, d_mask);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, d_grad, N * 4, hipMemcpyDeviceToHost, stream));
+    if (illum_out) HIPCHK(hipMemcpyAsync(illum_out, d_illu, N * 4, hipMemcpyDeviceToHost, stream));
+    if (resid_out) HIPCHK(hipMemcpyAsync(resid_out, fb + o_res, ndata * 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    *misfit = f;
+    float ms1 = 0, ms2 = 0;
+    HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+    HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+    timing[0] = ms1 + ms2; timing[1] = ms1; timing[2] = ms2;
+    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    pts = 2 * count_pts(plans);
+}
+
+// mod_copy_image of every shot's window image / illumination into device
+// full-model buffers (zeroed here), deterministic shot order.
+void Engine::scatter_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill) {
+    std::vector<std::vector<int>> inv(g.nx);
+    for (int s = 0; s < B.ns; s++)
+        for (int j = 0; j < plans[s].nxw; j++) {
+            const int ix = plans[s].colmap_raw[j];
+            if (ix < 1 || ix > g.nx) continue;   // ExtractPadModel.f90:686
+            inv[ix - 1].push_back((s << 20) | j);
+        }
+    std::vector<int> ptr(g.nx + 1, 0), idx;
+    for (int ix = 0; ix < g.nx; ix++) {
+        ptr[ix + 1] = ptr[ix] + (int)inv[ix].size();
+        idx.insert(idx.end(), inv[ix].begin(), inv[ix].end());
+    }
+    const size_t N = (size_t)g.nz * g.nx;
+    Layout L;
+    const size_t o_ptr = L.take(ptr.size() * 4), o_idx = L.take(std::max<size_t>(idx.size(), 1) * 4);
+    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_ci2 = L.take(sizeof(CopyImg) * B.ns);
+    scratch.reserve(L.off);
+    char *base = (char *)scratch.p;
+    HIPCHK(hipMemcpyAsync(base + o_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, stream));
+    if (!idx.empty()) HIPCHK(hipMemcpyAsync(base + o_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, stream));
+    std::vector<CopyImg> ci(B.ns), ci2(B.ns);
+    for (int s = 0; s < B.ns; s++) {
+        ci[s] = {B.shots_h[s].image, nullptr, plans[s].nxw};
+        ci2[s] = {B.shots_h[s].illum, nullptr, plans[s].nxw};
+    }
+    HIPCHK(hipMemcpyAsync(base + o_ci, ci.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(base + o_ci2, ci2.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemsetAsync(d_img, 0, N * 4, stream));
+    HIPCHK(hipMemsetAsync(d_ill, 0, N * 4, stream));
+    hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
+                       (const CopyImg *)(base + o_ci), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
+                       (const int *)(base + o_idx), d_img);
+    hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
+                       (const CopyImg *)(base + o_ci2), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
+                       (const int *)(base + o_idx), d_ill);
+    HIPCHK(hipGetLastError());
+    // the host copies above must outlive their async transfers
+    HIPCHK(hipStreamSynchronize(stream));
 }
 
 void ShotPlan::take_backward(const ShotPlan &o) {
@@ -951,6 +1182,33 @@ int sfd_set_tile(sfd_handle h, int tile_z, int tile_x) {
         e.tz = tile_z;
         e.tx = tile_x;
         e.autotune_pending = false;
+    });
+}
+
+int sfd_set_option(sfd_handle h, int opt, int value) {
+    return guarded(h, [&](Engine &e) {
+        switch (opt) {
+        case SFD_OPT_GRAPHS: e.use_graphs = value != 0; break;
+        case SFD_OPT_GRAPH_CHUNK:
+            if (value < 1) throw EngineError(SFD_EINVAL, "graph chunk must be >= 1");
+            e.graph_chunk = value;
+            break;
+        case SFD_OPT_XCD_REMAP: e.xcd_remap = value != 0; break;
+        case SFD_OPT_KERNEL_EVENTS:
+            if (value < 0) throw EngineError(SFD_EINVAL, "event sampling must be >= 0");
+            e.event_every = value;
+            break;
+        default: throw EngineError(SFD_EINVAL, "unknown option");
+        }
+    });
+}
+
+int sfd_kernel_stats(sfd_handle h, double *avg_us, int *nsamples) {
+    return guarded(h, [&](Engine &e) {
+        for (int k = 0; k < 2; k++) {
+            if (avg_us) avg_us[k] = e.kstat_n[k] ? e.kstat_us[k] / e.kstat_n[k] : 0.0;
+            if (nsamples) nsamples[k] = e.kstat_n[k];
+        }
     });
 }
 

@@ -175,6 +175,15 @@ class Engine:
                                             C.byref(o), C.byref(f), _ptr(grad), _ptr(illu), _ptr(res)))
         return f.value, grad.reshape(nx, nz).T, illu.reshape(nx, nz).T, self._split(res, shots)
 
+    def set_option(self, opt, value):
+        self._chk(self.lib.sfd_set_option(self.h, opt, int(value)))
+
+    def kernel_stats(self):
+        us = (C.c_double * 2)()
+        n = (C.c_int * 2)()
+        self._chk(self.lib.sfd_kernel_stats(self.h, us, n))
+        return dict(fwd_us=us[0], bwd_us=us[1], fwd_n=n[0], bwd_n=n[1])
+
     def last_timing(self):
         ms = (C.c_double * 4)()
         pts = (C.c_double * 1)()

@@ -212,3 +212,44 @@ def test_error_codes():
     with pytest.raises(sa.SeisFDError) as ei:
         e.set_model(vel)
     assert ei.value.code == -6
+
+
+@pytest.mark.parametrize("snap,illupow", [("fp32", 1.0), ("fp32", 0.5), ("fp16", 1.0)])
+def test_fwi_gradient_matches_oracle(snap, illupow):
+    """compute_fct_gdt: AMOD, residual/misfit, AGRAD, scatter, illumination
+    normalisation, mask and triangle smoothing, over a 3-shot batch."""
+    say("case: fwi gradient snap=%s illupow=%g" % (snap, illupow))
+    nz, nx, nt = 40, 96, 300
+    true = smooth_random(nz, nx, seed=21, vmin=1600, vmax=2600)
+    init = smooth_random(nz, nx, seed=22, vmin=1600, vmax=2500)
+    og = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12)
+    w = ricker(18.0, nt, 1e-3)
+    gsh, osh, dobs, mute = [], [], [], []
+    for k, sx in enumerate([200.0, 470.0, 760.0]):
+        rx = np.linspace(max(0.0, sx - 300), min(10.0 * (nx - 1), sx + 300), 41).astype(np.float32)
+        rz = np.full_like(rx, 15.0)
+        a, b = both_shots(12.0, sx, w, rz, rx)
+        gsh.append(a)
+        osh.append(b)
+        dobs.append(O.modeling(og, true, b, extract_kind=1))
+        m = np.ones((len(rx), nt), np.float32)
+        m[:, :20] = np.linspace(0, 1, 20, dtype=np.float32)
+        mute.append(m)
+    mask = np.ones((nz, nx), np.float32)
+    mask[:3, :] = 0.0
+    mask = np.asfortranarray(mask)
+    kw = dict(mute=mute, vpmask=mask, illupow=illupow, rect_z=3, rect_x=4, do_z=1, do_x=1)
+    ref = O.fwi_gradient(og, init, osh, dobs, **kw)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=init)
+    f, grad, illu, res = e.fwi_gradient(gsh, dobs, snap_mode=(sa.SFD_SNAP_FP16S if snap == "fp16" else
+                                                              sa.SFD_SNAP_FP32), **kw)
+    ef = abs(f - ref["f"]) / abs(ref["f"])
+    eg, ei = rel(grad, ref["grad"]), rel(illu, ref["illum"])
+    er = rel(np.concatenate([r.ravel() for r in res]), ref["resid"])
+    say("misfit rel=%.3e grad rel=%.3e illum rel=%.3e resid rel=%.3e grad bitwise=%s" %
+        (ef, eg, ei, er, np.array_equal(grad, ref["grad"])))
+    assert ef <= 1e-12 and er <= TOL
+    if snap == "fp32":
+        assert eg <= TOL and ei <= TOL
+    else:
+        assert eg <= 1e-3 and ei <= 1e-3
