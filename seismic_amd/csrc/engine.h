@@ -6,13 +6,8 @@
 #include <vector>
 
 #include "../../include/seisfd.h"
+#include "common.h"
 #include "fd_kernels.h"
-
-struct EngineError {
-    int code;
-    std::string msg;
-    EngineError(int c, std::string m) : code(c), msg(std::move(m)) {}
-};
 
 // Grow-only device buffer reused across calls (no allocation in steady state).
 struct DevBuf {

@@ -1,0 +1,374 @@
+// lbfgs.hip — device L-BFGS: restatement of libs/libinv/lbfgs.f
+// (LBFGS :49-558, MCSRCH :652-982, MCSTEP :985-1210, FREEZE_X :1-47)
+// with FP64 device vectors, and its C-ABI entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/seisfd.h"
+#include "common.h"
+#include "lbfgs_host.h"
+
+using namespace sfd;
+
+static dim3 vgrid(long n) { return dim3((unsigned)std::min<long>((n + 255) / 256, 4096)); }
+
+LbfgsDev::LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mask_h, int boundt_, int type_,
+                   const double *xinit_h)
+    : n(n_), m(m_), xmin(xmin_), xmax(xmax_), boundt(boundt_), type(type_) {
+    if (n < 1 || m < 1) throw EngineError(SFD_EINVAL, "L-BFGS needs n >= 1 and m >= 1");
+    HIPCHK(hipStreamCreate(&st));
+    const size_t b = (size_t)n * sizeof(double);
+    HIPCHK(hipMalloc(&x, b));
+    HIPCHK(hipMalloc(&g, b));
+    HIPCHK(hipMalloc(&q, b));
+    HIPCHK(hipMalloc(&diag, b));
+    HIPCHK(hipMalloc(&xinit, b));
+    HIPCHK(hipMalloc(&S, b * m));
+    HIPCHK(hipMalloc(&Y, b * m));
+    HIPCHK(hipMalloc(&part, sizeof(double) * kDotBlocks));
+    HIPCHK(hipMalloc(&res, sizeof(double)));
+    HIPCHK(hipHostMalloc(&res_h, sizeof(double)));
+    HIPCHK(hipMemsetAsync(S, 0, b * m, st));
+    HIPCHK(hipMemsetAsync(Y, 0, b * m, st));
+    if (xinit_h) HIPCHK(hipMemcpyAsync(xinit, xinit_h, b, hipMemcpyHostToDevice, st));
+    else HIPCHK(hipMemsetAsync(xinit, 0, b, st));
+    if (mask_h) {
+        HIPCHK(hipMalloc(&mask, (size_t)n * sizeof(float)));
+        HIPCHK(hipMemcpyAsync(mask, mask_h, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    rho.assign(m, 0.0);
+    alpha.assign(m, 0.0);
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+LbfgsDev::~LbfgsDev() {
+    for (void *p : {(void *)x, (void *)g, (void *)q, (void *)diag, (void *)xinit, (void *)S, (void *)Y,
+                    (void *)part, (void *)res, (void *)mask})
+        if (p) (void)hipFree(p);
+    if (res_h) (void)hipHostFree(res_h);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+double LbfgsDev::dot(const double *a, const double *b) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(256), 0, st, a, b, n, part);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, st, (const double *)part, kDotBlocks, res);
+    HIPCHK(hipMemcpyAsync(res_h, res, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return *res_h;
+}
+void LbfgsDev::axpby(double a, const double *xv, double b, double *y) {
+    hipLaunchKernelGGL(axpby_kernel, vgrid(n), dim3(256), 0, st, n, a, xv, b, y);
+}
+void LbfgsDev::scale_copy(double a, const double *xv, double *y) {
+    hipLaunchKernelGGL(copy_scaled_kernel, vgrid(n), dim3(256), 0, st, n, a, xv, y);
+}
+void LbfgsDev::freeze(double *xv) {
+    hipLaunchKernelGGL(freeze_kernel, vgrid(n), dim3(256), 0, st, n, xmin, xmax, (const float *)mask, type, xv,
+                       (const double *)xinit);
+}
+
+// MCSTEP (lbfgs.f:985-1210), scalar
+static void mcstep(double *stx, double *fx, double *dx, double *sty, double *fy, double *dy, double *stp, double fp,
+                   double dp, int *brackt, double stpmin, double stpmax, int *info) {
+    int bound;
+    double gamma, p, q, r, s, sgnd, stpc, stpf, stpq, theta;
+    *info = 0;
+    if ((*brackt && (*stp <= std::min(*stx, *sty) || *stp >= std::max(*stx, *sty))) || *dx * (*stp - *stx) >= 0.0 ||
+        stpmax < stpmin)
+        return;
+    sgnd = dp * (*dx / fabs(*dx));
+    if (fp > *fx) {
+        *info = 1;
+        bound = 1;
+        theta = 3 * (*fx - fp) / (*stp - *stx) + *dx + dp;
+        s = std::max(fabs(theta), std::max(fabs(*dx), fabs(dp)));
+        gamma = s * sqrt((theta / s) * (theta / s) - (*dx / s) * (dp / s));
+        if (*stp < *stx) gamma = -gamma;
+        p = (gamma - *dx) + theta;
+        q = ((gamma - *dx) + gamma) + dp;
+        r = p / q;
+        stpc = *stx + r * (*stp - *stx);
+        stpq = *stx + ((*dx / ((*fx - fp) / (*stp - *stx) + *dx)) / 2) * (*stp - *stx);
+        stpf = fabs(stpc - *stx) < fabs(stpq - *stx) ? stpc : stpc + (stpq - stpc) / 2;
+        *brackt = 1;
+    } else if (sgnd < 0.0) {
+        *info = 2;
+        bound = 0;
+        theta = 3 * (*fx - fp) / (*stp - *stx) + *dx + dp;
+        s = std::max(fabs(theta), std::max(fabs(*dx), fabs(dp)));
+        gamma = s * sqrt((theta / s) * (theta / s) - (*dx / s) * (dp / s));
+        if (*stp > *stx) gamma = -gamma;
+        p = (gamma - dp) + theta;
+        q = ((gamma - dp) + gamma) + *dx;
+        r = p / q;
+        stpc = *stp + r * (*stx - *stp);
+        stpq = *stp + (dp / (dp - *dx)) * (*stx - *stp);
+        stpf = fabs(stpc - *stp) > fabs(stpq - *stp) ? stpc : stpq;
+        *brackt = 1;
+    } else if (fabs(dp) < fabs(*dx)) {
+        *info = 3;
+        bound = 1;
+        theta = 3 * (*fx - fp) / (*stp - *stx) + *dx + dp;
+        s = std::max(fabs(theta), std::max(fabs(*dx), fabs(dp)));
+        gamma = s * sqrt(std::max(0.0, (theta / s) * (theta / s) - (*dx / s) * (dp / s)));
+        if (*stp > *stx) gamma = -gamma;
+        p = (gamma - dp) + theta;
+        q = (gamma + (*dx - dp)) + gamma;
+        r = p / q;
+        if (r < 0.0 && gamma != 0.0) stpc = *stp + r * (*stx - *stp);
+        else if (*stp > *stx) stpc = stpmax;
+        else stpc = stpmin;
+        stpq = *stp + (dp / (dp - *dx)) * (*stx - *stp);
+        if (*brackt) stpf = fabs(*stp - stpc) < fabs(*stp - stpq) ? stpc : stpq;
+        else stpf = fabs(*stp - stpc) > fabs(*stp - stpq) ? stpc : stpq;
+    } else {
+        *info = 4;
+        bound = 0;
+        if (*brackt) {
+            theta = 3 * (fp - *fy) / (*sty - *stp) + *dy + dp;
+            s = std::max(fabs(theta), std::max(fabs(*dy), fabs(dp)));
+            gamma = s * sqrt((theta / s) * (theta / s) - (*dy / s) * (dp / s));
+            if (*stp > *sty) gamma = -gamma;
+            p = (gamma - dp) + theta;
+            q = ((gamma - dp) + gamma) + *dy;
+            r = p / q;
+            stpc = *stp + r * (*sty - *stp);
+            stpf = stpc;
+        } else if (*stp > *stx) {
+            stpf = stpmax;
+        } else {
+            stpf = stpmin;
+        }
+    }
+    if (fp > *fx) {
+        *sty = *stp; *fy = fp; *dy = dp;
+    } else {
+        if (sgnd < 0.0) { *sty = *stx; *fy = *fx; *dy = *dx; }
+        *stx = *stp; *fx = fp; *dx = dp;
+    }
+    stpf = std::min(stpmax, stpf);
+    stpf = std::max(stpmin, stpf);
+    *stp = stpf;
+    if (*brackt && bound) {
+        if (*sty > *stx) *stp = std::min(*stx + 0.66 * (*sty - *stx), *stp);
+        else *stp = std::max(*stx + 0.66 * (*sty - *stx), *stp);
+    }
+}
+
+// MCSRCH (lbfgs.f:652-982); s = search direction (device), WA = diag.
+void LbfgsDev::mcsrch(double f, double *s) {
+    const double p5 = 0.5, p66 = 0.66, xtrapf = 4.0, zero = 0.0;
+    const double xtol = xtol_;
+    if (info == -1) goto L45;
+    infoc = 1;
+    if (n <= 0 || stp <= zero || ftol < zero || gtol < zero || xtol < zero || stpmin < zero || stpmax < stpmin ||
+        maxfev <= 0)
+        return;
+    dginit = dot(g, s);
+    if (dginit >= zero) return;   // not a descent direction (lbfgs.f:810-814)
+    brackt = 0;
+    stage1 = 1;
+    nfev = 0;
+    finit = f;
+    dgtest = ftol * dginit;
+    width = stpmax - stpmin;
+    width1 = width / p5;
+    scale_copy(1.0, x, diag);
+    stx = zero; fx = finit; dgx = dginit;
+    sty = zero; fy = finit; dgy = dginit;
+L30:
+    if (brackt) {
+        stmin = std::min(stx, sty);
+        stmax = std::max(stx, sty);
+    } else {
+        stmin = stx;
+        stmax = stp + xtrapf * (stp - stx);
+    }
+    stp = std::max(stp, stpmin);
+    stp = std::min(stp, stpmax);
+    if ((brackt && (stp <= stmin || stp >= stmax)) || nfev >= maxfev - 1 || infoc == 0 ||
+        (brackt && stmax - stmin <= xtol * stmax))
+        stp = stx;
+    // X = WA + STP*S
+    scale_copy(1.0, diag, x);
+    axpby(stp, s, 1.0, x);
+    if (boundt == 1) freeze(x);
+    info = -1;
+    return;
+L45:
+    info = 0;
+    nfev = nfev + 1;
+    {
+        const double dg = dot(g, s);
+        const double ftest1 = finit + stp * dgtest;
+        if ((brackt && (stp <= stmin || stp >= stmax)) || infoc == 0) info = 6;
+        if (stp == stpmax && f <= ftest1 && dg <= dgtest) info = 5;
+        if (stp == stpmin && (f > ftest1 || dg >= dgtest)) info = 4;
+        if (nfev >= maxfev) info = 3;
+        if (brackt && stmax - stmin <= xtol * stmax) info = 2;
+        if (f <= ftest1 && fabs(dg) <= gtol * (-dginit)) info = 1;
+        if (info != 0) return;
+        if (stage1 && f <= ftest1 && dg >= std::min(ftol, gtol) * dginit) stage1 = 0;
+        if (stage1 && f <= fx && f > ftest1) {
+            double fm = f - stp * dgtest;
+            double fxm = fx - stx * dgtest;
+            double fym = fy - sty * dgtest;
+            double dgm = dg - dgtest;
+            double dgxm = dgx - dgtest;
+            double dgym = dgy - dgtest;
+            mcstep(&stx, &fxm, &dgxm, &sty, &fym, &dgym, &stp, fm, dgm, &brackt, stmin, stmax, &infoc);
+            fx = fxm + stx * dgtest;
+            fy = fym + sty * dgtest;
+            dgx = dgxm + dgtest;
+            dgy = dgym + dgtest;
+        } else {
+            mcstep(&stx, &fx, &dgx, &sty, &fy, &dgy, &stp, f, dg, &brackt, stmin, stmax, &infoc);
+        }
+        if (brackt) {
+            if (fabs(sty - stx) >= p66 * width1) stp = stx + p5 * (sty - stx);
+            width1 = width;
+            width = fabs(sty - stx);
+        }
+    }
+    goto L30;
+}
+
+// LBFGS (lbfgs.f:49-558), DIAGCO = .false.
+void LbfgsDev::step(double f, int *iflag, int *myinfo, double eps, double xtol) {
+    xtol_ = xtol;
+    if (*iflag == 0) goto L10;
+    if (*iflag == 1) goto L172;
+L10:
+    iter = 0;
+    if (n <= 0 || m <= 0) {
+        *iflag = -3;
+        return;
+    }
+    if (gtol <= 1.0e-04) gtol = 9.0e-01;
+    nfun = 1;
+    point = 0;
+    finish = 0;
+    hipLaunchKernelGGL(fill_kernel, vgrid(n), dim3(256), 0, st, n, 1.0, diag);
+    scale_copy(-1.0, g, Sv(0));                  // W(ISPT+I) = -G(I)*DIAG(I), DIAG = 1
+    gnorm = sqrt(dot(g, g));
+    xnorm2 = sqrt(dot(x, x));
+    stp1 = boundt == 0 ? 1.0 / gnorm : xnorm2 / gnorm / 100;
+    ftol = 1.0e-4;
+    maxfev = 20;
+L80:
+    iter = iter + 1;
+    info = 0;
+    bound = iter - 1;
+    if (iter == 1) goto L165;
+    if (iter > m) bound = m;
+    ys = dot(Yv(npt), Sv(npt));
+    yy = dot(Yv(npt), Yv(npt));
+    hipLaunchKernelGGL(fill_kernel, vgrid(n), dim3(256), 0, st, n, ys / yy, diag);
+    cp = point == 0 ? m : point;
+    rho[cp - 1] = 1.0 / ys;
+    scale_copy(-1.0, g, q);
+    cp = point;
+    for (int i = 1; i <= bound; i++) {
+        cp = cp - 1;
+        if (cp == -1) cp = m - 1;
+        const double sq = dot(Sv(cp), q);
+        alpha[cp] = rho[cp] * sq;
+        axpby(-alpha[cp], Yv(cp), 1.0, q);
+    }
+    if (h0 && h0_active) h0(h0_ctx, q, n, st);     // learned initial inverse Hessian
+    else hipLaunchKernelGGL(mul_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)diag, q);
+    for (int i = 1; i <= bound; i++) {
+        const double yr = dot(Yv(cp), q);
+        double beta = rho[cp] * yr;
+        beta = alpha[cp] - beta;
+        axpby(beta, Sv(cp), 1.0, q);
+        cp = cp + 1;
+        if (cp == m) cp = 0;
+    }
+    scale_copy(1.0, q, Sv(point));
+L165:
+    nfev = 0;
+    stp = 1.0;
+    if (iter == 1) stp = stp1;
+    scale_copy(1.0, g, q);                       // W(I) = G(I): gradient kept for Y
+L172:
+    mcsrch(f, Sv(point));
+    if (info == -1) {
+        *iflag = 1;
+        return;
+    }
+    if (info != 1) {
+        *iflag = -1;
+        return;
+    }
+    nfun = nfun + nfev;
+    *myinfo = info;
+    if (boundt == 2 && info == 1) freeze(x);
+    npt = point;                                  // (the reference's NPT = POINT*N)
+    scale_copy(stp, Sv(npt), Sv(npt));
+    hipLaunchKernelGGL(sub_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)g, (const double *)q, Yv(npt));
+    point = point + 1;
+    if (point == m) point = 0;
+    gnorm = sqrt(dot(g, g));
+    {
+        double xnorm = sqrt(dot(x, x));
+        xnorm = std::max(1.0, xnorm);
+        if (gnorm / xnorm <= eps) finish = 1;
+    }
+    if (finish) {
+        *iflag = 0;
+        return;
+    }
+    goto L80;
+}
+
+// ---------------------------------------------------------------------------
+struct sfd_lbfgs {
+    LbfgsDev *d = nullptr;
+    std::string err;
+};
+
+extern "C" {
+
+int sfd_lbfgs_create(sfd_lbfgs_handle *lb, int n, int m, double xmin, double xmax, const float *mask, int boundt,
+                     int freeze_soft, const double *xinit) {
+    if (!lb) return SFD_EINVAL;
+    *lb = nullptr;
+    try {
+        sfd_lbfgs *h = new sfd_lbfgs();
+        h->d = new LbfgsDev(n, m, xmin, xmax, mask, boundt, freeze_soft, xinit);
+        *lb = h;
+        return SFD_OK;
+    } catch (const EngineError &e) {
+        return e.code;
+    }
+}
+
+int sfd_lbfgs_step(sfd_lbfgs_handle lb, double *x, double f, const double *g, int *iflag, int *info) {
+    if (!lb || !lb->d || !x || !g || !iflag || !info) return SFD_EINVAL;
+    LbfgsDev &d = *lb->d;
+    try {
+        const size_t b = (size_t)d.n * sizeof(double);
+        HIPCHK(hipMemcpyAsync(d.x, x, b, hipMemcpyHostToDevice, d.st));
+        HIPCHK(hipMemcpyAsync(d.g, g, b, hipMemcpyHostToDevice, d.st));
+        d.step(f, iflag, info, 1.0e-20, 2.220446049250313e-16);   // EPS, XTOL of Inversion.f90:79-80
+        HIPCHK(hipMemcpyAsync(x, d.x, b, hipMemcpyDeviceToHost, d.st));
+        HIPCHK(hipStreamSynchronize(d.st));
+        return SFD_OK;
+    } catch (const EngineError &e) {
+        lb->err = e.msg;
+        return e.code;
+    }
+}
+
+int sfd_lbfgs_destroy(sfd_lbfgs_handle lb) {
+    if (!lb) return SFD_EINVAL;
+    delete lb->d;
+    delete lb;
+    return SFD_OK;
+}
+
+}  // extern "C"

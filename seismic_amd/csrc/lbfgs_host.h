@@ -1,0 +1,50 @@
+// lbfgs_host.h — host control flow of the device L-BFGS (restates
+// libs/libinv/lbfgs.f; see lbfgs_dev.h for the vector kernels).
+//
+// W(1:N) of the reference is the device vector `q`; the S and Y histories are
+// device arrays of m vectors; the scalars W(N+1:N+M) (rho) and
+// W(N+M+1:N+2M) (alpha) live on the host.  DIAG doubles as MCSRCH's WA
+// exactly as in the reference call (lbfgs.f:486-488).
+#pragma once
+#include <vector>
+
+#include "common.h"
+#include "lbfgs_dev.h"
+
+struct LbfgsDev {
+    long n = 0;
+    int m = 5;
+    double xmin = 0, xmax = 0;
+    int boundt = 0, type = 1;
+    hipStream_t st = nullptr;
+    double *x = nullptr, *g = nullptr, *q = nullptr, *S = nullptr, *Y = nullptr, *diag = nullptr, *xinit = nullptr;
+    double *part = nullptr, *res = nullptr, *res_h = nullptr;
+    float *mask = nullptr;
+    std::vector<double> rho, alpha;
+    // LBFGS SAVE state
+    int iter = 0, nfun = 0, point = 0, maxfev = 20, info = 0, bound = 0, npt = 0, cp = 0, nfev = 0, finish = 0;
+    double gnorm = 0, xnorm2 = 0, stp1 = 0, ftol = 1e-4, stp = 0, ys = 0, yy = 0;
+    double gtol = 0.9, stpmin = 1e-20, stpmax = 1e20;   // block data LB2 (lbfgs.f:636-641)
+    // MCSRCH SAVE state
+    int infoc = 0, brackt = 0, stage1 = 0;
+    double xtol_ = 2.220446049250313e-16;
+    double dginit = 0, dgtest = 0, finit = 0, width = 0, width1 = 0, stx = 0, fx = 0, dgx = 0, sty = 0, fy = 0,
+           dgy = 0, stmin = 0, stmax = 0;
+    // optional preconditioner for the two-loop's initial matrix (H0 q)
+    void (*h0)(void *ctx, double *q, long n, hipStream_t st) = nullptr;
+    void *h0_ctx = nullptr;
+    bool h0_active = false;
+
+    LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mask_h, int boundt_, int type_,
+             const double *xinit_h);
+    ~LbfgsDev();
+    double dot(const double *a, const double *b);
+    void axpby(double a, const double *xv, double b, double *y);
+    void scale_copy(double a, const double *xv, double *y);
+    void freeze(double *xv);
+    double *Sv(int k) const { return S + (size_t)k * n; }
+    double *Yv(int k) const { return Y + (size_t)k * n; }
+    // One call of LBFGS(N,M,X,F,G,...) on the device vectors x, g.
+    void step(double f, int *iflag, int *myinfo, double eps, double xtol);
+    void mcsrch(double f, double *s);
+};

@@ -19,6 +19,7 @@
 #include "fd_kernels.h"
 #include "fwi_kernels.h"
 #include "engine.h"
+#include "common.h"
 
 using namespace sfd;
 
@@ -87,17 +88,6 @@ static void shot_window(const sfd_grid &g, const sfd_shot &s, float *oxw, int *n
 
 // ---------------------------------------------------------------------------
 
-#define HIPCHK(x)                                                                      \
-    do {                                                                               \
-        hipError_t e_ = (x);                                                           \
-        if (e_ != hipSuccess) {                                                        \
-            char b_[256];                                                              \
-            snprintf(b_, sizeof b_, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
-                     __FILE__, __LINE__);                                              \
-            throw EngineError(SFD_EHIP, b_);                                           \
-        }                                                                              \
-    } while (0)
-
 void DevBuf::reserve(size_t n) {
     if (n <= cap) return;
     if (p) (void)hipFree(p);
@@ -112,16 +102,6 @@ void DevBuf::reserve(size_t n) {
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
-
-// Bump allocator over one device buffer: plan offsets, then allocate once.
-struct Layout {
-    size_t off = 0;
-    size_t take(size_t bytes) {
-        size_t o = (off + 255) & ~(size_t)255;
-        off = o + bytes;
-        return o;
-    }
-};
 
 static void check_grid(const sfd_grid &g) {
     if (g.nz < 2 || g.nx < 1 || !(g.dz > 0) || !(g.dx > 0) || !(g.dt > 0))
@@ -1212,10 +1192,5 @@ int sfd_kernel_stats(sfd_handle h, double *avg_us, int *nsamples) {
     });
 }
 
-int sfd_lbfgs_create(sfd_lbfgs_handle *, int, int, double, double, const float *, int, int, const double *) {
-    return SFD_EUNSUPPORTED;
-}
-int sfd_lbfgs_step(sfd_lbfgs_handle, double *, double, const double *, int *, int *) { return SFD_EUNSUPPORTED; }
-int sfd_lbfgs_destroy(sfd_lbfgs_handle) { return SFD_EUNSUPPORTED; }
 
 }  // extern "C"

@@ -200,3 +200,39 @@ def device_info():
     if rc != 0:
         raise SeisFDError(rc, "no HIP device")
     return dict(name=name.value.decode(), cus=cu.value, hbm_bytes=mem.value)
+
+
+class LBFGS:
+    """Device L-BFGS with the reference's reverse-communication protocol
+    (LBFGS(N,M,X,F,G,...) of libs/libinv/lbfgs.f): call step(x, f, g) with
+    f, g evaluated at x; x is updated in place; returns (iflag, info)."""
+
+    def __init__(self, n, m=5, xmin=-1e30, xmax=1e30, mask=None, boundt=0, freeze_soft=True, xinit=None):
+        self.lib = abi.load()
+        self.n = int(n)
+        self._mask = None if mask is None else np.ascontiguousarray(mask, np.float32)
+        self._xinit = None if xinit is None else np.ascontiguousarray(xinit, np.float64)
+        h = C.c_void_p()
+        rc = self.lib.sfd_lbfgs_create(C.byref(h), self.n, int(m), float(xmin), float(xmax), _ptr(self._mask),
+                                       int(boundt), int(bool(freeze_soft)), _ptr(self._xinit))
+        if rc != 0:
+            raise SeisFDError(rc, "sfd_lbfgs_create failed")
+        self.h = h
+        self.iflag = C.c_int(0)
+        self.info = C.c_int(0)
+
+    def step(self, x, f, g):
+        assert x.dtype == np.float64 and x.flags.c_contiguous and x.size == self.n
+        gg = np.ascontiguousarray(g, np.float64)
+        rc = self.lib.sfd_lbfgs_step(self.h, _ptr(x), float(f), _ptr(gg), C.byref(self.iflag), C.byref(self.info))
+        if rc != 0:
+            raise SeisFDError(rc, "sfd_lbfgs_step failed")
+        return self.iflag.value, self.info.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.sfd_lbfgs_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -1,0 +1,59 @@
+"""Device L-BFGS (sfd_lbfgs_*) against the oracle's restatement of
+libs/libinv/lbfgs.f: same iterates on smooth problems (FP64; the device
+reductions sum in a different order, so equality is to ~1e-10)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+sa = pytest.importorskip("seismic_amd")
+pytestmark = pytest.mark.gpu
+
+
+def rosen(x):
+    f = np.sum(100.0 * (x[1::2] - x[::2] ** 2) ** 2 + (1 - x[::2]) ** 2)
+    g = np.zeros_like(x)
+    g[::2] = -400.0 * x[::2] * (x[1::2] - x[::2] ** 2) - 2 * (1 - x[::2])
+    g[1::2] = 200.0 * (x[1::2] - x[::2] ** 2)
+    return f, g
+
+
+def quad(A, b):
+    return lambda x: (0.5 * x @ A @ x - b @ x, A @ x - b)
+
+
+def run(opt, fg, x0, nmax=400):
+    x = x0.copy()
+    hist = []
+    for _ in range(nmax):
+        f, g = fg(x)
+        iflag, info = opt.step(x, f, g)
+        if info == 1:
+            hist.append((f, x.copy()))
+        if iflag <= 0:
+            break
+    return x, hist, iflag
+
+
+@pytest.mark.parametrize("problem", ["rosenbrock", "quadratic"])
+@pytest.mark.parametrize("boundt", [0, 2])
+def test_device_lbfgs_matches_oracle(problem, boundt):
+    print("case: lbfgs %s boundt=%d" % (problem, boundt), flush=True)
+    n = 64
+    rng = np.random.default_rng(1)
+    if problem == "rosenbrock":
+        fg = rosen
+        x0 = np.tile([-1.2, 1.0], n // 2).astype(np.float64)
+    else:
+        Q = rng.standard_normal((n, n))
+        fg = quad(Q @ Q.T + n * np.eye(n), rng.standard_normal(n))
+        x0 = np.zeros(n)
+    kw = dict(m=5, xmin=-10.0, xmax=10.0, boundt=boundt, freeze_soft=True)
+    xo, ho, fo = run(O.LBFGS(n, **{**kw, "freeze_soft": 1}), fg, x0)
+    xd, hd, fd = run(sa.LBFGS(n, **kw), fg, x0)
+    k = min(len(ho), len(hd), 25)
+    assert k >= 5
+    for (f1, x1), (f2, x2) in zip(ho[:k], hd[:k]):
+        assert abs(f1 - f2) <= 1e-8 * max(1.0, abs(f1))
+        assert np.linalg.norm(x1 - x2) <= 1e-7 * max(1.0, np.linalg.norm(x1))
+    print("accepted iterations compared: %d, final f oracle %.3e device %.3e" % (k, ho[-1][0], hd[-1][0]))
