@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--snap", choices=["fp32", "fp16"], default="fp32")
     ap.add_argument("--tile", default="64x32")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graphs", type=int, default=1, help="0: eager launches (for profilers)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -102,6 +103,7 @@ def main():
         sys.exit(2)
 
     import seismic_amd as sa
+    import seismic_amd.abi  # noqa: F401
     info = sa.device_info()
     log("device", info["name"], "CUs", info["cus"])
     true, mig, w, rz, rx = build_inputs()
@@ -109,6 +111,7 @@ def main():
     e = sa.Engine(g)
     tz, tx = (int(v) for v in args.tile.split("x"))
     e.set_tile(tz, tx)
+    e.set_option(sa.abi.SFD_OPT_GRAPHS, args.graphs)
     shot = sa.Shot(0.0, DZ * (NX // 2), w, rz, rx)
     log("modeling observed data (untimed)")
     e.set_model(true)

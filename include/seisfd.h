@@ -170,14 +170,14 @@ int  sfd_set_tile(sfd_handle h, int tile_z, int tile_x);
 
 /* Engine options (no reference counterpart: launch policy of the HIP path). */
 #define SFD_OPT_GRAPHS       1  /* 1: replay the time loop as HIP graphs (default) */
-#define SFD_OPT_GRAPH_CHUNK  2  /* time steps per graph (default 200)            */
+#define SFD_OPT_GRAPH_CHUNK  2  /* time steps per graph (default 199)            */
 #define SFD_OPT_XCD_REMAP    3  /* 1: XCD-contiguous tile order (default)          */
-#define SFD_OPT_KERNEL_EVENTS 4 /* sample every n-th step launch with HIP events (0 off, default 8) */
+#define SFD_OPT_KERNEL_EVENTS 4 /* 1: time the first step of every graph chunk with HIP events (default) */
 int  sfd_set_option(sfd_handle h, int opt, int value);
 
 /*
  * Step-kernel durations of the last call, from HIP events recorded on the
- * engine's stream around sampled launches: avg_us[0] forward-time pass,
+ * engine's stream around one launch per graph chunk: avg_us[0] forward-time pass,
  * avg_us[1] reverse-time pass; nsamples likewise.
  */
 int  sfd_kernel_stats(sfd_handle h, double *avg_us, int *nsamples);

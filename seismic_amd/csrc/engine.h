@@ -73,10 +73,10 @@ struct Engine {
     int tz = 64, tx = 32;
     bool autotune_pending = false;
     int xcd_remap = 1;
-    int graph_chunk = 200;
+    int graph_chunk = 199;   // not a multiple of 4: sampled steps rotate through snapshot phases
     bool use_graphs = true;
-    int event_every = 8;
-    std::vector<hipEvent_t> kev;          // sampled launch events (pairs)
+    int event_every = 1;
+    std::vector<hipEvent_t> kev;          // start/stop of the sampled launch
     double kstat_us[2] = {0, 0};
     int kstat_n[2] = {0, 0};
     double timing[4] = {0, 0, 0, 0};

@@ -113,25 +113,167 @@ __device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, 
     return ((const float *)S.snap)[(size_t)(idx - 1) * npix + wq];
 }
 
+// Rare per-tile work, kept out of the streaming body: receivers read from the
+// u2 tile (ExtractData, Extraction.f90:539-545 / :84-86).
+template <int LZ>
+__device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, const float *s_a) {
+    for (int k = S.ext_tile[t] + (int)threadIdx.x; k < S.ext_tile[t + 1]; k += kThreads) {
+        const int r = S.ext_rec[k];
+        const int pos = S.ext_pos[k];
+        const int lz = (pos & 0xffff) + 4, lx = (pos >> 16) + 4;
+        float *tr = S.traces + (size_t)r * S.nt + (it - 1);
+        if (S.ext_kind == EXT_NEAREST) {
+            *tr = *tr + s_a[lz + LZ * lx];
+        } else {
+            // trace += g*u*wz*wx, j outer, i inner
+            const float *w = S.ext_w + 16 * k;
+            const float g = w[0];
+            float acc = *tr;
+            for (int j = -3; j <= 3; j++)
+                for (int i = -3; i <= 3; i++) acc = acc + g * s_a[(lz + i) + LZ * (lx + j)] * w[4 + i] * w[11 + j];
+            *tr = acc;
+        }
+    }
+}
+
+// Snapshot store and imaging condition at one interior point (on u2, before
+// the stencil: Extraction.f90:595, Imaging.f90:265 / :365 / :738).
+__device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
+                                              float u2c, float vel) {
+    const int wz = iz - S.i0, wx = ix - S.i0;
+    const size_t wq = (size_t)wx * S.nz + wz;
+    const size_t npix = (size_t)S.nz * S.nxw;
+    const int idx = it / A.snapi;
+    if (A.store_snap) {
+        if (A.snap_fp16) {
+            const float sc = S.snap_scale[idx - 1];
+            const float inv = sc > 0.f ? 1.0f / sc : 0.f;
+            ((__half *)S.snap)[(size_t)(idx - 1) * npix + wq] = __float2half_rn(u2c * inv);
+        } else {
+            ((float *)S.snap)[(size_t)(idx - 1) * npix + wq] = u2c;
+        }
+    }
+    if (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) {
+        const float Sv = load_snap(S, A.snap_fp16, idx, npix, wq);
+        const float tp = S.taper[wx];
+        float img = S.image[wq];
+        if (A.imaging == IMG_RTM) {
+            img = img + u2c * Sv * tp;
+        } else if (A.imaging == IMG_FWI) {
+            const int im1 = idx - 1 > 1 ? idx - 1 : 1;
+            const int ip1 = idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap;
+            const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq);
+            const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq);
+            const float tmp = (Sm - 2.f * Sv + Sp) / (S.dt2snap * (vel * vel * vel));
+            img = img + u2c * tmp * tp;
+        } else {
+            img = img + 2.f * tp * u2c * Sv / vel;
+        }
+        S.image[wq] = img;
+        S.illum[wq] = S.illum[wq] + Sv * Sv * tp;
+    }
+}
+
+// Higdon absorbing boundary (Boundary.f90:119-153) on the u3 tile in s_b, for
+// tiles that touch the grid edge.  z-bands first (one thread per column with
+// ix in [nmin2, nmax2]), then x-bands (one thread per row with iz in
+// [nmin1, nmax1]), each a sequential recursion outwards.
+template <int TZ, int TX, bool ADJ>
+__device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bool top, bool bot, bool left,
+                                         bool right, const float *s_a, float *s_b, const float *__restrict__ u2,
+                                         const float *__restrict__ u1) {
+    constexpr int LZ = TZ + 8;
+    const int NZ = S.NZ, NX = S.NX, tid = threadIdx.x;
+    // u2 at (i, j): from the LDS tile (forward) or global memory (adjoint:
+    // the tile holds v^2 u2 there)
+    auto U2 = [&](int i, int j) -> float {
+        if (ADJ) return u2[addr(S, i, j)];
+        return s_a[(i - z0 + 4) + LZ * (j - x0 + 4)];
+    };
+    if (top || bot) {
+        for (int l = tid; l < 2 * TX; l += kThreads) {
+            const int cx = l % TX;
+            const bool dotop = l < TX;
+            if (dotop ? !top : !bot) continue;
+            const int ix = x0 + cx;
+            if (ix < 4 || ix > NX - 5) continue;
+            const float *g = S.hz + 16 * (ix - 4) + (dotop ? 0 : 8);
+            float *b = s_b + TZ * cx;
+            if (dotop) {
+                for (int i = 7; i >= 0; i--) {
+                    const int lz = i - z0;
+                    const float f = g[0] * b[lz + 1] + g[1] * b[lz + 2] + g[2] * U2(i, ix) + g[3] * U2(i + 1, ix) +
+                                    g[4] * U2(i + 2, ix) + g[5] * u1[addr(S, i, ix)] + g[6] * u1[addr(S, i + 1, ix)] +
+                                    g[7] * u1[addr(S, i + 2, ix)];
+                    b[lz] = -f;
+                }
+            } else {
+                for (int i = NZ - 8; i <= NZ - 1; i++) {
+                    const int lz = i - z0;
+                    const float f = g[0] * b[lz - 1] + g[1] * b[lz - 2] + g[2] * U2(i, ix) + g[3] * U2(i - 1, ix) +
+                                    g[4] * U2(i - 2, ix) + g[5] * u1[addr(S, i, ix)] + g[6] * u1[addr(S, i - 1, ix)] +
+                                    g[7] * u1[addr(S, i - 2, ix)];
+                    b[lz] = -f;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (left || right) {
+        for (int l = tid; l < 2 * TZ; l += kThreads) {
+            const int cz = l % TZ;
+            const bool doleft = l < TZ;
+            if (doleft ? !left : !right) continue;
+            const int i = z0 + cz;
+            if (i < 4 || i > NZ - 5) continue;
+            const float *g = S.hx + 16 * (i - 4) + (doleft ? 0 : 8);
+            if (doleft) {
+                for (int j = 7; j >= 0; j--) {
+                    const int lx = j - x0;
+                    const float f = g[0] * s_b[cz + TZ * (lx + 1)] + g[1] * s_b[cz + TZ * (lx + 2)] + g[2] * U2(i, j) +
+                                    g[3] * U2(i, j + 1) + g[4] * U2(i, j + 2) + g[5] * u1[addr(S, i, j)] +
+                                    g[6] * u1[addr(S, i, j + 1)] + g[7] * u1[addr(S, i, j + 2)];
+                    s_b[cz + TZ * lx] = -f;
+                }
+            } else {
+                for (int j = NX - 8; j <= NX - 1; j++) {
+                    const int lx = j - x0;
+                    const float f = g[0] * s_b[cz + TZ * (lx - 1)] + g[1] * s_b[cz + TZ * (lx - 2)] + g[2] * U2(i, j) +
+                                    g[3] * U2(i, j - 1) + g[4] * U2(i, j - 2) + g[5] * u1[addr(S, i, j)] +
+                                    g[6] * u1[addr(S, i, j - 1)] + g[7] * u1[addr(S, i, j - 2)];
+                    s_b[cz + TZ * lx] = -f;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The step kernel.  Thread (lane, xg) owns tile points (lane, xg + p*XG),
+// p < PX.  Its own u1, u0, u_-1 and v^2 are loaded into registers first, so
+// those loads are in flight while the u2 tile (+4-cell halo) is staged in
+// LDS; LDS column c is filled by x-group (c-4) mod XG, which makes every
+// thread load the u2 centre values it later needs.
 template <int TZ, int TX, bool ADJ>
 __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
     constexpr int XG = kThreads / TZ;   // x-groups of threads
     constexpr int PX = TX / XG;         // columns per thread
     constexpr int LZ = TZ + 8;          // LDS tile rows incl. halo
     constexpr int LX = TX + 8;
-    static_assert(kThreads % TZ == 0 && TX % XG == 0, "tile shape");
-    __shared__ float s_a[LX * LZ];      // u2 (forward) or v2*u2 (adjoint), with halo
-    __shared__ float s_b[TX * TZ];      // u3 tile for the Higdon recursions
-    __shared__ int s_inj[TX * TZ];      // per-cell contribution range (start, count)
+    static_assert(kThreads % TZ == 0 && TX % XG == 0 && TZ >= 8, "tile shape");
+    __shared__ float s_a[LX * LZ];      // u2 (forward) or v^2 u2 (adjoint), with halo
+    __shared__ union {
+        int inj[TX * TZ];               // per-cell contribution range (start<<8 | count)
+        float b[TX * TZ];               // u3 tile for the Higdon recursions
+    } s_u;
 
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
-    const int si = A.blk_shot[lb];
-    const ShotDev &S = A.shots[si];
+    const ShotDev &S = A.shots[A.blk_shot[lb]];
     const int it = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     if (it < 1 || it > S.nt) return;   // this shot has finished (ragged batch)
 
-    const int NZ = S.NZ, NX = S.NX;
+    const int NZ = S.NZ, NX = S.NX, LDZ = S.LDZ, zoff = S.zoff;
     const int t = lb - S.tile0;
     const int tzi = t % S.ntz, txi = t / S.ntz;
     const int z0 = tzi * TZ - S.zsh, x0 = txi * TX - S.xsh;
@@ -139,268 +281,142 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
     const float *__restrict__ u1 = S.u[(A.rot + 2) & 3];
     const float *__restrict__ u0 = S.u[(A.rot + 1) & 3];
     float *__restrict__ u3 = S.u[A.rot];      // u_{-1} in, u3 out
+    const float *__restrict__ v2p = S.v2;
     const int tid = threadIdx.x;
+    const int lane = tid % TZ, xg = tid / TZ;
+    const int iz = z0 + lane;
+    const bool zin = iz >= 4 && iz <= NZ - 5;
+    const size_t zo = (size_t)(zoff + iz);
 
-    // ---- stage u2 (or v2(clamped)*u2) tile + 4-cell halo in LDS
-    for (int l = tid; l < LX * LZ; l += kThreads) {
-        const int lz = l % LZ, lx = l / LZ;
-        const int iz = z0 - 4 + lz, ix = x0 - 4 + lx;
-        float val = 0.f;
-        if (iz >= 0 && iz < NZ && ix >= 0 && ix < NX) {
-            val = u2[addr(S, iz, ix)];
-            if (ADJ) {
-                // FDderivatives.f90:859-876: the velocity index of a neighbour is
-                // clamped to the FD region [4, N-5] of each axis.
-                const int cz = iz < 4 ? 4 : (iz > NZ - 5 ? NZ - 5 : iz);
-                const int cx = ix < 4 ? 4 : (ix > NX - 5 ? NX - 5 : ix);
-                val = S.v2[addr(S, cz, cx)] * val;
-            }
+    // ---- register prefetch of this thread's u1, u0, u_-1, v^2
+    float r1[PX], r0[PX], rm[PX], rv[PX], r2[PX];
+#pragma unroll
+    for (int p = 0; p < PX; p++) {
+        const int ix = x0 + xg + p * XG;
+        r1[p] = r0[p] = rm[p] = rv[p] = r2[p] = 0.f;
+        if (zin && ix >= 4 && ix <= NX - 5) {
+            const size_t q = (size_t)ix * LDZ + zo;
+            r1[p] = u1[q];
+            r0[p] = u0[q];
+            rm[p] = u3[q];
+            if (!ADJ) rv[p] = v2p[q];
         }
-        s_a[l] = val;
     }
+
+    // ---- stage u2 (or v^2 u2) tile + halo in LDS; keep own centre values.
+    // One LDS column: every lane its own row, lanes 0..7 the 4+4 halo rows.
+    auto fill_col = [&](int cl, float *own) {
+        const int ix = x0 - 4 + cl;
+        const bool xok = ix >= 0 && ix < NX;
+        const size_t cb = (size_t)ix * LDZ + zoff;
+        float val = 0.f;
+        if (xok && iz >= 0 && iz < NZ) {
+            val = u2[cb + iz];
+            if (own) *own = val;
+            if (ADJ) val = v2p[cb + iz] * val;
+        }
+        s_a[(lane + 4) + LZ * cl] = val;
+        if (lane < 8) {
+            const int lz = lane < 4 ? lane : TZ + lane;
+            const int jz = z0 - 4 + lz;
+            float hv = 0.f;
+            if (xok && jz >= 0 && jz < NZ) {
+                hv = u2[cb + jz];
+                if (ADJ) hv = v2p[cb + jz] * hv;
+            }
+            s_a[lz + LZ * cl] = hv;
+        }
+    };
+#pragma unroll
+    for (int p = 0; p < PX; p++) fill_col(xg + p * XG + 4, &r2[p]);
+    for (int h = xg; h < 8; h += XG) fill_col(h < 4 ? h : TX + h, nullptr);
 
     const int has_inj = A.inject && (S.inj_tile[t + 1] > S.inj_tile[t]);
-    if (has_inj) {
-        for (int l = tid; l < TX * TZ; l += kThreads) s_inj[l] = 0;
-    }
+    if (has_inj)
+        for (int l = tid; l < TX * TZ; l += kThreads) s_u.inj[l] = 0;
     __syncthreads();
     if (has_inj) {
         for (int k = S.inj_tile[t] + tid; k < S.inj_tile[t + 1]; k += kThreads) {
             const int c0 = S.inj_cptr[k], c1 = S.inj_cptr[k + 1];
-            s_inj[S.inj_cell[k]] = (c0 << 8) | (c1 - c0);
+            s_u.inj[S.inj_cell[k]] = (c0 << 8) | (c1 - c0);
         }
     }
+    if (A.extract && S.ext_tile && S.ext_tile[t + 1] > S.ext_tile[t]) extract_tile<LZ>(S, t, it, s_a);
+    if (has_inj) __syncthreads();
 
-    // ---- receivers whose centre lies in this tile (ExtractData, on u2)
-    if (A.extract && S.ext_tile) {
-        for (int k = S.ext_tile[t] + tid; k < S.ext_tile[t + 1]; k += kThreads) {
-            const int r = S.ext_rec[k];
-            const int pos = S.ext_pos[k];
-            const int lz = (pos & 0xffff) + 4, lx = (pos >> 16) + 4;
-            float *tr = S.traces + (size_t)r * S.nt + (it - 1);
-            if (S.ext_kind == EXT_NEAREST) {
-                // Extraction.f90:84-86
-                *tr = *tr + s_a[lz + LZ * lx];
-            } else {
-                // Extraction.f90:539-545: trace += g*u*wz*wx, j outer, i inner
-                const float *w = S.ext_w + 16 * k;
-                const float g = w[0];
-                float acc = *tr;
-#pragma unroll
-                for (int j = -3; j <= 3; j++)
-#pragma unroll
-                    for (int i = -3; i <= 3; i++)
-                        acc = acc + g * s_a[(lz + i) + LZ * (lx + j)] * w[4 + i] * w[11 + j];
-                *tr = acc;
-            }
-        }
-    }
-
-    const int lane = tid % TZ, xg = tid / TZ;
-    const int iz = z0 + lane;
-    const bool zin = iz >= 4 && iz <= NZ - 5;
+    const bool snap_step = (it % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
-    const bool snap_step = (it % A.snapi) == 0;
     const int lzc = lane + 4;
+    const float *c = A.c;
 
     float u3v[PX];
-    __syncthreads();   // s_inj complete
-
 #pragma unroll
     for (int p = 0; p < PX; p++) {
         const int lx = xg + p * XG;
         const int ix = x0 + lx;
-        const bool xin = ix >= 4 && ix <= NX - 5;
         u3v[p] = 0.f;
-        if (!(zin && xin)) continue;
-        const size_t q = addr(S, iz, ix);
-        const int lxc = lx + 4;
-        const float *col = s_a + LZ * lxc;
-        const float u2c = ADJ ? u2[q] : col[lzc];
-
-        // snapshot / imaging on u2 (interior window only)
-        if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
-            const int wz = iz - S.i0, wx = ix - S.i0;
-            const size_t wq = (size_t)wx * S.nz + wz;
-            const size_t npix = (size_t)S.nz * S.nxw;
-            const int idx = it / A.snapi;   // 1-based snapshot index of this time
-            if (A.store_snap) {            // Extraction.f90:595
-                if (A.snap_fp16) {
-                    const float sc = S.snap_scale[idx - 1];
-                    const float inv = sc > 0.f ? 1.0f / sc : 0.f;
-                    ((__half *)S.snap)[(size_t)(idx - 1) * npix + wq] = __float2half_rn(u2c * inv);
-                } else {
-                    ((float *)S.snap)[(size_t)(idx - 1) * npix + wq] = u2c;
-                }
-            }
-            if (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) {
-                const float Sv = load_snap(S, A.snap_fp16, idx, npix, wq);
-                const float tp = S.taper[wx];
-                float img = S.image[wq];
-                if (A.imaging == IMG_RTM) {
-                    img = img + u2c * Sv * tp;                      // Imaging.f90:265
-                } else if (A.imaging == IMG_FWI) {
-                    const int im1 = idx - 1 > 1 ? idx - 1 : 1;      // Imaging.f90:341-342
-                    const int ip1 = idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap;
-                    const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq);
-                    const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq);
-                    const float v = S.vel[q];
-                    const float tmp = (Sm - 2.f * Sv + Sp) / (S.dt2snap * (v * v * v));  // Imaging.f90:365
-                    img = img + u2c * tmp * tp;
-                } else {
-                    img = img + 2.f * tp * u2c * Sv / S.vel[q];     // Imaging.f90:738
-                }
-                S.image[wq] = img;
-                S.illum[wq] = S.illum[wq] + Sv * Sv * tp;
-            }
-        }
-
+        if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
+        const float u2c = r2[p];
+        const float *col = s_a + LZ * (lx + 4);
+        if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
+            snap_image_point(S, A, it, iz, ix, u2c,
+                             (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? S.vel[(size_t)ix * LDZ + zo] : 0.f);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
-        const float *c = A.c;
         const float tz = c[0] * (col[lzc]) + c[1] * (col[lzc + 1] + col[lzc - 1]) +
                          c[2] * (col[lzc + 2] + col[lzc - 2]) + c[3] * (col[lzc + 3] + col[lzc - 3]) +
                          c[4] * (col[lzc + 4] + col[lzc - 4]);
-        const float tx = c[5] * (col[lzc]) + c[6] * (s_a[lzc + LZ * (lxc + 1)] + s_a[lzc + LZ * (lxc - 1)]) +
-                         c[7] * (s_a[lzc + LZ * (lxc + 2)] + s_a[lzc + LZ * (lxc - 2)]) +
-                         c[8] * (s_a[lzc + LZ * (lxc + 3)] + s_a[lzc + LZ * (lxc - 3)]) +
-                         c[9] * (s_a[lzc + LZ * (lxc + 4)] + s_a[lzc + LZ * (lxc - 4)]);
-        float val = ADJ ? (tx + tz) : S.v2[q] * (tx + tz);
-
+        const float tx = c[5] * (col[lzc]) + c[6] * (col[lzc + LZ] + col[lzc - LZ]) +
+                         c[7] * (col[lzc + 2 * LZ] + col[lzc - 2 * LZ]) +
+                         c[8] * (col[lzc + 3 * LZ] + col[lzc - 3 * LZ]) +
+                         c[9] * (col[lzc + 4 * LZ] + col[lzc - 4 * LZ]);
+        float val = ADJ ? (tx + tz) : rv[p] * (tx + tz);
         // Injection: per-cell contributions in trace order (Injection.f90:434-440)
         if (has_inj) {
-            const int e = s_inj[lane + TZ * lx];
-            const int cnt = e & 0xff;
-            for (int k = e >> 8, kend = (e >> 8) + cnt; k < kend; k++) {
+            const int e = s_u.inj[lane + TZ * lx];
+            for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
                 const float tr = S.inj_traces[(size_t)S.inj_tr[k] * S.nt + (it - 1)];
                 val = val + S.inj_a[k] * tr * S.inj_wz[k] * S.inj_wx[k];
             }
         }
         // Born secondary source: Imaging.f90:779-783, forward time only
-        if (A.imaging == IMG_BORN && snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
+        if (A.imaging == IMG_BORN && (it % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
-            const int idx = it / A.snapi;
-            const float Sv = load_snap(S, A.snap_fp16, idx, (size_t)S.nz * S.nxw, wq);
-            val = val + 2.f * S.taper[wx] * S.born_m[wq] * Sv / S.vel[q];
+            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq);
+            val = val + 2.f * S.taper[wx] * S.born_m[wq] * Sv / S.vel[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
-        u3v[p] = (20.f * u2c - 6.f * u1[q] - 4.f * u0[q] + u3[q] + A.c12dt2 * val) * A.div11;
+        u3v[p] = (20.f * u2c - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * val) * A.div11;
     }
 
-    // ---- Higdon absorbing boundary (Boundary.f90:119-153) for edge tiles
     const bool top = z0 <= 0, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
     if (top || bot || left || right) {
-        __syncthreads();   // s_inj reads done
+        __syncthreads();   // s_u.inj reads done
 #pragma unroll
-        for (int p = 0; p < PX; p++) s_b[lane + TZ * (xg + p * XG)] = u3v[p];
+        for (int p = 0; p < PX; p++) s_u.b[lane + TZ * (xg + p * XG)] = u3v[p];
         __syncthreads();
-        // z-bands: one thread per tile column with ix in [nmin2, nmax2]
-        if (top || bot) {
-            for (int lx = tid; lx < 2 * TX; lx += kThreads) {
-                const int cx = lx % TX;
-                const bool dotop = lx < TX;
-                if (dotop ? !top : !bot) continue;
-                const int ix = x0 + cx;
-                if (ix < 4 || ix > NX - 5) continue;
-                const float *g = S.hz + 16 * (ix - 4) + (dotop ? 0 : 8);
-                float *b = s_b + TZ * cx;
-                const float *a = s_a + LZ * (cx + 4) + 4;   // a[lz] = u2 tile (forward)
-                if (dotop) {
-                    for (int i = 7; i >= 0; i--) {          // nmin1+3 .. nmin1-4
-                        const int lz = i - z0;
-                        float w0, w1, w2;
-                        if (ADJ) {
-                            w0 = u2[addr(S, i, ix)]; w1 = u2[addr(S, i + 1, ix)]; w2 = u2[addr(S, i + 2, ix)];
-                        } else {
-                            w0 = a[lz]; w1 = a[lz + 1]; w2 = a[lz + 2];
-                        }
-                        const float f = g[0] * b[lz + 1] + g[1] * b[lz + 2] + g[2] * w0 + g[3] * w1 + g[4] * w2 +
-                                        g[5] * u1[addr(S, i, ix)] + g[6] * u1[addr(S, i + 1, ix)] +
-                                        g[7] * u1[addr(S, i + 2, ix)];
-                        b[lz] = -f;
-                    }
-                } else {
-                    for (int i = NZ - 8; i <= NZ - 1; i++) {  // nmax1-3 .. nmax1+4
-                        const int lz = i - z0;
-                        float w0, w1, w2;
-                        if (ADJ) {
-                            w0 = u2[addr(S, i, ix)]; w1 = u2[addr(S, i - 1, ix)]; w2 = u2[addr(S, i - 2, ix)];
-                        } else {
-                            w0 = a[lz]; w1 = a[lz - 1]; w2 = a[lz - 2];
-                        }
-                        const float f = g[0] * b[lz - 1] + g[1] * b[lz - 2] + g[2] * w0 + g[3] * w1 + g[4] * w2 +
-                                        g[5] * u1[addr(S, i, ix)] + g[6] * u1[addr(S, i - 1, ix)] +
-                                        g[7] * u1[addr(S, i - 2, ix)];
-                        b[lz] = -f;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // x-bands: one thread per tile row with iz in [nmin1, nmax1]
-        if (left || right) {
-            for (int lz = tid; lz < 2 * TZ; lz += kThreads) {
-                const int cz = lz % TZ;
-                const bool doleft = lz < TZ;
-                if (doleft ? !left : !right) continue;
-                const int i = z0 + cz;
-                if (i < 4 || i > NZ - 5) continue;
-                const float *g = S.hx + 16 * (i - 4) + (doleft ? 0 : 8);
-                if (doleft) {
-                    for (int j = 7; j >= 0; j--) {
-                        const int lx = j - x0;
-                        float w0, w1, w2;
-                        if (ADJ) {
-                            w0 = u2[addr(S, i, j)]; w1 = u2[addr(S, i, j + 1)]; w2 = u2[addr(S, i, j + 2)];
-                        } else {
-                            w0 = s_a[(cz + 4) + LZ * (lx + 4)]; w1 = s_a[(cz + 4) + LZ * (lx + 5)];
-                            w2 = s_a[(cz + 4) + LZ * (lx + 6)];
-                        }
-                        const float f = g[0] * s_b[cz + TZ * (lx + 1)] + g[1] * s_b[cz + TZ * (lx + 2)] +
-                                        g[2] * w0 + g[3] * w1 + g[4] * w2 + g[5] * u1[addr(S, i, j)] +
-                                        g[6] * u1[addr(S, i, j + 1)] + g[7] * u1[addr(S, i, j + 2)];
-                        s_b[cz + TZ * lx] = -f;
-                    }
-                } else {
-                    for (int j = NX - 8; j <= NX - 1; j++) {
-                        const int lx = j - x0;
-                        float w0, w1, w2;
-                        if (ADJ) {
-                            w0 = u2[addr(S, i, j)]; w1 = u2[addr(S, i, j - 1)]; w2 = u2[addr(S, i, j - 2)];
-                        } else {
-                            w0 = s_a[(cz + 4) + LZ * (lx + 4)]; w1 = s_a[(cz + 4) + LZ * (lx + 3)];
-                            w2 = s_a[(cz + 4) + LZ * (lx + 2)];
-                        }
-                        const float f = g[0] * s_b[cz + TZ * (lx - 1)] + g[1] * s_b[cz + TZ * (lx - 2)] +
-                                        g[2] * w0 + g[3] * w1 + g[4] * w2 + g[5] * u1[addr(S, i, j)] +
-                                        g[6] * u1[addr(S, i, j - 1)] + g[7] * u1[addr(S, i, j - 2)];
-                        s_b[cz + TZ * lx] = -f;
-                    }
-                }
-            }
-        }
-        __syncthreads();
+        higdon_tile<TZ, TX, ADJ>(S, z0, x0, top, bot, left, right, s_a, s_u.b, u2, u1);
 #pragma unroll
-        for (int p = 0; p < PX; p++) u3v[p] = s_b[lane + TZ * (xg + p * XG)];
+        for (int p = 0; p < PX; p++) u3v[p] = s_u.b[lane + TZ * (xg + p * XG)];
     }
 
-    // ---- write u3: every non-corner cell of the padded grid that lies in this
-    // tile (FD region from the time update, edge bands from Higdon).  The max
-    // of |u3| over the window feeds the next snapshot's FP16 scale.
-    const bool prev_of_snap = A.store_snap && A.snap_fp16 &&
-        ((A.dir > 0 ? it + 1 : it - 1) % A.snapi) == 0;
+    // ---- write u3 on every non-corner cell of this tile (FD region from the
+    // time update, edge bands from Higdon); track max|u3| over the window
+    // when the next step stores an FP16 snapshot.
+    const bool prev_of_snap = A.store_snap && A.snap_fp16 && ((A.dir > 0 ? it + 1 : it - 1) % A.snapi) == 0;
     float vmax = 0.f;
+    if (iz >= 0 && iz < NZ) {
 #pragma unroll
-    for (int p = 0; p < PX; p++) {
-        const int ix = x0 + xg + p * XG;
-        if (iz < 0 || iz >= NZ || ix < 0 || ix >= NX) continue;
-        const bool zband = iz >= 4 && iz <= NZ - 5, xband = ix >= 4 && ix <= NX - 5;
-        if (!zband && !xband) continue;   // corner: never written by the reference
-        u3[addr(S, iz, ix)] = u3v[p];
-        if (prev_of_snap && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) vmax = fmaxf(vmax, fabsf(u3v[p]));
+        for (int p = 0; p < PX; p++) {
+            const int ix = x0 + xg + p * XG;
+            if (ix < 0 || ix >= NX) continue;
+            const bool xband = ix >= 4 && ix <= NX - 5;
+            if (!zin && !xband) continue;   // corner: never written by the reference
+            u3[(size_t)ix * LDZ + zo] = u3v[p];
+            if (prev_of_snap && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) vmax = fmaxf(vmax, fabsf(u3v[p]));
+        }
     }
     if (prev_of_snap) {
-        // wavefront max, then one atomic per wavefront (non-negative floats order as uints)
         for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
         if ((tid & 63) == 0 && vmax > 0.f) atomicMax(S.snap_max, __float_as_uint(vmax));
     }

@@ -534,59 +534,63 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 
 // Run nsteps of the lockstep time loop.  Steps are recorded into HIP graphs
 // of `graph_chunk` launches and replayed (the per-step arguments differ only
-// in n).  Every event_every-th launch is bracketed by HIP events on the engine
-// stream; their elapsed times give the step kernel's duration.
+// in n).  With sampling on, the first step of every chunk is launched on its
+// own between two HIP events on the engine stream; their elapsed times give
+// the step kernel's duration.
 void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, bool fp16_scale) {
     const int chunk = std::max(1, graph_chunk);
     const int dir = A.dir > 0 ? 0 : 1;
-    const int maxpairs = 64;
     if (event_every > 0 && kev.empty()) {
-        kev.resize(2 * maxpairs);
+        kev.resize(2);
         for (auto &e : kev) HIPCHK(hipEventCreate(&e));
     }
+    auto one = [&](int n) {
+        A.n = n;
+        A.rot = n & 3;
+        launch_step(B, A, adj);
+        if (fp16_scale) {
+            // forward time: after step it=n+1, publish the max for snapshot (n+2)/snapi
+            const int itn = n + 2;
+            if (itn % g.snapi == 0)
+                hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
+                                   itn / g.snapi);
+        }
+    };
     for (int n0 = 0; n0 < nsteps; n0 += chunk) {
         const int n1 = std::min(nsteps, n0 + chunk);
-        const bool use_graph = use_graphs && (n1 - n0) > 1;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        int npairs = 0;
-        if (use_graph) HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int n = n0; n < n1; n++) {
+        int n = n0;
+        const bool sample = event_every > 0;
+        if (sample) {
+            HIPCHK(hipEventRecord(kev[0], stream));
             A.n = n;
             A.rot = n & 3;
-            const bool sample = event_every > 0 && n % event_every == 0 && npairs < maxpairs;
-            if (sample) HIPCHK(hipEventRecord(kev[2 * npairs], stream));
             launch_step(B, A, adj);
-            if (sample) {
-                HIPCHK(hipEventRecord(kev[2 * npairs + 1], stream));
-                npairs++;
-            }
-            if (fp16_scale) {
-                // forward time: after step it=n+1, publish the max for snapshot (n+2)/snapi
-                const int itn = n + 2;
-                if (itn % g.snapi == 0)
-                    hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
-                                       itn / g.snapi);
-            }
+            HIPCHK(hipEventRecord(kev[1], stream));
+            if (fp16_scale && (n + 2) % g.snapi == 0)
+                hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
+                                   (n + 2) / g.snapi);
+            n++;
         }
-        if (use_graph) {
+        if (use_graphs && n1 - n > 1) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            for (; n < n1; n++) one(n);
             HIPCHK(hipStreamEndCapture(stream, &graph));
             HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             HIPCHK(hipGraphLaunch(exec, stream));
-        }
-        if (npairs > 0) {
-            HIPCHK(hipStreamSynchronize(stream));
-            for (int k = 0; k < npairs; k++) {
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, kev[2 * k], kev[2 * k + 1]));
-                kstat_us[dir] += 1000.0 * ms;
-                kstat_n[dir]++;
-            }
-        }
-        if (use_graph) {
             HIPCHK(hipStreamSynchronize(stream));
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
+        } else {
+            for (; n < n1; n++) one(n);
+        }
+        if (sample) {
+            float ms = 0;
+            HIPCHK(hipEventSynchronize(kev[1]));
+            HIPCHK(hipEventElapsedTime(&ms, kev[0], kev[1]));
+            kstat_us[dir] += 1000.0 * ms;
+            kstat_n[dir]++;
         }
     }
     HIPCHK(hipGetLastError());
@@ -1105,7 +1109,7 @@ int sfd_device_info(char *name, int len, int *cu_count, size_t *hbm_bytes) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return SFD_EHIP;
     if (name && len > 0) {
-        snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+        snprintf(name, (size_t)len, "%s%s(%s)", p.name, p.name[0] ? " " : "", p.gcnArchName);
     }
     if (cu_count) *cu_count = p.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;

@@ -47,7 +47,7 @@ def test_device_lbfgs_matches_oracle(problem, boundt):
     else:
         Q = rng.standard_normal((n, n))
         fg = quad(Q @ Q.T + n * np.eye(n), rng.standard_normal(n))
-        x0 = np.zeros(n)
+        x0 = np.ones(n)   # x0 = 0 would make STP1 = |x|/|g|/100 = 0 when boundt != 0 (lbfgs.f:336)
     kw = dict(m=5, xmin=-10.0, xmax=10.0, boundt=boundt, freeze_soft=True)
     xo, ho, fo = run(O.LBFGS(n, **{**kw, "freeze_soft": 1}), fg, x0)
     xd, hd, fd = run(sa.LBFGS(n, **kw), fg, x0)
