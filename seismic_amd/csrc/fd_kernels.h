@@ -25,6 +25,17 @@ namespace sfd {
 
 constexpr int kThreads = 256;
 
+// Global-address-space views of device pointers.  Pointers read from the
+// descriptor structs are generic to the compiler, which then emits FLAT
+// accesses; those also count against the LDS wait counter and serialise the
+// streaming loads with the LDS stencil reads.  Casting to address space 1
+// gives global_load/global_store.
+#define GLB __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GLB T *gp(T *p) { return (GLB T *)p; }
+template <class T>
+__device__ __forceinline__ const GLB T *gp(const T *p) { return (const GLB T *)p; }
+
 // Imaging modes (Imaging.f90)
 enum { IMG_NONE = 0, IMG_RTM = 1, IMG_FWI = 2, IMG_LSRTM = 3, IMG_BORN = 4 };
 // Extraction modes (Extraction.f90), -1 none
@@ -107,26 +118,27 @@ __device__ __forceinline__ int logical_block(const StepArgs &A) {
 // Snapshot idx (1-based) at window offset wq, decoded to FP32.
 __device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq) {
     if (fp16) {
-        const __half h = ((const __half *)S.snap)[(size_t)(idx - 1) * npix + wq];
-        return __half2float(h) * S.snap_scale[idx - 1];
+        const unsigned short h = gp((const unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq];
+        return __half2float(__ushort_as_half(h)) * gp(S.snap_scale)[idx - 1];
     }
-    return ((const float *)S.snap)[(size_t)(idx - 1) * npix + wq];
+    return gp((const float *)S.snap)[(size_t)(idx - 1) * npix + wq];
 }
 
 // Rare per-tile work, kept out of the streaming body: receivers read from the
 // u2 tile (ExtractData, Extraction.f90:539-545 / :84-86).
 template <int LZ>
 __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, const float *s_a) {
-    for (int k = S.ext_tile[t] + (int)threadIdx.x; k < S.ext_tile[t + 1]; k += kThreads) {
-        const int r = S.ext_rec[k];
-        const int pos = S.ext_pos[k];
+    const GLB int *et = gp(S.ext_tile);
+    for (int k = et[t] + (int)threadIdx.x; k < et[t + 1]; k += kThreads) {
+        const int r = gp(S.ext_rec)[k];
+        const int pos = gp(S.ext_pos)[k];
         const int lz = (pos & 0xffff) + 4, lx = (pos >> 16) + 4;
-        float *tr = S.traces + (size_t)r * S.nt + (it - 1);
+        GLB float *tr = gp(S.traces) + (size_t)r * S.nt + (it - 1);
         if (S.ext_kind == EXT_NEAREST) {
             *tr = *tr + s_a[lz + LZ * lx];
         } else {
             // trace += g*u*wz*wx, j outer, i inner
-            const float *w = S.ext_w + 16 * k;
+            const GLB float *w = gp(S.ext_w) + 16 * k;
             const float g = w[0];
             float acc = *tr;
             for (int j = -3; j <= 3; j++)
@@ -146,17 +158,18 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
     const int idx = it / A.snapi;
     if (A.store_snap) {
         if (A.snap_fp16) {
-            const float sc = S.snap_scale[idx - 1];
+            const float sc = gp(S.snap_scale)[idx - 1];
             const float inv = sc > 0.f ? 1.0f / sc : 0.f;
-            ((__half *)S.snap)[(size_t)(idx - 1) * npix + wq] = __float2half_rn(u2c * inv);
+            gp((unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq] = __half_as_ushort(__float2half_rn(u2c * inv));
         } else {
-            ((float *)S.snap)[(size_t)(idx - 1) * npix + wq] = u2c;
+            gp((float *)S.snap)[(size_t)(idx - 1) * npix + wq] = u2c;
         }
     }
     if (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) {
         const float Sv = load_snap(S, A.snap_fp16, idx, npix, wq);
-        const float tp = S.taper[wx];
-        float img = S.image[wq];
+        const float tp = gp(S.taper)[wx];
+        GLB float *imp = gp(S.image), *ilp = gp(S.illum);
+        float img = imp[wq];
         if (A.imaging == IMG_RTM) {
             img = img + u2c * Sv * tp;
         } else if (A.imaging == IMG_FWI) {
@@ -169,8 +182,8 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
         } else {
             img = img + 2.f * tp * u2c * Sv / vel;
         }
-        S.image[wq] = img;
-        S.illum[wq] = S.illum[wq] + Sv * Sv * tp;
+        imp[wq] = img;
+        ilp[wq] = ilp[wq] + Sv * Sv * tp;
     }
 }
 
@@ -180,8 +193,8 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
 // [nmin1, nmax1]), each a sequential recursion outwards.
 template <int TZ, int TX, bool ADJ>
 __device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bool top, bool bot, bool left,
-                                         bool right, const float *s_a, float *s_b, const float *__restrict__ u2,
-                                         const float *__restrict__ u1) {
+                                         bool right, const float *s_a, float *s_b, const GLB float *__restrict__ u2,
+                                         const GLB float *__restrict__ u1) {
     constexpr int LZ = TZ + 8;
     const int NZ = S.NZ, NX = S.NX, tid = threadIdx.x;
     // u2 at (i, j): from the LDS tile (forward) or global memory (adjoint:
@@ -197,7 +210,7 @@ __device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bo
             if (dotop ? !top : !bot) continue;
             const int ix = x0 + cx;
             if (ix < 4 || ix > NX - 5) continue;
-            const float *g = S.hz + 16 * (ix - 4) + (dotop ? 0 : 8);
+            const GLB float *g = gp(S.hz) + 16 * (ix - 4) + (dotop ? 0 : 8);
             float *b = s_b + TZ * cx;
             if (dotop) {
                 for (int i = 7; i >= 0; i--) {
@@ -226,7 +239,7 @@ __device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bo
             if (doleft ? !left : !right) continue;
             const int i = z0 + cz;
             if (i < 4 || i > NZ - 5) continue;
-            const float *g = S.hx + 16 * (i - 4) + (doleft ? 0 : 8);
+            const GLB float *g = gp(S.hx) + 16 * (i - 4) + (doleft ? 0 : 8);
             if (doleft) {
                 for (int j = 7; j >= 0; j--) {
                     const int lx = j - x0;
@@ -277,11 +290,11 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
     const int t = lb - S.tile0;
     const int tzi = t % S.ntz, txi = t / S.ntz;
     const int z0 = tzi * TZ - S.zsh, x0 = txi * TX - S.xsh;
-    float *__restrict__ u2 = S.u[(A.rot + 3) & 3];
-    const float *__restrict__ u1 = S.u[(A.rot + 2) & 3];
-    const float *__restrict__ u0 = S.u[(A.rot + 1) & 3];
-    float *__restrict__ u3 = S.u[A.rot];      // u_{-1} in, u3 out
-    const float *__restrict__ v2p = S.v2;
+    const GLB float *__restrict__ u2 = gp(S.u[(A.rot + 3) & 3]);
+    const GLB float *__restrict__ u1 = gp(S.u[(A.rot + 2) & 3]);
+    const GLB float *__restrict__ u0 = gp(S.u[(A.rot + 1) & 3]);
+    GLB float *__restrict__ u3 = gp(S.u[A.rot]);      // u_{-1} in, u3 out
+    const GLB float *__restrict__ v2p = gp(S.v2);
     const int tid = threadIdx.x;
     const int lane = tid % TZ, xg = tid / TZ;
     const int iz = z0 + lane;
@@ -331,17 +344,19 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
     for (int p = 0; p < PX; p++) fill_col(xg + p * XG + 4, &r2[p]);
     for (int h = xg; h < 8; h += XG) fill_col(h < 4 ? h : TX + h, nullptr);
 
-    const int has_inj = A.inject && (S.inj_tile[t + 1] > S.inj_tile[t]);
+    const GLB int *inj_tile = gp(S.inj_tile);
+    const int has_inj = A.inject && (inj_tile[t + 1] > inj_tile[t]);
     if (has_inj)
         for (int l = tid; l < TX * TZ; l += kThreads) s_u.inj[l] = 0;
     __syncthreads();
     if (has_inj) {
-        for (int k = S.inj_tile[t] + tid; k < S.inj_tile[t + 1]; k += kThreads) {
-            const int c0 = S.inj_cptr[k], c1 = S.inj_cptr[k + 1];
-            s_u.inj[S.inj_cell[k]] = (c0 << 8) | (c1 - c0);
+        const GLB int *cptr = gp(S.inj_cptr), *cell = gp(S.inj_cell);
+        for (int k = inj_tile[t] + tid; k < inj_tile[t + 1]; k += kThreads) {
+            const int c0 = cptr[k], c1 = cptr[k + 1];
+            s_u.inj[cell[k]] = (c0 << 8) | (c1 - c0);
         }
     }
-    if (A.extract && S.ext_tile && S.ext_tile[t + 1] > S.ext_tile[t]) extract_tile<LZ>(S, t, it, s_a);
+    if (A.extract && S.ext_tile && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t]) extract_tile<LZ>(S, t, it, s_a);
     if (has_inj) __syncthreads();
 
     const bool snap_step = (it % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
@@ -360,7 +375,7 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
         const float *col = s_a + LZ * (lx + 4);
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
             snap_image_point(S, A, it, iz, ix, u2c,
-                             (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? S.vel[(size_t)ix * LDZ + zo] : 0.f);
+                             (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
         const float tz = c[0] * (col[lzc]) + c[1] * (col[lzc + 1] + col[lzc - 1]) +
                          c[2] * (col[lzc + 2] + col[lzc - 2]) + c[3] * (col[lzc + 3] + col[lzc - 3]) +
@@ -373,9 +388,11 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
         // Injection: per-cell contributions in trace order (Injection.f90:434-440)
         if (has_inj) {
             const int e = s_u.inj[lane + TZ * lx];
+            const GLB float *trs = gp(S.inj_traces), *ia = gp(S.inj_a), *iwz = gp(S.inj_wz), *iwx = gp(S.inj_wx);
+            const GLB int *itr = gp(S.inj_tr);
             for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
-                const float tr = S.inj_traces[(size_t)S.inj_tr[k] * S.nt + (it - 1)];
-                val = val + S.inj_a[k] * tr * S.inj_wz[k] * S.inj_wx[k];
+                const float tr = trs[(size_t)itr[k] * S.nt + (it - 1)];
+                val = val + ia[k] * tr * iwz[k] * iwx[k];
             }
         }
         // Born secondary source: Imaging.f90:779-783, forward time only
@@ -383,7 +400,7 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
             const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq);
-            val = val + 2.f * S.taper[wx] * S.born_m[wq] * Sv / S.vel[(size_t)ix * LDZ + zo];
+            val = val + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
         u3v[p] = (20.f * u2c - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * val) * A.div11;
