@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--snap", choices=["fp32", "fp16"], default="fp32")
-    ap.add_argument("--tile", default="64x32")
+    ap.add_argument("--tile", default="32x32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=1, help="0: eager launches (for profilers)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)

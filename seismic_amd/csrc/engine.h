@@ -70,7 +70,7 @@ struct Engine {
     std::vector<float> vel_h, velb_h;
     float *d_vel = nullptr, *d_velb = nullptr;
     bool have_model = false;
-    int tz = 64, tx = 32;
+    int tz = 32, tx = 32;
     bool autotune_pending = false;
     int xcd_remap = 1;
     int graph_chunk = 199;   // not a multiple of 4: sampled steps rotate through snapshot phases

@@ -82,6 +82,8 @@ struct ShotDev {
 };
 
 struct StepArgs {
+    ShotDev s0;             // the shot itself when the batch has one shot (no table lookups)
+    int single;
     const ShotDev *shots;
     const int *blk_shot;    // logical tile -> shot
     int nblocks;            // logical tiles (grid may be padded to a multiple of 8)
@@ -268,7 +270,7 @@ __device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bo
 // LDS; LDS column c is filled by x-group (c-4) mod XG, which makes every
 // thread load the u2 centre values it later needs.
 template <int TZ, int TX, bool ADJ>
-__global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
+__device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
     constexpr int XG = kThreads / TZ;   // x-groups of threads
     constexpr int PX = TX / XG;         // columns per thread
     constexpr int LZ = TZ + 8;          // LDS tile rows incl. halo
@@ -280,9 +282,6 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
         float b[TX * TZ];               // u3 tile for the Higdon recursions
     } s_u;
 
-    const int lb = logical_block(A);
-    if (lb >= A.nblocks) return;
-    const ShotDev &S = A.shots[A.blk_shot[lb]];
     const int it = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     if (it < 1 || it > S.nt) return;   // this shot has finished (ragged batch)
 
@@ -436,6 +435,20 @@ __global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
     if (prev_of_snap) {
         for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
         if ((tid & 63) == 0 && vmax > 0.f) atomicMax(S.snap_max, __float_as_uint(vmax));
+    }
+}
+
+// Kernel entry: SINGLE batches read the shot descriptor from the kernel
+// arguments (scalar loads, no dependent table lookups); batched launches
+// find their shot through the logical-tile table.
+template <int TZ, int TX, bool ADJ, bool SINGLE>
+__global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
+    const int lb = logical_block(A);
+    if (lb >= A.nblocks) return;
+    if constexpr (SINGLE) {
+        fd_step_body<TZ, TX, ADJ>(A, A.s0, lb);
+    } else {
+        fd_step_body<TZ, TX, ADJ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

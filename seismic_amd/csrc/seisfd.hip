@@ -514,18 +514,24 @@ StepArgs Engine::base_args(const Batch &B) const {
 
 typedef void (*StepFn)(StepArgs);
 
-static StepFn pick_kernel(int TZ, int TX, bool adj) {
-#define K(a, b)                                                                             \
-    if (TZ == a && TX == b) return adj ? (StepFn)fd_step_kernel<a, b, true> : (StepFn)fd_step_kernel<a, b, false>;
-    K(64, 32) K(64, 64) K(32, 32) K(32, 64) K(128, 32) K(64, 128)
+static StepFn pick_kernel(int TZ, int TX, bool adj, bool single) {
+#define K(a, b)                                                                                      \
+    if (TZ == a && TX == b)                                                                          \
+        return adj ? (single ? (StepFn)fd_step_kernel<a, b, true, true> : (StepFn)fd_step_kernel<a, b, true, false>) \
+                   : (single ? (StepFn)fd_step_kernel<a, b, false, true> : (StepFn)fd_step_kernel<a, b, false, false>);
+    K(32, 32) K(64, 32) K(32, 64) K(64, 64)
 #undef K
     return nullptr;
 }
 
-bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false) != nullptr; }
+bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, false) != nullptr; }
 
 void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
-    StepFn f = pick_kernel(tz, tx, adj);
+    if (B.ns == 1) {
+        A.single = 1;
+        A.s0 = B.shots_h[0];
+    }
+    StepFn f = pick_kernel(tz, tx, adj, A.single != 0);
     if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
     int grid = B.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;

@@ -70,7 +70,7 @@ def test_forward_offgrid_all_extractions(ext):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("tile", [(64, 32), (32, 32), (64, 64), (128, 32), (32, 64), (64, 128)])
+@pytest.mark.parametrize("tile", [(64, 32), (32, 32), (64, 64), (32, 64)])
 def test_tile_shapes_bitwise(tile):
     """Every compiled tile shape gives the same gather (grid sizes chosen so
     tile-origin shifts and partial tiles occur)."""
