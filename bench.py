@@ -91,9 +91,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--snap", choices=["fp32", "fp16"], default="fp32")
-    ap.add_argument("--tile", default="32x32")
+    ap.add_argument("--tile", default="64x32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=1, help="0: eager launches (for profilers)")
+    ap.add_argument("--strip", type=int, default=0, help="columns per wavefront (0 auto, 4, 8, 16)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -112,6 +113,7 @@ def main():
     tz, tx = (int(v) for v in args.tile.split("x"))
     e.set_tile(tz, tx)
     e.set_option(sa.abi.SFD_OPT_GRAPHS, args.graphs)
+    e.set_option(sa.abi.SFD_OPT_STRIP_PX, args.strip)
     shot = sa.Shot(0.0, DZ * (NX // 2), w, rz, rx)
     log("modeling observed data (untimed)")
     e.set_model(true)
@@ -162,7 +164,7 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "rtm-single-shot-751x2301-nt7500", "grid": [NZ, NX], "dx_m": DZ, "dt_s": DT,
                       "nt": NT, "ntaper": NTAPER, "receivers": NX, "snapi": SNAPI, "snapshots": args.snap,
-                      "tile": args.tile, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
+                      "tile": args.tile, "strip": args.strip, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
                       "device": info["name"]},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

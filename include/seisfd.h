@@ -173,6 +173,7 @@ int  sfd_set_tile(sfd_handle h, int tile_z, int tile_x);
 #define SFD_OPT_GRAPH_CHUNK  2  /* time steps per graph (default 199)            */
 #define SFD_OPT_XCD_REMAP    3  /* 1: XCD-contiguous tile order (default)          */
 #define SFD_OPT_KERNEL_EVENTS 4 /* 1: time the first step of every graph chunk with HIP events (default) */
+#define SFD_OPT_STRIP_PX     5  /* columns per wavefront in the step kernel: 0 auto (4 for 64x32 tiles), 4, 8, 16 */
 int  sfd_set_option(sfd_handle h, int opt, int value);
 
 /*

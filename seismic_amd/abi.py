@@ -31,6 +31,7 @@ SFD_OPT_GRAPHS = 1
 SFD_OPT_GRAPH_CHUNK = 2
 SFD_OPT_XCD_REMAP = 3
 SFD_OPT_KERNEL_EVENTS = 4
+SFD_OPT_STRIP_PX = 5
 
 
 class sfd_grid(C.Structure):

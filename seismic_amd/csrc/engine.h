@@ -26,10 +26,10 @@ struct ShotPlan {
     int n_inj_traces = 1;
     std::vector<int> colsrc, colmap_raw;
     std::vector<float> taper;
-    std::vector<int> inj_tile, inj_cell, inj_cptr, inj_tr;
-    std::vector<float> inj_a, inj_wz, inj_wx;
-    std::vector<int> injb_tile, injb_cell, injb_cptr, injb_tr;
-    std::vector<float> injb_a, injb_wz, injb_wx;
+    std::vector<int> inj_tile, inj_cell, inj_cptr, inj_tptr, inj_ttr;
+    std::vector<float> inj_coef;   // 4 per contribution: v2r, sinc_z, sinc_x, local trace (int bits)
+    std::vector<int> injb_tile, injb_cell, injb_cptr, injb_tptr, injb_ttr;
+    std::vector<float> injb_coef;
     std::vector<int> ext_tile, ext_rec, ext_pos;
     std::vector<float> ext_w;
     int ext_kind = 0;
@@ -37,8 +37,8 @@ struct ShotPlan {
 };
 
 struct InjSet {
-    const int *tile = nullptr, *cell = nullptr, *cptr = nullptr, *tr = nullptr;
-    const float *a = nullptr, *wz = nullptr, *wx = nullptr;
+    const int *tile = nullptr, *cell = nullptr, *cptr = nullptr, *tptr = nullptr, *ttr = nullptr;
+    const sfd::InjCoef *coef = nullptr;
 };
 
 struct BatchNeeds {
@@ -70,9 +70,10 @@ struct Engine {
     std::vector<float> vel_h, velb_h;
     float *d_vel = nullptr, *d_velb = nullptr;
     bool have_model = false;
-    int tz = 32, tx = 32;
+    int tz = 64, tx = 32;
     bool autotune_pending = false;
     int xcd_remap = 1;
+    int strip_px = 0;        // columns per wavefront (0: tile width / 4)
     int graph_chunk = 199;   // not a multiple of 4: sampled steps rotate through snapshot phases
     bool use_graphs = true;
     int event_every = 1;

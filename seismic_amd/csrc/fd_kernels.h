@@ -24,6 +24,15 @@
 namespace sfd {
 
 constexpr int kThreads = 256;
+constexpr int kMaxTileTraces = 1024;   // distinct traces one tile may inject
+
+// One injection contribution: gain (v^2 or 1/(dz dx)), sinc weights, and the
+// tile-local index of the trace it injects.
+struct __attribute__((aligned(16))) InjCoef {
+    float a, wz, wx;
+    int tr;
+};
+typedef float float4v __attribute__((ext_vector_type(4)));
 
 // Global-address-space views of device pointers.  Pointers read from the
 // descriptor structs are generic to the compiler, which then emits FLAT
@@ -35,6 +44,15 @@ template <class T>
 __device__ __forceinline__ GLB T *gp(T *p) { return (GLB T *)p; }
 template <class T>
 __device__ __forceinline__ const GLB T *gp(const T *p) { return (const GLB T *)p; }
+// Wavefield element at a 32-bit element offset (< 2^30 cells per array):
+// a uniform base plus a 32-bit byte offset maps to the SGPR-base addressing
+// form, one VGPR per address instead of a 64-bit pair.
+__device__ __forceinline__ float ldw(const GLB float *base, unsigned e) {
+    return *(const GLB float *)((const GLB char *)base + (e << 2));
+}
+__device__ __forceinline__ void stw(GLB float *base, unsigned e, float v) {
+    *(GLB float *)((GLB char *)base + (e << 2)) = v;
+}
 
 // Imaging modes (Imaging.f90)
 enum { IMG_NONE = 0, IMG_RTM = 1, IMG_FWI = 2, IMG_LSRTM = 3, IMG_BORN = 4 };
@@ -58,10 +76,9 @@ struct ShotDev {
     const int *inj_tile;
     const int *inj_cell;    // local cell index lz + TZ*lx
     const int *inj_cptr;
-    const float *inj_a;     // v2r (gain) per contribution
-    const float *inj_wz;    // sinc z weight
-    const float *inj_wx;    // sinc x weight
-    const int *inj_tr;      // trace index per contribution
+    const InjCoef *inj_coef; // per contribution
+    const int *inj_tptr;    // per tile [inj_tptr[t], inj_tptr[t+1]) traces it injects
+    const int *inj_ttr;     // global trace index of each tile-local trace
     const float *inj_traces;// source traces, (nt, ntraces) time fastest
     // extraction: per tile [ext_tile[t], ext_tile[t+1]) receivers
     const int *ext_tile;
@@ -126,25 +143,24 @@ __device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, 
     return gp((const float *)S.snap)[(size_t)(idx - 1) * npix + wq];
 }
 
-// Rare per-tile work, kept out of the streaming body: receivers read from the
-// u2 tile (ExtractData, Extraction.f90:539-545 / :84-86).
-template <int LZ>
-__device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, const float *s_a) {
+// Receivers whose centre lies in this tile (ExtractData on u2,
+// Extraction.f90:539-545 / :84-86); the 7x7 footprint is read from memory.
+__device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, const GLB float *u2, int z0, int x0) {
     const GLB int *et = gp(S.ext_tile);
-    for (int k = et[t] + (int)threadIdx.x; k < et[t + 1]; k += kThreads) {
+    for (int k = et[t] + (int)threadIdx.x; k < et[t + 1]; k += (int)blockDim.x) {
         const int r = gp(S.ext_rec)[k];
         const int pos = gp(S.ext_pos)[k];
-        const int lz = (pos & 0xffff) + 4, lx = (pos >> 16) + 4;
+        const int iz = z0 + (pos & 0xffff), ix = x0 + (pos >> 16);
         GLB float *tr = gp(S.traces) + (size_t)r * S.nt + (it - 1);
         if (S.ext_kind == EXT_NEAREST) {
-            *tr = *tr + s_a[lz + LZ * lx];
+            *tr = *tr + u2[addr(S, iz, ix)];
         } else {
             // trace += g*u*wz*wx, j outer, i inner
             const GLB float *w = gp(S.ext_w) + 16 * k;
             const float g = w[0];
             float acc = *tr;
             for (int j = -3; j <= 3; j++)
-                for (int i = -3; i <= 3; i++) acc = acc + g * s_a[(lz + i) + LZ * (lx + j)] * w[4 + i] * w[11 + j];
+                for (int i = -3; i <= 3; i++) acc = acc + g * u2[addr(S, iz + i, ix + j)] * w[4 + i] * w[11 + j];
             *tr = acc;
         }
     }
@@ -153,7 +169,7 @@ __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, co
 // Snapshot store and imaging condition at one interior point (on u2, before
 // the stencil: Extraction.f90:595, Imaging.f90:265 / :365 / :738).
 __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
-                                              float u2c, float vel) {
+                                                 float u2c, float vel) {
     const int wz = iz - S.i0, wx = ix - S.i0;
     const size_t wq = (size_t)wx * S.nz + wz;
     const size_t npix = (size_t)S.nz * S.nxw;
@@ -189,53 +205,44 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
     }
 }
 
-// Higdon absorbing boundary (Boundary.f90:119-153) on the u3 tile in s_b, for
-// tiles that touch the grid edge.  z-bands first (one thread per column with
-// ix in [nmin2, nmax2]), then x-bands (one thread per row with iz in
-// [nmin1, nmax1]), each a sequential recursion outwards.
-template <int TZ, int TX, bool ADJ>
+// Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
+// the grid edge.  s_b holds the tile's u3, s_u2 / s_u1 its u2 / u1.  z-bands
+// first (one thread per column with ix in [nmin2, nmax2]), then x-bands (one
+// thread per row with iz in [nmin1, nmax1]), each a sequential recursion
+// outwards, exactly in the reference's order.
+template <int TZ, int TX>
 __device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bool top, bool bot, bool left,
-                                         bool right, const float *s_a, float *s_b, const GLB float *__restrict__ u2,
-                                         const GLB float *__restrict__ u1) {
-    constexpr int LZ = TZ + 8;
+                                            bool right, float *s_b, const float *s_u2, const float *s_u1) {
     const int NZ = S.NZ, NX = S.NX, tid = threadIdx.x;
-    // u2 at (i, j): from the LDS tile (forward) or global memory (adjoint:
-    // the tile holds v^2 u2 there)
-    auto U2 = [&](int i, int j) -> float {
-        if (ADJ) return u2[addr(S, i, j)];
-        return s_a[(i - z0 + 4) + LZ * (j - x0 + 4)];
-    };
+#define L(i, j) ((i) - z0 + TZ * ((j) - x0))
     if (top || bot) {
-        for (int l = tid; l < 2 * TX; l += kThreads) {
+        for (int l = tid; l < 2 * TX; l += (int)blockDim.x) {
             const int cx = l % TX;
             const bool dotop = l < TX;
             if (dotop ? !top : !bot) continue;
             const int ix = x0 + cx;
             if (ix < 4 || ix > NX - 5) continue;
             const GLB float *g = gp(S.hz) + 16 * (ix - 4) + (dotop ? 0 : 8);
-            float *b = s_b + TZ * cx;
             if (dotop) {
                 for (int i = 7; i >= 0; i--) {
-                    const int lz = i - z0;
-                    const float f = g[0] * b[lz + 1] + g[1] * b[lz + 2] + g[2] * U2(i, ix) + g[3] * U2(i + 1, ix) +
-                                    g[4] * U2(i + 2, ix) + g[5] * u1[addr(S, i, ix)] + g[6] * u1[addr(S, i + 1, ix)] +
-                                    g[7] * u1[addr(S, i + 2, ix)];
-                    b[lz] = -f;
+                    const float f = g[0] * s_b[L(i + 1, ix)] + g[1] * s_b[L(i + 2, ix)] + g[2] * s_u2[L(i, ix)] +
+                                    g[3] * s_u2[L(i + 1, ix)] + g[4] * s_u2[L(i + 2, ix)] + g[5] * s_u1[L(i, ix)] +
+                                    g[6] * s_u1[L(i + 1, ix)] + g[7] * s_u1[L(i + 2, ix)];
+                    s_b[L(i, ix)] = -f;
                 }
             } else {
                 for (int i = NZ - 8; i <= NZ - 1; i++) {
-                    const int lz = i - z0;
-                    const float f = g[0] * b[lz - 1] + g[1] * b[lz - 2] + g[2] * U2(i, ix) + g[3] * U2(i - 1, ix) +
-                                    g[4] * U2(i - 2, ix) + g[5] * u1[addr(S, i, ix)] + g[6] * u1[addr(S, i - 1, ix)] +
-                                    g[7] * u1[addr(S, i - 2, ix)];
-                    b[lz] = -f;
+                    const float f = g[0] * s_b[L(i - 1, ix)] + g[1] * s_b[L(i - 2, ix)] + g[2] * s_u2[L(i, ix)] +
+                                    g[3] * s_u2[L(i - 1, ix)] + g[4] * s_u2[L(i - 2, ix)] + g[5] * s_u1[L(i, ix)] +
+                                    g[6] * s_u1[L(i - 1, ix)] + g[7] * s_u1[L(i - 2, ix)];
+                    s_b[L(i, ix)] = -f;
                 }
             }
         }
         __syncthreads();
     }
     if (left || right) {
-        for (int l = tid; l < 2 * TZ; l += kThreads) {
+        for (int l = tid; l < 2 * TZ; l += (int)blockDim.x) {
             const int cz = l % TZ;
             const bool doleft = l < TZ;
             if (doleft ? !left : !right) continue;
@@ -244,47 +251,55 @@ __device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bo
             const GLB float *g = gp(S.hx) + 16 * (i - 4) + (doleft ? 0 : 8);
             if (doleft) {
                 for (int j = 7; j >= 0; j--) {
-                    const int lx = j - x0;
-                    const float f = g[0] * s_b[cz + TZ * (lx + 1)] + g[1] * s_b[cz + TZ * (lx + 2)] + g[2] * U2(i, j) +
-                                    g[3] * U2(i, j + 1) + g[4] * U2(i, j + 2) + g[5] * u1[addr(S, i, j)] +
-                                    g[6] * u1[addr(S, i, j + 1)] + g[7] * u1[addr(S, i, j + 2)];
-                    s_b[cz + TZ * lx] = -f;
+                    const float f = g[0] * s_b[L(i, j + 1)] + g[1] * s_b[L(i, j + 2)] + g[2] * s_u2[L(i, j)] +
+                                    g[3] * s_u2[L(i, j + 1)] + g[4] * s_u2[L(i, j + 2)] + g[5] * s_u1[L(i, j)] +
+                                    g[6] * s_u1[L(i, j + 1)] + g[7] * s_u1[L(i, j + 2)];
+                    s_b[L(i, j)] = -f;
                 }
             } else {
                 for (int j = NX - 8; j <= NX - 1; j++) {
-                    const int lx = j - x0;
-                    const float f = g[0] * s_b[cz + TZ * (lx - 1)] + g[1] * s_b[cz + TZ * (lx - 2)] + g[2] * U2(i, j) +
-                                    g[3] * U2(i, j - 1) + g[4] * U2(i, j - 2) + g[5] * u1[addr(S, i, j)] +
-                                    g[6] * u1[addr(S, i, j - 1)] + g[7] * u1[addr(S, i, j - 2)];
-                    s_b[cz + TZ * lx] = -f;
+                    const float f = g[0] * s_b[L(i, j - 1)] + g[1] * s_b[L(i, j - 2)] + g[2] * s_u2[L(i, j)] +
+                                    g[3] * s_u2[L(i, j - 1)] + g[4] * s_u2[L(i, j - 2)] + g[5] * s_u1[L(i, j)] +
+                                    g[6] * s_u1[L(i, j - 1)] + g[7] * s_u1[L(i, j - 2)];
+                    s_b[L(i, j)] = -f;
                 }
             }
         }
     }
+#undef L
     __syncthreads();
 }
 
-// The step kernel.  Thread (lane, xg) owns tile points (lane, xg + p*XG),
-// p < PX.  Its own u1, u0, u_-1 and v^2 are loaded into registers first, so
-// those loads are in flight while the u2 tile (+4-cell halo) is staged in
-// LDS; LDS column c is filled by x-group (c-4) mod XG, which makes every
-// thread load the u2 centre values it later needs.
-template <int TZ, int TX, bool ADJ>
+// The step kernel body.  A workgroup owns a 64 x 4*PX tile; each of its four
+// wavefronts owns a 64 x PX strip: lane = row, and the lane keeps u2 of its
+// row for the strip's columns plus 4 on each side in registers (the
+// x-derivative).  The z-derivative goes through a wavefront-private LDS
+// column (the 64 rows plus 4+4 halo rows loaded by lanes 0..7), so interior
+// tiles need no workgroup barrier.  Edge tiles (Higdon) and tiles with
+// injections synchronise the workgroup.
+template <int PX>
+struct StepShape {
+    static constexpr int TZ = 64;
+    static constexpr int TX = PX >= 8 ? 4 * PX : 32;   // >= 19 columns: Higdon bands stay in one tile
+    static constexpr int NW = TX / PX;                   // wavefronts per workgroup
+    static constexpr int THREADS = 64 * NW;
+};
+
+template <int PX, bool ADJ>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
-    constexpr int XG = kThreads / TZ;   // x-groups of threads
-    constexpr int PX = TX / XG;         // columns per thread
-    constexpr int LZ = TZ + 8;          // LDS tile rows incl. halo
-    constexpr int LX = TX + 8;
-    static_assert(kThreads % TZ == 0 && TX % XG == 0 && TZ >= 8, "tile shape");
-    __shared__ float s_a[LX * LZ];      // u2 (forward) or v^2 u2 (adjoint), with halo
+    constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
+    constexpr int NT = StepShape<PX>::THREADS;
+    __shared__ float s_col[NW][72];     // per-wavefront z-stencil column
+    __shared__ float s_b[TX * TZ];      // edge tiles: u3
+    __shared__ float s_u2[TX * TZ];     // edge tiles: u2
     __shared__ union {
         int inj[TX * TZ];               // per-cell contribution range (start<<8 | count)
-        float b[TX * TZ];               // u3 tile for the Higdon recursions
+        float u1[TX * TZ];              // edge tiles: u1
     } s_u;
+    __shared__ float s_trv[kMaxTileTraces];   // injection tiles: this step's trace samples
 
     const int it = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     if (it < 1 || it > S.nt) return;   // this shot has finished (ragged batch)
-
     const int NZ = S.NZ, NX = S.NX, LDZ = S.LDZ, zoff = S.zoff;
     const int t = lb - S.tile0;
     const int tzi = t % S.ntz, txi = t / S.ntz;
@@ -294,104 +309,109 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const GLB float *__restrict__ u0 = gp(S.u[(A.rot + 1) & 3]);
     GLB float *__restrict__ u3 = gp(S.u[A.rot]);      // u_{-1} in, u3 out
     const GLB float *__restrict__ v2p = gp(S.v2);
-    const int tid = threadIdx.x;
-    const int lane = tid % TZ, xg = tid / TZ;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int iz = z0 + lane;
+    const int xs = x0 + wv * PX;                      // first column of this wavefront's strip
+    const bool zarr = iz >= 0 && iz < NZ;
     const bool zin = iz >= 4 && iz <= NZ - 5;
     const size_t zo = (size_t)(zoff + iz);
+    // halo row served by lanes 0..7: 4 above, 4 below the strip
+    const int hz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;
+    const bool hok = lane < 8 && hz >= 0 && hz < NZ;
 
-    // ---- register prefetch of this thread's u1, u0, u_-1, v^2
-    float r1[PX], r0[PX], rm[PX], rv[PX], r2[PX];
+    // ---- receivers centred in this tile (reads u2 only; done before the
+    // streaming loads so its registers are not live alongside them)
+    if (A.extract && S.ext_tile && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t]) extract_tile(S, t, it, u2, z0, x0);
+
+    // ---- loads: u2 (and v^2 for the adjoint) window, halos, own u1/u0/u_-1/v^2.
+    // Addresses are clamped into the array and the loads are unconditional,
+    // so they issue back to back with no wait in between.  A clamped value
+    // only ever feeds cells outside the FD region, whose results are dropped
+    // (an FD cell's +-4 neighbours always lie inside the padded array).
+    // (32-bit element offsets: one shot's wavefield is far below 2^32 cells,
+    // and a uniform base plus a 32-bit offset needs one VGPR per address)
+    const int izc = iz < 0 ? 0 : (iz >= NZ ? NZ - 1 : iz);
+    const int hzc = hz < 0 ? 0 : (hz >= NZ ? NZ - 1 : hz);
+    const unsigned zoc = (unsigned)(zoff + izc), hzo = (unsigned)(zoff + hzc);
+    float wu[NWIN], wv2[ADJ ? NWIN : 1], hu[PX], hv[ADJ ? PX : 1];
+    float r1[PX], r0[PX], rm[PX], rv[PX];
+#pragma unroll
+    for (int k = 0; k < NWIN; k++) {
+        int ix = xs - 4 + k;
+        ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
+        wu[k] = ldw(u2, (unsigned)ix * (unsigned)LDZ + zoc);
+        if (ADJ) wv2[k] = ldw(v2p, (unsigned)ix * (unsigned)LDZ + zoc);
+    }
 #pragma unroll
     for (int p = 0; p < PX; p++) {
-        const int ix = x0 + xg + p * XG;
-        r1[p] = r0[p] = rm[p] = rv[p] = r2[p] = 0.f;
-        if (zin && ix >= 4 && ix <= NX - 5) {
-            const size_t q = (size_t)ix * LDZ + zo;
-            r1[p] = u1[q];
-            r0[p] = u0[q];
-            rm[p] = u3[q];
-            if (!ADJ) rv[p] = v2p[q];
-        }
+        int ix = xs + p;
+        ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
+        const unsigned cb = (unsigned)ix * (unsigned)LDZ;
+        hu[p] = ldw(u2, cb + hzo);
+        if (ADJ) hv[p] = ldw(v2p, cb + hzo);
+        r1[p] = ldw(u1, cb + zoc);
+        r0[p] = ldw(u0, cb + zoc);
+        rm[p] = ldw(u3, cb + zoc);
+        if (!ADJ) rv[p] = ldw(v2p, cb + zoc);
     }
+    (void)hok;
 
-    // ---- stage u2 (or v^2 u2) tile + halo in LDS; keep own centre values.
-    // One LDS column: every lane its own row, lanes 0..7 the 4+4 halo rows.
-    auto fill_col = [&](int cl, float *own) {
-        const int ix = x0 - 4 + cl;
-        const bool xok = ix >= 0 && ix < NX;
-        const size_t cb = (size_t)ix * LDZ + zoff;
-        float val = 0.f;
-        if (xok && iz >= 0 && iz < NZ) {
-            val = u2[cb + iz];
-            if (own) *own = val;
-            if (ADJ) val = v2p[cb + iz] * val;
-        }
-        s_a[(lane + 4) + LZ * cl] = val;
-        if (lane < 8) {
-            const int lz = lane < 4 ? lane : TZ + lane;
-            const int jz = z0 - 4 + lz;
-            float hv = 0.f;
-            if (xok && jz >= 0 && jz < NZ) {
-                hv = u2[cb + jz];
-                if (ADJ) hv = v2p[cb + jz] * hv;
-            }
-            s_a[lz + LZ * cl] = hv;
-        }
-    };
-#pragma unroll
-    for (int p = 0; p < PX; p++) fill_col(xg + p * XG + 4, &r2[p]);
-    for (int h = xg; h < 8; h += XG) fill_col(h < 4 ? h : TX + h, nullptr);
-
+    // ---- injection tiles (workgroup-uniform branch): per-cell contribution
+    // table, and this step's sample of every trace the tile injects, staged
+    // in LDS so a contribution costs one packed load and one LDS read.
     const GLB int *inj_tile = gp(S.inj_tile);
     const int has_inj = A.inject && (inj_tile[t + 1] > inj_tile[t]);
-    if (has_inj)
-        for (int l = tid; l < TX * TZ; l += kThreads) s_u.inj[l] = 0;
-    __syncthreads();
     if (has_inj) {
+        for (int l = tid; l < TX * TZ; l += NT) s_u.inj[l] = 0;
+        const GLB int *ttr = gp(S.inj_ttr);
+        const GLB float *trs = gp(S.inj_traces);
+        for (int j = gp(S.inj_tptr)[t] + tid, j0 = gp(S.inj_tptr)[t]; j < gp(S.inj_tptr)[t + 1]; j += NT)
+            s_trv[j - j0] = trs[(size_t)ttr[j] * S.nt + (it - 1)];
+        __syncthreads();
         const GLB int *cptr = gp(S.inj_cptr), *cell = gp(S.inj_cell);
-        for (int k = inj_tile[t] + tid; k < inj_tile[t + 1]; k += kThreads) {
+        for (int k = inj_tile[t] + tid; k < inj_tile[t + 1]; k += NT) {
             const int c0 = cptr[k], c1 = cptr[k + 1];
             s_u.inj[cell[k]] = (c0 << 8) | (c1 - c0);
         }
+        __syncthreads();
     }
-    if (A.extract && S.ext_tile && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t]) extract_tile<LZ>(S, t, it, s_a);
-    if (has_inj) __syncthreads();
 
     const bool snap_step = (it % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
-    const int lzc = lane + 4;
     const float *c = A.c;
+    float *col = s_col[wv];
 
     float u3v[PX];
 #pragma unroll
     for (int p = 0; p < PX; p++) {
-        const int lx = xg + p * XG;
-        const int ix = x0 + lx;
+        const int ix = xs + p;
+        const float u2c = wu[p + 4];
+        // z-stencil column through wavefront-private LDS (all 64 lanes take part)
+        col[lane + 4] = ADJ ? wv2[p + 4] * u2c : u2c;
+        if (lane < 8) col[lane < 4 ? lane : 64 + lane] = ADJ ? hv[p] * hu[p] : hu[p];
+        __builtin_amdgcn_wave_barrier();
+        const float tz = c[0] * (col[lane + 4]) + c[1] * (col[lane + 5] + col[lane + 3]) +
+                         c[2] * (col[lane + 6] + col[lane + 2]) + c[3] * (col[lane + 7] + col[lane + 1]) +
+                         c[4] * (col[lane + 8] + col[lane]);
+        __builtin_amdgcn_wave_barrier();
         u3v[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
-        const float u2c = r2[p];
-        const float *col = s_a + LZ * (lx + 4);
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
             snap_image_point(S, A, it, iz, ix, u2c,
                              (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
-        const float tz = c[0] * (col[lzc]) + c[1] * (col[lzc + 1] + col[lzc - 1]) +
-                         c[2] * (col[lzc + 2] + col[lzc - 2]) + c[3] * (col[lzc + 3] + col[lzc - 3]) +
-                         c[4] * (col[lzc + 4] + col[lzc - 4]);
-        const float tx = c[5] * (col[lzc]) + c[6] * (col[lzc + LZ] + col[lzc - LZ]) +
-                         c[7] * (col[lzc + 2 * LZ] + col[lzc - 2 * LZ]) +
-                         c[8] * (col[lzc + 3 * LZ] + col[lzc - 3 * LZ]) +
-                         c[9] * (col[lzc + 4 * LZ] + col[lzc - 4 * LZ]);
+#define WX(k) (ADJ ? wv2[p + 4 + (k)] * wu[p + 4 + (k)] : wu[p + 4 + (k)])
+        const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
+                         c[8] * (WX(3) + WX(-3)) + c[9] * (WX(4) + WX(-4));
+#undef WX
         float val = ADJ ? (tx + tz) : rv[p] * (tx + tz);
         // Injection: per-cell contributions in trace order (Injection.f90:434-440)
         if (has_inj) {
-            const int e = s_u.inj[lane + TZ * lx];
-            const GLB float *trs = gp(S.inj_traces), *ia = gp(S.inj_a), *iwz = gp(S.inj_wz), *iwx = gp(S.inj_wx);
-            const GLB int *itr = gp(S.inj_tr);
+            const int e = s_u.inj[lane + TZ * (wv * PX + p)];
+            const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
             for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
-                const float tr = trs[(size_t)itr[k] * S.nt + (it - 1)];
-                val = val + ia[k] * tr * iwz[k] * iwx[k];
+                const float4v w = co[k];   // gain, sinc z, sinc x, tile-local trace (int bits)
+                val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
             }
         }
         // Born secondary source: Imaging.f90:779-783, forward time only
@@ -407,48 +427,53 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 
     const bool top = z0 <= 0, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
     if (top || bot || left || right) {
-        __syncthreads();   // s_u.inj reads done
-#pragma unroll
-        for (int p = 0; p < PX; p++) s_u.b[lane + TZ * (xg + p * XG)] = u3v[p];
-        __syncthreads();
-        higdon_tile<TZ, TX, ADJ>(S, z0, x0, top, bot, left, right, s_a, s_u.b, u2, u1);
-#pragma unroll
-        for (int p = 0; p < PX; p++) u3v[p] = s_u.b[lane + TZ * (xg + p * XG)];
-    }
-
-    // ---- write u3 on every non-corner cell of this tile (FD region from the
-    // time update, edge bands from Higdon); track max|u3| over the window
-    // when the next step stores an FP16 snapshot.
-    const bool prev_of_snap = A.store_snap && A.snap_fp16 && ((A.dir > 0 ? it + 1 : it - 1) % A.snapi) == 0;
-    float vmax = 0.f;
-    if (iz >= 0 && iz < NZ) {
+        if (has_inj) __syncthreads();   // s_u.inj reads done before s_u.u1 is written
 #pragma unroll
         for (int p = 0; p < PX; p++) {
-            const int ix = x0 + xg + p * XG;
+            const int l = lane + TZ * (wv * PX + p);
+            s_b[l] = u3v[p];
+            s_u2[l] = wu[p + 4];
+            s_u.u1[l] = r1[p];
+        }
+        __syncthreads();
+        higdon_tile<TZ, TX>(S, z0, x0, top, bot, left, right, s_b, s_u2, s_u.u1);
+#pragma unroll
+        for (int p = 0; p < PX; p++) u3v[p] = s_b[lane + TZ * (wv * PX + p)];
+    }
+
+    // ---- write u3 on every non-corner cell of the strip (FD region from the
+    // time update, edge bands from Higdon); track max|u3| over the window when
+    // the next step stores an FP16 snapshot.
+    const bool prev_of_snap = A.store_snap && A.snap_fp16 && ((A.dir > 0 ? it + 1 : it - 1) % A.snapi) == 0;
+    float vmax = 0.f;
+    if (zarr) {
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            const int ix = xs + p;
             if (ix < 0 || ix >= NX) continue;
             const bool xband = ix >= 4 && ix <= NX - 5;
             if (!zin && !xband) continue;   // corner: never written by the reference
-            u3[(size_t)ix * LDZ + zo] = u3v[p];
+            stw(u3, (unsigned)ix * (unsigned)LDZ + (unsigned)(zoff + iz), u3v[p]);
             if (prev_of_snap && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) vmax = fmaxf(vmax, fabsf(u3v[p]));
         }
     }
     if (prev_of_snap) {
         for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-        if ((tid & 63) == 0 && vmax > 0.f) atomicMax(S.snap_max, __float_as_uint(vmax));
+        if (lane == 0 && vmax > 0.f) atomicMax(S.snap_max, __float_as_uint(vmax));
     }
 }
 
 // Kernel entry: SINGLE batches read the shot descriptor from the kernel
 // arguments (scalar loads, no dependent table lookups); batched launches
 // find their shot through the logical-tile table.
-template <int TZ, int TX, bool ADJ, bool SINGLE>
-__global__ void __launch_bounds__(kThreads) fd_step_kernel(StepArgs A) {
+template <int PX, bool ADJ, bool SINGLE>
+__global__ void __launch_bounds__(StepShape<PX>::THREADS) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
-        fd_step_body<TZ, TX, ADJ>(A, A.s0, lb);
+        fd_step_body<PX, ADJ>(A, A.s0, lb);
     } else {
-        fd_step_body<TZ, TX, ADJ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
+        fd_step_body<PX, ADJ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

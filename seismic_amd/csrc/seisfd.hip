@@ -257,27 +257,49 @@ void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc,
         return a.first < b.first;
     });
     p.inj_tile.assign(ntile + 1, 0);
-    p.inj_cell.clear(); p.inj_cptr.clear(); p.inj_a.clear(); p.inj_wz.clear(); p.inj_wx.clear(); p.inj_tr.clear();
+    p.inj_cell.clear(); p.inj_cptr.clear(); p.inj_coef.clear();
+    p.inj_tptr.assign(ntile + 1, 0);
+    p.inj_ttr.clear();
     long prev = -1;
+    int cur_tile = -1;
+    std::vector<int> local;   // trace -> tile-local index (-1: not yet used in this tile)
+    std::vector<int> used;
+    local.assign(std::max(nsrc, 1), -1);
     for (size_t i = 0; i < key.size(); i++) {
         const C &c = cs[key[i].second];
+        const int t = (int)(key[i].first / (tz * tx));
+        if (t != cur_tile) {
+            for (int u : used) local[u] = -1;
+            used.clear();
+            cur_tile = t;
+        }
         if (key[i].first != prev) {
-            const int t = (int)(key[i].first / (tz * tx));
             p.inj_cell.push_back((int)(key[i].first % (tz * tx)));
-            p.inj_cptr.push_back((int)p.inj_a.size());
+            p.inj_cptr.push_back((int)(p.inj_coef.size() / 4));
             p.inj_tile[t + 1]++;
             prev = key[i].first;
         }
-        p.inj_a.push_back(c.a);
-        p.inj_wz.push_back(c.wz);
-        p.inj_wx.push_back(c.wx);
-        p.inj_tr.push_back(c.tr);
+        if (local[c.tr] < 0) {
+            local[c.tr] = (int)used.size();
+            used.push_back(c.tr);
+            p.inj_ttr.push_back(c.tr);
+            p.inj_tptr[t + 1]++;
+            if ((int)used.size() > kMaxTileTraces)
+                throw EngineError(SFD_EUNSUPPORTED, "more than 1024 distinct injected traces in one tile");
+        }
+        int li = local[c.tr];
+        float lif;
+        memcpy(&lif, &li, 4);
+        p.inj_coef.insert(p.inj_coef.end(), {c.a, c.wz, c.wx, lif});
     }
-    p.inj_cptr.push_back((int)p.inj_a.size());
-    for (int t = 0; t < ntile; t++) p.inj_tile[t + 1] += p.inj_tile[t];
+    p.inj_cptr.push_back((int)(p.inj_coef.size() / 4));
+    for (int t = 0; t < ntile; t++) {
+        p.inj_tile[t + 1] += p.inj_tile[t];
+        p.inj_tptr[t + 1] += p.inj_tptr[t];
+    }
     for (size_t c = 0; c + 1 < p.inj_cptr.size(); c++)
         if (p.inj_cptr[c + 1] - p.inj_cptr[c] > 255) throw EngineError(SFD_EUNSUPPORTED, "more than 255 injections in one cell");
-    if (p.inj_a.size() >= (1u << 23)) throw EngineError(SFD_EUNSUPPORTED, "injection plan too large");
+    if (p.inj_coef.size() / 4 >= (1u << 23)) throw EngineError(SFD_EUNSUPPORTED, "injection plan too large");
 }
 
 // Extraction plan (Extraction.f90:71-246).
@@ -345,8 +367,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     std::vector<size_t> upn;
     struct Off {
         size_t u[4], v2, vel, hz, hx, colsrc, taper;
-        size_t inj_tile, inj_cell, inj_cptr, inj_a, inj_wz, inj_wx, inj_tr, inj_traces;
-        size_t injb_tile, injb_cell, injb_cptr, injb_a, injb_wz, injb_wx, injb_tr;
+        size_t inj_tile, inj_cell, inj_cptr, inj_coef, inj_tptr, inj_ttr, inj_traces;
+        size_t injb_tile, injb_cell, injb_cptr, injb_coef, injb_tptr, injb_ttr;
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
         size_t snap, snap_scale, snap_max, image, illum, born_m;
     };
@@ -368,19 +390,17 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         o.inj_tile = put(L, up, upn, p.inj_tile);
         o.inj_cell = put(L, up, upn, p.inj_cell);
         o.inj_cptr = put(L, up, upn, p.inj_cptr);
-        o.inj_a = put(L, up, upn, p.inj_a);
-        o.inj_wz = put(L, up, upn, p.inj_wz);
-        o.inj_wx = put(L, up, upn, p.inj_wx);
-        o.inj_tr = put(L, up, upn, p.inj_tr);
+        o.inj_coef = put(L, up, upn, p.inj_coef);
+        o.inj_tptr = put(L, up, upn, p.inj_tptr);
+        o.inj_ttr = put(L, up, upn, p.inj_ttr);
         o.inj_traces = L.take((size_t)p.nt * std::max(p.n_inj_traces, 1) * sizeof(float));
         if (need.backward_injection) {
             o.injb_tile = put(L, up, upn, p.injb_tile);
             o.injb_cell = put(L, up, upn, p.injb_cell);
             o.injb_cptr = put(L, up, upn, p.injb_cptr);
-            o.injb_a = put(L, up, upn, p.injb_a);
-            o.injb_wz = put(L, up, upn, p.injb_wz);
-            o.injb_wx = put(L, up, upn, p.injb_wx);
-            o.injb_tr = put(L, up, upn, p.injb_tr);
+            o.injb_coef = put(L, up, upn, p.injb_coef);
+            o.injb_tptr = put(L, up, upn, p.injb_tptr);
+            o.injb_ttr = put(L, up, upn, p.injb_ttr);
         }
         o.ext_tile = put(L, up, upn, p.ext_tile);
         o.ext_rec = put(L, up, upn, p.ext_rec);
@@ -426,20 +446,18 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.inj_tile = (const int *)(base + o.inj_tile);
         S.inj_cell = (const int *)(base + o.inj_cell);
         S.inj_cptr = (const int *)(base + o.inj_cptr);
-        S.inj_a = (const float *)(base + o.inj_a);
-        S.inj_wz = (const float *)(base + o.inj_wz);
-        S.inj_wx = (const float *)(base + o.inj_wx);
-        S.inj_tr = (const int *)(base + o.inj_tr);
+        S.inj_coef = (const InjCoef *)(base + o.inj_coef);
+        S.inj_tptr = (const int *)(base + o.inj_tptr);
+        S.inj_ttr = (const int *)(base + o.inj_ttr);
         S.inj_traces = (const float *)(base + o.inj_traces);
         if (need.backward_injection) {
             InjSet &I = B.injb[s];
             I.tile = (const int *)(base + o.injb_tile);
             I.cell = (const int *)(base + o.injb_cell);
             I.cptr = (const int *)(base + o.injb_cptr);
-            I.a = (const float *)(base + o.injb_a);
-            I.wz = (const float *)(base + o.injb_wz);
-            I.wx = (const float *)(base + o.injb_wx);
-            I.tr = (const int *)(base + o.injb_tr);
+            I.coef = (const InjCoef *)(base + o.injb_coef);
+            I.tptr = (const int *)(base + o.injb_tptr);
+            I.ttr = (const int *)(base + o.injb_ttr);
         }
         S.ext_tile = p.ext_tile.empty() ? nullptr : (const int *)(base + o.ext_tile);
         S.ext_rec = (const int *)(base + o.ext_rec);
@@ -514,12 +532,15 @@ StepArgs Engine::base_args(const Batch &B) const {
 
 typedef void (*StepFn)(StepArgs);
 
-static StepFn pick_kernel(int TZ, int TX, bool adj, bool single) {
-#define K(a, b)                                                                                      \
-    if (TZ == a && TX == b)                                                                          \
-        return adj ? (single ? (StepFn)fd_step_kernel<a, b, true, true> : (StepFn)fd_step_kernel<a, b, true, false>) \
-                   : (single ? (StepFn)fd_step_kernel<a, b, false, true> : (StepFn)fd_step_kernel<a, b, false, false>);
-    K(32, 32) K(64, 32) K(32, 64) K(64, 64)
+// Tiles are 64 rows x TX columns, one wavefront per 64 x PX strip.
+static int pick_px(int TZ, int TX) { return TZ != 64 ? 0 : (TX == 32 ? 4 : (TX == 64 ? 16 : 0)); }
+static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int px_override = 0) {
+    const int want = px_override ? px_override : pick_px(TZ, TX);
+#define K(px)                                                                                            \
+    if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
+        return adj ? (single ? (StepFn)fd_step_kernel<px, true, true> : (StepFn)fd_step_kernel<px, true, false>) \
+                   : (single ? (StepFn)fd_step_kernel<px, false, true> : (StepFn)fd_step_kernel<px, false, false>);
+    K(4) K(8) K(16)
 #undef K
     return nullptr;
 }
@@ -531,11 +552,13 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         A.single = 1;
         A.s0 = B.shots_h[0];
     }
-    StepFn f = pick_kernel(tz, tx, adj, A.single != 0);
+    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, strip_px);
     if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
     int grid = B.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
-    hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), 0, stream, A);
+    const int px = strip_px ? strip_px : pick_px(tz, tx);
+    const int threads = 64 * ((px >= 8 ? 4 * px : 32) / px);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
 }
 
 // Run nsteps of the lockstep time loop.  Steps are recorded into HIP graphs
@@ -759,8 +782,8 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     for (int s = 0; s < ns; s++) {
         ShotDev &S = B.shots_h[s];
         const InjSet &I = B.injb[s];
-        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_a = I.a;
-        S.inj_wz = I.wz; S.inj_wx = I.wx; S.inj_tr = I.tr;
+        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_coef = I.coef;
+        S.inj_tptr = I.tptr; S.inj_ttr = I.ttr;
         const size_t n = (size_t)shots[s].nt * shots[s].ntr;
         upload_inj_traces(B, s, traces_in + off, n);
         off += n;
@@ -934,8 +957,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     for (int s = 0; s < ns; s++) {
         ShotDev &S = B.shots_h[s];
         const InjSet &I = B.injb[s];
-        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_a = I.a;
-        S.inj_wz = I.wz; S.inj_wx = I.wx; S.inj_tr = I.tr;
+        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_coef = I.coef;
+        S.inj_tptr = I.tptr; S.inj_ttr = I.ttr;
     }
     upload_shots(B);
     StepArgs Ab = base_args(B);
@@ -1045,7 +1068,7 @@ void Engine::scatter_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img
 
 void ShotPlan::take_backward(const ShotPlan &o) {
     injb_tile = o.inj_tile; injb_cell = o.inj_cell; injb_cptr = o.inj_cptr;
-    injb_a = o.inj_a; injb_wz = o.inj_wz; injb_wx = o.inj_wx; injb_tr = o.inj_tr;
+    injb_coef = o.inj_coef; injb_tptr = o.inj_tptr; injb_ttr = o.inj_ttr;
 }
 
 void second_difference(std::vector<float> &w, float dt) {
@@ -1184,6 +1207,11 @@ int sfd_set_option(sfd_handle h, int opt, int value) {
             e.graph_chunk = value;
             break;
         case SFD_OPT_XCD_REMAP: e.xcd_remap = value != 0; break;
+        case SFD_OPT_STRIP_PX:
+            if (value != 0 && value != 4 && value != 8 && value != 16) throw EngineError(SFD_EINVAL, "strip width 4, 8 or 16");
+            if (value == 4 && !(e.tz == 64 && e.tx == 32)) throw EngineError(SFD_EINVAL, "strip width 4 needs 64x32 tiles");
+            e.strip_px = value;
+            break;
         case SFD_OPT_KERNEL_EVENTS:
             if (value < 0) throw EngineError(SFD_EINVAL, "event sampling must be >= 0");
             e.event_every = value;

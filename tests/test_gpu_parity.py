@@ -9,6 +9,7 @@ import oracle_lib as O
 from models import ricker, layered_anomaly, smooth_random
 
 sa = pytest.importorskip("seismic_amd")
+import seismic_amd.abi  # noqa: E402,F401
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
@@ -70,7 +71,7 @@ def test_forward_offgrid_all_extractions(ext):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("tile", [(64, 32), (32, 32), (64, 64), (32, 64)])
+@pytest.mark.parametrize("tile", [(64, 32, 0), (64, 32, 4), (64, 64, 0)])
 def test_tile_shapes_bitwise(tile):
     """Every compiled tile shape gives the same gather (grid sizes chosen so
     tile-origin shifts and partial tiles occur)."""
@@ -82,7 +83,8 @@ def test_tile_shapes_bitwise(tile):
     rz = np.full_like(rx, 15.0)
     gs, os_ = both_shots(25.0, 640.0, w, rz, rx)
     e = engine(nz, nx, 10.0, 10.0, 1e-3, 15, vel=vel)
-    e.set_tile(*tile)
+    e.set_tile(*tile[:2])
+    e.set_option(sa.abi.SFD_OPT_STRIP_PX, tile[2])
     gpu = e.forward([gs])[0]
     ref = O.modeling(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=15), vel, os_, extract_kind=0)
     say("tile %s rel L2 = %.3e bitwise=%s" % (tile, rel(gpu, ref), np.array_equal(gpu, ref)))
