@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--graphs", type=int, default=1, help="0: eager launches (for profilers)")
     ap.add_argument("--strip", type=int, default=0, help="columns per wavefront (0 auto, 4, 8, 16)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only the 512 MiB copy kernel (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -105,6 +107,14 @@ def main():
 
     import seismic_amd as sa
     import seismic_amd.abi  # noqa: F401
+    if args.copy_calibration:
+        import ctypes
+        ms = ctypes.c_double()
+        nbytes = 512 << 20
+        rc = sa.abi.load().sfd_copy_calibration(nbytes, 5, ctypes.byref(ms))
+        print(json.dumps({"copy_bytes": nbytes, "launches": 6, "avg_ms": ms.value, "rc": rc,
+                          "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9 if ms.value > 0 else None}))
+        sys.exit(0 if rc == 0 else 1)
     info = sa.device_info()
     log("device", info["name"], "CUs", info["cus"])
     true, mig, w, rz, rx = build_inputs()

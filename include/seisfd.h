@@ -169,19 +169,28 @@ int  sfd_last_timing(sfd_handle h, double *ms, double *pts);
 int  sfd_set_tile(sfd_handle h, int tile_z, int tile_x);
 
 /* Engine options (no reference counterpart: launch policy of the HIP path). */
-#define SFD_OPT_GRAPHS       1  /* 1: replay the time loop as HIP graphs (default) */
-#define SFD_OPT_GRAPH_CHUNK  2  /* time steps per graph (default 199)            */
+#define SFD_OPT_GRAPHS       1  /* 1: replay blocks of time steps as one HIP graph (default) */
+#define SFD_OPT_GRAPH_CHUNK  2  /* time steps per block (default 200, rounded up to a multiple of lcm(4, snapi)) */
 #define SFD_OPT_XCD_REMAP    3  /* 1: XCD-contiguous tile order (default)          */
-#define SFD_OPT_KERNEL_EVENTS 4 /* 1: time the first step of every graph chunk with HIP events (default) */
+#define SFD_OPT_KERNEL_EVENTS 4 /* 1: time every block of steps with HIP events (default) */
 #define SFD_OPT_STRIP_PX     5  /* columns per wavefront in the step kernel: 0 auto (4 for 64x32 tiles), 4, 8, 16 */
 int  sfd_set_option(sfd_handle h, int opt, int value);
 
 /*
  * Step-kernel durations of the last call, from HIP events recorded on the
- * engine's stream around one launch per graph chunk: avg_us[0] forward-time pass,
- * avg_us[1] reverse-time pass; nsamples likewise.
+ * engine's stream around every block of back-to-back step launches (block
+ * time / steps in the block): avg_us[0] forward-time pass, avg_us[1]
+ * reverse-time pass; nsamples = steps timed.
  */
 int  sfd_kernel_stats(sfd_handle h, double *avg_us, int *nsamples);
+
+/*
+ * Measurement helper (no reference counterpart): copies nbytes between two
+ * device buffers with a plain streaming kernel (copy_calib_kernel) `reps`
+ * times and returns the average kernel time.  Used to calibrate hardware
+ * byte counters (FETCH_SIZE/WRITE_SIZE) against a known traffic volume.
+ */
+int  sfd_copy_calibration(size_t nbytes, int reps, double *avg_ms);
 
 /* ---- L-BFGS (libs/libinv/lbfgs.f:49-558 LBFGS + MCSRCH + MCSTEP), reverse
  *      communication exactly as the reference's IFLAG protocol.  Vectors

@@ -70,7 +70,7 @@ EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_born", "sfd_lsrtm",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_set_option", "sfd_kernel_stats",
-    "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
+    "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
 ]
 
 _lib = None
@@ -105,6 +105,7 @@ def load():
     lib.sfd_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.sfd_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.sfd_kernel_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int)]
+    lib.sfd_copy_calibration.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]
     lib.sfd_lbfgs_create.argtypes = [P(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     lib.sfd_lbfgs_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,

@@ -74,10 +74,11 @@ struct Engine {
     bool autotune_pending = false;
     int xcd_remap = 1;
     int strip_px = 0;        // columns per wavefront (0: tile width / 4)
-    int graph_chunk = 199;   // not a multiple of 4: sampled steps rotate through snapshot phases
+    int graph_chunk = 200;   // steps per replayed block (rounded up to a multiple of lcm(4, snapi))
     bool use_graphs = true;
     int event_every = 1;
-    std::vector<hipEvent_t> kev;          // start/stop of the sampled launch
+    std::vector<hipEvent_t> kev;          // start/stop of every block of steps
+    DevBuf nbase;                         // device-held first step of the running block
     double kstat_us[2] = {0, 0};
     int kstat_n[2] = {0, 0};
     double timing[4] = {0, 0, 0, 0};
@@ -108,7 +109,7 @@ struct Engine {
     void zero_wavefields(Batch &B, std::vector<ShotPlan> &plans);
     sfd::StepArgs base_args(const Batch &B) const;
     void launch_step(const Batch &B, sfd::StepArgs A, bool adj);
-    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, bool fp16_scale);
+    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, int nfree, bool fp16_scale);
     void reset_kstats();
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);

@@ -105,7 +105,8 @@ struct StepArgs {
     const int *blk_shot;    // logical tile -> shot
     int nblocks;            // logical tiles (grid may be padded to a multiple of 8)
     int xcd_remap;          // map block ids so one XCD gets contiguous tiles
-    int n;                  // step counter from 0
+    int n;                  // step counter from 0 (plus *nbase when nbase is set)
+    const int *nbase;       // device-held step base of a replayed graph, or null
     int rot;                // n % 4
     int dir;                // +1 forward time, -1 reverse time
     int snapi;
@@ -298,8 +299,17 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     } s_u;
     __shared__ float s_trv[kMaxTileTraces];   // injection tiles: this step's trace samples
 
-    const int it = A.dir > 0 ? A.n + 1 : S.nt - A.n;
-    if (it < 1 || it > S.nt) return;   // this shot has finished (ragged batch)
+    // Reference time index.  `its` is the part fixed at launch; a replayed
+    // graph adds a device-held base (a multiple of 4 and of snapi, so `its`
+    // carries the residues the buffer rotation and snapshot tests need).
+    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    int it = its;
+    if (A.nbase) {
+        const int b = *gp(A.nbase);
+        it = A.dir > 0 ? its + b : its - b;
+    } else if (it < 1 || it > S.nt) {
+        return;   // this shot has finished (ragged batch)
+    }
     const int NZ = S.NZ, NX = S.NX, LDZ = S.LDZ, zoff = S.zoff;
     const int t = lb - S.tile0;
     const int tzi = t % S.ntz, txi = t / S.ntz;
@@ -376,7 +386,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         __syncthreads();
     }
 
-    const bool snap_step = (it % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
+    const bool snap_step = (its % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
     const float *c = A.c;
     float *col = s_col[wv];
@@ -415,7 +425,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             }
         }
         // Born secondary source: Imaging.f90:779-783, forward time only
-        if (A.imaging == IMG_BORN && (it % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
+        if (A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
             const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq);
@@ -444,7 +454,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // ---- write u3 on every non-corner cell of the strip (FD region from the
     // time update, edge bands from Higdon); track max|u3| over the window when
     // the next step stores an FP16 snapshot.
-    const bool prev_of_snap = A.store_snap && A.snap_fp16 && ((A.dir > 0 ? it + 1 : it - 1) % A.snapi) == 0;
+    const bool prev_of_snap = A.store_snap && A.snap_fp16 && ((A.dir > 0 ? its + 1 : its - 1) % A.snapi) == 0;
     float vmax = 0.f;
     if (zarr) {
 #pragma unroll
@@ -475,6 +485,12 @@ __global__ void __launch_bounds__(StepShape<PX>::THREADS) fd_step_kernel(StepArg
     } else {
         fd_step_body<PX, ADJ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
+}
+
+// Plain streaming copy used to calibrate hardware byte counters.
+__global__ void copy_calib_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n4) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // Edge-replicating window + pad of the full model for one shot
@@ -539,10 +555,15 @@ __global__ void higdon_tables_kernel(const float *__restrict__ vel, int NZ, int 
     }
 }
 
+// Step base of replayed graphs (one thread; plain vector stores).
+__global__ void set_base_kernel(int *nbase, int v) { *nbase = v; }
+__global__ void advance_base_kernel(int *nbase, int k) { *nbase = *nbase + k; }
+
 // Publish each shot's running max as the scale of snapshot idx and reset it.
-__global__ void snap_scale_kernel(const ShotDev *shots, int nshots, int idx) {
+__global__ void snap_scale_kernel(const ShotDev *shots, int nshots, int idx, const int *nbase, int snapi) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nshots) return;
+    if (nbase) idx += *nbase / snapi;
     const ShotDev &S = shots[s];
     if (idx >= 1 && idx <= S.ntsnap) S.snap_scale[idx - 1] = __uint_as_float(*S.snap_max);
     *S.snap_max = 0u;

@@ -561,68 +561,78 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
 }
 
-// Run nsteps of the lockstep time loop.  Steps are recorded into HIP graphs
-// of `graph_chunk` launches and replayed (the per-step arguments differ only
-// in n).  With sampling on, the first step of every chunk is launched on its
-// own between two HIP events on the engine stream; their elapsed times give
-// the step kernel's duration.
-void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, bool fp16_scale) {
-    const int chunk = std::max(1, graph_chunk);
+// Run nsteps of the lockstep time loop.  Steps n < nfree (every shot of the
+// batch still running) go in blocks of K steps, K = graph_chunk rounded up to
+// a multiple of lcm(4, snapi) so that buffer rotation and snapshot phases
+// repeat from block to block: the K step launches plus one base-advance launch
+// are captured once into a HIP graph and replayed, the kernels reading the
+// block's first step from device memory (d_nbase).  Remaining steps launch one
+// by one with a fixed n.  HIP events on the engine stream bracket every block;
+// block time / K is the per-launch step time reported by kernel_stats.
+void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfree, bool fp16_scale) {
     const int dir = A.dir > 0 ? 0 : 1;
-    if (event_every > 0 && kev.empty()) {
-        kev.resize(2);
-        for (auto &e : kev) HIPCHK(hipEventCreate(&e));
-    }
-    auto one = [&](int n) {
+    int period = 4;
+    while (period % g.snapi) period += 4;
+    const int K = std::max(period, (std::max(1, graph_chunk) + period - 1) / period * period);
+    const int nblk = std::min(nsteps, nfree) / K;
+    nbase.reserve(256);
+    int *d_nb = (int *)nbase.p;
+    auto one = [&](int n, const int *nb) {
         A.n = n;
         A.rot = n & 3;
+        A.nbase = nb;
         launch_step(B, A, adj);
         if (fp16_scale) {
             // forward time: after step it=n+1, publish the max for snapshot (n+2)/snapi
             const int itn = n + 2;
             if (itn % g.snapi == 0)
                 hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
-                                   itn / g.snapi);
+                                   itn / g.snapi, nb, g.snapi);
         }
     };
-    for (int n0 = 0; n0 < nsteps; n0 += chunk) {
-        const int n1 = std::min(nsteps, n0 + chunk);
-        int n = n0;
-        const bool sample = event_every > 0;
-        if (sample) {
-            HIPCHK(hipEventRecord(kev[0], stream));
-            A.n = n;
-            A.rot = n & 3;
-            launch_step(B, A, adj);
-            HIPCHK(hipEventRecord(kev[1], stream));
-            if (fp16_scale && (n + 2) % g.snapi == 0)
-                hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
-                                   (n + 2) / g.snapi);
-            n++;
-        }
-        if (use_graphs && n1 - n > 1) {
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
+    const bool timed = event_every > 0;
+    if (timed && (int)kev.size() < 2 * nblk) {
+        const size_t old = kev.size();
+        kev.resize(2 * nblk);
+        for (size_t i = old; i < kev.size(); i++) HIPCHK(hipEventCreate(&kev[i]));
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (nblk > 0) {
+        hipLaunchKernelGGL(set_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, 0);
+        if (use_graphs) {
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            for (; n < n1; n++) one(n);
+            for (int n = 0; n < K; n++) one(n, d_nb);
+            hipLaunchKernelGGL(advance_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, K);
             HIPCHK(hipStreamEndCapture(stream, &graph));
             HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            HIPCHK(hipGraphLaunch(exec, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-        } else {
-            for (; n < n1; n++) one(n);
-        }
-        if (sample) {
-            float ms = 0;
-            HIPCHK(hipEventSynchronize(kev[1]));
-            HIPCHK(hipEventElapsedTime(&ms, kev[0], kev[1]));
-            kstat_us[dir] += 1000.0 * ms;
-            kstat_n[dir]++;
         }
     }
+    for (int b = 0; b < nblk; b++) {
+        if (timed) HIPCHK(hipEventRecord(kev[2 * b], stream));
+        if (exec) {
+            HIPCHK(hipGraphLaunch(exec, stream));
+        } else {
+            for (int n = 0; n < K; n++) one(b * K + n, nullptr);
+        }
+        if (timed) HIPCHK(hipEventRecord(kev[2 * b + 1], stream));
+    }
+    for (int n = nblk * K; n < nsteps; n++) one(n, nullptr);
     HIPCHK(hipGetLastError());
+    if (timed && nblk > 0) {
+        HIPCHK(hipEventSynchronize(kev[2 * nblk - 1]));
+        for (int b = 0; b < nblk; b++) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, kev[2 * b], kev[2 * b + 1]));
+            kstat_us[dir] += 1000.0 * ms;
+            kstat_n[dir] += K;
+        }
+    }
+    if (exec) {
+        HIPCHK(hipStreamSynchronize(stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    }
 }
 
 void Engine::reset_kstats() {
@@ -633,6 +643,11 @@ void Engine::reset_kstats() {
 static int max_nt(const std::vector<ShotPlan> &plans) {
     int m = 0;
     for (const ShotPlan &p : plans) m = std::max(m, p.nt);
+    return m;
+}
+static int min_nt(const std::vector<ShotPlan> &plans) {
+    int m = plans.empty() ? 0 : plans[0].nt;
+    for (const ShotPlan &p : plans) m = std::min(m, p.nt);
     return m;
 }
 
@@ -677,7 +692,7 @@ void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extra
     A.extract = 1;
     auto t1 = std::chrono::steady_clock::now();
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(fwd_batch, A, false, max_nt(plans), false);
+    run_loop(fwd_batch, A, false, max_nt(plans), min_nt(plans), false);
     HIPCHK(hipEventRecord(ev1, stream));
     HIPCHK(hipEventSynchronize(ev1));
     size_t off = 0;
@@ -773,7 +788,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     A.store_snap = 1;
     A.snap_fp16 = need.snap_fp16;
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), need.snap_fp16);
+    run_loop(B, A, false, max_nt(plans), min_nt(plans), need.snap_fp16);
     HIPCHK(hipEventRecord(ev1, stream));
     // backward
     zero_wavefields(B, plans);
@@ -795,7 +810,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     Ab.inject = 1;
     Ab.snap_fp16 = need.snap_fp16;
     Ab.imaging = lsrtm ? IMG_LSRTM : IMG_RTM;
-    run_loop(B, Ab, true, max_nt(plans), false);
+    run_loop(B, Ab, true, max_nt(plans), min_nt(plans), false);
     HIPCHK(hipEventRecord(ev2, stream));
     HIPCHK(hipEventSynchronize(ev2));
     float ms1 = 0, ms2 = 0;
@@ -842,14 +857,14 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
     A.inject = 1;
     A.store_snap = 1;
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), false);
+    run_loop(B, A, false, max_nt(plans), min_nt(plans), false);
     HIPCHK(hipEventRecord(ev1, stream));
     zero_wavefields(B, plans);
     StepArgs A2 = base_args(B);
     A2.dir = 1;
     A2.extract = 1;
     A2.imaging = IMG_BORN;
-    run_loop(B, A2, false, max_nt(plans), false);
+    run_loop(B, A2, false, max_nt(plans), min_nt(plans), false);
     HIPCHK(hipEventRecord(ev2, stream));
     HIPCHK(hipEventSynchronize(ev2));
     size_t off = 0;
@@ -945,7 +960,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     A.store_snap = 1;
     A.snap_fp16 = need.snap_fp16;
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), need.snap_fp16);
+    run_loop(B, A, false, max_nt(plans), min_nt(plans), need.snap_fp16);
     // residual, FP64 misfit per trace, adjoint source into the injection buffers
     hipLaunchKernelGGL(residual_kernel, dim3(ntraces), dim3(256), 0, stream, (const TraceRef *)(fb + o_refs),
                        B.shots, (const float *)(fb + o_dobs), mute ? (const float *)(fb + o_mute) : nullptr,
@@ -966,7 +981,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     Ab.inject = 1;
     Ab.snap_fp16 = need.snap_fp16;
     Ab.imaging = IMG_FWI;
-    run_loop(B, Ab, false, max_nt(plans), false);
+    run_loop(B, Ab, false, max_nt(plans), min_nt(plans), false);
     HIPCHK(hipEventRecord(ev2, stream));
 
     // mod_copy_image into the full model, then the post-processing chain
@@ -1196,6 +1211,34 @@ int sfd_set_tile(sfd_handle h, int tile_z, int tile_x) {
         e.tx = tile_x;
         e.autotune_pending = false;
     });
+}
+
+int sfd_copy_calibration(size_t nbytes, int reps, double *avg_ms) {
+    if (nbytes < 16 || reps < 1 || !avg_ms) return SFD_EINVAL;
+    const size_t n4 = nbytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SFD_OK;
+    if (hipMalloc(&a, n4 * 16) != hipSuccess || hipMalloc(&b, n4 * 16) != hipSuccess) rc = SFD_ENOMEM;
+    if (rc == SFD_OK && (hipMemset(a, 0, n4 * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                         hipEventCreate(&e1) != hipSuccess))
+        rc = SFD_EHIP;
+    if (rc == SFD_OK) {
+        const int grid = 4096;
+        hipLaunchKernelGGL(copy_calib_kernel, dim3(grid), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n4);
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(copy_calib_kernel, dim3(grid), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n4);
+        (void)hipEventRecord(e1, 0);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SFD_EHIP;
+        *avg_ms = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int sfd_set_option(sfd_handle h, int opt, int value) {
