@@ -6,7 +6,7 @@ Workload (one "step" = one complete RTM of the shot):
   (1500-4500 m/s, 12 layers, dips +/-15 deg, 3 normal faults), migration
   velocity = that model Gaussian-smoothed with sigma 60 m; 20 Hz Ricker at the
   surface mid-x, 2301 surface receivers, dt = 0.4 ms, nt = 7500; snapshots
-  every 4 steps (FP32 by default, --snap fp16 for per-snapshot max-scaled FP16).
+  every 4 steps (FP32 by default, --snap fp16 for FP16 scaled by max|S| per snapshot tile).
   Observed data are modeled from the true model before timing starts.
 
 Metric: FD point-updates per second (Gpts/s) = sum over both propagations of

@@ -34,7 +34,7 @@ extern "C" {
 
 /* Snapshot storage for RTM / FWI source wavefields. */
 #define SFD_SNAP_FP32     0   /* reference behaviour: FP32 copies           */
-#define SFD_SNAP_FP16S    1   /* FP16 with one max|S| scale per snapshot     */
+#define SFD_SNAP_FP16S    1   /* FP16, one max|S| scale per snapshot and step-kernel tile */
 
 /* Receiver extraction semantics (libs/libwave/Extraction.f90). */
 #define SFD_EXTRACT_SINC_V2    0  /* Extraction_array_sinc: gain v^2 (:109-134)   */

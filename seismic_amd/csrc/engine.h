@@ -109,7 +109,7 @@ struct Engine {
     void zero_wavefields(Batch &B, std::vector<ShotPlan> &plans);
     sfd::StepArgs base_args(const Batch &B) const;
     void launch_step(const Batch &B, sfd::StepArgs A, bool adj);
-    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, int nfree, bool fp16_scale);
+    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, int nfree);
     void reset_kstats();
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);

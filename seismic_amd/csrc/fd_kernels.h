@@ -89,8 +89,7 @@ struct ShotDev {
     int ext_kind;
     // snapshots (window interior, nz*nxw per snapshot)
     void *snap;             // float* or __half*
-    float *snap_scale;      // per-snapshot max|S| (FP16 mode)
-    unsigned *snap_max;     // running max bits for the next snapshot (FP16 mode)
+    float *snap_scale;      // FP16 mode: max|S| over each tile's window cells, ntz*ntx per snapshot
     // imaging / Born
     float *image, *illum;   // nz*nxw window accumulators
     const float *born_m;    // nz*nxw reflectivity
@@ -136,10 +135,10 @@ __device__ __forceinline__ int logical_block(const StepArgs &A) {
 }
 
 // Snapshot idx (1-based) at window offset wq, decoded to FP32.
-__device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq) {
+__device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq, int t) {
     if (fp16) {
         const unsigned short h = gp((const unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq];
-        return __half2float(__ushort_as_half(h)) * gp(S.snap_scale)[idx - 1];
+        return __half2float(__ushort_as_half(h)) * gp(S.snap_scale)[(size_t)(idx - 1) * (S.ntz * S.ntx) + t];
     }
     return gp((const float *)S.snap)[(size_t)(idx - 1) * npix + wq];
 }
@@ -170,22 +169,20 @@ __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, co
 // Snapshot store and imaging condition at one interior point (on u2, before
 // the stencil: Extraction.f90:595, Imaging.f90:265 / :365 / :738).
 __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
-                                                 float u2c, float vel) {
+                                                 float u2c, float vel, int t, float inv16) {
     const int wz = iz - S.i0, wx = ix - S.i0;
     const size_t wq = (size_t)wx * S.nz + wz;
     const size_t npix = (size_t)S.nz * S.nxw;
     const int idx = it / A.snapi;
     if (A.store_snap) {
         if (A.snap_fp16) {
-            const float sc = gp(S.snap_scale)[idx - 1];
-            const float inv = sc > 0.f ? 1.0f / sc : 0.f;
-            gp((unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq] = __half_as_ushort(__float2half_rn(u2c * inv));
+            gp((unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq] = __half_as_ushort(__float2half_rn(u2c * inv16));
         } else {
             gp((float *)S.snap)[(size_t)(idx - 1) * npix + wq] = u2c;
         }
     }
     if (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) {
-        const float Sv = load_snap(S, A.snap_fp16, idx, npix, wq);
+        const float Sv = load_snap(S, A.snap_fp16, idx, npix, wq, t);
         const float tp = gp(S.taper)[wx];
         GLB float *imp = gp(S.image), *ilp = gp(S.illum);
         float img = imp[wq];
@@ -194,8 +191,8 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
         } else if (A.imaging == IMG_FWI) {
             const int im1 = idx - 1 > 1 ? idx - 1 : 1;
             const int ip1 = idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap;
-            const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq);
-            const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq);
+            const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq, t);
+            const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq, t);
             const float tmp = (Sm - 2.f * Sv + Sp) / (S.dt2snap * (vel * vel * vel));
             img = img + u2c * tmp * tp;
         } else {
@@ -388,6 +385,26 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 
     const bool snap_step = (its % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
+    // FP16 snapshots carry one scale per tile: max|u2| over the tile's window
+    // cells at this step, so values are stored as u2/max in [-1, 1].
+    float inv16 = 0.f;
+    if (snap_step && A.store_snap && A.snap_fp16) {
+        float m = 0.f;
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            const int ix = xs + p;
+            if (interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) m = fmaxf(m, fabsf(wu[p + 4]));
+        }
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        __shared__ float s_red[NW];
+        if (lane == 0) s_red[wv] = m;
+        __syncthreads();
+        m = s_red[0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) m = fmaxf(m, s_red[w]);
+        if (tid == 0) gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * (S.ntz * S.ntx) + t] = m;
+        inv16 = m > 0.f ? 1.0f / m : 0.f;
+    }
     const float *c = A.c;
     float *col = s_col[wv];
 
@@ -408,7 +425,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
             snap_image_point(S, A, it, iz, ix, u2c,
-                             (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f);
+                             (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f,
+                             t, inv16);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
 #define WX(k) (ADJ ? wv2[p + 4 + (k)] * wu[p + 4 + (k)] : wu[p + 4 + (k)])
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
@@ -428,7 +446,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
-            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq);
+            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t);
             val = val + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
@@ -452,10 +470,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     }
 
     // ---- write u3 on every non-corner cell of the strip (FD region from the
-    // time update, edge bands from Higdon); track max|u3| over the window when
-    // the next step stores an FP16 snapshot.
-    const bool prev_of_snap = A.store_snap && A.snap_fp16 && ((A.dir > 0 ? its + 1 : its - 1) % A.snapi) == 0;
-    float vmax = 0.f;
+    // time update, edge bands from Higdon).
     if (zarr) {
 #pragma unroll
         for (int p = 0; p < PX; p++) {
@@ -464,12 +479,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             const bool xband = ix >= 4 && ix <= NX - 5;
             if (!zin && !xband) continue;   // corner: never written by the reference
             stw(u3, (unsigned)ix * (unsigned)LDZ + (unsigned)(zoff + iz), u3v[p]);
-            if (prev_of_snap && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) vmax = fmaxf(vmax, fabsf(u3v[p]));
         }
-    }
-    if (prev_of_snap) {
-        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-        if (lane == 0 && vmax > 0.f) atomicMax(S.snap_max, __float_as_uint(vmax));
     }
 }
 
@@ -559,15 +569,6 @@ __global__ void higdon_tables_kernel(const float *__restrict__ vel, int NZ, int 
 __global__ void set_base_kernel(int *nbase, int v) { *nbase = v; }
 __global__ void advance_base_kernel(int *nbase, int k) { *nbase = *nbase + k; }
 
-// Publish each shot's running max as the scale of snapshot idx and reset it.
-__global__ void snap_scale_kernel(const ShotDev *shots, int nshots, int idx, const int *nbase, int snapi) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nshots) return;
-    if (nbase) idx += *nbase / snapi;
-    const ShotDev &S = shots[s];
-    if (idx >= 1 && idx <= S.ntsnap) S.snap_scale[idx - 1] = __uint_as_float(*S.snap_max);
-    *S.snap_max = 0u;
-}
 
 // mod_copy_image (ExtractPadModel.f90:662-695): nearest-index scatter-add of
 // every shot's window image into the full model.  One thread per model cell;

@@ -370,7 +370,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t inj_tile, inj_cell, inj_cptr, inj_coef, inj_tptr, inj_ttr, inj_traces;
         size_t injb_tile, injb_cell, injb_cptr, injb_coef, injb_tptr, injb_ttr;
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
-        size_t snap, snap_scale, snap_max, image, illum, born_m;
+        size_t snap, snap_scale, image, illum, born_m;
     };
     std::vector<Off> off(ns);
     int tile0 = 0;
@@ -409,8 +409,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         o.traces = L.take((size_t)p.nt * p.ntr * sizeof(float));
         const size_t npix = (size_t)g.nz * p.nxw;
         o.snap = L.take(need.snapshots ? (size_t)std::max(p.ntsnap, 1) * npix * (need.snap_fp16 ? 2 : 4) : 16);
-        o.snap_scale = L.take((size_t)std::max(p.ntsnap, 1) * sizeof(float));
-        o.snap_max = L.take(sizeof(unsigned));
+        o.snap_scale = L.take(need.snap_fp16 ? (size_t)std::max(p.ntsnap, 1) * p.ntz * p.ntx * sizeof(float) : 16);
         o.image = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.illum = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.born_m = L.take(need.born ? npix * sizeof(float) : 16);
@@ -467,7 +466,6 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.ext_kind = p.ext_kind;
         S.snap = base + o.snap;
         S.snap_scale = need.snap_fp16 ? (float *)(base + o.snap_scale) : nullptr;
-        S.snap_max = (unsigned *)(base + o.snap_max);
         S.image = (float *)(base + o.image);
         S.illum = (float *)(base + o.illum);
         S.born_m = (const float *)(base + o.born_m);
@@ -569,7 +567,7 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 // block's first step from device memory (d_nbase).  Remaining steps launch one
 // by one with a fixed n.  HIP events on the engine stream bracket every block;
 // block time / K is the per-launch step time reported by kernel_stats.
-void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfree, bool fp16_scale) {
+void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfree) {
     const int dir = A.dir > 0 ? 0 : 1;
     int period = 4;
     while (period % g.snapi) period += 4;
@@ -582,13 +580,6 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfre
         A.rot = n & 3;
         A.nbase = nb;
         launch_step(B, A, adj);
-        if (fp16_scale) {
-            // forward time: after step it=n+1, publish the max for snapshot (n+2)/snapi
-            const int itn = n + 2;
-            if (itn % g.snapi == 0)
-                hipLaunchKernelGGL(snap_scale_kernel, dim3((B.ns + 63) / 64), dim3(64), 0, stream, B.shots, B.ns,
-                                   itn / g.snapi, nb, g.snapi);
-        }
     };
     const bool timed = event_every > 0;
     if (timed && (int)kev.size() < 2 * nblk) {
@@ -692,7 +683,7 @@ void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extra
     A.extract = 1;
     auto t1 = std::chrono::steady_clock::now();
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(fwd_batch, A, false, max_nt(plans), min_nt(plans), false);
+    run_loop(fwd_batch, A, false, max_nt(plans), min_nt(plans));
     HIPCHK(hipEventRecord(ev1, stream));
     HIPCHK(hipEventSynchronize(ev1));
     size_t off = 0;
@@ -788,7 +779,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     A.store_snap = 1;
     A.snap_fp16 = need.snap_fp16;
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), min_nt(plans), need.snap_fp16);
+    run_loop(B, A, false, max_nt(plans), min_nt(plans));
     HIPCHK(hipEventRecord(ev1, stream));
     // backward
     zero_wavefields(B, plans);
@@ -810,7 +801,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     Ab.inject = 1;
     Ab.snap_fp16 = need.snap_fp16;
     Ab.imaging = lsrtm ? IMG_LSRTM : IMG_RTM;
-    run_loop(B, Ab, true, max_nt(plans), min_nt(plans), false);
+    run_loop(B, Ab, true, max_nt(plans), min_nt(plans));
     HIPCHK(hipEventRecord(ev2, stream));
     HIPCHK(hipEventSynchronize(ev2));
     float ms1 = 0, ms2 = 0;
@@ -857,14 +848,14 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
     A.inject = 1;
     A.store_snap = 1;
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), min_nt(plans), false);
+    run_loop(B, A, false, max_nt(plans), min_nt(plans));
     HIPCHK(hipEventRecord(ev1, stream));
     zero_wavefields(B, plans);
     StepArgs A2 = base_args(B);
     A2.dir = 1;
     A2.extract = 1;
     A2.imaging = IMG_BORN;
-    run_loop(B, A2, false, max_nt(plans), min_nt(plans), false);
+    run_loop(B, A2, false, max_nt(plans), min_nt(plans));
     HIPCHK(hipEventRecord(ev2, stream));
     HIPCHK(hipEventSynchronize(ev2));
     size_t off = 0;
@@ -960,7 +951,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     A.store_snap = 1;
     A.snap_fp16 = need.snap_fp16;
     HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), min_nt(plans), need.snap_fp16);
+    run_loop(B, A, false, max_nt(plans), min_nt(plans));
     // residual, FP64 misfit per trace, adjoint source into the injection buffers
     hipLaunchKernelGGL(residual_kernel, dim3(ntraces), dim3(256), 0, stream, (const TraceRef *)(fb + o_refs),
                        B.shots, (const float *)(fb + o_dobs), mute ? (const float *)(fb + o_mute) : nullptr,
@@ -981,7 +972,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     Ab.inject = 1;
     Ab.snap_fp16 = need.snap_fp16;
     Ab.imaging = IMG_FWI;
-    run_loop(B, Ab, false, max_nt(plans), min_nt(plans), false);
+    run_loop(B, Ab, false, max_nt(plans), min_nt(plans));
     HIPCHK(hipEventRecord(ev2, stream));
 
     // mod_copy_image into the full model, then the post-processing chain
