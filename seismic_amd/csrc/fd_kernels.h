@@ -25,6 +25,7 @@ namespace sfd {
 
 constexpr int kThreads = 256;
 constexpr int kMaxTileTraces = 1024;   // distinct traces one tile may inject
+constexpr int kMaxTileCoef = 2048;     // injection contributions of one tile staged in LDS
 
 // One injection contribution: gain (v^2 or 1/(dz dx)), sinc weights, and the
 // tile-local index of the trace it injects.
@@ -168,8 +169,11 @@ __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, co
 
 // Snapshot store and imaging condition at one interior point (on u2, before
 // the stencil: Extraction.f90:595, Imaging.f90:265 / :365 / :738).
+// Sv, img0, ill0, tp: the source snapshot, image, illumination and taper at
+// this point, loaded ahead by the caller (imaging modes only).
 __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
-                                                 float u2c, float vel, int t, float inv16) {
+                                                 float u2c, float vel, int t, float inv16, float Sv, float img0,
+                                                 float ill0, float tp) {
     const int wz = iz - S.i0, wx = ix - S.i0;
     const size_t wq = (size_t)wx * S.nz + wz;
     const size_t npix = (size_t)S.nz * S.nxw;
@@ -182,10 +186,8 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
         }
     }
     if (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) {
-        const float Sv = load_snap(S, A.snap_fp16, idx, npix, wq, t);
-        const float tp = gp(S.taper)[wx];
         GLB float *imp = gp(S.image), *ilp = gp(S.illum);
-        float img = imp[wq];
+        float img = img0;
         if (A.imaging == IMG_RTM) {
             img = img + u2c * Sv * tp;
         } else if (A.imaging == IMG_FWI) {
@@ -199,7 +201,7 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
             img = img + 2.f * tp * u2c * Sv / vel;
         }
         imp[wq] = img;
-        ilp[wq] = ilp[wq] + Sv * Sv * tp;
+        ilp[wq] = ill0 + Sv * Sv * tp;
     }
 }
 
@@ -295,6 +297,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         float u1[TX * TZ];              // edge tiles: u1
     } s_u;
     __shared__ float s_trv[kMaxTileTraces];   // injection tiles: this step's trace samples
+    __shared__ float4v s_coef[kMaxTileCoef];  // injection tiles: the tile's contributions
 
     // Reference time index.  `its` is the part fixed at launch; a replayed
     // graph adds a device-held base (a multiple of 4 and of snapi, so `its`
@@ -363,28 +366,57 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     }
     (void)hok;
 
+    // ---- imaging steps: the source snapshot, image, illumination and taper
+    // of this lane's cells, issued with the streaming loads (clamped window
+    // index; only window cells use them) so the column loop does not wait.
+    const bool snap_step = (its % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
+    const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
+    const bool img_step = snap_step && (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM);
+    float pS[PX], pI[PX], pL[PX], pT[PX];
+    if (img_step) {
+        const int wz = iz - S.i0 < 0 ? 0 : (iz - S.i0 >= S.nz ? S.nz - 1 : iz - S.i0);
+        const size_t npix = (size_t)S.nz * S.nxw;
+        const int idx = it / A.snapi;
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            int wx = xs + p - S.i0;
+            wx = wx < 0 ? 0 : (wx >= S.nxw ? S.nxw - 1 : wx);
+            const size_t wq = (size_t)wx * S.nz + wz;
+            pS[p] = load_snap(S, A.snap_fp16, idx, npix, wq, t);
+            pI[p] = gp(S.image)[wq];
+            pL[p] = gp(S.illum)[wq];
+            pT[p] = gp(S.taper)[wx];
+        }
+    }
+
     // ---- injection tiles (workgroup-uniform branch): per-cell contribution
     // table, and this step's sample of every trace the tile injects, staged
     // in LDS so a contribution costs one packed load and one LDS read.
     const GLB int *inj_tile = gp(S.inj_tile);
     const int has_inj = A.inject && (inj_tile[t + 1] > inj_tile[t]);
+    bool coef_lds = false;
     if (has_inj) {
         for (int l = tid; l < TX * TZ; l += NT) s_u.inj[l] = 0;
         const GLB int *ttr = gp(S.inj_ttr);
         const GLB float *trs = gp(S.inj_traces);
         for (int j = gp(S.inj_tptr)[t] + tid, j0 = gp(S.inj_tptr)[t]; j < gp(S.inj_tptr)[t + 1]; j += NT)
             s_trv[j - j0] = trs[(size_t)ttr[j] * S.nt + (it - 1)];
-        __syncthreads();
         const GLB int *cptr = gp(S.inj_cptr), *cell = gp(S.inj_cell);
+        const int cb = cptr[inj_tile[t]], ce = cptr[inj_tile[t + 1]];
+        coef_lds = ce - cb <= kMaxTileCoef;
+        if (coef_lds) {
+            const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
+            for (int j = cb + tid; j < ce; j += NT) s_coef[j - cb] = co[j];
+        }
+        __syncthreads();
+        const int base = coef_lds ? cb : 0;
         for (int k = inj_tile[t] + tid; k < inj_tile[t + 1]; k += NT) {
             const int c0 = cptr[k], c1 = cptr[k + 1];
-            s_u.inj[cell[k]] = (c0 << 8) | (c1 - c0);
+            s_u.inj[cell[k]] = ((c0 - base) << 8) | (c1 - c0);
         }
         __syncthreads();
     }
 
-    const bool snap_step = (its % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
-    const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
     // FP16 snapshots carry one scale per tile: max|u2| over the tile's window
     // cells at this step, so values are stored as u2/max in [-1, 1].
     float inv16 = 0.f;
@@ -426,7 +458,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
             snap_image_point(S, A, it, iz, ix, u2c,
                              (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f,
-                             t, inv16);
+                             t, inv16, pS[p], pI[p], pL[p], pT[p]);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
 #define WX(k) (ADJ ? wv2[p + 4 + (k)] * wu[p + 4 + (k)] : wu[p + 4 + (k)])
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
@@ -436,10 +468,18 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         // Injection: per-cell contributions in trace order (Injection.f90:434-440)
         if (has_inj) {
             const int e = s_u.inj[lane + TZ * (wv * PX + p)];
-            const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
-            for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
-                const float4v w = co[k];   // gain, sinc z, sinc x, tile-local trace (int bits)
-                val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
+            // gain, sinc z, sinc x, tile-local trace (int bits)
+            if (coef_lds) {
+                for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
+                    const float4v w = s_coef[k];
+                    val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
+                }
+            } else {
+                const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
+                for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
+                    const float4v w = co[k];
+                    val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
+                }
             }
         }
         // Born secondary source: Imaging.f90:779-783, forward time only

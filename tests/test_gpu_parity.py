@@ -145,6 +145,26 @@ def test_rtm_image_matches_oracle(lsrtm):
     assert ei <= TOL and el <= TOL
 
 
+def test_rtm_dense_receivers():
+    """Receivers every half cell: a tile holds more injection contributions
+    than its LDS stage (2048), so the reverse-time injection reads them from
+    global memory; the image must not change."""
+    say("case: rtm dense receivers")
+    nz, nx, nt = 40, 72, 200
+    vel, w, _, _, _ = _rtm_case(nz=nz, nx=nx, nt=nt, seed=5)
+    rx = np.arange(20.0, 10.0 * (nx - 3), 5.0).astype(np.float32)
+    rz = np.full_like(rx, 14.0)
+    data = (np.random.default_rng(5).standard_normal((len(rx), nt)) * 1e-2).astype(np.float32)
+    gs, os_ = both_shots(18.0, 300.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    img, ill = e.rtm([gs], [data])
+    out = O.rtm(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel, os_, data)
+    ei, el = rel(img, out["image"]), rel(ill, out["illum"])
+    say("%d receivers: image rel=%.3e illum rel=%.3e bitwise=%s" % (len(rx), ei, el,
+                                                                    np.array_equal(img, out["image"])))
+    assert ei <= TOL and el <= TOL
+
+
 def test_rtm_vel_bwd():
     """Optional second velocity for the backward pass (3DRTM.f90 vel2)."""
     say("case: rtm with vel2")
