@@ -34,11 +34,11 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_inputs(seed=2):
+def build_inputs(seed=2, nt=NT):
     from seismic_amd.synthetic import faulted_layers, gaussian_smooth, ricker
     true = faulted_layers(NZ, NX, DZ, DZ, seed=seed)
     mig = gaussian_smooth(true, 60.0 / DZ)
-    w = ricker(20.0, NT, DT)
+    w = ricker(20.0, nt, DT)
     rx = (np.arange(NX) * DZ).astype(np.float32)
     rz = np.zeros_like(rx)
     return true, mig, w, rz, rx
@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--graphs", type=int, default=1, help="0: eager launches (for profilers)")
     ap.add_argument("--strip", type=int, default=0, help="columns per wavefront (0 auto, 4, 8, 16)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--nt", type=int, default=NT, help="record length (shorter records for counter passes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the 512 MiB copy kernel (counter calibration) and exit")
     args = ap.parse_args()
@@ -117,7 +118,7 @@ def main():
         sys.exit(0 if rc == 0 else 1)
     info = sa.device_info()
     log("device", info["name"], "CUs", info["cus"])
-    true, mig, w, rz, rx = build_inputs()
+    true, mig, w, rz, rx = build_inputs(nt=args.nt)
     g = sa.make_grid(NZ, NX, DZ, DZ, DT, ntaper=NTAPER, snapi=SNAPI)
     e = sa.Engine(g)
     tz, tx = (int(v) for v in args.tile.split("x"))
@@ -172,8 +173,8 @@ def main():
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Gpts/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1000, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "rtm-single-shot-751x2301-nt7500", "grid": [NZ, NX], "dx_m": DZ, "dt_s": DT,
-                      "nt": NT, "ntaper": NTAPER, "receivers": NX, "snapi": SNAPI, "snapshots": args.snap,
+           "config": {"workload": "rtm-single-shot-751x2301-nt%d" % args.nt, "grid": [NZ, NX], "dx_m": DZ,
+                      "dt_s": DT, "nt": args.nt, "ntaper": NTAPER, "receivers": NX, "snapi": SNAPI, "snapshots": args.snap,
                       "tile": args.tile, "strip": args.strip, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
                       "device": info["name"]},
            "roofline": roof, "cpu_baseline": cpu}

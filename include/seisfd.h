@@ -206,6 +206,46 @@ int  sfd_lbfgs_step(sfd_lbfgs_handle lb, double *x, double f, const double *g,
                     int *iflag, int *info);
 int  sfd_lbfgs_destroy(sfd_lbfgs_handle lb);
 
+/* ---- In-process L-BFGS driver (libs/libwave/Inversion.f90:29-189 l_bfgs) --
+ * Options mirror InversionParam (Inversion_types.f90:46-58): niter (default
+ * 10), neval (3*niter), vpmin/vpmax (XMIN/XMAX), bound (const_type, default 2:
+ * clip + FREEZE_X at each accepted iteration; 1: clip at each evaluation),
+ * freeze_soft (default 1), MSAVE = m (5), eval0 = invparam%eval at entry. */
+typedef struct {
+    int    niter, neval, m, const_type, freeze_soft, eval0;
+    double xmin, xmax;
+} sfd_lbfgs_opts;
+
+/* Run report; the history arrays are optional caller buffers of hist_cap
+ * entries (one per evaluation): f_eval = unscaled misfit, t_eval = wall
+ * seconds since the start, accepted = 1 where the evaluation completed an
+ * iteration (MYINFO = 1). */
+typedef struct {
+    int     iterations, evaluations, iflag, nhist, hist_cap;
+    double  seconds;
+    double *f_eval, *t_eval;
+    int    *accepted;
+} sfd_lbfgs_report;
+
+/* fctgdt of Inversion.f90:31-39 with the model passed explicitly: evaluate
+ * the misfit f and its gradient g (FP32, the reference's sign: LBFGS is given
+ * -g) at the FP32 model x; return 0 on success. */
+typedef int (*sfd_fctgdt)(void *ctx, const float *x, float *g, double *f);
+
+/* l_bfgs: x (n, FP32) is the starting model on entry and the result on exit
+ * (the last accepted model, or the next trial model when niter is reached,
+ * as the reference returns).  vpmask / vpinit may be NULL (all ones / x). */
+int  sfd_lbfgs_run(const sfd_lbfgs_opts *o, int n, float *x, const float *vpmask,
+                   const double *vpinit, sfd_fctgdt fct, void *ctx,
+                   sfd_lbfgs_report *rep);
+
+/* FWI with l_bfgs, every evaluation being sfd_fwi_gradient on the engine's
+ * model (progs/afwi: compute_fct_gdt as fctgdt).  x: (nz, nx) column-major
+ * starting velocity, overwritten with the result. */
+int  sfd_fwi_lbfgs(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs,
+                   const float *mute, const float *vpmask, const sfd_fwi_opts *fo,
+                   const sfd_lbfgs_opts *lo, float *x, sfd_lbfgs_report *rep);
+
 #ifdef __cplusplus
 }
 #endif

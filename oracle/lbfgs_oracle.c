@@ -383,3 +383,77 @@ L172:
     }
     goto L80;
 }
+
+/*
+ * Inversion.f90:29-189 l_bfgs: the in-process driver around LBFGS.
+ * fct(ctx, x, g, f) evaluates the misfit f (double) and its gradient g (FP32,
+ * the reference's sign convention: LBFGS receives gd = -g) at the FP32 model x.
+ * On the first evaluation (eval == 0) g0 = maxval(x)/(100 maxval|g|) and
+ * f0 = maxval(x)^2/|f| are fixed (FP32 and FP64 as in :117-121) and scale every
+ * later g and f.  The LBFGS iterate is rounded through FP32 between calls
+ * (x = sngl(xd); xd = dble(x)), as the reference does (:127-136).
+ * Histories: f_eval[k] = unscaled misfit of evaluation k, accepted[k] = 1
+ * where that call completed an iteration (MYINFO == 1); cap bounds both.
+ * Returns the final IFLAG; *nevals, *niters count evaluations / iterations.
+ */
+typedef int (*orc_fctgdt)(void *ctx, const float *x, float *g, double *f);
+
+int orc_l_bfgs(int n, float *x, int niter, int neval, int eval0, int msave, double xmin, double xmax,
+               const float *mask, int const_type, int freeze_soft, const double *vpinit, orc_fctgdt fct,
+               void *ctx, double *f_eval, int *accepted, int cap, int *nevals, int *niters)
+{
+    const double EPS = 1.0e-20, XTOL = 2.220446049250313e-16;
+    double *wd = calloc((size_t)n * (2 * msave + 1) + 2 * msave, sizeof(double));
+    double *diagd = calloc(n, sizeof(double)), *gd = calloc(n, sizeof(double));
+    double *xd = calloc(n, sizeof(double)), *xsave = calloc(n, sizeof(double));
+    double *xdsave = calloc(n, sizeof(double));
+    float *g = calloc(n, sizeof(float));
+    orc_lbfgs_t *S = orc_lbfgs_new();
+    float g0 = 0.f;
+    double f0 = 0.0, fd = 0.0;
+    int i, iflag = 0, myinfo = 0, cont = 1, found = 0, count = 0, eval = eval0, k = 0;
+    for (i = 0; i < n; i++) xsave[i] = (double)x[i];
+    memcpy(xdsave, xsave, n * sizeof(double));
+    while (cont && !found) {
+        fct(ctx, x, g, &fd);
+        if (k < cap) f_eval[k] = fd;
+        if (eval == 0) {
+            float mx = x[0], mg = fabsf(g[0]);
+            for (i = 1; i < n; i++) {
+                if (x[i] > mx) mx = x[i];
+                if (fabsf(g[i]) > mg) mg = fabsf(g[i]);
+            }
+            g0 = mx / (100 * mg);
+            f0 = (double)(mx * mx) / fabs(fd);
+        }
+        for (i = 0; i < n; i++) g[i] = g0 * g[i];
+        fd = f0 * fd;
+        for (i = 0; i < n; i++) {
+            xd[i] = (double)x[i];
+            gd[i] = -(double)g[i];
+        }
+        memcpy(xdsave, xd, n * sizeof(double));
+        orc_lbfgs(S, n, msave, xd, fd, gd, diagd, EPS, XTOL, wd, &iflag, &myinfo, xmin, xmax, mask,
+                  const_type, freeze_soft, vpinit);
+        for (i = 0; i < n; i++) x[i] = (float)xd[i];
+        if (k < cap) accepted[k] = myinfo == 1;
+        if (myinfo == 1) {
+            if (const_type == 2) orc_freeze_x(xmin, xmax, mask, freeze_soft, xdsave, n, vpinit);
+            memcpy(xsave, xdsave, n * sizeof(double));
+            myinfo = 0;
+            count = count + 1;
+            if (count >= niter) found = 1;
+        }
+        if (iflag <= 0) cont = 0;
+        eval = eval + 1;
+        k = k + 1;
+        if (eval >= neval) cont = 0;
+    }
+    if (!cont)
+        for (i = 0; i < n; i++) x[i] = (float)xsave[i];
+    *nevals = k;
+    *niters = count;
+    free(wd); free(diagd); free(gd); free(xd); free(xsave); free(xdsave); free(g);
+    orc_lbfgs_free(S);
+    return iflag;
+}

@@ -65,12 +65,33 @@ class sfd_fwi_opts(C.Structure):
     ]
 
 
+class sfd_lbfgs_opts(C.Structure):
+    _fields_ = [
+        ("niter", C.c_int), ("neval", C.c_int), ("m", C.c_int), ("const_type", C.c_int),
+        ("freeze_soft", C.c_int), ("eval0", C.c_int),
+        ("xmin", C.c_double), ("xmax", C.c_double),
+    ]
+
+
+class sfd_lbfgs_report(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("evaluations", C.c_int), ("iflag", C.c_int), ("nhist", C.c_int),
+        ("hist_cap", C.c_int), ("seconds", C.c_double),
+        ("f_eval", C.POINTER(C.c_double)), ("t_eval", C.POINTER(C.c_double)),
+        ("accepted", C.POINTER(C.c_int)),
+    ]
+
+
+SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.POINTER(C.c_double))
+
 # Every symbol declared in include/seisfd.h, in declaration order.
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_born", "sfd_lsrtm",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_set_option", "sfd_kernel_stats",
     "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
+    "sfd_lbfgs_run", "sfd_fwi_lbfgs",
 ]
 
 _lib = None
@@ -111,5 +132,9 @@ def load():
     lib.sfd_lbfgs_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                    P(C.c_int), P(C.c_int)]
     lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
+    lib.sfd_lbfgs_run.argtypes = [P(sfd_lbfgs_opts), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  SFD_FCTGDT, C.c_void_p, P(sfd_lbfgs_report)]
+    lib.sfd_fwi_lbfgs.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  P(sfd_fwi_opts), P(sfd_lbfgs_opts), C.c_void_p, P(sfd_lbfgs_report)]
     _lib = lib
     return lib

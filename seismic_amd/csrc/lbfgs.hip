@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -310,6 +311,8 @@ L172:
     npt = point;                                  // (the reference's NPT = POINT*N)
     scale_copy(stp, Sv(npt), Sv(npt));
     hipLaunchKernelGGL(sub_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)g, (const double *)q, Yv(npt));
+    // accepted pair: gradient at the start of the line search (q) and the step taken
+    if (on_pair) on_pair(pair_ctx, q, Sv(npt), n, st);
     point = point + 1;
     if (point == m) point = 0;
     gnorm = sqrt(dot(g, g));
@@ -323,6 +326,117 @@ L172:
         return;
     }
     goto L80;
+}
+
+// ---------------------------------------------------------------------------
+// Inversion.f90:29-189 l_bfgs around the device LBFGS.  The control flow
+// (first-evaluation scaling g0/f0, FP32 round trip of the iterate, FREEZE_X
+// of the accepted model, niter/neval stops, the final model choice) follows
+// the reference line by line; the vector work is on the device in FP64.
+__global__ void lbfgs_prep_kernel(long n, const float *__restrict__ xf, float *__restrict__ gf, float g0,
+                                  double *__restrict__ xd, double *__restrict__ gd) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gs = g0 * gf[i];
+        gf[i] = gs;
+        xd[i] = (double)xf[i];
+        gd[i] = -(double)gs;
+    }
+}
+__global__ void to_float_kernel(long n, const double *__restrict__ xd, float *__restrict__ xf) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        xf[i] = (float)xd[i];
+}
+
+int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, const double *vpinit,
+                const LbfgsFct &fct, sfd_lbfgs_report *rep, LbfgsHooks *hooks) {
+    const double EPS = 1.0e-20, XTOL = 2.220446049250313e-16;   // Inversion.f90:79-80
+    const int msave = o.m > 0 ? o.m : 5;
+    std::vector<float> ones;
+    if (!mask) {
+        ones.assign(n, 1.0f);
+        mask = ones.data();
+    }
+    std::vector<double> vpi;
+    if (!vpinit) {
+        vpi.assign(x, x + n);
+        vpinit = vpi.data();
+    }
+    LbfgsDev d(n, msave, o.xmin, o.xmax, mask, o.const_type, o.freeze_soft, vpinit);
+    if (hooks) hooks->attach(d);
+    float *xf = nullptr, *gf = nullptr;
+    double *xdsave = nullptr, *xsave = nullptr;
+    HIPCHK(hipMalloc(&xf, n * sizeof(float)));
+    HIPCHK(hipMalloc(&gf, n * sizeof(float)));
+    HIPCHK(hipMalloc(&xdsave, n * sizeof(double)));
+    HIPCHK(hipMalloc(&xsave, n * sizeof(double)));
+    struct Free {
+        void *p[4];
+        ~Free() { for (void *q : p) if (q) (void)hipFree(q); }
+    } fr{{xf, gf, xdsave, xsave}};
+    const dim3 vg = vgrid(n), vb(256);
+    std::vector<float> g(n);
+    float g0 = 0.f;
+    double f0 = 0.0, fd = 0.0;
+    int iflag = 0, myinfo = 0, cont = 1, found = 0, count = 0, eval = o.eval0, k = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto secs = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    HIPCHK(hipMemcpyAsync(xf, x, n * sizeof(float), hipMemcpyHostToDevice, d.st));
+    hipLaunchKernelGGL(lbfgs_prep_kernel, vg, vb, 0, d.st, n, (const float *)xf, gf, 0.f, xsave, d.g);
+    HIPCHK(hipMemcpyAsync(xdsave, xsave, n * sizeof(double), hipMemcpyDeviceToDevice, d.st));
+    HIPCHK(hipStreamSynchronize(d.st));
+    while (cont && !found) {
+        const int rc = fct(x, g.data(), &fd);
+        if (rc != 0) throw EngineError(rc < 0 ? rc : SFD_ESTATE, "misfit/gradient evaluation failed");
+        if (rep && k < rep->hist_cap) {
+            if (rep->f_eval) rep->f_eval[k] = fd;
+            if (rep->t_eval) rep->t_eval[k] = secs();
+        }
+        if (eval == 0) {
+            float mx = x[0], mg = std::fabs(g[0]);
+            for (long i = 1; i < n; i++) {
+                mx = std::max(mx, x[i]);
+                mg = std::max(mg, std::fabs(g[i]));
+            }
+            g0 = mx / (100 * mg);
+            f0 = (double)(mx * mx) / std::fabs(fd);
+        }
+        fd = f0 * fd;
+        HIPCHK(hipMemcpyAsync(xf, x, n * sizeof(float), hipMemcpyHostToDevice, d.st));
+        HIPCHK(hipMemcpyAsync(gf, g.data(), n * sizeof(float), hipMemcpyHostToDevice, d.st));
+        hipLaunchKernelGGL(lbfgs_prep_kernel, vg, vb, 0, d.st, n, (const float *)xf, gf, g0, d.x, d.g);
+        HIPCHK(hipMemcpyAsync(xdsave, d.x, n * sizeof(double), hipMemcpyDeviceToDevice, d.st));
+        if (hooks) hooks->before_step(d, fd);
+        d.step(fd, &iflag, &myinfo, EPS, XTOL);
+        hipLaunchKernelGGL(to_float_kernel, vg, vb, 0, d.st, n, (const double *)d.x, xf);
+        HIPCHK(hipMemcpyAsync(x, xf, n * sizeof(float), hipMemcpyDeviceToHost, d.st));
+        HIPCHK(hipStreamSynchronize(d.st));
+        if (rep && k < rep->hist_cap && rep->accepted) rep->accepted[k] = myinfo == 1;
+        if (myinfo == 1) {
+            if (o.const_type == 2) d.freeze(xdsave);
+            HIPCHK(hipMemcpyAsync(xsave, xdsave, n * sizeof(double), hipMemcpyDeviceToDevice, d.st));
+            myinfo = 0;
+            count = count + 1;
+            if (count >= o.niter) found = 1;
+            if (hooks) hooks->accepted(d, count);
+        }
+        if (iflag <= 0) cont = 0;
+        eval = eval + 1;
+        k = k + 1;
+        if (eval >= o.neval) cont = 0;
+    }
+    if (!cont) {
+        hipLaunchKernelGGL(to_float_kernel, vg, vb, 0, d.st, n, (const double *)xsave, xf);
+        HIPCHK(hipMemcpyAsync(x, xf, n * sizeof(float), hipMemcpyDeviceToHost, d.st));
+    }
+    HIPCHK(hipStreamSynchronize(d.st));
+    if (rep) {
+        rep->iterations = count;
+        rep->evaluations = k;
+        rep->iflag = iflag;
+        rep->seconds = secs();
+        rep->nhist = std::min(k, rep->hist_cap);
+    }
+    return iflag;
 }
 
 // ---------------------------------------------------------------------------
@@ -360,6 +474,18 @@ int sfd_lbfgs_step(sfd_lbfgs_handle lb, double *x, double f, const double *g, in
         return SFD_OK;
     } catch (const EngineError &e) {
         lb->err = e.msg;
+        return e.code;
+    }
+}
+
+int sfd_lbfgs_run(const sfd_lbfgs_opts *o, int n, float *x, const float *vpmask, const double *vpinit,
+                  sfd_fctgdt fct, void *ctx, sfd_lbfgs_report *rep) {
+    if (!o || n < 1 || !x || !fct) return SFD_EINVAL;
+    try {
+        LbfgsFct f = [&](const float *xx, float *g, double *fv) { return fct(ctx, xx, g, fv); };
+        lbfgs_drive(*o, n, x, vpmask, vpinit, f, rep, nullptr);
+        return SFD_OK;
+    } catch (const EngineError &e) {
         return e.code;
     }
 }

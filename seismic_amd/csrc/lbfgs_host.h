@@ -6,10 +6,12 @@
 // W(N+M+1:N+2M) (alpha) live on the host.  DIAG doubles as MCSRCH's WA
 // exactly as in the reference call (lbfgs.f:486-488).
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "common.h"
 #include "lbfgs_dev.h"
+#include "lbfgs_api.h"
 
 struct LbfgsDev {
     long n = 0;
@@ -34,6 +36,9 @@ struct LbfgsDev {
     void (*h0)(void *ctx, double *q, long n, hipStream_t st) = nullptr;
     void *h0_ctx = nullptr;
     bool h0_active = false;
+    // optional observer of every accepted (gradient, step) pair
+    void (*on_pair)(void *ctx, const double *gk, const double *sk, long n, hipStream_t st) = nullptr;
+    void *pair_ctx = nullptr;
 
     LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mask_h, int boundt_, int type_,
              const double *xinit_h);
@@ -48,3 +53,4 @@ struct LbfgsDev {
     void step(double f, int *iflag, int *myinfo, double eps, double xtol);
     void mcsrch(double f, double *s);
 };
+

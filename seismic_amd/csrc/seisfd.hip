@@ -20,6 +20,7 @@
 #include "fwi_kernels.h"
 #include "engine.h"
 #include "common.h"
+#include "lbfgs_api.h"
 
 using namespace sfd;
 
@@ -1180,6 +1181,24 @@ int sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const floa
                      float *resid_out) {
     return guarded(h, [&](Engine &e) {
         e.fwi_gradient(shots, nshots, dobs, mute, vpmask, opt, misfit, grad, illum_out, resid_out);
+    });
+}
+
+// FWI inversion: Inversion.f90 l_bfgs with compute_fct_gdt as fctgdt
+// (progs/afwi: the model is set, then misfit and gradient are computed).
+int sfd_fwi_lbfgs(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs, const float *mute,
+                  const float *vpmask, const sfd_fwi_opts *fo, const sfd_lbfgs_opts *lo, float *x,
+                  sfd_lbfgs_report *rep) {
+    if (!lo || !fo || !x) return SFD_EINVAL;
+    return guarded(h, [&](Engine &e) {
+        const long n = (long)e.g.nz * e.g.nx;
+        std::vector<double> vpinit(x, x + n);
+        LbfgsFct f = [&](const float *xx, float *g, double *fv) {
+            e.set_model(xx, nullptr);
+            e.fwi_gradient(shots, nshots, dobs, mute, vpmask, fo, fv, g, nullptr, nullptr);
+            return 0;
+        };
+        lbfgs_drive(*lo, n, x, vpmask, vpinit.data(), f, rep, nullptr);
     });
 }
 

@@ -175,6 +175,24 @@ class Engine:
                                             C.byref(o), C.byref(f), _ptr(grad), _ptr(illu), _ptr(res)))
         return f.value, grad.reshape(nx, nz).T, illu.reshape(nx, nz).T, self._split(res, shots)
 
+    def fwi_lbfgs(self, shots, dobs, x0, niter=10, neval=None, vpmin=1500.0, vpmax=4500.0, bound=2,
+                  freeze_soft=True, m=5, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
+                  do_x=0, snap_mode=abi.SFD_SNAP_FP32):
+        """FWI by Inversion.f90 l_bfgs with compute_fct_gdt as the evaluation
+        (defaults of Inversion_types.f90:48-57).  Returns (model, report)."""
+        nz, nx = self.g.nz, self.g.nx
+        ob = self._cat(dobs, shots)
+        mu = None if mute is None else self._cat(mute, shots)
+        vm = None if vpmask is None else _model_buf(vpmask, nz, nx)
+        fo = sfd_fwi_opts()
+        fo.illupow, fo.rect_z, fo.rect_x, fo.do_z, fo.do_x, fo.snap_mode = illupow, rect_z, rect_x, do_z, do_x, snap_mode
+        lo = lbfgs_opts(niter, neval, m, bound, freeze_soft, vpmin, vpmax)
+        x = _model_buf(x0, nz, nx).copy()
+        rep, keep = _report(lo.neval)
+        self._chk(self.lib.sfd_fwi_lbfgs(self.h, self._shots(shots), len(shots), _ptr(ob), _ptr(mu), _ptr(vm),
+                                         C.byref(fo), C.byref(lo), _ptr(x), C.byref(rep)))
+        return x.reshape(nx, nz).T.copy(), _report_dict(rep, keep)
+
     def set_option(self, opt, value):
         self._chk(self.lib.sfd_set_option(self.h, opt, int(value)))
 
@@ -200,6 +218,63 @@ def device_info():
     if rc != 0:
         raise SeisFDError(rc, "no HIP device")
     return dict(name=name.value.decode(), cus=cu.value, hbm_bytes=mem.value)
+
+
+def lbfgs_opts(niter=10, neval=None, m=5, bound=2, freeze_soft=True, vpmin=1500.0, vpmax=4500.0, eval0=0):
+    o = abi.sfd_lbfgs_opts()
+    o.niter, o.neval, o.m, o.const_type = int(niter), int(3 * niter if neval is None else neval), int(m), int(bound)
+    o.freeze_soft, o.eval0, o.xmin, o.xmax = int(bool(freeze_soft)), int(eval0), float(vpmin), float(vpmax)
+    return o
+
+
+def _report(cap):
+    keep = dict(f=np.zeros(cap), t=np.zeros(cap), a=np.zeros(cap, np.int32))
+    r = abi.sfd_lbfgs_report()
+    r.hist_cap = cap
+    r.f_eval = keep["f"].ctypes.data_as(C.POINTER(C.c_double))
+    r.t_eval = keep["t"].ctypes.data_as(C.POINTER(C.c_double))
+    r.accepted = keep["a"].ctypes.data_as(C.POINTER(C.c_int))
+    return r, keep
+
+
+def _report_dict(r, keep):
+    k = r.nhist
+    return dict(iterations=r.iterations, evaluations=r.evaluations, iflag=r.iflag, seconds=r.seconds,
+                f_eval=keep["f"][:k].copy(), t_eval=keep["t"][:k].copy(), accepted=keep["a"][:k].astype(bool))
+
+
+def lbfgs_run(fct, x0, niter=10, neval=None, m=5, bound=2, freeze_soft=True, xmin=1500.0, xmax=4500.0,
+              vpmask=None, vpinit=None, eval0=0):
+    """Inversion.f90 l_bfgs on the device L-BFGS: fct(x) -> (f, g) with x, g
+    FP32 vectors (g in the reference's sign convention: LBFGS is given -g).
+    Returns (x, report)."""
+    lib = abi.load()
+    x = np.ascontiguousarray(x0, np.float32).copy()
+    n = x.size
+    lo = lbfgs_opts(niter, neval, m, bound, freeze_soft, xmin, xmax, eval0)
+    rep, keep = _report(lo.neval)
+    err = []
+
+    def cb(_ctx, xp, gp, fp):
+        try:
+            xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+            f, g = fct(xv)
+            np.ctypeslib.as_array(gp, shape=(n,))[:] = np.asarray(g, np.float32)
+            fp[0] = float(f)
+            return 0
+        except Exception as ex:   # reported after the run
+            err.append(ex)
+            return -5
+
+    cfun = abi.SFD_FCTGDT(cb)
+    vm = None if vpmask is None else np.ascontiguousarray(vpmask, np.float32)
+    vi = None if vpinit is None else np.ascontiguousarray(vpinit, np.float64)
+    rc = lib.sfd_lbfgs_run(C.byref(lo), n, _ptr(x), _ptr(vm), _ptr(vi), cfun, None, C.byref(rep))
+    if err:
+        raise err[0]
+    if rc != 0:
+        raise SeisFDError(rc, "sfd_lbfgs_run failed")
+    return x, _report_dict(rep, keep)
 
 
 class LBFGS:

@@ -340,3 +340,37 @@ class LBFGS:
             self.L.orc_lbfgs_free(self.S)
         except Exception:
             pass
+
+
+FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_double))
+
+
+def l_bfgs(fct, x0, niter=10, neval=None, m=5, bound=2, freeze_soft=True, xmin=1500.0, xmax=4500.0,
+           vpmask=None, vpinit=None, eval0=0):
+    """orc_l_bfgs (restatement of Inversion.f90 l_bfgs); fct(x) -> (f, g).
+    Returns (x, dict(f_eval, accepted, evaluations, iterations, iflag))."""
+    L = oracle()
+    L.orc_l_bfgs.restype = C.c_int
+    L.orc_l_bfgs.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.c_void_p, C.c_int, C.c_int, C.c_void_p, FCTGDT, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    x = np.ascontiguousarray(x0, np.float32).copy()
+    n = x.size
+    neval = 3 * niter if neval is None else neval
+    mask = np.ones(n, np.float32) if vpmask is None else np.ascontiguousarray(vpmask, np.float32)
+    vi = x.astype(np.float64) if vpinit is None else np.ascontiguousarray(vpinit, np.float64)
+    fe = np.zeros(neval)
+    acc = np.zeros(neval, np.int32)
+    ne, ni = C.c_int(), C.c_int()
+
+    def cb(_ctx, xp, gp, fp):
+        f, g = fct(np.ctypeslib.as_array(xp, shape=(n,)).copy())
+        np.ctypeslib.as_array(gp, shape=(n,))[:] = np.asarray(g, np.float32)
+        fp[0] = float(f)
+        return 0
+
+    cf = FCTGDT(cb)
+    iflag = L.orc_l_bfgs(n, ptr(x), niter, neval, eval0, m, xmin, xmax, ptr(mask), bound, int(bool(freeze_soft)),
+                         ptr(vi), cf, None, ptr(fe), ptr(acc), neval, C.byref(ne), C.byref(ni))
+    k = ne.value
+    return x, dict(f_eval=fe[:k], accepted=acc[:k].astype(bool), evaluations=k, iterations=ni.value, iflag=iflag)

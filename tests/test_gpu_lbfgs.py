@@ -57,3 +57,35 @@ def test_device_lbfgs_matches_oracle(problem, boundt):
         assert abs(f1 - f2) <= 1e-8 * max(1.0, abs(f1))
         assert np.linalg.norm(x1 - x2) <= 1e-7 * max(1.0, np.linalg.norm(x1))
     print("accepted iterations compared: %d, final f oracle %.3e device %.3e" % (k, ho[-1][0], hd[-1][0]))
+
+
+def _quad(n=48, seed=3):
+    rng = np.random.default_rng(seed)
+    Q = rng.standard_normal((n, n))
+    A = Q @ Q.T / n + np.eye(n)
+    xs = 2500.0 + 300.0 * rng.standard_normal(n)
+
+    def fct(x):
+        r = x.astype(np.float64) - xs
+        return 0.5 * r @ A @ r, -(A @ r)   # reference sign: the evaluation returns -gradient
+    return fct
+
+
+@pytest.mark.parametrize("bound", [0, 2])
+def test_device_l_bfgs_driver_matches_oracle(bound):
+    """sfd_lbfgs_run (Inversion.f90 l_bfgs around the device LBFGS) against
+    the oracle's restatement: same evaluations, same accepted iterations,
+    same returned model."""
+    print("case: l_bfgs driver bound=%d" % bound, flush=True)
+    fct = _quad()
+    x0 = np.full(48, 2000.0, np.float32)
+    kw = dict(niter=12, neval=80, bound=bound, xmin=1800.0, xmax=3200.0)
+    xo, ro = O.l_bfgs(fct, x0, **kw)
+    xd, rd = sa.lbfgs_run(fct, x0, **kw)
+    print("oracle evals %d iters %d f %.6e | device evals %d iters %d f %.6e" % (
+        ro["evaluations"], ro["iterations"], ro["f_eval"][-1], rd["evaluations"], rd["iterations"],
+        rd["f_eval"][-1]), flush=True)
+    assert rd["evaluations"] == ro["evaluations"] and rd["iterations"] == ro["iterations"]
+    assert np.array_equal(rd["accepted"], ro["accepted"])
+    assert np.allclose(rd["f_eval"], ro["f_eval"], rtol=1e-9, atol=1e-12 * ro["f_eval"][0])
+    assert np.allclose(xd, xo, rtol=1e-6, atol=0)

@@ -275,3 +275,41 @@ def test_fwi_gradient_matches_oracle(snap, illupow):
         assert eg <= TOL and ei <= TOL
     else:
         assert eg <= 1e-3 and ei <= 1e-3
+
+
+def test_fwi_lbfgs_matches_oracle_driver():
+    """FWI by l_bfgs (Inversion.f90:29-189) with compute_fct_gdt as the
+    evaluation: sfd_fwi_lbfgs against the oracle driver calling the oracle
+    gradient — same misfit at every evaluation, misfit decreasing."""
+    say("case: fwi l_bfgs")
+    nz, nx, nt = 40, 96, 300
+    true = smooth_random(nz, nx, seed=21, vmin=1600, vmax=2600)
+    init = smooth_random(nz, nx, seed=22, vmin=1600, vmax=2500)
+    og = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12)
+    w = ricker(18.0, nt, 1e-3)
+    gsh, osh, dobs = [], [], []
+    for sx in [200.0, 470.0, 760.0]:
+        rx = np.linspace(max(0.0, sx - 300), min(10.0 * (nx - 1), sx + 300), 41).astype(np.float32)
+        rz = np.full_like(rx, 15.0)
+        a, b = both_shots(12.0, sx, w, rz, rx)
+        gsh.append(a)
+        osh.append(b)
+        dobs.append(O.modeling(og, true, b, extract_kind=1))
+    kw = dict(rect_z=2, rect_x=3, do_z=1, do_x=1)
+
+    def fct(xv):
+        r = O.fwi_gradient(og, xv.reshape(nx, nz).T, osh, dobs, **kw)
+        return r["f"], np.asfortranarray(r["grad"]).ravel(order="F")
+
+    lk = dict(niter=3, neval=12, bound=2, vpmin=1500.0, vpmax=3000.0)
+    x0 = np.asfortranarray(init).ravel(order="F")
+    xo, ro = O.l_bfgs(fct, x0, niter=3, neval=12, bound=2, xmin=1500.0, xmax=3000.0)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=init)
+    xg, rg = e.fwi_lbfgs(gsh, dobs, init, **lk, **kw)
+    say("oracle f %s\ndevice f %s" % (np.array2string(ro["f_eval"], precision=6),
+                                       np.array2string(rg["f_eval"], precision=6)))
+    assert rg["evaluations"] == ro["evaluations"] and rg["iterations"] == ro["iterations"] == 3
+    assert np.allclose(rg["f_eval"], ro["f_eval"], rtol=1e-5)
+    assert rel(xg, xo.reshape(nx, nz).T) <= 1e-5
+    fa = rg["f_eval"][rg["accepted"]]
+    assert fa[-1] < rg["f_eval"][0]
