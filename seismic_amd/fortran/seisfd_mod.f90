@@ -1,0 +1,165 @@
+! seisfd_mod.f90 -- Fortran 2003 binding of the seisfd C-ABI (include/seisfd.h).
+!
+! A Fortran host program of the reference (libs/libwave, progs/rtm, progs/afwi)
+! uses this module instead of calling propagator_acoustic
+! (libs/libwave/Propagator.f90:27-270) or l_bfgs (libs/libwave/Inversion.f90:29-189):
+! the derived types below are bind(C) twins of the C structs, and every entry
+! point is an interface to the extern "C" symbol of libseisfd.so.  Arrays are
+! passed as they are held: vel(nz,nx), trace(nt,ntr), column-major.
+module seisfd_mod
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: SFD_OK = 0, SFD_EINVAL = -1, SFD_ENOMEM = -2, SFD_EHIP = -3, &
+       SFD_EUNSUPPORTED = -4, SFD_ESTATE = -5, SFD_EUNSTABLE = -6, SFD_ENOTFINITE = -7
+  integer(c_int), parameter :: SFD_SNAP_FP32 = 0, SFD_SNAP_FP16S = 1
+  integer(c_int), parameter :: SFD_EXTRACT_SINC_V2 = 0, SFD_EXTRACT_NEAREST = 1, &
+       SFD_EXTRACT_SINC_LSRTM = 2, SFD_EXTRACT_SINC_UNIT = 3
+
+  type, bind(C) :: sfd_grid
+     integer(c_int32_t) :: nz, nx
+     real(c_float)      :: oz, ox, dz, dx, dt
+     integer(c_int32_t) :: ntaper, lsinc, snapi
+     real(c_float)      :: aperture_x
+     integer(c_int32_t) :: rec_type, shot_type, surf_type
+  end type sfd_grid
+
+  type, bind(C) :: sfd_shot
+     real(c_float)      :: sz, sx
+     integer(c_int32_t) :: nt, ntr
+     type(c_ptr)        :: wavelet, rz, rx
+  end type sfd_shot
+
+  type, bind(C) :: sfd_fwi_opts
+     real(c_float)      :: illupow
+     integer(c_int32_t) :: rect_z, rect_x, do_z, do_x, snap_mode
+  end type sfd_fwi_opts
+
+  type, bind(C) :: sfd_lbfgs_opts
+     integer(c_int) :: niter, neval, m, const_type, freeze_soft, eval0
+     real(c_double) :: xmin, xmax
+  end type sfd_lbfgs_opts
+
+  type, bind(C) :: sfd_lbfgs_report
+     integer(c_int) :: iterations, evaluations, iflag, nhist, hist_cap
+     real(c_double) :: seconds
+     type(c_ptr)    :: f_eval, t_eval, accepted
+  end type sfd_lbfgs_report
+
+  interface
+     integer(c_int) function sfd_create(h, g) bind(C, name='sfd_create')
+       import :: c_int, c_ptr, sfd_grid
+       type(c_ptr), intent(out) :: h
+       type(sfd_grid), intent(in) :: g
+     end function sfd_create
+
+     integer(c_int) function sfd_destroy(h) bind(C, name='sfd_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function sfd_destroy
+
+     type(c_ptr) function sfd_last_error(h) bind(C, name='sfd_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function sfd_last_error
+
+     integer(c_int) function sfd_set_model(h, vel, vel_bwd) bind(C, name='sfd_set_model')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: h
+       real(c_float), intent(in) :: vel(*)
+       type(c_ptr), value :: vel_bwd
+     end function sfd_set_model
+
+     integer(c_int) function sfd_forward(h, shots, nshots, traces, extract_kind) bind(C, name='sfd_forward')
+       import :: c_int, c_ptr, c_float, sfd_shot
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots, extract_kind
+       real(c_float), intent(inout) :: traces(*)
+     end function sfd_forward
+
+     integer(c_int) function sfd_rtm(h, shots, nshots, traces, image, illum, snap_mode) bind(C, name='sfd_rtm')
+       import :: c_int, c_ptr, c_float, sfd_shot
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots, snap_mode
+       real(c_float), intent(in) :: traces(*)
+       real(c_float), intent(inout) :: image(*), illum(*)
+     end function sfd_rtm
+
+     integer(c_int) function sfd_born(h, shots, nshots, refl, traces) bind(C, name='sfd_born')
+       import :: c_int, c_ptr, c_float, sfd_shot
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: refl(*)
+       real(c_float), intent(inout) :: traces(*)
+     end function sfd_born
+
+     integer(c_int) function sfd_lsrtm(h, shots, nshots, traces, image, illum) bind(C, name='sfd_lsrtm')
+       import :: c_int, c_ptr, c_float, sfd_shot
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: traces(*)
+       real(c_float), intent(inout) :: image(*), illum(*)
+     end function sfd_lsrtm
+
+     integer(c_int) function sfd_fwi_gradient(h, shots, nshots, dobs, mute, vpmask, opt, misfit, grad, &
+          illum, resid) bind(C, name='sfd_fwi_gradient')
+       import :: c_int, c_ptr, c_float, c_double, sfd_shot, sfd_fwi_opts
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: dobs(*)
+       type(c_ptr), value :: mute, vpmask
+       type(sfd_fwi_opts), intent(in) :: opt
+       real(c_double), intent(out) :: misfit
+       real(c_float), intent(inout) :: grad(*)
+       type(c_ptr), value :: illum, resid
+     end function sfd_fwi_gradient
+
+     integer(c_int) function sfd_lbfgs_run(o, n, x, vpmask, vpinit, fct, ctx, rep) bind(C, name='sfd_lbfgs_run')
+       import :: c_int, c_ptr, c_float, c_funptr, sfd_lbfgs_opts, sfd_lbfgs_report
+       type(sfd_lbfgs_opts), intent(in) :: o
+       integer(c_int), value :: n
+       real(c_float), intent(inout) :: x(*)
+       type(c_ptr), value :: vpmask, vpinit
+       type(c_funptr), value :: fct
+       type(c_ptr), value :: ctx
+       type(sfd_lbfgs_report), intent(inout) :: rep
+     end function sfd_lbfgs_run
+
+     integer(c_int) function sfd_fwi_lbfgs(h, shots, nshots, dobs, mute, vpmask, fo, lo, x, rep) &
+          bind(C, name='sfd_fwi_lbfgs')
+       import :: c_int, c_ptr, c_float, sfd_shot, sfd_fwi_opts, sfd_lbfgs_opts, sfd_lbfgs_report
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: dobs(*)
+       type(c_ptr), value :: mute, vpmask
+       type(sfd_fwi_opts), intent(in) :: fo
+       type(sfd_lbfgs_opts), intent(in) :: lo
+       real(c_float), intent(inout) :: x(*)
+       type(sfd_lbfgs_report), intent(inout) :: rep
+     end function sfd_fwi_lbfgs
+  end interface
+
+contains
+
+  ! sfd_last_error as a Fortran string
+  function sfd_error_message(h) result(msg)
+    type(c_ptr), intent(in) :: h
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ' '
+    if (.not. c_associated(sfd_last_error(h))) return
+    call c_f_pointer(sfd_last_error(h), s, [256])
+    do i = 1, 256
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    end do
+  end function sfd_error_message
+
+end module seisfd_mod

@@ -31,3 +31,16 @@ def test_struct_sizes():
     assert ctypes.sizeof(abi.sfd_grid) == 14 * 4
     assert ctypes.sizeof(abi.sfd_shot) == 8 + 8 + 3 * 8
     assert ctypes.sizeof(abi.sfd_fwi_opts) == 6 * 4
+
+
+def test_fortran_binding_struct_sizes():
+    """The bind(C) types of seismic_amd/fortran/seisfd_mod.f90 have the C
+    structs' sizes (a Fortran caller passes them by reference)."""
+    import subprocess
+    exe = os.path.join(ROOT, "seismic_amd", "lib", "seisfd_example")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran example not built")
+    out = subprocess.run([exe, "--sizes"], capture_output=True, text=True, timeout=60).stdout.split()
+    want = [ctypes.sizeof(t) for t in (abi.sfd_grid, abi.sfd_shot, abi.sfd_lbfgs_opts, abi.sfd_lbfgs_report,
+                                  abi.sfd_fwi_opts)]
+    assert [int(v) for v in out] == want
