@@ -216,16 +216,49 @@ typedef struct {
     double xmin, xmax;
 } sfd_lbfgs_opts;
 
-/* Run report; the history arrays are optional caller buffers of hist_cap
+/* Run report: iflag = the final IFLAG, info = MCSRCH's last INFO
+ * (lbfgs.f:652-700 codes); the history arrays are optional caller buffers of hist_cap
  * entries (one per evaluation): f_eval = unscaled misfit, t_eval = wall
  * seconds since the start, accepted = 1 where the evaluation completed an
  * iteration (MYINFO = 1). */
 typedef struct {
-    int     iterations, evaluations, iflag, nhist, hist_cap;
+    int     iterations, evaluations, iflag, info, nhist, hist_cap;
     double  seconds;
     double *f_eval, *t_eval;
     int    *accepted;
+    /* learned preconditioner (sfd_fwi_lbfgs with precond): accepted iterations
+     * whose direction used the network, line searches abandoned and restarted
+     * plain, directions replaced because they were not descent directions,
+     * training wall time, training loss before / after */
+    int     pc_iterations, pc_fallbacks, pc_rejects;
+    double  pc_train_seconds, pc_loss_first, pc_loss_last;
 } sfd_lbfgs_report;
+
+/* Learned gradient preconditioner (no reference counterpart): a 4-layer 3x3
+ * CNN (1 -> 16 -> 16 -> 16 -> 1 channels, BF16 weights and activations, FP32
+ * accumulation on MFMA) trained on the GPU from the (gradient, accepted step)
+ * pairs of the first train_after L-BFGS iterations and then used as the
+ * initial inverse Hessian H0 of the two-loop recursion, scaled as the plain
+ * H0 is (ys / y.H0(y)).  A preconditioned direction that is not a descent
+ * direction is replaced by the plain one; a preconditioned line search that
+ * fails or needs more than max_evals evaluations is restarted plain from the
+ * same model; after max_fallbacks such restarts the network is switched off. */
+typedef struct {
+    int   enable;          /* 0: plain L-BFGS                                  */
+    int   train_after;     /* accepted iterations whose pairs train the net (3) */
+    int   train_steps;     /* Adam steps (300)                                 */
+    float lr;              /* Adam learning rate (2e-3)                        */
+    int   seed;            /* weight initialisation                            */
+    int   max_evals;       /* evaluations allowed per preconditioned search (4) */
+    int   max_fallbacks;   /* restarts before the network is switched off (3)  */
+    int   target;          /* training target for gradient g_k: 0 the accepted
+                              step s_k of that iteration; 1 the accepted model
+                              update from x_k to the newest model, sum_{j>=k} s_j */
+    int   use_net;         /* 1: the network is trained and applied (default)   */
+    int   illum_h0;        /* 1: wrap the operator in the illumination diagonal
+                              W = illum^(-1/2) (compute_fct_gdt's illumination at
+                              the newest evaluated model): H0 = g W N(W q)     */
+} sfd_precond_opts;
 
 /* fctgdt of Inversion.f90:31-39 with the model passed explicitly: evaluate
  * the misfit f and its gradient g (FP32, the reference's sign: LBFGS is given
@@ -244,7 +277,26 @@ int  sfd_lbfgs_run(const sfd_lbfgs_opts *o, int n, float *x, const float *vpmask
  * starting velocity, overwritten with the result. */
 int  sfd_fwi_lbfgs(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs,
                    const float *mute, const float *vpmask, const sfd_fwi_opts *fo,
-                   const sfd_lbfgs_opts *lo, float *x, sfd_lbfgs_report *rep);
+                   const sfd_lbfgs_opts *lo, const sfd_precond_opts *pc, float *x,
+                   sfd_lbfgs_report *rep);
+
+/* The preconditioner's network on its own (for tests and offline use).
+ * Parameters: SFD_CNN_NPARAM FP32 values, per layer l = 0..3 the weights
+ * W[t][co][ci] (t = 3x3 tap, (dz, dx) = (t%3 - 1, t/3 - 1); 16x16 channels,
+ * only ci = 0 used in layer 0 and co = 0 in layer 3) then 16 biases.
+ * Images: nimg stacked (nz, nx) column-major FP32 arrays. */
+#define SFD_CNN_NPARAM 9280
+typedef struct sfd_cnn *sfd_cnn_handle;
+int  sfd_cnn_create(sfd_cnn_handle *c, int nz, int nx, int nimg_max, int seed);
+int  sfd_cnn_destroy(sfd_cnn_handle c);
+int  sfd_cnn_get_params(sfd_cnn_handle c, float *theta);
+int  sfd_cnn_set_params(sfd_cnn_handle c, const float *theta);
+int  sfd_cnn_forward(sfd_cnn_handle c, int nimg, const float *x, float *y);
+/* MSE loss mean((y - target)^2) and its gradient with respect to every parameter */
+int  sfd_cnn_loss_grad(sfd_cnn_handle c, int nimg, const float *x, const float *target,
+                       double *loss, float *grad);
+int  sfd_cnn_train(sfd_cnn_handle c, int nimg, const float *x, const float *target, int steps,
+                   float lr, double *loss_first, double *loss_last);
 
 #ifdef __cplusplus
 }

@@ -75,11 +75,25 @@ class sfd_lbfgs_opts(C.Structure):
 
 class sfd_lbfgs_report(C.Structure):
     _fields_ = [
-        ("iterations", C.c_int), ("evaluations", C.c_int), ("iflag", C.c_int), ("nhist", C.c_int),
+        ("iterations", C.c_int), ("evaluations", C.c_int), ("iflag", C.c_int), ("info", C.c_int),
+        ("nhist", C.c_int),
         ("hist_cap", C.c_int), ("seconds", C.c_double),
         ("f_eval", C.POINTER(C.c_double)), ("t_eval", C.POINTER(C.c_double)),
         ("accepted", C.POINTER(C.c_int)),
+        ("pc_iterations", C.c_int), ("pc_fallbacks", C.c_int), ("pc_rejects", C.c_int),
+        ("pc_train_seconds", C.c_double), ("pc_loss_first", C.c_double), ("pc_loss_last", C.c_double),
     ]
+
+
+class sfd_precond_opts(C.Structure):
+    _fields_ = [
+        ("enable", C.c_int), ("train_after", C.c_int), ("train_steps", C.c_int), ("lr", C.c_float),
+        ("seed", C.c_int), ("max_evals", C.c_int), ("max_fallbacks", C.c_int), ("target", C.c_int),
+        ("use_net", C.c_int), ("illum_h0", C.c_int),
+    ]
+
+
+SFD_CNN_NPARAM = 9280
 
 
 SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -92,6 +106,8 @@ EXPORTS = [
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_set_option", "sfd_kernel_stats",
     "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgs_run", "sfd_fwi_lbfgs",
+    "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
+    "sfd_cnn_loss_grad", "sfd_cnn_train",
 ]
 
 _lib = None
@@ -135,6 +151,15 @@ def load():
     lib.sfd_lbfgs_run.argtypes = [P(sfd_lbfgs_opts), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                   SFD_FCTGDT, C.c_void_p, P(sfd_lbfgs_report)]
     lib.sfd_fwi_lbfgs.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  P(sfd_fwi_opts), P(sfd_lbfgs_opts), C.c_void_p, P(sfd_lbfgs_report)]
+                                  P(sfd_fwi_opts), P(sfd_lbfgs_opts), P(sfd_precond_opts), C.c_void_p,
+                                  P(sfd_lbfgs_report)]
+    lib.sfd_cnn_create.argtypes = [P(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.sfd_cnn_destroy.argtypes = [C.c_void_p]
+    lib.sfd_cnn_get_params.argtypes = [C.c_void_p, C.c_void_p]
+    lib.sfd_cnn_set_params.argtypes = [C.c_void_p, C.c_void_p]
+    lib.sfd_cnn_forward.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.sfd_cnn_loss_grad.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, P(C.c_double), C.c_void_p]
+    lib.sfd_cnn_train.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_float,
+                                  P(C.c_double), P(C.c_double)]
     _lib = lib
     return lib

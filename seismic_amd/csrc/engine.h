@@ -83,6 +83,7 @@ struct Engine {
     int kstat_n[2] = {0, 0};
     double timing[4] = {0, 0, 0, 0};
     double pts = 0;
+    float *raw_illum_host = nullptr;      // fwi_gradient: also copy the summed illumination here
     Batch fwd_batch;
     DevBuf scratch, fwi_mem;
 

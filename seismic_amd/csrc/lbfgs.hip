@@ -10,6 +10,7 @@
 
 #include "../../include/seisfd.h"
 #include "common.h"
+#include "lbfgs_dev.h"
 #include "lbfgs_host.h"
 
 using namespace sfd;
@@ -237,6 +238,57 @@ L45:
     goto L30;
 }
 
+// The two-loop recursion of lbfgs.f:367-420 (DIAGCO = .false.): S(point) = -H g,
+// with H0 = (ys/yy) I, or the learned H0 when use_h0.
+void LbfgsDev::two_loop(bool use_h0) {
+    used_h0 = use_h0;
+    ys = dot(Yv(npt), Sv(npt));
+    yy = dot(Yv(npt), Yv(npt));
+    hipLaunchKernelGGL(fill_kernel, vgrid(n), dim3(256), 0, st, n, ys / yy, diag);
+    cp = point == 0 ? m : point;
+    rho[cp - 1] = 1.0 / ys;
+    scale_copy(-1.0, g, q);
+    cp = point;
+    for (int i = 1; i <= bound; i++) {
+        cp = cp - 1;
+        if (cp == -1) cp = m - 1;
+        const double sq = dot(Sv(cp), q);
+        alpha[cp] = rho[cp] * sq;
+        axpby(-alpha[cp], Yv(cp), 1.0, q);
+    }
+    if (use_h0) h0(h0_ctx, q, n, st);     // learned initial inverse Hessian
+    else hipLaunchKernelGGL(mul_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)diag, q);
+    for (int i = 1; i <= bound; i++) {
+        const double yr = dot(Yv(cp), q);
+        double beta = rho[cp] * yr;
+        beta = alpha[cp] - beta;
+        axpby(beta, Sv(cp), 1.0, q);
+        cp = cp + 1;
+        if (cp == m) cp = 0;
+    }
+    scale_copy(1.0, q, Sv(point));
+}
+
+void LbfgsDev::mul_diag(double *v) {
+    hipLaunchKernelGGL(mul_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)diag, v);
+}
+
+// Give up the current (preconditioned) line search and restart it from its
+// starting point with the plain direction: X = WA, G = W(1:N), F = FINIT.
+// Returns the IFLAG of the restarted search (1: evaluate at the new X).
+int LbfgsDev::restart_plain() {
+    scale_copy(1.0, diag, x);
+    scale_copy(1.0, q, g);
+    const double f0 = finit;
+    two_loop(false);
+    nfev = 0;
+    stp = 1.0;
+    scale_copy(1.0, g, q);
+    info = 0;
+    mcsrch(f0, Sv(point));
+    return info == -1 ? 1 : -1;
+}
+
 // LBFGS (lbfgs.f:49-558), DIAGCO = .false.
 void LbfgsDev::step(double f, int *iflag, int *myinfo, double eps, double xtol) {
     xtol_ = xtol;
@@ -262,34 +314,15 @@ L10:
 L80:
     iter = iter + 1;
     info = 0;
+    used_h0 = false;
     bound = iter - 1;
     if (iter == 1) goto L165;
     if (iter > m) bound = m;
-    ys = dot(Yv(npt), Sv(npt));
-    yy = dot(Yv(npt), Yv(npt));
-    hipLaunchKernelGGL(fill_kernel, vgrid(n), dim3(256), 0, st, n, ys / yy, diag);
-    cp = point == 0 ? m : point;
-    rho[cp - 1] = 1.0 / ys;
-    scale_copy(-1.0, g, q);
-    cp = point;
-    for (int i = 1; i <= bound; i++) {
-        cp = cp - 1;
-        if (cp == -1) cp = m - 1;
-        const double sq = dot(Sv(cp), q);
-        alpha[cp] = rho[cp] * sq;
-        axpby(-alpha[cp], Yv(cp), 1.0, q);
+    two_loop(h0 && h0_active);
+    if (used_h0 && dot(g, Sv(point)) >= 0.0) {   // the learned H0 gave no descent direction
+        h0_rejects++;
+        two_loop(false);
     }
-    if (h0 && h0_active) h0(h0_ctx, q, n, st);     // learned initial inverse Hessian
-    else hipLaunchKernelGGL(mul_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)diag, q);
-    for (int i = 1; i <= bound; i++) {
-        const double yr = dot(Yv(cp), q);
-        double beta = rho[cp] * yr;
-        beta = alpha[cp] - beta;
-        axpby(beta, Sv(cp), 1.0, q);
-        cp = cp + 1;
-        if (cp == m) cp = 0;
-    }
-    scale_copy(1.0, q, Sv(point));
 L165:
     nfev = 0;
     stp = 1.0;
@@ -407,6 +440,7 @@ int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, co
         HIPCHK(hipMemcpyAsync(xdsave, d.x, n * sizeof(double), hipMemcpyDeviceToDevice, d.st));
         if (hooks) hooks->before_step(d, fd);
         d.step(fd, &iflag, &myinfo, EPS, XTOL);
+        if (hooks) iflag = hooks->after_step(d, iflag);
         hipLaunchKernelGGL(to_float_kernel, vg, vb, 0, d.st, n, (const double *)d.x, xf);
         HIPCHK(hipMemcpyAsync(x, xf, n * sizeof(float), hipMemcpyDeviceToHost, d.st));
         HIPCHK(hipStreamSynchronize(d.st));
@@ -433,6 +467,8 @@ int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, co
         rep->iterations = count;
         rep->evaluations = k;
         rep->iflag = iflag;
+        rep->info = d.info;
+        if (hooks) hooks->report(rep);
         rep->seconds = secs();
         rep->nhist = std::min(k, rep->hist_cap);
     }

@@ -15,6 +15,12 @@ struct LbfgsHooks {
     virtual void attach(LbfgsDev &d) {}
     virtual void before_step(LbfgsDev &d, double fscaled) {}
     virtual void accepted(LbfgsDev &d, int iteration) {}
+    // after each LBFGS call; may replace the IFLAG (e.g. restart a rejected search)
+    virtual int after_step(LbfgsDev &d, int iflag) { return iflag; }
+    virtual void report(sfd_lbfgs_report *rep) {}
+    // compute_fct_gdt's illumination at the newest evaluated model (host, n values)
+    virtual bool wants_illumination() const { return false; }
+    virtual void set_illumination(const float *illum, long n) {}
 };
 
 // Inversion.f90:29-189 l_bfgs; returns the final IFLAG.

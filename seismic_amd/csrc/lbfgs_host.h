@@ -10,7 +10,6 @@
 #include <vector>
 
 #include "common.h"
-#include "lbfgs_dev.h"
 #include "lbfgs_api.h"
 
 struct LbfgsDev {
@@ -36,6 +35,8 @@ struct LbfgsDev {
     void (*h0)(void *ctx, double *q, long n, hipStream_t st) = nullptr;
     void *h0_ctx = nullptr;
     bool h0_active = false;
+    bool used_h0 = false;      // the current search direction came from the learned H0
+    int h0_rejects = 0;        // learned directions replaced because they were not descent directions
     // optional observer of every accepted (gradient, step) pair
     void (*on_pair)(void *ctx, const double *gk, const double *sk, long n, hipStream_t st) = nullptr;
     void *pair_ctx = nullptr;
@@ -52,5 +53,8 @@ struct LbfgsDev {
     // One call of LBFGS(N,M,X,F,G,...) on the device vectors x, g.
     void step(double f, int *iflag, int *myinfo, double eps, double xtol);
     void mcsrch(double f, double *s);
+    void two_loop(bool use_h0);
+    int restart_plain();
+    void mul_diag(double *v);   // v *= DIAG
 };
 

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,7 @@
 #include "fwi_kernels.h"
 #include "engine.h"
 #include "common.h"
+#include "cnn_api.h"
 #include "lbfgs_api.h"
 
 using namespace sfd;
@@ -979,6 +981,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     // mod_copy_image into the full model, then the post-processing chain
     float *d_grad = (float *)(fb + o_grad), *d_illu = (float *)(fb + o_illu), *d_tmp = (float *)(fb + o_tmp);
     scatter_images(B, plans, d_grad, d_illu);
+    if (raw_illum_host)   // the summed illumination before illupow and normalisation
+        HIPCHK(hipMemcpyAsync(raw_illum_host, d_illu, N * 4, hipMemcpyDeviceToHost, stream));
     std::vector<double> ft(ntraces);
     HIPCHK(hipMemcpyAsync(ft.data(), fb + o_ft, sizeof(double) * ntraces, hipMemcpyDeviceToHost, stream));
     double scaling = 0;
@@ -1187,18 +1191,24 @@ int sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const floa
 // FWI inversion: Inversion.f90 l_bfgs with compute_fct_gdt as fctgdt
 // (progs/afwi: the model is set, then misfit and gradient are computed).
 int sfd_fwi_lbfgs(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs, const float *mute,
-                  const float *vpmask, const sfd_fwi_opts *fo, const sfd_lbfgs_opts *lo, float *x,
-                  sfd_lbfgs_report *rep) {
+                  const float *vpmask, const sfd_fwi_opts *fo, const sfd_lbfgs_opts *lo,
+                  const sfd_precond_opts *pc, float *x, sfd_lbfgs_report *rep) {
     if (!lo || !fo || !x) return SFD_EINVAL;
     return guarded(h, [&](Engine &e) {
         const long n = (long)e.g.nz * e.g.nx;
         std::vector<double> vpinit(x, x + n);
+        std::unique_ptr<LbfgsHooks> hooks;
+        if (pc && pc->enable) hooks.reset(make_cnn_precond(e.g.nz, e.g.nx, *pc));
+        std::vector<float> il(hooks && hooks->wants_illumination() ? n : 0);
         LbfgsFct f = [&](const float *xx, float *g, double *fv) {
             e.set_model(xx, nullptr);
+            e.raw_illum_host = il.empty() ? nullptr : il.data();
             e.fwi_gradient(shots, nshots, dobs, mute, vpmask, fo, fv, g, nullptr, nullptr);
+            e.raw_illum_host = nullptr;
+            if (!il.empty()) hooks->set_illumination(il.data(), n);
             return 0;
         };
-        lbfgs_drive(*lo, n, x, vpmask, vpinit.data(), f, rep, nullptr);
+        lbfgs_drive(*lo, n, x, vpmask, vpinit.data(), f, rep, hooks.get());
     });
 }
 

@@ -177,9 +177,11 @@ class Engine:
 
     def fwi_lbfgs(self, shots, dobs, x0, niter=10, neval=None, vpmin=1500.0, vpmax=4500.0, bound=2,
                   freeze_soft=True, m=5, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
-                  do_x=0, snap_mode=abi.SFD_SNAP_FP32):
+                  do_x=0, snap_mode=abi.SFD_SNAP_FP32, precond=None):
         """FWI by Inversion.f90 l_bfgs with compute_fct_gdt as the evaluation
-        (defaults of Inversion_types.f90:48-57).  Returns (model, report)."""
+        (defaults of Inversion_types.f90:48-57).  precond: None for plain
+        L-BFGS, or a dict of sfd_precond_opts fields (enable is implied) for
+        the learned CNN preconditioner.  Returns (model, report)."""
         nz, nx = self.g.nz, self.g.nx
         ob = self._cat(dobs, shots)
         mu = None if mute is None else self._cat(mute, shots)
@@ -187,10 +189,12 @@ class Engine:
         fo = sfd_fwi_opts()
         fo.illupow, fo.rect_z, fo.rect_x, fo.do_z, fo.do_x, fo.snap_mode = illupow, rect_z, rect_x, do_z, do_x, snap_mode
         lo = lbfgs_opts(niter, neval, m, bound, freeze_soft, vpmin, vpmax)
+        pc = precond_opts(**(precond or {})) if precond is not None else None
         x = _model_buf(x0, nz, nx).copy()
         rep, keep = _report(lo.neval)
         self._chk(self.lib.sfd_fwi_lbfgs(self.h, self._shots(shots), len(shots), _ptr(ob), _ptr(mu), _ptr(vm),
-                                         C.byref(fo), C.byref(lo), _ptr(x), C.byref(rep)))
+                                         C.byref(fo), C.byref(lo), None if pc is None else C.byref(pc), _ptr(x),
+                                         C.byref(rep)))
         return x.reshape(nx, nz).T.copy(), _report_dict(rep, keep)
 
     def set_option(self, opt, value):
@@ -227,6 +231,15 @@ def lbfgs_opts(niter=10, neval=None, m=5, bound=2, freeze_soft=True, vpmin=1500.
     return o
 
 
+def precond_opts(train_after=3, train_steps=300, lr=2e-3, seed=1, max_evals=4, max_fallbacks=3, enable=1,
+                 target=0, use_net=1, illum_h0=0):
+    o = abi.sfd_precond_opts()
+    o.target, o.use_net, o.illum_h0 = int(target), int(use_net), int(illum_h0)
+    o.enable, o.train_after, o.train_steps, o.lr = int(enable), int(train_after), int(train_steps), float(lr)
+    o.seed, o.max_evals, o.max_fallbacks = int(seed), int(max_evals), int(max_fallbacks)
+    return o
+
+
 def _report(cap):
     keep = dict(f=np.zeros(cap), t=np.zeros(cap), a=np.zeros(cap, np.int32))
     r = abi.sfd_lbfgs_report()
@@ -239,7 +252,9 @@ def _report(cap):
 
 def _report_dict(r, keep):
     k = r.nhist
-    return dict(iterations=r.iterations, evaluations=r.evaluations, iflag=r.iflag, seconds=r.seconds,
+    return dict(iterations=r.iterations, evaluations=r.evaluations, iflag=r.iflag, info=r.info, seconds=r.seconds,
+                pc_iterations=r.pc_iterations, pc_fallbacks=r.pc_fallbacks, pc_rejects=r.pc_rejects,
+                pc_train_seconds=r.pc_train_seconds, pc_loss=(r.pc_loss_first, r.pc_loss_last),
                 f_eval=keep["f"][:k].copy(), t_eval=keep["t"][:k].copy(), accepted=keep["a"][:k].astype(bool))
 
 
@@ -275,6 +290,70 @@ def lbfgs_run(fct, x0, niter=10, neval=None, m=5, bound=2, freeze_soft=True, xmi
     if rc != 0:
         raise SeisFDError(rc, "sfd_lbfgs_run failed")
     return x, _report_dict(rep, keep)
+
+
+class CNN:
+    """The preconditioner's network through sfd_cnn_* (parameters: FP32 vector
+    of SFD_CNN_NPARAM, layer l weights W[t][co][ci] then 16 biases)."""
+
+    def __init__(self, nz, nx, nimg_max=4, seed=1):
+        self.lib = abi.load()
+        self.nz, self.nx, self.nimg_max = int(nz), int(nx), int(nimg_max)
+        h = C.c_void_p()
+        rc = self.lib.sfd_cnn_create(C.byref(h), self.nz, self.nx, self.nimg_max, int(seed))
+        if rc != 0:
+            raise SeisFDError(rc, "sfd_cnn_create failed")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise SeisFDError(rc, what)
+
+    def _imgs(self, x):
+        x = np.asarray(x, np.float32)
+        n = x.size // (self.nz * self.nx)
+        # (nimg, nz, nx) -> column-major images stacked
+        return np.ascontiguousarray(np.stack([np.asfortranarray(im).ravel(order="F") for im in
+                                              x.reshape(n, self.nz, self.nx)])), n
+
+    def params(self):
+        th = np.zeros(abi.SFD_CNN_NPARAM, np.float32)
+        self._chk(self.lib.sfd_cnn_get_params(self.h, _ptr(th)), "sfd_cnn_get_params")
+        return th
+
+    def set_params(self, th):
+        th = np.ascontiguousarray(th, np.float32)
+        self._chk(self.lib.sfd_cnn_set_params(self.h, _ptr(th)), "sfd_cnn_set_params")
+
+    def forward(self, x):
+        xs, n = self._imgs(x)
+        y = np.zeros_like(xs)
+        self._chk(self.lib.sfd_cnn_forward(self.h, n, _ptr(xs), _ptr(y)), "sfd_cnn_forward")
+        return np.stack([im.reshape(self.nx, self.nz).T for im in y])
+
+    def loss_grad(self, x, t):
+        xs, n = self._imgs(x)
+        ts, _ = self._imgs(t)
+        loss = C.c_double()
+        g = np.zeros(abi.SFD_CNN_NPARAM, np.float32)
+        self._chk(self.lib.sfd_cnn_loss_grad(self.h, n, _ptr(xs), _ptr(ts), C.byref(loss), _ptr(g)), "loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, steps, lr):
+        xs, n = self._imgs(x)
+        ts, _ = self._imgs(t)
+        l0, l1 = C.c_double(), C.c_double()
+        self._chk(self.lib.sfd_cnn_train(self.h, n, _ptr(xs), _ptr(ts), int(steps), float(lr), C.byref(l0),
+                                         C.byref(l1)), "sfd_cnn_train")
+        return l0.value, l1.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.sfd_cnn_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 class LBFGS:

@@ -41,10 +41,18 @@ module seisfd_mod
   end type sfd_lbfgs_opts
 
   type, bind(C) :: sfd_lbfgs_report
-     integer(c_int) :: iterations, evaluations, iflag, nhist, hist_cap
+     integer(c_int) :: iterations, evaluations, iflag, info, nhist, hist_cap
      real(c_double) :: seconds
      type(c_ptr)    :: f_eval, t_eval, accepted
+     integer(c_int) :: pc_iterations, pc_fallbacks, pc_rejects
+     real(c_double) :: pc_train_seconds, pc_loss_first, pc_loss_last
   end type sfd_lbfgs_report
+
+  type, bind(C) :: sfd_precond_opts
+     integer(c_int) :: enable, train_after, train_steps
+     real(c_float)  :: lr
+     integer(c_int) :: seed, max_evals, max_fallbacks, target, use_net, illum_h0
+  end type sfd_precond_opts
 
   interface
      integer(c_int) function sfd_create(h, g) bind(C, name='sfd_create')
@@ -130,7 +138,7 @@ module seisfd_mod
        type(sfd_lbfgs_report), intent(inout) :: rep
      end function sfd_lbfgs_run
 
-     integer(c_int) function sfd_fwi_lbfgs(h, shots, nshots, dobs, mute, vpmask, fo, lo, x, rep) &
+     integer(c_int) function sfd_fwi_lbfgs(h, shots, nshots, dobs, mute, vpmask, fo, lo, pc, x, rep) &
           bind(C, name='sfd_fwi_lbfgs')
        import :: c_int, c_ptr, c_float, sfd_shot, sfd_fwi_opts, sfd_lbfgs_opts, sfd_lbfgs_report
        type(c_ptr), value :: h
@@ -140,6 +148,7 @@ module seisfd_mod
        type(c_ptr), value :: mute, vpmask
        type(sfd_fwi_opts), intent(in) :: fo
        type(sfd_lbfgs_opts), intent(in) :: lo
+       type(c_ptr), value :: pc     ! c_loc of a sfd_precond_opts, or c_null_ptr for plain L-BFGS
        real(c_float), intent(inout) :: x(*)
        type(sfd_lbfgs_report), intent(inout) :: rep
      end function sfd_fwi_lbfgs
