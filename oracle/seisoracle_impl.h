@@ -579,6 +579,154 @@ int FN(modeling)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const RE
     return 0;
 }
 
+/* Exact discrete adjoint of FN(modeling) (no reference counterpart): the
+ * transpose of the linear map wavelet -> traces of the time loop above
+ * (sinc injection with gain v^2, forward stencil, 5-level update, Higdon
+ * bands, extraction of any kind).  Cotangents L_k of the levels U_k are
+ * accumulated backwards in time; each Higdon overwrite u3(i) <- -(sum g u)
+ * becomes "a = G(i); G(i) = 0; every input += -g a" in reverse order.
+ * dbar: (nt, ntr) time fastest; sbar: nt samples. */
+static void CAT(adj_higdon_, SFX)(PROP *P, REAL *L1, REAL *L2, REAL *G, int i, int j, int axis)
+{
+    const ORC_BOUNDS *b = &P->b;
+    const REAL *g;
+    int di = 0, dj = 0;
+    switch (axis) {
+    case -1: g = P->gz + 16 * (j - b->nmin2); di = 1; break;
+    case 1: g = P->gz + 16 * (j - b->nmin2) + 8; di = -1; break;
+    case -2: g = P->gx + 16 * (i - b->nmin1); dj = 1; break;
+    default: g = P->gx + 16 * (i - b->nmin1) + 8; dj = -1; break;
+    }
+    const REAL a = AT(P, G, i, j);
+    AT(P, G, i, j) = 0;
+    AT(P, G, i + di, j + dj) = AT(P, G, i + di, j + dj) - g[0] * a;
+    AT(P, G, i + 2 * di, j + 2 * dj) = AT(P, G, i + 2 * di, j + 2 * dj) - g[1] * a;
+    AT(P, L2, i, j) = AT(P, L2, i, j) - g[2] * a;
+    AT(P, L2, i + di, j + dj) = AT(P, L2, i + di, j + dj) - g[3] * a;
+    AT(P, L2, i + 2 * di, j + 2 * dj) = AT(P, L2, i + 2 * di, j + 2 * dj) - g[4] * a;
+    AT(P, L1, i, j) = AT(P, L1, i, j) - g[5] * a;
+    AT(P, L1, i + di, j + dj) = AT(P, L1, i + di, j + dj) - g[6] * a;
+    AT(P, L1, i + 2 * di, j + 2 * dj) = AT(P, L1, i + 2 * di, j + 2 * dj) - g[7] * a;
+}
+
+int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *dbar, REAL *sbar,
+                         int extract_kind)
+{
+    PROP P;
+    float zz = s->sz, xx = s->sx;
+    if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
+    const ORC_BOUNDS *b = &P.b;
+    const size_t n = (size_t)b->NZ * b->NX;
+    const int l = P.lsinc, minx = (int)(-l * 0.5), maxx = (int)(l * 0.5);
+    const REAL div11 = (REAL)1. / (REAL)11, dt2 = P.dt * P.dt;
+    REAL *buf[5], *G, *W, sz[64], sx[64], wz[64], wx[64];
+    int k, i, j, it, r;
+    P.nsou = 1;
+    P.sou = CAT(make_traces_, SFX)(1, &zz, &xx, s->nt);
+    P.ndat = s->ntr;
+    P.dat = CAT(make_traces_, SFX)(s->ntr, s->rz, s->rx, s->nt);
+    CAT(init_coefs_, SFX)(&P);
+    CAT(higdon_tables_, SFX)(&P);
+    for (int a = 0; a < 2; a++)
+        CAT(find_i_d_, SFX)(&P.sou[0].icoord[a], &P.sou[0].dcoord[a], P.sou[0].coord[a],
+                            a == 0 ? P.oz : P.oxw, a == 0 ? P.dz : P.dx);
+    for (r = 0; r < P.ndat; r++)
+        for (int a = 0; a < 2; a++)
+            CAT(find_i_d_, SFX)(&P.dat[r].icoord[a], &P.dat[r].dcoord[a], P.dat[r].coord[a],
+                                a == 0 ? P.oz : P.oxw, a == 0 ? P.dz : P.dx);
+    for (k = 0; k < 5; k++) buf[k] = (REAL *)calloc(n, sizeof(REAL));
+    W = (REAL *)calloc(n, sizeof(REAL));
+    /* buf[0] = L_{k+1} (becomes G), buf[1..4] = L_k .. L_{k-3} */
+    TR *so = &P.sou[0];
+    REAL vs = AT(&P, P.vel, so->icoord[0], so->icoord[1]), v2s = vs * vs;
+    FN(mksinc)(sz, l, so->dcoord[0] / P.dz);
+    FN(mksinc)(sx, l, so->dcoord[1] / P.dx);
+    for (it = 0; it < s->nt; it++) sbar[it] = 0;
+    for (k = s->nt; k >= 1; k--) {
+        G = buf[0];
+        /* extraction at step k+1 read U_{k+1}: G += E^T dbar(k+1) */
+        if (k + 1 <= s->nt) {
+            for (r = 0; r < P.ndat; r++) {
+                TR *d = &P.dat[r];
+                REAL db = dbar[(size_t)r * s->nt + k], gg;
+                if (extract_kind == SFD_EXTRACT_NEAREST) {
+                    AT(&P, G, d->icoord[0], d->icoord[1]) = AT(&P, G, d->icoord[0], d->icoord[1]) + db;
+                    continue;
+                }
+                if (extract_kind == SFD_EXTRACT_SINC_V2) {
+                    REAL v = AT(&P, P.vel, d->icoord[0], d->icoord[1]);
+                    gg = v * v;
+                } else if (extract_kind == SFD_EXTRACT_SINC_LSRTM) {
+                    gg = (REAL)1 / (P.dz * P.dx);
+                } else {
+                    gg = 1;
+                }
+                FN(mksinc)(wz, l, d->dcoord[0] / P.dz);
+                FN(mksinc)(wx, l, d->dcoord[1] / P.dx);
+                for (j = minx; j <= maxx; j++)
+                    for (i = minx; i <= maxx; i++)
+                        AT(&P, G, i + d->icoord[0], j + d->icoord[1]) =
+                            AT(&P, G, i + d->icoord[0], j + d->icoord[1]) + gg * db * wz[maxx + i] * wx[maxx + j];
+            }
+        }
+        /* Higdon of step k (z-bands, then x-bands), reversed: x-bands first */
+        for (j = b->nmax2 + 4; j >= b->nmax2 - 3; j--)
+            for (i = b->nmin1; i <= b->nmax1; i++) CAT(adj_higdon_, SFX)(&P, buf[2], buf[1], G, i, j, 2);
+        for (j = b->nmin2 - 4; j <= b->nmin2 + 3; j++)
+            for (i = b->nmin1; i <= b->nmax1; i++) CAT(adj_higdon_, SFX)(&P, buf[2], buf[1], G, i, j, -2);
+        for (j = b->nmin2; j <= b->nmax2; j++) {
+            for (i = b->nmax1 + 4; i >= b->nmax1 - 3; i--) CAT(adj_higdon_, SFX)(&P, buf[2], buf[1], G, i, j, 1);
+            for (i = b->nmin1 - 4; i <= b->nmin1 + 3; i++) CAT(adj_higdon_, SFX)(&P, buf[2], buf[1], G, i, j, -1);
+        }
+        /* time update of step k: U_{k+1} = (20U_k - 6U_{k-1} - 4U_{k-2} + U_{k-3} + 12dt^2 (v^2 lap U_k + inj))/11 */
+        memset(buf[4], 0, n * sizeof(REAL));   /* L_{k-3}: its first contribution is this step's */
+        memset(W, 0, n * sizeof(REAL));
+        for (j = b->nmin2; j <= b->nmax2; j++)
+            for (i = b->nmin1; i <= b->nmax1; i++) {
+                size_t q = ORC_IDX(b, i, j);
+                REAL gq = G[q];
+                buf[1][q] = buf[1][q] + (REAL)20. * div11 * gq;
+                buf[2][q] = buf[2][q] - (REAL)6. * div11 * gq;
+                buf[3][q] = buf[3][q] - (REAL)4. * div11 * gq;
+                buf[4][q] = buf[4][q] + div11 * gq;
+                REAL v = AT(&P, P.vel, i, j);
+                W[q] = (REAL)12. * dt2 * div11 * gq * v * v;
+            }
+        /* the stencil's transpose: L_k(c') += sum over FD cells c of c_tap W(c) */
+        for (j = b->nmin2; j <= b->nmax2; j++)
+            for (i = b->nmin1; i <= b->nmax1; i++) {
+                REAL w = AT(&P, W, i, j);
+                if (w == 0) continue;
+                AT(&P, buf[1], i, j) = AT(&P, buf[1], i, j) + (P.c[0] + P.c[5]) * w;
+                for (int t = 1; t <= 4; t++) {
+                    AT(&P, buf[1], i + t, j) = AT(&P, buf[1], i + t, j) + P.c[t] * w;
+                    AT(&P, buf[1], i - t, j) = AT(&P, buf[1], i - t, j) + P.c[t] * w;
+                    AT(&P, buf[1], i, j + t) = AT(&P, buf[1], i, j + t) + P.c[5 + t] * w;
+                    AT(&P, buf[1], i, j - t) = AT(&P, buf[1], i, j - t) + P.c[5 + t] * w;
+                }
+            }
+        /* the injection's transpose: sbar(k) = 12 dt^2/11 sum v2r wz wx G */
+        {
+            REAL acc = 0;
+            for (j = minx; j <= maxx; j++)
+                for (i = minx; i <= maxx; i++)
+                    acc = acc + v2s * sz[maxx + i] * sx[maxx + j] *
+                                    AT(&P, G, i + so->icoord[0], j + so->icoord[1]);
+            sbar[k - 1] = (REAL)12. * dt2 * div11 * acc;
+        }
+        /* rotate: L_k becomes the next step's L_{k+1} */
+        REAL *tmp = buf[0];
+        buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = tmp;
+    }
+    for (k = 0; k < 5; k++) free(buf[k]);
+    free(W);
+    free(P.gz); free(P.gx);
+    CAT(free_traces_, SFX)(P.sou, 1);
+    CAT(free_traces_, SFX)(P.dat, s->ntr);
+    free(P.vel);
+    return 0;
+}
+
 /* progs/rtm/to_disk_to_memory_RTM.f90:358-492 RTM_to_memory (lsrtm=0) and
  * :494-579 LSRTM_to_memory (lsrtm=1), then mod_copy_image into image/illum
  * (full model, accumulated with +=).  vel_bwd may be NULL.  capture

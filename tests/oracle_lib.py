@@ -374,3 +374,18 @@ def l_bfgs(fct, x0, niter=10, neval=None, m=5, bound=2, freeze_soft=True, xmin=1
                          ptr(vi), cf, None, ptr(fe), ptr(acc), neval, C.byref(ne), C.byref(ni))
     k = ne.value
     return x, dict(f_eval=fe[:k], accepted=acc[:k].astype(bool), evaluations=k, iterations=ni.value, iflag=iflag)
+
+
+def modeling_adjoint(g, vel, shot, dbar, extract_kind=0, prec="d"):
+    """Exact adjoint of modeling(): sbar = F^T dbar (nt samples)."""
+    L = oracle()
+    T = _dt(prec)
+    v = colmajor(vel, T)
+    fn = getattr(L, "orc_%s_modeling_adjoint" % prec)
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    db = np.ascontiguousarray(np.asarray(dbar, dtype=T).reshape(shot.ntr, shot.nt))
+    sb = np.zeros(shot.nt, dtype=T)
+    cs = shot.c()
+    rc = fn(C.byref(g), ptr(v), C.byref(cs), ptr(db), ptr(sb), extract_kind)
+    assert rc == 0
+    return sb
