@@ -134,6 +134,19 @@ int  sfd_rtm(sfd_handle h, const sfd_shot *shots, int nshots,
 int  sfd_born(sfd_handle h, const sfd_shot *shots, int nshots,
               const float *refl, float *traces_out);
 
+/*
+ * Exact discrete adjoint of sfd_forward (no reference counterpart: the
+ * reference's reverse-time pass, as in sfd_rtm, runs the adjoint-style
+ * stencil forward-in-reverse with the absorbing boundary re-applied, which is
+ * not the transpose of the forward operator).  With F the linear map from a
+ * shot's wavelet to its traces (sinc injection with gain v^2, 5-level update,
+ * Higdon bands, extraction of extract_kind), sbar = F^T dbar: dbar (nt, ntr)
+ * per shot concatenated as sfd_forward writes traces, sbar nt per shot.
+ * <F s, d> = <s, F^T d> holds to FP32 rounding (the dot-product test).
+ */
+int  sfd_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots,
+                       const float *dbar, float *sbar, int extract_kind);
+
 /* LSRTM imaging (to_disk_to_memory_RTM.f90:494-579, Imaging.f90:698-745). */
 int  sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots,
                const float *traces_in, float *image, float *illum);
@@ -249,7 +262,8 @@ typedef struct {
     int   train_steps;     /* Adam steps (300)                                 */
     float lr;              /* Adam learning rate (2e-3)                        */
     int   seed;            /* weight initialisation                            */
-    int   max_evals;       /* evaluations allowed per preconditioned search (4) */
+    int   max_evals;       /* evaluations allowed per preconditioned search (4);
+                              0 abandons every preconditioned search at once */
     int   max_fallbacks;   /* restarts before the network is switched off (3)  */
     int   target;          /* training target for gradient g_k: 0 the accepted
                               step s_k of that iteration; 1 the accepted model

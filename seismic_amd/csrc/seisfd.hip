@@ -20,6 +20,7 @@
 #include "fd_kernels.h"
 #include "fwi_kernels.h"
 #include "engine.h"
+#include "adjoint_kernels.h"
 #include "common.h"
 #include "cnn_api.h"
 #include "lbfgs_api.h"
@@ -235,11 +236,13 @@ void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc,
         float a;
         if (kind == 3) {
             a = 1.f / (g.dz * g.dx);
+        } else if (kind == 4 || kind == 5) {
+            a = 1.f;
         } else {
             const float v = padded_vel(vh, p, icz, icx);
             a = v * v;
         }
-        if (kind == 2) {
+        if (kind == 2 || kind == 5) {
             cs.push_back({iz, ix, k, a, 1.f, 1.f});
             continue;
         }
@@ -742,6 +745,94 @@ void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image,
     }
 }
 
+// ---- exact discrete adjoint of forward modeling (no reference counterpart) --
+// sbar = F^T dbar per shot, F the map wavelet -> traces of Engine::forward
+// with the given extraction.  Receivers become sources of the transposed
+// extraction (injection plan kinds 1, 5, 3, 4 for SINC_V2, NEAREST,
+// SINC_LSRTM, SINC_UNIT); three launches per step (adjoint_kernels.h).
+void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind) {
+    check_shots(shots, ns);
+    if (!dbar || !sbar) throw EngineError(SFD_EINVAL, "NULL dbar/sbar");
+    if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract_kind");
+    if (!(tz == 64 && (tx == 32 || tx == 64))) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
+    static const int kind_of[4] = {1, 5, 3, 4};
+    reset_kstats();
+    size_t off = 0, sboff = 0;
+    for (int s = 0; s < ns; s++) {
+        const sfd_shot &sh = shots[s];
+        std::vector<ShotPlan> plans(1);
+        ShotPlan &p = plans[0];
+        plan_shot(sh, p);
+        plan_injection(p, vel_h, sh.ntr, sh.rz, sh.rx, kind_of[extract_kind]);
+        p.n_inj_traces = std::max(1, sh.ntr);
+        BatchNeeds need;
+        Batch &B = fwd_batch;
+        build_batch(B, plans, need);
+        upload_inj_traces(B, 0, dbar + off, (size_t)sh.nt * sh.ntr);
+        prepare_models(B, plans, d_vel);
+        // the source footprint of Injection_sinc (j outer, i inner)
+        int icz, icx;
+        float dcz, dcx;
+        find_i_d(&icz, &dcz, sh.sz, g.oz, g.dz);
+        find_i_d(&icx, &dcx, sh.sx, p.oxw, g.dx);
+        const int iz0 = icz - 1 + p.i0, ix0 = icx - 1 + p.i0;
+        const float v = padded_vel(vel_h, p, icz, icx), v2s = v * v;
+        float wz[7], wx[7];
+        sfd_host_mksinc(wz, 7, dcz / g.dz);
+        sfd_host_mksinc(wx, 7, dcx / g.dx);
+        std::vector<int> addr;
+        std::vector<float> coef;
+        for (int j = -3; j <= 3; j++)
+            for (int i = -3; i <= 3; i++) {
+                addr.push_back((ix0 + j) * p.LDZ + p.zoff + iz0 + i);
+                coef.push_back(v2s * wz[3 + i] * wx[3 + j]);
+            }
+        const size_t nf = (size_t)p.LDZ * p.NX;
+        Layout L;
+        const size_t o_lev = L.take(6 * nf * 4), o_addr = L.take(addr.size() * 4), o_coef = L.take(coef.size() * 4),
+                     o_sb = L.take((size_t)sh.nt * 4);
+        scratch.reserve(L.off);
+        char *base = (char *)scratch.p;
+        HIPCHK(hipMemsetAsync(base + o_lev, 0, 6 * nf * 4, stream));
+        HIPCHK(hipMemcpyAsync(base + o_addr, addr.data(), addr.size() * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(base + o_coef, coef.data(), coef.size() * 4, hipMemcpyHostToDevice, stream));
+        const StepArgs SA = base_args(B);
+        AdjStep A;
+        memset(&A, 0, sizeof A);
+        A.S = B.shots_h[0];
+        A.nsrc = (int)addr.size();
+        A.src_addr = (const int *)(base + o_addr);
+        A.src_coef = (const float *)(base + o_coef);
+        A.sbar = (float *)(base + o_sb);
+        for (int q = 0; q < 10; q++) A.c[q] = SA.c[q];
+        A.c12dt2 = SA.c12dt2;
+        A.div11 = SA.div11;
+        float *lev[5];
+        for (int q = 0; q < 5; q++) lev[q] = (float *)(base + o_lev) + q * nf;
+        A.W = (float *)(base + o_lev) + 5 * nf;
+        const int ntile = p.ntz * p.ntx;
+        const unsigned ucells = (unsigned)(((size_t)p.NZ * p.NX + 255) / 256);
+        for (int k = sh.nt; k >= 1; k--) {
+            A.G = lev[0]; A.L0 = lev[1]; A.L1 = lev[2]; A.L2 = lev[3]; A.L3 = lev[4];
+            A.k = k;
+            if (tx == 32)
+                hipLaunchKernelGGL((k7_gamma_kernel<64, 32>), dim3(ntile), dim3(256), 0, stream, A);
+            else
+                hipLaunchKernelGGL((k7_gamma_kernel<64, 64>), dim3(ntile), dim3(256), 0, stream, A);
+            hipLaunchKernelGGL(k7_update_kernel, dim3(ucells), dim3(256), 0, stream, A);
+            hipLaunchKernelGGL(k7_source_kernel, dim3(1), dim3(64), 0, stream, A);
+            float *tmp = lev[0];
+            lev[0] = lev[1]; lev[1] = lev[2]; lev[2] = lev[3]; lev[3] = lev[4]; lev[4] = tmp;
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(sbar + sboff, base + o_sb, (size_t)sh.nt * 4,
+                              hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        off += (size_t)sh.nt * sh.ntr;
+        sboff += (size_t)sh.nt;
+    }
+}
+
 // ---- RTM / LSRTM (to_disk_to_memory_RTM.f90:358-579) -------------------------
 void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
                  bool lsrtm) {
@@ -1210,6 +1301,11 @@ int sfd_fwi_lbfgs(sfd_handle h, const sfd_shot *shots, int nshots, const float *
         };
         lbfgs_drive(*lo, n, x, vpmask, vpinit.data(), f, rep, hooks.get());
     });
+}
+
+int sfd_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots, const float *dbar, float *sbar,
+                      int extract_kind) {
+    return guarded(h, [&](Engine &e) { e.adjoint_exact(shots, nshots, dbar, sbar, extract_kind); });
 }
 
 int sfd_last_timing(sfd_handle h, double *ms, double *pts) {

@@ -197,6 +197,18 @@ class Engine:
                                          C.byref(rep)))
         return x.reshape(nx, nz).T.copy(), _report_dict(rep, keep)
 
+    def adjoint_exact(self, shots, dbar, extract_kind=abi.SFD_EXTRACT_SINC_V2):
+        """F^T dbar for each shot (F: wavelet -> traces of forward()); returns a list of nt arrays."""
+        db = self._cat(dbar, shots)
+        sb = np.zeros(sum(s.nt for s in shots), np.float32)
+        self._chk(self.lib.sfd_adjoint_exact(self.h, self._shots(shots), len(shots), _ptr(db), _ptr(sb),
+                                             extract_kind))
+        out, o = [], 0
+        for s in shots:
+            out.append(sb[o:o + s.nt].copy())
+            o += s.nt
+        return out
+
     def set_option(self, opt, value):
         self._chk(self.lib.sfd_set_option(self.h, opt, int(value)))
 

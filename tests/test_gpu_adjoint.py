@@ -1,0 +1,76 @@
+"""Exact discrete adjoint (sfd_adjoint_exact) on the GPU: against the
+oracle's adjoint (itself pinned by FP64 dot-product tests at 1e-13), and the
+dot-product test <F s, d> = <s, F^T d> with F = sfd_forward, in FP32.
+
+Two measures of the FP32 dot-test mismatch D = <F s, d> - <s, F^T d>:
+  rel_cs = |D| / (|F s| |d|)      (scale of the vectors; <= 1e-6 asserted)
+  rel    = |D| / |<F s, d>|       (random d makes <F s, d> a cancelling sum,
+                                   so this grows with the cancellation; <= 1e-4)"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from models import smooth_random, layered_anomaly, ricker
+
+sa = pytest.importorskip("seismic_amd")
+pytestmark = pytest.mark.gpu
+
+
+def dot_test(d, db, s, sb):
+    lhs = float(np.dot(d.ravel().astype(np.float64), db.ravel()))
+    rhs = float(np.dot(s.astype(np.float64), sb))
+    den = float(np.linalg.norm(d.astype(np.float64)) * np.linalg.norm(db.astype(np.float64)))
+    return lhs, rhs, abs(lhs - rhs) / den, abs(lhs - rhs) / abs(lhs)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.mark.parametrize("ext", [0, 1, 2, 3])
+def test_adjoint_matches_oracle_and_dot_test(ext):
+    print("case: exact adjoint ext=%d" % ext, flush=True)
+    nz, nx, nt = 47, 83, 400
+    vel = smooth_random(nz, nx, seed=ext + 1, vmin=1500, vmax=2600)
+    rng = np.random.default_rng(ext)
+    rx = np.linspace(13.3, 10.0 * (nx - 1) - 7.7, 23).astype(np.float32)
+    rz = np.full_like(rx, 31.7)
+    s = rng.standard_normal(nt).astype(np.float32)
+    e = sa.Engine(sa.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12))
+    e.set_model(vel)
+    shot = sa.Shot(44.4, 333.3, s, rz, rx)
+    d = e.forward([shot], extract_kind=ext)[0]
+    db = rng.standard_normal(d.shape).astype(np.float32)
+    sb = e.adjoint_exact([shot], [db], extract_kind=ext)[0]
+    ref = O.modeling_adjoint(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel,
+                             O.Shot(44.4, 333.3, s, rz, rx), db, extract_kind=ext, prec="f")
+    lhs, rhs, rcs, rl = dot_test(d, db, s, sb)
+    print("vs oracle rel %.2e; dot test <Fs,d>=%.9e <s,F^T d>=%.9e rel_cs %.2e rel %.2e" % (
+        rel(sb, ref), lhs, rhs, rcs, rl), flush=True)
+    assert rel(sb, ref) <= 1e-5
+    assert rcs <= 1e-6 and rl <= 1e-4
+
+
+def test_adjoint_dot_test_config1():
+    """Config 1 geometry (201x401 at 10 m, 40-cell pad, 401 receivers, nt 2000)."""
+    print("case: exact adjoint dot test config1", flush=True)
+    nz, nx, nt = 201, 401, 2000
+    vel = layered_anomaly(nz, nx, 10.0, 10.0, seed=1)
+    rng = np.random.default_rng(11)
+    rx = (np.arange(nx) * 10.0).astype(np.float32)
+    rz = np.full_like(rx, 20.0)
+    s = ricker(15.0, nt, 1e-3)
+    e = sa.Engine(sa.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=40))
+    e.set_model(vel)
+    shot = sa.Shot(20.0, 2000.0, s, rz, rx)
+    d = e.forward([shot])[0]
+    db = rng.standard_normal(d.shape).astype(np.float32)
+    sb = e.adjoint_exact([shot], [db])[0]
+    lhs, rhs, rcs, rl = dot_test(d, db, s, sb)
+    print("<Fs,d>=%.9e <s,F^T d>=%.9e rel_cs %.2e rel %.2e" % (lhs, rhs, rcs, rl), flush=True)
+    assert rcs <= 1e-6 and rl <= 1e-4
+    # d = F s itself: no cancellation, <F s, F s> = <s, F^T F s>
+    sb2 = e.adjoint_exact([shot], [d])[0]
+    lhs, rhs, rcs, rl = dot_test(d, d, s, sb2)
+    print("normal: <Fs,Fs>=%.9e <s,F^T F s>=%.9e rel %.2e" % (lhs, rhs, rl), flush=True)
+    assert rl <= 1e-6
