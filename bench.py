@@ -85,6 +85,91 @@ def cpu_baseline(mig, w, rz, rx, budget_s=15.0):
             "sample": "C oracle, %d forward steps of the full FD grid, %.1f s" % (ns, t)}
 
 
+def fwi_leg(niter=20):
+    """Config 4 FWI: 201 x 601 at 10 m, 1500->3500 m/s gradient + 4 seeded
+    +/-250 m/s Gaussians, initial model = Gaussian sigma 200 m; 32 shots of a
+    5 Hz Ricker, 601 surface receivers, dt 1 ms, nt 3000; L-BFGS m = 5 for
+    `niter` iterations, plain and with the learned preconditioner.  Time to
+    misfit = wall seconds until the preconditioned run's accepted misfit is at
+    or below the plain run's final accepted misfit (null if never)."""
+    import seismic_amd as sa
+    from seismic_amd.synthetic import gradient_anomalies, gaussian_smooth, ricker
+    nz, nx, dx, dt, nt = 201, 601, 10.0, 1e-3, 3000
+    true = gradient_anomalies(nz, nx, dx, dx, seed=4)
+    init = gaussian_smooth(true, 200.0 / dx)
+    w = ricker(5.0, nt, dt)
+    rx = (np.arange(nx) * dx).astype(np.float32)
+    rz = np.zeros_like(rx)
+    shots = [sa.Shot(0.0, float(sx), w, rz, rx) for sx in np.linspace(0.0, dx * (nx - 1), 32)]
+    e = sa.Engine(sa.make_grid(nz, nx, dx, dx, dt, ntaper=40, snapi=4))
+    e.set_model(true)
+    dobs = e.forward(shots, extract_kind=sa.SFD_EXTRACT_NEAREST)
+    kw = dict(niter=niter, neval=3 * niter, bound=0, illupow=0.0)
+    _, rp = e.fwi_lbfgs(shots, dobs, init, **kw)
+    log("fwi plain: %d iterations, %.2f s" % (rp["iterations"], rp["seconds"]))
+    _, rc = e.fwi_lbfgs(shots, dobs, init, precond=dict(train_after=3, train_steps=300), **kw)
+    log("fwi preconditioned: %d iterations, %.2f s" % (rc["iterations"], rc["seconds"]))
+    fp = rp["f_eval"][rp["accepted"]]
+    fc, tc = rc["f_eval"][rc["accepted"]], rc["t_eval"][rc["accepted"]]
+    target = float(fp[-1])
+    hit = np.nonzero(fc <= target)[0]
+    t_pc = float(tc[hit[0]]) if hit.size else None
+    t_plain = float(rp["t_eval"][rp["accepted"]][-1])
+    return {"workload": "fwi-201x601-32shots-nt3000-lbfgs%d" % niter,
+            "plain": {"seconds": round(rp["seconds"], 3), "iterations": rp["iterations"],
+                      "evaluations": rp["evaluations"], "final_misfit_rel": float(fp[-1] / rp["f_eval"][0])},
+            "preconditioned": {"seconds": round(rc["seconds"], 3), "iterations": rc["iterations"],
+                               "evaluations": rc["evaluations"], "with_network": rc["pc_iterations"],
+                               "restarts": rc["pc_fallbacks"], "train_seconds": round(rc["pc_train_seconds"], 3),
+                               "final_misfit_rel": float(fc[-1] / rc["f_eval"][0])},
+            "time_to_plain_final_misfit_s": {"plain": round(t_plain, 3),
+                                             "preconditioned": None if t_pc is None else round(t_pc, 3)},
+            "speedup": None if t_pc is None else round(t_plain / t_pc, 3)}
+
+
+def batched_leg(seed=3):
+    """Config 3: 96 shots spread uniformly over the surface of the 751 x 2301
+    faulted model (4 m), each with a centred receiver spread of U{200..1200}
+    traces (clipped to the model) and nt in U{4000..7500} at dt 0.4 ms (seeded);
+    observed data from the true model, then one batched forward + adjoint
+    gradient call (compute_fct_gdt) on the smoothed model with FP16 snapshots
+    every 4 steps, tile shape chosen by the autotuner.  Gpts/s counts both
+    propagations of every shot over its own FD region and record length."""
+    import seismic_amd as sa
+    true, mig, _, _, _ = build_inputs(nt=16)
+    rng = np.random.default_rng(seed)
+    shots, pts = [], 0.0
+    for sx in np.linspace(0.0, DZ * (NX - 1), 96):
+        nr = int(rng.integers(200, 1201))
+        nt = int(rng.integers(4000, 7501))
+        j0 = int(round(sx / DZ)) - nr // 2
+        rx = (np.clip(np.arange(j0, j0 + nr), 0, NX - 1) * DZ).astype(np.float32)
+        rx = np.unique(rx)
+        from seismic_amd.synthetic import ricker
+        shots.append(sa.Shot(0.0, float(sx), ricker(20.0, nt, DT), np.zeros_like(rx), rx))
+    e = sa.Engine(sa.make_grid(NZ, NX, DZ, DZ, DT, ntaper=NTAPER, snapi=SNAPI))
+    e.set_model(true)
+    t0 = time.perf_counter()
+    dobs = e.forward(shots, extract_kind=sa.SFD_EXTRACT_NEAREST)
+    t_obs = time.perf_counter() - t0
+    pts_fwd = e.last_timing()["pts"]
+    e.set_model(mig)
+    e.set_tile(0, 0)
+    t0 = time.perf_counter()
+    f, grad, _, _ = e.fwi_gradient(shots, dobs, snap_mode=sa.SFD_SNAP_FP16S)
+    wall = time.perf_counter() - t0
+    tm, ks, tile = e.last_timing(), e.kernel_stats(), e.get_tile()
+    return {"workload": "batched-96-shots-751x2301-ragged", "shots": len(shots),
+            "receivers": [int(min(s.ntr for s in shots)), int(max(s.ntr for s in shots))],
+            "nt": [int(min(s.nt for s in shots)), int(max(s.nt for s in shots))],
+            "gradient_s": round(wall, 3), "Gpts_per_s": round(tm["pts"] / wall / 1e9, 3),
+            "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
+            "observed_data_s": round(t_obs, 3), "observed_data_Gpts_per_s": round(pts_fwd / t_obs / 1e9, 3),
+            "avg_launch_us": [round(ks["fwd_us"], 2), round(ks["bwd_us"], 2)],
+            "tile": list(tile["tile"]), "strip": tile["strip"], "autotune": tile["log"],
+            "snapshots": "fp16", "misfit": f}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +182,8 @@ def main():
     ap.add_argument("--strip", type=int, default=0, help="columns per wavefront (0 auto, 4, 8, 16)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--nt", type=int, default=NT, help="record length (shorter records for counter passes)")
+    ap.add_argument("--no-fwi", action="store_true", help="skip the config-4 FWI time-to-misfit leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the config-3 batched gradient leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the 512 MiB copy kernel (counter calibration) and exit")
     args = ap.parse_args()
@@ -166,6 +253,14 @@ def main():
                                     "achieved": round(ach_bwd, 1) if ach_bwd else None,
                                     "algorithmic_bytes_per_launch": bytes_bwd,
                                     "avg_launch_us": round(ks["bwd_us"], 3)}}
+    batched = None
+    if not args.no_batched:
+        log("batched 96-shot gradient leg")
+        batched = batched_leg()
+    fwi = None
+    if not args.no_fwi:
+        log("fwi time-to-misfit leg")
+        fwi = fwi_leg()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -177,7 +272,8 @@ def main():
                       "dt_s": DT, "nt": args.nt, "ntaper": NTAPER, "receivers": NX, "snapi": SNAPI, "snapshots": args.snap,
                       "tile": args.tile, "strip": args.strip, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
                       "device": info["name"]},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "fwi": fwi,
+           "batched": batched}
     print(json.dumps(out), flush=True)
 
 

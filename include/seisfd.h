@@ -176,10 +176,14 @@ int  sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots,
 int  sfd_last_timing(sfd_handle h, double *ms, double *pts);
 
 /*
- * Step kernel tile configuration (tile_z x tile_x cells per workgroup).
- * Chosen by the autotuner when tile_z <= 0.
+ * Step kernel tile configuration (tile_z x tile_x cells per workgroup):
+ * 64x32 (default) or 64x64.  tile_z <= 0 asks the autotuner to choose the
+ * tile shape and wavefront strip width on the next modeling call, by timing
+ * a short forward run of every compiled shape on that call's batch.
+ * sfd_get_tile reports the shape in use and, after an autotune, the timings.
  */
 int  sfd_set_tile(sfd_handle h, int tile_z, int tile_x);
+int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, char *log, int len);
 
 /* Engine options (no reference counterpart: launch policy of the HIP path). */
 #define SFD_OPT_GRAPHS       1  /* 1: replay blocks of time steps as one HIP graph (default) */

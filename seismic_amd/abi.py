@@ -103,7 +103,7 @@ SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_born", "sfd_adjoint_exact", "sfd_lsrtm",
-    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_set_option", "sfd_kernel_stats",
+    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_kernel_stats",
     "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
@@ -141,6 +141,7 @@ def load():
                                      C.c_void_p, C.c_void_p]
     lib.sfd_last_timing.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.sfd_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.sfd_get_tile.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]
     lib.sfd_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.sfd_kernel_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int)]
     lib.sfd_copy_calibration.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]

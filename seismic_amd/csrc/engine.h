@@ -72,6 +72,7 @@ struct Engine {
     bool have_model = false;
     int tz = 64, tx = 32;
     bool autotune_pending = false;
+    std::string tune_log;                 // candidates and times of the last autotune
     int xcd_remap = 1;
     int strip_px = 0;        // columns per wavefront (0: tile width / 4)
     int graph_chunk = 200;   // steps per replayed block (rounded up to a multiple of lcm(4, snapi))
@@ -99,7 +100,8 @@ struct Engine {
                       const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
 
     static bool tile_supported(int TZ, int TX);
-    void check_shots(const sfd_shot *shots, int nshots) const;
+    void check_shots(const sfd_shot *shots, int nshots);
+    void autotune(const sfd_shot *shots, int ns);
     float padded_vel(const std::vector<float> &vh, const ShotPlan &p, int icz, int icx) const;
     void plan_shot(const sfd_shot &s, ShotPlan &p) const;
     void plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz, const float *sx,

@@ -434,6 +434,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         m = s_red[0];
 #pragma unroll
         for (int w = 1; w < NW; w++) m = fmaxf(m, s_red[w]);
+        // tiles whose largest value is below 1e-30 store zeros (1/m would overflow FP32)
+        if (!(m > 1e-30f)) m = 0.f;
         if (tid == 0) gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * (S.ntz * S.ntx) + t] = m;
         inv16 = m > 0.f ? 1.0f / m : 0.f;
     }

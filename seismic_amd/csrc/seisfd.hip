@@ -659,11 +659,67 @@ void Engine::upload_inj_traces(Batch &B, int s, const float *tr, size_t count) {
     HIPCHK(hipMemcpyAsync((void *)B.shots_h[s].inj_traces, tr, count * sizeof(float), hipMemcpyHostToDevice, stream));
 }
 
-void Engine::check_shots(const sfd_shot *shots, int nshots) const {
+void Engine::check_shots(const sfd_shot *shots, int nshots) {
     if (!have_model) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model first");
     if (!shots || nshots < 1) throw EngineError(SFD_EINVAL, "no shots");
     for (int s = 0; s < nshots; s++)
         if (!shots[s].wavelet || shots[s].nt < 1 || shots[s].ntr < 1) throw EngineError(SFD_EINVAL, "bad shot descriptor");
+    if (autotune_pending) autotune(shots, nshots);
+}
+
+// ---- tile / strip autotuner -------------------------------------------------
+// Plans the batch for every compiled step-kernel shape (64x32 tiles with 4- or
+// 8-column wavefront strips, 64x64 tiles with 16-column strips), times a short
+// run of forward launches of each on the real batch with HIP events, and keeps
+// the fastest.  Runs on the first call after sfd_set_tile(h, 0, 0).
+void Engine::autotune(const sfd_shot *shots, int ns) {
+    struct Cand { int tz, tx, px; };
+    static const Cand cands[] = {{64, 32, 4}, {64, 32, 8}, {64, 64, 16}};
+    int nmin = shots[0].nt;
+    for (int s = 1; s < ns; s++) nmin = std::min(nmin, shots[s].nt);
+    const int steps = std::max(1, std::min(32, nmin - 1));
+    double best = 1e30;
+    Cand pick = cands[0];
+    tune_log.clear();
+    for (const Cand &cd : cands) {
+        tz = cd.tz; tx = cd.tx; strip_px = cd.px;
+        std::vector<ShotPlan> plans(ns);
+        for (int s = 0; s < ns; s++) {
+            plan_shot(shots[s], plans[s]);
+            plan_injection(plans[s], vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
+            plans[s].n_inj_traces = 1;
+        }
+        BatchNeeds need;
+        build_batch(fwd_batch, plans, need);
+        for (int s = 0; s < ns; s++) upload_inj_traces(fwd_batch, s, shots[s].wavelet, shots[s].nt);
+        prepare_models(fwd_batch, plans, d_vel);
+        StepArgs A = base_args(fwd_batch);
+        A.dir = 1;
+        A.inject = 1;
+        for (int n = 0; n < 4 && n < steps; n++) {   // warm-up
+            A.n = n; A.rot = n & 3;
+            launch_step(fwd_batch, A, false);
+        }
+        HIPCHK(hipEventRecord(ev0, stream));
+        for (int n = 0; n < steps; n++) {
+            A.n = n; A.rot = n & 3;
+            launch_step(fwd_batch, A, false);
+        }
+        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        const double us = 1000.0 * ms / steps;
+        char buf[96];
+        snprintf(buf, sizeof buf, "%s%dx%d/px%d %.2f us", tune_log.empty() ? "" : "; ", cd.tz, cd.tx, cd.px, us);
+        tune_log += buf;
+        if (us < best) {
+            best = us;
+            pick = cd;
+        }
+    }
+    tz = pick.tz; tx = pick.tx; strip_px = pick.px;
+    autotune_pending = false;
 }
 
 // ---- forward modeling (Acoustic_modeling_sep.f90 / AMOD) -------------------
@@ -1326,6 +1382,15 @@ int sfd_set_tile(sfd_handle h, int tile_z, int tile_x) {
         e.tz = tile_z;
         e.tx = tile_x;
         e.autotune_pending = false;
+    });
+}
+
+int sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, char *log, int len) {
+    return guarded(h, [&](Engine &e) {
+        if (tile_z) *tile_z = e.tz;
+        if (tile_x) *tile_x = e.tx;
+        if (strip_px) *strip_px = e.strip_px ? e.strip_px : (e.tx == 32 ? 4 : 16);
+        if (log && len > 0) snprintf(log, (size_t)len, "%s", e.tune_log.c_str());
     });
 }
 

@@ -112,6 +112,12 @@ class Engine:
         self._chk(self.lib.sfd_set_tile(self.h, tz, tx))
 
     # Acoustic_modeling_sep.f90 / AMOD_to_memory
+    def get_tile(self):
+        tz, tx, px = C.c_int(), C.c_int(), C.c_int()
+        buf = C.create_string_buffer(512)
+        self._chk(self.lib.sfd_get_tile(self.h, C.byref(tz), C.byref(tx), C.byref(px), buf, 512))
+        return dict(tile=(tz.value, tx.value), strip=px.value, log=buf.value.decode())
+
     def forward(self, shots, extract_kind=abi.SFD_EXTRACT_SINC_V2):
         tot = sum(s.nt * s.ntr for s in shots)
         out = np.zeros(tot, dtype=np.float32)

@@ -91,6 +91,26 @@ def test_tile_shapes_bitwise(tile):
     assert rel(gpu, ref) <= TOL
 
 
+def test_autotuner_picks_a_compiled_shape():
+    """sfd_set_tile(h, 0, 0): the next call times every compiled step-kernel
+    shape on its batch and keeps the fastest; the gather is unchanged."""
+    say("case: autotuner")
+    nz, nx, nt = 61, 129, 250
+    vel = smooth_random(nz, nx, seed=7)
+    w = ricker(25.0, nt, 1e-3)
+    rx = np.linspace(5.0, 10.0 * (nx - 1) - 5.0, 50).astype(np.float32)
+    rz = np.full_like(rx, 15.0)
+    gs, os_ = both_shots(25.0, 640.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 15, vel=vel)
+    e.set_tile(0, 0)
+    gpu = e.forward([gs])[0]
+    t = e.get_tile()
+    say("autotune: %s -> %s strip %d" % (t["log"], t["tile"], t["strip"]))
+    assert t["log"].count("us") == 3 and t["tile"] in [(64, 32), (64, 64)]
+    ref = O.modeling(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=15), vel, os_, extract_kind=0)
+    assert np.array_equal(gpu, ref)
+
+
 def test_batched_ragged_shots():
     """Several shots with uneven receiver spreads and record lengths in one call."""
     say("case: batched ragged 5 shots")
@@ -198,6 +218,19 @@ def test_rtm_fp16_snapshots_vs_fp64():
     assert e32 <= 2 * eo + 1e-7
     # FP16 max-scaled storage: within 1e-3 of the FP64 image
     assert e16 <= 1e-3
+
+
+def test_fp16_snapshots_with_denormal_amplitudes_stay_finite():
+    """A wavefield whose tiles hold only tiny (down to denormal) values: the
+    per-tile FP16 scale must not overflow (1/max) into Inf/NaN images."""
+    say("case: fp16 snapshots, tiny amplitudes")
+    nz, nx, nt = 48, 72, 320
+    vel, w, rx, rz, data = _rtm_case(nz=nz, nx=nx, nt=nt, seed=9)
+    w = (w * np.float32(1e-36)).astype(np.float32)
+    gs = sa.Shot(18.0, 355.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    img, ill = e.rtm([gs], [data], snap_mode=sa.SFD_SNAP_FP16S)
+    assert np.isfinite(img).all() and np.isfinite(ill).all()
 
 
 def test_born_matches_oracle():
