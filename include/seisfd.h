@@ -187,7 +187,7 @@ int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, char *l
 
 /* Engine options (no reference counterpart: launch policy of the HIP path). */
 #define SFD_OPT_GRAPHS       1  /* 1: replay blocks of time steps as one HIP graph (default) */
-#define SFD_OPT_GRAPH_CHUNK  2  /* time steps per block (default 200, rounded up to a multiple of lcm(4, snapi)) */
+#define SFD_OPT_GRAPH_CHUNK  2  /* time steps per block (default 200, rounded up to a multiple of lcm(6, snapi)) */
 #define SFD_OPT_XCD_REMAP    3  /* 1: XCD-contiguous tile order (default)          */
 #define SFD_OPT_KERNEL_EVENTS 4 /* 1: time every block of steps with HIP events (default) */
 #define SFD_OPT_STRIP_PX     5  /* columns per wavefront in the step kernel: 0 auto (4 for 64x32 tiles), 4, 8, 16 */

@@ -75,7 +75,7 @@ struct Engine {
     std::string tune_log;                 // candidates and times of the last autotune
     int xcd_remap = 1;
     int strip_px = 0;        // columns per wavefront (0: tile width / 4)
-    int graph_chunk = 200;   // steps per replayed block (rounded up to a multiple of lcm(4, snapi))
+    int graph_chunk = 200;   // steps per replayed block (rounded up to a multiple of lcm(kLevels, snapi))
     bool use_graphs = true;
     int event_every = 1;
     std::vector<hipEvent_t> kev;          // start/stop of every block of steps

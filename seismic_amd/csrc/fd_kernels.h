@@ -26,6 +26,10 @@ namespace sfd {
 constexpr int kThreads = 256;
 constexpr int kMaxTileTraces = 1024;   // distinct traces one tile may inject
 constexpr int kMaxTileCoef = 2048;     // injection contributions of one tile staged in LDS
+constexpr int kLevels = 6;             // wavefield level buffers per shot (two-step launches write two)
+
+// buffer of level U_{n+d} given rot = n mod kLevels
+__host__ __device__ __forceinline__ int lev(int rot, int d) { return (rot + d + 2 * kLevels) % kLevels; }
 
 // One injection contribution: gain (v^2 or 1/(dz dx)), sinc weights, and the
 // tile-local index of the trace it injects.
@@ -62,7 +66,7 @@ enum { EXT_SINC_V2 = 0, EXT_NEAREST = 1, EXT_SINC_LSRTM = 2, EXT_SINC_UNIT = 3 }
 
 // Per-shot device descriptor (all pointers are device pointers).
 struct ShotDev {
-    float *u[4];            // rotating wavefield buffers, LDZ*NX each
+    float *u[kLevels];      // wavefield levels, LDZ*NX each: U_j lives in u[j mod kLevels]
     const float *v2;        // vel*vel, padded
     const float *vel;       // vel, padded (imaging / Born)
     const float *hz;        // Higdon z-band coefficients: 16 per column ix in [4, NX-5]
@@ -107,7 +111,7 @@ struct StepArgs {
     int xcd_remap;          // map block ids so one XCD gets contiguous tiles
     int n;                  // step counter from 0 (plus *nbase when nbase is set)
     const int *nbase;       // device-held step base of a replayed graph, or null
-    int rot;                // n % 4
+    int rot;                // n mod kLevels: u2 = U_n is u[rot]
     int dir;                // +1 forward time, -1 reverse time
     int snapi;
     int store_snap;         // 1: store u2 as the snapshot of this time
@@ -300,7 +304,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     __shared__ float4v s_coef[kMaxTileCoef];  // injection tiles: the tile's contributions
 
     // Reference time index.  `its` is the part fixed at launch; a replayed
-    // graph adds a device-held base (a multiple of 4 and of snapi, so `its`
+    // graph adds a device-held base (a multiple of kLevels and of snapi, so `its`
     // carries the residues the buffer rotation and snapshot tests need).
     const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     int it = its;
@@ -314,10 +318,11 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const int t = lb - S.tile0;
     const int tzi = t % S.ntz, txi = t / S.ntz;
     const int z0 = tzi * TZ - S.zsh, x0 = txi * TX - S.xsh;
-    const GLB float *__restrict__ u2 = gp(S.u[(A.rot + 3) & 3]);
-    const GLB float *__restrict__ u1 = gp(S.u[(A.rot + 2) & 3]);
-    const GLB float *__restrict__ u0 = gp(S.u[(A.rot + 1) & 3]);
-    GLB float *__restrict__ u3 = gp(S.u[A.rot]);      // u_{-1} in, u3 out
+    const GLB float *__restrict__ u2 = gp(S.u[A.rot]);
+    const GLB float *__restrict__ u1 = gp(S.u[lev(A.rot, -1)]);
+    const GLB float *__restrict__ u0 = gp(S.u[lev(A.rot, -2)]);
+    const GLB float *__restrict__ um = gp(S.u[lev(A.rot, -3)]);   // u_{-1}
+    GLB float *__restrict__ u3 = gp(S.u[lev(A.rot, 1)]);
     const GLB float *__restrict__ v2p = gp(S.v2);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int iz = z0 + lane;
@@ -361,7 +366,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (ADJ) hv[p] = ldw(v2p, cb + hzo);
         r1[p] = ldw(u1, cb + zoc);
         r0[p] = ldw(u0, cb + zoc);
-        rm[p] = ldw(u3, cb + zoc);
+        rm[p] = ldw(um, cb + zoc);
         if (!ADJ) rv[p] = ldw(v2p, cb + zoc);
     }
     (void)hok;

@@ -372,7 +372,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     std::vector<std::pair<size_t, const void *>> up;
     std::vector<size_t> upn;
     struct Off {
-        size_t u[4], v2, vel, hz, hx, colsrc, taper;
+        size_t u[kLevels], v2, vel, hz, hx, colsrc, taper;
         size_t inj_tile, inj_cell, inj_cptr, inj_coef, inj_tptr, inj_ttr, inj_traces;
         size_t injb_tile, injb_cell, injb_cptr, injb_coef, injb_tptr, injb_ttr;
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
@@ -386,7 +386,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         ShotPlan &p = plans[s];
         Off &o = off[s];
         const size_t nf = (size_t)p.LDZ * p.NX;
-        for (int k = 0; k < 4; k++) o.u[k] = L.take(nf * sizeof(float));
+        for (int k = 0; k < kLevels; k++) o.u[k] = L.take(nf * sizeof(float));
         o.v2 = L.take(nf * sizeof(float));
         o.vel = L.take(nf * sizeof(float));
         o.hz = L.take((size_t)16 * (p.NX - 8) * sizeof(float));
@@ -440,7 +440,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         Off &o = off[s];
         ShotDev &S = B.shots_h[s];
         memset(&S, 0, sizeof S);
-        for (int k = 0; k < 4; k++) S.u[k] = (float *)(base + o.u[k]);
+        for (int k = 0; k < kLevels; k++) S.u[k] = (float *)(base + o.u[k]);
         S.v2 = (const float *)(base + o.v2);
         S.vel = (const float *)(base + o.vel);
         S.hz = (const float *)(base + o.hz);
@@ -507,7 +507,7 @@ void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float 
 void Engine::zero_wavefields(Batch &B, std::vector<ShotPlan> &plans) {
     for (int s = 0; s < B.ns; s++) {
         const size_t nf = (size_t)plans[s].LDZ * plans[s].NX * sizeof(float);
-        for (int k = 0; k < 4; k++) HIPCHK(hipMemsetAsync(B.shots_h[s].u[k], 0, nf, stream));
+        for (int k = 0; k < kLevels; k++) HIPCHK(hipMemsetAsync(B.shots_h[s].u[k], 0, nf, stream));
     }
 }
 
@@ -567,7 +567,7 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 
 // Run nsteps of the lockstep time loop.  Steps n < nfree (every shot of the
 // batch still running) go in blocks of K steps, K = graph_chunk rounded up to
-// a multiple of lcm(4, snapi) so that buffer rotation and snapshot phases
+// a multiple of lcm(kLevels, snapi) so that buffer rotation and snapshot phases
 // repeat from block to block: the K step launches plus one base-advance launch
 // are captured once into a HIP graph and replayed, the kernels reading the
 // block's first step from device memory (d_nbase).  Remaining steps launch one
@@ -575,15 +575,15 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 // block time / K is the per-launch step time reported by kernel_stats.
 void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfree) {
     const int dir = A.dir > 0 ? 0 : 1;
-    int period = 4;
-    while (period % g.snapi) period += 4;
+    int period = kLevels;
+    while (period % g.snapi) period += kLevels;
     const int K = std::max(period, (std::max(1, graph_chunk) + period - 1) / period * period);
     const int nblk = std::min(nsteps, nfree) / K;
     nbase.reserve(256);
     int *d_nb = (int *)nbase.p;
     auto one = [&](int n, const int *nb) {
         A.n = n;
-        A.rot = n & 3;
+        A.rot = n % kLevels;
         A.nbase = nb;
         launch_step(B, A, adj);
     };
@@ -697,12 +697,12 @@ void Engine::autotune(const sfd_shot *shots, int ns) {
         A.dir = 1;
         A.inject = 1;
         for (int n = 0; n < 4 && n < steps; n++) {   // warm-up
-            A.n = n; A.rot = n & 3;
+            A.n = n; A.rot = n % kLevels;
             launch_step(fwd_batch, A, false);
         }
         HIPCHK(hipEventRecord(ev0, stream));
         for (int n = 0; n < steps; n++) {
-            A.n = n; A.rot = n & 3;
+            A.n = n; A.rot = n % kLevels;
             launch_step(fwd_batch, A, false);
         }
         HIPCHK(hipEventRecord(ev1, stream));
