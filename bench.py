@@ -166,7 +166,8 @@ def batched_leg(seed=3):
             "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
             "observed_data_s": round(t_obs, 3), "observed_data_Gpts_per_s": round(pts_fwd / t_obs / 1e9, 3),
             "avg_launch_us": [round(ks["fwd_us"], 2), round(ks["bwd_us"], 2)],
-            "tile": list(tile["tile"]), "strip": tile["strip"], "autotune": tile["log"],
+            "tile": list(tile["tile"]), "strip": tile["strip"], "time_block": tile["time_block"],
+            "autotune": tile["log"],
             "snapshots": "fp16", "misfit": f}
 
 
@@ -180,6 +181,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=1, help="0: eager launches (for profilers)")
     ap.add_argument("--strip", type=int, default=0, help="columns per wavefront (0 auto, 4, 8, 16)")
+    ap.add_argument("--time-block", type=int, default=1, choices=[1, 2], help="time steps per launch")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--nt", type=int, default=NT, help="record length (shorter records for counter passes)")
     ap.add_argument("--no-fwi", action="store_true", help="skip the config-4 FWI time-to-misfit leg")
@@ -212,6 +214,7 @@ def main():
     e.set_tile(tz, tx)
     e.set_option(sa.abi.SFD_OPT_GRAPHS, args.graphs)
     e.set_option(sa.abi.SFD_OPT_STRIP_PX, args.strip)
+    e.set_option(sa.abi.SFD_OPT_TIME_BLOCK, args.time_block)
     shot = sa.Shot(0.0, DZ * (NX // 2), w, rz, rx)
     log("modeling observed data (untimed)")
     e.set_model(true)
@@ -270,7 +273,7 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "rtm-single-shot-751x2301-nt%d" % args.nt, "grid": [NZ, NX], "dx_m": DZ,
                       "dt_s": DT, "nt": args.nt, "ntaper": NTAPER, "receivers": NX, "snapi": SNAPI, "snapshots": args.snap,
-                      "tile": args.tile, "strip": args.strip, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
+                      "tile": args.tile, "strip": args.strip, "time_block": args.time_block, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
                       "device": info["name"]},
            "roofline": roof, "cpu_baseline": cpu, "fwi": fwi,
            "batched": batched}

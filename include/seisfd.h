@@ -178,12 +178,14 @@ int  sfd_last_timing(sfd_handle h, double *ms, double *pts);
 /*
  * Step kernel tile configuration (tile_z x tile_x cells per workgroup):
  * 64x32 (default) or 64x64.  tile_z <= 0 asks the autotuner to choose the
- * tile shape and wavefront strip width on the next modeling call, by timing
- * a short forward run of every compiled shape on that call's batch.
+ * tile shape, wavefront strip width and time block (SFD_OPT_TIME_BLOCK) on
+ * the next modeling call, by timing a short forward run of every compiled
+ * combination on that call's batch.
  * sfd_get_tile reports the shape in use and, after an autotune, the timings.
  */
 int  sfd_set_tile(sfd_handle h, int tile_z, int tile_x);
-int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, char *log, int len);
+int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, int *time_block, char *log,
+                  int len);
 
 /* Engine options (no reference counterpart: launch policy of the HIP path). */
 #define SFD_OPT_GRAPHS       1  /* 1: replay blocks of time steps as one HIP graph (default) */
@@ -191,6 +193,7 @@ int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, char *l
 #define SFD_OPT_XCD_REMAP    3  /* 1: XCD-contiguous tile order (default)          */
 #define SFD_OPT_KERNEL_EVENTS 4 /* 1: time every block of steps with HIP events (default) */
 #define SFD_OPT_STRIP_PX     5  /* columns per wavefront in the step kernel: 0 auto (4 for 64x32 tiles), 4, 8, 16 */
+#define SFD_OPT_TIME_BLOCK   6  /* time steps per launch: 1 (default) or 2 (64x32 tiles; not Born passes) */
 int  sfd_set_option(sfd_handle h, int opt, int value);
 
 /*

@@ -32,6 +32,7 @@ SFD_OPT_GRAPH_CHUNK = 2
 SFD_OPT_XCD_REMAP = 3
 SFD_OPT_KERNEL_EVENTS = 4
 SFD_OPT_STRIP_PX = 5
+SFD_OPT_TIME_BLOCK = 6
 
 
 class sfd_grid(C.Structure):
@@ -141,7 +142,7 @@ def load():
                                      C.c_void_p, C.c_void_p]
     lib.sfd_last_timing.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.sfd_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
-    lib.sfd_get_tile.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]
+    lib.sfd_get_tile.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]
     lib.sfd_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.sfd_kernel_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int)]
     lib.sfd_copy_calibration.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]

@@ -39,6 +39,17 @@ struct ShotPlan {
 struct InjSet {
     const int *tile = nullptr, *cell = nullptr, *cptr = nullptr, *tptr = nullptr, *ttr = nullptr;
     const sfd::InjCoef *coef = nullptr;
+    // region plan of the two-step kernel
+    const int *tile2 = nullptr, *cell2 = nullptr, *cptr2 = nullptr, *tptr2 = nullptr, *ttr2 = nullptr;
+    const sfd::InjCoef *coef2 = nullptr;
+    void apply(sfd::ShotDev &S) const;
+};
+
+// Injection plan over the two-step kernel's regions (tile + 4-cell ring).
+struct RegionPlan {
+    std::vector<int> tile, cell, cptr, tptr, ttr;
+    std::vector<float> coef;
+    bool ok = true;   // false: a region needs more than the kernel's trace table
 };
 
 struct BatchNeeds {
@@ -51,6 +62,7 @@ struct BatchNeeds {
 
 struct Batch {
     DevBuf mem;
+    std::vector<RegionPlan> rp, rpb;   // two-step kernel region plans (forward / backward injections)
     int ns = 0, nblocks = 0;
     std::vector<sfd::ShotDev> shots_h;
     std::vector<InjSet> injb;
@@ -75,6 +87,7 @@ struct Engine {
     std::string tune_log;                 // candidates and times of the last autotune
     int xcd_remap = 1;
     int strip_px = 0;        // columns per wavefront (0: tile width / 4)
+    int time_block = 1;      // steps per launch: 1, or 2 (two-step kernel where it applies)
     int graph_chunk = 200;   // steps per replayed block (rounded up to a multiple of lcm(kLevels, snapi))
     bool use_graphs = true;
     int event_every = 1;
@@ -85,6 +98,7 @@ struct Engine {
     double timing[4] = {0, 0, 0, 0};
     double pts = 0;
     float *raw_illum_host = nullptr;      // fwi_gradient: also copy the summed illumination here
+    bool tb_ok = true;                    // the current batch's region plans fit the two-step kernel
     Batch fwd_batch;
     DevBuf scratch, fwi_mem;
 
@@ -114,6 +128,8 @@ struct Engine {
     sfd::StepArgs base_args(const Batch &B) const;
     void launch_step(const Batch &B, sfd::StepArgs A, bool adj);
     void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, int nfree);
+    void launch_step2(const Batch &B, sfd::StepArgs A, bool adj);
+    bool use_step2(const sfd::StepArgs &A) const;
     void reset_kstats();
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);

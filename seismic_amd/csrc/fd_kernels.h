@@ -84,6 +84,10 @@ struct ShotDev {
     const InjCoef *inj_coef; // per contribution
     const int *inj_tptr;    // per tile [inj_tptr[t], inj_tptr[t+1]) traces it injects
     const int *inj_ttr;     // global trace index of each tile-local trace
+    // the same injections planned per 72x40 region (tile + 4-cell ring) for
+    // the two-step kernel: region-local cells, per-region trace tables
+    const int *inj2_tile, *inj2_cell, *inj2_cptr, *inj2_tptr, *inj2_ttr;
+    const InjCoef *inj2_coef;
     const float *inj_traces;// source traces, (nt, ntraces) time fastest
     // extraction: per tile [ext_tile[t], ext_tile[t+1]) receivers
     const int *ext_tile;

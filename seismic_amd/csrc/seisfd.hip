@@ -20,6 +20,7 @@
 #include "fd_kernels.h"
 #include "fwi_kernels.h"
 #include "engine.h"
+#include "fd_step2.h"
 #include "adjoint_kernels.h"
 #include "common.h"
 #include "cnn_api.h"
@@ -217,6 +218,72 @@ static inline void cell_tile(const ShotPlan &p, int TZ, int TX, int iz, int ix, 
     *local = (iz - z0) + TZ * (ix - x0);
 }
 
+// Region plan of the two-step kernel from a per-tile plan: for every tile,
+// the cells of its own and its 8 neighbours' plans that fall inside its
+// 72x40 region (tile + 4-cell ring), with their contributions in the same
+// order, re-indexed region-locally and against a per-region trace table.
+static RegionPlan region_plan(const ShotPlan &p, const std::vector<int> &tile, const std::vector<int> &cell,
+                              const std::vector<int> &cptr, const std::vector<float> &coef,
+                              const std::vector<int> &tptr, const std::vector<int> &ttr) {
+    const int TZ = B2Z, TX = B2X, ntile = p.ntz * p.ntx;
+    RegionPlan r;
+    r.tile.assign(ntile + 1, 0);
+    r.tptr.assign(ntile + 1, 0);
+    std::vector<int> local;
+    std::vector<int> used;
+    for (int t = 0; t < ntile; t++) {
+        const int tzi = t % p.ntz, txi = t / p.ntz;
+        const int z0 = tzi * TZ - p.zsh, x0 = txi * TX - p.xsh;
+        for (int u : used) local[u] = -1;
+        used.clear();
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dz = -1; dz <= 1; dz++) {
+                const int nzi = tzi + dz, nxi = txi + dx;
+                if (nzi < 0 || nzi >= p.ntz || nxi < 0 || nxi >= p.ntx) continue;
+                const int n = nzi + p.ntz * nxi;
+                const int nz0 = nzi * TZ - p.zsh, nx0 = nxi * TX - p.xsh;
+                for (int k = tile[n]; k < tile[n + 1]; k++) {
+                    const int lz = cell[k] % TZ, lx = cell[k] / TZ;
+                    const int ez = nz0 + lz - (z0 - 4), ex = nx0 + lx - (x0 - 4);
+                    if (ez < 0 || ez >= R1Z || ex < 0 || ex >= R1X) continue;
+                    r.cell.push_back(ez + R1Z * ex);
+                    r.cptr.push_back((int)(r.coef.size() / 4));
+                    r.tile[t + 1]++;
+                    for (int q = cptr[k]; q < cptr[k + 1]; q++) {
+                        int li;
+                        memcpy(&li, &coef[4 * q + 3], 4);
+                        const int gtr = ttr[tptr[n] + li];
+                        if (gtr >= (int)local.size()) local.resize(gtr + 1, -1);
+                        if (local[gtr] < 0) {
+                            local[gtr] = (int)used.size();
+                            used.push_back(gtr);
+                            r.ttr.push_back(gtr);
+                            r.tptr[t + 1]++;
+                        }
+                        int nl = local[gtr];
+                        float nlf;
+                        memcpy(&nlf, &nl, 4);
+                        r.coef.insert(r.coef.end(), {coef[4 * q], coef[4 * q + 1], coef[4 * q + 2], nlf});
+                    }
+                }
+            }
+        if ((int)used.size() > kMaxTileTraces2) r.ok = false;
+    }
+    r.cptr.push_back((int)(r.coef.size() / 4));
+    for (int t = 0; t < ntile; t++) {
+        r.tile[t + 1] += r.tile[t];
+        r.tptr[t + 1] += r.tptr[t];
+    }
+    if (r.coef.size() / 4 >= (1u << 23)) r.ok = false;
+    return r;
+}
+
+void InjSet::apply(sfd::ShotDev &S) const {
+    S.inj_tile = tile; S.inj_cell = cell; S.inj_cptr = cptr; S.inj_coef = coef; S.inj_tptr = tptr; S.inj_ttr = ttr;
+    S.inj2_tile = tile2; S.inj2_cell = cell2; S.inj2_cptr = cptr2; S.inj2_coef = coef2; S.inj2_tptr = tptr2;
+    S.inj2_ttr = ttr2;
+}
+
 // Injection plan: kind 1 = Injection_sinc (gain v^2, Injection.f90:151-194),
 // 2 = Injection_simple_noomp (v^2, :63-104), 3 = Injection_LSRTM_sinc (1/(dz dx), :239-272).
 void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz_, const float *sx_,
@@ -375,10 +442,16 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t u[kLevels], v2, vel, hz, hx, colsrc, taper;
         size_t inj_tile, inj_cell, inj_cptr, inj_coef, inj_tptr, inj_ttr, inj_traces;
         size_t injb_tile, injb_cell, injb_cptr, injb_coef, injb_tptr, injb_ttr;
+        size_t inj2[6], injb2[6];
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
         size_t snap, snap_scale, image, illum, born_m;
     };
     std::vector<Off> off(ns);
+    const bool tb = tz == B2Z && tx == B2X;
+    std::vector<RegionPlan> &rp = B.rp, &rpb = B.rpb;   // kept in the batch: uploads read them asynchronously
+    rp.assign(tb ? ns : 0, RegionPlan());
+    rpb.assign(tb && need.backward_injection ? ns : 0, RegionPlan());
+    tb_ok = tb;
     int tile0 = 0;
     B.blk_shot_h.clear();
     B.colsrc_d.clear();
@@ -407,6 +480,19 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
             o.injb_coef = put(L, up, upn, p.injb_coef);
             o.injb_tptr = put(L, up, upn, p.injb_tptr);
             o.injb_ttr = put(L, up, upn, p.injb_ttr);
+        }
+        auto put_region = [&](RegionPlan &r, size_t *o6) {
+            o6[0] = put(L, up, upn, r.tile); o6[1] = put(L, up, upn, r.cell); o6[2] = put(L, up, upn, r.cptr);
+            o6[3] = put(L, up, upn, r.coef); o6[4] = put(L, up, upn, r.tptr); o6[5] = put(L, up, upn, r.ttr);
+            tb_ok = tb_ok && r.ok;
+        };
+        if (tb) {
+            rp[s] = region_plan(p, p.inj_tile, p.inj_cell, p.inj_cptr, p.inj_coef, p.inj_tptr, p.inj_ttr);
+            put_region(rp[s], o.inj2);
+            if (need.backward_injection) {
+                rpb[s] = region_plan(p, p.injb_tile, p.injb_cell, p.injb_cptr, p.injb_coef, p.injb_tptr, p.injb_ttr);
+                put_region(rpb[s], o.injb2);
+            }
         }
         o.ext_tile = put(L, up, upn, p.ext_tile);
         o.ext_rec = put(L, up, upn, p.ext_rec);
@@ -463,6 +549,16 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
             I.coef = (const InjCoef *)(base + o.injb_coef);
             I.tptr = (const int *)(base + o.injb_tptr);
             I.ttr = (const int *)(base + o.injb_ttr);
+            if (tb) {
+                I.tile2 = (const int *)(base + o.injb2[0]); I.cell2 = (const int *)(base + o.injb2[1]);
+                I.cptr2 = (const int *)(base + o.injb2[2]); I.coef2 = (const InjCoef *)(base + o.injb2[3]);
+                I.tptr2 = (const int *)(base + o.injb2[4]); I.ttr2 = (const int *)(base + o.injb2[5]);
+            }
+        }
+        if (tb) {
+            S.inj2_tile = (const int *)(base + o.inj2[0]); S.inj2_cell = (const int *)(base + o.inj2[1]);
+            S.inj2_cptr = (const int *)(base + o.inj2[2]); S.inj2_coef = (const InjCoef *)(base + o.inj2[3]);
+            S.inj2_tptr = (const int *)(base + o.inj2[4]); S.inj2_ttr = (const int *)(base + o.inj2[5]);
         }
         S.ext_tile = p.ext_tile.empty() ? nullptr : (const int *)(base + o.ext_tile);
         S.ext_rec = (const int *)(base + o.ext_rec);
@@ -565,6 +661,25 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
 }
 
+// The two-step kernel applies with 64x32 tiles, region plans within its trace
+// table, and no Born source term.
+bool Engine::use_step2(const StepArgs &A) const {
+    return time_block == 2 && tb_ok && tz == B2Z && tx == B2X && A.imaging != IMG_BORN;
+}
+
+void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
+    if (B.ns == 1) {
+        A.single = 1;
+        A.s0 = B.shots_h[0];
+    }
+    int grid = B.nblocks;
+    if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
+    typedef void (*Fn)(StepArgs);
+    const Fn f = adj ? (A.single ? (Fn)fd_step2_kernel<true, true> : (Fn)fd_step2_kernel<true, false>)
+                     : (A.single ? (Fn)fd_step2_kernel<false, true> : (Fn)fd_step2_kernel<false, false>);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads2), 0, stream, A);
+}
+
 // Run nsteps of the lockstep time loop.  Steps n < nfree (every shot of the
 // batch still running) go in blocks of K steps, K = graph_chunk rounded up to
 // a multiple of lcm(kLevels, snapi) so that buffer rotation and snapshot phases
@@ -581,12 +696,20 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfre
     const int nblk = std::min(nsteps, nfree) / K;
     nbase.reserve(256);
     int *d_nb = (int *)nbase.p;
+    const bool two = use_step2(A);   // two steps per launch where both lie in every shot's record
     auto one = [&](int n, const int *nb) {
         A.n = n;
         A.rot = n % kLevels;
         A.nbase = nb;
         launch_step(B, A, adj);
     };
+    auto pair = [&](int n, const int *nb) {
+        A.n = n;
+        A.rot = n % kLevels;
+        A.nbase = nb;
+        launch_step2(B, A, adj);
+    };
+    const int inc = two ? 2 : 1;
     const bool timed = event_every > 0;
     if (timed && (int)kev.size() < 2 * nblk) {
         const size_t old = kev.size();
@@ -599,7 +722,7 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfre
         hipLaunchKernelGGL(set_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, 0);
         if (use_graphs) {
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            for (int n = 0; n < K; n++) one(n, d_nb);
+            for (int n = 0; n < K; n += inc) two ? pair(n, d_nb) : one(n, d_nb);
             hipLaunchKernelGGL(advance_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, K);
             HIPCHK(hipStreamEndCapture(stream, &graph));
             HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -610,11 +733,19 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfre
         if (exec) {
             HIPCHK(hipGraphLaunch(exec, stream));
         } else {
-            for (int n = 0; n < K; n++) one(b * K + n, nullptr);
+            for (int n = 0; n < K; n += inc) two ? pair(b * K + n, nullptr) : one(b * K + n, nullptr);
         }
         if (timed) HIPCHK(hipEventRecord(kev[2 * b + 1], stream));
     }
-    for (int n = nblk * K; n < nsteps; n++) one(n, nullptr);
+    for (int n = nblk * K; n < nsteps;) {
+        if (two && n + 1 < nfree) {
+            pair(n, nullptr);
+            n += 2;
+        } else {
+            one(n, nullptr);
+            n++;
+        }
+    }
     HIPCHK(hipGetLastError());
     if (timed && nblk > 0) {
         HIPCHK(hipEventSynchronize(kev[2 * nblk - 1]));
@@ -669,20 +800,24 @@ void Engine::check_shots(const sfd_shot *shots, int nshots) {
 
 // ---- tile / strip autotuner -------------------------------------------------
 // Plans the batch for every compiled step-kernel shape (64x32 tiles with 4- or
-// 8-column wavefront strips, 64x64 tiles with 16-column strips), times a short
-// run of forward launches of each on the real batch with HIP events, and keeps
-// the fastest.  Runs on the first call after sfd_set_tile(h, 0, 0).
+// 8-column wavefront strips, 64x64 tiles with 16-column strips, and 64x32
+// tiles with two time steps per launch), times a short run of forward launches
+// of each on the real batch with HIP events (per step), and keeps the fastest.
+// Runs on the first call after sfd_set_tile(h, 0, 0).
 void Engine::autotune(const sfd_shot *shots, int ns) {
-    struct Cand { int tz, tx, px; };
-    static const Cand cands[] = {{64, 32, 4}, {64, 32, 8}, {64, 64, 16}};
+    struct Cand { int tz, tx, px, tb; };
+    static const Cand cands[] = {{64, 32, 4, 1}, {64, 32, 8, 1}, {64, 64, 16, 1}, {64, 32, 4, 2}};
     int nmin = shots[0].nt;
     for (int s = 1; s < ns; s++) nmin = std::min(nmin, shots[s].nt);
-    const int steps = std::max(1, std::min(32, nmin - 1));
+    const int steps1 = std::max(1, std::min(32, nmin));         // one-step launches timed
+    const int steps2 = std::min(32, nmin - 1) / 2 * 2;          // two-step: both steps inside every record
     double best = 1e30;
     Cand pick = cands[0];
     tune_log.clear();
     for (const Cand &cd : cands) {
-        tz = cd.tz; tx = cd.tx; strip_px = cd.px;
+        if (cd.tb == 2 && steps2 < 2) continue;
+        const int steps = cd.tb == 2 ? steps2 : steps1;
+        tz = cd.tz; tx = cd.tx; strip_px = cd.px; time_block = cd.tb;
         std::vector<ShotPlan> plans(ns);
         for (int s = 0; s < ns; s++) {
             plan_shot(shots[s], plans[s]);
@@ -696,29 +831,31 @@ void Engine::autotune(const sfd_shot *shots, int ns) {
         StepArgs A = base_args(fwd_batch);
         A.dir = 1;
         A.inject = 1;
-        for (int n = 0; n < 4 && n < steps; n++) {   // warm-up
+        const bool two = use_step2(A);
+        auto adv = [&](int n) {
             A.n = n; A.rot = n % kLevels;
-            launch_step(fwd_batch, A, false);
-        }
+            if (two) launch_step2(fwd_batch, A, false);
+            else launch_step(fwd_batch, A, false);
+        };
+        const int inc = two ? 2 : 1;
+        for (int n = 0; n + inc <= 4 && n + inc < nmin; n += inc) adv(n);   // warm-up
         HIPCHK(hipEventRecord(ev0, stream));
-        for (int n = 0; n < steps; n++) {
-            A.n = n; A.rot = n % kLevels;
-            launch_step(fwd_batch, A, false);
-        }
+        for (int n = 0; n + inc <= steps; n += inc) adv(n);
         HIPCHK(hipEventRecord(ev1, stream));
         HIPCHK(hipEventSynchronize(ev1));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         const double us = 1000.0 * ms / steps;
         char buf[96];
-        snprintf(buf, sizeof buf, "%s%dx%d/px%d %.2f us", tune_log.empty() ? "" : "; ", cd.tz, cd.tx, cd.px, us);
+        snprintf(buf, sizeof buf, "%s%dx%d/px%d/tb%d %.2f us", tune_log.empty() ? "" : "; ", cd.tz, cd.tx, cd.px,
+                 cd.tb, us);
         tune_log += buf;
         if (us < best) {
             best = us;
             pick = cd;
         }
     }
-    tz = pick.tz; tx = pick.tx; strip_px = pick.px;
+    tz = pick.tz; tx = pick.tx; strip_px = pick.px; time_block = pick.tb;
     autotune_pending = false;
 }
 
@@ -937,9 +1074,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     size_t off = 0;
     for (int s = 0; s < ns; s++) {
         ShotDev &S = B.shots_h[s];
-        const InjSet &I = B.injb[s];
-        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_coef = I.coef;
-        S.inj_tptr = I.tptr; S.inj_ttr = I.ttr;
+        B.injb[s].apply(S);
         const size_t n = (size_t)shots[s].nt * shots[s].ntr;
         upload_inj_traces(B, s, traces_in + off, n);
         off += n;
@@ -1112,9 +1247,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     zero_wavefields(B, plans);
     for (int s = 0; s < ns; s++) {
         ShotDev &S = B.shots_h[s];
-        const InjSet &I = B.injb[s];
-        S.inj_tile = I.tile; S.inj_cell = I.cell; S.inj_cptr = I.cptr; S.inj_coef = I.coef;
-        S.inj_tptr = I.tptr; S.inj_ttr = I.ttr;
+        B.injb[s].apply(S);
     }
     upload_shots(B);
     StepArgs Ab = base_args(B);
@@ -1385,8 +1518,9 @@ int sfd_set_tile(sfd_handle h, int tile_z, int tile_x) {
     });
 }
 
-int sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, char *log, int len) {
+int sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, int *time_block, char *log, int len) {
     return guarded(h, [&](Engine &e) {
+        if (time_block) *time_block = e.time_block;
         if (tile_z) *tile_z = e.tz;
         if (tile_x) *tile_x = e.tx;
         if (strip_px) *strip_px = e.strip_px ? e.strip_px : (e.tx == 32 ? 4 : 16);
@@ -1431,6 +1565,10 @@ int sfd_set_option(sfd_handle h, int opt, int value) {
             e.graph_chunk = value;
             break;
         case SFD_OPT_XCD_REMAP: e.xcd_remap = value != 0; break;
+        case SFD_OPT_TIME_BLOCK:
+            if (value != 1 && value != 2) throw EngineError(SFD_EINVAL, "time block must be 1 or 2");
+            e.time_block = value;
+            break;
         case SFD_OPT_STRIP_PX:
             if (value != 0 && value != 4 && value != 8 && value != 16) throw EngineError(SFD_EINVAL, "strip width 4, 8 or 16");
             if (value == 4 && !(e.tz == 64 && e.tx == 32)) throw EngineError(SFD_EINVAL, "strip width 4 needs 64x32 tiles");

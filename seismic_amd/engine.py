@@ -113,10 +113,10 @@ class Engine:
 
     # Acoustic_modeling_sep.f90 / AMOD_to_memory
     def get_tile(self):
-        tz, tx, px = C.c_int(), C.c_int(), C.c_int()
+        tz, tx, px, tb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         buf = C.create_string_buffer(512)
-        self._chk(self.lib.sfd_get_tile(self.h, C.byref(tz), C.byref(tx), C.byref(px), buf, 512))
-        return dict(tile=(tz.value, tx.value), strip=px.value, log=buf.value.decode())
+        self._chk(self.lib.sfd_get_tile(self.h, C.byref(tz), C.byref(tx), C.byref(px), C.byref(tb), buf, 512))
+        return dict(tile=(tz.value, tx.value), strip=px.value, time_block=tb.value, log=buf.value.decode())
 
     def forward(self, shots, extract_kind=abi.SFD_EXTRACT_SINC_V2):
         tot = sum(s.nt * s.ntr for s in shots)
