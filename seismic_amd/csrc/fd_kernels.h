@@ -291,6 +291,8 @@ struct StepShape {
     static constexpr int TX = PX >= 8 ? 4 * PX : 32;   // >= 19 columns: Higdon bands stay in one tile
     static constexpr int NW = TX / PX;                   // wavefronts per workgroup
     static constexpr int THREADS = 64 * NW;
+    // 64 x 4 strips: three 8-wave workgroups per CU need <= 80 VGPRs (6 waves per SIMD)
+    static constexpr int WAVES_PER_EU = PX == 4 ? 6 : 1;
 };
 
 template <int PX, bool ADJ>
@@ -298,14 +300,25 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
     __shared__ float s_col[NW][72];     // per-wavefront z-stencil column
-    __shared__ float s_b[TX * TZ];      // edge tiles: u3
-    __shared__ float s_u2[TX * TZ];     // edge tiles: u2
+    // Injection tables are read only inside the column loop and the edge
+    // buffers are written only after it (behind a barrier on injection
+    // tiles), so the two phases share one LDS block.  Keeping the block small
+    // is what lets three 512-thread workgroups share a CU.
     __shared__ union {
-        int inj[TX * TZ];               // per-cell contribution range (start<<8 | count)
-        float u1[TX * TZ];              // edge tiles: u1
-    } s_u;
-    __shared__ float s_trv[kMaxTileTraces];   // injection tiles: this step's trace samples
-    __shared__ float4v s_coef[kMaxTileCoef];  // injection tiles: the tile's contributions
+        struct {
+            int cell[TX * TZ];              // per-cell contribution range (start<<8 | count)
+            float trv[kMaxTileTraces];      // this step's trace samples
+            float4v coef[kMaxTileCoef];     // the tile's contributions
+        } inj;
+        struct {
+            float b[TX * TZ];               // u3
+            float u2[TX * TZ];              // u2
+            float u1[TX * TZ];              // u1
+        } edge;
+    } s_p;
+    int *const s_inj = s_p.inj.cell;
+    float *const s_trv = s_p.inj.trv;
+    float4v *const s_coef = s_p.inj.coef;
 
     // Reference time index.  `its` is the part fixed at launch; a replayed
     // graph adds a device-held base (a multiple of kLevels and of snapi, so `its`
@@ -328,7 +341,9 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const GLB float *__restrict__ um = gp(S.u[lev(A.rot, -3)]);   // u_{-1}
     GLB float *__restrict__ u3 = gp(S.u[lev(A.rot, 1)]);
     const GLB float *__restrict__ v2p = gp(S.v2);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // wv is the same on every lane of a wavefront; readfirstlane lets the
+    // compiler keep per-column addresses and taper reads in scalar registers
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int iz = z0 + lane;
     const int xs = x0 + wv * PX;                      // first column of this wavefront's strip
     const bool zarr = iz >= 0 && iz < NZ;
@@ -374,6 +389,14 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (!ADJ) rv[p] = ldw(v2p, cb + zoc);
     }
     (void)hok;
+    // Adjoint: the stencil reads v^2 u2 (FDderivatives.f90:877-887); form the
+    // products once so the outer window cells keep one register, not two.
+    if (ADJ) {
+#pragma unroll
+        for (int k = 0; k < NWIN; k++) wv2[k] = wv2[k] * wu[k];
+#pragma unroll
+        for (int p = 0; p < PX; p++) hv[p] = hv[p] * hu[p];
+    }
 
     // ---- imaging steps: the source snapshot, image, illumination and taper
     // of this lane's cells, issued with the streaming loads (clamped window
@@ -405,7 +428,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const int has_inj = A.inject && (inj_tile[t + 1] > inj_tile[t]);
     bool coef_lds = false;
     if (has_inj) {
-        for (int l = tid; l < TX * TZ; l += NT) s_u.inj[l] = 0;
+        for (int l = tid; l < TX * TZ; l += NT) s_inj[l] = 0;
         const GLB int *ttr = gp(S.inj_ttr);
         const GLB float *trs = gp(S.inj_traces);
         for (int j = gp(S.inj_tptr)[t] + tid, j0 = gp(S.inj_tptr)[t]; j < gp(S.inj_tptr)[t + 1]; j += NT)
@@ -421,7 +444,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         const int base = coef_lds ? cb : 0;
         for (int k = inj_tile[t] + tid; k < inj_tile[t + 1]; k += NT) {
             const int c0 = cptr[k], c1 = cptr[k + 1];
-            s_u.inj[cell[k]] = ((c0 - base) << 8) | (c1 - c0);
+            s_inj[cell[k]] = ((c0 - base) << 8) | (c1 - c0);
         }
         __syncthreads();
     }
@@ -457,8 +480,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         const int ix = xs + p;
         const float u2c = wu[p + 4];
         // z-stencil column through wavefront-private LDS (all 64 lanes take part)
-        col[lane + 4] = ADJ ? wv2[p + 4] * u2c : u2c;
-        if (lane < 8) col[lane < 4 ? lane : 64 + lane] = ADJ ? hv[p] * hu[p] : hu[p];
+        col[lane + 4] = ADJ ? wv2[p + 4] : u2c;
+        if (lane < 8) col[lane < 4 ? lane : 64 + lane] = ADJ ? hv[p] : hu[p];
         __builtin_amdgcn_wave_barrier();
         const float tz = c[0] * (col[lane + 4]) + c[1] * (col[lane + 5] + col[lane + 3]) +
                          c[2] * (col[lane + 6] + col[lane + 2]) + c[3] * (col[lane + 7] + col[lane + 1]) +
@@ -471,14 +494,14 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
                              (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f,
                              t, inv16, pS[p], pI[p], pL[p], pT[p]);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
-#define WX(k) (ADJ ? wv2[p + 4 + (k)] * wu[p + 4 + (k)] : wu[p + 4 + (k)])
+#define WX(k) (ADJ ? wv2[p + 4 + (k)] : wu[p + 4 + (k)])
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
                          c[8] * (WX(3) + WX(-3)) + c[9] * (WX(4) + WX(-4));
 #undef WX
         float val = ADJ ? (tx + tz) : rv[p] * (tx + tz);
         // Injection: per-cell contributions in trace order (Injection.f90:434-440)
         if (has_inj) {
-            const int e = s_u.inj[lane + TZ * (wv * PX + p)];
+            const int e = s_inj[lane + TZ * (wv * PX + p)];
             // gain, sinc z, sinc x, tile-local trace (int bits)
             if (coef_lds) {
                 for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
@@ -506,16 +529,17 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 
     const bool top = z0 <= 0, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
     if (top || bot || left || right) {
-        if (has_inj) __syncthreads();   // s_u.inj reads done before s_u.u1 is written
+        if (has_inj) __syncthreads();   // injection tables read before the edge buffers overwrite them
+        float *s_b = s_p.edge.b;
 #pragma unroll
         for (int p = 0; p < PX; p++) {
             const int l = lane + TZ * (wv * PX + p);
             s_b[l] = u3v[p];
-            s_u2[l] = wu[p + 4];
-            s_u.u1[l] = r1[p];
+            s_p.edge.u2[l] = wu[p + 4];
+            s_p.edge.u1[l] = r1[p];
         }
         __syncthreads();
-        higdon_tile<TZ, TX>(S, z0, x0, top, bot, left, right, s_b, s_u2, s_u.u1);
+        higdon_tile<TZ, TX>(S, z0, x0, top, bot, left, right, s_b, s_p.edge.u2, s_p.edge.u1);
 #pragma unroll
         for (int p = 0; p < PX; p++) u3v[p] = s_b[lane + TZ * (wv * PX + p)];
     }
@@ -538,7 +562,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // arguments (scalar loads, no dependent table lookups); batched launches
 // find their shot through the logical-tile table.
 template <int PX, bool ADJ, bool SINGLE>
-__global__ void __launch_bounds__(StepShape<PX>::THREADS) fd_step_kernel(StepArgs A) {
+__global__ void __launch_bounds__(StepShape<PX>::THREADS)
+__attribute__((amdgpu_waves_per_eu(StepShape<PX>::WAVES_PER_EU))) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
