@@ -4,7 +4,15 @@
  * MCSTEP (:985-1210), block data LB2 (:636-641: GTOL=0.9, STPMIN=1e-20,
  * STPMAX=1e20) and the in-repo DDOT (libs/libinv/blas.f:58-105, unrolled by 5).
  *
- * TEST INFRASTRUCTURE ONLY (the checker for the engine's sfd_lbfgs_*).
+ * Also the out-of-core variant libs/libnlinv/nlinv_lbfgs.f: FREEZE_XOC (:1-22),
+ * LBFGSOC (:31-599), MCSRCHOC (:614-967), block data LB4 (:981-986, the same
+ * GTOL/STPMIN/STPMAX).  It is LBFGS/MCSRCH with a per-parameter clip, its own
+ * STP1 rule and the SAVE state written to files between calls; orc_lbfgsoc
+ * keeps that state in memory (the engine's file round trip is checked by
+ * equality with it).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the engine's sfd_lbfgs_* and
+ * sfd_lbfgsoc_step).
  *
  * The Fortran SAVE variables and GOTO re-entry points become the fields of
  * orc_lbfgs_t and an explicit `entry` switch; the reverse-communication
@@ -62,6 +70,36 @@ void orc_freeze_x(double xmin, double xmax, const float *mask, int type, double 
             if (mask[i] == 2) x[i] = xinit[i];
         }
     }
+}
+
+/* nlinv_lbfgs.f:1-22 FREEZE_XOC: parameter j's block of M = N/NPARAM values
+ * (integer division; a remainder is left alone) clipped to [XMIN(j), XMAX(j)] */
+void orc_freeze_xoc(const double *xmin, const double *xmax, int nparam, double *x, int n)
+{
+    int i, j, m = n / nparam;
+    for (j = 0; j < nparam; j++)
+        for (i = j * m; i < (j + 1) * m; i++) {
+            x[i] = fmax(xmin[j], x[i]);
+            x[i] = fmin(xmax[j], x[i]);
+        }
+}
+
+/* The bound handling of one variant: FREEZE_X (libinv) or FREEZE_XOC (libnlinv). */
+typedef struct {
+    int oc;
+    double xmin, xmax;
+    const float *mask;
+    int type;
+    const double *xinit;
+    const double *pxmin, *pxmax;
+    int nparam;
+    int boundt;
+} orc_clip_t;
+
+static void clip(const orc_clip_t *c, double *x, int n)
+{
+    if (c->oc) orc_freeze_xoc(c->pxmin, c->pxmax, c->nparam, x, n);
+    else orc_freeze_x(c->xmin, c->xmax, c->mask, c->type, x, n, c->xinit);
 }
 
 /* lbfgs.f:985-1210 MCSTEP */
@@ -176,18 +214,20 @@ static void mcstep(double *stx, double *fx, double *dx, double *sty, double *fy,
     *stp = stpf;
     if (*brackt && bound) {
         if (*sty > *stx)
-            *stp = fmin(*stx + 0.66 * (*sty - *stx), *stp);
+            /* 0.66 is a default-REAL literal there: its FP32 value, widened (lbfgs.f:1203) */
+            *stp = fmin(*stx + (double)0.66f * (*sty - *stx), *stp);
         else
-            *stp = fmax(*stx + 0.66 * (*sty - *stx), *stp);
+            *stp = fmax(*stx + (double)0.66f * (*sty - *stx), *stp);
     }
 }
 
 /* lbfgs.f:652-982 MCSRCH; wa is the reference's DIAG array (passed as WA) */
 static void mcsrch(orc_lbfgs_t *S, int n, double *x, double f, const double *g, const double *s,
                    double *stp, double ftol, double xtol, int maxfev, int *info, int *nfev, double *wa,
-                   double xmin, double xmax, const float *mask, int boundt, int type, const double *xinit)
+                   const orc_clip_t *c)
 {
-    const double p5 = 0.5, p66 = 0.66, xtrapf = 4.0, zero = 0.0;
+    /* XTRAPF: 4.0D1 in libinv's MCSRCH (lbfgs.f:792), 4.0D0 in MCSRCHOC (nlinv_lbfgs.f:752) */
+    const double p5 = 0.5, p66 = 0.66, xtrapf = c->oc ? 4.0 : 40.0, zero = 0.0;
     int j;
     double dg, dgm, dgxm, dgym, fm, fxm, fym, ftest1;
     if (*info == -1) goto L45;
@@ -226,7 +266,7 @@ L30:
         (S->brackt && S->stmax - S->stmin <= xtol * S->stmax))
         *stp = S->stx;
     for (j = 0; j < n; j++) x[j] = wa[j] + *stp * s[j];
-    if (boundt == 1) orc_freeze_x(xmin, xmax, mask, type, x, n, xinit);
+    if (c->boundt == 1) clip(c, x, n);
     *info = -1;
     return;
 L45:
@@ -278,10 +318,12 @@ orc_lbfgs_t *orc_lbfgs_new(void)
 }
 void orc_lbfgs_free(orc_lbfgs_t *S) { free(S); }
 
-/* lbfgs.f:49-558 LBFGS (DIAGCO=.false., IPRINT ignored: output only) */
-void orc_lbfgs(orc_lbfgs_t *S, int n, int m, double *x, double f, const double *g, double *diag, double eps,
-               double xtol, double *w, int *iflag, int *myinfo, double xmin, double xmax, const float *mask,
-               int boundt, int type, const double *xinit)
+/* lbfgs.f:49-558 LBFGS (DIAGCO=.false., IPRINT ignored: output only), and
+ * LBFGSOC (nlinv_lbfgs.f:31-599) when c->oc: the two differ only in the clip
+ * and in STP1 (below). */
+static void lbfgs_core(orc_lbfgs_t *S, int n, int m, double *x, double f, const double *g, double *diag,
+                       double eps, double xtol, double *w, int *iflag, int *myinfo, const orc_clip_t *c,
+                       int stp1_opt, double step_init)
 {
     int i, j, inmc, iycn, iscn;
     double sq, yr, beta, xnorm;
@@ -301,10 +343,18 @@ L10:
     for (i = 1; i <= n; i++) w[S->ispt + i - 1] = -g[i - 1] * diag[i - 1];
     S->gnorm = sqrt(orc_ddot(n, g, g));
     S->xnorm2 = sqrt(orc_ddot(n, x, x));
-    if (boundt == 0)
-        S->stp1 = 1.0 / S->gnorm;
-    else
-        S->stp1 = S->xnorm2 / S->gnorm / 100;
+    if (!c->oc) {                        /* lbfgs.f:333-338 */
+        if (c->boundt == 0)
+            S->stp1 = 1.0 / S->gnorm;
+        else
+            S->stp1 = S->xnorm2 / S->gnorm / 100;
+    } else {                             /* nlinv_lbfgs.f:336-344 */
+        if (stp1_opt == 1)
+            S->stp1 = S->xnorm2 / S->gnorm / 100;
+        else
+            S->stp1 = 1.0 / S->gnorm;
+        if (step_init != 999999.0) S->stp1 = step_init;
+    }
     S->ftol = 1.0e-4;
     S->maxfev = 20;
 L80:
@@ -354,7 +404,7 @@ L165:
     for (i = 0; i < n; i++) w[i] = g[i];
 L172:
     mcsrch(S, n, x, f, g, &w[S->ispt + S->point * n], &S->stp, S->ftol, xtol, S->maxfev, &S->info, &S->nfev,
-           diag, xmin, xmax, mask, boundt, type, xinit);
+           diag, c);
     if (S->info == -1) {
         *iflag = 1;
         return;
@@ -365,7 +415,7 @@ L172:
     }
     S->nfun = S->nfun + S->nfev;
     *myinfo = S->info;
-    if (boundt == 2 && S->info == 1) orc_freeze_x(xmin, xmax, mask, type, x, n, xinit);
+    if (c->boundt == 2 && S->info == 1) clip(c, x, n);
     S->npt = S->point * n;
     for (i = 1; i <= n; i++) {
         w[S->ispt + S->npt + i - 1] = S->stp * w[S->ispt + S->npt + i - 1];
@@ -382,6 +432,24 @@ L172:
         return;
     }
     goto L80;
+}
+
+void orc_lbfgs(orc_lbfgs_t *S, int n, int m, double *x, double f, const double *g, double *diag, double eps,
+               double xtol, double *w, int *iflag, int *myinfo, double xmin, double xmax, const float *mask,
+               int boundt, int type, const double *xinit)
+{
+    orc_clip_t c = {0, xmin, xmax, mask, type, xinit, NULL, NULL, 0, boundt};
+    lbfgs_core(S, n, m, x, f, g, diag, eps, xtol, w, iflag, myinfo, &c, 0, 999999.0);
+}
+
+/* LBFGSOC(N,M,X,F,G,.FALSE.,DIAG,IPRINT,EPS,XTOL,STEPINIT,W,IFLAG,MYINFO,...,
+ * XMIN,XMAX,NPARAM,SPT1_OPT,BOUNDT) with the SAVE state in S instead of files. */
+void orc_lbfgsoc(orc_lbfgs_t *S, int n, int m, double *x, double f, const double *g, double *diag, double eps,
+                 double xtol, double step_init, double *w, int *iflag, int *myinfo, const double *xmin,
+                 const double *xmax, int nparam, int stp1_opt, int boundt)
+{
+    orc_clip_t c = {1, 0.0, 0.0, NULL, 0, NULL, xmin, xmax, nparam, boundt};
+    lbfgs_core(S, n, m, x, f, g, diag, eps, xtol, w, iflag, myinfo, &c, stp1_opt, step_init);
 }
 
 /*

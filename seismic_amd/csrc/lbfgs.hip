@@ -155,14 +155,16 @@ static void mcstep(double *stx, double *fx, double *dx, double *sty, double *fy,
     stpf = std::max(stpmin, stpf);
     *stp = stpf;
     if (*brackt && bound) {
-        if (*sty > *stx) *stp = std::min(*stx + 0.66 * (*sty - *stx), *stp);
-        else *stp = std::max(*stx + 0.66 * (*sty - *stx), *stp);
+        // 0.66 is a default-REAL literal in lbfgs.f:1203-1206: its FP32 value, widened
+        const double p66r = (double)0.66f;
+        if (*sty > *stx) *stp = std::min(*stx + p66r * (*sty - *stx), *stp);
+        else *stp = std::max(*stx + p66r * (*sty - *stx), *stp);
     }
 }
 
 // MCSRCH (lbfgs.f:652-982); s = search direction (device), WA = diag.
 void LbfgsDev::mcsrch(double f, double *s) {
-    const double p5 = 0.5, p66 = 0.66, xtrapf = 4.0, zero = 0.0;
+    const double p5 = 0.5, p66 = 0.66, zero = 0.0;
     const double xtol = xtol_;
     if (info == -1) goto L45;
     infoc = 1;

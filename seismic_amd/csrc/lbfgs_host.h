@@ -26,6 +26,9 @@ struct LbfgsDev {
     int iter = 0, nfun = 0, point = 0, maxfev = 20, info = 0, bound = 0, npt = 0, cp = 0, nfev = 0, finish = 0;
     double gnorm = 0, xnorm2 = 0, stp1 = 0, ftol = 1e-4, stp = 0, ys = 0, yy = 0;
     double gtol = 0.9, stpmin = 1e-20, stpmax = 1e20;   // block data LB2 (lbfgs.f:636-641)
+    // extrapolation bound of the line search: 4.0D1 in MCSRCH (lbfgs.f:792),
+    // 4.0D0 in the out-of-core MCSRCHOC (nlinv_lbfgs.f:752)
+    double xtrapf = 40.0;
     // MCSRCH SAVE state
     int infoc = 0, brackt = 0, stage1 = 0;
     double xtol_ = 2.220446049250313e-16;

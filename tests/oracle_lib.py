@@ -15,6 +15,7 @@ from seismic_amd.abi import sfd_grid, sfd_shot, sfd_fwi_opts
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libseisref.so")
+NLREF_SO = os.path.join(ROOT, "oracle", "_ref", "libnlinvref.so")
 
 _FP = C.POINTER(C.c_float)
 _DP = C.POINTER(C.c_double)
@@ -340,6 +341,108 @@ class LBFGS:
             self.L.orc_lbfgs_free(self.S)
         except Exception:
             pass
+
+
+class LBFGSOC:
+    """orc_lbfgsoc state machine (restatement of nlinv_lbfgs.f LBFGSOC; the
+    SAVE state stays in memory between calls)."""
+
+    def __init__(self, n, m=5, xmin=(-1e30,), xmax=(1e30,), stp1_opt=1, step_init=999999.0, boundt=2):
+        L = oracle()
+        L.orc_lbfgs_new.restype = C.c_void_p
+        L.orc_lbfgsoc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_void_p,
+                                  C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p,
+                                  C.c_int, C.c_int, C.c_int]
+        L.orc_lbfgs_free.argtypes = [C.c_void_p]
+        self.L = L
+        self.S = L.orc_lbfgs_new()
+        self.n, self.m = n, m
+        self.diag = np.zeros(n)
+        self.w = np.zeros(n * (2 * m + 1) + 2 * m)
+        self.iflag = C.c_int(0)
+        self.myinfo = C.c_int(0)
+        self.xmin = np.ascontiguousarray(xmin, np.float64)
+        self.xmax = np.ascontiguousarray(xmax, np.float64)
+        self.stp1_opt, self.step_init, self.boundt = stp1_opt, step_init, boundt
+
+    def step(self, x, f, g, eps=1e-20, xtol=np.finfo(np.float64).eps):
+        gg = np.ascontiguousarray(g, np.float64)
+        self.L.orc_lbfgsoc(self.S, self.n, self.m, ptr(x), f, ptr(gg), ptr(self.diag), eps, xtol,
+                           self.step_init, ptr(self.w), C.byref(self.iflag), C.byref(self.myinfo),
+                           ptr(self.xmin), ptr(self.xmax), len(self.xmin), self.stp1_opt, self.boundt)
+        return self.iflag.value, self.myinfo.value
+
+    def __del__(self):
+        try:
+            self.L.orc_lbfgs_free(self.S)
+        except Exception:
+            pass
+
+
+def nlref_available():
+    return os.path.exists(NLREF_SO)
+
+
+def _fstr(s):
+    return C.create_string_buffer(s.encode().ljust(1024), 1024)
+
+
+class RefLBFGSOC:
+    """The reference's own LBFGSOC (oracle/_ref/libnlinvref.so), its scalar
+    state in the text files LbfgsDat.dat / McsrchDat.dat of `state_dir`
+    (read and rewritten in place every call); W and DIAG are the caller's
+    arrays, as L_BFGS.f90 keeps them in WorkArray.dat / Diag.dat."""
+
+    def __init__(self, n, state_dir, m=5, xmin=(-1e30,), xmax=(1e30,), stp1_opt=1, step_init=999999.0,
+                 boundt=2):
+        self.L = C.CDLL(NLREF_SO, mode=os.RTLD_LOCAL)
+        self.n, self.m = n, m
+        self.diag = np.zeros(n)
+        self.w = np.zeros(n * (2 * m + 1) + 2 * m)
+        self.iflag = C.c_int(0)
+        self.myinfo = C.c_int(0)
+        self.xmin = np.ascontiguousarray(xmin, np.float64)
+        self.xmax = np.ascontiguousarray(xmax, np.float64)
+        self.stp1_opt, self.step_init, self.boundt = stp1_opt, step_init, boundt
+        self.lb = _fstr(os.path.join(state_dir, "LbfgsDat.dat"))
+        self.mc = _fstr(os.path.join(state_dir, "McsrchDat.dat"))
+        self.iprint = (C.c_int * 2)(-1, 0)
+
+    def step(self, x, f, g, eps=1e-20, xtol=np.finfo(np.float64).eps):
+        gg = np.ascontiguousarray(g, np.float64)
+        I, D = lambda v: C.byref(C.c_int(v)), lambda v: C.byref(C.c_double(v))
+        L1024 = C.c_int64(1024)
+        self.L.lbfgsoc_(I(self.n), I(self.m), ptr(x), D(f), ptr(gg), I(0), ptr(self.diag), self.iprint, D(eps),
+                        D(xtol), D(self.step_init), ptr(self.w), C.byref(self.iflag), C.byref(self.myinfo),
+                        self.lb, self.lb, self.mc, self.mc, ptr(self.xmin), ptr(self.xmax), I(len(self.xmin)),
+                        I(self.stp1_opt), I(self.boundt), L1024, L1024, L1024, L1024)
+        return self.iflag.value, self.myinfo.value
+
+
+class RefLBFGS:
+    """The reference's own LBFGS of libs/libinv/lbfgs.f (oracle/_ref/libseisref.so)."""
+
+    def __init__(self, n, m=5, xmin=-1e30, xmax=1e30, mask=None, boundt=0, freeze_soft=1, xinit=None):
+        self.L = ref()
+        self.n, self.m = n, m
+        self.diag = np.zeros(n)
+        self.w = np.zeros(n * (2 * m + 1) + 2 * m)
+        self.iflag = C.c_int(0)
+        self.myinfo = C.c_int(0)
+        self.xmin, self.xmax = xmin, xmax
+        self.mask = np.ones(n, np.float32) if mask is None else np.asarray(mask, np.float32)
+        self.boundt, self.type = boundt, 1 if freeze_soft else 0
+        self.xinit = np.zeros(n) if xinit is None else np.asarray(xinit, np.float64)
+        self.iprint = (C.c_int * 2)(-1, 0)
+
+    def step(self, x, f, g, eps=1e-20, xtol=np.finfo(np.float64).eps):
+        gg = np.ascontiguousarray(g, np.float64)
+        I, D = lambda v: C.byref(C.c_int(v)), lambda v: C.byref(C.c_double(v))
+        self.L.lbfgs_(I(self.n), I(self.m), ptr(x), D(f), ptr(gg), I(0), ptr(self.diag), self.iprint, D(eps),
+                      D(xtol), ptr(self.w), C.byref(self.iflag), C.byref(self.myinfo), D(self.xmin),
+                      D(self.xmax), ptr(self.mask), I(self.boundt), I(self.type), ptr(self.xinit))
+        return self.iflag.value, self.myinfo.value
 
 
 FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_double))
