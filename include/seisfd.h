@@ -31,6 +31,7 @@ extern "C" {
 #define SFD_ESTATE       -5   /* call order (e.g. no model set)               */
 #define SFD_EUNSTABLE    -6   /* Courant check failed (ReadSouVelRho.f90:629-675) */
 #define SFD_ENOTFINITE   -7   /* NaN/Inf met in a misfit or wavefield guard   */
+#define SFD_EIO          -8   /* state file missing, unreadable or not matching */
 
 /* Snapshot storage for RTM / FWI source wavefields. */
 #define SFD_SNAP_FP32     0   /* reference behaviour: FP32 copies           */
@@ -225,6 +226,39 @@ int  sfd_lbfgs_create(sfd_lbfgs_handle *lb, int n, int m, double xmin, double xm
 int  sfd_lbfgs_step(sfd_lbfgs_handle lb, double *x, double f, const double *g,
                     int *iflag, int *info);
 int  sfd_lbfgs_destroy(sfd_lbfgs_handle lb);
+
+/* ---- Out-of-core L-BFGS (libs/libnlinv/nlinv_lbfgs.f: LBFGSOC :31-599,
+ *      MCSRCHOC :614-967, FREEZE_XOC :1-22), driven one model update per call
+ *      as progs/inversion/L_BFGS.f90:54-98 drives it: the complete optimiser
+ *      state (the SAVE scalars of LBFGSOC and MCSRCHOC, the work array W and
+ *      DIAG) lives in a file between calls, so every call may be a new
+ *      process.  The vectors are FP64 on the device during the call. ------ */
+typedef struct {
+    int     m;           /* MSAVE (nlinv_read_array_params.f90:218 sets 5)       */
+    int     nparam;      /* parameter blocks of n / nparam values (FREEZE_XOC)   */
+    const double *xmin;  /* nparam lower bounds                                  */
+    const double *xmax;  /* nparam upper bounds                                  */
+    int     stp1_opt;    /* 1: first step |x|/|g|/100; otherwise 1/|g| (:336-341) */
+    double  step_init;   /* first step when not 999999 (:342-344)                */
+    int     clip_type;   /* BOUNDT: 1 clip every trial model (MCSRCHOC :850),
+                            2 every accepted model (:521-524), 0 never           */
+} sfd_lbfgsoc_opts;
+
+/* One call of LBFGSOC(N,M,X,F,G,.FALSE.,DIAG,IPRINT,EPS,XTOL,STEPINIT,W,IFLAG,
+ * MYINFO,...,XMIN,XMAX,NPARAM,SPT1_OPT,BOUNDT) with EPS = 1e-20 and XTOL =
+ * machine epsilon (nlinv_read_array_params.f90:227-228).
+ *   iflag (in/out): 0 starts a run (state_in is not read); 1 continues the run
+ *     whose state state_in holds.  On exit: 1 = evaluate f, g at the new x;
+ *     0 = converged; < 0 = LBFGSOC's error codes (-1 line search failed).
+ *   info (in/out): MYINFO, set to 1 by a call that completed an iteration and
+ *     otherwise left as passed (L_BFGS.f90 keeps it in Info.dat).
+ *   x (n, in/out): the model f and g were evaluated at; on exit the next model.
+ *   state_out receives the new state unless iflag < 0 on exit (written to a
+ *   temporary file, then renamed); it may name the same file as state_in.
+ * The state file also holds n, m and the bounds of the run: a mismatch with
+ * the arguments is SFD_EIO. */
+int  sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfgsoc_opts *o,
+                      int n, double *x, double f, const double *g, int *iflag, int *info);
 
 /* ---- In-process L-BFGS driver (libs/libwave/Inversion.f90:29-189 l_bfgs) --
  * Options mirror InversionParam (Inversion_types.f90:46-58): niter (default

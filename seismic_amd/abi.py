@@ -74,6 +74,14 @@ class sfd_lbfgs_opts(C.Structure):
     ]
 
 
+class sfd_lbfgsoc_opts(C.Structure):
+    _fields_ = [
+        ("m", C.c_int), ("nparam", C.c_int),
+        ("xmin", C.c_void_p), ("xmax", C.c_void_p),
+        ("stp1_opt", C.c_int), ("step_init", C.c_double), ("clip_type", C.c_int),
+    ]
+
+
 class sfd_lbfgs_report(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("evaluations", C.c_int), ("iflag", C.c_int), ("info", C.c_int),
@@ -106,7 +114,7 @@ EXPORTS = [
     "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_born", "sfd_adjoint_exact", "sfd_lsrtm",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_kernel_stats",
     "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
-    "sfd_lbfgs_run", "sfd_fwi_lbfgs",
+    "sfd_lbfgsoc_step", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
 ]
@@ -151,6 +159,8 @@ def load():
     lib.sfd_lbfgs_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                    P(C.c_int), P(C.c_int)]
     lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
+    lib.sfd_lbfgsoc_step.argtypes = [C.c_char_p, C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p,
+                                     C.c_double, C.c_void_p, P(C.c_int), P(C.c_int)]
     lib.sfd_lbfgs_run.argtypes = [P(sfd_lbfgs_opts), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                   SFD_FCTGDT, C.c_void_p, P(sfd_lbfgs_report)]
     lib.sfd_fwi_lbfgs.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "../../include/seisfd.h"
 #include "common.h"
@@ -68,6 +70,13 @@ void LbfgsDev::scale_copy(double a, const double *xv, double *y) {
     hipLaunchKernelGGL(copy_scaled_kernel, vgrid(n), dim3(256), 0, st, n, a, xv, y);
 }
 void LbfgsDev::freeze(double *xv) {
+    if (nparam > 0) {   // FREEZE_XOC (nlinv_lbfgs.f:1-22): block j of n/nparam values; a remainder is left alone
+        const long blk = n / nparam;
+        for (int j = 0; j < nparam; j++)
+            hipLaunchKernelGGL(clip_block_kernel, vgrid(blk), dim3(256), 0, st, (long)j * blk, blk, pxmin[j],
+                               pxmax[j], xv);
+        return;
+    }
     hipLaunchKernelGGL(freeze_kernel, vgrid(n), dim3(256), 0, st, n, xmin, xmax, (const float *)mask, type, xv,
                        (const double *)xinit);
 }
@@ -310,7 +319,12 @@ L10:
     scale_copy(-1.0, g, Sv(0));                  // W(ISPT+I) = -G(I)*DIAG(I), DIAG = 1
     gnorm = sqrt(dot(g, g));
     xnorm2 = sqrt(dot(x, x));
-    stp1 = boundt == 0 ? 1.0 / gnorm : xnorm2 / gnorm / 100;
+    if (stp1_opt < 0) {
+        stp1 = boundt == 0 ? 1.0 / gnorm : xnorm2 / gnorm / 100;     // lbfgs.f:333-338
+    } else {
+        stp1 = stp1_opt == 1 ? xnorm2 / gnorm / 100 : 1.0 / gnorm;   // nlinv_lbfgs.f:336-341
+        if (step_init != 999999.0) stp1 = step_init;                  // :342-344
+    }
     ftol = 1.0e-4;
     maxfev = 20;
 L80:
@@ -361,6 +375,105 @@ L172:
         return;
     }
     goto L80;
+}
+
+// ---------------------------------------------------------------------------
+// Optimiser state on file (the out-of-core protocol of nlinv_lbfgs.f, which
+// keeps the LBFGSOC/MCSRCHOC SAVE scalars in LbfgsDat.dat / McsrchDat.dat and
+// W, DIAG in WorkArray.dat / Diag.dat): one binary file, scalars exact.
+namespace {
+constexpr char kStateMagic[8] = {'S', 'F', 'D', 'L', 'B', 'O', 'C', '1'};
+struct StateHead {
+    char magic[8];
+    int64_t n;
+    int32_t m, nparam;
+    int32_t iter, nfun, point, maxfev, info, bound, npt, cp, nfev, finish, infoc, brackt, stage1, pad;
+    double gnorm, xnorm2, stp1, ftol, stp, ys, yy, gtol, stpmin, stpmax, xtrapf;
+    double dginit, dgtest, finit, width, width1, stx, fx, dgx, sty, fy, dgy, stmin, stmax;
+};
+
+struct File {
+    FILE *f = nullptr;
+    ~File() { if (f) fclose(f); }
+};
+
+void wr(FILE *f, const void *p, size_t b) {
+    if (b && fwrite(p, 1, b, f) != b) throw EngineError(SFD_EIO, "L-BFGS state: write failed");
+}
+void rd(FILE *f, void *p, size_t b) {
+    if (b && fread(p, 1, b, f) != b) throw EngineError(SFD_EIO, "L-BFGS state: file truncated");
+}
+}  // namespace
+
+void LbfgsDev::save_state(const char *path) {
+    StateHead h{};
+    memcpy(h.magic, kStateMagic, 8);
+    h.n = n; h.m = m; h.nparam = nparam;
+    h.iter = iter; h.nfun = nfun; h.point = point; h.maxfev = maxfev; h.info = info; h.bound = bound;
+    h.npt = npt; h.cp = cp; h.nfev = nfev; h.finish = finish; h.infoc = infoc; h.brackt = brackt;
+    h.stage1 = stage1;
+    h.gnorm = gnorm; h.xnorm2 = xnorm2; h.stp1 = stp1; h.ftol = ftol; h.stp = stp; h.ys = ys; h.yy = yy;
+    h.gtol = gtol; h.stpmin = stpmin; h.stpmax = stpmax; h.xtrapf = xtrapf;
+    h.dginit = dginit; h.dgtest = dgtest; h.finit = finit; h.width = width; h.width1 = width1; h.stx = stx;
+    h.fx = fx; h.dgx = dgx; h.sty = sty; h.fy = fy; h.dgy = dgy; h.stmin = stmin; h.stmax = stmax;
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        File fo;
+        fo.f = fopen(tmp.c_str(), "wb");
+        if (!fo.f) throw EngineError(SFD_EIO, "L-BFGS state: cannot create " + tmp);
+        wr(fo.f, &h, sizeof h);
+        wr(fo.f, pxmin.data(), sizeof(double) * nparam);
+        wr(fo.f, pxmax.data(), sizeof(double) * nparam);
+        wr(fo.f, rho.data(), sizeof(double) * m);
+        wr(fo.f, alpha.data(), sizeof(double) * m);
+        std::vector<double> buf(n);
+        auto put = [&](const double *dv) {
+            HIPCHK(hipMemcpyAsync(buf.data(), dv, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            wr(fo.f, buf.data(), sizeof(double) * n);
+        };
+        put(q);
+        put(diag);
+        for (int k = 0; k < m; k++) put(Sv(k));
+        for (int k = 0; k < m; k++) put(Yv(k));
+        if (fflush(fo.f) != 0) throw EngineError(SFD_EIO, "L-BFGS state: write failed");
+    }
+    if (rename(tmp.c_str(), path) != 0) throw EngineError(SFD_EIO, std::string("L-BFGS state: cannot rename to ") + path);
+}
+
+void LbfgsDev::load_state(const char *path) {
+    File fi;
+    fi.f = fopen(path, "rb");
+    if (!fi.f) throw EngineError(SFD_EIO, std::string("L-BFGS state: cannot open ") + path);
+    StateHead h;
+    rd(fi.f, &h, sizeof h);
+    if (memcmp(h.magic, kStateMagic, 8) != 0) throw EngineError(SFD_EIO, "L-BFGS state: not a state file");
+    if (h.n != n || h.m != m || h.nparam != nparam)
+        throw EngineError(SFD_EIO, "L-BFGS state: n, m or nparam differ from the run's");
+    std::vector<double> b0(nparam), b1(nparam);
+    rd(fi.f, b0.data(), sizeof(double) * nparam);
+    rd(fi.f, b1.data(), sizeof(double) * nparam);
+    if (b0 != pxmin || b1 != pxmax) throw EngineError(SFD_EIO, "L-BFGS state: bounds differ from the run's");
+    iter = h.iter; nfun = h.nfun; point = h.point; maxfev = h.maxfev; info = h.info; bound = h.bound;
+    npt = h.npt; cp = h.cp; nfev = h.nfev; finish = h.finish; infoc = h.infoc; brackt = h.brackt;
+    stage1 = h.stage1;
+    gnorm = h.gnorm; xnorm2 = h.xnorm2; stp1 = h.stp1; ftol = h.ftol; stp = h.stp; ys = h.ys; yy = h.yy;
+    gtol = h.gtol; stpmin = h.stpmin; stpmax = h.stpmax; xtrapf = h.xtrapf;
+    dginit = h.dginit; dgtest = h.dgtest; finit = h.finit; width = h.width; width1 = h.width1; stx = h.stx;
+    fx = h.fx; dgx = h.dgx; sty = h.sty; fy = h.fy; dgy = h.dgy; stmin = h.stmin; stmax = h.stmax;
+    rd(fi.f, rho.data(), sizeof(double) * m);
+    rd(fi.f, alpha.data(), sizeof(double) * m);
+    std::vector<double> buf(n);
+    auto get = [&](double *dv) {
+        rd(fi.f, buf.data(), sizeof(double) * n);
+        HIPCHK(hipMemcpyAsync(dv, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    };
+    get(q);
+    get(diag);
+    for (int k = 0; k < m; k++) get(Sv(k));
+    for (int k = 0; k < m; k++) get(Yv(k));
+    if (fgetc(fi.f) != EOF) throw EngineError(SFD_EIO, "L-BFGS state: trailing data");
 }
 
 // ---------------------------------------------------------------------------
@@ -522,6 +635,35 @@ int sfd_lbfgs_run(const sfd_lbfgs_opts *o, int n, float *x, const float *vpmask,
     try {
         LbfgsFct f = [&](const float *xx, float *g, double *fv) { return fct(ctx, xx, g, fv); };
         lbfgs_drive(*o, n, x, vpmask, vpinit, f, rep, nullptr);
+        return SFD_OK;
+    } catch (const EngineError &e) {
+        return e.code;
+    }
+}
+
+int sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfgsoc_opts *o, int n, double *x,
+                     double f, const double *g, int *iflag, int *info) {
+    if (!o || !state_out || n < 1 || !x || !g || !iflag || !info || o->m < 1 || o->nparam < 1 ||
+        o->nparam > n || !o->xmin || !o->xmax)
+        return SFD_EINVAL;
+    if (*iflag != 0 && *iflag != 1) return SFD_EINVAL;   // DIAGCO = .FALSE.: no IFLAG 2 re-entry
+    if (*iflag == 1 && !state_in) return SFD_EINVAL;
+    try {
+        LbfgsDev d(n, o->m, 0.0, 0.0, nullptr, o->clip_type, 1, nullptr);
+        d.nparam = o->nparam;
+        d.pxmin.assign(o->xmin, o->xmin + o->nparam);
+        d.pxmax.assign(o->xmax, o->xmax + o->nparam);
+        d.stp1_opt = o->stp1_opt;
+        d.step_init = o->step_init;
+        d.xtrapf = 4.0;   // MCSRCHOC's extrapolation bound (nlinv_lbfgs.f:752)
+        if (*iflag == 1) d.load_state(state_in);
+        const size_t b = (size_t)n * sizeof(double);
+        HIPCHK(hipMemcpyAsync(d.x, x, b, hipMemcpyHostToDevice, d.st));
+        HIPCHK(hipMemcpyAsync(d.g, g, b, hipMemcpyHostToDevice, d.st));
+        d.step(f, iflag, info, 1.0e-20, 2.220446049250313e-16);   // nlinv_read_array_params.f90:227-228
+        HIPCHK(hipMemcpyAsync(x, d.x, b, hipMemcpyDeviceToHost, d.st));
+        HIPCHK(hipStreamSynchronize(d.st));
+        if (*iflag >= 0) d.save_state(state_out);
         return SFD_OK;
     } catch (const EngineError &e) {
         return e.code;

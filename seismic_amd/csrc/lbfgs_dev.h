@@ -80,4 +80,14 @@ __global__ void freeze_kernel(long n, double xmin, double xmax, const float *__r
     }
 }
 
+// FREEZE_XOC (nlinv_lbfgs.f:1-22) on one parameter block x[off, off + len)
+__global__ void clip_block_kernel(long off, long len, double lo, double hi, double *__restrict__ x) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len; i += (long)gridDim.x * blockDim.x) {
+        double v = x[off + i];
+        v = fmax(lo, v);
+        v = fmin(hi, v);
+        x[off + i] = v;
+    }
+}
+
 }  // namespace sfd

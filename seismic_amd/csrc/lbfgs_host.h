@@ -40,6 +40,11 @@ struct LbfgsDev {
     bool h0_active = false;
     bool used_h0 = false;      // the current search direction came from the learned H0
     int h0_rejects = 0;        // learned directions replaced because they were not descent directions
+    // out-of-core variant (LBFGSOC): per-parameter clip, its STP1 rule
+    int nparam = 0;                      // > 0: FREEZE_XOC blocks instead of FREEZE_X
+    std::vector<double> pxmin, pxmax;
+    int stp1_opt = -1;                   // < 0: libinv's rule (lbfgs.f:333-338)
+    double step_init = 999999.0;
     // optional observer of every accepted (gradient, step) pair
     void (*on_pair)(void *ctx, const double *gk, const double *sk, long n, hipStream_t st) = nullptr;
     void *pair_ctx = nullptr;
@@ -59,5 +64,8 @@ struct LbfgsDev {
     void two_loop(bool use_h0);
     int restart_plain();
     void mul_diag(double *v);   // v *= DIAG
+    // the complete optimiser state (scalars, W = q/rho/alpha/S/Y, DIAG)
+    void save_state(const char *path);
+    void load_state(const char *path);
 };
 

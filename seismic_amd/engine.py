@@ -8,6 +8,7 @@ column-major).  All compute runs in libseisfd.so on the GPU; there is no CPU
 fallback: a missing library or device raises.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -372,6 +373,37 @@ class CNN:
                 self.h = None
         except Exception:
             pass
+
+
+class LBFGSOC:
+    """Out-of-core L-BFGS (LBFGSOC of libs/libnlinv/nlinv_lbfgs.f): every
+    step(x, f, g) is one sfd_lbfgsoc_step call that reads the optimiser state
+    from `state_path` (except the first) and writes it back, so the object
+    holds nothing but the options, IFLAG and MYINFO and a run can be resumed
+    from the file by a new object (resume=True) in another process."""
+
+    def __init__(self, n, state_path, m=5, xmin=(-1e30,), xmax=(1e30,), stp1_opt=1, step_init=999999.0,
+                 clip_type=2, resume=False, info=0):
+        self.lib = abi.load()
+        self.n = int(n)
+        self.path = os.fsencode(state_path)
+        self._xmin = np.ascontiguousarray(xmin, np.float64)
+        self._xmax = np.ascontiguousarray(xmax, np.float64)
+        if self._xmin.shape != self._xmax.shape or self._xmin.ndim != 1:
+            raise ValueError("xmin and xmax need one value per parameter block")
+        self.opts = abi.sfd_lbfgsoc_opts(int(m), self._xmin.size, self._xmin.ctypes.data, self._xmax.ctypes.data,
+                                         int(stp1_opt), float(step_init), int(clip_type))
+        self.iflag = C.c_int(1 if resume else 0)
+        self.info = C.c_int(int(info))
+
+    def step(self, x, f, g):
+        assert x.dtype == np.float64 and x.flags.c_contiguous and x.size == self.n
+        gg = np.ascontiguousarray(g, np.float64)
+        rc = self.lib.sfd_lbfgsoc_step(self.path, self.path, C.byref(self.opts), self.n, _ptr(x), float(f),
+                                       _ptr(gg), C.byref(self.iflag), C.byref(self.info))
+        if rc != 0:
+            raise SeisFDError(rc, "sfd_lbfgsoc_step failed")
+        return self.iflag.value, self.info.value
 
 
 class LBFGS:

@@ -6,6 +6,15 @@
 ! usage: seisfd_example OUTDIR
 !   writes OUTDIR/vel.f32 (nz,nx), OUTDIR/wavelet.f32 (nt), OUTDIR/gather.f32
 !   (nt,ntr) and OUTDIR/lbfgs.txt (iterations, evaluations, final misfit).
+!
+! usage: seisfd_example --lbfgsoc DIR
+!   one model update of the out-of-core L-BFGS per process, as
+!   progs/inversion/L_BFGS.f90 runs LBFGSOC: reads DIR/opts.txt (n m nparam
+!   stp1_opt step_init clip_type, then xmin(1:nparam), xmax(1:nparam)),
+!   DIR/ctl.txt (iflag info; absent = 0 0, like Iflag.dat / Info.dat),
+!   DIR/model.f64 (n doubles) and DIR/fg.f64 (f, then n doubles of g);
+!   keeps the optimiser state in DIR/state.bin and writes the next model to
+!   DIR/model.f64 and "iflag info" to DIR/ctl.txt.
 module example_fct
   use iso_c_binding
   implicit none
@@ -41,6 +50,7 @@ program seisfd_example
   type(sfd_lbfgs_opts) :: lo
   type(sfd_lbfgs_report) :: rep
   type(sfd_fwi_opts) :: fo
+  type(sfd_lbfgsoc_opts) :: oco
   type(c_ptr) :: h
   character(len=512) :: outdir
   integer :: i, j, rc, u
@@ -49,7 +59,13 @@ program seisfd_example
   call get_command_argument(1, outdir)
   if (trim(outdir) == '--sizes') then
      ! struct sizes as Fortran sees them (checked against C by the tests)
-     write(*, '(5i6)') c_sizeof(g), c_sizeof(shot(1)), c_sizeof(lo), c_sizeof(rep), c_sizeof(fo)
+     write(*, '(6i6)') c_sizeof(g), c_sizeof(shot(1)), c_sizeof(lo), c_sizeof(rep), c_sizeof(fo), &
+          c_sizeof(oco)
+     stop
+  end if
+  if (trim(outdir) == '--lbfgsoc') then
+     call get_command_argument(2, outdir)
+     call lbfgsoc_update(trim(outdir))
      stop
   end if
   pi = 3.14159265
@@ -118,4 +134,57 @@ program seisfd_example
   close(u)
   write(*, '(a, i0, a, i0, a, es10.3)') 'seisfd_example: gather done; l_bfgs iterations ', rep%iterations, &
        ' evaluations ', rep%evaluations, ' max|x-x*| ', maxval(abs(dble(x) - xstar))
+contains
+
+  ! One LBFGSOC update (progs/inversion/L_BFGS.f90:54-98) on the files of dir.
+  subroutine lbfgsoc_update(dir)
+    character(len=*), intent(in) :: dir
+    type(sfd_lbfgsoc_opts) :: o
+    real(c_double), allocatable, target :: xm(:), gd(:), bmin(:), bmax(:)
+    real(c_double) :: fv, step_init
+    integer(c_int) :: n, m, nparam, stp1_opt, clip_type, iflag, info, rc
+    character(len=1024) :: state
+    logical :: have
+    integer :: u
+    open(newunit=u, file=dir//'/opts.txt', status='old')
+    read(u, *) n, m, nparam, stp1_opt, step_init, clip_type
+    allocate(xm(n), gd(n), bmin(nparam), bmax(nparam))
+    read(u, *) bmin
+    read(u, *) bmax
+    close(u)
+    iflag = 0
+    info = 0
+    inquire(file=dir//'/ctl.txt', exist=have)
+    if (have) then
+       open(newunit=u, file=dir//'/ctl.txt', status='old')
+       read(u, *) iflag, info
+       close(u)
+    end if
+    open(newunit=u, file=dir//'/model.f64', access='stream', form='unformatted', status='old')
+    read(u) xm
+    close(u)
+    open(newunit=u, file=dir//'/fg.f64', access='stream', form='unformatted', status='old')
+    read(u) fv, gd
+    close(u)
+    o%m = m
+    o%nparam = nparam
+    o%xmin = c_loc(bmin)
+    o%xmax = c_loc(bmax)
+    o%stp1_opt = stp1_opt
+    o%step_init = step_init
+    o%clip_type = clip_type
+    state = dir//'/state.bin'//c_null_char
+    rc = sfd_lbfgsoc_step(state, state, o, n, xm, fv, gd, iflag, info)
+    if (rc /= 0) then
+       write(0, *) 'sfd_lbfgsoc_step failed: ', rc
+       error stop 2
+    end if
+    open(newunit=u, file=dir//'/model.f64', access='stream', form='unformatted', status='replace')
+    write(u) xm
+    close(u)
+    open(newunit=u, file=dir//'/ctl.txt', status='replace')
+    write(u, '(2i4)') iflag, info
+    close(u)
+  end subroutine lbfgsoc_update
+
 end program seisfd_example

@@ -40,6 +40,15 @@ module seisfd_mod
      real(c_double) :: xmin, xmax
   end type sfd_lbfgs_opts
 
+  ! out-of-core L-BFGS options (LBFGSOC of libs/libnlinv/nlinv_lbfgs.f)
+  type, bind(C) :: sfd_lbfgsoc_opts
+     integer(c_int) :: m, nparam
+     type(c_ptr)    :: xmin, xmax      ! c_loc of nparam real(c_double) bounds
+     integer(c_int) :: stp1_opt
+     real(c_double) :: step_init
+     integer(c_int) :: clip_type
+  end type sfd_lbfgsoc_opts
+
   type, bind(C) :: sfd_lbfgs_report
      integer(c_int) :: iterations, evaluations, iflag, info, nhist, hist_cap
      real(c_double) :: seconds
@@ -136,6 +145,19 @@ module seisfd_mod
        real(c_float), intent(inout) :: grad(*)
        type(c_ptr), value :: illum, resid
      end function sfd_fwi_gradient
+
+     ! one LBFGSOC call; state_in / state_out are NUL-terminated file names
+     integer(c_int) function sfd_lbfgsoc_step(state_in, state_out, o, n, x, f, g, iflag, info) &
+          bind(C, name='sfd_lbfgsoc_step')
+       import :: c_int, c_char, c_double, sfd_lbfgsoc_opts
+       character(kind=c_char), intent(in) :: state_in(*), state_out(*)
+       type(sfd_lbfgsoc_opts), intent(in) :: o
+       integer(c_int), value :: n
+       real(c_double), intent(inout) :: x(*)
+       real(c_double), value :: f
+       real(c_double), intent(in) :: g(*)
+       integer(c_int), intent(inout) :: iflag, info
+     end function sfd_lbfgsoc_step
 
      integer(c_int) function sfd_lbfgs_run(o, n, x, vpmask, vpinit, fct, ctx, rep) bind(C, name='sfd_lbfgs_run')
        import :: c_int, c_ptr, c_float, c_funptr, sfd_lbfgs_opts, sfd_lbfgs_report

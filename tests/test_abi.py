@@ -31,6 +31,8 @@ def test_struct_sizes():
     assert ctypes.sizeof(abi.sfd_grid) == 14 * 4
     assert ctypes.sizeof(abi.sfd_shot) == 8 + 8 + 3 * 8
     assert ctypes.sizeof(abi.sfd_fwi_opts) == 6 * 4
+    # int m, nparam; two pointers; int stp1_opt (+4 pad); double; int clip_type (+4 pad)
+    assert ctypes.sizeof(abi.sfd_lbfgsoc_opts) == 48
 
 
 def test_fortran_binding_struct_sizes():
@@ -42,5 +44,5 @@ def test_fortran_binding_struct_sizes():
         pytest.skip("Fortran example not built")
     out = subprocess.run([exe, "--sizes"], capture_output=True, text=True, timeout=60).stdout.split()
     want = [ctypes.sizeof(t) for t in (abi.sfd_grid, abi.sfd_shot, abi.sfd_lbfgs_opts, abi.sfd_lbfgs_report,
-                                  abi.sfd_fwi_opts)]
+                                  abi.sfd_fwi_opts, abi.sfd_lbfgsoc_opts)]
     assert [int(v) for v in out] == want
