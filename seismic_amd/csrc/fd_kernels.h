@@ -214,63 +214,91 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
 }
 
 // Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
-// the grid edge.  s_b holds the tile's u3, s_u2 / s_u1 its u2 / u1.  z-bands
-// first (one thread per column with ix in [nmin2, nmax2]), then x-bands (one
-// thread per row with iz in [nmin1, nmax1]), each a sequential recursion
-// outwards, exactly in the reference's order.
+// the grid edge.  s_b holds the tile's u3, s_u2 / s_u1 its u2 / u1, column
+// stride TP = TZ + 1 (a thread walking a column and the lanes of a row then
+// fall on distinct LDS banks).  z-bands first (one thread per column with ix
+// in [nmin2, nmax2]), then x-bands (one thread per row with iz in [nmin1,
+// nmax1]), each a sequential recursion outwards, exactly in the reference's
+// order.  The coefficients of a thread's lines are loaded before the tile is
+// staged (higdon_load_g) so their latency overlaps the staging barrier.
+struct HigdonG {
+    float z[8], x[8];
+};
+
 template <int TZ, int TX>
-__device__ __forceinline__ void higdon_tile(const ShotDev &S, int z0, int x0, bool top, bool bot, bool left,
-                                            bool right, float *s_b, const float *s_u2, const float *s_u1) {
-    const int NZ = S.NZ, NX = S.NX, tid = threadIdx.x;
-#define L(i, j) ((i) - z0 + TZ * ((j) - x0))
-    if (top || bot) {
-        for (int l = tid; l < 2 * TX; l += (int)blockDim.x) {
-            const int cx = l % TX;
-            const bool dotop = l < TX;
-            if (dotop ? !top : !bot) continue;
-            const int ix = x0 + cx;
-            if (ix < 4 || ix > NX - 5) continue;
+__device__ __forceinline__ void higdon_load_g(const ShotDev &S, int z0, int x0, bool top, bool bot, bool left,
+                                              bool right, HigdonG &hg) {
+    const int tid = threadIdx.x, NZ = S.NZ, NX = S.NX;
+    if ((top || bot) && tid < 2 * TX) {
+        const bool dotop = tid < TX;
+        const int ix = x0 + tid % TX;
+        if ((dotop ? top : bot) && ix >= 4 && ix <= NX - 5) {
             const GLB float *g = gp(S.hz) + 16 * (ix - 4) + (dotop ? 0 : 8);
-            if (dotop) {
-                for (int i = 7; i >= 0; i--) {
-                    const float f = g[0] * s_b[L(i + 1, ix)] + g[1] * s_b[L(i + 2, ix)] + g[2] * s_u2[L(i, ix)] +
-                                    g[3] * s_u2[L(i + 1, ix)] + g[4] * s_u2[L(i + 2, ix)] + g[5] * s_u1[L(i, ix)] +
-                                    g[6] * s_u1[L(i + 1, ix)] + g[7] * s_u1[L(i + 2, ix)];
-                    s_b[L(i, ix)] = -f;
-                }
-            } else {
-                for (int i = NZ - 8; i <= NZ - 1; i++) {
-                    const float f = g[0] * s_b[L(i - 1, ix)] + g[1] * s_b[L(i - 2, ix)] + g[2] * s_u2[L(i, ix)] +
-                                    g[3] * s_u2[L(i - 1, ix)] + g[4] * s_u2[L(i - 2, ix)] + g[5] * s_u1[L(i, ix)] +
-                                    g[6] * s_u1[L(i - 1, ix)] + g[7] * s_u1[L(i - 2, ix)];
-                    s_b[L(i, ix)] = -f;
-                }
+#pragma unroll
+            for (int k = 0; k < 8; k++) hg.z[k] = g[k];
+        }
+    }
+    if ((left || right) && tid < 2 * TZ) {
+        const bool doleft = tid < TZ;
+        const int i = z0 + tid % TZ;
+        if ((doleft ? left : right) && i >= 4 && i <= NZ - 5) {
+            const GLB float *g = gp(S.hx) + 16 * (i - 4) + (doleft ? 0 : 8);
+#pragma unroll
+            for (int k = 0; k < 8; k++) hg.x[k] = g[k];
+        }
+    }
+}
+
+// One band line in registers.  Cells c_0 .. c_9 run from the inner side
+// outwards (c_k at LDS index base + k * step): c_0, c_1 are FD cells whose u3
+// is final, c_2 .. c_9 the band cells, each overwritten from the two cells
+// inside it: u3(c_k) = -(g0 u3(c_k-1) + g1 u3(c_k-2) + g2 u2(c_k) + g3 u2(c_k-1)
+// + g4 u2(c_k-2) + g5 u1(c_k) + g6 u1(c_k-1) + g7 u1(c_k-2)), summed left to right.
+__device__ __forceinline__ void higdon_line(const float *g, float *s_b, const float *s_u2, const float *s_u1,
+                                            int base, int step) {
+    float a[10], c[10], b[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        a[k] = s_u2[base + k * step];
+        c[k] = s_u1[base + k * step];
+    }
+    b[0] = s_b[base];
+    b[1] = s_b[base + step];
+#pragma unroll
+    for (int k = 2; k < 10; k++) {
+        const float f = g[0] * b[k - 1] + g[1] * b[k - 2] + g[2] * a[k] + g[3] * a[k - 1] + g[4] * a[k - 2] +
+                        g[5] * c[k] + g[6] * c[k - 1] + g[7] * c[k - 2];
+        b[k] = -f;
+    }
+#pragma unroll
+    for (int k = 2; k < 10; k++) s_b[base + k * step] = b[k];
+}
+
+template <int TZ, int TX>
+__device__ __forceinline__ void higdon_tile(const ShotDev &S, const HigdonG &hg, int z0, int x0, bool top,
+                                            bool bot, bool left, bool right, float *s_b, const float *s_u2,
+                                            const float *s_u1) {
+    constexpr int TP = TZ + 1;
+    const int NZ = S.NZ, NX = S.NX, tid = threadIdx.x;
+#define L(i, j) ((i) - z0 + TP * ((j) - x0))
+    if (top || bot) {
+        if (tid < 2 * TX) {
+            const bool dotop = tid < TX;
+            const int ix = x0 + tid % TX;
+            if ((dotop ? top : bot) && ix >= 4 && ix <= NX - 5) {
+                if (dotop) higdon_line(hg.z, s_b, s_u2, s_u1, L(9, ix), -1);           // rows 9, 8 | 7 .. 0
+                else higdon_line(hg.z, s_b, s_u2, s_u1, L(NZ - 10, ix), 1);            // rows NZ-10, NZ-9 | NZ-8 ..
             }
         }
         __syncthreads();
     }
     if (left || right) {
-        for (int l = tid; l < 2 * TZ; l += (int)blockDim.x) {
-            const int cz = l % TZ;
-            const bool doleft = l < TZ;
-            if (doleft ? !left : !right) continue;
-            const int i = z0 + cz;
-            if (i < 4 || i > NZ - 5) continue;
-            const GLB float *g = gp(S.hx) + 16 * (i - 4) + (doleft ? 0 : 8);
-            if (doleft) {
-                for (int j = 7; j >= 0; j--) {
-                    const float f = g[0] * s_b[L(i, j + 1)] + g[1] * s_b[L(i, j + 2)] + g[2] * s_u2[L(i, j)] +
-                                    g[3] * s_u2[L(i, j + 1)] + g[4] * s_u2[L(i, j + 2)] + g[5] * s_u1[L(i, j)] +
-                                    g[6] * s_u1[L(i, j + 1)] + g[7] * s_u1[L(i, j + 2)];
-                    s_b[L(i, j)] = -f;
-                }
-            } else {
-                for (int j = NX - 8; j <= NX - 1; j++) {
-                    const float f = g[0] * s_b[L(i, j - 1)] + g[1] * s_b[L(i, j - 2)] + g[2] * s_u2[L(i, j)] +
-                                    g[3] * s_u2[L(i, j - 1)] + g[4] * s_u2[L(i, j - 2)] + g[5] * s_u1[L(i, j)] +
-                                    g[6] * s_u1[L(i, j - 1)] + g[7] * s_u1[L(i, j - 2)];
-                    s_b[L(i, j)] = -f;
-                }
+        if (tid < 2 * TZ) {
+            const bool doleft = tid < TZ;
+            const int i = z0 + tid % TZ;
+            if ((doleft ? left : right) && i >= 4 && i <= NZ - 5) {
+                if (doleft) higdon_line(hg.x, s_b, s_u2, s_u1, L(i, 9), -TP);          // columns 9, 8 | 7 .. 0
+                else higdon_line(hg.x, s_b, s_u2, s_u1, L(i, NX - 10), TP);            // columns NX-10, NX-9 | ..
             }
         }
     }
@@ -299,6 +327,7 @@ template <int PX, bool ADJ>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
     constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
+    static_assert(NT >= 2 * TX && NT >= 2 * TZ, "one thread per Higdon line");
     __shared__ float s_col[NW][72];     // per-wavefront z-stencil column
     // Injection tables are read only inside the column loop and the edge
     // buffers are written only after it (behind a barrier on injection
@@ -311,9 +340,9 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             float4v coef[kMaxTileCoef];     // the tile's contributions
         } inj;
         struct {
-            float b[TX * TZ];               // u3
-            float u2[TX * TZ];              // u2
-            float u1[TX * TZ];              // u1
+            float b[TX * (TZ + 1)];         // u3, column stride TZ + 1
+            float u2[TX * (TZ + 1)];        // u2
+            float u1[TX * (TZ + 1)];        // u1
         } edge;
     } s_p;
     int *const s_inj = s_p.inj.cell;
@@ -529,19 +558,22 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 
     const bool top = z0 <= 0, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
     if (top || bot || left || right) {
+        HigdonG hg;
+        higdon_load_g<TZ, TX>(S, z0, x0, top, bot, left, right, hg);
         if (has_inj) __syncthreads();   // injection tables read before the edge buffers overwrite them
         float *s_b = s_p.edge.b;
+        constexpr int TP = TZ + 1;
 #pragma unroll
         for (int p = 0; p < PX; p++) {
-            const int l = lane + TZ * (wv * PX + p);
+            const int l = lane + TP * (wv * PX + p);
             s_b[l] = u3v[p];
             s_p.edge.u2[l] = wu[p + 4];
             s_p.edge.u1[l] = r1[p];
         }
         __syncthreads();
-        higdon_tile<TZ, TX>(S, z0, x0, top, bot, left, right, s_b, s_p.edge.u2, s_p.edge.u1);
+        higdon_tile<TZ, TX>(S, hg, z0, x0, top, bot, left, right, s_b, s_p.edge.u2, s_p.edge.u1);
 #pragma unroll
-        for (int p = 0; p < PX; p++) u3v[p] = s_b[lane + TZ * (wv * PX + p)];
+        for (int p = 0; p < PX; p++) u3v[p] = s_b[lane + TP * (wv * PX + p)];
     }
 
     // ---- write u3 on every non-corner cell of the strip (FD region from the
