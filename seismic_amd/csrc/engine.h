@@ -42,6 +42,7 @@ struct InjSet {
     // region plan of the two-step kernel
     const int *tile2 = nullptr, *cell2 = nullptr, *cptr2 = nullptr, *tptr2 = nullptr, *ttr2 = nullptr;
     const sfd::InjCoef *coef2 = nullptr;
+    int lo = 0, hi = 0;   // range of injecting tiles
     void apply(sfd::ShotDev &S) const;
 };
 

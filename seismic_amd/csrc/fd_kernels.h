@@ -84,6 +84,7 @@ struct ShotDev {
     const InjCoef *inj_coef; // per contribution
     const int *inj_tptr;    // per tile [inj_tptr[t], inj_tptr[t+1]) traces it injects
     const int *inj_ttr;     // global trace index of each tile-local trace
+    int inj_lo, inj_hi;     // every injecting tile t lies in [inj_lo, inj_hi)
     // the same injections planned per 72x40 region (tile + 4-cell ring) for
     // the two-step kernel: region-local cells, per-region trace tables
     const int *inj2_tile, *inj2_cell, *inj2_cptr, *inj2_tptr, *inj2_ttr;
@@ -96,6 +97,7 @@ struct ShotDev {
     const float *ext_w;     // 16 floats per receiver: gain, wz[7], wx[7], pad
     float *traces;          // extracted traces (nt, ntr)
     int ext_kind;
+    int ext_lo, ext_hi;     // every extracting tile t lies in [ext_lo, ext_hi)
     // snapshots (window interior, nz*nxw per snapshot)
     void *snap;             // float* or __half*
     float *snap_scale;      // FP16 mode: max|S| over each tile's window cells, ntz*ntx per snapshot
@@ -323,7 +325,11 @@ struct StepShape {
     static constexpr int WAVES_PER_EU = PX == 4 ? 6 : 1;
 };
 
-template <int PX, bool ADJ>
+// SNAP = false compiles the body without the snapshot, imaging and Born code:
+// the host launches it for steps where no shot of the batch is at a multiple
+// of snapi (three steps in four), which keeps those steps' code and registers
+// to the stencil, the update, injection, extraction and the Higdon bands.
+template <int PX, bool ADJ, bool SNAP>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
     constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
@@ -384,7 +390,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 
     // ---- receivers centred in this tile (reads u2 only; done before the
     // streaming loads so its registers are not live alongside them)
-    if (A.extract && S.ext_tile && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t]) extract_tile(S, t, it, u2, z0, x0);
+    if (A.extract && S.ext_tile && t >= S.ext_lo && t < S.ext_hi && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t])
+        extract_tile(S, t, it, u2, z0, x0);
 
     // ---- loads: u2 (and v^2 for the adjoint) window, halos, own u1/u0/u_-1/v^2.
     // Addresses are clamped into the array and the loads are unconditional,
@@ -430,7 +437,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // ---- imaging steps: the source snapshot, image, illumination and taper
     // of this lane's cells, issued with the streaming loads (clamped window
     // index; only window cells use them) so the column loop does not wait.
-    const bool snap_step = (its % A.snapi) == 0 && (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
+    const bool snap_step = SNAP && (its % A.snapi) == 0 &&
+                           (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
     const bool img_step = snap_step && (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM);
     float pS[PX], pI[PX], pL[PX], pT[PX];
@@ -454,7 +462,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // table, and this step's sample of every trace the tile injects, staged
     // in LDS so a contribution costs one packed load and one LDS read.
     const GLB int *inj_tile = gp(S.inj_tile);
-    const int has_inj = A.inject && (inj_tile[t + 1] > inj_tile[t]);
+    const int has_inj = A.inject && t >= S.inj_lo && t < S.inj_hi && (inj_tile[t + 1] > inj_tile[t]);
     bool coef_lds = false;
     if (has_inj) {
         for (int l = tid; l < TX * TZ; l += NT) s_inj[l] = 0;
@@ -546,7 +554,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             }
         }
         // Born secondary source: Imaging.f90:779-783, forward time only
-        if (A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
+        if (SNAP && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
             const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t);
@@ -593,15 +601,15 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // Kernel entry: SINGLE batches read the shot descriptor from the kernel
 // arguments (scalar loads, no dependent table lookups); batched launches
 // find their shot through the logical-tile table.
-template <int PX, bool ADJ, bool SINGLE>
+template <int PX, bool ADJ, bool SINGLE, bool SNAP>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
 __attribute__((amdgpu_waves_per_eu(StepShape<PX>::WAVES_PER_EU))) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
-        fd_step_body<PX, ADJ>(A, A.s0, lb);
+        fd_step_body<PX, ADJ, SNAP>(A, A.s0, lb);
     } else {
-        fd_step_body<PX, ADJ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
+        fd_step_body<PX, ADJ, SNAP>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

@@ -278,8 +278,19 @@ static RegionPlan region_plan(const ShotPlan &p, const std::vector<int> &tile, c
     return r;
 }
 
+// [lo, hi): the tiles t with tab[t+1] > tab[t] (empty: 0, 0)
+static void tile_range(const std::vector<int> &tab, int &lo, int &hi) {
+    lo = hi = 0;
+    for (int t = 0; t + 1 < (int)tab.size(); t++)
+        if (tab[t + 1] > tab[t]) {
+            if (hi == 0) lo = t;
+            hi = t + 1;
+        }
+}
+
 void InjSet::apply(sfd::ShotDev &S) const {
     S.inj_tile = tile; S.inj_cell = cell; S.inj_cptr = cptr; S.inj_coef = coef; S.inj_tptr = tptr; S.inj_ttr = ttr;
+    S.inj_lo = lo; S.inj_hi = hi;
     S.inj2_tile = tile2; S.inj2_cell = cell2; S.inj2_cptr = cptr2; S.inj2_coef = coef2; S.inj2_tptr = tptr2;
     S.inj2_ttr = ttr2;
 }
@@ -540,6 +551,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.inj_coef = (const InjCoef *)(base + o.inj_coef);
         S.inj_tptr = (const int *)(base + o.inj_tptr);
         S.inj_ttr = (const int *)(base + o.inj_ttr);
+        tile_range(p.inj_tile, S.inj_lo, S.inj_hi);
         S.inj_traces = (const float *)(base + o.inj_traces);
         if (need.backward_injection) {
             InjSet &I = B.injb[s];
@@ -549,6 +561,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
             I.coef = (const InjCoef *)(base + o.injb_coef);
             I.tptr = (const int *)(base + o.injb_tptr);
             I.ttr = (const int *)(base + o.injb_ttr);
+            tile_range(p.injb_tile, I.lo, I.hi);
             if (tb) {
                 I.tile2 = (const int *)(base + o.injb2[0]); I.cell2 = (const int *)(base + o.injb2[1]);
                 I.cptr2 = (const int *)(base + o.injb2[2]); I.coef2 = (const InjCoef *)(base + o.injb2[3]);
@@ -566,6 +579,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.ext_w = (const float *)(base + o.ext_w);
         S.traces = (float *)(base + o.traces);
         S.ext_kind = p.ext_kind;
+        tile_range(p.ext_tile, S.ext_lo, S.ext_hi);
         S.snap = base + o.snap;
         S.snap_scale = need.snap_fp16 ? (float *)(base + o.snap_scale) : nullptr;
         S.image = (float *)(base + o.image);
@@ -634,25 +648,39 @@ typedef void (*StepFn)(StepArgs);
 
 // Tiles are 64 rows x TX columns, one wavefront per 64 x PX strip.
 static int pick_px(int TZ, int TX) { return TZ != 64 ? 0 : (TX == 32 ? 4 : (TX == 64 ? 16 : 0)); }
-static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int px_override = 0) {
+template <int PX, bool SNAP>
+static StepFn step_fn(bool adj, bool single) {
+    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, SNAP> : (StepFn)fd_step_kernel<PX, true, false, SNAP>)
+               : (single ? (StepFn)fd_step_kernel<PX, false, true, SNAP> : (StepFn)fd_step_kernel<PX, false, false, SNAP>);
+}
+static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, bool snap, int px_override = 0) {
     const int want = px_override ? px_override : pick_px(TZ, TX);
 #define K(px)                                                                                            \
     if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
-        return adj ? (single ? (StepFn)fd_step_kernel<px, true, true> : (StepFn)fd_step_kernel<px, true, false>) \
-                   : (single ? (StepFn)fd_step_kernel<px, false, true> : (StepFn)fd_step_kernel<px, false, false>);
+        return snap ? step_fn<px, true>(adj, single) : step_fn<px, false>(adj, single);
     K(4) K(8) K(16)
 #undef K
     return nullptr;
 }
 
-bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, false) != nullptr; }
+bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, false, true) != nullptr; }
+
+// Whether some shot of the batch is at a step it = its (its % snapi == 0) of
+// its record in launch n (fd_step_body: its = n+1 forward, nt-n backward).
+static bool snap_launch(const Batch &B, const StepArgs &A) {
+    for (int s = 0; s < B.ns; s++) {
+        const int its = A.dir > 0 ? A.n + 1 : B.shots_h[s].nt - A.n;
+        if (its % A.snapi == 0) return true;
+    }
+    return false;
+}
 
 void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     if (B.ns == 1) {
         A.single = 1;
         A.s0 = B.shots_h[0];
     }
-    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, strip_px);
+    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, snap_launch(B, A), strip_px);
     if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
     int grid = B.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
