@@ -30,6 +30,7 @@ struct ShotPlan {
     std::vector<float> inj_coef;   // 4 per contribution: v2r, sinc_z, sinc_x, local trace (int bits)
     std::vector<int> injb_tile, injb_cell, injb_cptr, injb_tptr, injb_ttr;
     std::vector<float> injb_coef;
+    std::vector<int> inj_desc, inj_map, injb_desc, injb_map;   // step-kernel form of the two sets
     std::vector<int> ext_tile, ext_rec, ext_pos;
     std::vector<float> ext_w;
     int ext_kind = 0;
@@ -43,6 +44,7 @@ struct InjSet {
     const int *tile2 = nullptr, *cell2 = nullptr, *cptr2 = nullptr, *tptr2 = nullptr, *ttr2 = nullptr;
     const sfd::InjCoef *coef2 = nullptr;
     int lo = 0, hi = 0;   // range of injecting tiles
+    const int *desc = nullptr, *map = nullptr;
     void apply(sfd::ShotDev &S) const;
 };
 
@@ -115,8 +117,8 @@ struct Engine {
                       const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
 
     static bool tile_supported(int TZ, int TX);
-    void check_shots(const sfd_shot *shots, int nshots);
-    void autotune(const sfd_shot *shots, int ns);
+    void check_shots(const sfd_shot *shots, int nshots, int snaps = -1);
+    void autotune(const sfd_shot *shots, int ns, int snaps);
     float padded_vel(const std::vector<float> &vh, const ShotPlan &p, int icz, int icx) const;
     void plan_shot(const sfd_shot &s, ShotPlan &p) const;
     void plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz, const float *sx,

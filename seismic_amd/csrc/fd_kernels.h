@@ -85,6 +85,12 @@ struct ShotDev {
     const int *inj_tptr;    // per tile [inj_tptr[t], inj_tptr[t+1]) traces it injects
     const int *inj_ttr;     // global trace index of each tile-local trace
     int inj_lo, inj_hi;     // every injecting tile t lies in [inj_lo, inj_hi)
+    // per tile 8 ints {first contribution, end, first tile-local trace (into
+    // inj_ttr), end, offset of the tile's cell map in inj_map, 0, 0, 0}; the
+    // cell map holds, for every cell of an injecting tile (lz + TZ*lx), its
+    // contribution range relative to the tile's first: start << 8 | count
+    const int *inj_desc;
+    const int *inj_map;
     // the same injections planned per 72x40 region (tile + 4-cell ring) for
     // the two-step kernel: region-local cells, per-region trace tables
     const int *inj2_tile, *inj2_cell, *inj2_cptr, *inj2_tptr, *inj2_ttr;
@@ -321,8 +327,10 @@ struct StepShape {
     static constexpr int TX = PX >= 8 ? 4 * PX : 32;   // >= 19 columns: Higdon bands stay in one tile
     static constexpr int NW = TX / PX;                   // wavefronts per workgroup
     static constexpr int THREADS = 64 * NW;
-    // 64 x 4 strips: three 8-wave workgroups per CU need <= 80 VGPRs (6 waves per SIMD)
+    // 64 x 4 strips: three 8-wave workgroups per CU need <= 80 VGPRs (6 waves
+    // per SIMD), four need <= 64 (8 waves; the lean variant's LDS allows four)
     static constexpr int WAVES_PER_EU = PX == 4 ? 6 : 1;
+    static constexpr int WAVES_LEAN = PX == 4 ? 8 : 1;
 };
 
 // SNAP = false compiles the body without the snapshot, imaging and Born code:
@@ -338,12 +346,12 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // Injection tables are read only inside the column loop and the edge
     // buffers are written only after it (behind a barrier on injection
     // tiles), so the two phases share one LDS block.  Keeping the block small
-    // is what lets three 512-thread workgroups share a CU.
+    // is what lets four 512-thread workgroups share a CU.
     __shared__ union {
         struct {
-            int cell[TX * TZ];              // per-cell contribution range (start<<8 | count)
             float trv[kMaxTileTraces];      // this step's trace samples
             float4v coef[kMaxTileCoef];     // the tile's contributions
+            int cmap[SNAP ? TX * TZ : 1];   // SNAP: each thread's cell-map entries, parked
         } inj;
         struct {
             float b[TX * (TZ + 1)];         // u3, column stride TZ + 1
@@ -351,7 +359,6 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             float u1[TX * (TZ + 1)];        // u1
         } edge;
     } s_p;
-    int *const s_inj = s_p.inj.cell;
     float *const s_trv = s_p.inj.trv;
     float4v *const s_coef = s_p.inj.coef;
 
@@ -458,30 +465,38 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         }
     }
 
-    // ---- injection tiles (workgroup-uniform branch): per-cell contribution
-    // table, and this step's sample of every trace the tile injects, staged
-    // in LDS so a contribution costs one packed load and one LDS read.
-    const GLB int *inj_tile = gp(S.inj_tile);
-    const int has_inj = A.inject && t >= S.inj_lo && t < S.inj_hi && (inj_tile[t + 1] > inj_tile[t]);
+    // ---- injection tiles (workgroup-uniform branch): the tile's descriptor,
+    // each lane's cell-map entries (read straight into registers), and this
+    // step's sample of every trace the tile injects plus the tile's
+    // contributions staged in LDS, so a contribution costs one packed load and
+    // one LDS read.  One barrier; no table walk depends on another load.
+    bool has_inj = A.inject && t >= S.inj_lo && t < S.inj_hi;
+    int ic0 = 0, ic1 = 0, it0 = 0, it1 = 0, imap = 0;
+    if (has_inj) {
+        const GLB int *dd = gp(S.inj_desc) + 8 * t;
+        ic0 = dd[0]; ic1 = dd[1]; it0 = dd[2]; it1 = dd[3]; imap = dd[4];
+        has_inj = ic1 > ic0;
+    }
+    // The lean variant keeps the map entries in registers; the snapshot /
+    // imaging variant, whose registers hold the imaging operands, parks them in
+    // thread-private LDS slots (read back by the same thread: no barrier).
+    int me[PX];
     bool coef_lds = false;
     if (has_inj) {
-        for (int l = tid; l < TX * TZ; l += NT) s_inj[l] = 0;
+        const GLB int *mp = gp(S.inj_map) + imap;
+#pragma unroll
+        for (int p = 0; p < PX; p++) me[p] = mp[lane + TZ * (wv * PX + p)];
+        if constexpr (SNAP) {
+#pragma unroll
+            for (int p = 0; p < PX; p++) s_p.inj.cmap[lane + TZ * (wv * PX + p)] = me[p];
+        }
         const GLB int *ttr = gp(S.inj_ttr);
         const GLB float *trs = gp(S.inj_traces);
-        for (int j = gp(S.inj_tptr)[t] + tid, j0 = gp(S.inj_tptr)[t]; j < gp(S.inj_tptr)[t + 1]; j += NT)
-            s_trv[j - j0] = trs[(size_t)ttr[j] * S.nt + (it - 1)];
-        const GLB int *cptr = gp(S.inj_cptr), *cell = gp(S.inj_cell);
-        const int cb = cptr[inj_tile[t]], ce = cptr[inj_tile[t + 1]];
-        coef_lds = ce - cb <= kMaxTileCoef;
+        for (int j = it0 + tid; j < it1; j += NT) s_trv[j - it0] = trs[(size_t)ttr[j] * S.nt + (it - 1)];
+        coef_lds = ic1 - ic0 <= kMaxTileCoef;
         if (coef_lds) {
             const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
-            for (int j = cb + tid; j < ce; j += NT) s_coef[j - cb] = co[j];
-        }
-        __syncthreads();
-        const int base = coef_lds ? cb : 0;
-        for (int k = inj_tile[t] + tid; k < inj_tile[t + 1]; k += NT) {
-            const int c0 = cptr[k], c1 = cptr[k + 1];
-            s_inj[cell[k]] = ((c0 - base) << 8) | (c1 - c0);
+            for (int j = ic0 + tid; j < ic1; j += NT) s_coef[j - ic0] = co[j];
         }
         __syncthreads();
     }
@@ -538,7 +553,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         float val = ADJ ? (tx + tz) : rv[p] * (tx + tz);
         // Injection: per-cell contributions in trace order (Injection.f90:434-440)
         if (has_inj) {
-            const int e = s_inj[lane + TZ * (wv * PX + p)];
+            const int e = SNAP ? s_p.inj.cmap[lane + TZ * (wv * PX + p)] : me[p];
             // gain, sinc z, sinc x, tile-local trace (int bits)
             if (coef_lds) {
                 for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
@@ -546,7 +561,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
                     val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
                 }
             } else {
-                const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
+                const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
                 for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
                     const float4v w = co[k];
                     val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
@@ -603,7 +618,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // find their shot through the logical-tile table.
 template <int PX, bool ADJ, bool SINGLE, bool SNAP>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
-__attribute__((amdgpu_waves_per_eu(StepShape<PX>::WAVES_PER_EU))) fd_step_kernel(StepArgs A) {
+__attribute__((amdgpu_waves_per_eu(SNAP ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
