@@ -333,12 +333,14 @@ struct StepShape {
     static constexpr int WAVES_LEAN = PX == 4 ? 8 : 1;
 };
 
-// SNAP = false compiles the body without the snapshot, imaging and Born code:
-// the host launches it for steps where no shot of the batch is at a multiple
-// of snapi (three steps in four), which keeps those steps' code and registers
-// to the stencil, the update, injection, extraction and the Higdon bands.
-template <int PX, bool ADJ, bool SNAP>
+// MODE selects the code compiled in: 0 (lean) has no snapshot, imaging or
+// Born code and is launched for steps where no shot of the batch is at a
+// multiple of snapi (three steps in four); 1 adds the snapshot store (forward
+// passes that keep snapshots); 2 adds imaging and the Born source term, whose
+// operands take registers the other modes do not spend.
+template <int PX, bool ADJ, int MODE>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
+    constexpr bool SNAP = MODE != 0, IMG = MODE == 2;
     constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
     static_assert(NT >= 2 * TX && NT >= 2 * TZ, "one thread per Higdon line");
@@ -351,7 +353,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         struct {
             float trv[kMaxTileTraces];      // this step's trace samples
             float4v coef[kMaxTileCoef];     // the tile's contributions
-            int cmap[SNAP ? TX * TZ : 1];   // SNAP: each thread's cell-map entries, parked
+            int cmap[IMG ? TX * TZ : 1];    // MODE 2: each thread's cell-map entries, parked
         } inj;
         struct {
             float b[TX * (TZ + 1)];         // u3, column stride TZ + 1
@@ -447,7 +449,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const bool snap_step = SNAP && (its % A.snapi) == 0 &&
                            (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
-    const bool img_step = snap_step && (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM);
+    const bool img_step = IMG && snap_step && (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM);
     float pS[PX], pI[PX], pL[PX], pT[PX];
     if (img_step) {
         const int wz = iz - S.i0 < 0 ? 0 : (iz - S.i0 >= S.nz ? S.nz - 1 : iz - S.i0);
@@ -477,16 +479,16 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         ic0 = dd[0]; ic1 = dd[1]; it0 = dd[2]; it1 = dd[3]; imap = dd[4];
         has_inj = ic1 > ic0;
     }
-    // The lean variant keeps the map entries in registers; the snapshot /
-    // imaging variant, whose registers hold the imaging operands, parks them in
-    // thread-private LDS slots (read back by the same thread: no barrier).
+    // Modes 0 and 1 keep the map entries in registers; the imaging mode,
+    // whose registers hold the imaging operands, parks them in thread-private
+    // LDS slots (read back by the same thread: no barrier).
     int me[PX];
     bool coef_lds = false;
     if (has_inj) {
         const GLB int *mp = gp(S.inj_map) + imap;
 #pragma unroll
         for (int p = 0; p < PX; p++) me[p] = mp[lane + TZ * (wv * PX + p)];
-        if constexpr (SNAP) {
+        if constexpr (IMG) {
 #pragma unroll
             for (int p = 0; p < PX; p++) s_p.inj.cmap[lane + TZ * (wv * PX + p)] = me[p];
         }
@@ -526,7 +528,11 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const float *c = A.c;
     float *col = s_col[wv];
 
-    float u3v[PX];
+    // Three passes over the strip's PX cells of a lane: stencil (and the
+    // snapshot / imaging point), then the injections of all PX cells
+    // interleaved (each cell's contributions still in its own order, but the
+    // PX chains of LDS reads overlap), then the Born term and the time update.
+    float val[PX];
 #pragma unroll
     for (int p = 0; p < PX; p++) {
         const int ix = xs + p;
@@ -539,7 +545,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
                          c[2] * (col[lane + 6] + col[lane + 2]) + c[3] * (col[lane + 7] + col[lane + 1]) +
                          c[4] * (col[lane + 8] + col[lane]);
         __builtin_amdgcn_wave_barrier();
-        u3v[p] = 0.f;
+        val[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
             snap_image_point(S, A, it, iz, ix, u2c,
@@ -550,33 +556,70 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
                          c[8] * (WX(3) + WX(-3)) + c[9] * (WX(4) + WX(-4));
 #undef WX
-        float val = ADJ ? (tx + tz) : rv[p] * (tx + tz);
-        // Injection: per-cell contributions in trace order (Injection.f90:434-440)
-        if (has_inj) {
-            const int e = SNAP ? s_p.inj.cmap[lane + TZ * (wv * PX + p)] : me[p];
-            // gain, sinc z, sinc x, tile-local trace (int bits)
-            if (coef_lds) {
-                for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
-                    const float4v w = s_coef[k];
-                    val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
+        val[p] = ADJ ? (tx + tz) : rv[p] * (tx + tz);
+    }
+    // Injection: per-cell contributions in trace order (Injection.f90:434-440).
+    // Only FD cells carry contributions (the plan rejects footprints outside).
+    if (has_inj) {
+        int kb[PX], kn[PX];
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            const int e = IMG ? s_p.inj.cmap[lane + TZ * (wv * PX + p)] : me[p];
+            kb[p] = e >> 8;
+            kn[p] = e & 0xff;
+        }
+        // gain, sinc z, sinc x, tile-local trace (int bits).  The PX cells'
+        // loads are unconditional so they issue together: a cell past its
+        // count re-reads the tile's first contribution (staged on every
+        // injecting tile) and its sum is not taken.
+        // All PX cells at once in the reverse-time kernels (receiver lines
+        // inject into every cell of a strip); two at a time in the forward
+        // kernels, whose registers for four would spill.
+        constexpr int NB = ADJ ? PX : 2;
+        auto contribute = [&](int j, int p0, const float4v (&w)[NB]) {
+            float tv[NB];
+#pragma unroll
+            for (int q = 0; q < NB; q++) tv[q] = s_trv[__float_as_int(w[q][3])];
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const int p = p0 + q;
+                const float cv = w[q][0] * tv[q] * w[q][1] * w[q][2];
+                val[p] = j < kn[p] ? val[p] + cv : val[p];
+            }
+        };
+        const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
+#pragma unroll
+        for (int p0 = 0; p0 < PX; p0 += NB) {
+            int km = 0;
+#pragma unroll
+            for (int q = 0; q < NB; q++) km = kn[p0 + q] > km ? kn[p0 + q] : km;
+            for (int j = 0; j < km; j++) {
+                float4v w[NB];
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    const int k = kb[p0 + q] + (j < kn[p0 + q] ? j : 0);
+                    w[q] = coef_lds ? s_coef[k] : co[k];
                 }
-            } else {
-                const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
-                for (int k = e >> 8, kend = (e >> 8) + (e & 0xff); k < kend; k++) {
-                    const float4v w = co[k];
-                    val = val + w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
-                }
+                contribute(j, p0, w);
             }
         }
+    }
+    float u3v[PX];
+#pragma unroll
+    for (int p = 0; p < PX; p++) {
+        const int ix = xs + p;
+        u3v[p] = 0.f;
+        if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
+        float v = val[p];
         // Born secondary source: Imaging.f90:779-783, forward time only
-        if (SNAP && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
+        if (IMG && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
             const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t);
-            val = val + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
+            v = v + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
-        u3v[p] = (20.f * u2c - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * val) * A.div11;
+        u3v[p] = (20.f * wu[p + 4] - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * v) * A.div11;
     }
 
     const bool top = z0 <= 0, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
@@ -616,15 +659,15 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // Kernel entry: SINGLE batches read the shot descriptor from the kernel
 // arguments (scalar loads, no dependent table lookups); batched launches
 // find their shot through the logical-tile table.
-template <int PX, bool ADJ, bool SINGLE, bool SNAP>
+template <int PX, bool ADJ, bool SINGLE, int MODE>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
-__attribute__((amdgpu_waves_per_eu(SNAP ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
+__attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
-        fd_step_body<PX, ADJ, SNAP>(A, A.s0, lb);
+        fd_step_body<PX, ADJ, MODE>(A, A.s0, lb);
     } else {
-        fd_step_body<PX, ADJ, SNAP>(A, A.shots[gp(A.blk_shot)[lb]], lb);
+        fd_step_body<PX, ADJ, MODE>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

@@ -684,31 +684,35 @@ typedef void (*StepFn)(StepArgs);
 
 // Tiles are 64 rows x TX columns, one wavefront per 64 x PX strip.
 static int pick_px(int TZ, int TX) { return TZ != 64 ? 0 : (TX == 32 ? 4 : (TX == 64 ? 16 : 0)); }
-template <int PX, bool SNAP>
+template <int PX, int MODE>
 static StepFn step_fn(bool adj, bool single) {
-    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, SNAP> : (StepFn)fd_step_kernel<PX, true, false, SNAP>)
-               : (single ? (StepFn)fd_step_kernel<PX, false, true, SNAP> : (StepFn)fd_step_kernel<PX, false, false, SNAP>);
+    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, MODE> : (StepFn)fd_step_kernel<PX, true, false, MODE>)
+               : (single ? (StepFn)fd_step_kernel<PX, false, true, MODE> : (StepFn)fd_step_kernel<PX, false, false, MODE>);
 }
-static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, bool snap, int px_override = 0) {
+static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int px_override = 0) {
     const int want = px_override ? px_override : pick_px(TZ, TX);
 #define K(px)                                                                                            \
     if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
-        return snap ? step_fn<px, true>(adj, single) : step_fn<px, false>(adj, single);
+        return mode == 2 ? step_fn<px, 2>(adj, single)                                                   \
+                         : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
     K(4) K(8) K(16)
 #undef K
     return nullptr;
 }
 
-bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, false, true) != nullptr; }
+bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, false, 0) != nullptr; }
 
-// Whether some shot of the batch is at a step it = its (its % snapi == 0) of
-// its record in launch n (fd_step_body: its = n+1 forward, nt-n backward).
-static bool snap_launch(const Batch &B, const StepArgs &A) {
+// The code the launch needs (fd_step_body MODE): 0 unless the call stores
+// snapshots or images and some shot of the batch is at a step it = its
+// (its % snapi == 0) of its record in launch n (its = n+1 forward, nt-n
+// backward); then 2 when it images (or adds the Born source), else 1.
+static int snap_launch(const Batch &B, const StepArgs &A) {
+    if (!A.store_snap && A.imaging == IMG_NONE) return 0;
     for (int s = 0; s < B.ns; s++) {
         const int its = A.dir > 0 ? A.n + 1 : B.shots_h[s].nt - A.n;
-        if (its % A.snapi == 0) return true;
+        if (its % A.snapi == 0) return A.imaging != IMG_NONE ? 2 : 1;
     }
-    return false;
+    return 0;
 }
 
 void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
