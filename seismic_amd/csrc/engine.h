@@ -136,6 +136,7 @@ struct Engine {
     void reset_kstats();
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);
+    void traces_time_major(const Batch &B);
     void gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image, float *illum);
     void scatter_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill);
 };

@@ -86,7 +86,8 @@ struct ShotDev {
     const int *inj_ttr;     // global trace index of each tile-local trace
     int inj_lo, inj_hi;     // every injecting tile t lies in [inj_lo, inj_hi)
     // per tile 8 ints {first contribution, end, first tile-local trace (into
-    // inj_ttr), end, offset of the tile's cell map in inj_map, 0, 0, 0}; the
+    // inj_ttr), end, offset of the tile's cell map in inj_map, the tile's
+    // first global trace when its traces are consecutive (else -1), 0, 0}; the
     // cell map holds, for every cell of an injecting tile (lz + TZ*lx), its
     // contribution range relative to the tile's first: start << 8 | count
     const int *inj_desc;
@@ -96,6 +97,8 @@ struct ShotDev {
     const int *inj2_tile, *inj2_cell, *inj2_cptr, *inj2_tptr, *inj2_ttr;
     const InjCoef *inj2_coef;
     const float *inj_traces;// source traces, (nt, ntraces) time fastest
+    float *inj_tm;          // the same traces time-major, [(it-1) * inj_ntr + trace]
+    int inj_ntr;
     // extraction: per tile [ext_tile[t], ext_tile[t+1]) receivers
     const int *ext_tile;
     const int *ext_rec;     // receiver index
@@ -402,6 +405,34 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     if (A.extract && S.ext_tile && t >= S.ext_lo && t < S.ext_hi && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t])
         extract_tile(S, t, it, u2, z0, x0);
 
+    // ---- injection tiles: descriptor, this step's trace samples and the
+    // lane's cell-map entries are requested before the streaming loads (a
+    // wave's loads return in order, so these come back first); they reach LDS
+    // with the contributions further down, behind one barrier.
+    bool has_inj = A.inject && t >= S.inj_lo && t < S.inj_hi;
+    int ic0 = 0, ic1 = 0, it0 = 0, it1 = 0, imap = 0;
+    constexpr int kTrPerThread = (kMaxTileTraces + NT - 1) / NT;
+    float trv_r[kTrPerThread];
+    int me[PX];
+    if (has_inj) {
+        const GLB int *dd = gp(S.inj_desc) + 8 * t;
+        ic0 = dd[0]; ic1 = dd[1]; it0 = dd[2]; it1 = dd[3]; imap = dd[4];
+        const int tbase = dd[5];
+        has_inj = ic1 > ic0;
+        if (has_inj) {
+            const GLB float *tm = gp((const float *)S.inj_tm) + (size_t)(it - 1) * S.inj_ntr;
+            const GLB int *ttr = gp(S.inj_ttr);
+#pragma unroll
+            for (int r = 0; r < kTrPerThread; r++) {
+                const int j = it0 + tid + r * NT;
+                trv_r[r] = j < it1 ? tm[tbase >= 0 ? tbase + (j - it0) : ttr[j]] : 0.f;
+            }
+            const GLB int *mp = gp(S.inj_map) + imap;
+#pragma unroll
+            for (int p = 0; p < PX; p++) me[p] = mp[lane + TZ * (wv * PX + p)];
+        }
+    }
+
     // ---- loads: u2 (and v^2 for the adjoint) window, halos, own u1/u0/u_-1/v^2.
     // Addresses are clamped into the array and the loads are unconditional,
     // so they issue back to back with no wait in between.  A clamped value
@@ -467,34 +498,23 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         }
     }
 
-    // ---- injection tiles (workgroup-uniform branch): the tile's descriptor,
-    // each lane's cell-map entries (read straight into registers), and this
-    // step's sample of every trace the tile injects plus the tile's
-    // contributions staged in LDS, so a contribution costs one packed load and
-    // one LDS read.  One barrier; no table walk depends on another load.
-    bool has_inj = A.inject && t >= S.inj_lo && t < S.inj_hi;
-    int ic0 = 0, ic1 = 0, it0 = 0, it1 = 0, imap = 0;
-    if (has_inj) {
-        const GLB int *dd = gp(S.inj_desc) + 8 * t;
-        ic0 = dd[0]; ic1 = dd[1]; it0 = dd[2]; it1 = dd[3]; imap = dd[4];
-        has_inj = ic1 > ic0;
-    }
-    // Modes 0 and 1 keep the map entries in registers; the imaging mode,
-    // whose registers hold the imaging operands, parks them in thread-private
-    // LDS slots (read back by the same thread: no barrier).
-    int me[PX];
+    // ---- injection tiles (workgroup-uniform branch): the trace samples and
+    // the tile's contributions staged in LDS, so a contribution costs one
+    // packed load and one LDS read.  Modes 0 and 1 keep the cell-map entries
+    // in registers; the imaging mode, whose registers hold the imaging
+    // operands, parks them in thread-private LDS slots (read back by the same
+    // thread: no barrier).
     bool coef_lds = false;
     if (has_inj) {
-        const GLB int *mp = gp(S.inj_map) + imap;
-#pragma unroll
-        for (int p = 0; p < PX; p++) me[p] = mp[lane + TZ * (wv * PX + p)];
         if constexpr (IMG) {
 #pragma unroll
             for (int p = 0; p < PX; p++) s_p.inj.cmap[lane + TZ * (wv * PX + p)] = me[p];
         }
-        const GLB int *ttr = gp(S.inj_ttr);
-        const GLB float *trs = gp(S.inj_traces);
-        for (int j = it0 + tid; j < it1; j += NT) s_trv[j - it0] = trs[(size_t)ttr[j] * S.nt + (it - 1)];
+#pragma unroll
+        for (int r = 0; r < kTrPerThread; r++) {
+            const int j = tid + r * NT;
+            if (it0 + j < it1) s_trv[j] = trv_r[r];
+        }
         coef_lds = ic1 - ic0 <= kMaxTileCoef;
         if (coef_lds) {
             const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
@@ -741,6 +761,22 @@ __global__ void higdon_tables_kernel(const float *__restrict__ vel, int NZ, int 
 
 // Step base of replayed graphs (one thread; plain vector stores).
 __global__ void set_base_kernel(int *nbase, int v) { *nbase = v; }
+
+// src (ntr, nt) time fastest -> dst (nt, ntr) trace fastest, through 32 x 32
+// LDS tiles (both sides coalesced).  Grid (ceil(nt/32), ceil(ntr/32)), 32 x 8.
+__global__ void traces_tm_kernel(const float *__restrict__ src, float *__restrict__ dst, int nt, int ntr) {
+    __shared__ float tile[32][33];
+    const int t0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = r0 + k, it = t0 + threadIdx.x;
+        tile[k][threadIdx.x] = (r < ntr && it < nt) ? src[(size_t)r * nt + it] : 0.f;
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int it = t0 + k, r = r0 + threadIdx.x;
+        if (it < nt && r < ntr) dst[(size_t)it * ntr + r] = tile[threadIdx.x][k];
+    }
+}
 __global__ void advance_base_kernel(int *nbase, int k) { *nbase = *nbase + k; }
 
 

@@ -293,8 +293,8 @@ static void tile_range(const std::vector<int> &tab, int &lo, int &hi) {
 // and, for every injecting tile, a dense map over its cells of
 // (start - first) << 8 | count.
 static void inj_dense(int ntile, int ncell, const std::vector<int> &tile, const std::vector<int> &cell,
-                      const std::vector<int> &cptr, const std::vector<int> &tptr, std::vector<int> &desc,
-                      std::vector<int> &map) {
+                      const std::vector<int> &cptr, const std::vector<int> &tptr, const std::vector<int> &ttr,
+                      std::vector<int> &desc, std::vector<int> &map) {
     desc.assign((size_t)ntile * 8, 0);
     map.clear();
     for (int t = 0; t < ntile && t + 1 < (int)tile.size(); t++) {
@@ -306,6 +306,9 @@ static void inj_dense(int ntile, int ncell, const std::vector<int> &tile, const 
         dt[2] = tptr[t];
         dt[3] = tptr[t + 1];
         dt[4] = (int)map.size();
+        dt[5] = ttr[dt[2]];   // consecutive global traces: the first; else -1
+        for (int j = dt[2] + 1; j < dt[3]; j++)
+            if (ttr[j] != ttr[j - 1] + 1) dt[5] = -1;
         map.resize(map.size() + ncell, 0);
         int *m = &map[(size_t)dt[4]];
         for (int c = c0; c < c1; c++) m[cell[c]] = ((cptr[c] - dt[0]) << 8) | (cptr[c + 1] - cptr[c]);
@@ -475,7 +478,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     std::vector<size_t> upn;
     struct Off {
         size_t u[kLevels], v2, vel, hz, hx, colsrc, taper;
-        size_t inj_tile, inj_cell, inj_cptr, inj_coef, inj_tptr, inj_ttr, inj_traces;
+        size_t inj_tile, inj_cell, inj_cptr, inj_coef, inj_tptr, inj_ttr, inj_traces, inj_tm;
         size_t injb_tile, injb_cell, injb_cptr, injb_coef, injb_tptr, injb_ttr;
         size_t inj_desc, inj_map, injb_desc, injb_map;
         size_t inj2[6], injb2[6];
@@ -508,10 +511,12 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         o.inj_coef = put(L, up, upn, p.inj_coef);
         o.inj_tptr = put(L, up, upn, p.inj_tptr);
         o.inj_ttr = put(L, up, upn, p.inj_ttr);
-        inj_dense(p.ntz * p.ntx, tz * tx, p.inj_tile, p.inj_cell, p.inj_cptr, p.inj_tptr, p.inj_desc, p.inj_map);
+        inj_dense(p.ntz * p.ntx, tz * tx, p.inj_tile, p.inj_cell, p.inj_cptr, p.inj_tptr, p.inj_ttr, p.inj_desc,
+                  p.inj_map);
         o.inj_desc = put(L, up, upn, p.inj_desc);
         o.inj_map = put(L, up, upn, p.inj_map);
         o.inj_traces = L.take((size_t)p.nt * std::max(p.n_inj_traces, 1) * sizeof(float));
+        o.inj_tm = L.take((size_t)p.nt * std::max(p.n_inj_traces, 1) * sizeof(float));
         if (need.backward_injection) {
             o.injb_tile = put(L, up, upn, p.injb_tile);
             o.injb_cell = put(L, up, upn, p.injb_cell);
@@ -519,8 +524,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
             o.injb_coef = put(L, up, upn, p.injb_coef);
             o.injb_tptr = put(L, up, upn, p.injb_tptr);
             o.injb_ttr = put(L, up, upn, p.injb_ttr);
-            inj_dense(p.ntz * p.ntx, tz * tx, p.injb_tile, p.injb_cell, p.injb_cptr, p.injb_tptr, p.injb_desc,
-                      p.injb_map);
+            inj_dense(p.ntz * p.ntx, tz * tx, p.injb_tile, p.injb_cell, p.injb_cptr, p.injb_tptr, p.injb_ttr,
+                      p.injb_desc, p.injb_map);
             o.injb_desc = put(L, up, upn, p.injb_desc);
             o.injb_map = put(L, up, upn, p.injb_map);
         }
@@ -587,6 +592,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.inj_desc = (const int *)(base + o.inj_desc);
         S.inj_map = (const int *)(base + o.inj_map);
         S.inj_traces = (const float *)(base + o.inj_traces);
+        S.inj_tm = (float *)(base + o.inj_tm);
+        S.inj_ntr = std::max(p.n_inj_traces, 1);
         if (need.backward_injection) {
             InjSet &I = B.injb[s];
             I.tile = (const int *)(base + o.injb_tile);
@@ -756,7 +763,19 @@ void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
 // block's first step from device memory (d_nbase).  Remaining steps launch one
 // by one with a fixed n.  HIP events on the engine stream bracket every block;
 // block time / K is the per-launch step time reported by kernel_stats.
+// The step kernel reads injected traces time-major (one row per step): copy
+// each shot's traces (as uploaded or as the residual kernel wrote them).
+void Engine::traces_time_major(const Batch &B) {
+    for (int s = 0; s < B.ns; s++) {
+        const ShotDev &S = B.shots_h[s];
+        const dim3 grid((S.nt + 31) / 32, (S.inj_ntr + 31) / 32);
+        hipLaunchKernelGGL(traces_tm_kernel, grid, dim3(32, 8), 0, stream, S.inj_traces, S.inj_tm, S.nt, S.inj_ntr);
+    }
+    HIPCHK(hipGetLastError());
+}
+
 void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfree) {
+    if (A.inject) traces_time_major(B);
     const int dir = A.dir > 0 ? 0 : 1;
     int period = kLevels;
     while (period % g.snapi) period += kLevels;
@@ -901,6 +920,7 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
         need.snap_fp16 = snaps == SFD_SNAP_FP16S;
         build_batch(fwd_batch, plans, need);
         for (int s = 0; s < ns; s++) upload_inj_traces(fwd_batch, s, shots[s].wavelet, shots[s].nt);
+        traces_time_major(fwd_batch);
         prepare_models(fwd_batch, plans, d_vel);
         StepArgs A = base_args(fwd_batch);
         A.dir = 1;
