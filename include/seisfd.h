@@ -306,9 +306,12 @@ typedef struct {
     int   max_evals;       /* evaluations allowed per preconditioned search (4);
                               0 abandons every preconditioned search at once */
     int   max_fallbacks;   /* restarts before the network is switched off (3)  */
-    int   target;          /* training target for gradient g_k: 0 the accepted
-                              step s_k of that iteration; 1 the accepted model
-                              update from x_k to the newest model, sum_{j>=k} s_j */
+    int   target;          /* training pairs: 0 gradient g_k -> accepted step
+                              s_k of that iteration; 1 g_k -> the accepted model
+                              update from x_k to the newest model, sum_{j>=k} s_j;
+                              2 the secant pairs y_k = g_{k+1} - g_k -> s_k (the
+                              quasi-Newton condition H y = s), in the coordinates
+                              of the illumination wrap when illum_h0 is set */
     int   use_net;         /* 1: the network is trained and applied (default)   */
     int   illum_h0;        /* 1: wrap the operator in the illumination diagonal
                               W = illum^(-1/2) (compute_fct_gdt's illumination at

@@ -191,7 +191,7 @@ struct CnnPrecond : LbfgsHooks {
     sfd_precond_opts o;
     Cnn net;
     LbfgsDev *d = nullptr;
-    std::vector<double *> gk, sk;
+    std::vector<double *> gk, sk, yk;
     double *tmp = nullptr, *tmp2 = nullptr, *tmp3 = nullptr, *gmax = nullptr;
     float *illum = nullptr, *w = nullptr;
     bool trained = false, have_w = false;
@@ -210,20 +210,24 @@ struct CnnPrecond : LbfgsHooks {
     ~CnnPrecond() override {
         for (double *p : gk) (void)hipFree(p);
         for (double *p : sk) (void)hipFree(p);
+        for (double *p : yk) (void)hipFree(p);
         for (void *p : {(void *)tmp, (void *)tmp2, (void *)tmp3, (void *)gmax, (void *)illum, (void *)w})
             if (p) (void)hipFree(p);
     }
-    static void pair_thunk(void *ctx, const double *g, const double *s, long n, hipStream_t st) {
+    static void pair_thunk(void *ctx, const double *g, const double *s, const double *y, long n, hipStream_t st) {
         CnnPrecond *c = (CnnPrecond *)ctx;
         if (c->trained || (int)c->gk.size() >= c->o.train_after) return;
-        double *gc = nullptr, *sc = nullptr;
+        double *gc = nullptr, *sc = nullptr, *yc = nullptr;
         HIPCHK(hipMalloc(&gc, n * sizeof(double)));
         HIPCHK(hipMalloc(&sc, n * sizeof(double)));
+        HIPCHK(hipMalloc(&yc, n * sizeof(double)));
         HIPCHK(hipMemcpyAsync(gc, g, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(sc, s, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(yc, y, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         c->gk.push_back(gc);
         c->sk.push_back(sc);
+        c->yk.push_back(yc);
     }
     static void h0_thunk(void *ctx, double *q, long n, hipStream_t st) { ((CnnPrecond *)ctx)->apply(q); }
 
@@ -276,12 +280,25 @@ struct CnnPrecond : LbfgsHooks {
                 hipLaunchKernelGGL(accumulate_kernel, stream_grid(P), dim3(256), 0, net.st, P,
                                    (const double *)sk[k + 1], sk[k]);
         }
+        // target 2 (secant pairs): N maps y_k to s_k, in the coordinates of the
+        // operator W N(W .): inputs W y_k, targets s_k / W
+        const bool secant = o.target == 2;
+        const float *wp = (o.illum_h0 && have_w) ? w : nullptr;
         for (int k = 0; k < K; k++) {
-            net.maxabs(gk[k], P, gmax);
-            net.maxabs(sk[k], P, gmax + 1);
-            hipLaunchKernelGGL(pair_kernel, stream_grid(P), dim3(256), 0, net.st, P, (const double *)gk[k],
-                               (const double *)gmax, (const double *)sk[k], (const double *)(gmax + 1),
-                               net.x32 + k * P, net.x32 + (K + k) * P, net.t32 + k * P, net.t32 + (K + k) * P);
+            const double *in = gk[k], *tg = sk[k];
+            if (secant) {
+                hipLaunchKernelGGL(scale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, (const double *)yk[k],
+                                   1.0, tmp);
+                hipLaunchKernelGGL(unscale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, (const double *)sk[k],
+                                   tmp2);
+                in = tmp;
+                tg = tmp2;
+            }
+            net.maxabs(in, P, gmax);
+            net.maxabs(tg, P, gmax + 1);
+            hipLaunchKernelGGL(pair_kernel, stream_grid(P), dim3(256), 0, net.st, P, in, (const double *)gmax, tg,
+                               (const double *)(gmax + 1), net.x32 + k * P, net.x32 + (K + k) * P, net.t32 + k * P,
+                               net.t32 + (K + k) * P, secant ? 1.0 : -1.0);
         }
         net.train(2 * K, o.train_steps, o.lr, &loss_first, &loss_last);
         HIPCHK(hipStreamSynchronize(net.st));
@@ -299,6 +316,10 @@ struct CnnPrecond : LbfgsHooks {
         if (!dd.used_h0) return iflag;
         const bool failed = iflag < 0 || (iflag == 1 && dd.info == -1 && dd.nfev >= o.max_evals);
         if (!failed) return iflag;
+        if (++fallbacks >= o.max_fallbacks) dd.h0_active = false;
+        return dd.restart_plain();
+    }
+    int rejected_trial(LbfgsDev &dd) override {
         if (++fallbacks >= o.max_fallbacks) dd.h0_active = false;
         return dd.restart_plain();
     }

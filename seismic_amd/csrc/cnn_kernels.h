@@ -269,10 +269,10 @@ __global__ void maxabs_final_kernel(int np, const double *__restrict__ part, dou
 // the negated copy as image K + k), FP32.
 __global__ void pair_kernel(long P, const double *__restrict__ gk, const double *__restrict__ gmax,
                             const double *__restrict__ sk, const double *__restrict__ smax, float *__restrict__ x,
-                            float *__restrict__ xneg, float *__restrict__ tgt, float *__restrict__ tneg) {
-    const double a = *gmax > 0.0 ? 1.0 / *gmax : 0.0, b = *smax > 0.0 ? 1.0 / *smax : 0.0;
+                            float *__restrict__ xneg, float *__restrict__ tgt, float *__restrict__ tneg, double sgn) {
+    const double a = *gmax > 0.0 ? 1.0 / *gmax : 0.0, b = *smax > 0.0 ? sgn / *smax : 0.0;
     for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
-        const float xv = (float)(gk[p] * a), tv = (float)(-sk[p] * b);
+        const float xv = (float)(gk[p] * a), tv = (float)(sk[p] * b);
         x[p] = xv;
         xneg[p] = -xv;
         tgt[p] = tv;
@@ -293,6 +293,13 @@ __global__ void scale_w_kernel(long n, const float *__restrict__ w, const double
                                double *__restrict__ y) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         y[i] = s * (w ? (double)w[i] : 1.0) * x[i];
+}
+
+// y = x / w  (w may be null: 1)
+__global__ void unscale_w_kernel(long n, const float *__restrict__ w, const double *__restrict__ x,
+                                 double *__restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = w ? (w[i] > 0.f ? x[i] / (double)w[i] : 0.0) : x[i];
 }
 
 // y += x

@@ -361,7 +361,7 @@ L172:
     scale_copy(stp, Sv(npt), Sv(npt));
     hipLaunchKernelGGL(sub_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)g, (const double *)q, Yv(npt));
     // accepted pair: gradient at the start of the line search (q) and the step taken
-    if (on_pair) on_pair(pair_ctx, q, Sv(npt), n, st);
+    if (on_pair) on_pair(pair_ctx, q, Sv(npt), Yv(npt), n, st);
     point = point + 1;
     if (point == m) point = 0;
     gnorm = sqrt(dot(g, g));
@@ -533,7 +533,31 @@ int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, co
     HIPCHK(hipMemcpyAsync(xdsave, xsave, n * sizeof(double), hipMemcpyDeviceToDevice, d.st));
     HIPCHK(hipStreamSynchronize(d.st));
     while (cont && !found) {
-        const int rc = fct(x, g.data(), &fd);
+        int rc;
+        try {
+            rc = fct(x, g.data(), &fd);
+        } catch (const EngineError &err) {
+            // A trial model along a preconditioner's direction that the
+            // evaluation rejects (non-finite or non-positive velocity, Courant
+            // limit) is a failed preconditioned search: the hook restarts it
+            // plain from the same model.  Any other failure propagates.
+            const bool model_range = err.code == SFD_EINVAL || err.code == SFD_EUNSTABLE;
+            if (!(hooks && d.used_h0 && model_range)) throw;
+            iflag = hooks->rejected_trial(d);
+            if (iflag != 1) throw;
+            hipLaunchKernelGGL(to_float_kernel, vg, vb, 0, d.st, n, (const double *)d.x, xf);
+            HIPCHK(hipMemcpyAsync(x, xf, n * sizeof(float), hipMemcpyDeviceToHost, d.st));
+            HIPCHK(hipStreamSynchronize(d.st));
+            if (rep && k < rep->hist_cap) {
+                if (rep->f_eval) rep->f_eval[k] = std::nan("");
+                if (rep->t_eval) rep->t_eval[k] = secs();
+                if (rep->accepted) rep->accepted[k] = 0;
+            }
+            eval = eval + 1;
+            k = k + 1;
+            if (eval >= o.neval) cont = 0;
+            continue;
+        }
         if (rc != 0) throw EngineError(rc < 0 ? rc : SFD_ESTATE, "misfit/gradient evaluation failed");
         if (rep && k < rep->hist_cap) {
             if (rep->f_eval) rep->f_eval[k] = fd;

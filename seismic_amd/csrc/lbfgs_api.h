@@ -17,6 +17,9 @@ struct LbfgsHooks {
     virtual void accepted(LbfgsDev &d, int iteration) {}
     // after each LBFGS call; may replace the IFLAG (e.g. restart a rejected search)
     virtual int after_step(LbfgsDev &d, int iflag) { return iflag; }
+    // the evaluation rejected the trial model of a search along the hook's own
+    // direction; returns the IFLAG of the search that replaces it (< 0: none)
+    virtual int rejected_trial(LbfgsDev &d) { return -1; }
     virtual void report(sfd_lbfgs_report *rep) {}
     // compute_fct_gdt's illumination at the newest evaluated model (host, n values)
     virtual bool wants_illumination() const { return false; }

@@ -46,7 +46,7 @@ struct LbfgsDev {
     int stp1_opt = -1;                   // < 0: libinv's rule (lbfgs.f:333-338)
     double step_init = 999999.0;
     // optional observer of every accepted (gradient, step) pair
-    void (*on_pair)(void *ctx, const double *gk, const double *sk, long n, hipStream_t st) = nullptr;
+    void (*on_pair)(void *ctx, const double *gk, const double *sk, const double *yk, long n, hipStream_t st) = nullptr;
     void *pair_ctx = nullptr;
 
     LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mask_h, int boundt_, int type_,
