@@ -127,6 +127,16 @@ int  sfd_rtm(sfd_handle h, const sfd_shot *shots, int nshots,
              const float *traces_in, float *image, float *illum, int snap_mode);
 
 /*
+ * Keep traces resident in HBM (no reference counterpart): count floats, the
+ * shots' (nt, ntr) traces concatenated in shot order as sfd_rtm takes them.
+ * sfd_rtm / sfd_lsrtm called with traces_in = NULL then read this copy
+ * instead of transferring host data each call (repeated imaging, LSRTM
+ * iterations); the count must equal the call's sum of nt * ntr.  traces =
+ * NULL or count = 0 releases the copy.
+ */
+int  sfd_set_traces(sfd_handle h, const float *traces, size_t count);
+
+/*
  * Born modeling (to_disk_to_memory_BORN.f90:123-220): pass 1 forward with
  * snapshots (source second-differenced as readsou does with Born=.true.,
  * ReadSouVelRho.f90:318-326), pass 2 scattering from refl(nz,nx) via

@@ -104,6 +104,9 @@ struct Engine {
     bool tb_ok = true;                    // the current batch's region plans fit the two-step kernel
     Batch fwd_batch;
     DevBuf scratch, fwi_mem;
+    DevBuf resident;                      // traces kept in HBM by sfd_set_traces
+    size_t resident_n = 0;
+    void set_traces(const float *tr, size_t n);
 
     explicit Engine(const sfd_grid &grid);
     ~Engine();

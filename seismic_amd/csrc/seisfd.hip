@@ -1124,11 +1124,26 @@ void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, flo
 }
 
 // ---- RTM / LSRTM (to_disk_to_memory_RTM.f90:358-579) -------------------------
+// Keep a device copy of traces (per shot (nt, ntr), concatenated in shot
+// order) for RTM / LSRTM calls that pass no host traces.
+void Engine::set_traces(const float *tr, size_t n) {
+    resident_n = 0;
+    if (!tr || n == 0) return;
+    resident.reserve(n * sizeof(float));
+    HIPCHK(hipMemcpy(resident.p, tr, n * sizeof(float), hipMemcpyHostToDevice));
+    resident_n = n;
+}
+
 void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
                  bool lsrtm) {
     check_shots(shots, ns, snap_mode);
     reset_kstats();
-    if (!traces_in) throw EngineError(SFD_EINVAL, "traces_in is NULL");
+    if (!traces_in) {
+        size_t total = 0;
+        for (int s = 0; s < ns; s++) total += (size_t)shots[s].nt * shots[s].ntr;
+        if (resident_n == 0) throw EngineError(SFD_EINVAL, "traces_in is NULL and no traces are resident");
+        if (total != resident_n) throw EngineError(SFD_EINVAL, "resident traces do not match the shots' nt * ntr");
+    }
     if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
     auto t0 = std::chrono::steady_clock::now();
     std::vector<ShotPlan> plans(ns);
@@ -1173,7 +1188,11 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
         ShotDev &S = B.shots_h[s];
         B.injb[s].apply(S);
         const size_t n = (size_t)shots[s].nt * shots[s].ntr;
-        upload_inj_traces(B, s, traces_in + off, n);
+        if (traces_in)
+            upload_inj_traces(B, s, traces_in + off, n);
+        else
+            HIPCHK(hipMemcpyAsync((void *)S.inj_traces, (const float *)resident.p + off, n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, stream));
         off += n;
         std::vector<float> z((size_t)g.nz * plans[s].nxw, 0.f);
     }
@@ -1547,6 +1566,10 @@ int sfd_forward(sfd_handle h, const sfd_shot *shots, int nshots, float *traces_o
 int sfd_rtm(sfd_handle h, const sfd_shot *shots, int nshots, const float *traces_in, float *image, float *illum,
             int snap_mode) {
     return guarded(h, [&](Engine &e) { e.rtm(shots, nshots, traces_in, image, illum, snap_mode, false); });
+}
+
+int sfd_set_traces(sfd_handle h, const float *traces, size_t count) {
+    return guarded(h, [&](Engine &e) { e.set_traces(traces, count); });
 }
 
 int sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots, const float *traces_in, float *image, float *illum) {

@@ -141,16 +141,25 @@ class Engine:
 
     # RTM_to_memory
     def rtm(self, shots, traces, snap_mode=abi.SFD_SNAP_FP32, image=None, illum=None):
-        d = self._cat(traces, shots)
+        """traces None: the copy kept in HBM by set_traces."""
+        d = None if traces is None else self._cat(traces, shots)
         nz, nx = self.g.nz, self.g.nx
         img = np.zeros(nz * nx, np.float32) if image is None else np.ascontiguousarray(np.asarray(image, np.float32).T).ravel()
         ill = np.zeros(nz * nx, np.float32) if illum is None else np.ascontiguousarray(np.asarray(illum, np.float32).T).ravel()
         self._chk(self.lib.sfd_rtm(self.h, self._shots(shots), len(shots), _ptr(d), _ptr(img), _ptr(ill), snap_mode))
         return img.reshape(nx, nz).T, ill.reshape(nx, nz).T
 
+    def set_traces(self, traces, shots):
+        """Keep the shots' traces in HBM for rtm/lsrtm(shots, None); traces None releases them."""
+        if traces is None:
+            self._chk(self.lib.sfd_set_traces(self.h, None, 0))
+            return
+        self._resident = self._cat(traces, shots)
+        self._chk(self.lib.sfd_set_traces(self.h, _ptr(self._resident), self._resident.size))
+
     # LSRTM_to_memory
     def lsrtm(self, shots, traces):
-        d = self._cat(traces, shots)
+        d = None if traces is None else self._cat(traces, shots)
         nz, nx = self.g.nz, self.g.nx
         img = np.zeros(nz * nx, np.float32)
         ill = np.zeros(nz * nx, np.float32)

@@ -11,7 +11,7 @@ module seisfd_mod
   implicit none
 
   integer(c_int), parameter :: SFD_OK = 0, SFD_EINVAL = -1, SFD_ENOMEM = -2, SFD_EHIP = -3, &
-       SFD_EUNSUPPORTED = -4, SFD_ESTATE = -5, SFD_EUNSTABLE = -6, SFD_ENOTFINITE = -7
+       SFD_EUNSUPPORTED = -4, SFD_ESTATE = -5, SFD_EUNSTABLE = -6, SFD_ENOTFINITE = -7, SFD_EIO = -8
   integer(c_int), parameter :: SFD_SNAP_FP32 = 0, SFD_SNAP_FP16S = 1
   integer(c_int), parameter :: SFD_EXTRACT_SINC_V2 = 0, SFD_EXTRACT_NEAREST = 1, &
        SFD_EXTRACT_SINC_LSRTM = 2, SFD_EXTRACT_SINC_UNIT = 3
@@ -103,6 +103,15 @@ module seisfd_mod
        real(c_float), intent(in) :: traces(*)
        real(c_float), intent(inout) :: image(*), illum(*)
      end function sfd_rtm
+
+     ! keep traces in HBM for sfd_rtm / sfd_lsrtm called with a NULL traces_in
+     ! (bind sfd_rtm with a type(c_ptr), value argument to pass c_null_ptr)
+     integer(c_int) function sfd_set_traces(h, traces, count) bind(C, name='sfd_set_traces')
+       import :: c_int, c_ptr, c_float, c_size_t
+       type(c_ptr), value :: h
+       real(c_float), intent(in) :: traces(*)
+       integer(c_size_t), value :: count
+     end function sfd_set_traces
 
      integer(c_int) function sfd_born(h, shots, nshots, refl, traces) bind(C, name='sfd_born')
        import :: c_int, c_ptr, c_float, sfd_shot

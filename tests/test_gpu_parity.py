@@ -165,6 +165,29 @@ def test_rtm_image_matches_oracle(lsrtm):
     assert ei <= TOL and el <= TOL
 
 
+def test_rtm_resident_traces():
+    """sfd_set_traces: RTM and LSRTM reading the observed data from HBM give
+    the images of the host-traces calls bitwise; a resident copy whose length
+    does not match the shots is rejected."""
+    say("case: rtm resident traces")
+    nz, nx, nt = 48, 72, 320
+    vel, w, rx, rz, data = _rtm_case(nz=nz, nx=nx, nt=nt)
+    gs, _ = both_shots(18.0, 355.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    img_h, ill_h = e.rtm([gs], [data])
+    ls_h, _ = e.lsrtm([gs], [data])
+    e.set_traces([data], [gs])
+    img_r, ill_r = e.rtm([gs], None)
+    ls_r, _ = e.lsrtm([gs], None)
+    assert np.array_equal(img_h, img_r) and np.array_equal(ill_h, ill_r) and np.array_equal(ls_h, ls_r)
+    short = sa.Shot(gs.sz, gs.sx, w[: nt // 2], rz, rx)
+    with pytest.raises(sa.SeisFDError):
+        e.rtm([short], None)
+    e.set_traces(None, None)
+    with pytest.raises(sa.SeisFDError):
+        e.rtm([gs], None)
+
+
 def test_rtm_dense_receivers():
     """Receivers every half cell: a tile holds more injection contributions
     than its LDS stage (2048), so the reverse-time injection reads them from
