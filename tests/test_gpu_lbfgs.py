@@ -94,3 +94,24 @@ def test_device_l_bfgs_driver_matches_oracle(bound):
     assert np.array_equal(rd["accepted"], ro["accepted"])
     assert np.allclose(rd["f_eval"], ro["f_eval"], rtol=1e-9, atol=1e-12 * ro["f_eval"][0])
     assert np.allclose(xd, xo, rtol=1e-6, atol=0)
+
+
+def test_device_l_bfgs_nan_misfit_matches_oracle():
+    """Fault injection (Inversion.f90:158-169): a misfit that turns NaN ends
+    the device driver with IFLAG = -1 after the same evaluations as the
+    oracle's driver, returning the same last accepted model."""
+    print("case: l_bfgs NaN misfit", flush=True)
+    fct = _quad()
+
+    def faulty(calls):
+        def f(x):
+            calls.append(1)
+            v, g = fct(x)
+            return (float("nan") if len(calls) > 4 else v), g
+        return f
+    x0 = np.full(48, 2000.0, np.float32)
+    xo, ro = O.l_bfgs(faulty([]), x0, niter=20, neval=200, bound=0)
+    xd, rd = sa.lbfgs_run(faulty([]), x0, niter=20, neval=200, bound=0)
+    assert ro["iflag"] == -1 and rd["iflag"] == -1
+    assert rd["evaluations"] == ro["evaluations"] and rd["iterations"] == ro["iterations"]
+    assert np.all(np.isfinite(xd)) and np.allclose(xd, xo, rtol=1e-6)

@@ -1,7 +1,7 @@
 """Pins of the oracle's l_bfgs driver (Inversion.f90:29-189) by what its
 control flow fixes in closed form: the first trial model, the stop rules and
 the model returned at each stop.  LBFGS itself is pinned against the
-reference's compiled lbfgs.f elsewhere (test_oracle_vs_ref)."""
+reference's compiled lbfgs.f in test_oracle_lbfgsoc.py."""
 import numpy as np
 
 import oracle_lib as O
@@ -75,3 +75,21 @@ def test_bounds_clip_accepted_models():
     x, rep = O.l_bfgs(fct, x0, niter=8, neval=60, bound=2, xmin=1900.0, xmax=2600.0)
     assert rep["iterations"] >= 1
     assert x.min() >= 1900.0 - 1e-3 and x.max() <= 2600.0 + 1e-3
+
+
+def test_nan_misfit_ends_the_run_with_iflag_minus_one():
+    """Fault injection (Inversion.f90:158-169): a misfit that turns NaN makes
+    the line search fail (INFO != 1), LBFGS returns IFLAG = -1 and l_bfgs
+    stops, returning the last accepted model (finite)."""
+    fct, _ = quad_problem()
+    calls = []
+
+    def bad(x):
+        calls.append(1)
+        f, g = fct(x)
+        return (float("nan") if len(calls) > 4 else f), g
+    x0 = np.full(40, 2000.0, np.float32)
+    x, rep = O.l_bfgs(bad, x0, niter=20, neval=200, bound=0)
+    assert rep["iflag"] == -1
+    assert rep["iterations"] >= 1 and rep["evaluations"] < 200
+    assert np.all(np.isfinite(x))

@@ -152,3 +152,23 @@ def test_lbfgs_quadratic_finite_termination():
             break
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
     assert all(b <= a + 1e-12 for a, b in zip(fs, fs[1:]))
+
+
+def test_higdon_absorbs_outgoing_energy():
+    """Energy decay with the absorbing boundary: in a homogeneous model the
+    wavefield energy over the interior, after the wavelet has left the
+    60 x 80 grid, falls below 1e-5 of its peak (a reflecting edge would keep
+    it near 1)."""
+    nz, nx, nt = 60, 80, 900
+    g = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=20, snapi=10)
+    vel = np.full((nz, nx), 2000.0, np.float32)
+    w = ricker(15.0, nt, 1e-3)
+    rx = np.linspace(0.0, 790.0, 80).astype(np.float32)
+    rz = np.full_like(rx, 100.0)
+    _, wave = O.modeling(g, vel, O.Shot(300.0, 400.0, w, rz, rx), snaps=True)
+    assert wave.shape[1:] == (nx, nz)
+    E = (wave.astype(np.float64) ** 2).sum(axis=(1, 2))
+    k = int(E.argmax())
+    assert E[-1] < 1e-5 * E[k]
+    # 400 ms after the peak the wavelet has crossed the 800 m model at 2000 m/s
+    assert np.all(E[k + 40:] < 1e-4 * E[k])
