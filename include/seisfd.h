@@ -158,6 +158,17 @@ int  sfd_born(sfd_handle h, const sfd_shot *shots, int nshots,
 int  sfd_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots,
                        const float *dbar, float *sbar, int extract_kind);
 
+/*
+ * Exact discrete adjoint of sfd_born (no reference counterpart: the
+ * reference's LSRTM imaging, sfd_lsrtm, is the reverse-time approximation of
+ * it).  With B the linear map from refl(nz,nx) to the traces of sfd_born
+ * (background pass fixed), mbar = B^T dbar summed over the shots: dbar (nt,
+ * ntr) per shot concatenated as sfd_born writes traces, mbar (nz, nx)
+ * overwritten.  <B m, d> = <m, B^T d> holds to FP32 rounding.
+ */
+int  sfd_born_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots,
+                            const float *dbar, float *mbar);
+
 /* LSRTM imaging (to_disk_to_memory_RTM.f90:494-579, Imaging.f90:698-745). */
 int  sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots,
                const float *traces_in, float *image, float *illum);

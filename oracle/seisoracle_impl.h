@@ -609,12 +609,16 @@ static void CAT(adj_higdon_, SFX)(PROP *P, REAL *L1, REAL *L2, REAL *G, int i, i
     AT(P, L1, i + 2 * di, j + 2 * dj) = AT(P, L1, i + 2 * di, j + 2 * dj) - g[7] * a;
 }
 
-int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *dbar, REAL *sbar,
-                         int extract_kind)
+/* The reverse sweep shared by FN(modeling_adjoint) and FN(born_adjoint).
+ * P: a frame from setup_shot_ (its padded model is freed here).  sbar
+ * (optional) receives the transpose of the source injection; with bsnaps,
+ * btap and bmbar given, every step k with k % snapi == 0 also accumulates
+ * the transpose of Injection_Born (Imaging.f90:779-783, the term
+ * 2 taper m S_k / v added to the stencil value) into bmbar (nz, nxw). */
+static int CAT(adjoint_core_, SFX)(PROP P, const sfd_shot *s, const REAL *dbar, REAL *sbar, int extract_kind,
+                                   const REAL *bsnaps, const REAL *btap, REAL *bmbar)
 {
-    PROP P;
     float zz = s->sz, xx = s->sx;
-    if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
     const ORC_BOUNDS *b = &P.b;
     const size_t n = (size_t)b->NZ * b->NX;
     const int l = P.lsinc, minx = (int)(-l * 0.5), maxx = (int)(l * 0.5);
@@ -641,7 +645,8 @@ int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, 
     REAL vs = AT(&P, P.vel, so->icoord[0], so->icoord[1]), v2s = vs * vs;
     FN(mksinc)(sz, l, so->dcoord[0] / P.dz);
     FN(mksinc)(sx, l, so->dcoord[1] / P.dx);
-    for (it = 0; it < s->nt; it++) sbar[it] = 0;
+    if (sbar)
+        for (it = 0; it < s->nt; it++) sbar[it] = 0;
     for (k = s->nt; k >= 1; k--) {
         G = buf[0];
         /* extraction at step k+1 read U_{k+1}: G += E^T dbar(k+1) */
@@ -705,8 +710,19 @@ int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, 
                     AT(&P, buf[1], i, j - t) = AT(&P, buf[1], i, j - t) + P.c[5 + t] * w;
                 }
             }
+        /* Injection_Born's transpose: mbar += 12 dt^2/11 G 2 taper S_k / v */
+        if (bmbar && k % P.snapi == 0 && k / P.snapi <= P.ntsnap) {
+            const size_t nw = (size_t)P.nz * P.nxw;
+            const REAL *S = bsnaps + (size_t)(k / P.snapi - 1) * nw;
+            for (j = 1; j <= P.nxw; j++)
+                for (i = 1; i <= P.nz; i++) {
+                    const size_t q = (i - 1) + (size_t)(j - 1) * P.nz;
+                    bmbar[q] = bmbar[q] + (REAL)12. * dt2 * div11 * AT(&P, G, i, j) *
+                                              ((REAL)2 * btap[j - 1] * S[q] / AT(&P, P.vel, i, j));
+                }
+        }
         /* the injection's transpose: sbar(k) = 12 dt^2/11 sum v2r wz wx G */
-        {
+        if (sbar) {
             REAL acc = 0;
             for (j = minx; j <= maxx; j++)
                 for (i = minx; i <= maxx; i++)
@@ -725,6 +741,57 @@ int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, 
     CAT(free_traces_, SFX)(P.dat, s->ntr);
     free(P.vel);
     return 0;
+}
+
+int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *dbar, REAL *sbar,
+                         int extract_kind)
+{
+    PROP P;
+    if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
+    return CAT(adjoint_core_, SFX)(P, s, dbar, sbar, extract_kind, NULL, NULL, NULL);
+}
+
+/* Exact discrete adjoint of FN(born) (no reference counterpart: the
+ * reference's LSRTM imaging is the reverse-time approximation of it).  The
+ * map refl -> traces is linear with the background pass fixed: pass 1 is
+ * rerun to get the snapshots, the reverse sweep with the LSRTM extraction's
+ * transpose accumulates the window's mbar, and the transpose of the window's
+ * nearest-column gather (FN(window_pad), edge columns clamped) adds it into
+ * mbar (nz, nx).  mbar is accumulated with +=. */
+int FN(born_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *wavelet,
+                     const REAL *dbar, REAL *mbar)
+{
+    PROP P;
+    float zz = s->sz, xx = s->sx;
+    if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
+    const size_t n = (size_t)P.nz * P.nxw;
+    REAL *snaps = (REAL *)calloc(n * (P.ntsnap > 0 ? P.ntsnap : 1), sizeof(REAL));
+    REAL *mw = (REAL *)calloc(n, sizeof(REAL));
+    REAL *tap = (REAL *)malloc(sizeof(REAL) * P.nxw);
+    if (!snaps || !mw || !tap) return -2;
+    P.stencil = 0; P.inject = 1; P.nsou = 1;
+    P.sou = CAT(make_traces_, SFX)(1, &zz, &xx, s->nt);
+    for (int it = 0; it < s->nt; it++) P.sou[0].trace[it] = wavelet[it];
+    CAT(second_diff_, SFX)(P.sou[0].trace, s->nt, P.dt);
+    P.wave = snaps;
+    P.tmin = 1; P.tmax = s->nt; P.tstep = 1;
+    CAT(propagate_, SFX)(&P);
+    CAT(free_traces_, SFX)(P.sou, 1);
+    P.sou = NULL; P.nsou = 0; P.inject = 0; P.wave = NULL;
+    FN(compute_taper)(tap, P.nxw);
+    const REAL oxw = P.oxw;
+    const int nxw = P.nxw;
+    int rc = CAT(adjoint_core_, SFX)(P, s, dbar, NULL, SFD_EXTRACT_SINC_LSRTM, snaps, tap, mw);
+    for (int j = 1; rc == 0 && j <= nxw; j++) {
+        REAL cxx = (REAL)(j - 1) * (REAL)g->dx + oxw;
+        int ix = NINT((cxx - (REAL)g->ox) / (REAL)g->dx) + 1;
+        ix = ix < 1 ? 1 : (ix > g->nx ? g->nx : ix);
+        for (int i = 1; i <= g->nz; i++)
+            mbar[(i - 1) + (size_t)(ix - 1) * g->nz] =
+                mbar[(i - 1) + (size_t)(ix - 1) * g->nz] + mw[(i - 1) + (size_t)(j - 1) * g->nz];
+    }
+    free(snaps); free(mw); free(tap);
+    return rc;
 }
 
 /* progs/rtm/to_disk_to_memory_RTM.f90:358-492 RTM_to_memory (lsrtm=0) and

@@ -111,7 +111,8 @@ SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.
 # Every symbol declared in include/seisfd.h, in declaration order.
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
-    "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact", "sfd_lsrtm",
+    "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
+    "sfd_born_adjoint_exact", "sfd_lsrtm",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_kernel_stats",
     "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgsoc_step", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
@@ -144,6 +145,7 @@ def load():
     lib.sfd_set_traces.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     lib.sfd_born.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p]
     lib.sfd_adjoint_exact.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    lib.sfd_born_adjoint_exact.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p]
     lib.sfd_lsrtm.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,
                               C.c_void_p]
     lib.sfd_fwi_gradient.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,

@@ -158,4 +158,21 @@ __global__ void k7_source_kernel(AdjStep A) {
     A.sbar[A.k - 1] = A.c12dt2 * A.div11 * acc;
 }
 
+// Transpose of Injection_Born (Imaging.f90:779-783) at step k: the forward
+// adds 2 taper m S_k / v to the stencil value of every window cell, which the
+// time update scales by c12dt2 div11, so mbar(window) += that factor times G.
+// S.snap holds the FP32 snapshots of the background pass (index k / snapi).
+__global__ void __launch_bounds__(256) k7_born_kernel(AdjStep A, int snap_idx, float *mbar) {
+    const ShotDev &S = A.S;
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)S.nz * S.nxw) return;
+    const int wz = (int)(idx % S.nz), wx = (int)(idx / S.nz);
+    const int iz = wz + S.i0, ix = wx + S.i0;
+    if (iz < 4 || iz > S.NZ - 5 || ix < 4 || ix > S.NX - 5) return;
+    const size_t q = (size_t)ix * S.LDZ + S.zoff + iz;
+    const float Sv = ((const float *)S.snap)[(size_t)(snap_idx - 1) * S.nz * S.nxw + idx];
+    const float b = 2.f * S.taper[wx] * Sv / S.vel[q];
+    mbar[idx] = mbar[idx] + A.c12dt2 * A.div11 * A.G[q] * b;
+}
+
 }  // namespace sfd

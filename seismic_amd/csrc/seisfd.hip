@@ -1040,6 +1040,88 @@ void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image,
 // with the given extraction.  Receivers become sources of the transposed
 // extraction (injection plan kinds 1, 5, 3, 4 for SINC_V2, NEAREST,
 // SINC_LSRTM, SINC_UNIT); three launches per step (adjoint_kernels.h).
+// Exact transpose of born(): per shot the background pass is rerun with
+// FP32 snapshots, then the K7 reverse sweep runs with the receivers' LSRTM
+// sinc extraction transposed as its source, and k7_born_kernel collects the
+// window's mbar at every snapshot step; the host adds the window into the
+// model through the transpose of born()'s clamped column gather.
+void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *mbar) {
+    check_shots(shots, ns, SFD_SNAP_FP32);
+    if (!dbar || !mbar) throw EngineError(SFD_EINVAL, "NULL dbar/mbar");
+    if (!(tz == 64 && (tx == 32 || tx == 64))) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
+    reset_kstats();
+    std::fill(mbar, mbar + (size_t)g.nz * g.nx, 0.f);
+    size_t off = 0;
+    for (int s = 0; s < ns; s++) {
+        const sfd_shot &sh = shots[s];
+        std::vector<ShotPlan> plans(1);
+        ShotPlan &p = plans[0];
+        plan_shot(sh, p);
+        plan_injection(p, vel_h, 1, &sh.sz, &sh.sx, 1);
+        ShotPlan tmp = p;
+        plan_injection(tmp, vel_h, sh.ntr, sh.rz, sh.rx, 3);   // transpose of the LSRTM sinc extraction
+        p.take_backward(tmp);
+        p.n_inj_traces = std::max(1, sh.ntr);
+        BatchNeeds need;
+        need.backward_injection = true;
+        need.snapshots = true;
+        Batch &B = fwd_batch;
+        build_batch(B, plans, need);
+        std::vector<float> w(sh.wavelet, sh.wavelet + sh.nt);
+        second_difference(w, g.dt);
+        upload_inj_traces(B, 0, w.data(), w.size());
+        prepare_models(B, plans, d_vel);
+        StepArgs A1 = base_args(B);
+        A1.dir = 1;
+        A1.inject = 1;
+        A1.store_snap = 1;
+        run_loop(B, A1, false, sh.nt, sh.nt);
+        ShotDev S = B.shots_h[0];
+        B.injb[0].apply(S);
+        upload_inj_traces(B, 0, dbar + off, (size_t)sh.nt * sh.ntr);
+        const size_t nf = (size_t)p.LDZ * p.NX, nw = (size_t)g.nz * p.nxw;
+        Layout L;
+        const size_t o_lev = L.take(6 * nf * 4), o_mb = L.take(nw * 4);
+        scratch.reserve(L.off);
+        char *base = (char *)scratch.p;
+        HIPCHK(hipMemsetAsync(base + o_lev, 0, 6 * nf * 4, stream));
+        HIPCHK(hipMemsetAsync(base + o_mb, 0, nw * 4, stream));
+        const StepArgs SA = base_args(B);
+        AdjStep A;
+        memset(&A, 0, sizeof A);
+        A.S = S;
+        for (int q = 0; q < 10; q++) A.c[q] = SA.c[q];
+        A.c12dt2 = SA.c12dt2;
+        A.div11 = SA.div11;
+        float *lev[5];
+        for (int q = 0; q < 5; q++) lev[q] = (float *)(base + o_lev) + q * nf;
+        A.W = (float *)(base + o_lev) + 5 * nf;
+        float *d_mb = (float *)(base + o_mb);
+        const int ntile = p.ntz * p.ntx;
+        const unsigned ucells = (unsigned)(((size_t)p.NZ * p.NX + 255) / 256);
+        const unsigned wcells = (unsigned)((nw + 255) / 256);
+        for (int k = sh.nt; k >= 1; k--) {
+            A.G = lev[0]; A.L0 = lev[1]; A.L1 = lev[2]; A.L2 = lev[3]; A.L3 = lev[4];
+            A.k = k;
+            if (tx == 32)
+                hipLaunchKernelGGL((k7_gamma_kernel<64, 32>), dim3(ntile), dim3(256), 0, stream, A);
+            else
+                hipLaunchKernelGGL((k7_gamma_kernel<64, 64>), dim3(ntile), dim3(256), 0, stream, A);
+            hipLaunchKernelGGL(k7_update_kernel, dim3(ucells), dim3(256), 0, stream, A);
+            if (k % g.snapi == 0)
+                hipLaunchKernelGGL(k7_born_kernel, dim3(wcells), dim3(256), 0, stream, A, k / g.snapi, d_mb);
+            float *t = lev[0];
+            lev[0] = lev[1]; lev[1] = lev[2]; lev[2] = lev[3]; lev[3] = lev[4]; lev[4] = t;
+        }
+        HIPCHK(hipGetLastError());
+        std::vector<float> mw(nw);
+        HIPCHK(hipMemcpyAsync(mw.data(), d_mb, nw * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int j = 0; j < p.nxw; j++)
+            for (int i = 0; i < g.nz; i++) mbar[(size_t)p.colsrc[j] * g.nz + i] += mw[(size_t)j * g.nz + i];
+        off += (size_t)sh.nt * sh.ntr;
+    }
+}
 void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind) {
     check_shots(shots, ns);
     if (!dbar || !sbar) throw EngineError(SFD_EINVAL, "NULL dbar/sbar");
@@ -1612,6 +1694,9 @@ int sfd_fwi_lbfgs(sfd_handle h, const sfd_shot *shots, int nshots, const float *
     });
 }
 
+int sfd_born_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots, const float *dbar, float *mbar) {
+    return guarded(h, [&](Engine &e) { e.born_adjoint_exact(shots, nshots, dbar, mbar); });
+}
 int sfd_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots, const float *dbar, float *sbar,
                       int extract_kind) {
     return guarded(h, [&](Engine &e) { e.adjoint_exact(shots, nshots, dbar, sbar, extract_kind); });

@@ -174,6 +174,13 @@ class Engine:
         self._chk(self.lib.sfd_born(self.h, self._shots(shots), len(shots), _ptr(m), _ptr(out)))
         return self._split(out, shots)
 
+    def born_adjoint_exact(self, shots, dbar):
+        """B^T dbar summed over the shots (B: refl -> traces of born()); returns (nz, nx)."""
+        db = self._cat(dbar, shots)
+        mb = np.zeros(self.g.nz * self.g.nx, np.float32)
+        self._chk(self.lib.sfd_born_adjoint_exact(self.h, self._shots(shots), len(shots), _ptr(db), _ptr(mb)))
+        return mb.reshape(self.g.nx, self.g.nz).T.copy()
+
     # compute_fct_gdt
     def fwi_gradient(self, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
                      do_x=0, snap_mode=abi.SFD_SNAP_FP32):

@@ -132,6 +132,16 @@ module seisfd_mod
        real(c_float), intent(inout) :: sbar(*)
      end function sfd_adjoint_exact
 
+     integer(c_int) function sfd_born_adjoint_exact(h, shots, nshots, dbar, mbar) &
+          bind(C, name='sfd_born_adjoint_exact')
+       import :: c_int, c_ptr, c_float, sfd_shot
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: dbar(*)
+       real(c_float), intent(inout) :: mbar(*)
+     end function sfd_born_adjoint_exact
+
      integer(c_int) function sfd_lsrtm(h, shots, nshots, traces, image, illum) bind(C, name='sfd_lsrtm')
        import :: c_int, c_ptr, c_float, sfd_shot
        type(c_ptr), value :: h

@@ -194,6 +194,22 @@ def born(g, vel, refl, shot, prec="f"):
     return tr
 
 
+def born_adjoint(g, vel, shot, dbar, prec="d"):
+    """Exact adjoint of born(): mbar (nz, nx) = B^T dbar, dbar (ntr, nt)."""
+    L = oracle()
+    T = _dt(prec)
+    v = colmajor(vel, T)
+    w = np.ascontiguousarray(shot.wavelet, dtype=T)
+    db = np.ascontiguousarray(np.asarray(dbar, dtype=T).reshape(shot.ntr, shot.nt))
+    mb = np.zeros(g.nz * g.nx, dtype=T)
+    s = shot.c()
+    fn = getattr(L, "orc_%s_born_adjoint" % prec)
+    fn.argtypes = [C.c_void_p] * 6
+    rc = fn(C.byref(g), ptr(v), C.byref(s), ptr(w), ptr(db), ptr(mb))
+    assert rc == 0
+    return mb.reshape(g.nx, g.nz).T
+
+
 def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1,
                  do_z=0, do_x=0, prec="f"):
     """compute_fct_gdt restatement.  dobs: list of (ntr, nt) arrays per shot."""

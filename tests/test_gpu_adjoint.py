@@ -74,3 +74,44 @@ def test_adjoint_dot_test_config1():
     lhs, rhs, rcs, rl = dot_test(d, d, s, sb2)
     print("normal: <Fs,Fs>=%.9e <s,F^T F s>=%.9e rel %.2e" % (lhs, rhs, rl), flush=True)
     assert rl <= 1e-6
+
+
+@pytest.mark.parametrize("snapi,aperture", [(1, 0.0), (4, 900.0)])
+def test_born_adjoint_matches_oracle_and_dot_test(snapi, aperture):
+    """sfd_born_adjoint_exact against the oracle's born_adjoint (pinned by the
+    FP64 identity <B m, d> = <m, B^T d>) in the same FP32 arithmetic, and the
+    same identity in FP32 with B = sfd_born on two shots (the adjoint sums
+    them into one model)."""
+    print("case: born adjoint snapi=%d aperture=%g" % (snapi, aperture), flush=True)
+    nz, nx, nt = 52, 90, 420
+    vel = smooth_random(nz, nx, seed=3, vmin=1500, vmax=2600)
+    rng = np.random.default_rng(snapi)
+    e = sa.Engine(sa.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12, snapi=snapi, aperture_x=aperture))
+    e.set_model(vel)
+    w = ricker(18.0, nt, 1e-3)
+    shots = []
+    for k, sx in enumerate((233.3, 611.0)):
+        rx = np.linspace(sx - 200.0, sx + 200.0, 31).astype(np.float32)
+        shots.append(sa.Shot(24.4, sx, w, np.full_like(rx, 31.7), rx))
+    m = np.asfortranarray(rng.standard_normal((nz, nx)).astype(np.float32))
+    d = e.born(shots, m)
+    db = [rng.standard_normal(x.shape).astype(np.float32) for x in d]
+    mb = e.born_adjoint_exact(shots, db)
+    og = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12, snapi=snapi, aperture=aperture)
+    ref = sum(O.born_adjoint(og, vel, O.Shot(s.sz, s.sx, w, s.rz, s.rx), x, prec="f")
+              for s, x in zip(shots, db))
+    ref64 = sum(O.born_adjoint(og, vel, O.Shot(s.sz, s.sx, w, s.rz, s.rx), x, prec="d")
+                for s, x in zip(shots, db))
+    lhs = sum(float(np.dot(a.ravel().astype(np.float64), b.ravel())) for a, b in zip(d, db))
+    rhs = float(np.dot(m.ravel().astype(np.float64), mb.ravel()))
+    dn = np.sqrt(sum(float(np.sum(a.astype(np.float64) ** 2)) for a in d))
+    bn = np.sqrt(sum(float(np.sum(b.astype(np.float64) ** 2)) for b in db))
+    rcs, rl = abs(lhs - rhs) / (dn * bn), abs(lhs - rhs) / abs(lhs)
+    print("vs oracle FP32 rel %.2e, FP64 rel %.2e; <Bm,d>=%.9e <m,B^T d>=%.9e rel_cs %.2e rel %.2e" % (
+        rel(mb, ref), rel(mb, ref64), lhs, rhs, rcs, rl), flush=True)
+    # mbar sums products of two wavefields over time with a random dbar, so
+    # FP32 rounding shows at ~1e-5: the FP32 oracle itself is 1.5e-5 from FP64
+    # here.  The device must be no further from FP64 than twice that.
+    assert rel(mb, ref64) <= 2 * rel(ref, ref64)
+    assert rel(mb, ref) <= 3e-5
+    assert rcs <= 1e-6 and rl <= 1e-4

@@ -46,3 +46,29 @@ def test_modeling_adjoint_sees_the_boundary():
     lhs, rhs = float(np.sum(d * db)), float(np.sum(np.asarray(shot.wavelet, np.float64) * sb))
     print("<Fs,d>=%.15e <s,F^T d>=%.15e rel=%.2e" % (lhs, rhs, abs(lhs - rhs) / abs(lhs)))
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)   # 900 FP64 steps: rounding ~1e-12
+
+
+@pytest.mark.parametrize("snapi,aperture", [(1, 0.0), (4, 0.0), (2, 900.0)])
+def test_born_adjoint_dot_product_fp64(snapi, aperture):
+    """<B m, d> = <m, B^T d> for the oracle's Born modeling B (background
+    pass with the second-differenced wavelet, Injection_Born every snapi
+    steps from the stored snapshots, LSRTM sinc extraction).  A wide
+    aperture puts window columns outside the model, so the transpose of the
+    clamped nearest-column window gather is exercised too."""
+    nz, nx, nt = 28, 40, 240
+    vel = smooth_random(nz, nx, seed=7, vmin=1500, vmax=2600)
+    g = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12, snapi=snapi, aperture=aperture)
+    rng = np.random.default_rng(snapi)
+    rx = np.linspace(31.3, 10.0 * (nx - 4), 11).astype(np.float32)
+    rz = np.full_like(rx, 27.2)
+    w = np.zeros(nt, np.float32)
+    w[5:60] = np.sin(np.linspace(0, 3 * np.pi, 55)) * np.hanning(55)
+    shot = O.Shot(36.4, 181.7, w, rz, rx)
+    m = rng.standard_normal((nz, nx))
+    d = O.born(g, vel, np.asfortranarray(m), shot, prec="d")
+    db = rng.standard_normal(d.shape)
+    mb = O.born_adjoint(g, vel, shot, db, prec="d")
+    lhs, rhs = float(np.sum(d * db)), float(np.sum(m * mb))
+    print("snapi=%d aperture=%g <Bm,d>=%.15e <m,B^T d>=%.15e rel=%.2e" % (snapi, aperture, lhs, rhs,
+                                                                        abs(lhs - rhs) / abs(lhs)))
+    assert abs(lhs) > 0 and abs(lhs - rhs) <= 1e-12 * abs(lhs)
