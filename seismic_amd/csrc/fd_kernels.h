@@ -415,9 +415,14 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     float trv_r[kTrPerThread];
     int me[PX];
     if (has_inj) {
+        // tile-uniform: readfirstlane keeps them in scalar registers
         const GLB int *dd = gp(S.inj_desc) + 8 * t;
-        ic0 = dd[0]; ic1 = dd[1]; it0 = dd[2]; it1 = dd[3]; imap = dd[4];
-        const int tbase = dd[5];
+        ic0 = __builtin_amdgcn_readfirstlane(dd[0]);
+        ic1 = __builtin_amdgcn_readfirstlane(dd[1]);
+        it0 = __builtin_amdgcn_readfirstlane(dd[2]);
+        it1 = __builtin_amdgcn_readfirstlane(dd[3]);
+        imap = __builtin_amdgcn_readfirstlane(dd[4]);
+        const int tbase = __builtin_amdgcn_readfirstlane(dd[5]);
         has_inj = ic1 > ic0;
         if (has_inj) {
             const GLB float *tm = gp((const float *)S.inj_tm) + (size_t)(it - 1) * S.inj_ntr;
@@ -504,7 +509,6 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // in registers; the imaging mode, whose registers hold the imaging
     // operands, parks them in thread-private LDS slots (read back by the same
     // thread: no barrier).
-    bool coef_lds = false;
     if (has_inj) {
         if constexpr (IMG) {
 #pragma unroll
@@ -515,12 +519,23 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             const int j = tid + r * NT;
             if (it0 + j < it1) s_trv[j] = trv_r[r];
         }
-        coef_lds = ic1 - ic0 <= kMaxTileCoef;
-        if (coef_lds) {
-            const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef);
-            for (int j = ic0 + tid; j < ic1; j += NT) s_coef[j - ic0] = co[j];
-        }
+        // The tile's contributions reach LDS in chunks of kMaxTileCoef (one
+        // chunk except on the densest receiver tiles); the first chunk's
+        // loads are issued here, with the trace samples' stores.
+        const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
+        const int c1 = ic1 - ic0 < kMaxTileCoef ? ic1 - ic0 : kMaxTileCoef;
+        for (int j = tid; j < c1; j += NT) s_coef[j] = co[j];
         __syncthreads();
+        // Fold this step's sample into each staged contribution: slot 0
+        // becomes (gain * sample) * sinc_z, the first two products of the
+        // term in the reference's order, so a cell reads one LDS entry per
+        // contribution with no dependent gather and its term slot0 * sinc_x
+        // rounds exactly as the unfolded product.  Each thread folds what it
+        // staged; the injection pass reads the entries after a barrier.
+        for (int j = tid; j < c1; j += NT) {
+            const float4v w = s_coef[j];
+            s_coef[j][0] = w[0] * s_trv[__float_as_int(w[3])] * w[1];
+        }
     }
 
     // FP16 snapshots carry one scale per tile: max|u2| over the tile's window
@@ -588,39 +603,47 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             kb[p] = e >> 8;
             kn[p] = e & 0xff;
         }
-        // gain, sinc z, sinc x, tile-local trace (int bits).  The PX cells'
-        // loads are unconditional so they issue together: a cell past its
-        // count re-reads the tile's first contribution (staged on every
-        // injecting tile) and its sum is not taken.
-        // All PX cells at once in the reverse-time kernels (receiver lines
-        // inject into every cell of a strip); two at a time in the forward
-        // kernels, whose registers for four would spill.
-        constexpr int NB = ADJ ? PX : 2;
-        auto contribute = [&](int j, int p0, const float4v (&w)[NB]) {
-            float tv[NB];
-#pragma unroll
-            for (int q = 0; q < NB; q++) tv[q] = s_trv[__float_as_int(w[q][3])];
-#pragma unroll
-            for (int q = 0; q < NB; q++) {
-                const int p = p0 + q;
-                const float cv = w[q][0] * tv[q] * w[q][1] * w[q][2];
-                val[p] = j < kn[p] ? val[p] + cv : val[p];
-            }
-        };
-        const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
-#pragma unroll
-        for (int p0 = 0; p0 < PX; p0 += NB) {
+        // Chunk by chunk (the first one staged and folded above), each
+        // cell's contributions in its own order; the PX cells' LDS reads are
+        // unconditional so they issue together (a cell past its count reads
+        // entry 0 and its sum is not taken).
+        const int nco = ic1 - ic0;
+        if (nco <= kMaxTileCoef) {
             int km = 0;
 #pragma unroll
-            for (int q = 0; q < NB; q++) km = kn[p0 + q] > km ? kn[p0 + q] : km;
+            for (int p = 0; p < PX; p++) km = kn[p] > km ? kn[p] : km;
+            __syncthreads();
             for (int j = 0; j < km; j++) {
-                float4v w[NB];
+                float a[PX], b[PX];
 #pragma unroll
-                for (int q = 0; q < NB; q++) {
-                    const int k = kb[p0 + q] + (j < kn[p0 + q] ? j : 0);
-                    w[q] = coef_lds ? s_coef[k] : co[k];
+                for (int p = 0; p < PX; p++) {
+                    const int k = j < kn[p] ? kb[p] + j : 0;
+                    a[p] = s_coef[k][0];
+                    b[p] = s_coef[k][2];
                 }
-                contribute(j, p0, w);
+#pragma unroll
+                for (int p = 0; p < PX; p++) val[p] = j < kn[p] ? val[p] + a[p] * b[p] : val[p];
+            }
+        } else {
+            // more contributions than one chunk (dense receivers): chunk by
+            // chunk, one cell at a time, in each cell's order
+            const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
+            for (int cb = 0; cb < nco; cb += kMaxTileCoef) {
+                const int ce = nco - cb < kMaxTileCoef ? nco : cb + kMaxTileCoef;
+                if (cb > 0) {
+                    __syncthreads();   // every thread is done with the previous chunk
+                    for (int j = cb + tid; j < ce; j += NT) {
+                        float4v w = co[j];
+                        w[0] = w[0] * s_trv[__float_as_int(w[3])] * w[1];
+                        s_coef[j - cb] = w;
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int p = 0; p < PX; p++) {
+                    const int g0 = kb[p] > cb ? kb[p] : cb, g1 = kb[p] + kn[p] < ce ? kb[p] + kn[p] : ce;
+                    for (int g = g0; g < g1; g++) val[p] = val[p] + s_coef[g - cb][0] * s_coef[g - cb][2];
+                }
             }
         }
     }
