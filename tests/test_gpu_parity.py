@@ -190,8 +190,8 @@ def test_rtm_resident_traces():
 
 def test_rtm_dense_receivers():
     """Receivers every half cell: a tile holds more injection contributions
-    than its LDS stage (2048), so the reverse-time injection reads them from
-    global memory; the image must not change."""
+    (about 3000) than its LDS stage (2048), so the reverse-time injection
+    stages them in two chunks; the image must not change."""
     say("case: rtm dense receivers")
     nz, nx, nt = 40, 72, 200
     vel, w, _, _, _ = _rtm_case(nz=nz, nx=nx, nt=nt, seed=5)
