@@ -154,13 +154,22 @@ __device__ __forceinline__ int logical_block(const StepArgs &A) {
     return b;
 }
 
-// Snapshot idx (1-based) at window offset wq, decoded to FP32.
-__device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq, int t) {
+// Snapshot idx (1-based) at window offset wq, decoded to FP32.  stream: a
+// non-temporal load, for the imaging point's own snapshot, read once by RTM,
+// LSRTM and Born, so it does not push the wavefield levels of the next steps
+// out of the caches.  FWI imaging also reads the neighbouring snapshots
+// idx -+ 1 with cached loads; a snapshot's first read there is as a
+// neighbour, which keeps it for the two imaging steps that follow.
+__device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq, int t,
+                                           bool stream) {
+    const size_t q = (size_t)(idx - 1) * npix + wq;
     if (fp16) {
-        const unsigned short h = gp((const unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq];
+        const GLB unsigned short *p = gp((const unsigned short *)S.snap) + q;
+        const unsigned short h = stream ? __builtin_nontemporal_load(p) : *p;
         return __half2float(__ushort_as_half(h)) * gp(S.snap_scale)[(size_t)(idx - 1) * (S.ntz * S.ntx) + t];
     }
-    return gp((const float *)S.snap)[(size_t)(idx - 1) * npix + wq];
+    const GLB float *p = gp((const float *)S.snap) + q;
+    return stream ? __builtin_nontemporal_load(p) : *p;
 }
 
 // Receivers whose centre lies in this tile (ExtractData on u2,
@@ -190,6 +199,9 @@ __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, co
 // the stencil: Extraction.f90:595, Imaging.f90:265 / :365 / :738).
 // Sv, img0, ill0, tp: the source snapshot, image, illumination and taper at
 // this point, loaded ahead by the caller (imaging modes only).
+// FWI: the FWI imaging condition (the launch images with IMG_FWI); otherwise
+// RTM or LSRTM as A.imaging says (nothing for other values).
+template <bool FWI>
 __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
                                                  float u2c, float vel, int t, float inv16, float Sv, float img0,
                                                  float ill0, float tp) {
@@ -199,21 +211,22 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
     const int idx = it / A.snapi;
     if (A.store_snap) {
         if (A.snap_fp16) {
-            gp((unsigned short *)S.snap)[(size_t)(idx - 1) * npix + wq] = __half_as_ushort(__float2half_rn(u2c * inv16));
+            __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(u2c * inv16)),
+                                        gp((unsigned short *)S.snap) + (size_t)(idx - 1) * npix + wq);
         } else {
-            gp((float *)S.snap)[(size_t)(idx - 1) * npix + wq] = u2c;
+            __builtin_nontemporal_store(u2c, gp((float *)S.snap) + (size_t)(idx - 1) * npix + wq);
         }
     }
-    if (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) {
+    if (FWI || A.imaging == IMG_RTM || A.imaging == IMG_LSRTM) {
         GLB float *imp = gp(S.image), *ilp = gp(S.illum);
         float img = img0;
-        if (A.imaging == IMG_RTM) {
+        if (!FWI && A.imaging == IMG_RTM) {
             img = img + u2c * Sv * tp;
-        } else if (A.imaging == IMG_FWI) {
+        } else if (FWI) {
             const int im1 = idx - 1 > 1 ? idx - 1 : 1;
             const int ip1 = idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap;
-            const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq, t);
-            const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq, t);
+            const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq, t, false);
+            const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq, t, false);
             const float tmp = (Sm - 2.f * Sv + Sp) / (S.dt2snap * (vel * vel * vel));
             img = img + u2c * tmp * tp;
         } else {
@@ -339,11 +352,13 @@ struct StepShape {
 // MODE selects the code compiled in: 0 (lean) has no snapshot, imaging or
 // Born code and is launched for steps where no shot of the batch is at a
 // multiple of snapi (three steps in four); 1 adds the snapshot store (forward
-// passes that keep snapshots); 2 adds imaging and the Born source term, whose
-// operands take registers the other modes do not spend.
+// passes that keep snapshots); 2 adds RTM / LSRTM imaging and the Born
+// source term, whose operands take registers the other modes do not spend;
+// 3 is FWI imaging (three snapshots per point), kept out of mode 2 so the
+// RTM kernel carries none of its code.
 template <int PX, bool ADJ, int MODE>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
-    constexpr bool SNAP = MODE != 0, IMG = MODE == 2;
+    constexpr bool SNAP = MODE != 0, IMG = MODE >= 2;
     constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
     static_assert(NT >= 2 * TX && NT >= 2 * TZ, "one thread per Higdon line");
@@ -485,7 +500,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const bool snap_step = SNAP && (its % A.snapi) == 0 &&
                            (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
-    const bool img_step = IMG && snap_step && (A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM);
+    const bool img_step = IMG && snap_step && (MODE == 3 || A.imaging == IMG_RTM || A.imaging == IMG_LSRTM);
     float pS[PX], pI[PX], pL[PX], pT[PX];
     if (img_step) {
         const int wz = iz - S.i0 < 0 ? 0 : (iz - S.i0 >= S.nz ? S.nz - 1 : iz - S.i0);
@@ -496,7 +511,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             int wx = xs + p - S.i0;
             wx = wx < 0 ? 0 : (wx >= S.nxw ? S.nxw - 1 : wx);
             const size_t wq = (size_t)wx * S.nz + wz;
-            pS[p] = load_snap(S, A.snap_fp16, idx, npix, wq, t);
+            pS[p] = load_snap(S, A.snap_fp16, idx, npix, wq, t, true);
             pI[p] = gp(S.image)[wq];
             pL[p] = gp(S.illum)[wq];
             pT[p] = gp(S.taper)[wx];
@@ -583,9 +598,9 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         val[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
-            snap_image_point(S, A, it, iz, ix, u2c,
-                             (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f,
-                             t, inv16, pS[p], pI[p], pL[p], pT[p]);
+            snap_image_point<MODE == 3>(S, A, it, iz, ix, u2c,
+                                        (MODE == 3 || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f,
+                                        t, inv16, pS[p], pI[p], pL[p], pT[p]);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
 #define WX(k) (ADJ ? wv2[p + 4 + (k)] : wu[p + 4 + (k)])
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
@@ -655,10 +670,10 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
         float v = val[p];
         // Born secondary source: Imaging.f90:779-783, forward time only
-        if (IMG && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
+        if (MODE == 2 && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
-            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t);
+            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t, true);
             v = v + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
@@ -704,7 +719,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // find their shot through the logical-tile table.
 template <int PX, bool ADJ, bool SINGLE, int MODE>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
-__attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
+__attribute__((amdgpu_waves_per_eu(MODE >= 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {

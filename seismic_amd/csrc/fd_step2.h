@@ -98,13 +98,16 @@ __device__ __forceinline__ void snap_image_tile(const ShotDev &S, const StepArgs
         float Sv = 0.f, img0 = 0.f, ill0 = 0.f, tp = 0.f, vel = 0.f;
         if (img) {
             const size_t wq = (size_t)(ix - S.i0) * S.nz + (iz - S.i0);
-            Sv = load_snap(S, A.snap_fp16, it / A.snapi, npix, wq, t);
+            Sv = load_snap(S, A.snap_fp16, it / A.snapi, npix, wq, t, true);
             img0 = gp(S.image)[wq];
             ill0 = gp(S.illum)[wq];
             tp = gp(S.taper)[ix - S.i0];
             if (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) vel = gp(S.vel)[(size_t)ix * S.LDZ + S.zoff + iz];
         }
-        snap_image_point(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, t, inv16, Sv, img0, ill0, tp);
+        if (A.imaging == IMG_FWI)
+            snap_image_point<true>(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, t, inv16, Sv, img0, ill0, tp);
+        else
+            snap_image_point<false>(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, t, inv16, Sv, img0, ill0, tp);
     }
 }
 

@@ -700,8 +700,9 @@ static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int p
     const int want = px_override ? px_override : pick_px(TZ, TX);
 #define K(px)                                                                                            \
     if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
-        return mode == 2 ? step_fn<px, 2>(adj, single)                                                   \
-                         : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
+        return mode == 3 ? step_fn<px, 3>(adj, single)                                                   \
+               : mode == 2 ? step_fn<px, 2>(adj, single)                                                 \
+                           : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
     K(4) K(8) K(16)
 #undef K
     return nullptr;
@@ -712,12 +713,13 @@ bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, 
 // The code the launch needs (fd_step_body MODE): 0 unless the call stores
 // snapshots or images and some shot of the batch is at a step it = its
 // (its % snapi == 0) of its record in launch n (its = n+1 forward, nt-n
-// backward); then 2 when it images (or adds the Born source), else 1.
+// backward); then 3 when it images with the FWI condition, 2 when it images
+// otherwise (or adds the Born source), else 1.
 static int snap_launch(const Batch &B, const StepArgs &A) {
     if (!A.store_snap && A.imaging == IMG_NONE) return 0;
     for (int s = 0; s < B.ns; s++) {
         const int its = A.dir > 0 ? A.n + 1 : B.shots_h[s].nt - A.n;
-        if (its % A.snapi == 0) return A.imaging != IMG_NONE ? 2 : 1;
+        if (its % A.snapi == 0) return A.imaging == IMG_FWI ? 3 : (A.imaging != IMG_NONE ? 2 : 1);
     }
     return 0;
 }
