@@ -195,6 +195,20 @@ __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, co
     }
 }
 
+// The imaging conditions at one point: RTM img + u S t (Imaging.f90:265),
+// FWI img + u (S- - 2S + S+) / (dt_snap^2 v^3) t (:365), LSRTM
+// img + 2 t u S / v (:738).
+template <bool FWI>
+__device__ __forceinline__ float image_value(int imaging, float img, float u2c, float Sv, float Sm, float Sp,
+                                             float vel, float tp, float dt2snap) {
+    if (FWI) {
+        const float tmp = (Sm - 2.f * Sv + Sp) / (dt2snap * (vel * vel * vel));
+        return img + u2c * tmp * tp;
+    }
+    if (imaging == IMG_RTM) return img + u2c * Sv * tp;
+    return img + 2.f * tp * u2c * Sv / vel;
+}
+
 // Snapshot store and imaging condition at one interior point (on u2, before
 // the stencil: Extraction.f90:595, Imaging.f90:265 / :365 / :738).
 // Sv, img0, ill0, tp: the source snapshot, image, illumination and taper at
@@ -219,21 +233,85 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
     }
     if (FWI || A.imaging == IMG_RTM || A.imaging == IMG_LSRTM) {
         GLB float *imp = gp(S.image), *ilp = gp(S.illum);
-        float img = img0;
-        if (!FWI && A.imaging == IMG_RTM) {
-            img = img + u2c * Sv * tp;
-        } else if (FWI) {
-            const int im1 = idx - 1 > 1 ? idx - 1 : 1;
-            const int ip1 = idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap;
-            const float Sm = load_snap(S, A.snap_fp16, im1, npix, wq, t, false);
-            const float Sp = load_snap(S, A.snap_fp16, ip1, npix, wq, t, false);
-            const float tmp = (Sm - 2.f * Sv + Sp) / (S.dt2snap * (vel * vel * vel));
-            img = img + u2c * tmp * tp;
-        } else {
-            img = img + 2.f * tp * u2c * Sv / vel;
+        float Sm = 0.f, Sp = 0.f;
+        if (FWI) {
+            Sm = load_snap(S, A.snap_fp16, idx - 1 > 1 ? idx - 1 : 1, npix, wq, t, false);
+            Sp = load_snap(S, A.snap_fp16, idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap, npix, wq, t, false);
         }
-        imp[wq] = img;
+        imp[wq] = image_value<FWI>(A.imaging, img0, u2c, Sv, Sm, Sp, vel, tp, S.dt2snap);
         ilp[wq] = ill0 + Sv * Sv * tp;
+    }
+}
+
+// Snapshot store at one interior point (Extraction.f90:595 on u2, before the
+// stencil), non-temporal: the snapshots are read back only in the reverse
+// pass, so they should not push the wavefield levels out of the caches.
+__device__ __forceinline__ void store_snap_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
+                                                 float u2c, float inv16) {
+    const size_t wq = (size_t)(ix - S.i0) * S.nz + (iz - S.i0);
+    const size_t q = (size_t)(it / A.snapi - 1) * S.nz * S.nxw + wq;
+    if (A.snap_fp16)
+        __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(u2c * inv16)), gp((unsigned short *)S.snap) + q);
+    else
+        __builtin_nontemporal_store(u2c, gp((float *)S.snap) + q);
+}
+
+// Imaging at one step of a reverse pass (Imaging.f90:228-275 RTM, :322-370
+// FWI, :698-745 LSRTM) on the u2 that the step kernel of the same step
+// reads and does not write.  Grid: blockIdx.x = group of kImgCols window
+// columns, blockIdx.y = 256-row chunk, blockIdx.z = shot (SINGLE: the shot
+// descriptor comes from the kernel arguments).  A thread issues all loads of
+// its kImgCols cells before any use.  Launched beside the lean step kernel,
+// so that kernel keeps its occupancy and the imaging streams at its own
+// pace.  The FP16 snapshot scale is per step-kernel tile: 64 rows by
+// 1 << txs columns.
+constexpr int kImgCols = 4;
+template <bool FWI, bool SINGLE>
+__global__ void __launch_bounds__(256) image_kernel(StepArgs A, int txs) {
+    const ShotDev &S = SINGLE ? A.s0 : A.shots[blockIdx.z];
+    const int wx0 = blockIdx.x * kImgCols, wz = blockIdx.y * 256 + threadIdx.x;
+    if (wx0 >= S.nxw || wz >= S.nz) return;
+    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    int it = its;
+    if (A.nbase) {
+        const int b = *gp(A.nbase);
+        it = A.dir > 0 ? its + b : its - b;
+    } else if (it < 1 || it > S.nt) {
+        return;   // this shot has finished (ragged batch)
+    }
+    if (its % A.snapi != 0) return;
+    const int idx = it / A.snapi, iz = wz + S.i0;
+    const size_t npix = (size_t)S.nz * S.nxw;
+    const GLB float *u2 = gp(S.u[A.rot]);
+    const bool vel_needed = FWI || A.imaging == IMG_LSRTM;
+    float u[kImgCols], Sv[kImgCols], Sm[kImgCols], Sp[kImgCols], I0[kImgCols], L0[kImgCols], T[kImgCols],
+        V[kImgCols];
+#pragma unroll
+    for (int k = 0; k < kImgCols; k++) {
+        const int wx = wx0 + k < S.nxw ? wx0 + k : S.nxw - 1;   // clamped: loads stay in the window
+        const int ix = wx + S.i0;
+        const size_t q = (size_t)wx * S.nz + wz;
+        const size_t a = (size_t)ix * S.LDZ + S.zoff + iz;
+        const int t = ((iz + S.zsh) >> 6) + ((ix + S.xsh) >> txs) * S.ntz;
+        u[k] = u2[a];
+        Sv[k] = load_snap(S, A.snap_fp16, idx, npix, q, t, true);
+        if (FWI) {
+            Sm[k] = load_snap(S, A.snap_fp16, idx - 1 > 1 ? idx - 1 : 1, npix, q, t, false);
+            Sp[k] = load_snap(S, A.snap_fp16, idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap, npix, q, t, false);
+        } else {
+            Sm[k] = Sp[k] = 0.f;
+        }
+        I0[k] = gp(S.image)[q];
+        L0[k] = gp(S.illum)[q];
+        T[k] = gp(S.taper)[wx];
+        V[k] = vel_needed ? gp(S.vel)[a] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < kImgCols; k++) {
+        if (wx0 + k >= S.nxw) break;
+        const size_t q = (size_t)(wx0 + k) * S.nz + wz;
+        gp(S.image)[q] = image_value<FWI>(A.imaging, I0[k], u[k], Sv[k], Sm[k], Sp[k], V[k], T[k], S.dt2snap);
+        gp(S.illum)[q] = L0[k] + Sv[k] * Sv[k] * T[k];
     }
 }
 
@@ -352,13 +430,12 @@ struct StepShape {
 // MODE selects the code compiled in: 0 (lean) has no snapshot, imaging or
 // Born code and is launched for steps where no shot of the batch is at a
 // multiple of snapi (three steps in four); 1 adds the snapshot store (forward
-// passes that keep snapshots); 2 adds RTM / LSRTM imaging and the Born
-// source term, whose operands take registers the other modes do not spend;
-// 3 is FWI imaging (three snapshots per point), kept out of mode 2 so the
-// RTM kernel carries none of its code.
+// passes that keep snapshots); 2 adds the Born source term (Born's
+// scattering pass), whose operands take registers the other modes do not
+// spend.  Imaging steps of reverse passes launch mode 0 and image_kernel.
 template <int PX, bool ADJ, int MODE>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
-    constexpr bool SNAP = MODE != 0, IMG = MODE >= 2;
+    constexpr bool IMG = MODE == 2;
     constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
     static_assert(NT >= 2 * TX && NT >= 2 * TZ, "one thread per Higdon line");
@@ -494,29 +571,10 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         for (int p = 0; p < PX; p++) hv[p] = hv[p] * hu[p];
     }
 
-    // ---- imaging steps: the source snapshot, image, illumination and taper
-    // of this lane's cells, issued with the streaming loads (clamped window
-    // index; only window cells use them) so the column loop does not wait.
-    const bool snap_step = SNAP && (its % A.snapi) == 0 &&
-                           (A.store_snap || (A.imaging != IMG_NONE && A.imaging != IMG_BORN));
+    // ---- snapshot steps of a forward pass that keeps snapshots (mode 1;
+    // imaging runs in image_kernel beside the lean kernel)
+    const bool snap_step = MODE == 1 && (its % A.snapi) == 0 && A.store_snap;
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
-    const bool img_step = IMG && snap_step && (MODE == 3 || A.imaging == IMG_RTM || A.imaging == IMG_LSRTM);
-    float pS[PX], pI[PX], pL[PX], pT[PX];
-    if (img_step) {
-        const int wz = iz - S.i0 < 0 ? 0 : (iz - S.i0 >= S.nz ? S.nz - 1 : iz - S.i0);
-        const size_t npix = (size_t)S.nz * S.nxw;
-        const int idx = it / A.snapi;
-#pragma unroll
-        for (int p = 0; p < PX; p++) {
-            int wx = xs + p - S.i0;
-            wx = wx < 0 ? 0 : (wx >= S.nxw ? S.nxw - 1 : wx);
-            const size_t wq = (size_t)wx * S.nz + wz;
-            pS[p] = load_snap(S, A.snap_fp16, idx, npix, wq, t, true);
-            pI[p] = gp(S.image)[wq];
-            pL[p] = gp(S.illum)[wq];
-            pT[p] = gp(S.taper)[wx];
-        }
-    }
 
     // ---- injection tiles (workgroup-uniform branch): the trace samples and
     // the tile's contributions staged in LDS, so a contribution costs one
@@ -598,9 +656,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         val[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
         if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
-            snap_image_point<MODE == 3>(S, A, it, iz, ix, u2c,
-                                        (MODE == 3 || A.imaging == IMG_LSRTM) ? gp(S.vel)[(size_t)ix * LDZ + zo] : 0.f,
-                                        t, inv16, pS[p], pI[p], pL[p], pT[p]);
+            store_snap_point(S, A, it, iz, ix, u2c, inv16);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
 #define WX(k) (ADJ ? wv2[p + 4 + (k)] : wu[p + 4 + (k)])
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
@@ -670,7 +726,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
         float v = val[p];
         // Born secondary source: Imaging.f90:779-783, forward time only
-        if (MODE == 2 && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
+        if (IMG && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
             const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t, true);
@@ -719,7 +775,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // find their shot through the logical-tile table.
 template <int PX, bool ADJ, bool SINGLE, int MODE>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
-__attribute__((amdgpu_waves_per_eu(MODE >= 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
+__attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
