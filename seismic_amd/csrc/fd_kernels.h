@@ -124,6 +124,10 @@ struct StepArgs {
     const int *blk_shot;    // logical tile -> shot
     int nblocks;            // logical tiles (grid may be padded to a multiple of 8)
     int xcd_remap;          // map block ids so one XCD gets contiguous tiles
+    int step_grid;          // blocks of the launch that step tiles (the rest image, mode 3)
+    int img_gx, img_gy;     // mode 3: column groups and 256-row chunks of the largest window
+    int nshots;             // mode 3: shots of the batch
+    int img_txs;            // mode 3: log2 of the tile width (FP16 snapshot scale per tile)
     int n;                  // step counter from 0 (plus *nbase when nbase is set)
     const int *nbase;       // device-held step base of a replayed graph, or null
     int rot;                // n mod kLevels: u2 = U_n is u[rot]
@@ -148,7 +152,7 @@ __device__ __forceinline__ size_t addr(const ShotDev &S, int iz, int ix) {
 __device__ __forceinline__ int logical_block(const StepArgs &A) {
     int b = blockIdx.x;
     if (A.xcd_remap) {
-        int per = gridDim.x >> 3;
+        int per = A.step_grid >> 3;
         b = (b & 7) * per + (b >> 3);
     }
     return b;
@@ -257,19 +261,17 @@ __device__ __forceinline__ void store_snap_point(const ShotDev &S, const StepArg
 }
 
 // Imaging at one step of a reverse pass (Imaging.f90:228-275 RTM, :322-370
-// FWI, :698-745 LSRTM) on the u2 that the step kernel of the same step
-// reads and does not write.  Grid: blockIdx.x = group of kImgCols window
-// columns, blockIdx.y = 256-row chunk, blockIdx.z = shot (SINGLE: the shot
-// descriptor comes from the kernel arguments).  A thread issues all loads of
-// its kImgCols cells before any use.  Launched beside the lean step kernel,
-// so that kernel keeps its occupancy and the imaging streams at its own
-// pace.  The FP16 snapshot scale is per step-kernel tile: 64 rows by
-// 1 << txs columns.
+// FWI, :698-745 LSRTM) on the u2 that the step tiles of the same launch read
+// and do not write: 256 threads take 256 rows (chunk gy) of kImgCols window
+// columns (group gx) and issue all loads of their cells before any use.
+// These blocks follow the step tiles in the launch (mode 3), so they start
+// in the slots the first finished tiles free and the step tiles keep their
+// occupancy.  The FP16 snapshot scale is per step tile: 64 rows by
+// 1 << img_txs columns.
 constexpr int kImgCols = 4;
-template <bool FWI, bool SINGLE>
-__global__ void __launch_bounds__(256) image_kernel(StepArgs A, int txs) {
-    const ShotDev &S = SINGLE ? A.s0 : A.shots[blockIdx.z];
-    const int wx0 = blockIdx.x * kImgCols, wz = blockIdx.y * 256 + threadIdx.x;
+template <bool FWI>
+__device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S, int gx, int gy, int lane256) {
+    const int wx0 = gx * kImgCols, wz = gy * 256 + lane256;
     if (wx0 >= S.nxw || wz >= S.nz) return;
     const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     int it = its;
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(256) image_kernel(StepArgs A, int txs) {
         return;   // this shot has finished (ragged batch)
     }
     if (its % A.snapi != 0) return;
-    const int idx = it / A.snapi, iz = wz + S.i0;
+    const int idx = it / A.snapi, iz = wz + S.i0, txs = A.img_txs;
     const size_t npix = (size_t)S.nz * S.nxw;
     const GLB float *u2 = gp(S.u[A.rot]);
     const bool vel_needed = FWI || A.imaging == IMG_LSRTM;
@@ -313,6 +315,14 @@ __global__ void __launch_bounds__(256) image_kernel(StepArgs A, int txs) {
         gp(S.image)[q] = image_value<FWI>(A.imaging, I0[k], u[k], Sv[k], Sm[k], Sp[k], V[k], T[k], S.dt2snap);
         gp(S.illum)[q] = L0[k] + Sv[k] * Sv[k] * T[k];
     }
+}
+
+// Batched launches image in a kernel of their own beside the lean step
+// (grid: column groups, 256-row chunks, shots): folded into the step launch,
+// the batched variant's shot-table reads spill registers.
+template <bool FWI>
+__global__ void __launch_bounds__(256) image_kernel(StepArgs A) {
+    image_cells<FWI>(A, A.shots[blockIdx.z], (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x);
 }
 
 // Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
@@ -432,7 +442,9 @@ struct StepShape {
 // multiple of snapi (three steps in four); 1 adds the snapshot store (forward
 // passes that keep snapshots); 2 adds the Born source term (Born's
 // scattering pass), whose operands take registers the other modes do not
-// spend.  Imaging steps of reverse passes launch mode 0 and image_kernel.
+// spend; 3 is mode 0 plus imaging blocks after the step tiles (imaging
+// steps of reverse passes of one shot, image_cells; batches launch mode 0
+// and image_kernel).
 template <int PX, bool ADJ, int MODE>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
     constexpr bool IMG = MODE == 2;
@@ -572,7 +584,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     }
 
     // ---- snapshot steps of a forward pass that keeps snapshots (mode 1;
-    // imaging runs in image_kernel beside the lean kernel)
+    // reverse-pass imaging runs in the imaging blocks of mode 3)
     const bool snap_step = MODE == 1 && (its % A.snapi) == 0 && A.store_snap;
     const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
 
@@ -776,12 +788,28 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 template <int PX, bool ADJ, bool SINGLE, int MODE>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
 __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
+    // mode 3: the lean step plus imaging blocks after the step tiles
+    constexpr int BODY = MODE == 3 ? 0 : MODE;
+    if constexpr (MODE == 3) {
+        if ((int)blockIdx.x >= A.step_grid) {
+            constexpr int NV = StepShape<PX>::THREADS / 256;   // 256-thread imaging groups per block
+            const int vb = ((int)blockIdx.x - A.step_grid) * NV + (int)threadIdx.x / 256;
+            const int gx = vb % A.img_gx, r = vb / A.img_gx, gy = r % A.img_gy, s = r / A.img_gy;
+            if (s >= (SINGLE ? 1 : A.nshots)) return;
+            const ShotDev &S = SINGLE ? A.s0 : A.shots[s];
+            if (A.imaging == IMG_FWI)
+                image_cells<true>(A, S, gx, gy, (int)threadIdx.x & 255);
+            else
+                image_cells<false>(A, S, gx, gy, (int)threadIdx.x & 255);
+            return;
+        }
+    }
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
-        fd_step_body<PX, ADJ, MODE>(A, A.s0, lb);
+        fd_step_body<PX, ADJ, BODY>(A, A.s0, lb);
     } else {
-        fd_step_body<PX, ADJ, MODE>(A, A.shots[gp(A.blk_shot)[lb]], lb);
+        fd_step_body<PX, ADJ, BODY>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

@@ -700,8 +700,9 @@ static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int p
     const int want = px_override ? px_override : pick_px(TZ, TX);
 #define K(px)                                                                                            \
     if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
-        return mode == 2 ? step_fn<px, 2>(adj, single)                                                   \
-                         : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
+        return mode == 3   ? step_fn<px, 3>(adj, single)                                                 \
+               : mode == 2 ? step_fn<px, 2>(adj, single)                                                 \
+                           : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
     K(4) K(8) K(16)
 #undef K
     return nullptr;
@@ -713,7 +714,7 @@ bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, 
 // snapshots or images and some shot of the batch is at a step it = its
 // (its % snapi == 0) of its record in launch n (its = n+1 forward, nt-n
 // backward); then 2 when it images or adds the Born source, else 1.  Imaging
-// (RTM, LSRTM, FWI) launches mode 0 and image_kernel instead of mode 2.
+// (RTM, LSRTM, FWI) launches mode 3 (the lean step plus imaging blocks).
 static int snap_launch(const Batch &B, const StepArgs &A) {
     if (!A.store_snap && A.imaging == IMG_NONE) return 0;
     for (int s = 0; s < B.ns; s++) {
@@ -729,24 +730,36 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         A.s0 = B.shots_h[0];
     }
     const int mode = snap_launch(B, A);
+    // reverse-pass imaging: one shot, mode 3; a batch, mode 0 + image_kernel
     const bool image = mode == 2 && A.imaging != IMG_BORN;
-    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? 0 : mode, strip_px);
+    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? (A.single ? 3 : 0) : mode, strip_px);
     if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
     int grid = B.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
+    A.step_grid = grid;
     const int px = strip_px ? strip_px : pick_px(tz, tx);
     const int threads = 64 * ((px >= 8 ? 4 * px : 32) / px);
-    hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
     if (image) {
         if (tz != 64 || (tx & (tx - 1))) throw EngineError(SFD_EUNSUPPORTED, "imaging needs 64-row, 2^k-column tiles");
-        int nxw = 1, txs = 0;
+        int nxw = 1;
         for (int s = 0; s < B.ns; s++) nxw = std::max(nxw, B.shots_h[s].nxw);
-        while ((1 << txs) < tx) txs++;
-        const dim3 ig((nxw + kImgCols - 1) / kImgCols, (g.nz + 255) / 256, B.ns);
-        typedef void (*ImgFn)(StepArgs, int);
-        const ImgFn fi = A.imaging == IMG_FWI ? (A.single ? (ImgFn)image_kernel<true, true> : (ImgFn)image_kernel<true, false>)
-                                              : (A.single ? (ImgFn)image_kernel<false, true> : (ImgFn)image_kernel<false, false>);
-        hipLaunchKernelGGL(fi, ig, dim3(256), 0, stream, A, txs);
+        A.img_gx = (nxw + kImgCols - 1) / kImgCols;
+        A.img_gy = (g.nz + 255) / 256;
+        A.nshots = B.ns;
+        A.img_txs = 0;
+        while ((1 << A.img_txs) < tx) A.img_txs++;
+        if (A.single) {
+            const int groups = A.img_gx * A.img_gy, per_block = threads / 256;
+            grid += (groups + per_block - 1) / per_block;
+        }
+    }
+    hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
+    if (image && !A.single) {
+        const dim3 ig(A.img_gx, A.img_gy, B.ns);
+        if (A.imaging == IMG_FWI)
+            hipLaunchKernelGGL(image_kernel<true>, ig, dim3(256), 0, stream, A);
+        else
+            hipLaunchKernelGGL(image_kernel<false>, ig, dim3(256), 0, stream, A);
     }
 }
 
@@ -763,6 +776,7 @@ void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
     }
     int grid = B.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
+    A.step_grid = grid;
     typedef void (*Fn)(StepArgs);
     const Fn f = adj ? (A.single ? (Fn)fd_step2_kernel<true, true> : (Fn)fd_step2_kernel<true, false>)
                      : (A.single ? (Fn)fd_step2_kernel<false, true> : (Fn)fd_step2_kernel<false, false>);
