@@ -539,6 +539,18 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             const GLB int *mp = gp(S.inj_map) + imap;
 #pragma unroll
             for (int p = 0; p < PX; p++) me[p] = mp[lane + TZ * (wv * PX + p)];
+            // The tile's contributions reach LDS in chunks of kMaxTileCoef
+            // (one chunk except on the densest receiver tiles).  The first
+            // chunk goes straight from memory into LDS (no registers), so
+            // its loads can be issued here, ahead of the streaming loads:
+            // each lane writes 16 B at the wave's base + lane * 16, which is
+            // entry j for a wave whose base is entry j - lane.
+            const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
+            const int c1 = ic1 - ic0 < kMaxTileCoef ? ic1 - ic0 : kMaxTileCoef;
+            for (int j = tid; j < c1; j += NT)
+                __builtin_amdgcn_global_load_lds((const void *)(co + j),
+                                                 (__attribute__((address_space(3))) void *)(s_coef + (j - lane)), 16, 0,
+                                                 0);
         }
     }
 
@@ -604,12 +616,9 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             const int j = tid + r * NT;
             if (it0 + j < it1) s_trv[j] = trv_r[r];
         }
-        // The tile's contributions reach LDS in chunks of kMaxTileCoef (one
-        // chunk except on the densest receiver tiles); the first chunk's
-        // loads are issued here, with the trace samples' stores.
-        const GLB float4v *co = (const GLB float4v *)gp(S.inj_coef) + ic0;
         const int c1 = ic1 - ic0 < kMaxTileCoef ? ic1 - ic0 : kMaxTileCoef;
-        for (int j = tid; j < c1; j += NT) s_coef[j] = co[j];
+        // the memory-to-LDS loads of the first chunk count in vmcnt
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         // Fold this step's sample into each staged contribution: slot 0
         // becomes (gain * sample) * sinc_z, the first two products of the
