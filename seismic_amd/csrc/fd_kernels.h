@@ -475,13 +475,17 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // graph adds a device-held base (a multiple of kLevels and of snapi, so `its`
     // carries the residues the buffer rotation and snapshot tests need).
     const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
-    int it = its;
-    if (A.nbase) {
+    if (!A.nbase && (its < 1 || its > S.nt)) return;   // this shot has finished (ragged batch)
+    // The time index it = its (+ base) is read only where it is used
+    // (extraction and injection tiles, snapshot and Born steps): the base is
+    // a scalar memory read, and scalar reads return out of order, so reading
+    // it up front would hold every tile's kernel-argument reads, and with
+    // them its streaming loads, for one memory round trip.
+    auto time_index = [&]() -> int {
+        if (!A.nbase) return its;
         const int b = *gp(A.nbase);
-        it = A.dir > 0 ? its + b : its - b;
-    } else if (it < 1 || it > S.nt) {
-        return;   // this shot has finished (ragged batch)
-    }
+        return A.dir > 0 ? its + b : its - b;
+    };
     const int NZ = S.NZ, NX = S.NX, LDZ = S.LDZ, zoff = S.zoff;
     const int t = lb - S.tile0;
     const int tzi = t % S.ntz, txi = t / S.ntz;
@@ -507,7 +511,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // ---- receivers centred in this tile (reads u2 only; done before the
     // streaming loads so its registers are not live alongside them)
     if (A.extract && S.ext_tile && t >= S.ext_lo && t < S.ext_hi && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t])
-        extract_tile(S, t, it, u2, z0, x0);
+        extract_tile(S, t, time_index(), u2, z0, x0);
 
     // ---- injection tiles: descriptor, this step's trace samples and the
     // lane's cell-map entries are requested before the streaming loads (a
@@ -529,7 +533,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         const int tbase = __builtin_amdgcn_readfirstlane(dd[5]);
         has_inj = ic1 > ic0;
         if (has_inj) {
-            const GLB float *tm = gp((const float *)S.inj_tm) + (size_t)(it - 1) * S.inj_ntr;
+            const GLB float *tm = gp((const float *)S.inj_tm) + (size_t)(time_index() - 1) * S.inj_ntr;
             const GLB int *ttr = gp(S.inj_ttr);
 #pragma unroll
             for (int r = 0; r < kTrPerThread; r++) {
@@ -594,6 +598,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
         for (int p = 0; p < PX; p++) hv[p] = hv[p] * hu[p];
     }
+
+    const int it = (MODE == 1 || MODE == 2) ? time_index() : its;   // snapshot and Born steps
 
     // ---- snapshot steps of a forward pass that keeps snapshots (mode 1;
     // reverse-pass imaging runs in the imaging blocks of mode 3)
