@@ -622,20 +622,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             const int j = tid + r * NT;
             if (it0 + j < it1) s_trv[j] = trv_r[r];
         }
-        const int c1 = ic1 - ic0 < kMaxTileCoef ? ic1 - ic0 : kMaxTileCoef;
-        // the memory-to-LDS loads of the first chunk count in vmcnt
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
-        // Fold this step's sample into each staged contribution: slot 0
-        // becomes (gain * sample) * sinc_z, the first two products of the
-        // term in the reference's order, so a cell reads one LDS entry per
-        // contribution with no dependent gather and its term slot0 * sinc_x
-        // rounds exactly as the unfolded product.  Each thread folds what it
-        // staged; the injection pass reads the entries after a barrier.
-        for (int j = tid; j < c1; j += NT) {
-            const float4v w = s_coef[j];
-            s_coef[j][0] = w[0] * s_trv[__float_as_int(w[3])] * w[1];
-        }
+        // the staged entries are folded after the stencil pass, which so
+        // runs while the memory-to-LDS loads are still arriving
     }
 
     // FP16 snapshots carry one scale per tile: max|u2| over the tile's window
@@ -706,6 +694,22 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         // unconditional so they issue together (a cell past its count reads
         // entry 0 and its sum is not taken).
         const int nco = ic1 - ic0;
+        {
+            // the memory-to-LDS loads of the first chunk count in vmcnt
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            // Fold this step's sample into each staged contribution: slot 0
+            // becomes (gain * sample) * sinc_z, the first two products of
+            // the term in the reference's order, so a cell reads one LDS
+            // entry per contribution with no dependent gather and its term
+            // slot0 * sinc_x rounds exactly as the unfolded product.  Each
+            // thread folds the entries its own lane staged.
+            const int c1 = nco < kMaxTileCoef ? nco : kMaxTileCoef;
+            for (int j = tid; j < c1; j += NT) {
+                const float4v w = s_coef[j];
+                s_coef[j][0] = w[0] * s_trv[__float_as_int(w[3])] * w[1];
+            }
+        }
         if (nco <= kMaxTileCoef) {
             int km = 0;
 #pragma unroll
