@@ -191,6 +191,21 @@ void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
     p.i0 = 4 + g.ntaper;
     const int rz_ = (p.NZ - 1) % tz;
     p.zsh = rz_ >= 9 ? 0 : 9 - rz_;
+    {
+        // Where it adds no tile row, shift the rows so that the shallowest
+        // injection footprint (source or receivers, 3 rows above the centre)
+        // lies below the top tile row: the top Higdon band and the injection
+        // staging then run in different tiles, and a launch ends with its
+        // slowest tile.  The top tile keeps rows 0..9 (band + the two rows
+        // its recursion reads), the bottom tile the last 10 rows.
+        float zmin = s.sz;
+        for (int r = 0; r < s.ntr; r++) zmin = std::min(zmin, s.rz[r]);
+        const int rmin = p.i0 + (int)std::floor((zmin - g.oz) / g.dz) - 4;
+        const int want = tz - rmin;   // the smallest shift that puts row rmin in tile row 1
+        if (want > p.zsh && want <= tz - 10 && (p.NZ - 1 + want) % tz >= 9 &&
+            (p.NZ + want + tz - 1) / tz == (p.NZ + p.zsh + tz - 1) / tz)
+            p.zsh = want;
+    }
     const int rx_ = (p.NX - 1) % tx;
     p.xsh = rx_ >= 9 ? 0 : 9 - rx_;
     p.ntz = (p.NZ + p.zsh + tz - 1) / tz;
