@@ -17,6 +17,15 @@ struct DevBuf {
     ~DevBuf();
 };
 
+// Pinned host memory that grows on demand (device-to-host copies of images
+// at full PCIe rate, no per-call allocation).
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n);
+    ~HostBuf();
+};
+
 // Host-side plan of one shot: window, padded-grid geometry, tiles, and the
 // injection / extraction tables of the step kernel.
 struct ShotPlan {
@@ -104,6 +113,7 @@ struct Engine {
     bool tb_ok = true;                    // the current batch's region plans fit the two-step kernel
     Batch fwd_batch;
     DevBuf scratch, fwi_mem;
+    HostBuf img_host;
     DevBuf resident;                      // traces kept in HBM by sfd_set_traces
     size_t resident_n = 0;
     void set_traces(const float *tr, size_t n);

@@ -107,6 +107,20 @@ void DevBuf::reserve(size_t n) {
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
+void HostBuf::reserve(size_t n) {
+    if (n <= cap) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipHostMalloc(&p, n) != hipSuccess) {
+        p = nullptr;
+        throw EngineError(SFD_ENOMEM, "pinned host allocation of " + std::to_string(n) + " bytes failed");
+    }
+    cap = n;
+}
+HostBuf::~HostBuf() {
+    if (p) (void)hipHostFree(p);
+}
 
 static void check_grid(const sfd_grid &g) {
     if (g.nz < 2 || g.nx < 1 || !(g.dz > 0) || !(g.dx > 0) || !(g.dt > 0))
@@ -1061,7 +1075,8 @@ void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image,
     char *base = (char *)scratch.p;
     HIPCHK(hipMemcpyAsync(base + o_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, stream));
     if (!idx.empty()) HIPCHK(hipMemcpyAsync(base + o_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, stream));
-    std::vector<float> h(N);
+    img_host.reserve(N * 4);
+    float *h = (float *)img_host.p;
     for (int pass = 0; pass < 2; pass++) {
         float *dst = pass == 0 ? image : illum;
         if (!dst) continue;
@@ -1072,7 +1087,7 @@ void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image,
         hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
                            (const CopyImg *)(base + o_ci), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
                            (const int *)(base + o_idx), (float *)(base + o_out));
-        HIPCHK(hipMemcpyAsync(h.data(), base + o_out, N * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(h, base + o_out, N * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         for (size_t q = 0; q < N; q++) dst[q] = dst[q] + h[q];
     }
