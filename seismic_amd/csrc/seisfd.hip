@@ -725,11 +725,17 @@ static StepFn step_fn(bool adj, bool single) {
     return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, MODE> : (StepFn)fd_step_kernel<PX, true, false, MODE>)
                : (single ? (StepFn)fd_step_kernel<PX, false, true, MODE> : (StepFn)fd_step_kernel<PX, false, false, MODE>);
 }
+// Mode 3 (step tiles + imaging blocks) is launched for one shot only.
+template <int PX>
+static StepFn step_fn_image(bool adj, bool single) {
+    if (!single) return nullptr;
+    return adj ? (StepFn)fd_step_kernel<PX, true, true, 3> : (StepFn)fd_step_kernel<PX, false, true, 3>;
+}
 static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int px_override = 0) {
     const int want = px_override ? px_override : pick_px(TZ, TX);
 #define K(px)                                                                                            \
     if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
-        return mode == 3   ? step_fn<px, 3>(adj, single)                                                 \
+        return mode == 3   ? step_fn_image<px>(adj, single)                                              \
                : mode == 2 ? step_fn<px, 2>(adj, single)                                                 \
                            : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
     K(4) K(8) K(16)
