@@ -883,6 +883,27 @@ int FN(born)(const sfd_grid *g, const REAL *vel, const REAL *refl, const sfd_sho
     return 0;
 }
 
+/* Check helper: one interior time step — FD_scheme's forward stencil
+ * (FDderivatives.f90:715-743) then the 5-level TimeDer update
+ * (FDderivatives.f90:974-976) — on the FD region of a frame with nz x nxw
+ * interior cells and ntaper taper cells, with no injection and no absorbing
+ * boundary.  um, u0, u1, u2 (u_{-1} .. u2), vel and u3 are padded arrays of
+ * the frame (orc_bounds: NZ x NX, z fastest); u3 is written on FD cells. */
+int FN(test_step)(const sfd_grid *g, int nxw, const REAL *vel, const REAL *um, const REAL *u0, const REAL *u1,
+                  const REAL *u2, REAL *u3)
+{
+    PROP P;
+    memset(&P, 0, sizeof P);
+    P.nz = g->nz; P.nxw = nxw;
+    P.dz = g->dz; P.dx = g->dx; P.dt = g->dt;
+    orc_bounds(&P.b, g->nz, nxw, g->ntaper);
+    P.vel = (REAL *)vel;
+    CAT(init_coefs_, SFX)(&P);
+    CAT(fd_forward_, SFX)(&P, u2, u3);
+    CAT(time_update_, SFX)(&P, um, u0, u1, u2, u3);
+    return 0;
+}
+
 /* Check helper: the AGRAD backward pass (forward stencil, Injection_simple of
  * the traces, reverse time) with u2 captured at every imaging step. */
 int FN(test_agrad_capture)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *data,

@@ -508,3 +508,18 @@ def modeling_adjoint(g, vel, shot, dbar, extract_kind=0, prec="d"):
     rc = fn(C.byref(g), ptr(v), C.byref(cs), ptr(db), ptr(sb), extract_kind)
     assert rc == 0
     return sb
+
+
+def test_step(g, nxw, vel, levels, prec="d"):
+    """One interior step of the oracle (stencil + 5-level update, no boundary):
+    levels = (u_-1, u0, u1, u2) as padded (NX, NZ) arrays; returns u3 (NX, NZ),
+    written on FD cells (zeros elsewhere)."""
+    L = oracle()
+    T = _dt(prec)
+    arrs = [np.ascontiguousarray(a, dtype=T) for a in (vel,) + tuple(levels)]
+    u3 = np.zeros_like(arrs[1])
+    fn = getattr(L, "orc_%s_test_step" % prec)
+    fn.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+    rc = fn(C.byref(g), nxw, *[ptr(a) for a in arrs], ptr(u3))
+    assert rc == 0
+    return u3

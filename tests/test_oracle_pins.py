@@ -172,3 +172,64 @@ def test_higdon_absorbs_outgoing_energy():
     assert E[-1] < 1e-5 * E[k]
     # 400 ms after the peak the wavelet has crossed the 800 m model at 2000 m/s
     assert np.all(E[k + 40:] < 1e-4 * E[k])
+
+
+def _frame(nz=16, nxw=16, ntaper=4, dx=10.0, dt=2.0 ** -9, v=2000.0):
+    """Padded frame of the oracle's single step: centred coordinates (X, Z)
+    of every padded cell, (NX, NZ) arrays, and the mask of FD cells."""
+    g = O.make_grid(nz, nxw, dx, dx, dt, ntaper=ntaper)
+    NZ, NX = nz + 2 * ntaper + 8, nxw + 2 * ntaper + 8
+    z = (np.arange(NZ) - (ntaper + 3) - 1) * dx
+    x = (np.arange(NX) - (ntaper + 3) - 1) * dx
+    X, Z = np.meshgrid(x - x.mean(), z - z.mean(), indexing="ij")
+    fd = np.zeros((NX, NZ), bool)
+    fd[4:NX - 4, 4:NZ - 4] = True
+    return g, nxw, X, Z, np.full((NX, NZ), v), fd
+
+
+def _one_step_error(frame, f, dt):
+    g, nxw, X, Z, vel, fd = frame
+    u3 = O.test_step(g, nxw, vel, [f(X, Z, t) for t in (-3 * dt, -2 * dt, -dt, 0.0)])
+    ex = f(X, Z, dt)
+    return float(np.abs(u3 - ex)[fd].max() / np.abs(ex[fd]).max())
+
+
+def test_time_scheme_exact_for_polynomials_of_degree_4():
+    """u = (d.x - v t)^k solves the wave equation; the 8th-order stencil is
+    exact for it and the 5-level update (20u2 - 6u1 - 4u0 + u_-1 + 12dt^2
+    v^2 lap u2) / 11 reproduces polynomials of degree <= 4 in t, so one step
+    of the oracle gives u(t + dt) to FP64 rounding for k <= 4, along x, z and
+    an oblique direction.  k = 5 is not reproduced (the scheme's order), which
+    shows the check can fail.  dt = 2^-9 s is exact in the grid's FP32 field."""
+    dt, v = 2.0 ** -9, 2000.0
+    fr = _frame(dt=dt, v=v)
+    for cx, cz in ((1.0, 0.0), (0.0, -1.0), (0.6, 0.8)):
+        for k in range(1, 6):
+            e = _one_step_error(fr, lambda X, Z, t: (cx * X + cz * Z - v * t) ** k, dt)
+            print("direction (%.1f, %.1f) k=%d: one-step relative error %.2e" % (cx, cz, k, e))
+            if k <= 4:
+                assert e <= 1e-13
+            else:
+                assert e >= 1e-9
+
+
+def test_plane_wave_dispersion():
+    """A plane wave cos(k.x - w t) with the continuous relation w = v |k|,
+    advanced one step from its exact history: the one-step error over w dt
+    estimates the scheme's relative phase-velocity error.  It must stay below
+    1e-3 at 5 points per wavelength and 1e-4 at 10 (Courant 0.2), and fall as
+    (w dt)^4 (the time scheme's order) as the sampling doubles.  A 1% error in
+    the phase velocity, or a stencil coefficient off by a few percent, breaks
+    the bounds."""
+    dt, v, dx = 2.0 ** -10, 2000.0, 10.0
+    fr = _frame(nz=40, nxw=40, dt=dt, v=v, dx=dx)
+    err = {}
+    for ppw in (5, 10, 20):
+        kx = 2 * np.pi / (ppw * dx)
+        kz = 0.5 * kx
+        w = v * np.hypot(kx, kz)
+        e = _one_step_error(fr, lambda X, Z, t: np.cos(kx * X + kz * Z - w * t), dt)
+        err[ppw] = e / (w * dt)
+        print("%d points per wavelength: phase-velocity error ~ %.2e" % (ppw, err[ppw]))
+    assert err[5] < 1e-3 and err[10] < 1e-4
+    assert 12.0 < err[10] / err[20] < 20.0
