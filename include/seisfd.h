@@ -277,7 +277,8 @@ typedef struct {
  *   state_out receives the new state unless iflag < 0 on exit (written to a
  *   temporary file, then renamed); it may name the same file as state_in.
  * The state file also holds n, m and the bounds of the run: a mismatch with
- * the arguments is SFD_EIO. */
+ * the arguments is SFD_EIO, and so is a file whose closing 64-bit checksum
+ * does not match its contents (truncated or damaged). */
 int  sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfgsoc_opts *o,
                       int n, double *x, double f, const double *g, int *iflag, int *info);
 

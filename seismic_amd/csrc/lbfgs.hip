@@ -380,9 +380,11 @@ L172:
 // ---------------------------------------------------------------------------
 // Optimiser state on file (the out-of-core protocol of nlinv_lbfgs.f, which
 // keeps the LBFGSOC/MCSRCHOC SAVE scalars in LbfgsDat.dat / McsrchDat.dat and
-// W, DIAG in WorkArray.dat / Diag.dat): one binary file, scalars exact.
+// W, DIAG in WorkArray.dat / Diag.dat): one binary file, scalars exact, ended
+// by a 64-bit FNV-1a checksum of everything before it (8-byte words, then any
+// tail bytes), so a damaged file is refused rather than resumed from.
 namespace {
-constexpr char kStateMagic[8] = {'S', 'F', 'D', 'L', 'B', 'O', 'C', '1'};
+constexpr char kStateMagic[8] = {'S', 'F', 'D', 'L', 'B', 'O', 'C', '2'};
 struct StateHead {
     char magic[8];
     int64_t n;
@@ -394,14 +396,27 @@ struct StateHead {
 
 struct File {
     FILE *f = nullptr;
+    uint64_t sum = 14695981039346656037ull;   // FNV-1a offset basis
     ~File() { if (f) fclose(f); }
+    void add(const void *p, size_t b) {
+        const unsigned char *c = (const unsigned char *)p;
+        size_t k = 0;
+        for (; k + 8 <= b; k += 8) {
+            uint64_t w;
+            memcpy(&w, c + k, 8);
+            sum = (sum ^ w) * 1099511628211ull;
+        }
+        for (; k < b; k++) sum = (sum ^ c[k]) * 1099511628211ull;
+    }
 };
 
-void wr(FILE *f, const void *p, size_t b) {
-    if (b && fwrite(p, 1, b, f) != b) throw EngineError(SFD_EIO, "L-BFGS state: write failed");
+void wr(File &fo, const void *p, size_t b) {
+    if (b && fwrite(p, 1, b, fo.f) != b) throw EngineError(SFD_EIO, "L-BFGS state: write failed");
+    fo.add(p, b);
 }
-void rd(FILE *f, void *p, size_t b) {
-    if (b && fread(p, 1, b, f) != b) throw EngineError(SFD_EIO, "L-BFGS state: file truncated");
+void rd(File &fi, void *p, size_t b) {
+    if (b && fread(p, 1, b, fi.f) != b) throw EngineError(SFD_EIO, "L-BFGS state: file truncated");
+    fi.add(p, b);
 }
 }  // namespace
 
@@ -421,22 +436,23 @@ void LbfgsDev::save_state(const char *path) {
         File fo;
         fo.f = fopen(tmp.c_str(), "wb");
         if (!fo.f) throw EngineError(SFD_EIO, "L-BFGS state: cannot create " + tmp);
-        wr(fo.f, &h, sizeof h);
-        wr(fo.f, pxmin.data(), sizeof(double) * nparam);
-        wr(fo.f, pxmax.data(), sizeof(double) * nparam);
-        wr(fo.f, rho.data(), sizeof(double) * m);
-        wr(fo.f, alpha.data(), sizeof(double) * m);
+        wr(fo, &h, sizeof h);
+        wr(fo, pxmin.data(), sizeof(double) * nparam);
+        wr(fo, pxmax.data(), sizeof(double) * nparam);
+        wr(fo, rho.data(), sizeof(double) * m);
+        wr(fo, alpha.data(), sizeof(double) * m);
         std::vector<double> buf(n);
         auto put = [&](const double *dv) {
             HIPCHK(hipMemcpyAsync(buf.data(), dv, sizeof(double) * n, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            wr(fo.f, buf.data(), sizeof(double) * n);
+            wr(fo, buf.data(), sizeof(double) * n);
         };
         put(q);
         put(diag);
         for (int k = 0; k < m; k++) put(Sv(k));
         for (int k = 0; k < m; k++) put(Yv(k));
-        if (fflush(fo.f) != 0) throw EngineError(SFD_EIO, "L-BFGS state: write failed");
+        const uint64_t sum = fo.sum;
+        if (fwrite(&sum, 1, 8, fo.f) != 8 || fflush(fo.f) != 0) throw EngineError(SFD_EIO, "L-BFGS state: write failed");
     }
     if (rename(tmp.c_str(), path) != 0) throw EngineError(SFD_EIO, std::string("L-BFGS state: cannot rename to ") + path);
 }
@@ -446,13 +462,13 @@ void LbfgsDev::load_state(const char *path) {
     fi.f = fopen(path, "rb");
     if (!fi.f) throw EngineError(SFD_EIO, std::string("L-BFGS state: cannot open ") + path);
     StateHead h;
-    rd(fi.f, &h, sizeof h);
+    rd(fi, &h, sizeof h);
     if (memcmp(h.magic, kStateMagic, 8) != 0) throw EngineError(SFD_EIO, "L-BFGS state: not a state file");
     if (h.n != n || h.m != m || h.nparam != nparam)
         throw EngineError(SFD_EIO, "L-BFGS state: n, m or nparam differ from the run's");
     std::vector<double> b0(nparam), b1(nparam);
-    rd(fi.f, b0.data(), sizeof(double) * nparam);
-    rd(fi.f, b1.data(), sizeof(double) * nparam);
+    rd(fi, b0.data(), sizeof(double) * nparam);
+    rd(fi, b1.data(), sizeof(double) * nparam);
     if (b0 != pxmin || b1 != pxmax) throw EngineError(SFD_EIO, "L-BFGS state: bounds differ from the run's");
     iter = h.iter; nfun = h.nfun; point = h.point; maxfev = h.maxfev; info = h.info; bound = h.bound;
     npt = h.npt; cp = h.cp; nfev = h.nfev; finish = h.finish; infoc = h.infoc; brackt = h.brackt;
@@ -461,11 +477,11 @@ void LbfgsDev::load_state(const char *path) {
     gtol = h.gtol; stpmin = h.stpmin; stpmax = h.stpmax; xtrapf = h.xtrapf;
     dginit = h.dginit; dgtest = h.dgtest; finit = h.finit; width = h.width; width1 = h.width1; stx = h.stx;
     fx = h.fx; dgx = h.dgx; sty = h.sty; fy = h.fy; dgy = h.dgy; stmin = h.stmin; stmax = h.stmax;
-    rd(fi.f, rho.data(), sizeof(double) * m);
-    rd(fi.f, alpha.data(), sizeof(double) * m);
+    rd(fi, rho.data(), sizeof(double) * m);
+    rd(fi, alpha.data(), sizeof(double) * m);
     std::vector<double> buf(n);
     auto get = [&](double *dv) {
-        rd(fi.f, buf.data(), sizeof(double) * n);
+        rd(fi, buf.data(), sizeof(double) * n);
         HIPCHK(hipMemcpyAsync(dv, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     };
@@ -473,6 +489,10 @@ void LbfgsDev::load_state(const char *path) {
     get(diag);
     for (int k = 0; k < m; k++) get(Sv(k));
     for (int k = 0; k < m; k++) get(Yv(k));
+    const uint64_t sum = fi.sum;
+    uint64_t stored = 0;
+    if (fread(&stored, 1, 8, fi.f) != 8) throw EngineError(SFD_EIO, "L-BFGS state: file truncated");
+    if (stored != sum) throw EngineError(SFD_EIO, "L-BFGS state: checksum mismatch (damaged file)");
     if (fgetc(fi.f) != EOF) throw EngineError(SFD_EIO, "L-BFGS state: trailing data");
 }
 

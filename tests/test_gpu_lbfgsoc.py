@@ -163,3 +163,31 @@ def test_fortran_lbfgsoc_one_process_per_update(tmp_path):
     ro = drive(O.LBFGSOC(n, m=5, xmin=bmin, xmax=bmax, stp1_opt=1, boundt=2), fg, np.full(n, 1.0))
     compare(ro, rec, len(rec))
     print("process-per-update calls: %d" % len(rec))
+
+
+def test_lbfgsoc_damaged_state_file(tmp_path):
+    """The state file ends with a checksum of its contents: a truncated file,
+    one flipped byte in the stored vectors or in the checksum itself is
+    SFD_EIO; the undamaged file resumes."""
+    print("case: lbfgsoc damaged state", flush=True)
+    fg, n = bounded_quad(), 40
+    kw = dict(m=5, xmin=(0.5, -2.0), xmax=(3.0, 0.3), stp1_opt=1, clip_type=2)
+    path = str(tmp_path / "s.bin")
+    lb = sa.LBFGSOC(n, path, **kw)
+    x = np.full(n, 1.0)
+    for _ in range(4):
+        f, g = fg(x)
+        lb.step(x, f, g)
+    good = open(path, "rb").read()
+    flip = lambda b, k: b[:k] + bytes([b[k] ^ 0x10]) + b[k + 1:]
+    for name, bad in (("truncated", good[:-16]), ("vector byte", flip(good, len(good) // 2)),
+                      ("checksum byte", flip(good, len(good) - 3))):
+        open(path, "wb").write(bad)
+        f, g = fg(x)
+        with pytest.raises(sa.SeisFDError) as e:
+            sa.LBFGSOC(n, path, resume=True, info=lb.info.value, **kw).step(x.copy(), f, g)
+        assert e.value.code == -8, name
+    open(path, "wb").write(good)
+    f, g = fg(x)
+    iflag, _ = sa.LBFGSOC(n, path, resume=True, info=lb.info.value, **kw).step(x.copy(), f, g)
+    assert iflag in (0, 1)
