@@ -1,7 +1,7 @@
 """GPU parity: the HIP engine (through the C-ABI) against the CPU oracle on
-the same seeded inputs.  Bar: relative L2 <= 1e-5 in FP32 (north-star
-tolerance); the engine is built to reproduce the reference's FP32 rounding, so
-most cases are also reported (and asserted where stated) bitwise."""
+the same seeded inputs.  Bar: relative L2 <= 1e-5 in FP32; the engine is
+built to reproduce the reference's FP32 rounding, so most cases are also
+reported (and asserted where stated) bitwise."""
 import numpy as np
 import pytest
 
