@@ -699,15 +699,14 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             // Fold this step's sample into each staged contribution: slot 0
-            // becomes (gain * sample) * sinc_z, the first two products of
-            // the term in the reference's order, so a cell reads one LDS
-            // entry per contribution with no dependent gather and its term
-            // slot0 * sinc_x rounds exactly as the unfolded product.  Each
-            // thread folds the entries its own lane staged.
+            // becomes the whole term ((gain * sample) * sinc_z) * sinc_x,
+            // the reference's product in its order, so a cell reads one LDS
+            // value per contribution, with no dependent gather, and adds it.
+            // Each thread folds the entries its own lane staged.
             const int c1 = nco < kMaxTileCoef ? nco : kMaxTileCoef;
             for (int j = tid; j < c1; j += NT) {
                 const float4v w = s_coef[j];
-                s_coef[j][0] = w[0] * s_trv[__float_as_int(w[3])] * w[1];
+                s_coef[j][0] = w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
             }
         }
         if (nco <= kMaxTileCoef) {
@@ -715,16 +714,19 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
             for (int p = 0; p < PX; p++) km = kn[p] > km ? kn[p] : km;
             __syncthreads();
-            for (int j = 0; j < km; j++) {
-                float a[PX], b[PX];
+#pragma unroll 1
+            for (int j = 0; j < km; j += 2) {
+                float c[PX], c2[PX];
 #pragma unroll
                 for (int p = 0; p < PX; p++) {
-                    const int k = j < kn[p] ? kb[p] + j : 0;
-                    a[p] = s_coef[k][0];
-                    b[p] = s_coef[k][2];
+                    c[p] = s_coef[j < kn[p] ? kb[p] + j : 0][0];
+                    c2[p] = s_coef[j + 1 < kn[p] ? kb[p] + j + 1 : 0][0];
                 }
 #pragma unroll
-                for (int p = 0; p < PX; p++) val[p] = j < kn[p] ? val[p] + a[p] * b[p] : val[p];
+                for (int p = 0; p < PX; p++) {
+                    val[p] = j < kn[p] ? val[p] + c[p] : val[p];
+                    val[p] = j + 1 < kn[p] ? val[p] + c2[p] : val[p];
+                }
             }
         } else {
             // more contributions than one chunk (dense receivers): chunk by
@@ -736,7 +738,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
                     __syncthreads();   // every thread is done with the previous chunk
                     for (int j = cb + tid; j < ce; j += NT) {
                         float4v w = co[j];
-                        w[0] = w[0] * s_trv[__float_as_int(w[3])] * w[1];
+                        w[0] = w[0] * s_trv[__float_as_int(w[3])] * w[1] * w[2];
                         s_coef[j - cb] = w;
                     }
                 }
@@ -744,7 +746,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
                 for (int p = 0; p < PX; p++) {
                     const int g0 = kb[p] > cb ? kb[p] : cb, g1 = kb[p] + kn[p] < ce ? kb[p] + kn[p] : ce;
-                    for (int g = g0; g < g1; g++) val[p] = val[p] + s_coef[g - cb][0] * s_coef[g - cb][2];
+                    for (int g = g0; g < g1; g++) val[p] = val[p] + s_coef[g - cb][0];
                 }
             }
         }
