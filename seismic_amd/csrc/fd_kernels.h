@@ -513,10 +513,11 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     if (A.extract && S.ext_tile && t >= S.ext_lo && t < S.ext_hi && gp(S.ext_tile)[t + 1] > gp(S.ext_tile)[t])
         extract_tile(S, t, time_index(), u2, z0, x0);
 
-    // ---- injection tiles: descriptor, this step's trace samples and the
-    // lane's cell-map entries are requested before the streaming loads (a
-    // wave's loads return in order, so these come back first); they reach LDS
-    // with the contributions further down, behind one barrier.
+    // ---- injection tiles: descriptor, this step's trace samples, the lane's
+    // cell-map entries and the first chunk of contributions (memory to LDS)
+    // are requested before the streaming loads (a wave's loads return in
+    // order, so these come back first); the samples reach LDS further down,
+    // and the contributions are folded after the stencil pass.
     bool has_inj = A.inject && t >= S.inj_lo && t < S.inj_hi;
     int ic0 = 0, ic1 = 0, it0 = 0, it1 = 0, imap = 0;
     constexpr int kTrPerThread = (kMaxTileTraces + NT - 1) / NT;
