@@ -512,7 +512,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t inj_desc, inj_map, injb_desc, injb_map;
         size_t inj2[6], injb2[6];
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
-        size_t snap, snap_scale, image, illum, born_m;
+        size_t snap, snap_bytes, snap_scale, image, illum, born_m;
     };
     std::vector<Off> off(ns);
     const bool tb = tz == B2Z && tx == B2X;
@@ -577,7 +577,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         o.ext_w = put(L, up, upn, p.ext_w);
         o.traces = L.take((size_t)p.nt * p.ntr * sizeof(float));
         const size_t npix = (size_t)g.nz * p.nxw;
-        o.snap = L.take(need.snapshots ? (size_t)std::max(p.ntsnap, 1) * npix * (need.snap_fp16 ? 2 : 4) : 16);
+        o.snap_bytes = need.snapshots ? (size_t)std::max(p.ntsnap, 1) * npix * (need.snap_fp16 ? 2 : 4) : 16;
+        o.snap = L.take(o.snap_bytes);
         o.snap_scale = L.take(need.snap_fp16 ? (size_t)std::max(p.ntsnap, 1) * p.ntz * p.ntx * sizeof(float) : 16);
         o.image = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.illum = L.take(need.imaging ? npix * sizeof(float) : 16);
@@ -592,8 +593,18 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     const size_t o_shots = L.take(sizeof(ShotDev) * ns);
     B.mem.reserve(L.off);
     char *base = (char *)B.mem.p;
-    // zero everything once (wavefields, accumulators, traces), then upload plans
-    HIPCHK(hipMemsetAsync(base, 0, L.off, stream));
+    // zero everything once (wavefields, accumulators, traces) except the
+    // snapshots, then upload plans.  A pass that stores snapshots writes
+    // every slot 1..ntsnap before any step reads it, so they need no fill
+    // (on a 751x2301 single-shot RTM that fill was 13 GB).
+    {
+        size_t z0 = 0;
+        for (int s = 0; s < ns; s++) {
+            if (off[s].snap > z0) HIPCHK(hipMemsetAsync(base + z0, 0, off[s].snap - z0, stream));
+            z0 = off[s].snap + off[s].snap_bytes;
+        }
+        if (L.off > z0) HIPCHK(hipMemsetAsync(base + z0, 0, L.off - z0, stream));
+    }
     for (size_t i = 0; i < up.size(); i++)
         HIPCHK(hipMemcpyAsync(base + up[i].first, up[i].second, upn[i], hipMemcpyHostToDevice, stream));
     B.shots_h.assign(ns, ShotDev());
