@@ -244,6 +244,13 @@ def main():
     t1 = time.perf_counter()
     e.rtm([shot], [dobs], snap_mode=snap)
     host_rate = e.last_timing()["pts"] / (time.perf_counter() - t1) / 1e9
+    # the same RTM planned afresh (no reuse of the previous call's tables):
+    # host planning and table upload inside the measured call
+    e.set_option(sa.abi.SFD_OPT_PLAN_REUSE, 0)
+    t1 = time.perf_counter()
+    e.rtm([shot], None, snap_mode=snap)
+    fresh_rate = e.last_timing()["pts"] / (time.perf_counter() - t1) / 1e9
+    e.set_option(sa.abi.SFD_OPT_PLAN_REUSE, 1)
 
     # roofline of the step kernel: algorithmic bytes per launch / measured duration
     nfd = (NZ + 2 * NTAPER) * (NX + 2 * NTAPER)
@@ -286,6 +293,8 @@ def main():
                       "tile": args.tile, "strip": args.strip, "time_block": args.time_block, "pass_ms": [round(tm["fwd_ms"], 1), round(tm["bwd_ms"], 1)],
                       "inputs": "observed data resident in HBM (sfd_set_traces) before the timed region",
                       "host_inputs_Gpts_per_s": round(host_rate, 3),
+                      "plan": "tables planned by the warm-up call and reused (SFD_OPT_PLAN_REUSE)",
+                      "fresh_plan_Gpts_per_s": round(fresh_rate, 3),
                       "device": info["name"]},
            "roofline": roof, "cpu_baseline": cpu, "fwi": fwi,
            "batched": batched}

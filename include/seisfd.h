@@ -216,6 +216,8 @@ int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, int *ti
 #define SFD_OPT_KERNEL_EVENTS 4 /* 1: time every block of steps with HIP events (default) */
 #define SFD_OPT_STRIP_PX     5  /* columns per wavefront in the step kernel: 0 auto (4 for 64x32 tiles), 4, 8, 16 */
 #define SFD_OPT_TIME_BLOCK   6  /* time steps per launch: 1 (default) or 2 (64x32 tiles; not Born passes) */
+#define SFD_OPT_PLAN_REUSE   7  /* 1: an RTM/LSRTM call with the same shots, model and tile settings as
+                                   the previous call reuses its plan tables on the device (default) */
 int  sfd_set_option(sfd_handle h, int opt, int value);
 
 /*

@@ -33,6 +33,7 @@ SFD_OPT_XCD_REMAP = 3
 SFD_OPT_KERNEL_EVENTS = 4
 SFD_OPT_STRIP_PX = 5
 SFD_OPT_TIME_BLOCK = 6
+SFD_OPT_PLAN_REUSE = 7
 
 
 class sfd_grid(C.Structure):

@@ -82,6 +82,12 @@ struct Batch {
     std::vector<int> blk_shot_h;
     sfd::ShotDev *shots = nullptr;
     const int *blk_shot = nullptr;
+    // Reuse by a later call with the same plan (empty: none): the key of the
+    // plan the device tables hold, the shot descriptors as built and the
+    // byte ranges zeroed at build time (everything but the snapshots).
+    std::string key;
+    std::vector<sfd::ShotDev> shots_built;
+    std::vector<std::pair<size_t, size_t>> zero_ranges;
 };
 
 void second_difference(std::vector<float> &w, float dt);
@@ -92,6 +98,8 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     std::vector<float> vel_h, velb_h;
+    uint64_t model_version = 0;           // bumped by every set_model (plans read vel_h / velb_h)
+    std::vector<ShotPlan> rtm_plans;      // host plan behind fwd_batch when its key is an RTM's
     float *d_vel = nullptr, *d_velb = nullptr;
     bool have_model = false;
     int tz = 64, tx = 32;
@@ -100,6 +108,7 @@ struct Engine {
     int xcd_remap = 1;
     int strip_px = 0;        // columns per wavefront (0: tile width / 4)
     int time_block = 1;      // steps per launch: 1, or 2 (two-step kernel where it applies)
+    bool plan_reuse = true;  // SFD_OPT_PLAN_REUSE
     int graph_chunk = 200;   // steps per replayed block (rounded up to a multiple of lcm(kLevels, snapi))
     bool use_graphs = true;
     int event_every = 1;
@@ -139,6 +148,8 @@ struct Engine {
                         int kind) const;
     void plan_extraction(ShotPlan &p, const std::vector<float> &vh, const sfd_shot &s, int kind) const;
     void build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeeds &need);
+    void reset_batch(Batch &B);
+    std::string plan_key(const char *kind, const sfd_shot *shots, int ns, int a, int b) const;
     void upload_shots(Batch &B);
     void prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float *d_model);
     void zero_wavefields(Batch &B, std::vector<ShotPlan> &plans);

@@ -170,6 +170,7 @@ void Engine::set_model(const float *vel, const float *vel_bwd) {
         if (!(vel[i] > 0.f) || !std::isfinite(vel[i])) throw EngineError(SFD_EINVAL, "velocity must be finite and > 0");
     vel_check(g, vel, n);
     vel_h.assign(vel, vel + n);
+    model_version++;
     if (!d_vel) HIPCHK(hipMalloc(&d_vel, n * sizeof(float)));
     HIPCHK(hipMemcpy(d_vel, vel, n * sizeof(float), hipMemcpyHostToDevice));
     if (vel_bwd) {
@@ -501,6 +502,7 @@ static size_t put(Layout &L, std::vector<std::pair<size_t, const void *>> &up,
 }
 
 void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeeds &need) {
+    B.key.clear();   // the device tables no longer hold the plan any key named
     const int ns = (int)plans.size();
     Layout L;
     std::vector<std::pair<size_t, const void *>> up;
@@ -605,6 +607,17 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         }
         if (L.off > z0) HIPCHK(hipMemsetAsync(base + z0, 0, L.off - z0, stream));
     }
+    // what a later call on the same tables must clear again: the levels and
+    // the accumulators (image, illumination, extracted traces)
+    B.zero_ranges.clear();
+    for (int s = 0; s < ns; s++) {
+        const Off &o = off[s];
+        const size_t nf = (size_t)plans[s].LDZ * plans[s].NX * sizeof(float);
+        for (int k = 0; k < kLevels; k++) B.zero_ranges.push_back({o.u[k], nf});
+        B.zero_ranges.push_back({o.image, o.illum - o.image});
+        B.zero_ranges.push_back({o.illum, o.born_m - o.illum});
+        B.zero_ranges.push_back({o.traces, o.snap - o.traces});
+    }
     for (size_t i = 0; i < up.size(); i++)
         HIPCHK(hipMemcpyAsync(base + up[i].first, up[i].second, upn[i], hipMemcpyHostToDevice, stream));
     B.shots_h.assign(ns, ShotDev());
@@ -676,6 +689,37 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     B.shots = (ShotDev *)(base + o_shots);
     B.ns = ns;
     upload_shots(B);
+    B.shots_built = B.shots_h;
+}
+
+// Back to the state build_batch left (zeroed levels, image, illumination
+// and extracted traces; the shot descriptors as built, e.g. the forward
+// injection set) for a call whose plan the device tables already hold.
+void Engine::reset_batch(Batch &B) {
+    char *base = (char *)B.mem.p;
+    for (const auto &r : B.zero_ranges) HIPCHK(hipMemsetAsync(base + r.first, 0, r.second, stream));
+    B.shots_h = B.shots_built;
+    upload_shots(B);
+}
+
+// Everything a plan depends on: the kind of call and its flags, the shots'
+// geometry and record lengths, the model (version), tile and strip settings.
+std::string Engine::plan_key(const char *kind, const sfd_shot *shots, int ns, int a, int b) const {
+    std::string k(kind);
+    auto add = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+    const int hdr[8] = {ns, a, b, tz, tx, strip_px, time_block, xcd_remap};
+    add(hdr, sizeof hdr);
+    add(&model_version, sizeof model_version);
+    for (int s = 0; s < ns; s++) {
+        const sfd_shot &sh = shots[s];
+        const int n2[2] = {sh.nt, sh.ntr};
+        add(n2, sizeof n2);
+        add(&sh.sz, sizeof sh.sz);
+        add(&sh.sx, sizeof sh.sx);
+        add(sh.rz, sizeof(float) * sh.ntr);
+        add(sh.rx, sizeof(float) * sh.ntr);
+    }
+    return k;
 }
 
 void Engine::upload_shots(Batch &B) {
@@ -1303,24 +1347,34 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
     }
     if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
     auto t0 = std::chrono::steady_clock::now();
-    std::vector<ShotPlan> plans(ns);
-    for (int s = 0; s < ns; s++) {
-        ShotPlan &p = plans[s];
-        plan_shot(shots[s], p);
-        plan_injection(p, vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
-        // backward injection: receivers as sources (Injection_sinc / Injection_LSRTM_sinc)
-        ShotPlan tmp = p;
-        plan_injection(tmp, velb_h.empty() ? vel_h : velb_h, shots[s].ntr, shots[s].rz, shots[s].rx, lsrtm ? 3 : 1);
-        p.take_backward(tmp);
-        p.n_inj_traces = std::max(1, shots[s].ntr);
-    }
     BatchNeeds need;
     need.backward_injection = true;
     need.snapshots = true;
     need.snap_fp16 = snap_mode == SFD_SNAP_FP16S;
     need.imaging = true;
     Batch &B = fwd_batch;
-    build_batch(B, plans, need);
+    // a repeated call with the same plan (same shots, model, flags and tile
+    // settings) reuses the tables already on the device
+    const std::string key = plan_key(lsrtm ? "lsrtm" : "rtm", shots, ns, snap_mode, 0);
+    if (plan_reuse && !B.key.empty() && B.key == key) {
+        reset_batch(B);
+    } else {
+        std::vector<ShotPlan> fresh(ns);
+        for (int s = 0; s < ns; s++) {
+            ShotPlan &p = fresh[s];
+            plan_shot(shots[s], p);
+            plan_injection(p, vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
+            // backward injection: receivers as sources (Injection_sinc / Injection_LSRTM_sinc)
+            ShotPlan tmp = p;
+            plan_injection(tmp, velb_h.empty() ? vel_h : velb_h, shots[s].ntr, shots[s].rz, shots[s].rx, lsrtm ? 3 : 1);
+            p.take_backward(tmp);
+            p.n_inj_traces = std::max(1, shots[s].ntr);
+        }
+        rtm_plans = std::move(fresh);
+        build_batch(B, rtm_plans, need);
+        B.key = key;
+    }
+    std::vector<ShotPlan> &plans = rtm_plans;
     // forward source (second-differenced for LSRTM, ReadSouVelRho.f90:318-326)
     for (int s = 0; s < ns; s++) {
         std::vector<float> w(shots[s].wavelet, shots[s].wavelet + shots[s].nt);
@@ -1840,6 +1894,7 @@ int sfd_set_option(sfd_handle h, int opt, int value) {
     return guarded(h, [&](Engine &e) {
         switch (opt) {
         case SFD_OPT_GRAPHS: e.use_graphs = value != 0; break;
+        case SFD_OPT_PLAN_REUSE: e.plan_reuse = value != 0; break;
         case SFD_OPT_GRAPH_CHUNK:
             if (value < 1) throw EngineError(SFD_EINVAL, "graph chunk must be >= 1");
             e.graph_chunk = value;

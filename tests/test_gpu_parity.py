@@ -369,3 +369,33 @@ def test_fwi_lbfgs_matches_oracle_driver():
     assert rel(xg, xo.reshape(nx, nz).T) <= 1e-5
     fa = rg["f_eval"][rg["accepted"]]
     assert fa[-1] < rg["f_eval"][0]
+
+
+def test_rtm_repeated_calls_reuse_the_plan():
+    """Repeated RTMs of one shot reuse the device tables (the plan is keyed by
+    the shots, the model and the tile settings): the images stay bitwise equal
+    to a fresh engine's, also after a new model, another kind of call or a
+    tile change in between, each of which plans anew."""
+    say("case: rtm plan reuse")
+    nz, nx, nt = 48, 72, 320
+    vel, w, rx, rz, data = _rtm_case(nz=nz, nx=nx, nt=nt, seed=6)
+    gs, os_ = both_shots(18.0, 355.0, w, rz, rx)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    first = e.rtm([gs], [data])
+    again = e.rtm([gs], [data])
+    assert all(np.array_equal(a, b) for a, b in zip(first, again))
+    vel2 = (vel * 1.03).astype(np.float32)
+    e.set_model(vel2)
+    want = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel2).rtm([gs], [data])
+    same = lambda got: all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert same(e.rtm([gs], [data]))
+    e.forward([gs])
+    assert same(e.rtm([gs], [data]))
+    e.lsrtm([gs], [data])
+    assert same(e.rtm([gs], [data]))
+    e.set_tile(64, 64)
+    assert same(e.rtm([gs], [data]))
+    e.set_option(sa.abi.SFD_OPT_PLAN_REUSE, 0)
+    assert same(e.rtm([gs], [data])) and same(e.rtm([gs], [data]))
+    out = O.rtm(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel2, os_, data)
+    assert rel(want[0], out["image"]) <= TOL and rel(want[1], out["illum"]) <= TOL
