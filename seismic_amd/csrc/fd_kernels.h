@@ -262,16 +262,18 @@ __device__ __forceinline__ void store_snap_point(const ShotDev &S, const StepArg
 
 // Imaging at one step of a reverse pass (Imaging.f90:228-275 RTM, :322-370
 // FWI, :698-745 LSRTM) on the u2 that the step tiles of the same launch read
-// and do not write: 256 threads take 256 rows (chunk gy) of kImgCols window
+// and do not write: 256 threads take 256 rows (chunk gy) of COLS window
 // columns (group gx) and issue all loads of their cells before any use.
 // These blocks follow the step tiles in the launch (mode 3), so they start
 // in the slots the first finished tiles free and the step tiles keep their
 // occupancy.  The FP16 snapshot scale is per step tile: 64 rows by
 // 1 << img_txs columns.
-constexpr int kImgCols = 4;
-template <bool FWI>
+// Columns per thread, measured on the 751x2301 RTM and the 96-shot batch:
+// two for the imaging blocks of a one-shot launch, four for image_kernel.
+constexpr int kImgColsSingle = 2, kImgColsBatch = 4;
+template <bool FWI, int COLS>
 __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S, int gx, int gy, int lane256) {
-    const int wx0 = gx * kImgCols, wz = gy * 256 + lane256;
+    const int wx0 = gx * COLS, wz = gy * 256 + lane256;
     if (wx0 >= S.nxw || wz >= S.nz) return;
     const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     int it = its;
@@ -286,10 +288,9 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
     const size_t npix = (size_t)S.nz * S.nxw;
     const GLB float *u2 = gp(S.u[A.rot]);
     const bool vel_needed = FWI || A.imaging == IMG_LSRTM;
-    float u[kImgCols], Sv[kImgCols], Sm[kImgCols], Sp[kImgCols], I0[kImgCols], L0[kImgCols], T[kImgCols],
-        V[kImgCols];
+    float u[COLS], Sv[COLS], Sm[COLS], Sp[COLS], I0[COLS], L0[COLS], T[COLS], V[COLS];
 #pragma unroll
-    for (int k = 0; k < kImgCols; k++) {
+    for (int k = 0; k < COLS; k++) {
         const int wx = wx0 + k < S.nxw ? wx0 + k : S.nxw - 1;   // clamped: loads stay in the window
         const int ix = wx + S.i0;
         const size_t q = (size_t)wx * S.nz + wz;
@@ -309,7 +310,7 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
         V[k] = vel_needed ? gp(S.vel)[a] : 0.f;
     }
 #pragma unroll
-    for (int k = 0; k < kImgCols; k++) {
+    for (int k = 0; k < COLS; k++) {
         if (wx0 + k >= S.nxw) break;
         const size_t q = (size_t)(wx0 + k) * S.nz + wz;
         gp(S.image)[q] = image_value<FWI>(A.imaging, I0[k], u[k], Sv[k], Sm[k], Sp[k], V[k], T[k], S.dt2snap);
@@ -322,7 +323,7 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
 // the batched variant's shot-table reads spill registers.
 template <bool FWI>
 __global__ void __launch_bounds__(256) image_kernel(StepArgs A) {
-    image_cells<FWI>(A, A.shots[blockIdx.z], (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x);
+    image_cells<FWI, kImgColsBatch>(A, A.shots[blockIdx.z], (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x);
 }
 
 // Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
@@ -820,9 +821,9 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : Ste
             if (s >= (SINGLE ? 1 : A.nshots)) return;
             const ShotDev &S = SINGLE ? A.s0 : A.shots[s];
             if (A.imaging == IMG_FWI)
-                image_cells<true>(A, S, gx, gy, (int)threadIdx.x & 255);
+                image_cells<true, kImgColsSingle>(A, S, gx, gy, (int)threadIdx.x & 255);
             else
-                image_cells<false>(A, S, gx, gy, (int)threadIdx.x & 255);
+                image_cells<false, kImgColsSingle>(A, S, gx, gy, (int)threadIdx.x & 255);
             return;
         }
     }

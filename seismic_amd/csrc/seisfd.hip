@@ -833,7 +833,8 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         if (tz != 64 || (tx & (tx - 1))) throw EngineError(SFD_EUNSUPPORTED, "imaging needs 64-row, 2^k-column tiles");
         int nxw = 1;
         for (int s = 0; s < B.ns; s++) nxw = std::max(nxw, B.shots_h[s].nxw);
-        A.img_gx = (nxw + kImgCols - 1) / kImgCols;
+        const int cols = A.single ? kImgColsSingle : kImgColsBatch;
+        A.img_gx = (nxw + cols - 1) / cols;
         A.img_gy = (g.nz + 255) / 256;
         A.nshots = B.ns;
         A.img_txs = 0;
