@@ -399,3 +399,34 @@ def test_rtm_repeated_calls_reuse_the_plan():
     assert same(e.rtm([gs], [data])) and same(e.rtm([gs], [data]))
     out = O.rtm(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel2, os_, data)
     assert rel(want[0], out["image"]) <= TOL and rel(want[1], out["illum"]) <= TOL
+
+
+@pytest.mark.parametrize("lsrtm", [False, True])
+def test_rtm_batch_of_ragged_shots(lsrtm):
+    """A batch of three shots with different record lengths and receiver
+    spreads (batched launches: lean step kernel plus the separate imaging
+    kernel): the images are the oracle's per-shot images summed in shot
+    order."""
+    say("case: %s batch of 3 ragged shots" % ("lsrtm" if lsrtm else "rtm"))
+    nz, nx = 44, 90
+    vel = smooth_random(nz, nx, seed=12)
+    rng = np.random.default_rng(12)
+    og = O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12)
+    e = engine(nz, nx, 10.0, 10.0, 1e-3, 12, vel=vel)
+    gsh, datas, img_ref, ill_ref = [], [], 0.0, 0.0
+    for sx, nt, nr in ((150.0, 300, 21), (455.0, 260, 33), (700.0, 330, 17)):
+        w = ricker(18.0, nt, 1e-3)
+        rx = np.linspace(max(10.0, sx - 200), min(10.0 * (nx - 2), sx + 200), nr).astype(np.float32)
+        rz = np.full_like(rx, 14.0)
+        a, b = both_shots(18.0, sx, w, rz, rx)
+        data = (rng.standard_normal((nr, nt)) * 1e-2).astype(np.float32)
+        out = O.rtm(og, vel, b, data, lsrtm=lsrtm)
+        img_ref = img_ref + out["image"]
+        ill_ref = ill_ref + out["illum"]
+        gsh.append(a)
+        datas.append(data)
+    img, ill = (e.lsrtm if lsrtm else e.rtm)(gsh, datas)
+    ei, el = rel(img, img_ref), rel(ill, ill_ref)
+    say("image rel=%.3e illum rel=%.3e bitwise=%s/%s" % (ei, el, np.array_equal(img, img_ref),
+                                                         np.array_equal(ill, ill_ref)))
+    assert ei <= TOL and el <= TOL
