@@ -588,7 +588,9 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (ADJ) hv[p] = ldw(v2p, cb + hzo);
         r1[p] = ldw(u1, cb + zoc);
         r0[p] = ldw(u0, cb + zoc);
-        rm[p] = ldw(um, cb + zoc);
+        // u_-1 is read by this step only (two steps on, its buffer takes a
+        // new level): a non-temporal load leaves the caches to the others
+        rm[p] = __builtin_nontemporal_load((const GLB float *)((const GLB char *)um + ((cb + zoc) << 2)));
         if (!ADJ) rv[p] = ldw(v2p, cb + zoc);
     }
     (void)hok;
