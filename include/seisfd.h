@@ -129,7 +129,8 @@ int  sfd_rtm(sfd_handle h, const sfd_shot *shots, int nshots,
 /*
  * Keep traces resident in HBM (no reference counterpart): count floats, the
  * shots' (nt, ntr) traces concatenated in shot order as sfd_rtm takes them.
- * sfd_rtm / sfd_lsrtm called with traces_in = NULL then read this copy
+ * sfd_rtm / sfd_lsrtm called with traces_in = NULL (sfd_fwi_gradient with
+ * dobs = NULL) then read this copy
  * instead of transferring host data each call (repeated imaging, LSRTM
  * iterations); the count must equal the call's sum of nt * ntr.  traces =
  * NULL or count = 0 releases the copy.
@@ -182,7 +183,8 @@ int  sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots,
  * then illumination normalisation, mask and triangle smoothing
  * (Smoothing.f90:87-227).  mute and vpmask may be NULL (all ones).
  * grad is the reference's `grad` (its sign convention: l_bfgs passes -grad
- * to LBFGS, Inversion.f90:128).  resid_out may be NULL.
+ * to LBFGS, Inversion.f90:128).  resid_out may be NULL.  dobs may be NULL:
+ * the observed traces are then the copy kept in HBM by sfd_set_traces.
  */
 int  sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots,
                       const float *dobs, const float *mute, const float *vpmask,
@@ -218,15 +220,34 @@ int  sfd_get_tile(sfd_handle h, int *tile_z, int *tile_x, int *strip_px, int *ti
 #define SFD_OPT_TIME_BLOCK   6  /* time steps per launch: 1 (default) or 2 (64x32 tiles; not Born passes) */
 #define SFD_OPT_PLAN_REUSE   7  /* 1: an RTM/LSRTM call with the same shots, model and tile settings as
                                    the previous call reuses its plan tables on the device (default) */
+#define SFD_OPT_BATCH_MIB    8  /* device memory (MiB) one wave of shots may use; 0 (default): the
+                                   memory free on the device.  Calls whose shots do not fit run them
+                                   in consecutive waves (shot order), with results equal to one batch */
 int  sfd_set_option(sfd_handle h, int opt, int value);
+
+/* Shot waves the last modeling call ran in (1: one batch).  The reference
+ * runs shots one per OpenMP thread (compute_function_gradient.f90:246-256);
+ * a wave is the batch of consecutive shots that fits in device memory. */
+int  sfd_last_waves(sfd_handle h, int *waves);
 
 /*
  * Step-kernel durations of the last call, from HIP events recorded on the
  * engine's stream around every block of back-to-back step launches (block
  * time / steps in the block): avg_us[0] forward-time pass, avg_us[1]
- * reverse-time pass; nsamples = steps timed.
+ * reverse-time pass; nsamples = steps timed (every step of the pass).
  */
 int  sfd_kernel_stats(sfd_handle h, double *avg_us, int *nsamples);
+
+/*
+ * Totals of the same event-timed launches, per pass ([0] forward time, [1]
+ * reverse time): us = kernel microseconds, bytes = algorithmic HBM bytes (24
+ * per FD point-update, plus the snapshot write on forward snapshot steps and
+ * the snapshot read(s) and 16 B of image and illumination read-modify-write
+ * per window cell on reverse imaging steps), pts = FD point-updates, steps.
+ * bytes / us is the achieved bandwidth of the pass's launches.  No reference
+ * counterpart (the reference's per-phase timers, Propagator.f90:165-258).
+ */
+int  sfd_kernel_totals(sfd_handle h, double *us, double *bytes, double *pts, int *steps);
 
 /*
  * Measurement helper (no reference counterpart): copies nbytes between two

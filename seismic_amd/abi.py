@@ -34,6 +34,7 @@ SFD_OPT_KERNEL_EVENTS = 4
 SFD_OPT_STRIP_PX = 5
 SFD_OPT_TIME_BLOCK = 6
 SFD_OPT_PLAN_REUSE = 7
+SFD_OPT_BATCH_MIB = 8
 
 
 class sfd_grid(C.Structure):
@@ -114,8 +115,8 @@ EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
     "sfd_born_adjoint_exact", "sfd_lsrtm",
-    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_kernel_stats",
-    "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
+    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
+    "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgsoc_step", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
@@ -157,6 +158,8 @@ def load():
     lib.sfd_get_tile.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]
     lib.sfd_set_option.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.sfd_kernel_stats.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int)]
+    lib.sfd_last_waves.argtypes = [C.c_void_p, P(C.c_int)]
+    lib.sfd_kernel_totals.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_int)]
     lib.sfd_copy_calibration.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]
     lib.sfd_lbfgs_create.argtypes = [P(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_void_p, C.c_int, C.c_int, C.c_void_p]

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
 #include <random>
 #include <string>
 #include <vector>
@@ -359,6 +360,7 @@ LbfgsHooks *make_cnn_precond(int nz, int nx, const sfd_precond_opts &o) { return
 // ---------------------------------------------------------------------------
 struct sfd_cnn {
     Cnn *net = nullptr;
+    ~sfd_cnn() { delete net; }
 };
 
 extern "C" {
@@ -366,46 +368,36 @@ extern "C" {
 int sfd_cnn_create(sfd_cnn_handle *c, int nz, int nx, int nimg_max, int seed) {
     if (!c) return SFD_EINVAL;
     *c = nullptr;
-    try {
-        sfd_cnn *h = new sfd_cnn();
+    return abi_call(nullptr, [&]() {
+        std::unique_ptr<sfd_cnn> h(new sfd_cnn());
         h->net = new Cnn(nz, nx, nimg_max, seed);
-        *c = h;
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+        *c = h.release();
+    });
 }
 
 int sfd_cnn_destroy(sfd_cnn_handle c) {
     if (!c) return SFD_EINVAL;
-    delete c->net;
     delete c;
     return SFD_OK;
 }
 
 int sfd_cnn_get_params(sfd_cnn_handle c, float *theta) {
     if (!c || !theta) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         c->net->get_params(theta);
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 int sfd_cnn_set_params(sfd_cnn_handle c, const float *theta) {
     if (!c || !theta) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         c->net->set_params(theta);
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 int sfd_cnn_forward(sfd_cnn_handle c, int nimg, const float *x, float *y) {
     if (!c || !x || !y) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         Cnn &n = *c->net;
         n.check_nimg(nimg);
         const long NP = n.P * nimg;
@@ -414,15 +406,12 @@ int sfd_cnn_forward(sfd_cnn_handle c, int nimg, const float *x, float *y) {
         n.forward(nimg);
         HIPCHK(hipMemcpyAsync(y, n.y, NP * sizeof(float), hipMemcpyDeviceToHost, n.st));
         HIPCHK(hipStreamSynchronize(n.st));
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 int sfd_cnn_loss_grad(sfd_cnn_handle c, int nimg, const float *x, const float *target, double *loss, float *grad) {
     if (!c || !x || !target) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         Cnn &n = *c->net;
         n.check_nimg(nimg);
         const long NP = n.P * nimg;
@@ -433,16 +422,13 @@ int sfd_cnn_loss_grad(sfd_cnn_handle c, int nimg, const float *x, const float *t
         const double l = n.backward(nimg);
         if (loss) *loss = l;
         if (grad) HIPCHK(hipMemcpy(grad, n.grad, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 int sfd_cnn_train(sfd_cnn_handle c, int nimg, const float *x, const float *target, int steps, float lr,
                   double *loss_first, double *loss_last) {
     if (!c || !x || !target || steps < 0) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         Cnn &n = *c->net;
         n.check_nimg(nimg);
         const long NP = n.P * nimg;
@@ -450,10 +436,7 @@ int sfd_cnn_train(sfd_cnn_handle c, int nimg, const float *x, const float *targe
         HIPCHK(hipMemcpyAsync(n.t32, target, NP * sizeof(float), hipMemcpyHostToDevice, n.st));
         n.train(nimg, steps, lr, loss_first, loss_last);
         HIPCHK(hipStreamSynchronize(n.st));
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 }  // extern "C"

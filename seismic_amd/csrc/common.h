@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <new>
+#include <stdexcept>
 #include <string>
 
 #include "../../include/seisfd.h"
@@ -34,3 +36,25 @@ struct Layout {
     }
 };
 
+
+// Runs f for a C entry point so that no exception crosses the ABI (a Fortran
+// or Python host would terminate): engine errors keep their code, failed host
+// allocations are SFD_ENOMEM, anything else SFD_EINVAL.  err (optional)
+// receives the message, cleared on success.
+template <class F>
+static int abi_call(std::string *err, F &&f) {
+    try {
+        f();
+        if (err) err->clear();
+        return SFD_OK;
+    } catch (const EngineError &e) {
+        if (err) *err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (err) *err = "host allocation failed";
+        return SFD_ENOMEM;
+    } catch (const std::exception &e) {
+        if (err) *err = e.what();
+        return SFD_EINVAL;
+    }
+}

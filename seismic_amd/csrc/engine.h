@@ -74,6 +74,8 @@ struct BatchNeeds {
 
 struct Batch {
     DevBuf mem;
+    char *ext_base = nullptr;   // when set: the batch lives here (ext_cap bytes), mem is not used
+    size_t ext_cap = 0;
     std::vector<RegionPlan> rp, rpb;   // two-step kernel region plans (forward / backward injections)
     int ns = 0, nblocks = 0;
     std::vector<sfd::ShotDev> shots_h;
@@ -82,6 +84,14 @@ struct Batch {
     std::vector<int> blk_shot_h;
     sfd::ShotDev *shots = nullptr;
     const int *blk_shot = nullptr;
+    // Tiles are laid out shot by shot in order of decreasing record length
+    // (order[k] = the shot in place k, nt_sorted[k] its nt, tiles_upto[k] the
+    // tiles of places 0..k-1), so the shots still running at any step form a
+    // prefix of the tiles: a launch covers only those (ragged batches).
+    std::vector<int> order, nt_sorted, tiles_upto;
+    const int *order_d = nullptr;
+    int active_shots(int n) const;   // shots with nt > n (step n is in their record)
+    int active_tiles(int n) const { return tiles_upto[active_shots(n)]; }
     // Reuse by a later call with the same plan (empty: none): the key of the
     // plan the device tables hold, the shot descriptors as built and the
     // byte ranges zeroed at build time (everything but the snapshots).
@@ -114,14 +124,20 @@ struct Engine {
     int event_every = 1;
     std::vector<hipEvent_t> kev;          // start/stop of every block of steps
     DevBuf nbase;                         // device-held first step of the running block
+    // per pass (0 forward time, 1 reverse time): event-timed microseconds,
+    // steps, FD point-updates and algorithmic HBM bytes of the timed launches
     double kstat_us[2] = {0, 0};
     int kstat_n[2] = {0, 0};
+    double kstat_pts[2] = {0, 0}, kstat_bytes[2] = {0, 0};
     double timing[4] = {0, 0, 0, 0};
     double pts = 0;
     float *raw_illum_host = nullptr;      // fwi_gradient: also copy the summed illumination here
     bool tb_ok = true;                    // the current batch's region plans fit the two-step kernel
     Batch fwd_batch;
-    DevBuf scratch, fwi_mem;
+    bool eager_untimed = false;           // no graphs, events or device step base (caller-memory batches)
+    DevBuf tmp_mem, fwi_mem, acc_mem;
+    size_t batch_bytes_cap = 0;           // SFD_OPT_BATCH_BYTES (0: free device memory decides)
+    int waves_last = 0;                   // shot waves of the last call
     HostBuf img_host;
     DevBuf resident;                      // traces kept in HBM by sfd_set_traces
     size_t resident_n = 0;
@@ -130,7 +146,7 @@ struct Engine {
     explicit Engine(const sfd_grid &grid);
     ~Engine();
     void set_model(const float *vel, const float *vel_bwd);
-    void forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind);
+    void forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind, Batch *ext = nullptr);
     void rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
              bool lsrtm);
     void born(const sfd_shot *shots, int ns, const float *refl, float *traces_out);
@@ -140,7 +156,7 @@ struct Engine {
                       const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
 
     static bool tile_supported(int TZ, int TX);
-    void check_shots(const sfd_shot *shots, int nshots, int snaps = -1);
+    void check_shots(const sfd_shot *shots, int nshots, int snaps = -1, bool may_tune = true);
     void autotune(const sfd_shot *shots, int ns, int snaps);
     float padded_vel(const std::vector<float> &vh, const ShotPlan &p, int icz, int icx) const;
     void plan_shot(const sfd_shot &s, ShotPlan &p) const;
@@ -155,13 +171,15 @@ struct Engine {
     void zero_wavefields(Batch &B, std::vector<ShotPlan> &plans);
     sfd::StepArgs base_args(const Batch &B) const;
     void launch_step(const Batch &B, sfd::StepArgs A, bool adj);
-    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps, int nfree);
+    void run_loop(const Batch &B, sfd::StepArgs A, bool adj, int nsteps);
     void launch_step2(const Batch &B, sfd::StepArgs A, bool adj);
+    double step_bytes(const Batch &B, const sfd::StepArgs &A, int n, bool two) const;
     bool use_step2(const sfd::StepArgs &A) const;
     void reset_kstats();
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);
     void traces_time_major(const Batch &B);
-    void gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image, float *illum);
-    void scatter_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill);
+    void accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill);
+    size_t shot_bytes(const sfd_shot &s, const BatchNeeds &need) const;
+    std::vector<std::pair<int, int>> plan_waves(const sfd_shot *shots, int ns, const BatchNeeds &need);
 };

@@ -122,7 +122,9 @@ struct StepArgs {
     int single;
     const ShotDev *shots;
     const int *blk_shot;    // logical tile -> shot
-    int nblocks;            // logical tiles (grid may be padded to a multiple of 8)
+    const int *shot_order;  // shots by decreasing record length (image_kernel's z index -> shot)
+    int nblocks;            // logical tiles of this launch: those of the shots still running,
+                            // a prefix of the layout (grid may be padded to a multiple of 8)
     int xcd_remap;          // map block ids so one XCD gets contiguous tiles
     int step_grid;          // blocks of the launch that step tiles (the rest image, mode 3)
     int img_gx, img_gy;     // mode 3: column groups and 256-row chunks of the largest window
@@ -319,11 +321,13 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
 }
 
 // Batched launches image in a kernel of their own beside the lean step
-// (grid: column groups, 256-row chunks, shots): folded into the step launch,
-// the batched variant's shot-table reads spill registers.
+// (grid: column groups, 256-row chunks, the shots still running in order of
+// decreasing record length): folded into the step launch, the batched
+// variant's shot-table reads spill registers.
 template <bool FWI>
 __global__ void __launch_bounds__(256) image_kernel(StepArgs A) {
-    image_cells<FWI, kImgColsBatch>(A, A.shots[blockIdx.z], (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x);
+    image_cells<FWI, kImgColsBatch>(A, A.shots[gp(A.shot_order)[blockIdx.z]], (int)blockIdx.x, (int)blockIdx.y,
+                                    (int)threadIdx.x);
 }
 
 // Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
@@ -911,6 +915,20 @@ __global__ void higdon_tables_kernel(const float *__restrict__ vel, int NZ, int 
     }
 }
 
+// Zero n bytes at p: a grid-stride fill with 16-byte vector stores over the
+// aligned middle and byte stores for the unaligned head and tail, used where
+// a memset node must not be recorded.
+__global__ void fill_zero_kernel(char *__restrict__ p, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x, t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t head = (16 - ((uintptr_t)p & 15)) & 15;
+    if (head > n) head = n;
+    const size_t n16 = (n - head) / 16, tail = head + 16 * n16;
+    for (size_t i = t0; i < head; i += stride) p[i] = 0;
+    float4 *q = (float4 *)(p + head);
+    for (size_t i = t0; i < n16; i += stride) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (size_t i = tail + t0; i < n; i += stride) p[i] = 0;
+}
+
 // Step base of replayed graphs (one thread; plain vector stores).
 __global__ void set_base_kernel(int *nbase, int v) { *nbase = v; }
 
@@ -934,26 +952,24 @@ __global__ void advance_base_kernel(int *nbase, int k) { *nbase = *nbase + k; }
 
 // mod_copy_image (ExtractPadModel.f90:662-695): nearest-index scatter-add of
 // every shot's window image into the full model.  One thread per model cell;
-// shots are summed in shot order, so the result is deterministic.
+// for model column ix, inv_shot / inv_col [inv_ptr[ix], inv_ptr[ix+1]) list
+// the (shot, window column) pairs that map to it in shot order, so the sum
+// runs in shot order and starts from what `out` holds (calls for successive
+// shot waves add in the same order as one call for all of them).
 struct CopyImg {
     const float *img;       // nz*nxw
     const int *colmap;      // window column -> model column or -1
     int nxw;
 };
-__global__ void copy_image_kernel(const CopyImg *__restrict__ shots, int nshots, int nz, int nx,
-                                  const int *__restrict__ inv_ptr, const int *__restrict__ inv_idx,
+__global__ void copy_image_kernel(const CopyImg *__restrict__ shots, int nz, int nx, const int *__restrict__ inv_ptr,
+                                  const int *__restrict__ inv_shot, const int *__restrict__ inv_col,
                                   float *__restrict__ out) {
-    // inv_ptr/inv_idx: for model column ix, the (shot, window column) pairs
-    // that map to it, in shot order, packed as shot<<20 | wcol.
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
     const int ix = blockIdx.y;
     if (iz >= nz || ix >= nx) return;
     float acc = out[(size_t)ix * nz + iz];
-    for (int k = inv_ptr[ix]; k < inv_ptr[ix + 1]; k++) {
-        const int e = inv_idx[k];
-        const CopyImg &c = shots[e >> 20];
-        acc = acc + c.img[(size_t)(e & 0xfffff) * nz + iz];
-    }
+    for (int k = inv_ptr[ix]; k < inv_ptr[ix + 1]; k++)
+        acc = acc + shots[inv_shot[k]].img[(size_t)inv_col[k] * nz + iz];
     out[(size_t)ix * nz + iz] = acc;
 }
 

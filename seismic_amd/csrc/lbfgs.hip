@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "../../include/seisfd.h"
@@ -23,6 +24,15 @@ LbfgsDev::LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mas
                    const double *xinit_h)
     : n(n_), m(m_), xmin(xmin_), xmax(xmax_), boundt(boundt_), type(type_) {
     if (n < 1 || m < 1) throw EngineError(SFD_EINVAL, "L-BFGS needs n >= 1 and m >= 1");
+    try {
+        init(mask_h, xinit_h);
+    } catch (...) {
+        release();   // a constructor that throws runs no destructor
+        throw;
+    }
+}
+
+void LbfgsDev::init(const float *mask_h, const double *xinit_h) {
     HIPCHK(hipStreamCreate(&st));
     const size_t b = (size_t)n * sizeof(double);
     HIPCHK(hipMalloc(&x, b));
@@ -48,12 +58,17 @@ LbfgsDev::LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mas
     HIPCHK(hipStreamSynchronize(st));
 }
 
-LbfgsDev::~LbfgsDev() {
+LbfgsDev::~LbfgsDev() { release(); }
+
+void LbfgsDev::release() {
     for (void *p : {(void *)x, (void *)g, (void *)q, (void *)diag, (void *)xinit, (void *)S, (void *)Y,
                     (void *)part, (void *)res, (void *)mask})
         if (p) (void)hipFree(p);
     if (res_h) (void)hipHostFree(res_h);
     if (st) (void)hipStreamDestroy(st);
+    x = g = q = diag = xinit = S = Y = part = res = res_h = nullptr;
+    mask = nullptr;
+    st = nullptr;
 }
 
 double LbfgsDev::dot(const double *a, const double *b) {
@@ -638,6 +653,7 @@ int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, co
 struct sfd_lbfgs {
     LbfgsDev *d = nullptr;
     std::string err;
+    ~sfd_lbfgs() { delete d; }
 };
 
 extern "C" {
@@ -646,43 +662,33 @@ int sfd_lbfgs_create(sfd_lbfgs_handle *lb, int n, int m, double xmin, double xma
                      int freeze_soft, const double *xinit) {
     if (!lb) return SFD_EINVAL;
     *lb = nullptr;
-    try {
-        sfd_lbfgs *h = new sfd_lbfgs();
+    return abi_call(nullptr, [&]() {
+        std::unique_ptr<sfd_lbfgs> h(new sfd_lbfgs());
         h->d = new LbfgsDev(n, m, xmin, xmax, mask, boundt, freeze_soft, xinit);
-        *lb = h;
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+        *lb = h.release();
+    });
 }
 
 int sfd_lbfgs_step(sfd_lbfgs_handle lb, double *x, double f, const double *g, int *iflag, int *info) {
     if (!lb || !lb->d || !x || !g || !iflag || !info) return SFD_EINVAL;
     LbfgsDev &d = *lb->d;
-    try {
+    return abi_call(&lb->err, [&]() {
         const size_t b = (size_t)d.n * sizeof(double);
         HIPCHK(hipMemcpyAsync(d.x, x, b, hipMemcpyHostToDevice, d.st));
         HIPCHK(hipMemcpyAsync(d.g, g, b, hipMemcpyHostToDevice, d.st));
         d.step(f, iflag, info, 1.0e-20, 2.220446049250313e-16);   // EPS, XTOL of Inversion.f90:79-80
         HIPCHK(hipMemcpyAsync(x, d.x, b, hipMemcpyDeviceToHost, d.st));
         HIPCHK(hipStreamSynchronize(d.st));
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        lb->err = e.msg;
-        return e.code;
-    }
+    });
 }
 
 int sfd_lbfgs_run(const sfd_lbfgs_opts *o, int n, float *x, const float *vpmask, const double *vpinit,
                   sfd_fctgdt fct, void *ctx, sfd_lbfgs_report *rep) {
     if (!o || n < 1 || !x || !fct) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         LbfgsFct f = [&](const float *xx, float *g, double *fv) { return fct(ctx, xx, g, fv); };
         lbfgs_drive(*o, n, x, vpmask, vpinit, f, rep, nullptr);
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 int sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfgsoc_opts *o, int n, double *x,
@@ -692,7 +698,7 @@ int sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfg
         return SFD_EINVAL;
     if (*iflag != 0 && *iflag != 1) return SFD_EINVAL;   // DIAGCO = .FALSE.: no IFLAG 2 re-entry
     if (*iflag == 1 && !state_in) return SFD_EINVAL;
-    try {
+    return abi_call(nullptr, [&]() {
         LbfgsDev d(n, o->m, 0.0, 0.0, nullptr, o->clip_type, 1, nullptr);
         d.nparam = o->nparam;
         d.pxmin.assign(o->xmin, o->xmin + o->nparam);
@@ -708,15 +714,11 @@ int sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfg
         HIPCHK(hipMemcpyAsync(x, d.x, b, hipMemcpyDeviceToHost, d.st));
         HIPCHK(hipStreamSynchronize(d.st));
         if (*iflag >= 0) d.save_state(state_out);
-        return SFD_OK;
-    } catch (const EngineError &e) {
-        return e.code;
-    }
+    });
 }
 
 int sfd_lbfgs_destroy(sfd_lbfgs_handle lb) {
     if (!lb) return SFD_EINVAL;
-    delete lb->d;
     delete lb;
     return SFD_OK;
 }

@@ -52,6 +52,8 @@ struct LbfgsDev {
     LbfgsDev(long n_, int m_, double xmin_, double xmax_, const float *mask_h, int boundt_, int type_,
              const double *xinit_h);
     ~LbfgsDev();
+    void init(const float *mask_h, const double *xinit_h);
+    void release();
     double dot(const double *a, const double *b);
     void axpby(double a, const double *xv, double b, double *y);
     void scale_copy(double a, const double *xv, double *y);

@@ -585,27 +585,56 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         o.image = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.illum = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.born_m = L.take(need.born ? npix * sizeof(float) : 16);
+    }
+    // tiles shot by shot, longest record first (stable in shot order)
+    B.order.resize(ns);
+    for (int s = 0; s < ns; s++) B.order[s] = s;
+    std::stable_sort(B.order.begin(), B.order.end(), [&](int a, int b) { return plans[a].nt > plans[b].nt; });
+    B.nt_sorted.assign(ns, 0);
+    B.tiles_upto.assign(ns + 1, 0);
+    for (int k = 0; k < ns; k++) {
+        ShotPlan &p = plans[B.order[k]];
         p.tile0 = tile0;
         const int nt_ = p.ntz * p.ntx;
-        for (int k = 0; k < nt_; k++) B.blk_shot_h.push_back(s);
+        for (int q = 0; q < nt_; q++) B.blk_shot_h.push_back(B.order[k]);
         tile0 += nt_;
+        B.nt_sorted[k] = p.nt;
+        B.tiles_upto[k + 1] = tile0;
     }
     B.nblocks = tile0;
     const size_t o_blk = put(L, up, upn, B.blk_shot_h);
+    const size_t o_order = put(L, up, upn, B.order);
     const size_t o_shots = L.take(sizeof(ShotDev) * ns);
-    B.mem.reserve(L.off);
-    char *base = (char *)B.mem.p;
+    char *base;
+    if (B.ext_base) {   // caller-provided memory (the allocation-free entry point)
+        if (L.off > B.ext_cap)
+            throw EngineError(SFD_ENOMEM, "the batch needs " + std::to_string(L.off) + " bytes of device memory, " +
+                                              "more than the " + std::to_string(B.ext_cap) + " the scratch array holds");
+        base = B.ext_base;
+    } else {
+        B.mem.reserve(L.off);
+        base = (char *)B.mem.p;
+    }
     // zero everything once (wavefields, accumulators, traces) except the
     // snapshots, then upload plans.  A pass that stores snapshots writes
     // every slot 1..ntsnap before any step reads it, so they need no fill
     // (on a 751x2301 single-shot RTM that fill was 13 GB).
+    auto zero = [&](size_t o, size_t n) {
+        if (B.ext_base) {   // a fill kernel, not a memset node, for callers that capture
+            hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((n / 16 + 255) / 256 + 1, 8192)),
+                               dim3(256), 0, stream, base + o, n);
+            HIPCHK(hipGetLastError());
+        } else {
+            HIPCHK(hipMemsetAsync(base + o, 0, n, stream));
+        }
+    };
     {
         size_t z0 = 0;
         for (int s = 0; s < ns; s++) {
-            if (off[s].snap > z0) HIPCHK(hipMemsetAsync(base + z0, 0, off[s].snap - z0, stream));
+            if (off[s].snap > z0) zero(z0, off[s].snap - z0);
             z0 = off[s].snap + off[s].snap_bytes;
         }
-        if (L.off > z0) HIPCHK(hipMemsetAsync(base + z0, 0, L.off - z0, stream));
+        if (L.off > z0) zero(z0, L.off - z0);
     }
     // what a later call on the same tables must clear again: the levels and
     // the accumulators (image, illumination, extracted traces)
@@ -686,6 +715,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         B.colsrc_d.push_back((const int *)(base + o.colsrc));
     }
     B.blk_shot = (const int *)(base + o_blk);
+    B.order_d = (const int *)(base + o_order);
     B.shots = (ShotDev *)(base + o_shots);
     B.ns = ns;
     upload_shots(B);
@@ -722,6 +752,11 @@ std::string Engine::plan_key(const char *kind, const sfd_shot *shots, int ns, in
     return k;
 }
 
+int Batch::active_shots(int n) const {
+    // nt_sorted is non-increasing: the first place with nt <= n ends the prefix
+    return (int)(std::lower_bound(nt_sorted.begin(), nt_sorted.end(), n, std::greater<int>()) - nt_sorted.begin());
+}
+
 void Engine::upload_shots(Batch &B) {
     HIPCHK(hipMemcpyAsync((void *)B.shots, B.shots_h.data(), sizeof(ShotDev) * B.ns, hipMemcpyHostToDevice, stream));
 }
@@ -753,6 +788,7 @@ StepArgs Engine::base_args(const Batch &B) const {
     memset(&A, 0, sizeof A);
     A.shots = B.shots;
     A.blk_shot = B.blk_shot;
+    A.shot_order = B.order_d;
     A.nblocks = B.nblocks;
     A.xcd_remap = xcd_remap;
     A.snapi = g.snapi;
@@ -807,8 +843,9 @@ bool Engine::tile_supported(int TZ, int TX) { return pick_kernel(TZ, TX, false, 
 // (RTM, LSRTM, FWI) launches mode 3 (the lean step plus imaging blocks).
 static int snap_launch(const Batch &B, const StepArgs &A) {
     if (!A.store_snap && A.imaging == IMG_NONE) return 0;
-    for (int s = 0; s < B.ns; s++) {
-        const int its = A.dir > 0 ? A.n + 1 : B.shots_h[s].nt - A.n;
+    const int na = B.active_shots(A.n);
+    for (int k = 0; k < na; k++) {
+        const int its = A.dir > 0 ? A.n + 1 : B.nt_sorted[k] - A.n;
         if (its % A.snapi == 0) return A.imaging != IMG_NONE ? 2 : 1;
     }
     return 0;
@@ -824,7 +861,10 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     const bool image = mode == 2 && A.imaging != IMG_BORN;
     StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? (A.single ? 3 : 0) : mode, strip_px);
     if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
-    int grid = B.nblocks;
+    // the shots still running at step n: a prefix of the tile layout
+    const int nact = B.active_shots(A.n);
+    A.nblocks = B.tiles_upto[nact];
+    int grid = A.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
     A.step_grid = grid;
     const int px = strip_px ? strip_px : pick_px(tz, tx);
@@ -846,7 +886,7 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     }
     hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
     if (image && !A.single) {
-        const dim3 ig(A.img_gx, A.img_gy, B.ns);
+        const dim3 ig(A.img_gx, A.img_gy, nact);
         if (A.imaging == IMG_FWI)
             hipLaunchKernelGGL(image_kernel<true>, ig, dim3(256), 0, stream, A);
         else
@@ -865,7 +905,9 @@ void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
         A.single = 1;
         A.s0 = B.shots_h[0];
     }
-    int grid = B.nblocks;
+    // both steps of the launch lie in the records of the same shots (run_loop)
+    A.nblocks = B.active_tiles(A.n);
+    int grid = A.nblocks;
     if (A.xcd_remap) grid = ((grid + 7) / 8) * 8;
     A.step_grid = grid;
     typedef void (*Fn)(StepArgs);
@@ -874,14 +916,6 @@ void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
     hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads2), 0, stream, A);
 }
 
-// Run nsteps of the lockstep time loop.  Steps n < nfree (every shot of the
-// batch still running) go in blocks of K steps, K = graph_chunk rounded up to
-// a multiple of lcm(kLevels, snapi) so that buffer rotation and snapshot phases
-// repeat from block to block: the K step launches plus one base-advance launch
-// are captured once into a HIP graph and replayed, the kernels reading the
-// block's first step from device memory (d_nbase).  Remaining steps launch one
-// by one with a fixed n.  HIP events on the engine stream bracket every block;
-// block time / K is the per-launch step time reported by kernel_stats.
 // The step kernel reads injected traces time-major (one row per step): copy
 // each shot's traces (as uploaded or as the residual kernel wrote them).
 void Engine::traces_time_major(const Batch &B) {
@@ -893,16 +927,85 @@ void Engine::traces_time_major(const Batch &B) {
     HIPCHK(hipGetLastError());
 }
 
-void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfree) {
+// Ends a stream capture that an exception interrupts, so the stream is
+// usable again and the partial graph is freed.
+struct CaptureGuard {
+    hipStream_t s = nullptr;
+    bool on = false;
+    void begin(hipStream_t st) {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        s = st;
+        on = true;
+    }
+    hipGraph_t end() {
+        hipGraph_t gr = nullptr;
+        on = false;
+        HIPCHK(hipStreamEndCapture(s, &gr));
+        return gr;
+    }
+    ~CaptureGuard() {
+        if (!on) return;
+        hipGraph_t gr = nullptr;
+        (void)hipStreamEndCapture(s, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+    }
+};
+
+// Algorithmic HBM bytes of step n (two-step launch: steps n and n + 1) over
+// the shots running then: 24 B per FD point-update (read U_n, U_{n-1},
+// U_{n-2}, U_{n-3}, v^2; write U_{n+1}), plus on a shot's snapshot steps the
+// snapshot write (forward) or the snapshot read(s) and the image and
+// illumination read-modify-write (reverse-time imaging), per window cell.
+double Engine::step_bytes(const Batch &B, const StepArgs &A, int n, bool two) const {
+    double b = 0;
+    const double sb = A.snap_fp16 ? 2.0 : 4.0;
+    for (int m = n; m < n + (two ? 2 : 1); m++) {
+        const int na = B.active_shots(m);
+        for (int k = 0; k < na; k++) {
+            const ShotDev &S = B.shots_h[B.order[k]];
+            b += 24.0 * (double)(S.NZ - 8) * (double)(S.NX - 8);
+            const int its = A.dir > 0 ? m + 1 : S.nt - m;
+            if (its % A.snapi) continue;
+            const double npix = (double)S.nz * (double)S.nxw;
+            if (A.store_snap) b += sb * npix;
+            if (A.imaging == IMG_BORN) b += (sb + 8.0) * npix;   // snapshot, reflectivity, velocity
+            else if (A.imaging != IMG_NONE) b += ((A.imaging == IMG_FWI ? 3.0 : 1.0) * sb + 16.0) * npix;
+        }
+    }
+    return b;
+}
+
+// Run steps 0 .. nsteps-1 of the lockstep time loop over a batch.  The shots
+// still running at step n (nt > n) are a prefix of the tile layout, and every
+// launch covers exactly their tiles.  Steps n < min nt (every shot running)
+// go in blocks of K steps, K = graph_chunk rounded up to a multiple of
+// lcm(kLevels, snapi) so that buffer rotation and snapshot phases repeat from
+// block to block: the K step launches plus one base-advance launch are
+// captured once into a HIP graph and replayed, the kernels reading the
+// block's first step from device memory (d_nbase).  The remaining steps (the
+// ragged tail, where shots finish one after another and launches shrink)
+// are captured in chunks of at most K steps with their step index and grid
+// fixed, each chunk one graph launch, captured while the GPU runs the
+// previous ones.  HIP events on the engine stream bracket every block and
+// chunk: kernel_stats covers every step of the pass.
+void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps) {
+    if (nsteps < 1) return;
     if (A.inject) traces_time_major(B);
     const int dir = A.dir > 0 ? 0 : 1;
+    // the step kernel must exist before any capture starts
+    if (!pick_kernel(tz, tx, adj, B.ns == 1, 0, strip_px)) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
     int period = kLevels;
     while (period % g.snapi) period += kLevels;
     const int K = std::max(period, (std::max(1, graph_chunk) + period - 1) / period * period);
-    const int nblk = std::min(nsteps, nfree) / K;
-    nbase.reserve(256);
-    int *d_nb = (int *)nbase.p;
-    const bool two = use_step2(A);   // two steps per launch where both lie in every shot's record
+    const int nfree = std::min(nsteps, B.nt_sorted.back());   // steps with every shot running
+    const int nblk = nfree / K;
+    const bool graphs_on = use_graphs && !eager_untimed;
+    int *d_nb = nullptr;
+    if (graphs_on && nblk > 0) {
+        nbase.reserve(256);
+        d_nb = (int *)nbase.p;
+    }
+    const bool two = use_step2(A);   // two steps per launch where both lie in the same shots' records
     auto one = [&](int n, const int *nb) {
         A.n = n;
         A.rot = n % kLevels;
@@ -916,72 +1019,132 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps, int nfre
         launch_step2(B, A, adj);
     };
     const int inc = two ? 2 : 1;
-    const bool timed = event_every > 0;
-    if (timed && (int)kev.size() < 2 * nblk) {
+    // the tail in chunks [n0, n1) of at most K steps
+    std::vector<std::pair<int, int>> chunks;
+    for (int n0 = nblk * K; n0 < nsteps; n0 += K) chunks.push_back({n0, std::min(nsteps, n0 + K)});
+    const bool timed = event_every > 0 && !eager_untimed;
+    const int nspan = nblk + (int)chunks.size();
+    if (timed && (int)kev.size() < 2 * nspan) {
         const size_t old = kev.size();
-        kev.resize(2 * nblk);
+        kev.resize(2 * nspan);
         for (size_t i = old; i < kev.size(); i++) HIPCHK(hipEventCreate(&kev[i]));
     }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<double> span_bytes, span_pts;
+    std::vector<int> span_steps;
+    auto pts_of = [&](int n) {
+        double p = 0;
+        const int na = B.active_shots(n);
+        for (int k = 0; k < na; k++) {
+            const ShotDev &S = B.shots_h[B.order[k]];
+            p += (double)(S.NZ - 8) * (double)(S.NX - 8);
+        }
+        return p;
+    };
+    std::vector<std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    struct GraphsFree {
+        std::vector<std::pair<hipGraph_t, hipGraphExec_t>> &g;
+        ~GraphsFree() {
+            for (auto &q : g) {
+                if (q.second) (void)hipGraphExecDestroy(q.second);
+                if (q.first) (void)hipGraphDestroy(q.first);
+            }
+        }
+    } graphs_free{graphs};
+    int span = 0;
+    // ---- blocks of K steps with every shot running: one graph, replayed
     if (nblk > 0) {
-        hipLaunchKernelGGL(set_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, 0);
-        if (use_graphs) {
-            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        double bk = 0, pk = 0;
+        for (int n = 0; n < K; n += inc) bk += step_bytes(B, A, n, two);
+        pk = (double)K * pts_of(0);
+        hipGraphExec_t exec = nullptr;
+        if (graphs_on) {
+            hipLaunchKernelGGL(set_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, 0);
+            CaptureGuard cg;
+            cg.begin(stream);
             for (int n = 0; n < K; n += inc) two ? pair(n, d_nb) : one(n, d_nb);
             hipLaunchKernelGGL(advance_base_kernel, dim3(1), dim3(1), 0, stream, d_nb, K);
-            HIPCHK(hipStreamEndCapture(stream, &graph));
-            HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            hipGraph_t gr = cg.end();
+            graphs.push_back({gr, nullptr});
+            HIPCHK(hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0));
+            graphs.back().second = exec;
+        }
+        for (int b = 0; b < nblk; b++, span++) {
+            if (timed) HIPCHK(hipEventRecord(kev[2 * span], stream));
+            if (exec) {
+                HIPCHK(hipGraphLaunch(exec, stream));
+            } else {
+                for (int n = 0; n < K; n += inc) two ? pair(b * K + n, nullptr) : one(b * K + n, nullptr);
+            }
+            if (timed) HIPCHK(hipEventRecord(kev[2 * span + 1], stream));
+            // the replayed block's bytes: its snapshot phases repeat every K steps
+            span_bytes.push_back(bk);
+            span_pts.push_back(pk);
+            span_steps.push_back(K);
         }
     }
-    for (int b = 0; b < nblk; b++) {
-        if (timed) HIPCHK(hipEventRecord(kev[2 * b], stream));
-        if (exec) {
-            HIPCHK(hipGraphLaunch(exec, stream));
-        } else {
-            for (int n = 0; n < K; n += inc) two ? pair(b * K + n, nullptr) : one(b * K + n, nullptr);
+    // ---- the tail: chunks with fixed step indices and shrinking grids
+    for (const auto &c : chunks) {
+        double bk = 0, pk = 0;
+        auto launches = [&]() {
+            for (int n = c.first; n < c.second;) {
+                const bool p2 = two && n + 1 < c.second && B.active_shots(n) == B.active_shots(n + 1);
+                if (p2) {
+                    pair(n, nullptr);
+                    bk += step_bytes(B, A, n, true);
+                    pk += pts_of(n) + pts_of(n + 1);
+                    n += 2;
+                } else {
+                    one(n, nullptr);
+                    bk += step_bytes(B, A, n, false);
+                    pk += pts_of(n);
+                    n++;
+                }
+            }
+        };
+        hipGraphExec_t exec = nullptr;
+        if (graphs_on) {
+            CaptureGuard cg;
+            cg.begin(stream);
+            launches();
+            hipGraph_t gr = cg.end();
+            graphs.push_back({gr, nullptr});
+            HIPCHK(hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0));
+            graphs.back().second = exec;
         }
-        if (timed) HIPCHK(hipEventRecord(kev[2 * b + 1], stream));
-    }
-    for (int n = nblk * K; n < nsteps;) {
-        if (two && n + 1 < nfree) {
-            pair(n, nullptr);
-            n += 2;
-        } else {
-            one(n, nullptr);
-            n++;
-        }
+        if (timed) HIPCHK(hipEventRecord(kev[2 * span], stream));
+        if (exec) HIPCHK(hipGraphLaunch(exec, stream));
+        else launches();
+        if (timed) HIPCHK(hipEventRecord(kev[2 * span + 1], stream));
+        span_bytes.push_back(bk);
+        span_pts.push_back(pk);
+        span_steps.push_back(c.second - c.first);
+        span++;
     }
     HIPCHK(hipGetLastError());
-    if (timed && nblk > 0) {
-        HIPCHK(hipEventSynchronize(kev[2 * nblk - 1]));
-        for (int b = 0; b < nblk; b++) {
+    if (timed && span > 0) {
+        HIPCHK(hipEventSynchronize(kev[2 * span - 1]));
+        for (int b = 0; b < span; b++) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, kev[2 * b], kev[2 * b + 1]));
             kstat_us[dir] += 1000.0 * ms;
-            kstat_n[dir] += K;
+            kstat_n[dir] += span_steps[b];
+            kstat_bytes[dir] += span_bytes[b];
+            kstat_pts[dir] += span_pts[b];
         }
     }
-    if (exec) {
-        HIPCHK(hipStreamSynchronize(stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-    }
+    // the graphs must outlive their launches
+    if (!graphs.empty()) HIPCHK(hipStreamSynchronize(stream));
 }
 
 void Engine::reset_kstats() {
     kstat_us[0] = kstat_us[1] = 0;
     kstat_n[0] = kstat_n[1] = 0;
+    kstat_pts[0] = kstat_pts[1] = kstat_bytes[0] = kstat_bytes[1] = 0;
 }
 
 static int max_nt(const std::vector<ShotPlan> &plans) {
     int m = 0;
     for (const ShotPlan &p : plans) m = std::max(m, p.nt);
-    return m;
-}
-static int min_nt(const std::vector<ShotPlan> &plans) {
-    int m = plans.empty() ? 0 : plans[0].nt;
-    for (const ShotPlan &p : plans) m = std::min(m, p.nt);
     return m;
 }
 
@@ -996,12 +1159,12 @@ void Engine::upload_inj_traces(Batch &B, int s, const float *tr, size_t count) {
     HIPCHK(hipMemcpyAsync((void *)B.shots_h[s].inj_traces, tr, count * sizeof(float), hipMemcpyHostToDevice, stream));
 }
 
-void Engine::check_shots(const sfd_shot *shots, int nshots, int snaps) {
+void Engine::check_shots(const sfd_shot *shots, int nshots, int snaps, bool may_tune) {
     if (!have_model) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model first");
     if (!shots || nshots < 1) throw EngineError(SFD_EINVAL, "no shots");
     for (int s = 0; s < nshots; s++)
         if (!shots[s].wavelet || shots[s].nt < 1 || shots[s].ntr < 1) throw EngineError(SFD_EINVAL, "bad shot descriptor");
-    if (autotune_pending) autotune(shots, nshots, snaps);
+    if (autotune_pending && may_tune) autotune(shots, nshots, snaps);
 }
 
 // ---- tile / strip autotuner -------------------------------------------------
@@ -1075,84 +1238,171 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
     autotune_pending = false;
 }
 
-// ---- forward modeling (Acoustic_modeling_sep.f90 / AMOD) -------------------
-void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind) {
-    check_shots(shots, ns);
-    reset_kstats();
-    if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract kind");
-    auto t0 = std::chrono::steady_clock::now();
-    std::vector<ShotPlan> plans(ns);
-    for (int s = 0; s < ns; s++) {
-        plan_shot(shots[s], plans[s]);
-        plan_injection(plans[s], vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
-        plans[s].n_inj_traces = 1;
-        plan_extraction(plans[s], vel_h, shots[s], extract_kind);
-    }
-    BatchNeeds need;
-    build_batch(fwd_batch, plans, need);
-    for (int s = 0; s < ns; s++) upload_inj_traces(fwd_batch, s, shots[s].wavelet, shots[s].nt);
-    prepare_models(fwd_batch, plans, d_vel);
-    StepArgs A = base_args(fwd_batch);
-    A.dir = 1;
-    A.inject = 1;
-    A.extract = 1;
-    auto t1 = std::chrono::steady_clock::now();
-    HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(fwd_batch, A, false, max_nt(plans), min_nt(plans));
-    HIPCHK(hipEventRecord(ev1, stream));
-    HIPCHK(hipEventSynchronize(ev1));
-    size_t off = 0;
-    for (int s = 0; s < ns; s++) {
-        const size_t n = (size_t)plans[s].nt * plans[s].ntr;
-        HIPCHK(hipMemcpy(traces_out + off, fwd_batch.shots_h[s].traces, n * sizeof(float), hipMemcpyDeviceToHost));
-        off += n;
-    }
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-    timing[0] = ms; timing[1] = ms; timing[2] = 0;
-    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    pts = count_pts(plans);
+// ---- HBM capacity: shot waves --------------------------------------------
+// Device bytes one shot adds to a batch (build_batch's layout: wavefield
+// levels, model copies, injection / extraction tables, traces, snapshots and
+// image buffers), from its window alone, with a margin for the plan tables.
+size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
+    float oxw;
+    int nxw;
+    shot_window(g, s, &oxw, &nxw);
+    const size_t NZ = (size_t)g.nz + 2 * g.ntaper + 8, NX = (size_t)nxw + 2 * g.ntaper + 8;
+    const size_t LDZ = (NZ + 64 + 31) / 32 * 32;
+    const size_t ntile = (NZ + 64) / 64 * ((NX + 32) / 32 + 1);
+    const size_t npix = (size_t)g.nz * nxw, ntr = (size_t)std::max(s.ntr, 1), nt = (size_t)s.nt;
+    const size_t ntsnap = (size_t)std::max(s.nt / g.snapi, 1);
+    size_t b = (kLevels + 2) * LDZ * NX * 4;                 // levels, v^2, vel
+    b += 16 * (NX + NZ) * 4 + 2 * NX * 4;                     // Higdon tables, window columns, taper
+    b += 3 * nt * ntr * 4;                                    // injected (two layouts) and extracted traces
+    b += ntr * 49 * 16 * 4 + ntr * 16 * 4 + ntile * 64;       // contributions (both sets, two-step regions)
+    b += (ntr + 4) * 2 * 64 * 32 * 4;                         // dense cell maps of the injecting tiles
+    if (need.snapshots) b += ntsnap * npix * (need.snap_fp16 ? 2 : 4) + (need.snap_fp16 ? ntsnap * ntile * 4 : 0);
+    if (need.imaging) b += 2 * npix * 4;
+    if (need.born) b += npix * 4;
+    return b + (b >> 4) + (1 << 20);
 }
 
-// Scatter window images to the full model (mod_copy_image) and add into host arrays.
-void Engine::gather_images(Batch &B, std::vector<ShotPlan> &plans, float *image, float *illum) {
-    // inverse map: model column -> (shot, window column), shot order
-    std::vector<std::vector<int>> inv(g.nx);
+// Contiguous ranges of shots [first, end) whose batches fit the device: the
+// memory free now plus the batch buffer already held (it is reused or
+// replaced), less a reserve for the call's other buffers, or the
+// SFD_OPT_BATCH_BYTES cap when that is set.  One wave when everything fits.
+// Waves follow shot order, so images, gradients and misfits summed over
+// waves are added in the same order as in one batch.
+std::vector<std::pair<int, int>> Engine::plan_waves(const sfd_shot *shots, int ns, const BatchNeeds &need) {
+    size_t avail;
+    if (batch_bytes_cap > 0) {
+        avail = batch_bytes_cap;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t reserve = std::max<size_t>((size_t)2 << 30, tot / 50);
+        avail = fr + fwd_batch.mem.cap;
+        avail = avail > reserve ? avail - reserve : 0;
+    }
+    std::vector<std::pair<int, int>> w;
+    size_t acc = 0;
+    int first = 0;
+    for (int s = 0; s < ns; s++) {
+        const size_t b = shot_bytes(shots[s], need);
+        if (s > first && acc + b > avail) {
+            w.push_back({first, s});
+            first = s;
+            acc = 0;
+        }
+        acc += b;
+        if (acc > avail && s == first && batch_bytes_cap == 0)
+            throw EngineError(SFD_ENOMEM, "one shot needs " + std::to_string(b >> 20) + " MiB of device memory, " +
+                                              std::to_string(avail >> 20) + " MiB are available");
+    }
+    w.push_back({first, ns});
+    waves_last = (int)w.size();
+    return w;
+}
+
+// ---- forward modeling (Acoustic_modeling_sep.f90 / AMOD) -------------------
+// ext: a batch in caller-provided device memory (the allocation-free entry
+// point): one wave, no autotune, eager untimed launches.
+void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extract_kind, Batch *ext) {
+    check_shots(shots, ns, -1, ext == nullptr);
+    reset_kstats();
+    if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract kind");
+    BatchNeeds need;
+    const auto waves = ext ? std::vector<std::pair<int, int>>{{0, ns}} : plan_waves(shots, ns, need);
+    Batch &fwd_batch = ext ? *ext : this->fwd_batch;
+    struct EagerScope {   // no graphs, no events, no device step base during an allocation-free call
+        Engine &e;
+        bool old;
+        EagerScope(Engine &e_, bool on) : e(e_), old(e_.eager_untimed) { e.eager_untimed = old || on; }
+        ~EagerScope() { e.eager_untimed = old; }
+    } eager_scope(*this, ext != nullptr);
+    double setup_ms = 0, run_ms = 0;
+    size_t off = 0;
+    pts = 0;
+    for (const auto &wv : waves) {
+        const int s0 = wv.first, nw = wv.second - wv.first;
+        auto ta = std::chrono::steady_clock::now();
+        std::vector<ShotPlan> plans(nw);
+        for (int s = 0; s < nw; s++) {
+            const sfd_shot &sh = shots[s0 + s];
+            plan_shot(sh, plans[s]);
+            plan_injection(plans[s], vel_h, 1, &sh.sz, &sh.sx, 1);
+            plans[s].n_inj_traces = 1;
+            plan_extraction(plans[s], vel_h, sh, extract_kind);
+        }
+        build_batch(fwd_batch, plans, need);
+        for (int s = 0; s < nw; s++) upload_inj_traces(fwd_batch, s, shots[s0 + s].wavelet, shots[s0 + s].nt);
+        prepare_models(fwd_batch, plans, d_vel);
+        StepArgs A = base_args(fwd_batch);
+        A.dir = 1;
+        A.inject = 1;
+        A.extract = 1;
+        setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+        HIPCHK(hipEventRecord(ev0, stream));
+        run_loop(fwd_batch, A, false, max_nt(plans));
+        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK(hipEventSynchronize(ev1));
+        for (int s = 0; s < nw; s++) {
+            const size_t n = (size_t)plans[s].nt * plans[s].ntr;
+            HIPCHK(hipMemcpy(traces_out + off, fwd_batch.shots_h[s].traces, n * sizeof(float), hipMemcpyDeviceToHost));
+            off += n;
+        }
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        run_ms += ms;
+        pts += count_pts(plans);
+    }
+    timing[0] = run_ms; timing[1] = run_ms; timing[2] = 0;
+    timing[3] = setup_ms;
+}
+
+// Add every shot's window image into device full-model buffers (mod_copy_image,
+// nearest index, shots in order); the buffers are zeroed by the caller.
+void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill) {
+    std::vector<std::vector<std::pair<int, int>>> inv(g.nx);
     for (int s = 0; s < B.ns; s++)
         for (int j = 0; j < plans[s].nxw; j++) {
             const int ix = plans[s].colmap_raw[j];
             if (ix < 1 || ix > g.nx) continue;   // ExtractPadModel.f90:686
-            inv[ix - 1].push_back((s << 20) | j);
+            inv[ix - 1].push_back({s, j});
         }
-    std::vector<int> ptr(g.nx + 1, 0), idx;
+    std::vector<int> ptr(g.nx + 1, 0), ishot, icol;
     for (int ix = 0; ix < g.nx; ix++) {
         ptr[ix + 1] = ptr[ix] + (int)inv[ix].size();
-        idx.insert(idx.end(), inv[ix].begin(), inv[ix].end());
+        for (const auto &e : inv[ix]) {
+            ishot.push_back(e.first);
+            icol.push_back(e.second);
+        }
     }
-    const size_t N = (size_t)g.nz * g.nx;
     Layout L;
-    const size_t o_ptr = L.take(ptr.size() * 4), o_idx = L.take(std::max<size_t>(idx.size(), 1) * 4);
-    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_out = L.take(N * 4);
-    scratch.reserve(L.off);
-    char *base = (char *)scratch.p;
+    const size_t o_ptr = L.take(ptr.size() * 4), o_shot = L.take(std::max<size_t>(ishot.size(), 1) * 4),
+                 o_col = L.take(std::max<size_t>(icol.size(), 1) * 4);
+    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_ci2 = L.take(sizeof(CopyImg) * B.ns);
+    tmp_mem.reserve(L.off);
+    char *base = (char *)tmp_mem.p;
     HIPCHK(hipMemcpyAsync(base + o_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, stream));
-    if (!idx.empty()) HIPCHK(hipMemcpyAsync(base + o_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, stream));
-    img_host.reserve(N * 4);
-    float *h = (float *)img_host.p;
-    for (int pass = 0; pass < 2; pass++) {
-        float *dst = pass == 0 ? image : illum;
-        if (!dst) continue;
-        std::vector<CopyImg> ci(B.ns);
-        for (int s = 0; s < B.ns; s++) ci[s] = {pass == 0 ? B.shots_h[s].image : B.shots_h[s].illum, nullptr, plans[s].nxw};
-        HIPCHK(hipMemcpyAsync(base + o_ci, ci.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemsetAsync(base + o_out, 0, N * 4, stream));
-        hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
-                           (const CopyImg *)(base + o_ci), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
-                           (const int *)(base + o_idx), (float *)(base + o_out));
-        HIPCHK(hipMemcpyAsync(h, base + o_out, N * 4, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + h[q];
+    if (!ishot.empty()) {
+        HIPCHK(hipMemcpyAsync(base + o_shot, ishot.data(), ishot.size() * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(base + o_col, icol.data(), icol.size() * 4, hipMemcpyHostToDevice, stream));
     }
+    std::vector<CopyImg> ci(B.ns), ci2(B.ns);
+    for (int s = 0; s < B.ns; s++) {
+        ci[s] = {B.shots_h[s].image, nullptr, plans[s].nxw};
+        ci2[s] = {B.shots_h[s].illum, nullptr, plans[s].nxw};
+    }
+    HIPCHK(hipMemcpyAsync(base + o_ci, ci.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(base + o_ci2, ci2.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
+    const dim3 grid((g.nz + 255) / 256, g.nx);
+    if (d_img)
+        hipLaunchKernelGGL(copy_image_kernel, grid, dim3(256), 0, stream, (const CopyImg *)(base + o_ci), g.nz, g.nx,
+                           (const int *)(base + o_ptr), (const int *)(base + o_shot), (const int *)(base + o_col),
+                           d_img);
+    if (d_ill)
+        hipLaunchKernelGGL(copy_image_kernel, grid, dim3(256), 0, stream, (const CopyImg *)(base + o_ci2), g.nz, g.nx,
+                           (const int *)(base + o_ptr), (const int *)(base + o_shot), (const int *)(base + o_col),
+                           d_ill);
+    HIPCHK(hipGetLastError());
+    // the host tables above must outlive their async transfers
+    HIPCHK(hipStreamSynchronize(stream));
 }
 
 // ---- exact discrete adjoint of forward modeling (no reference counterpart) --
@@ -1195,15 +1445,15 @@ void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar
         A1.dir = 1;
         A1.inject = 1;
         A1.store_snap = 1;
-        run_loop(B, A1, false, sh.nt, sh.nt);
+        run_loop(B, A1, false, sh.nt);
         ShotDev S = B.shots_h[0];
         B.injb[0].apply(S);
         upload_inj_traces(B, 0, dbar + off, (size_t)sh.nt * sh.ntr);
         const size_t nf = (size_t)p.LDZ * p.NX, nw = (size_t)g.nz * p.nxw;
         Layout L;
         const size_t o_lev = L.take(6 * nf * 4), o_mb = L.take(nw * 4);
-        scratch.reserve(L.off);
-        char *base = (char *)scratch.p;
+        tmp_mem.reserve(L.off);
+        char *base = (char *)tmp_mem.p;
         HIPCHK(hipMemsetAsync(base + o_lev, 0, 6 * nf * 4, stream));
         HIPCHK(hipMemsetAsync(base + o_mb, 0, nw * 4, stream));
         const StepArgs SA = base_args(B);
@@ -1283,8 +1533,8 @@ void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, flo
         Layout L;
         const size_t o_lev = L.take(6 * nf * 4), o_addr = L.take(addr.size() * 4), o_coef = L.take(coef.size() * 4),
                      o_sb = L.take((size_t)sh.nt * 4);
-        scratch.reserve(L.off);
-        char *base = (char *)scratch.p;
+        tmp_mem.reserve(L.off);
+        char *base = (char *)tmp_mem.p;
         HIPCHK(hipMemsetAsync(base + o_lev, 0, 6 * nf * 4, stream));
         HIPCHK(hipMemcpyAsync(base + o_addr, addr.data(), addr.size() * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(base + o_coef, coef.data(), coef.size() * 4, hipMemcpyHostToDevice, stream));
@@ -1347,83 +1597,107 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
         if (total != resident_n) throw EngineError(SFD_EINVAL, "resident traces do not match the shots' nt * ntr");
     }
     if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
-    auto t0 = std::chrono::steady_clock::now();
     BatchNeeds need;
     need.backward_injection = true;
     need.snapshots = true;
     need.snap_fp16 = snap_mode == SFD_SNAP_FP16S;
     need.imaging = true;
     Batch &B = fwd_batch;
-    // a repeated call with the same plan (same shots, model, flags and tile
-    // settings) reuses the tables already on the device
-    const std::string key = plan_key(lsrtm ? "lsrtm" : "rtm", shots, ns, snap_mode, 0);
-    if (plan_reuse && !B.key.empty() && B.key == key) {
-        reset_batch(B);
-    } else {
-        std::vector<ShotPlan> fresh(ns);
-        for (int s = 0; s < ns; s++) {
-            ShotPlan &p = fresh[s];
-            plan_shot(shots[s], p);
-            plan_injection(p, vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
-            // backward injection: receivers as sources (Injection_sinc / Injection_LSRTM_sinc)
-            ShotPlan tmp = p;
-            plan_injection(tmp, velb_h.empty() ? vel_h : velb_h, shots[s].ntr, shots[s].rz, shots[s].rx, lsrtm ? 3 : 1);
-            p.take_backward(tmp);
-            p.n_inj_traces = std::max(1, shots[s].ntr);
-        }
-        rtm_plans = std::move(fresh);
-        build_batch(B, rtm_plans, need);
-        B.key = key;
-    }
-    std::vector<ShotPlan> &plans = rtm_plans;
-    // forward source (second-differenced for LSRTM, ReadSouVelRho.f90:318-326)
-    for (int s = 0; s < ns; s++) {
-        std::vector<float> w(shots[s].wavelet, shots[s].wavelet + shots[s].nt);
-        if (lsrtm) second_difference(w, g.dt);
-        upload_inj_traces(B, s, w.data(), w.size());
-    }
-    prepare_models(B, plans, d_vel);
-    auto t1 = std::chrono::steady_clock::now();
-    StepArgs A = base_args(B);
-    A.dir = 1;
-    A.inject = 1;
-    A.store_snap = 1;
-    A.snap_fp16 = need.snap_fp16;
-    HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), min_nt(plans));
-    HIPCHK(hipEventRecord(ev1, stream));
-    // backward
-    zero_wavefields(B, plans);
-    if (d_velb) prepare_models(B, plans, d_velb);
+    const auto waves = plan_waves(shots, ns, need);
+    // full-model image and illumination summed over every wave on the device
+    const size_t N = (size_t)g.nz * g.nx;
+    acc_mem.reserve(2 * N * sizeof(float) + 256);
+    float *d_img = (float *)acc_mem.p, *d_ill = d_img + N;
+    HIPCHK(hipMemsetAsync(d_img, 0, 2 * N * sizeof(float), stream));
+    double setup_ms = 0, fwd_ms = 0, bwd_ms = 0;
     size_t off = 0;
-    for (int s = 0; s < ns; s++) {
-        ShotDev &S = B.shots_h[s];
-        B.injb[s].apply(S);
-        const size_t n = (size_t)shots[s].nt * shots[s].ntr;
-        if (traces_in)
-            upload_inj_traces(B, s, traces_in + off, n);
-        else
-            HIPCHK(hipMemcpyAsync((void *)S.inj_traces, (const float *)resident.p + off, n * sizeof(float),
-                                  hipMemcpyDeviceToDevice, stream));
-        off += n;
-        std::vector<float> z((size_t)g.nz * plans[s].nxw, 0.f);
+    pts = 0;
+    for (const auto &wv : waves) {
+        const int s0 = wv.first, nw = wv.second - wv.first;
+        const sfd_shot *ws = shots + s0;
+        auto ta = std::chrono::steady_clock::now();
+        // a repeated one-wave call with the same plan (same shots, model, flags
+        // and tile settings) reuses the tables already on the device
+        const std::string key = waves.size() == 1 ? plan_key(lsrtm ? "lsrtm" : "rtm", ws, nw, snap_mode, 0) : "";
+        if (plan_reuse && !key.empty() && !B.key.empty() && B.key == key) {
+            reset_batch(B);
+        } else {
+            std::vector<ShotPlan> fresh(nw);
+            for (int s = 0; s < nw; s++) {
+                ShotPlan &p = fresh[s];
+                plan_shot(ws[s], p);
+                plan_injection(p, vel_h, 1, &ws[s].sz, &ws[s].sx, 1);
+                // backward injection: receivers as sources (Injection_sinc / Injection_LSRTM_sinc)
+                ShotPlan tmp = p;
+                plan_injection(tmp, velb_h.empty() ? vel_h : velb_h, ws[s].ntr, ws[s].rz, ws[s].rx, lsrtm ? 3 : 1);
+                p.take_backward(tmp);
+                p.n_inj_traces = std::max(1, ws[s].ntr);
+            }
+            rtm_plans = std::move(fresh);
+            build_batch(B, rtm_plans, need);
+            B.key = key;
+        }
+        std::vector<ShotPlan> &plans = rtm_plans;
+        // forward source (second-differenced for LSRTM, ReadSouVelRho.f90:318-326)
+        for (int s = 0; s < nw; s++) {
+            std::vector<float> w(ws[s].wavelet, ws[s].wavelet + ws[s].nt);
+            if (lsrtm) second_difference(w, g.dt);
+            upload_inj_traces(B, s, w.data(), w.size());
+            HIPCHK(hipStreamSynchronize(stream));   // w is freed at the end of this iteration
+        }
+        prepare_models(B, plans, d_vel);
+        setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+        StepArgs A = base_args(B);
+        A.dir = 1;
+        A.inject = 1;
+        A.store_snap = 1;
+        A.snap_fp16 = need.snap_fp16;
+        HIPCHK(hipEventRecord(ev0, stream));
+        run_loop(B, A, false, max_nt(plans));
+        HIPCHK(hipEventRecord(ev1, stream));
+        // backward
+        zero_wavefields(B, plans);
+        if (d_velb) prepare_models(B, plans, d_velb);
+        for (int s = 0; s < nw; s++) {
+            ShotDev &S = B.shots_h[s];
+            B.injb[s].apply(S);
+            const size_t n = (size_t)ws[s].nt * ws[s].ntr;
+            if (traces_in)
+                upload_inj_traces(B, s, traces_in + off, n);
+            else
+                HIPCHK(hipMemcpyAsync((void *)S.inj_traces, (const float *)resident.p + off, n * sizeof(float),
+                                      hipMemcpyDeviceToDevice, stream));
+            off += n;
+        }
+        upload_shots(B);
+        StepArgs Ab = base_args(B);
+        Ab.dir = -1;
+        Ab.inject = 1;
+        Ab.snap_fp16 = need.snap_fp16;
+        Ab.imaging = lsrtm ? IMG_LSRTM : IMG_RTM;
+        run_loop(B, Ab, true, max_nt(plans));
+        HIPCHK(hipEventRecord(ev2, stream));
+        HIPCHK(hipEventSynchronize(ev2));
+        float ms1 = 0, ms2 = 0;
+        HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+        fwd_ms += ms1;
+        bwd_ms += ms2;
+        pts += 2 * count_pts(plans);
+        accumulate_images(B, plans, image ? d_img : nullptr, illum ? d_ill : nullptr);
     }
-    upload_shots(B);
-    StepArgs Ab = base_args(B);
-    Ab.dir = -1;
-    Ab.inject = 1;
-    Ab.snap_fp16 = need.snap_fp16;
-    Ab.imaging = lsrtm ? IMG_LSRTM : IMG_RTM;
-    run_loop(B, Ab, true, max_nt(plans), min_nt(plans));
-    HIPCHK(hipEventRecord(ev2, stream));
-    HIPCHK(hipEventSynchronize(ev2));
-    float ms1 = 0, ms2 = 0;
-    HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
-    HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
-    timing[0] = ms1 + ms2; timing[1] = ms1; timing[2] = ms2;
-    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    pts = 2 * count_pts(plans);
-    gather_images(B, plans, image, illum);
+    timing[0] = fwd_ms + bwd_ms; timing[1] = fwd_ms; timing[2] = bwd_ms;
+    timing[3] = setup_ms;
+    // image += sum over shots (the caller's arrays accumulate, as the reference's do)
+    img_host.reserve(N * sizeof(float));
+    float *h = (float *)img_host.p;
+    for (int pass = 0; pass < 2; pass++) {
+        float *dst = pass == 0 ? image : illum;
+        if (!dst) continue;
+        HIPCHK(hipMemcpyAsync(h, pass == 0 ? d_img : d_ill, N * sizeof(float), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + h[q];
+    }
 }
 
 // ---- Born modeling (to_disk_to_memory_BORN.f90:123-220) ----------------------
@@ -1431,113 +1705,107 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
     check_shots(shots, ns, SFD_SNAP_FP32);
     reset_kstats();
     if (!refl || !traces_out) throw EngineError(SFD_EINVAL, "NULL refl/traces_out");
-    auto t0 = std::chrono::steady_clock::now();
-    std::vector<ShotPlan> plans(ns);
-    for (int s = 0; s < ns; s++) {
-        plan_shot(shots[s], plans[s]);
-        plan_injection(plans[s], vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
-        plans[s].n_inj_traces = 1;
-        plan_extraction(plans[s], vel_h, shots[s], EXT_SINC_LSRTM);
-    }
     BatchNeeds need;
     need.snapshots = true;
     need.born = true;
     Batch &B = fwd_batch;
-    build_batch(B, plans, need);
-    for (int s = 0; s < ns; s++) {
-        std::vector<float> w(shots[s].wavelet, shots[s].wavelet + shots[s].nt);
-        second_difference(w, g.dt);   // readsou with Born=.true.
-        upload_inj_traces(B, s, w.data(), w.size());
-        // reflectivity window (readref + mod_window_pad interior)
-        std::vector<float> m((size_t)g.nz * plans[s].nxw);
-        for (int j = 0; j < plans[s].nxw; j++)
-            for (int i = 0; i < g.nz; i++) m[(size_t)j * g.nz + i] = refl[(size_t)plans[s].colsrc[j] * g.nz + i];
-        HIPCHK(hipMemcpyAsync((void *)B.shots_h[s].born_m, m.data(), m.size() * 4, hipMemcpyHostToDevice, stream));
-    }
-    prepare_models(B, plans, d_vel);
-    auto t1 = std::chrono::steady_clock::now();
-    StepArgs A = base_args(B);
-    A.dir = 1;
-    A.inject = 1;
-    A.store_snap = 1;
-    HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), min_nt(plans));
-    HIPCHK(hipEventRecord(ev1, stream));
-    zero_wavefields(B, plans);
-    StepArgs A2 = base_args(B);
-    A2.dir = 1;
-    A2.extract = 1;
-    A2.imaging = IMG_BORN;
-    run_loop(B, A2, false, max_nt(plans), min_nt(plans));
-    HIPCHK(hipEventRecord(ev2, stream));
-    HIPCHK(hipEventSynchronize(ev2));
+    const auto waves = plan_waves(shots, ns, need);
+    double setup_ms = 0, fwd_ms = 0, bwd_ms = 0;
     size_t off = 0;
-    for (int s = 0; s < ns; s++) {
-        const size_t n = (size_t)plans[s].nt * plans[s].ntr;
-        HIPCHK(hipMemcpy(traces_out + off, B.shots_h[s].traces, n * sizeof(float), hipMemcpyDeviceToHost));
-        off += n;
+    pts = 0;
+    for (const auto &wv : waves) {
+        const int s0 = wv.first, nw = wv.second - wv.first;
+        const sfd_shot *ws = shots + s0;
+        auto ta = std::chrono::steady_clock::now();
+        std::vector<ShotPlan> plans(nw);
+        for (int s = 0; s < nw; s++) {
+            plan_shot(ws[s], plans[s]);
+            plan_injection(plans[s], vel_h, 1, &ws[s].sz, &ws[s].sx, 1);
+            plans[s].n_inj_traces = 1;
+            plan_extraction(plans[s], vel_h, ws[s], EXT_SINC_LSRTM);
+        }
+        build_batch(B, plans, need);
+        for (int s = 0; s < nw; s++) {
+            std::vector<float> w(ws[s].wavelet, ws[s].wavelet + ws[s].nt);
+            second_difference(w, g.dt);   // readsou with Born=.true.
+            upload_inj_traces(B, s, w.data(), w.size());
+            // reflectivity window (readref + mod_window_pad interior)
+            std::vector<float> m((size_t)g.nz * plans[s].nxw);
+            for (int j = 0; j < plans[s].nxw; j++)
+                for (int i = 0; i < g.nz; i++) m[(size_t)j * g.nz + i] = refl[(size_t)plans[s].colsrc[j] * g.nz + i];
+            HIPCHK(hipMemcpyAsync((void *)B.shots_h[s].born_m, m.data(), m.size() * 4, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));   // w and m are freed at the end of this iteration
+        }
+        prepare_models(B, plans, d_vel);
+        setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+        StepArgs A = base_args(B);
+        A.dir = 1;
+        A.inject = 1;
+        A.store_snap = 1;
+        HIPCHK(hipEventRecord(ev0, stream));
+        run_loop(B, A, false, max_nt(plans));
+        HIPCHK(hipEventRecord(ev1, stream));
+        zero_wavefields(B, plans);
+        StepArgs A2 = base_args(B);
+        A2.dir = 1;
+        A2.extract = 1;
+        A2.imaging = IMG_BORN;
+        run_loop(B, A2, false, max_nt(plans));
+        HIPCHK(hipEventRecord(ev2, stream));
+        HIPCHK(hipEventSynchronize(ev2));
+        for (int s = 0; s < nw; s++) {
+            const size_t n = (size_t)plans[s].nt * plans[s].ntr;
+            HIPCHK(hipMemcpy(traces_out + off, B.shots_h[s].traces, n * sizeof(float), hipMemcpyDeviceToHost));
+            off += n;
+        }
+        float ms1 = 0, ms2 = 0;
+        HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+        fwd_ms += ms1;
+        bwd_ms += ms2;
+        pts += 2 * count_pts(plans);
     }
-    float ms1 = 0, ms2 = 0;
-    HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
-    HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
-    timing[0] = ms1 + ms2; timing[1] = ms1; timing[2] = ms2;
-    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    pts = 2 * count_pts(plans);
+    timing[0] = fwd_ms + bwd_ms; timing[1] = fwd_ms; timing[2] = bwd_ms;
+    timing[3] = setup_ms;
 }
 
 // ---- FWI misfit + gradient (compute_function_gradient.f90:208-350) ---------
+// Shots run in waves (plan_waves); every wave adds its shots' per-trace misfits
+// and window images into the call's buffers in shot order, then the
+// post-processing chain runs once on the sums.
 void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
                           const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out) {
     check_shots(shots, ns, opt ? opt->snap_mode : -1);
     reset_kstats();
-    if (!dobs || !opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL dobs/opt/misfit/grad");
+    if (!opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL opt/misfit/grad");
     if (opt->snap_mode != SFD_SNAP_FP32 && opt->snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
     if ((opt->do_z && opt->rect_z < 1) || (opt->do_x && opt->rect_x < 1)) throw EngineError(SFD_EINVAL, "bad smoothing length");
     if ((opt->do_z && opt->rect_z > g.nz) || (opt->do_x && opt->rect_x > g.nx))
         throw EngineError(SFD_EINVAL, "smoothing box longer than the model (Smoothing.f90:202 boxconv)");
-    auto t0 = std::chrono::steady_clock::now();
-    std::vector<ShotPlan> plans(ns);
     long ndata = 0;
     int ntraces = 0;
     for (int s = 0; s < ns; s++) {
-        ShotPlan &p = plans[s];
-        plan_shot(shots[s], p);
-        plan_injection(p, vel_h, 1, &shots[s].sz, &shots[s].sx, 1);        // Injection_sinc (AMOD)
-        plan_extraction(p, vel_h, shots[s], EXT_NEAREST);                     // Extraction_array_simple_afwi
-        ShotPlan tmp = p;
-        plan_injection(tmp, vel_h, shots[s].ntr, shots[s].rz, shots[s].rx, 2);  // Injection_simple (AGRAD)
-        p.take_backward(tmp);
-        p.n_inj_traces = std::max(1, shots[s].ntr);
         ndata += (long)shots[s].nt * shots[s].ntr;
         ntraces += shots[s].ntr;
     }
+    if (!dobs) {
+        if (resident_n == 0) throw EngineError(SFD_EINVAL, "dobs is NULL and no traces are resident");
+        if ((size_t)ndata != resident_n) throw EngineError(SFD_EINVAL, "resident traces do not match the shots' nt * ntr");
+    }
+    auto t0 = std::chrono::steady_clock::now();
     BatchNeeds need;
     need.backward_injection = true;
     need.snapshots = true;
     need.snap_fp16 = opt->snap_mode == SFD_SNAP_FP16S;
     need.imaging = true;
     Batch &B = fwd_batch;
-    build_batch(B, plans, need);
-    for (int s = 0; s < ns; s++) upload_inj_traces(B, s, shots[s].wavelet, shots[s].nt);
-    prepare_models(B, plans, d_vel);
+    const auto waves = plan_waves(shots, ns, need);
 
-    // batch-level FWI buffers
+    // call-level FWI buffers
     const size_t N = (size_t)g.nz * g.nx;
-    std::vector<TraceRef> refs;
-    refs.reserve(ntraces);
-    {
-        long off = 0;
-        for (int s = 0; s < ns; s++)
-            for (int r = 0; r < shots[s].ntr; r++) {
-                refs.push_back({s, r, shots[s].nt, off});
-                off += shots[s].nt;
-            }
-    }
-    std::vector<float *> adjp(ns);
-    for (int s = 0; s < ns; s++) adjp[s] = (float *)B.shots_h[s].inj_traces;
     const int rblocks = 1024;
     Layout L;
-    const size_t o_dobs = L.take(ndata * 4), o_mute = L.take(mute ? ndata * 4 : 16), o_res = L.take(ndata * 4);
+    const size_t o_dobs = L.take(dobs ? ndata * 4 : 16), o_mute = L.take(mute ? ndata * 4 : 16), o_res = L.take(ndata * 4);
     const size_t o_refs = L.take(sizeof(TraceRef) * ntraces), o_adj = L.take(sizeof(float *) * ns);
     const size_t o_ft = L.take(sizeof(double) * ntraces);
     const size_t o_grad = L.take(N * 4), o_illu = L.take(N * 4), o_tmp = L.take(N * 4);
@@ -1549,46 +1817,97 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     const size_t o_pp = L.take((size_t)(maxn + maxr) * nlines * 4), o_qq = L.take((size_t)(maxn + 2 * maxr) * nlines * 4);
     fwi_mem.reserve(L.off);
     char *fb = (char *)fwi_mem.p;
-    HIPCHK(hipMemcpyAsync(fb + o_dobs, dobs, ndata * 4, hipMemcpyHostToDevice, stream));
+    const float *d_dobs = dobs ? (const float *)(fb + o_dobs) : (const float *)resident.p;
+    if (dobs) HIPCHK(hipMemcpyAsync(fb + o_dobs, dobs, ndata * 4, hipMemcpyHostToDevice, stream));
     if (mute) HIPCHK(hipMemcpyAsync(fb + o_mute, mute, ndata * 4, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(fb + o_refs, refs.data(), sizeof(TraceRef) * ntraces, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(fb + o_adj, adjp.data(), sizeof(float *) * ns, hipMemcpyHostToDevice, stream));
     if (vpmask) HIPCHK(hipMemcpyAsync(fb + o_mask, vpmask, N * 4, hipMemcpyHostToDevice, stream));
-    auto t1 = std::chrono::steady_clock::now();
-
-    // AMOD: forward modeling with nearest extraction and snapshots
-    StepArgs A = base_args(B);
-    A.dir = 1;
-    A.inject = 1;
-    A.extract = 1;
-    A.store_snap = 1;
-    A.snap_fp16 = need.snap_fp16;
-    HIPCHK(hipEventRecord(ev0, stream));
-    run_loop(B, A, false, max_nt(plans), min_nt(plans));
-    // residual, FP64 misfit per trace, adjoint source into the injection buffers
-    hipLaunchKernelGGL(residual_kernel, dim3(ntraces), dim3(256), 0, stream, (const TraceRef *)(fb + o_refs),
-                       B.shots, (const float *)(fb + o_dobs), mute ? (const float *)(fb + o_mute) : nullptr,
-                       (float *)(fb + o_res), (float *const *)(fb + o_adj), (double *)(fb + o_ft));
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev1, stream));
-    // AGRAD: reverse time, forward stencil, simple injection, FWI imaging condition
-    zero_wavefields(B, plans);
-    for (int s = 0; s < ns; s++) {
-        ShotDev &S = B.shots_h[s];
-        B.injb[s].apply(S);
-    }
-    upload_shots(B);
-    StepArgs Ab = base_args(B);
-    Ab.dir = -1;
-    Ab.inject = 1;
-    Ab.snap_fp16 = need.snap_fp16;
-    Ab.imaging = IMG_FWI;
-    run_loop(B, Ab, false, max_nt(plans), min_nt(plans));
-    HIPCHK(hipEventRecord(ev2, stream));
-
-    // mod_copy_image into the full model, then the post-processing chain
     float *d_grad = (float *)(fb + o_grad), *d_illu = (float *)(fb + o_illu), *d_tmp = (float *)(fb + o_tmp);
-    scatter_images(B, plans, d_grad, d_illu);
+    HIPCHK(hipMemsetAsync(d_grad, 0, N * 4, stream));
+    HIPCHK(hipMemsetAsync(d_illu, 0, N * 4, stream));
+    double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    double fwd_ms = 0, bwd_ms = 0;
+    pts = 0;
+    long doff = 0;     // first sample of the wave's traces in the concatenated data
+    int toff = 0;      // first trace of the wave
+    for (const auto &wv : waves) {
+        const int s0 = wv.first, nw = wv.second - wv.first;
+        const sfd_shot *ws = shots + s0;
+        auto ta = std::chrono::steady_clock::now();
+        std::vector<ShotPlan> plans(nw);
+        for (int s = 0; s < nw; s++) {
+            ShotPlan &p = plans[s];
+            plan_shot(ws[s], p);
+            plan_injection(p, vel_h, 1, &ws[s].sz, &ws[s].sx, 1);        // Injection_sinc (AMOD)
+            plan_extraction(p, vel_h, ws[s], EXT_NEAREST);                 // Extraction_array_simple_afwi
+            ShotPlan tmp = p;
+            plan_injection(tmp, vel_h, ws[s].ntr, ws[s].rz, ws[s].rx, 2);  // Injection_simple (AGRAD)
+            p.take_backward(tmp);
+            p.n_inj_traces = std::max(1, ws[s].ntr);
+        }
+        build_batch(B, plans, need);
+        for (int s = 0; s < nw; s++) upload_inj_traces(B, s, ws[s].wavelet, ws[s].nt);
+        prepare_models(B, plans, d_vel);
+        std::vector<TraceRef> refs;
+        int wtr = 0;
+        {
+            long o = doff;
+            for (int s = 0; s < nw; s++)
+                for (int r = 0; r < ws[s].ntr; r++) {
+                    refs.push_back({s, r, ws[s].nt, o});
+                    o += ws[s].nt;
+                    wtr++;
+                }
+        }
+        std::vector<float *> adjp(nw);
+        for (int s = 0; s < nw; s++) adjp[s] = (float *)B.shots_h[s].inj_traces;
+        HIPCHK(hipMemcpyAsync(fb + o_refs, refs.data(), sizeof(TraceRef) * wtr, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(fb + o_adj, adjp.data(), sizeof(float *) * nw, hipMemcpyHostToDevice, stream));
+        setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+
+        // AMOD: forward modeling with nearest extraction and snapshots
+        StepArgs A = base_args(B);
+        A.dir = 1;
+        A.inject = 1;
+        A.extract = 1;
+        A.store_snap = 1;
+        A.snap_fp16 = need.snap_fp16;
+        HIPCHK(hipEventRecord(ev0, stream));
+        run_loop(B, A, false, max_nt(plans));
+        // residual, FP64 misfit per trace, adjoint source into the injection buffers
+        hipLaunchKernelGGL(residual_kernel, dim3(wtr), dim3(256), 0, stream, (const TraceRef *)(fb + o_refs), B.shots,
+                           d_dobs, mute ? (const float *)(fb + o_mute) : nullptr, (float *)(fb + o_res),
+                           (float *const *)(fb + o_adj), (double *)(fb + o_ft) + toff);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev1, stream));
+        // AGRAD: reverse time, forward stencil, simple injection, FWI imaging condition
+        zero_wavefields(B, plans);
+        for (int s = 0; s < nw; s++) {
+            ShotDev &S = B.shots_h[s];
+            B.injb[s].apply(S);
+        }
+        upload_shots(B);
+        StepArgs Ab = base_args(B);
+        Ab.dir = -1;
+        Ab.inject = 1;
+        Ab.snap_fp16 = need.snap_fp16;
+        Ab.imaging = IMG_FWI;
+        run_loop(B, Ab, false, max_nt(plans));
+        HIPCHK(hipEventRecord(ev2, stream));
+        // mod_copy_image into the full model (summed over waves in shot order);
+        // it waits for the stream, so the wave's host tables may go
+        accumulate_images(B, plans, d_grad, d_illu);
+        float ms1 = 0, ms2 = 0;
+        HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+        fwd_ms += ms1;
+        bwd_ms += ms2;
+        pts += 2 * count_pts(plans);
+        for (int s = 0; s < nw; s++) doff += (long)ws[s].nt * ws[s].ntr;
+        toff += wtr;
+    }
+    auto tp = std::chrono::steady_clock::now();
+
+    // post-processing chain
     if (raw_illum_host)   // the summed illumination before illupow and normalisation
         HIPCHK(hipMemcpyAsync(raw_illum_host, d_illu, N * 4, hipMemcpyDeviceToHost, stream));
     std::vector<double> ft(ntraces);
@@ -1634,55 +1953,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     if (resid_out) HIPCHK(hipMemcpyAsync(resid_out, fb + o_res, ndata * 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     *misfit = f;
-    float ms1 = 0, ms2 = 0;
-    HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
-    HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
-    timing[0] = ms1 + ms2; timing[1] = ms1; timing[2] = ms2;
-    timing[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    pts = 2 * count_pts(plans);
-}
-
-// mod_copy_image of every shot's window image / illumination into device
-// full-model buffers (zeroed here), deterministic shot order.
-void Engine::scatter_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill) {
-    std::vector<std::vector<int>> inv(g.nx);
-    for (int s = 0; s < B.ns; s++)
-        for (int j = 0; j < plans[s].nxw; j++) {
-            const int ix = plans[s].colmap_raw[j];
-            if (ix < 1 || ix > g.nx) continue;   // ExtractPadModel.f90:686
-            inv[ix - 1].push_back((s << 20) | j);
-        }
-    std::vector<int> ptr(g.nx + 1, 0), idx;
-    for (int ix = 0; ix < g.nx; ix++) {
-        ptr[ix + 1] = ptr[ix] + (int)inv[ix].size();
-        idx.insert(idx.end(), inv[ix].begin(), inv[ix].end());
-    }
-    const size_t N = (size_t)g.nz * g.nx;
-    Layout L;
-    const size_t o_ptr = L.take(ptr.size() * 4), o_idx = L.take(std::max<size_t>(idx.size(), 1) * 4);
-    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_ci2 = L.take(sizeof(CopyImg) * B.ns);
-    scratch.reserve(L.off);
-    char *base = (char *)scratch.p;
-    HIPCHK(hipMemcpyAsync(base + o_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, stream));
-    if (!idx.empty()) HIPCHK(hipMemcpyAsync(base + o_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, stream));
-    std::vector<CopyImg> ci(B.ns), ci2(B.ns);
-    for (int s = 0; s < B.ns; s++) {
-        ci[s] = {B.shots_h[s].image, nullptr, plans[s].nxw};
-        ci2[s] = {B.shots_h[s].illum, nullptr, plans[s].nxw};
-    }
-    HIPCHK(hipMemcpyAsync(base + o_ci, ci.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(base + o_ci2, ci2.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemsetAsync(d_img, 0, N * 4, stream));
-    HIPCHK(hipMemsetAsync(d_ill, 0, N * 4, stream));
-    hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
-                       (const CopyImg *)(base + o_ci), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
-                       (const int *)(base + o_idx), d_img);
-    hipLaunchKernelGGL(copy_image_kernel, dim3((g.nz + 255) / 256, g.nx), dim3(256), 0, stream,
-                       (const CopyImg *)(base + o_ci2), B.ns, g.nz, g.nx, (const int *)(base + o_ptr),
-                       (const int *)(base + o_idx), d_ill);
-    HIPCHK(hipGetLastError());
-    // the host copies above must outlive their async transfers
-    HIPCHK(hipStreamSynchronize(stream));
+    timing[0] = fwd_ms + bwd_ms; timing[1] = fwd_ms; timing[2] = bwd_ms;
+    timing[3] = setup_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
 }
 
 void ShotPlan::take_backward(const ShotPlan &o) {
@@ -1730,13 +2002,19 @@ int sfd_create(sfd_handle *h, const sfd_grid *g) {
     if (!h || !g) return SFD_EINVAL;
     *h = nullptr;
     try {
-        sfd_ctx *c = new sfd_ctx();
+        std::unique_ptr<sfd_ctx> c(new sfd_ctx());
         c->e = new Engine(*g);
-        *h = c;
+        *h = c.release();
         return SFD_OK;
     } catch (const EngineError &e) {
         g_create_err = e.msg;
         return e.code;
+    } catch (const std::bad_alloc &) {
+        g_create_err = "host allocation failed";
+        return SFD_ENOMEM;
+    } catch (const std::exception &e) {
+        g_create_err = e.what();
+        return SFD_EINVAL;
     }
 }
 
@@ -1849,6 +2127,9 @@ int sfd_set_tile(sfd_handle h, int tile_z, int tile_x) {
         if (!Engine::tile_supported(tile_z, tile_x)) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
         e.tz = tile_z;
         e.tx = tile_x;
+        // a strip width set for the previous shape that this one does not
+        // compile falls back to the shape's default
+        if (e.strip_px && !pick_kernel(e.tz, e.tx, false, false, 0, e.strip_px)) e.strip_px = 0;
         e.autotune_pending = false;
     });
 }
@@ -1906,15 +2187,39 @@ int sfd_set_option(sfd_handle h, int opt, int value) {
             e.time_block = value;
             break;
         case SFD_OPT_STRIP_PX:
-            if (value != 0 && value != 4 && value != 8 && value != 16) throw EngineError(SFD_EINVAL, "strip width 4, 8 or 16");
-            if (value == 4 && !(e.tz == 64 && e.tx == 32)) throw EngineError(SFD_EINVAL, "strip width 4 needs 64x32 tiles");
+            // compiled pairs: 64x32 tiles with 4- or 8-column strips, 64x64 with 16
+            if (value != 0 && !pick_kernel(e.tz, e.tx, false, false, 0, value))
+                throw EngineError(SFD_EINVAL, "strip width " + std::to_string(value) + " is not compiled for " +
+                                                  std::to_string(e.tz) + "x" + std::to_string(e.tx) +
+                                                  " tiles (64x32: 4 or 8; 64x64: 16)");
             e.strip_px = value;
+            break;
+        case SFD_OPT_BATCH_MIB:
+            if (value < 0) throw EngineError(SFD_EINVAL, "batch memory cap must be >= 0 MiB");
+            e.batch_bytes_cap = (size_t)value << 20;
             break;
         case SFD_OPT_KERNEL_EVENTS:
             if (value < 0) throw EngineError(SFD_EINVAL, "event sampling must be >= 0");
             e.event_every = value;
             break;
         default: throw EngineError(SFD_EINVAL, "unknown option");
+        }
+    });
+}
+
+int sfd_last_waves(sfd_handle h, int *waves) {
+    return guarded(h, [&](Engine &e) {
+        if (waves) *waves = e.waves_last;
+    });
+}
+
+int sfd_kernel_totals(sfd_handle h, double *us, double *bytes, double *pts, int *steps) {
+    return guarded(h, [&](Engine &e) {
+        for (int k = 0; k < 2; k++) {
+            if (us) us[k] = e.kstat_us[k];
+            if (bytes) bytes[k] = e.kstat_bytes[k];
+            if (pts) pts[k] = e.kstat_pts[k];
+            if (steps) steps[k] = e.kstat_n[k];
         }
     });
 }

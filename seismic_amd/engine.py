@@ -184,8 +184,9 @@ class Engine:
     # compute_fct_gdt
     def fwi_gradient(self, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
                      do_x=0, snap_mode=abi.SFD_SNAP_FP32):
+        """dobs None: the copy kept in HBM by set_traces."""
         nz, nx = self.g.nz, self.g.nx
-        ob = self._cat(dobs, shots)
+        ob = None if dobs is None else self._cat(dobs, shots)
         mu = None if mute is None else self._cat(mute, shots)
         vm = None if vpmask is None else _model_buf(vpmask, nz, nx)
         o = sfd_fwi_opts()
@@ -193,7 +194,7 @@ class Engine:
         f = C.c_double()
         grad = np.zeros(nz * nx, np.float32)
         illu = np.zeros(nz * nx, np.float32)
-        res = np.zeros_like(ob)
+        res = np.zeros(sum(s.nt * s.ntr for s in shots), np.float32)
         self._chk(self.lib.sfd_fwi_gradient(self.h, self._shots(shots), len(shots), _ptr(ob), _ptr(mu), _ptr(vm),
                                             C.byref(o), C.byref(f), _ptr(grad), _ptr(illu), _ptr(res)))
         return f.value, grad.reshape(nx, nz).T, illu.reshape(nx, nz).T, self._split(res, shots)
@@ -240,6 +241,20 @@ class Engine:
         n = (C.c_int * 2)()
         self._chk(self.lib.sfd_kernel_stats(self.h, us, n))
         return dict(fwd_us=us[0], bwd_us=us[1], fwd_n=n[0], bwd_n=n[1])
+
+    def last_waves(self):
+        """Shot waves the last call ran in (1: one batch)."""
+        w = C.c_int()
+        self._chk(self.lib.sfd_last_waves(self.h, C.byref(w)))
+        return w.value
+
+    def kernel_totals(self):
+        """Per pass (forward time, reverse time) of the last call: event-timed
+        kernel microseconds, algorithmic HBM bytes, FD point-updates, steps."""
+        us, by, pts = (C.c_double * 2)(), (C.c_double * 2)(), (C.c_double * 2)()
+        n = (C.c_int * 2)()
+        self._chk(self.lib.sfd_kernel_totals(self.h, us, by, pts, n))
+        return dict(us=[us[0], us[1]], bytes=[by[0], by[1]], pts=[pts[0], pts[1]], steps=[n[0], n[1]])
 
     def last_timing(self):
         ms = (C.c_double * 4)()
