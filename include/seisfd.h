@@ -115,6 +115,21 @@ int  sfd_forward(sfd_handle h, const sfd_shot *shots, int nshots,
                  float *traces_out, int extract_kind);
 
 /*
+ * Allocation-free forward modeling (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as sfd_forward, but the call allocates nothing: the
+ * batch (wavefield levels, model copies, plan tables, traces) lives in the
+ * library's device scratch array of 64 MiB (one per GPU, held until the
+ * process ends); launches are eager and untimed (no graphs, events or
+ * autotuning).  A batch larger than the array is SFD_ENOMEM with the sizes in
+ * the message.  The array is shared by every call on a GPU: calls that
+ * overlap on one GPU race.  The model must be set (sfd_set_model) before.
+ */
+int  sfd_forward_noalloc(sfd_handle h, const sfd_shot *shots, int nshots,
+                         float *traces_out, int extract_kind);
+
+/*
  * RTM of one or more shots (to_disk_to_memory_RTM.f90:358-492): forward with
  * snapshots every snapi steps, reference-style backward with the adjoint
  * stencil (FDderivatives.f90:847-891) injecting traces_in with

@@ -134,7 +134,12 @@ struct Engine {
     float *raw_illum_host = nullptr;      // fwi_gradient: also copy the summed illumination here
     bool tb_ok = true;                    // the current batch's region plans fit the two-step kernel
     Batch fwd_batch;
-    bool eager_untimed = false;           // no graphs, events or device step base (caller-memory batches)
+    // allocation-free forward modeling: its batch in the library's device
+    // scratch array (this handle's device's copy), eager untimed launches
+    Batch noalloc_batch;
+    char *scratch_dev = nullptr;
+    std::string scratch_err;
+    bool eager_untimed = false;
     DevBuf tmp_mem, fwi_mem, acc_mem;
     size_t batch_bytes_cap = 0;           // SFD_OPT_BATCH_BYTES (0: free device memory decides)
     int waves_last = 0;                   // shot waves of the last call

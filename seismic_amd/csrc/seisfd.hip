@@ -12,7 +12,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1971,6 +1973,35 @@ void second_difference(std::vector<float> &w, float dt) {
 }
 
 // ---------------------------------------------------------------------------
+// Allocation-free forward modeling.  Its batch (wavefield levels, model
+// copies, plan tables, traces) lives in this file-scope device array instead
+// of memory the engine allocates.  Each GPU the library runs on holds its own
+// copy at its own address until the process ends; the host-side name below is
+// a stand-in, not that address: host code uses only the per-device address
+// hipGetSymbolAddress returns (taken once per device by sfd_create, with the
+// device current).  Every call on a GPU shares that GPU's array, and the
+// array keeps the last call's contents: a call zeroes what must start at zero
+// before its first step, and calls that overlap on one GPU race.
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, char *> g_scratch_addr;   // device -> its copy of scratch
+
+static char *scratch_address(int dev, std::string &err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(dev);
+    if (it != g_scratch_addr.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr) {
+        err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc);
+        return nullptr;
+    }
+    g_scratch_addr[dev] = (char *)ptr;
+    return (char *)ptr;
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI
 
 struct sfd_ctx {
@@ -2004,6 +2035,10 @@ int sfd_create(sfd_handle *h, const sfd_grid *g) {
     try {
         std::unique_ptr<sfd_ctx> c(new sfd_ctx());
         c->e = new Engine(*g);
+        // the allocation-free entry point's array, for the device current now
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) c->e->scratch_dev = scratch_address(dev, c->e->scratch_err);
+        else c->e->scratch_err = "no current HIP device";
         *h = c.release();
         return SFD_OK;
     } catch (const EngineError &e) {
@@ -2050,6 +2085,16 @@ int sfd_forward(sfd_handle h, const sfd_shot *shots, int nshots, float *traces_o
     return guarded(h, [&](Engine &e) {
         if (!traces_out) throw EngineError(SFD_EINVAL, "traces_out is NULL");
         e.forward(shots, nshots, traces_out, extract_kind);
+    });
+}
+
+int sfd_forward_noalloc(sfd_handle h, const sfd_shot *shots, int nshots, float *traces_out, int extract_kind) {
+    return guarded(h, [&](Engine &e) {
+        if (!traces_out) throw EngineError(SFD_EINVAL, "traces_out is NULL");
+        if (!e.scratch_dev) throw EngineError(SFD_EHIP, "device scratch array unavailable: " + e.scratch_err);
+        e.noalloc_batch.ext_base = e.scratch_dev;
+        e.noalloc_batch.ext_cap = kScratchBytes;
+        e.forward(shots, nshots, traces_out, extract_kind, &e.noalloc_batch);
     });
 }
 
