@@ -20,11 +20,17 @@
  *   - FD coefficients: polynomial exactness (degree <= 9), closed form;
  *   - mksinc: shift-0 delta, unit sum; Higdon_coeff: closed formula;
  *   - imaging / FWI gradient / Born: brute-force sums over captured fields;
+ *     Born against the central difference of the pinned forward modeling;
+ *     LSRTM imaging against the exact discrete Born adjoint; the FWI
+ *     gradient against central differences of its misfit (illupow 0 and 1);
+ *     the illumination normalisation by its scale invariance and closed form
+ *     (tests/test_oracle_pins_imaging.py, tests/test_oracle_gradient.py);
  *   - triangle smoothing: impulse -> triangle, constant preservation;
- *   - L-BFGS (oracle/lbfgs_oracle.c): trajectory equal to lbfgs.f via _ref.
- * Parity unpinned: window/pad helpers (shot_window, window_pad, copy_image)
- * restate ExtractPadModel.f90/ReadSouVelRho.f90, which need SEPlib and cannot
- * be built here; they are checked against closed-form expectations only.
+ *   - L-BFGS (oracle/lbfgs_oracle.c): trajectory equal to lbfgs.f via _ref;
+ *   - window/pad helpers (shot_window, window_pad, copy_image), which restate
+ *     ExtractPadModel.f90/ReadSouVelRho.f90 (they need SEPlib and cannot be
+ *     built here): a table of windows, apertures, off-grid and off-model
+ *     columns derived by hand from the geometry.
  */
 #include <math.h>
 #include <stdlib.h>

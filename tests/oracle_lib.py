@@ -129,6 +129,19 @@ def padded_vel(g, vel, shot, prec="f"):
     return out.reshape(b.NX, b.NZ).T, oxw, nxw, b
 
 
+def copy_image(g, oxw, nxw, small, big=None, prec="d"):
+    """mod_copy_image: add the window image small (nz, nxw) into the model
+    image big (nz, nx), nearest index, skipping columns off the model."""
+    L = oracle()
+    T = _dt(prec)
+    sm = colmajor(small, T)
+    bg = colmajor(np.zeros((g.nz, g.nx)) if big is None else big, T)
+    fn = getattr(L, "orc_%s_copy_image" % prec)
+    fn.argtypes = [C.c_void_p, (C.c_float if prec == "f" else C.c_double), C.c_int, C.c_void_p, C.c_void_p]
+    fn(C.byref(g), oxw, nxw, ptr(sm), ptr(bg))
+    return bg.reshape(g.nx, g.nz).T
+
+
 def modeling(g, vel, shot, extract_kind=0, prec="f", snaps=False):
     L = oracle()
     T = _dt(prec)
