@@ -36,6 +36,7 @@ extern "C" {
 /* Snapshot storage for RTM / FWI source wavefields. */
 #define SFD_SNAP_FP32     0   /* reference behaviour: FP32 copies           */
 #define SFD_SNAP_FP16S    1   /* FP16, one max|S| scale per snapshot and step-kernel tile */
+#define SFD_SNAP_FP16G    2   /* FP16, one max|S| scale per snapshot (the whole window) */
 
 /* Receiver extraction semantics (libs/libwave/Extraction.f90). */
 #define SFD_EXTRACT_SINC_V2    0  /* Extraction_array_sinc: gain v^2 (:109-134)   */

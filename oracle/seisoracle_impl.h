@@ -168,9 +168,10 @@ static void CAT(fd_forward_, SFX)(PROP *P, const REAL *u2, REAL *u3)
 {
     const ORC_BOUNDS *b = &P->b;
     const REAL *c = P->c;
-    int i, j;
-    for (j = b->nmin2; j <= b->nmax2; j++)
-        for (i = b->nmin1; i <= b->nmax1; i++) {
+    /* columns are independent (no reduction): threads do not change results */
+#pragma omp parallel for schedule(static)
+    for (int j = b->nmin2; j <= b->nmax2; j++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
             REAL tz = c[0] * (AT(P, u2, i, j)) + c[1] * (AT(P, u2, i + 1, j) + AT(P, u2, i - 1, j)) +
                       c[2] * (AT(P, u2, i + 2, j) + AT(P, u2, i - 2, j)) +
                       c[3] * (AT(P, u2, i + 3, j) + AT(P, u2, i - 3, j)) +
@@ -190,15 +191,15 @@ static void CAT(fd_adjoint_, SFX)(PROP *P, const REAL *u2, REAL *u3)
 {
     const ORC_BOUNDS *b = &P->b;
     const REAL *c = P->c;
-    int i, j, k;
 #define V2(ii, jj) (AT(P, P->vel, ii, jj) * AT(P, P->vel, ii, jj))
-    for (j = b->nmin2; j <= b->nmax2; j++) {
-        int jp[5], jm[5];
+#pragma omp parallel for schedule(static)
+    for (int j = b->nmin2; j <= b->nmax2; j++) {
+        int jp[5], jm[5], k;
         for (k = 1; k <= 4; k++) {
             jp[k] = (j + k) < b->nmax2 ? (j + k) : b->nmax2;
             jm[k] = (j - k) > b->nmin2 ? (j - k) : b->nmin2;
         }
-        for (i = b->nmin1; i <= b->nmax1; i++) {
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
             int ip[5], im[5];
             for (k = 1; k <= 4; k++) {
                 ip[k] = (i + k) < b->nmax1 ? (i + k) : b->nmax1;
@@ -226,9 +227,9 @@ static void CAT(time_update_, SFX)(PROP *P, const REAL *u_1, const REAL *u0, con
 {
     const ORC_BOUNDS *b = &P->b;
     REAL div11 = (REAL)1. / (REAL)11, dt2 = P->dt * P->dt;
-    int i, j;
-    for (j = b->nmin2; j <= b->nmax2; j++)
-        for (i = b->nmin1; i <= b->nmax1; i++) {
+#pragma omp parallel for schedule(static)
+    for (int j = b->nmin2; j <= b->nmax2; j++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
             size_t q = ORC_IDX(b, i, j);
             u3[q] = ((REAL)20. * u2[q] - (REAL)6. * u1[q] - (REAL)4. * u0[q] + u_1[q] +
                      (REAL)12. * dt2 * u3[q]) * div11;
@@ -350,13 +351,13 @@ static void CAT(extract_, SFX)(PROP *P, const REAL *u, int it)
 /* Extraction.f90:562-601 Extraction_wavefield (surf_type 0) */
 static void CAT(extract_wave_, SFX)(PROP *P, const REAL *u, int it)
 {
-    int i, j;
     if (it % P->snapi != 0) return;
     P->wcounter++;
     if (P->wcounter > P->ntsnap) return;
     REAL *w = P->wave + (size_t)(P->wcounter - 1) * P->nz * P->nxw;
-    for (j = 1; j <= P->nxw; j++)
-        for (i = 1; i <= P->nz; i++) w[(i - 1) + (size_t)(j - 1) * P->nz] = AT(P, u, i, j);
+#pragma omp parallel for schedule(static)
+    for (int jj = 1; jj <= P->nxw; jj++)
+        for (int ii = 1; ii <= P->nz; ii++) w[(ii - 1) + (size_t)(jj - 1) * P->nz] = AT(P, u, ii, jj);
 }
 
 /* Imaging.f90:228-275 (RTM), :322-370 (FWI), :698-745 (LSRTM), :747-788 (Born) */
@@ -369,9 +370,10 @@ static void CAT(imaging_, SFX)(PROP *P, REAL *u, int it)
     if (P->imaging == 4) {
         /* Injection_Born: u3 += 2*taper*m*S(counter)/v in forward time order */
         const REAL *S = P->snaps + (size_t)(P->icounter - 1) * n;
-        for (j = 1; j <= P->nxw; j++) {
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j <= P->nxw; j++) {
             REAL t = P->taperx[j - 1];
-            for (i = 1; i <= P->nz; i++) {
+            for (int i = 1; i <= P->nz; i++) {
                 size_t q = (i - 1) + (size_t)(j - 1) * P->nz;
                 AT(P, u, i, j) = AT(P, u, i, j) + (REAL)2 * t * P->born_m[q] * S[q] / AT(P, P->vel, i, j);
             }
@@ -390,9 +392,11 @@ static void CAT(imaging_, SFX)(PROP *P, REAL *u, int it)
         for (j = 1; j <= P->nxw; j++)
             for (i = 1; i <= P->nz; i++) c[(i - 1) + (size_t)(j - 1) * P->nz] = AT(P, u, i, j);
     }
-    for (j = 1; j <= P->nxw; j++) {
+#pragma omp parallel for schedule(static)
+    for (int jj = 1; jj <= P->nxw; jj++) {
+        const int j = jj;
         REAL t = P->taperx[j - 1];
-        for (i = 1; i <= P->nz; i++) {
+        for (int i = 1; i <= P->nz; i++) {
             size_t q = (i - 1) + (size_t)(j - 1) * P->nz;
             REAL uu = AT(P, u, i, j);
             if (P->imaging == 1) {

@@ -21,6 +21,7 @@ SFD_ENOTFINITE = -7
 
 SFD_SNAP_FP32 = 0
 SFD_SNAP_FP16S = 1
+SFD_SNAP_FP16G = 2
 
 SFD_EXTRACT_SINC_V2 = 0
 SFD_EXTRACT_NEAREST = 1

@@ -68,6 +68,7 @@ struct BatchNeeds {
     bool backward_injection = false;
     bool snapshots = false;
     bool snap_fp16 = false;
+    bool snap_global = false;   // FP16 with one scale per snapshot (SFD_SNAP_FP16G)
     bool imaging = false;
     bool born = false;
 };
@@ -90,6 +91,7 @@ struct Batch {
     // prefix of the tiles: a launch covers only those (ragged batches).
     std::vector<int> order, nt_sorted, tiles_upto;
     const int *order_d = nullptr;
+    float *snap_part = nullptr;   // per-shot partial maxima of the per-snapshot FP16 scale
     int active_shots(int n) const;   // shots with nt > n (step n is in their record)
     int active_tiles(int n) const { return tiles_upto[active_shots(n)]; }
     // Reuse by a later call with the same plan (empty: none): the key of the

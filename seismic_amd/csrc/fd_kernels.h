@@ -136,7 +136,10 @@ struct StepArgs {
     int dir;                // +1 forward time, -1 reverse time
     int snapi;
     int store_snap;         // 1: store u2 as the snapshot of this time
-    int snap_fp16;          // snapshots are FP16 with a per-snapshot scale
+    int snap_fp16;          // snapshots are FP16 (u / scale)
+    int snap_global;        // FP16: one scale per snapshot (max |u| over the window, computed by
+                            // snap_max_kernel / snap_scale_kernel before the launch) instead of one per tile
+    float *snap_part;       // snap_global: per-shot partial maxima (kSnapParts per place of the batch)
     int imaging;            // IMG_*
     int extract;            // 1: run receiver extraction
     int inject;             // 1: run injection
@@ -642,7 +645,11 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // FP16 snapshots carry one scale per tile: max|u2| over the tile's window
     // cells at this step, so values are stored as u2/max in [-1, 1].
     float inv16 = 0.f;
-    if (snap_step && A.store_snap && A.snap_fp16) {
+    if (snap_step && A.store_snap && A.snap_fp16 && A.snap_global) {
+        // one scale per snapshot, already in every tile's slot
+        const float m = gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * (S.ntz * S.ntx) + t];
+        inv16 = m > 0.f ? 1.0f / m : 0.f;
+    } else if (snap_step && A.store_snap && A.snap_fp16) {
         float m = 0.f;
 #pragma unroll
         for (int p = 0; p < PX; p++) {
@@ -913,6 +920,58 @@ __global__ void higdon_tables_kernel(const float *__restrict__ vel, int NZ, int 
         nu = vel[(size_t)(NX - 5) * LDZ + zoff + iz] * dt * dxi;
         higdon_coeff(nu, hx + 16 * k + 8);
     }
+}
+
+// One FP16 scale per snapshot (SFD_SNAP_FP16G): max |u2| over a shot's window
+// interior at its snapshot steps, in two passes without atomics.  Pass 1:
+// grid (kSnapParts, running shots); block b of place z strides over the
+// window columns and writes its maximum to snap_part[z * kSnapParts + b].
+// Pass 2: one block per place reduces its partials and writes the scale
+// (0 below 1e-30, as the per-tile form) into every tile slot of the snapshot,
+// so the decoders read it as they read a per-tile scale.
+constexpr int kSnapParts = 64;
+__device__ __forceinline__ int snap_step_of(const StepArgs &A, const ShotDev &S, int *idx) {
+    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    int it = its;
+    if (A.nbase) {
+        const int b = *gp(A.nbase);
+        it = A.dir > 0 ? its + b : its - b;
+    }
+    if (its % A.snapi != 0 || it < 1 || it > S.nt) return 0;
+    *idx = it / A.snapi;
+    return 1;
+}
+__global__ void __launch_bounds__(256) snap_max_kernel(StepArgs A) {
+    const int z = blockIdx.y;
+    const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.shot_order)[z]];
+    int idx;
+    float m = 0.f;
+    if (snap_step_of(A, S, &idx)) {
+        const GLB float *u2 = gp(S.u[A.rot]);
+        for (int wx = blockIdx.x; wx < S.nxw; wx += gridDim.x) {
+            const size_t c = (size_t)(wx + S.i0) * S.LDZ + S.zoff + S.i0;
+            for (int wz = threadIdx.x; wz < S.nz; wz += blockDim.x) m = fmaxf(m, fabsf(u2[c + wz]));
+        }
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gp(A.snap_part)[(size_t)z * kSnapParts + blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(256) snap_scale_kernel(StepArgs A) {
+    const int z = blockIdx.x;
+    const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.shot_order)[z]];
+    int idx;
+    if (!snap_step_of(A, S, &idx)) return;
+    float m = 0.f;
+    for (int k = 0; k < kSnapParts; k++) m = fmaxf(m, gp(A.snap_part)[(size_t)z * kSnapParts + k]);
+    if (!(m > 1e-30f)) m = 0.f;
+    const int ntile = S.ntz * S.ntx;
+    for (int t = threadIdx.x; t < ntile; t += blockDim.x) gp(S.snap_scale)[(size_t)(idx - 1) * ntile + t] = m;
 }
 
 // Zero n bytes at p: a grid-stride fill with 16-byte vector stores over the

@@ -606,6 +606,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     B.nblocks = tile0;
     const size_t o_blk = put(L, up, upn, B.blk_shot_h);
     const size_t o_order = put(L, up, upn, B.order);
+    const size_t o_part = L.take(need.snap_global ? (size_t)kSnapParts * ns * sizeof(float) : 16);
     const size_t o_shots = L.take(sizeof(ShotDev) * ns);
     char *base;
     if (B.ext_base) {   // caller-provided memory (the allocation-free entry point)
@@ -718,6 +719,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
     }
     B.blk_shot = (const int *)(base + o_blk);
     B.order_d = (const int *)(base + o_order);
+    B.snap_part = (float *)(base + o_part);
     B.shots = (ShotDev *)(base + o_shots);
     B.ns = ns;
     upload_shots(B);
@@ -791,6 +793,7 @@ StepArgs Engine::base_args(const Batch &B) const {
     A.shots = B.shots;
     A.blk_shot = B.blk_shot;
     A.shot_order = B.order_d;
+    A.snap_part = B.snap_part;
     A.nblocks = B.nblocks;
     A.xcd_remap = xcd_remap;
     A.snapi = g.snapi;
@@ -886,6 +889,11 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
             grid += (groups + per_block - 1) / per_block;
         }
     }
+    if (mode == 1 && A.store_snap && A.snap_fp16 && A.snap_global) {
+        // one FP16 scale per snapshot: max |u2| over each window before the step stores it
+        hipLaunchKernelGGL(snap_max_kernel, dim3(kSnapParts, nact), dim3(256), 0, stream, A);
+        hipLaunchKernelGGL(snap_scale_kernel, dim3(nact), dim3(256), 0, stream, A);
+    }
     hipLaunchKernelGGL(f, dim3(grid), dim3(threads), 0, stream, A);
     if (image && !A.single) {
         const dim3 ig(A.img_gx, A.img_gy, nact);
@@ -899,7 +907,7 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 // The two-step kernel applies with 64x32 tiles, region plans within its trace
 // table, and no Born source term.
 bool Engine::use_step2(const StepArgs &A) const {
-    return time_block == 2 && tb_ok && tz == B2Z && tx == B2X && A.imaging != IMG_BORN;
+    return time_block == 2 && tb_ok && tz == B2Z && tx == B2X && A.imaging != IMG_BORN && !A.snap_global;
 }
 
 void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
@@ -969,7 +977,7 @@ double Engine::step_bytes(const Batch &B, const StepArgs &A, int n, bool two) co
             const int its = A.dir > 0 ? m + 1 : S.nt - m;
             if (its % A.snapi) continue;
             const double npix = (double)S.nz * (double)S.nxw;
-            if (A.store_snap) b += sb * npix;
+            if (A.store_snap) b += (A.snap_global ? sb + 4.0 : sb) * npix;   // + the scale's read of u2
             if (A.imaging == IMG_BORN) b += (sb + 8.0) * npix;   // snapshot, reflectivity, velocity
             else if (A.imaging != IMG_NONE) b += ((A.imaging == IMG_FWI ? 3.0 : 1.0) * sb + 16.0) * npix;
         }
@@ -1201,7 +1209,8 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
         }
         BatchNeeds need;
         need.snapshots = snaps >= 0;
-        need.snap_fp16 = snaps == SFD_SNAP_FP16S;
+        need.snap_fp16 = snaps == SFD_SNAP_FP16S || snaps == SFD_SNAP_FP16G;
+        need.snap_global = snaps == SFD_SNAP_FP16G;
         build_batch(fwd_batch, plans, need);
         for (int s = 0; s < ns; s++) upload_inj_traces(fwd_batch, s, shots[s].wavelet, shots[s].nt);
         traces_time_major(fwd_batch);
@@ -1211,6 +1220,7 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
         A.inject = 1;
         A.store_snap = need.snapshots;
         A.snap_fp16 = need.snap_fp16;
+        A.snap_global = need.snap_global;
         const bool two = use_step2(A);
         auto adv = [&](int n) {
             A.n = n; A.rot = n % kLevels;
@@ -1598,11 +1608,13 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
         if (resident_n == 0) throw EngineError(SFD_EINVAL, "traces_in is NULL and no traces are resident");
         if (total != resident_n) throw EngineError(SFD_EINVAL, "resident traces do not match the shots' nt * ntr");
     }
-    if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
+    if (snap_mode != SFD_SNAP_FP32 && snap_mode != SFD_SNAP_FP16S && snap_mode != SFD_SNAP_FP16G)
+        throw EngineError(SFD_EINVAL, "bad snap_mode");
     BatchNeeds need;
     need.backward_injection = true;
     need.snapshots = true;
-    need.snap_fp16 = snap_mode == SFD_SNAP_FP16S;
+    need.snap_fp16 = snap_mode == SFD_SNAP_FP16S || snap_mode == SFD_SNAP_FP16G;
+    need.snap_global = snap_mode == SFD_SNAP_FP16G;
     need.imaging = true;
     Batch &B = fwd_batch;
     const auto waves = plan_waves(shots, ns, need);
@@ -1654,6 +1666,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
         A.inject = 1;
         A.store_snap = 1;
         A.snap_fp16 = need.snap_fp16;
+        A.snap_global = need.snap_global;
         HIPCHK(hipEventRecord(ev0, stream));
         run_loop(B, A, false, max_nt(plans));
         HIPCHK(hipEventRecord(ev1, stream));
@@ -1780,7 +1793,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     check_shots(shots, ns, opt ? opt->snap_mode : -1);
     reset_kstats();
     if (!opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL opt/misfit/grad");
-    if (opt->snap_mode != SFD_SNAP_FP32 && opt->snap_mode != SFD_SNAP_FP16S) throw EngineError(SFD_EINVAL, "bad snap_mode");
+    if (opt->snap_mode != SFD_SNAP_FP32 && opt->snap_mode != SFD_SNAP_FP16S && opt->snap_mode != SFD_SNAP_FP16G)
+        throw EngineError(SFD_EINVAL, "bad snap_mode");
     if ((opt->do_z && opt->rect_z < 1) || (opt->do_x && opt->rect_x < 1)) throw EngineError(SFD_EINVAL, "bad smoothing length");
     if ((opt->do_z && opt->rect_z > g.nz) || (opt->do_x && opt->rect_x > g.nx))
         throw EngineError(SFD_EINVAL, "smoothing box longer than the model (Smoothing.f90:202 boxconv)");
@@ -1798,7 +1812,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     BatchNeeds need;
     need.backward_injection = true;
     need.snapshots = true;
-    need.snap_fp16 = opt->snap_mode == SFD_SNAP_FP16S;
+    need.snap_fp16 = opt->snap_mode == SFD_SNAP_FP16S || opt->snap_mode == SFD_SNAP_FP16G;
+    need.snap_global = opt->snap_mode == SFD_SNAP_FP16G;
     need.imaging = true;
     Batch &B = fwd_batch;
     const auto waves = plan_waves(shots, ns, need);
@@ -1873,6 +1888,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
         A.extract = 1;
         A.store_snap = 1;
         A.snap_fp16 = need.snap_fp16;
+        A.snap_global = need.snap_global;
         HIPCHK(hipEventRecord(ev0, stream));
         run_loop(B, A, false, max_nt(plans));
         // residual, FP64 misfit per trace, adjoint source into the injection buffers
