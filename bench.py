@@ -296,14 +296,23 @@ def main():
     log("device", info["name"], "CUs", info["cus"])
     b = batched_headline(sa, args)
     value = b["pts"] / b["wall"] / 1e9
-    traffic = None
+    tr = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("bytes_per_launch_fwd")
+            tr = json.load(f)
     roof = None
     if b["fwd"]:
+        # HBM traffic per launch: the calibrated FETCH_SIZE + WRITE_SIZE counters
+        # of the forward-pass launches (measured with every shot running, records
+        # capped at 400 steps) over their algorithmic bytes, applied to this
+        # run's algorithmic bytes per launch
+        traffic = round(tr["ratio_fwd"] * b["fwd"]["algorithmic_bytes_per_launch"]) if tr else None
         roof = {"bound": "hbm", "kernel": b["fwd"]["kernel"], "achieved": b["fwd"]["achieved"],
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b["fwd"]["frac"], "traffic": traffic,
+                "traffic_measured": None if not tr else {
+                    "bytes_per_launch": tr["bytes_per_launch_fwd"],
+                    "algorithmic_bytes_per_launch": tr["algorithmic_bytes_per_launch_fwd"],
+                    "ratio": tr["ratio_fwd"], "source": "profiles/fd_step_batched96_traffic.json"},
                 "algorithmic_bytes_per_launch": b["fwd"]["algorithmic_bytes_per_launch"],
                 "avg_launch_us": b["fwd"]["avg_launch_us"], "launches": b["fwd"]["launches"],
                 "reverse_time_pass": b["bwd"]}
