@@ -343,7 +343,7 @@ def main():
                       "shots": len(shots), "receivers": [int(min(s.ntr for s in shots)), int(max(s.ntr for s in shots))],
                       "nt": [int(min(s.nt for s in shots)), int(max(s.nt for s in shots))],
                       "nt_capped_at": args.nt_max or None, "shot_steps": steps_total,
-                      "snapshots": "fp16, one max|S| scale per snapshot and 64x32 tile",
+                      "snapshots": "fp16, one max|S| scale per snapshot and 64x4 block",
                       "inputs": "observed data resident in HBM (sfd_set_traces) before the timed region",
                       "plan": "every timed call plans its batch afresh",
                       "host_inputs_Gpts_per_s": round(b["host_rate"], 3),

@@ -35,7 +35,7 @@ extern "C" {
 
 /* Snapshot storage for RTM / FWI source wavefields. */
 #define SFD_SNAP_FP32     0   /* reference behaviour: FP32 copies           */
-#define SFD_SNAP_FP16S    1   /* FP16, one max|S| scale per snapshot and step-kernel tile */
+#define SFD_SNAP_FP16S    1   /* FP16, one max|S| scale per snapshot and 64 x 4 cell block */
 #define SFD_SNAP_FP16G    2   /* FP16, one max|S| scale per snapshot (the whole window) */
 
 /* Receiver extraction semantics (libs/libwave/Extraction.f90). */
@@ -320,6 +320,33 @@ typedef struct {
  * does not match its contents (truncated or damaged). */
 int  sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfgsoc_opts *o,
                       int n, double *x, double f, const double *g, int *iflag, int *info);
+
+/* One run of progs/inversion/L_BFGS.f90 with lbfgs_type 1 (:54-98) on the
+ * reference's own state files in directory dir, so that a job may alternate
+ * between that program and this call, or switch mid-inversion, and resume
+ * from the other's files.  Replaces the program's file handling
+ * (nlinv_read_array_params.f90:139-248, L_BFGS.f90:69-90) and LBFGSOC's and
+ * MCSRCHOC's state I/O (nlinv_lbfgs.f:275-281, :391-397, :509-565, :755-760,
+ * :854-905); the SEP files of x, f and g stay with the caller, who passes
+ * their values (the reference stores them as FP32).
+ *   Reads:  Iflag.dat, Info.dat (list-directed integers; absent = 0); for an
+ *           IFLAG other than 0: Diag.dat and WorkArray.dat (one sequential
+ *           unformatted record each, DIAG = n doubles, W = n(2m+1)+2m doubles:
+ *           q, rho, alpha, S, Y), LbfgsDat.dat (LBFGSOC's 33 scalars) and,
+ *           when its INFO is not 0 and IFLAG is 1, McsrchDat.dat (MCSRCHOC's
+ *           25 scalars).  IFLAG 2 (DIAGCO re-entry) is SFD_EINVAL.
+ *   Writes: LbfgsDatOut.dat and McsrchDatOut.dat where LBFGSOC and MCSRCHOC
+ *           write them; WorkArrayOut.dat, DiagOut.dat, IflagOut.dat and
+ *           InfoOut.dat (format I2) always.  Moving the *Out.dat files onto
+ *           the input names before the next run is the caller's, as with the
+ *           reference program.
+ *   Text scalars are written with the fewest digits that read back exactly;
+ *   records use 4-byte markers (8-byte ones are read too); a W record of
+ *   2 GiB or more is SFD_EINVAL.  iflag and info receive IFLAG and MYINFO
+ *   as written; x (n, in/out) becomes the next model.  Unreadable or
+ *   inconsistent files (another n or m, ONE/ZERO not 1/0) are SFD_EIO. */
+int  sfd_lbfgsoc_step_files(const char *dir, const sfd_lbfgsoc_opts *o, int n, double *x, double f,
+                            const double *g, int *iflag, int *info);
 
 /* ---- In-process L-BFGS driver (libs/libwave/Inversion.f90:29-189 l_bfgs) --
  * Options mirror InversionParam (Inversion_types.f90:46-58): niter (default

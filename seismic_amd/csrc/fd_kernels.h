@@ -109,7 +109,9 @@ struct ShotDev {
     int ext_lo, ext_hi;     // every extracting tile t lies in [ext_lo, ext_hi)
     // snapshots (window interior, nz*nxw per snapshot)
     void *snap;             // float* or __half*
-    float *snap_scale;      // FP16 mode: max|S| over each tile's window cells, ntz*ntx per snapshot
+    float *snap_scale;      // FP16 mode: max|S| over the window cells of each 64 x 4 block of the tile
+                            // grid (rows of tiles, 4-column groups), nsb per snapshot
+    int nsb;                // FP16 scale blocks per snapshot: ntz * ntx * (tile width / 4)
     // imaging / Born
     float *image, *illum;   // nz*nxw window accumulators
     const float *born_m;    // nz*nxw reflectivity
@@ -129,7 +131,6 @@ struct StepArgs {
     int step_grid;          // blocks of the launch that step tiles (the rest image, mode 3)
     int img_gx, img_gy;     // mode 3: column groups and 256-row chunks of the largest window
     int nshots;             // mode 3: shots of the batch
-    int img_txs;            // mode 3: log2 of the tile width (FP16 snapshot scale per tile)
     int n;                  // step counter from 0 (plus *nbase when nbase is set)
     const int *nbase;       // device-held step base of a replayed graph, or null
     int rot;                // n mod kLevels: u2 = U_n is u[rot]
@@ -138,7 +139,7 @@ struct StepArgs {
     int store_snap;         // 1: store u2 as the snapshot of this time
     int snap_fp16;          // snapshots are FP16 (u / scale)
     int snap_global;        // FP16: one scale per snapshot (max |u| over the window, computed by
-                            // snap_max_kernel / snap_scale_kernel before the launch) instead of one per tile
+                            // snap_max_kernel / snap_scale_kernel before the launch) instead of one per block
     float *snap_part;       // snap_global: per-shot partial maxima (kSnapParts per place of the batch)
     int imaging;            // IMG_*
     int extract;            // 1: run receiver extraction
@@ -169,13 +170,18 @@ __device__ __forceinline__ int logical_block(const StepArgs &A) {
 // out of the caches.  FWI imaging also reads the neighbouring snapshots
 // idx -+ 1 with cached loads; a snapshot's first read there is as a
 // neighbour, which keeps it for the two imaging steps that follow.
-__device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq, int t,
+// FP16 scale block of padded cell (iz, ix): 64 rows of a tile row by 4
+// columns, numbered down the tile rows first.
+__device__ __forceinline__ int snap_block(const ShotDev &S, int iz, int ix) {
+    return ((iz + S.zsh) >> 6) + S.ntz * ((ix + S.xsh) >> 2);
+}
+__device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq, int sb,
                                            bool stream) {
     const size_t q = (size_t)(idx - 1) * npix + wq;
     if (fp16) {
         const GLB unsigned short *p = gp((const unsigned short *)S.snap) + q;
         const unsigned short h = stream ? __builtin_nontemporal_load(p) : *p;
-        return __half2float(__ushort_as_half(h)) * gp(S.snap_scale)[(size_t)(idx - 1) * (S.ntz * S.ntx) + t];
+        return __half2float(__ushort_as_half(h)) * gp(S.snap_scale)[(size_t)(idx - 1) * S.nsb + sb];
     }
     const GLB float *p = gp((const float *)S.snap) + q;
     return stream ? __builtin_nontemporal_load(p) : *p;
@@ -226,7 +232,7 @@ __device__ __forceinline__ float image_value(int imaging, float img, float u2c, 
 // RTM or LSRTM as A.imaging says (nothing for other values).
 template <bool FWI>
 __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
-                                                 float u2c, float vel, int t, float inv16, float Sv, float img0,
+                                                 float u2c, float vel, int sb, float inv16, float Sv, float img0,
                                                  float ill0, float tp) {
     const int wz = iz - S.i0, wx = ix - S.i0;
     const size_t wq = (size_t)wx * S.nz + wz;
@@ -244,25 +250,12 @@ __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArg
         GLB float *imp = gp(S.image), *ilp = gp(S.illum);
         float Sm = 0.f, Sp = 0.f;
         if (FWI) {
-            Sm = load_snap(S, A.snap_fp16, idx - 1 > 1 ? idx - 1 : 1, npix, wq, t, false);
-            Sp = load_snap(S, A.snap_fp16, idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap, npix, wq, t, false);
+            Sm = load_snap(S, A.snap_fp16, idx - 1 > 1 ? idx - 1 : 1, npix, wq, sb, false);
+            Sp = load_snap(S, A.snap_fp16, idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap, npix, wq, sb, false);
         }
         imp[wq] = image_value<FWI>(A.imaging, img0, u2c, Sv, Sm, Sp, vel, tp, S.dt2snap);
         ilp[wq] = ill0 + Sv * Sv * tp;
     }
-}
-
-// Snapshot store at one interior point (Extraction.f90:595 on u2, before the
-// stencil), non-temporal: the snapshots are read back only in the reverse
-// pass, so they should not push the wavefield levels out of the caches.
-__device__ __forceinline__ void store_snap_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
-                                                 float u2c, float inv16) {
-    const size_t wq = (size_t)(ix - S.i0) * S.nz + (iz - S.i0);
-    const size_t q = (size_t)(it / A.snapi - 1) * S.nz * S.nxw + wq;
-    if (A.snap_fp16)
-        __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(u2c * inv16)), gp((unsigned short *)S.snap) + q);
-    else
-        __builtin_nontemporal_store(u2c, gp((float *)S.snap) + q);
 }
 
 // Imaging at one step of a reverse pass (Imaging.f90:228-275 RTM, :322-370
@@ -271,8 +264,7 @@ __device__ __forceinline__ void store_snap_point(const ShotDev &S, const StepArg
 // columns (group gx) and issue all loads of their cells before any use.
 // These blocks follow the step tiles in the launch (mode 3), so they start
 // in the slots the first finished tiles free and the step tiles keep their
-// occupancy.  The FP16 snapshot scale is per step tile: 64 rows by
-// 1 << img_txs columns.
+// occupancy.
 // Columns per thread, measured on the 751x2301 RTM and the 96-shot batch:
 // two for the imaging blocks of a one-shot launch, four for image_kernel.
 constexpr int kImgColsSingle = 2, kImgColsBatch = 4;
@@ -289,7 +281,7 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
         return;   // this shot has finished (ragged batch)
     }
     if (its % A.snapi != 0) return;
-    const int idx = it / A.snapi, iz = wz + S.i0, txs = A.img_txs;
+    const int idx = it / A.snapi, iz = wz + S.i0;
     const size_t npix = (size_t)S.nz * S.nxw;
     const GLB float *u2 = gp(S.u[A.rot]);
     const bool vel_needed = FWI || A.imaging == IMG_LSRTM;
@@ -300,7 +292,7 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
         const int ix = wx + S.i0;
         const size_t q = (size_t)wx * S.nz + wz;
         const size_t a = (size_t)ix * S.LDZ + S.zoff + iz;
-        const int t = ((iz + S.zsh) >> 6) + ((ix + S.xsh) >> txs) * S.ntz;
+        const int t = snap_block(S, iz, ix);
         u[k] = u2[a];
         Sv[k] = load_snap(S, A.snap_fp16, idx, npix, q, t, true);
         if (FWI) {
@@ -642,31 +634,36 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         // runs while the memory-to-LDS loads are still arriving
     }
 
-    // FP16 snapshots carry one scale per tile: max|u2| over the tile's window
-    // cells at this step, so values are stored as u2/max in [-1, 1].
-    float inv16 = 0.f;
-    if (snap_step && A.store_snap && A.snap_fp16 && A.snap_global) {
-        // one scale per snapshot, already in every tile's slot
-        const float m = gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * (S.ntz * S.ntx) + t];
-        inv16 = m > 0.f ? 1.0f / m : 0.f;
-    } else if (snap_step && A.store_snap && A.snap_fp16) {
-        float m = 0.f;
+    // FP16 snapshots carry one scale per 64 x 4 block (a wavefront's strip
+    // holds PX / 4 of them): max|u2| over the block's window cells at this
+    // step, so values are stored as u2/max in [-1, 1].  A wavefront reduction
+    // finds it: no LDS and no workgroup barrier on snapshot steps.
+    constexpr int NB = PX / 4;
+    float inv16[NB];
 #pragma unroll
-        for (int p = 0; p < PX; p++) {
-            const int ix = xs + p;
-            if (interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) m = fmaxf(m, fabsf(wu[p + 4]));
+    for (int q = 0; q < NB; q++) inv16[q] = 0.f;
+    if (snap_step && A.store_snap && A.snap_fp16) {
+        const size_t sbase = (size_t)(it / A.snapi - 1) * S.nsb + tzi + (size_t)S.ntz * ((xs + S.xsh) >> 2);
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            float m;
+            if (A.snap_global) {
+                // one scale per snapshot, already in every block's slot
+                m = gp(S.snap_scale)[sbase + (size_t)S.ntz * q];
+            } else {
+                m = 0.f;
+#pragma unroll
+                for (int p = 4 * q; p < 4 * q + 4; p++) {
+                    const int ix = xs + p;
+                    if (interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) m = fmaxf(m, fabsf(wu[p + 4]));
+                }
+                for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+                // blocks whose largest value is below 1e-30 store zeros (1/m would overflow FP32)
+                if (!(m > 1e-30f)) m = 0.f;
+                if (lane == 0) gp(S.snap_scale)[sbase + (size_t)S.ntz * q] = m;
+            }
+            inv16[q] = m > 0.f ? 1.0f / m : 0.f;
         }
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        __shared__ float s_red[NW];
-        if (lane == 0) s_red[wv] = m;
-        __syncthreads();
-        m = s_red[0];
-#pragma unroll
-        for (int w = 1; w < NW; w++) m = fmaxf(m, s_red[w]);
-        // tiles whose largest value is below 1e-30 store zeros (1/m would overflow FP32)
-        if (!(m > 1e-30f)) m = 0.f;
-        if (tid == 0) gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * (S.ntz * S.ntx) + t] = m;
-        inv16 = m > 0.f ? 1.0f / m : 0.f;
     }
     const float *c = A.c;
     float *col = s_col[wv];
@@ -690,8 +687,6 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         __builtin_amdgcn_wave_barrier();
         val[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
-        if (snap_step && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw)
-            store_snap_point(S, A, it, iz, ix, u2c, inv16);
         // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
 #define WX(k) (ADJ ? wv2[p + 4 + (k)] : wu[p + 4 + (k)])
         const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
@@ -699,6 +694,24 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #undef WX
         val[p] = ADJ ? (tx + tz) : rv[p] * (tx + tz);
     }
+    // Snapshot store (Extraction.f90:595, u2 of this step on the window),
+    // non-temporal: the snapshots are read back only in the reverse pass, so
+    // they should not push the wavefield levels out of the caches.
+    if (snap_step && interior_z) {
+        const size_t sbase = (size_t)(it / A.snapi - 1) * S.nz * S.nxw + (iz - S.i0);
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            const int ix = xs + p;
+            if (ix < S.i0 || ix >= S.i0 + S.nxw) continue;
+            const size_t q = sbase + (size_t)(ix - S.i0) * S.nz;
+            if (A.snap_fp16)
+                __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(wu[p + 4] * inv16[p / 4])),
+                                            gp((unsigned short *)S.snap) + q);
+            else
+                __builtin_nontemporal_store(wu[p + 4], gp((float *)S.snap) + q);
+        }
+    }
+
     // Injection: per-cell contributions in trace order (Injection.f90:434-440).
     // Only FD cells carry contributions (the plan rejects footprints outside).
     if (has_inj) {
@@ -782,7 +795,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         if (IMG && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
             const int wz = iz - S.i0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
-            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, t, true);
+            const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, snap_block(S, iz, ix), true);
             v = v + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
@@ -927,8 +940,8 @@ __global__ void higdon_tables_kernel(const float *__restrict__ vel, int NZ, int 
 // grid (kSnapParts, running shots); block b of place z strides over the
 // window columns and writes its maximum to snap_part[z * kSnapParts + b].
 // Pass 2: one block per place reduces its partials and writes the scale
-// (0 below 1e-30, as the per-tile form) into every tile slot of the snapshot,
-// so the decoders read it as they read a per-tile scale.
+// (0 below 1e-30, as the per-block form) into every block slot of the
+// snapshot, so the decoders read it as they read a per-block scale.
 constexpr int kSnapParts = 64;
 __device__ __forceinline__ int snap_step_of(const StepArgs &A, const ShotDev &S, int *idx) {
     const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
@@ -970,8 +983,7 @@ __global__ void __launch_bounds__(256) snap_scale_kernel(StepArgs A) {
     float m = 0.f;
     for (int k = 0; k < kSnapParts; k++) m = fmaxf(m, gp(A.snap_part)[(size_t)z * kSnapParts + k]);
     if (!(m > 1e-30f)) m = 0.f;
-    const int ntile = S.ntz * S.ntx;
-    for (int t = threadIdx.x; t < ntile; t += blockDim.x) gp(S.snap_scale)[(size_t)(idx - 1) * ntile + t] = m;
+    for (int b = threadIdx.x; b < S.nsb; b += blockDim.x) gp(S.snap_scale)[(size_t)(idx - 1) * S.nsb + b] = m;
 }
 
 // Zero n bytes at p: a grid-stride fill with 16-byte vector stores over the

@@ -73,22 +73,25 @@ template <typename U>
 __device__ __forceinline__ void snap_image_tile(const ShotDev &S, const StepArgs &A, int t, int it, int z0, int x0,
                                                 float *s_red, U u) {
     const int tid = threadIdx.x;
-    float inv16 = 0.f;
+    // FP16 scales per 64 x 4 block: column maxima by wavefront reductions
+    // (a wavefront's 64 lanes walk one tile column), then the four columns
+    // of each block; s_red holds B2X column maxima and B2X / 4 block scales.
     if (A.store_snap && A.snap_fp16) {
-        float m = 0.f;
         for (int l = tid; l < B2Z * B2X; l += kThreads2) {
             const int iz = z0 + l % B2Z, ix = x0 + l / B2Z;
-            if (iz >= S.i0 && iz < S.i0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw) m = fmaxf(m, fabsf(u(l % B2Z, l / B2Z)));
+            float m = 0.f;
+            if (iz >= S.i0 && iz < S.i0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw) m = fabsf(u(l % B2Z, l / B2Z));
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if ((tid & 63) == 0) s_red[l / B2Z] = m;
         }
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if ((tid & 63) == 0) s_red[tid >> 6] = m;
         __syncthreads();
-        m = s_red[0];
-#pragma unroll
-        for (int w = 1; w < kThreads2 / 64; w++) m = fmaxf(m, s_red[w]);
-        if (!(m > 1e-30f)) m = 0.f;
-        if (tid == 0) gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * (S.ntz * S.ntx) + t] = m;
-        inv16 = m > 0.f ? 1.0f / m : 0.f;
+        if (tid < B2X / 4) {
+            float m = fmaxf(fmaxf(s_red[4 * tid], s_red[4 * tid + 1]), fmaxf(s_red[4 * tid + 2], s_red[4 * tid + 3]));
+            if (!(m > 1e-30f)) m = 0.f;
+            gp(S.snap_scale)[(size_t)(it / A.snapi - 1) * S.nsb + snap_block(S, z0, x0 + 4 * tid)] = m;
+            s_red[B2X + tid] = m;
+        }
+        __syncthreads();
     }
     const bool img = A.imaging == IMG_RTM || A.imaging == IMG_FWI || A.imaging == IMG_LSRTM;
     const size_t npix = (size_t)S.nz * S.nxw;
@@ -96,18 +99,24 @@ __device__ __forceinline__ void snap_image_tile(const ShotDev &S, const StepArgs
         const int iz = z0 + l % B2Z, ix = x0 + l / B2Z;
         if (!(iz >= S.i0 && iz < S.i0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw)) continue;
         float Sv = 0.f, img0 = 0.f, ill0 = 0.f, tp = 0.f, vel = 0.f;
+        const int sb = snap_block(S, iz, ix);
+        float inv16 = 0.f;
+        if (A.store_snap && A.snap_fp16) {
+            const float m = s_red[B2X + (l / B2Z) / 4];
+            inv16 = m > 0.f ? 1.0f / m : 0.f;
+        }
         if (img) {
             const size_t wq = (size_t)(ix - S.i0) * S.nz + (iz - S.i0);
-            Sv = load_snap(S, A.snap_fp16, it / A.snapi, npix, wq, t, true);
+            Sv = load_snap(S, A.snap_fp16, it / A.snapi, npix, wq, sb, true);
             img0 = gp(S.image)[wq];
             ill0 = gp(S.illum)[wq];
             tp = gp(S.taper)[ix - S.i0];
             if (A.imaging == IMG_FWI || A.imaging == IMG_LSRTM) vel = gp(S.vel)[(size_t)ix * S.LDZ + S.zoff + iz];
         }
         if (A.imaging == IMG_FWI)
-            snap_image_point<true>(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, t, inv16, Sv, img0, ill0, tp);
+            snap_image_point<true>(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, sb, inv16, Sv, img0, ill0, tp);
         else
-            snap_image_point<false>(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, t, inv16, Sv, img0, ill0, tp);
+            snap_image_point<false>(S, A, it, iz, ix, u(l % B2Z, l / B2Z), vel, sb, inv16, Sv, img0, ill0, tp);
     }
 }
 
@@ -210,7 +219,7 @@ __device__ __forceinline__ void fd_step2_body(const StepArgs &A, const ShotDev &
     __shared__ float s_d[B2X * B2Z];             // U_{n+2}
     __shared__ int s_inj[R1X * R1Z];             // per region cell: (first contribution << 8) | count
     __shared__ float s_trv[2][kMaxTileTraces2];  // both steps' trace samples
-    __shared__ float s_red[NT / 64];
+    __shared__ float s_red[B2X + B2X / 4];   // FP16 snapshot scales: column maxima, block scales
 
     const int its1 = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     int it1 = its1;

@@ -583,7 +583,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         const size_t npix = (size_t)g.nz * p.nxw;
         o.snap_bytes = need.snapshots ? (size_t)std::max(p.ntsnap, 1) * npix * (need.snap_fp16 ? 2 : 4) : 16;
         o.snap = L.take(o.snap_bytes);
-        o.snap_scale = L.take(need.snap_fp16 ? (size_t)std::max(p.ntsnap, 1) * p.ntz * p.ntx * sizeof(float) : 16);
+        o.snap_scale = L.take(need.snap_fp16 ? (size_t)std::max(p.ntsnap, 1) * p.ntz * p.ntx * (tx / 4) * sizeof(float)
+                                             : 16);
         o.image = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.illum = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.born_m = L.take(need.born ? npix * sizeof(float) : 16);
@@ -710,6 +711,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         tile_range(p.ext_tile, S.ext_lo, S.ext_hi);
         S.snap = base + o.snap;
         S.snap_scale = need.snap_fp16 ? (float *)(base + o.snap_scale) : nullptr;
+        S.nsb = p.ntz * p.ntx * (tx / 4);
         S.image = (float *)(base + o.image);
         S.illum = (float *)(base + o.illum);
         S.born_m = (const float *)(base + o.born_m);
@@ -882,8 +884,6 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         A.img_gx = (nxw + cols - 1) / cols;
         A.img_gy = (g.nz + 255) / 256;
         A.nshots = B.ns;
-        A.img_txs = 0;
-        while ((1 << A.img_txs) < tx) A.img_txs++;
         if (A.single) {
             const int groups = A.img_gx * A.img_gy, per_block = threads / 256;
             grid += (groups + per_block - 1) / per_block;
@@ -1268,7 +1268,7 @@ size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
     b += 3 * nt * ntr * 4;                                    // injected (two layouts) and extracted traces
     b += ntr * 49 * 16 * 4 + ntr * 16 * 4 + ntile * 64;       // contributions (both sets, two-step regions)
     b += (ntr + 4) * 2 * 64 * 32 * 4;                         // dense cell maps of the injecting tiles
-    if (need.snapshots) b += ntsnap * npix * (need.snap_fp16 ? 2 : 4) + (need.snap_fp16 ? ntsnap * ntile * 4 : 0);
+    if (need.snapshots) b += ntsnap * npix * (need.snap_fp16 ? 2 : 4) + (need.snap_fp16 ? ntsnap * ntile * 16 * 4 : 0);
     if (need.imaging) b += 2 * npix * 4;
     if (need.born) b += npix * 4;
     return b + (b >> 4) + (1 << 20);

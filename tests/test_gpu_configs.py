@@ -57,7 +57,7 @@ def test_rtm751x2301_fp32_matches_oracle_and_fp16_forms_vs_fp64():
     """Observed gather and FP32-snapshot image against the FP32 oracle (bar:
     relative L2 <= 1e-5; bitwise reported), then the image error of each
     snapshot storage form against the FP64 oracle."""
-    say("case: rtm 751x2301 nt 7500 (observed data, FP32 / FP16 per tile / FP16 per snapshot)")
+    say("case: rtm 751x2301 nt 7500 (observed data, FP32 / FP16 per 64x4 block / FP16 per snapshot)")
     fx = fixture("rtm751x2301")
     w = syn.rtm_751x2301_workload()
     e = engine(w)
@@ -79,7 +79,7 @@ def test_rtm751x2301_fp32_matches_oracle_and_fp16_forms_vs_fp64():
     assert rel(il32[DEC_IMG], fx["illum32_dec"]) <= 1e-5
     ref64 = fx["image64_dec"]
     errs = {"fp32": rel(im32[DEC_IMG], ref64)}
-    for name, mode in (("fp16_tile", sa.SFD_SNAP_FP16S), ("fp16_snapshot", sa.SFD_SNAP_FP16G)):
+    for name, mode in (("fp16_block", sa.SFD_SNAP_FP16S), ("fp16_snapshot", sa.SFD_SNAP_FP16G)):
         im, _ = e.rtm(s, None, snap_mode=mode)
         errs[name] = rel(im[DEC_IMG], ref64)
     say("image rel L2 vs the FP64 oracle (every 8th depth and column): " +
@@ -88,7 +88,7 @@ def test_rtm751x2301_fp32_matches_oracle_and_fp16_forms_vs_fp64():
     # fixture's image norms differ by 1.6e-4); the FP16 forms must not add
     # more than a few times that
     assert errs["fp32"] <= 1e-2
-    assert errs["fp16_tile"] <= 3e-2 and errs["fp16_snapshot"] <= 5e-2
+    assert errs["fp16_block"] <= 3e-2 and errs["fp16_snapshot"] <= 5e-2
 
 
 def test_batched96_gathers_and_gradient_sums():
@@ -203,7 +203,7 @@ def test_salt_fwi_gradient_properties():
     _, g32, _ = fg(v0, sa.SFD_SNAP_FP32)
     _, g16s, _ = fg(v0, sa.SFD_SNAP_FP16S)
     e16g, e16s = rel(g, g32), rel(g16s, g32)
-    say("gradient rel L2 vs FP32 snapshots: FP16 per snapshot %.3e, FP16 per tile %.3e" % (e16g, e16s))
+    say("gradient rel L2 vs FP32 snapshots: FP16 per snapshot %.3e, FP16 per block %.3e" % (e16g, e16s))
     assert e16g <= 5e-2 and e16s <= 5e-2
     e.set_option(abi.SFD_OPT_BATCH_MIB, 12000)
     fw, gw, _ = fg(v0)
