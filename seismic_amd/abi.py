@@ -118,7 +118,7 @@ EXPORTS = [
     "sfd_born_adjoint_exact", "sfd_lsrtm",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
     "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
-    "sfd_lbfgsoc_step", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
+    "sfd_lbfgsoc_step", "sfd_lbfgsoc_step_files", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
 ]
@@ -170,6 +170,8 @@ def load():
     lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
     lib.sfd_lbfgsoc_step.argtypes = [C.c_char_p, C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p,
                                      C.c_double, C.c_void_p, P(C.c_int), P(C.c_int)]
+    lib.sfd_lbfgsoc_step_files.argtypes = [C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p, C.c_double,
+                                           C.c_void_p, P(C.c_int), P(C.c_int)]
     lib.sfd_lbfgs_run.argtypes = [P(sfd_lbfgs_opts), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                   SFD_FCTGDT, C.c_void_p, P(sfd_lbfgs_report)]
     lib.sfd_fwi_lbfgs.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -13,6 +13,7 @@
 
 #include "../../include/seisfd.h"
 #include "common.h"
+#include "ftn_io.h"
 #include "lbfgs_dev.h"
 #include "lbfgs_host.h"
 
@@ -225,28 +226,32 @@ L30:
     axpby(stp, s, 1.0, x);
     if (boundt == 1) freeze(x);
     info = -1;
+    if (on_mcsrch_write) on_mcsrch_write();
     return;
 L45:
     info = 0;
     nfev = nfev + 1;
     {
-        const double dg = dot(g, s);
-        const double ftest1 = finit + stp * dgtest;
+        dg = dot(g, s);
+        ftest1 = finit + stp * dgtest;
         if ((brackt && (stp <= stmin || stp >= stmax)) || infoc == 0) info = 6;
         if (stp == stpmax && f <= ftest1 && dg <= dgtest) info = 5;
         if (stp == stpmin && (f > ftest1 || dg >= dgtest)) info = 4;
         if (nfev >= maxfev) info = 3;
         if (brackt && stmax - stmin <= xtol * stmax) info = 2;
         if (f <= ftest1 && fabs(dg) <= gtol * (-dginit)) info = 1;
-        if (info != 0) return;
+        if (info != 0) {
+            if (on_mcsrch_write) on_mcsrch_write();
+            return;
+        }
         if (stage1 && f <= ftest1 && dg >= std::min(ftol, gtol) * dginit) stage1 = 0;
         if (stage1 && f <= fx && f > ftest1) {
-            double fm = f - stp * dgtest;
-            double fxm = fx - stx * dgtest;
-            double fym = fy - sty * dgtest;
-            double dgm = dg - dgtest;
-            double dgxm = dgx - dgtest;
-            double dgym = dgy - dgtest;
+            fm = f - stp * dgtest;
+            fxm = fx - stx * dgtest;
+            fym = fy - sty * dgtest;
+            dgm = dg - dgtest;
+            dgxm = dgx - dgtest;
+            dgym = dgy - dgtest;
             mcstep(&stx, &fxm, &dgxm, &sty, &fym, &dgym, &stp, fm, dgm, &brackt, stmin, stmax, &infoc);
             fx = fxm + stx * dgtest;
             fy = fym + sty * dgtest;
@@ -278,17 +283,21 @@ void LbfgsDev::two_loop(bool use_h0) {
     for (int i = 1; i <= bound; i++) {
         cp = cp - 1;
         if (cp == -1) cp = m - 1;
-        const double sq = dot(Sv(cp), q);
+        sq = dot(Sv(cp), q);
         alpha[cp] = rho[cp] * sq;
         axpby(-alpha[cp], Yv(cp), 1.0, q);
+        inmc = (int)(n + m + cp + 1);                 // the reference's W indexes, 1-based
+        iycn = (int)(n + 2 * m + (long)m * n + (long)cp * n);
     }
     if (use_h0) h0(h0_ctx, q, n, st);     // learned initial inverse Hessian
     else hipLaunchKernelGGL(mul_kernel, vgrid(n), dim3(256), 0, st, n, (const double *)diag, q);
     for (int i = 1; i <= bound; i++) {
-        const double yr = dot(Yv(cp), q);
-        double beta = rho[cp] * yr;
+        yr = dot(Yv(cp), q);
+        beta = rho[cp] * yr;
         beta = alpha[cp] - beta;
         axpby(beta, Sv(cp), 1.0, q);
+        inmc = (int)(n + m + cp + 1);
+        iscn = (int)(n + 2 * m + (long)cp * n);
         cp = cp + 1;
         if (cp == m) cp = 0;
     }
@@ -363,6 +372,7 @@ L172:
     mcsrch(f, Sv(point));
     if (info == -1) {
         *iflag = 1;
+        if (on_lbfgs_write) on_lbfgs_write();
         return;
     }
     if (info != 1) {
@@ -380,15 +390,15 @@ L172:
     point = point + 1;
     if (point == m) point = 0;
     gnorm = sqrt(dot(g, g));
-    {
-        double xnorm = sqrt(dot(x, x));
-        xnorm = std::max(1.0, xnorm);
-        if (gnorm / xnorm <= eps) finish = 1;
-    }
+    xnorm = sqrt(dot(x, x));
+    xnorm = std::max(1.0, xnorm);
+    if (gnorm / xnorm <= eps) finish = 1;
     if (finish) {
         *iflag = 0;
+        if (on_lbfgs_write) on_lbfgs_write();
         return;
     }
+    if (on_lbfgs_write) on_lbfgs_write();
     goto L80;
 }
 
@@ -650,6 +660,165 @@ int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, co
 }
 
 // ---------------------------------------------------------------------------
+// The state files of progs/inversion/L_BFGS.f90 with lbfgs_type 1: LBFGSOC's
+// and MCSRCHOC's SAVE scalars in the list-directed text files LbfgsDat.dat and
+// McsrchDat.dat (nlinv_lbfgs.f:276-281, :755-760), W and DIAG as one
+// unformatted record each (WorkArray.dat, Diag.dat; L_BFGS.f90:69-78 and
+// nlinv_read_array_params.f90:237-248), IFLAG and MYINFO as one integer each
+// (Iflag.dat, Info.dat).  Reads take the input names, writes the *Out names
+// (nlinv_read_array_params.f90:139-151).  The record order of the two text
+// files and of W (q, rho, alpha, the m steps S, the m gradient changes Y) is
+// the reference's, so either program can resume from the other's files.
+namespace lbx {
+
+void read_lbfgs_dat(LbfgsDev &d, const std::string &path, bool check_layout) {
+    ftn::ListReader r(path);
+    double one = 1.0, zero = 0.0;
+    int ispt = 0, iypt = 0, npt_ref = 0, i_ = 0;
+    r.statement();
+    r.real(&d.gtol); r.real(&one); r.real(&zero); r.real(&d.gnorm); r.real(&d.stp1); r.real(&d.ftol);
+    r.real(&d.stpmin);
+    r.statement();
+    r.real(&d.stpmax); r.real(&d.stp); r.real(&d.ys); r.real(&d.yy); r.real(&d.sq); r.real(&d.yr);
+    r.real(&d.beta); r.real(&d.xnorm);
+    r.statement();
+    r.integer(&d.mp); r.integer(&d.lp); r.integer(&d.iter); r.integer(&d.nfun); r.integer(&d.point);
+    r.integer(&ispt); r.integer(&iypt); r.integer(&d.maxfev); r.integer(&d.info);
+    r.statement();
+    r.integer(&d.bound); r.integer(&npt_ref); r.integer(&d.cp); r.integer(&i_); r.integer(&d.nfev);
+    r.integer(&d.inmc); r.integer(&d.iycn); r.integer(&d.iscn);
+    r.statement();
+    r.logical(&d.finish);
+    // ONE and ZERO are variables of LBFGSOC that the file overwrites
+    if (one != 1.0 || zero != 0.0) throw EngineError(SFD_EIO, path + ": ONE and ZERO are not 1 and 0");
+    if (!check_layout) return;
+    const long n = d.n, m = d.m;
+    if (ispt != n + 2 * m || iypt != n + 2 * m + n * m)
+        throw EngineError(SFD_EIO, path + ": ISPT/IYPT were written for another N or M");
+    if (npt_ref % n != 0 || npt_ref / n >= m || d.point < 0 || d.point >= m || d.cp < 0 || d.cp > m)
+        throw EngineError(SFD_EIO, path + ": NPT, POINT or CP out of range");
+    d.npt = (int)(npt_ref / n);   // the reference's NPT is the offset POINT*N
+}
+
+void write_lbfgs_dat(const LbfgsDev &d, const std::string &path) {
+    ftn::ListWriter w;
+    const long n = d.n, m = d.m;
+    w.real(d.gtol); w.real(1.0); w.real(0.0); w.real(d.gnorm); w.real(d.stp1); w.real(d.ftol); w.real(d.stpmin);
+    w.end_record();
+    w.real(d.stpmax); w.real(d.stp); w.real(d.ys); w.real(d.yy); w.real(d.sq); w.real(d.yr); w.real(d.beta);
+    w.real(d.xnorm);
+    w.end_record();
+    w.integer(d.mp); w.integer(d.lp); w.integer(d.iter); w.integer(d.nfun); w.integer(d.point);
+    w.integer((int)(n + 2 * m)); w.integer((int)(n + 2 * m + n * m)); w.integer(d.maxfev); w.integer(d.info);
+    w.end_record();
+    // I is LBFGSOC's loop index, N + 1 after the loops that precede every write
+    w.integer(d.bound); w.integer((int)(d.npt * n)); w.integer(d.cp); w.integer((int)(n + 1)); w.integer(d.nfev);
+    w.integer(d.inmc); w.integer(d.iycn); w.integer(d.iscn);
+    w.end_record();
+    w.logical(d.finish);
+    w.end_record();
+    w.save(path);
+}
+
+void read_mcsrch_dat(LbfgsDev &d, const std::string &path) {
+    ftn::ListReader r(path);
+    int j = 0;
+    r.statement();
+    r.integer(&d.infoc); r.integer(&j);
+    r.statement();
+    r.logical(&d.brackt); r.logical(&d.stage1);
+    r.statement();
+    r.real(&d.dg); r.real(&d.dgm); r.real(&d.dginit); r.real(&d.dgtest); r.real(&d.dgx); r.real(&d.dgxm);
+    r.real(&d.dgy); r.real(&d.dgym);
+    r.statement();
+    r.real(&d.finit); r.real(&d.ftest1); r.real(&d.fm); r.real(&d.fx); r.real(&d.fxm); r.real(&d.fy); r.real(&d.fym);
+    r.real(&d.stx); r.real(&d.sty);
+    r.statement();
+    r.real(&d.stmin); r.real(&d.stmax); r.real(&d.width); r.real(&d.width1);
+}
+
+void write_mcsrch_dat(const LbfgsDev &d, const std::string &path) {
+    ftn::ListWriter w;
+    // J is MCSRCHOC's loop index, N + 1 after the loops that precede every write
+    w.integer(d.infoc); w.integer((int)(d.n + 1));
+    w.end_record();
+    w.logical(d.brackt); w.logical(d.stage1);
+    w.end_record();
+    w.real(d.dg); w.real(d.dgm); w.real(d.dginit); w.real(d.dgtest); w.real(d.dgx); w.real(d.dgxm); w.real(d.dgy);
+    w.real(d.dgym);
+    w.end_record();
+    w.real(d.finit); w.real(d.ftest1); w.real(d.fm); w.real(d.fx); w.real(d.fxm); w.real(d.fy); w.real(d.fym);
+    w.real(d.stx); w.real(d.sty);
+    w.end_record();
+    w.real(d.stmin); w.real(d.stmax); w.real(d.width); w.real(d.width1);
+    w.end_record();
+    w.save(path);
+}
+
+// W(1:N) = q, W(N+1:N+M) = rho, W(N+M+1:N+2M) = alpha, then S and Y
+// (nlinv_lbfgs.f:308-322), staged through one host vector of n values.
+void read_work(LbfgsDev &d, const std::string &path) {
+    const size_t n = d.n, m = d.m;
+    ftn::RecordReader r(path, n * (2 * m + 1) + 2 * m);
+    std::vector<double> buf(n);
+    auto get = [&](double *dv) {
+        r.read(buf.data(), n);
+        HIPCHK(hipMemcpyAsync(dv, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, d.st));
+        HIPCHK(hipStreamSynchronize(d.st));
+    };
+    get(d.q);
+    r.read(d.rho.data(), m);
+    r.read(d.alpha.data(), m);
+    for (size_t k = 0; k < m; k++) get(d.Sv((int)k));
+    for (size_t k = 0; k < m; k++) get(d.Yv((int)k));
+}
+
+void write_work(const LbfgsDev &d, const std::string &path) {
+    const size_t n = d.n, m = d.m;
+    ftn::RecordWriter w(path, n * (2 * m + 1) + 2 * m);
+    std::vector<double> buf(n);
+    auto put = [&](const double *dv) {
+        HIPCHK(hipMemcpyAsync(buf.data(), dv, sizeof(double) * n, hipMemcpyDeviceToHost, d.st));
+        HIPCHK(hipStreamSynchronize(d.st));
+        w.write(buf.data(), n);
+    };
+    put(d.q);
+    w.write(d.rho.data(), m);
+    w.write(d.alpha.data(), m);
+    for (size_t k = 0; k < m; k++) put(d.Sv((int)k));
+    for (size_t k = 0; k < m; k++) put(d.Yv((int)k));
+    w.close();
+}
+
+void read_vector(LbfgsDev &d, double *dv, const std::string &path) {
+    ftn::RecordReader r(path, d.n);
+    std::vector<double> buf(d.n);
+    r.read(buf.data(), d.n);
+    HIPCHK(hipMemcpyAsync(dv, buf.data(), sizeof(double) * d.n, hipMemcpyHostToDevice, d.st));
+    HIPCHK(hipStreamSynchronize(d.st));
+}
+
+void write_vector(const LbfgsDev &d, const double *dv, const std::string &path) {
+    ftn::RecordWriter w(path, d.n);
+    std::vector<double> buf(d.n);
+    HIPCHK(hipMemcpyAsync(buf.data(), dv, sizeof(double) * d.n, hipMemcpyDeviceToHost, d.st));
+    HIPCHK(hipStreamSynchronize(d.st));
+    w.write(buf.data(), d.n);
+    w.close();
+}
+
+int read_flag(const std::string &path) {   // absent: 0 (nlinv_read_array_params.f90:154-174)
+    int v = 0;
+    if (!ftn::exists(path)) return 0;
+    ftn::ListReader r(path);
+    r.statement();
+    r.integer(&v);
+    return v;
+}
+
+}  // namespace lbx
+
+// ---------------------------------------------------------------------------
 struct sfd_lbfgs {
     LbfgsDev *d = nullptr;
     std::string err;
@@ -714,6 +883,51 @@ int sfd_lbfgsoc_step(const char *state_in, const char *state_out, const sfd_lbfg
         HIPCHK(hipMemcpyAsync(x, d.x, b, hipMemcpyDeviceToHost, d.st));
         HIPCHK(hipStreamSynchronize(d.st));
         if (*iflag >= 0) d.save_state(state_out);
+    });
+}
+
+int sfd_lbfgsoc_step_files(const char *dir, const sfd_lbfgsoc_opts *o, int n, double *x, double f,
+                           const double *g, int *iflag, int *info) {
+    if (!dir || !o || n < 1 || !x || !g || !iflag || !info || o->m < 1 || o->nparam < 1 || o->nparam > n ||
+        !o->xmin || !o->xmax)
+        return SFD_EINVAL;
+    return abi_call(nullptr, [&]() {
+        const std::string D = std::string(dir) + "/";
+        int fl = lbx::read_flag(D + "Iflag.dat");
+        int mi = lbx::read_flag(D + "Info.dat");
+        // DIAGCO = .FALSE.: LBFGSOC never returns IFLAG 2, and resuming with it
+        // would run the two-loop on a DIAG it did not compute
+        if (fl == 2) throw EngineError(SFD_EINVAL, D + "Iflag.dat holds 2 (DIAGCO re-entry is not supported)");
+        LbfgsDev d(n, o->m, 0.0, 0.0, nullptr, o->clip_type, 1, nullptr);
+        d.nparam = o->nparam;
+        d.pxmin.assign(o->xmin, o->xmin + o->nparam);
+        d.pxmax.assign(o->xmax, o->xmax + o->nparam);
+        d.stp1_opt = o->stp1_opt;
+        d.step_init = o->step_init;
+        d.xtrapf = 4.0;   // MCSRCHOC's extrapolation bound (nlinv_lbfgs.f:752)
+        if (fl != 0) {
+            // L_BFGS.f90 reads DIAG and W for any IFLAG but 0; LBFGSOC then reads
+            // its scalars and, for IFLAG 1, resumes the line search (MCSRCHOC
+            // reads its own file when INFO is not 0); other values start afresh
+            lbx::read_vector(d, d.diag, D + "Diag.dat");
+            lbx::read_work(d, D + "WorkArray.dat");
+            lbx::read_lbfgs_dat(d, D + "LbfgsDat.dat", fl == 1);
+            if (fl == 1 && d.info != 0) lbx::read_mcsrch_dat(d, D + "McsrchDat.dat");
+        }
+        d.on_lbfgs_write = [&]() { lbx::write_lbfgs_dat(d, D + "LbfgsDatOut.dat"); };
+        d.on_mcsrch_write = [&]() { lbx::write_mcsrch_dat(d, D + "McsrchDatOut.dat"); };
+        const size_t b = (size_t)n * sizeof(double);
+        HIPCHK(hipMemcpyAsync(d.x, x, b, hipMemcpyHostToDevice, d.st));
+        HIPCHK(hipMemcpyAsync(d.g, g, b, hipMemcpyHostToDevice, d.st));
+        d.step(f, &fl, &mi, 1.0e-20, 2.220446049250313e-16);   // nlinv_read_array_params.f90:227-228
+        HIPCHK(hipMemcpyAsync(x, d.x, b, hipMemcpyDeviceToHost, d.st));
+        HIPCHK(hipStreamSynchronize(d.st));
+        lbx::write_work(d, D + "WorkArrayOut.dat");
+        lbx::write_vector(d, d.diag, D + "DiagOut.dat");
+        ftn::ListWriter::save_text(D + "IflagOut.dat", ftn::i2_record(fl));
+        ftn::ListWriter::save_text(D + "InfoOut.dat", ftn::i2_record(mi));
+        *iflag = fl;
+        *info = mi;
     });
 }
 

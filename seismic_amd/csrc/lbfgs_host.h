@@ -45,6 +45,16 @@ struct LbfgsDev {
     std::vector<double> pxmin, pxmax;
     int stp1_opt = -1;                   // < 0: libinv's rule (lbfgs.f:333-338)
     double step_init = 999999.0;
+    // Scratch scalars of LBFGSOC / MCSRCHOC that the reference's state files
+    // carry although every use recomputes them first (nlinv_lbfgs.f:277-281,
+    // :756-760): kept so that a state file read and written again keeps them.
+    double sq = 0, yr = 0, beta = 0, xnorm = 0;
+    int inmc = 0, iycn = 0, iscn = 0;
+    double dg = 0, dgm = 0, dgxm = 0, dgym = 0, ftest1 = 0, fm = 0, fxm = 0, fym = 0;
+    int mp = 1020, lp = 0;   // COMMON /LB5/ units (block data LB4, nlinv_lbfgs.f:981-986)
+    // optional callbacks at the points where LBFGSOC / MCSRCHOC write their
+    // state files (nlinv_lbfgs.f:509-516, :548-565; :854-861, :898-905)
+    std::function<void()> on_lbfgs_write, on_mcsrch_write;
     // optional observer of every accepted (gradient, step) pair
     void (*on_pair)(void *ctx, const double *gk, const double *sk, const double *yk, long n, hipStream_t st) = nullptr;
     void *pair_ctx = nullptr;

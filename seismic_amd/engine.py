@@ -437,6 +437,38 @@ class LBFGSOC:
         return self.iflag.value, self.info.value
 
 
+def lbfgsoc_step_files(state_dir, x, f, g, m=5, xmin=(-1e30,), xmax=(1e30,), stp1_opt=1, step_init=999999.0,
+                       clip_type=2):
+    """One run of progs/inversion/L_BFGS.f90 (lbfgs_type 1) on the reference's
+    state files in `state_dir` (Iflag.dat, Info.dat, Diag.dat, WorkArray.dat,
+    LbfgsDat.dat, McsrchDat.dat in; the *Out.dat files out): sfd_lbfgsoc_step_files.
+    x (float64, n) is updated in place; returns (iflag, info)."""
+    lib = abi.load()
+    assert x.dtype == np.float64 and x.flags.c_contiguous
+    xmin = np.ascontiguousarray(xmin, np.float64)
+    xmax = np.ascontiguousarray(xmax, np.float64)
+    if xmin.shape != xmax.shape or xmin.ndim != 1:
+        raise ValueError("xmin and xmax need one value per parameter block")
+    opts = abi.sfd_lbfgsoc_opts(int(m), xmin.size, xmin.ctypes.data, xmax.ctypes.data, int(stp1_opt),
+                                float(step_init), int(clip_type))
+    gg = np.ascontiguousarray(g, np.float64)
+    iflag, info = C.c_int(0), C.c_int(0)
+    rc = lib.sfd_lbfgsoc_step_files(os.fsencode(state_dir), C.byref(opts), x.size, _ptr(x), float(f), _ptr(gg),
+                                    C.byref(iflag), C.byref(info))
+    if rc != 0:
+        raise SeisFDError(rc, "sfd_lbfgsoc_step_files failed")
+    return iflag.value, info.value
+
+
+def lbfgsoc_rotate_files(state_dir):
+    """Moves the *Out.dat files of a run onto the names the next run reads
+    (the step between two runs of the reference program)."""
+    for name in ("Iflag", "Info", "Diag", "WorkArray", "LbfgsDat", "McsrchDat"):
+        src = os.path.join(state_dir, name + "Out.dat")
+        if os.path.exists(src):
+            os.replace(src, os.path.join(state_dir, name + ".dat"))
+
+
 class LBFGS:
     """Device L-BFGS with the reference's reverse-communication protocol
     (LBFGS(N,M,X,F,G,...) of libs/libinv/lbfgs.f): call step(x, f, g) with

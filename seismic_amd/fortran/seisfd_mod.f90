@@ -178,6 +178,19 @@ module seisfd_mod
        integer(c_int), intent(inout) :: iflag, info
      end function sfd_lbfgsoc_step
 
+     ! one run of progs/inversion/L_BFGS.f90 on the reference's state files in dir
+     integer(c_int) function sfd_lbfgsoc_step_files(dir, o, n, x, f, g, iflag, info) &
+          bind(C, name='sfd_lbfgsoc_step_files')
+       import :: c_int, c_char, c_double, sfd_lbfgsoc_opts
+       character(kind=c_char), intent(in) :: dir(*)
+       type(sfd_lbfgsoc_opts), intent(in) :: o
+       integer(c_int), value :: n
+       real(c_double), intent(inout) :: x(*)
+       real(c_double), value :: f
+       real(c_double), intent(in) :: g(*)
+       integer(c_int), intent(out) :: iflag, info
+     end function sfd_lbfgsoc_step_files
+
      integer(c_int) function sfd_lbfgs_run(o, n, x, vpmask, vpinit, fct, ctx, rep) bind(C, name='sfd_lbfgs_run')
        import :: c_int, c_ptr, c_float, c_funptr, sfd_lbfgs_opts, sfd_lbfgs_report
        type(sfd_lbfgs_opts), intent(in) :: o

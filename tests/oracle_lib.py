@@ -449,6 +449,23 @@ class RefLBFGSOC:
         return self.iflag.value, self.myinfo.value
 
 
+def ref_lbfgs_x(state_dir, x, f, g, m=5, xmin=(-1e30,), xmax=(1e30,), stp1_opt=1, step_init=999999.0, boundt=2):
+    """One run of progs/inversion/L_BFGS.f90 (lbfgs_type 1) on the state files
+    of `state_dir`, by the reference's LBFGSOC and the program's own Fortran
+    file handling (oracle/ref_lbfgs_x.f90); x is updated in place; returns
+    (iflag, info)."""
+    L = C.CDLL(NLREF_SO, mode=os.RTLD_LOCAL)
+    d = os.fsencode(state_dir)
+    xmin = np.ascontiguousarray(xmin, np.float64)
+    xmax = np.ascontiguousarray(xmax, np.float64)
+    gg = np.ascontiguousarray(g, np.float64)
+    iflag, info = C.c_int(0), C.c_int(0)
+    L.ref_lbfgs_x(C.c_char_p(d), C.c_int(len(d)), C.c_int(x.size), C.c_int(m), ptr(x), C.c_double(f), ptr(gg),
+                  C.c_int(xmin.size), ptr(xmin), ptr(xmax), C.c_int(stp1_opt), C.c_double(step_init),
+                  C.c_int(boundt), C.byref(iflag), C.byref(info))
+    return iflag.value, info.value
+
+
 class RefLBFGS:
     """The reference's own LBFGS of libs/libinv/lbfgs.f (oracle/_ref/libseisref.so)."""
 
