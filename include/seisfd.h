@@ -32,6 +32,8 @@ extern "C" {
 #define SFD_EUNSTABLE    -6   /* Courant check failed (ReadSouVelRho.f90:629-675) */
 #define SFD_ENOTFINITE   -7   /* NaN/Inf met in a misfit or wavefield guard   */
 #define SFD_EIO          -8   /* state file missing, unreadable or not matching */
+#define SFD_EALIASED     -9   /* vel_check sampling stop: vmin / fmax < 2 max(dz,dx)
+                                 (ReadSouVelRho.f90:688-752)                  */
 
 /* Snapshot storage for RTM / FWI source wavefields. */
 #define SFD_SNAP_FP32     0   /* reference behaviour: FP32 copies           */
@@ -61,6 +63,9 @@ typedef struct sfd_grid {
     int32_t rec_type;        /* 0 only (no receiver ghost)                    */
     int32_t shot_type;       /* 0 only (no source ghost)                      */
     int32_t surf_type;       /* 0 only (absorbing top, Boundary.f90:1623)      */
+    float   fmax;            /* source's highest frequency (Hz) for vel_check's
+                                sampling stop; 0: the reference default 30
+                                (ReadParam.f90:18)                            */
 } sfd_grid;
 
 /*
@@ -77,12 +82,30 @@ typedef struct sfd_shot {
     const float *rx;         /* ntr receiver lateral positions (m)            */
 } sfd_shot;
 
+/* Data misfit norms (fct_compute / gdt_compute, libs/libinv/Norm.f90:12-141),
+ * numbered as the reference's dat_nrm_type (Inversion_types.f90:61-67). */
+#define SFD_NORM_L2         2   /* f = sum(r^2)/2 in FP64; adjoint source r        */
+#define SFD_NORM_L1         1   /* f = sum|r|; adjoint source sign(r)              */
+#define SFD_NORM_HUBER     12   /* |r|-t/2 or r^2/(2t); clamp(r/t, -1, 1)          */
+#define SFD_NORM_CAUCHY     3   /* log(1+(r/t)^2)/2; r/(t^2+r^2)                   */
+#define SFD_NORM_HYPERBOLIC 4   /* sqrt(t^2+r^2)-t; (r/t)/sqrt(1+r^2/t^2) (Norm.f90
+                                   only: the reference's parameter parsing cannot
+                                   select it)                                      */
+
 /* FWI gradient post-processing options (compute_function_gradient.f90:332-343). */
 typedef struct sfd_fwi_opts {
     float   illupow;         /* illumination power, Inversion_types.f90:60    */
     int32_t rect_z, rect_x;  /* triangle smoothing half-lengths (cells)       */
     int32_t do_z, do_x;      /* smoothing switches (Smoothing.f90:57-61)      */
     int32_t snap_mode;       /* SFD_SNAP_*                                    */
+    int32_t norm;            /* SFD_NORM_* (0 = L2), data_nrm_type             */
+    float   norm_thresh;     /* data_threshold t (Inversion_types.f90:62); 0 =
+                                per shot, max|r| of the first trace whose
+                                residual is not all zero, as Compute_OF_RES_ADJ
+                                passes its one THRESH to every trace
+                                (Objective_Function_Residual_Adjsource.f90:26-38;
+                                Norm.f90:36-40).  The non-L2 norms sum each
+                                trace's FP32 terms in FP32, in time order.    */
 } sfd_fwi_opts;
 
 typedef struct sfd_ctx *sfd_handle;
@@ -190,11 +213,32 @@ int  sfd_born_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots,
 int  sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots,
                const float *traces_in, float *image, float *illum);
 
+/* Offset-dependent top and bottom mutes (MuteParam, Mute.f90:14-46; the
+ * reference's defaults: v0 1500 m/s, t0 0 s, v1 10000 m/s, t1 40 s, tramp
+ * 0.5 s, Init_MuteParam :30-34). */
+typedef struct sfd_mute_opts {
+    float v0, t0;            /* top mute at sqrt(t0^2 + (h/v0)^2)              */
+    float v1, t1;            /* bottom mute at sqrt(t1^2 + (h/v1)^2)           */
+    float tramp;             /* taper length (s): sin^2 rise, cos^2 fall        */
+} sfd_mute_opts;
+
+/*
+ * The mute mask of every trace (MuteParam_compute_mask, Mute.f90:50-98, with
+ * compute_top_mute :171-195 and compute_bottom_mute :197-219): per shot an
+ * (nt, ntr) block, blocks concatenated in shot order as sfd_fwi_gradient's
+ * `mute` argument takes them.  The offset is h = x_receiver - x_source (the
+ * reference adds the 3D y offset, 0 in 2D); ot is the traces' time origin
+ * (s) and the sampling interval is the grid's dt.
+ */
+int  sfd_mute_mask(sfd_handle h, const sfd_shot *shots, int nshots, float ot, const sfd_mute_opts *opt,
+                   float *mask_out);
+
 /*
  * FWI misfit and gradient for a batch of shots, the body of compute_fct_gdt
  * (progs/afwi/compute_function_gradient.f90:208-350): per shot AMOD
- * (nearest extraction), residual w(dobs-dmod) and FP64 L2 misfit
- * (Objective_Function_Residual_Adjsource.f90:17-40, libs/libinv/Norm.f90:28),
+ * (nearest extraction), residual w(dobs-dmod), misfit and adjoint source
+ * under opt->norm (Objective_Function_Residual_Adjsource.f90:17-40,
+ * libs/libinv/Norm.f90:12-141; L2: FP64, dprod),
  * AGRAD (simple injection, FWI imaging Imaging.f90:322-422), scatter-add,
  * then illumination normalisation, mask and triangle smoothing
  * (Smoothing.f90:87-227).  mute and vpmask may be NULL (all ones).

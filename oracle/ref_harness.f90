@@ -7,7 +7,8 @@
 ! cannot be compiled here: it uses Imaging_mod, which needs SEPlib
 ! (libs/libwave/Imaging.f90:8).  The harness adds no arithmetic of its own:
 ! every floating-point operation on the wavefield is a call into the
-! reference's modules.  Used only to pin oracle/seisoracle.c.
+! reference's modules, and the misfit norms are Norm_mod's own functions.
+! Used only to pin oracle/seisoracle.c.
 module ref_harness
   use iso_c_binding
   use GeneralParam_types
@@ -22,8 +23,38 @@ module ref_harness
   use Injection_mod
   use Extraction_mod
   use Taper_mod
+  use Norm_mod
   implicit none
 contains
+
+  ! One trace through the reference's misfit and adjoint-source norms
+  ! (libs/libinv/Norm.f90:12-141), with THRESH passed by reference as
+  ! Compute_OF_RES_ADJ passes its local copy (the routines may overwrite it).
+  function ref_fct_compute(ntype, rd, nd, thresh) result(f) bind(C, name='ref_fct_compute')
+    integer(c_int), value :: ntype, nd
+    real(c_float), intent(in) :: rd(nd)
+    real(c_float), intent(inout) :: thresh
+    real(c_double) :: f
+    real, allocatable :: r(:)
+    real :: th
+    allocate(r(nd))
+    r = rd
+    th = thresh
+    f = fct_compute(ntype, r, nd, th)
+    thresh = th
+    deallocate(r)
+  end function ref_fct_compute
+
+  subroutine ref_gdt_compute(ntype, rd, nd, thresh) bind(C, name='ref_gdt_compute')
+    integer(c_int), value :: ntype, nd
+    real(c_float), intent(inout) :: rd(nd)
+    real(c_float), intent(inout) :: thresh
+    integer :: stat
+    real :: th
+    th = thresh
+    stat = gdt_compute(ntype, rd, nd, th)
+    thresh = th
+  end subroutine ref_gdt_compute
 
   subroutine ref_mksinc(sinc, lsinc, shift) bind(C, name='ref_mksinc')
     integer(c_int), value :: lsinc

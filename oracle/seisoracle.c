@@ -64,6 +64,8 @@ void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
 #define SIN sinf
 #define COS cosf
 #define POW powf
+#define SQRT sqrtf
+#define LOG logf
 #include "seisoracle_impl.h"
 #undef REAL
 #undef SFX
@@ -72,6 +74,8 @@ void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
 #undef SIN
 #undef COS
 #undef POW
+#undef SQRT
+#undef LOG
 
 #define REAL double
 #define SFX d
@@ -80,4 +84,61 @@ void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
 #define SIN sin
 #define COS cos
 #define POW pow
+#define SQRT sqrt
+#define LOG log
 #include "seisoracle_impl.h"
+
+/* Mute.f90:50-98 MuteParam_compute_mask with compute_top_mute (:171-195) and
+ * compute_bottom_mute (:197-219), literally: per trace mute = 0, the sin^2
+ * loop, mute(it+1:) = 1; then 1, the cos^2 loop, mute(it+1+nramp:) = 0; the
+ * mask is their product.  hy = 0 (2D).  mask: per shot (nt, ntr) blocks. */
+static void mute_top(float v0, float t0m, float tramp, float dt, float ot, float hx, int nt, float *m)
+{
+    const float pi = 3.1415926535f;
+    int nramp = (int)(tramp / dt + 1.5f);
+    float q = hx / v0;
+    float tm = sqrtf(t0m * t0m + q * q);
+    int it = (int)((tm - ot) / dt + 1.5f);
+    for (int i = 1; i <= nt; i++) m[i - 1] = 0.f;
+    int beg = it - nramp > 1 ? it - nramp : 1, end = it < nt ? it : nt;
+    for (int i = beg; i <= end; i++) {
+        float s = sinf((float)(i - it + nramp) * pi / (float)(2 * nramp));
+        m[i - 1] = s * s;
+    }
+    for (int i = (it + 1 > 1 ? it + 1 : 1); i <= nt; i++) m[i - 1] = 1.f;
+}
+
+static void mute_bottom(float v1, float t1m, float tramp, float dt, float ot, float hx, int nt, float *m)
+{
+    const float pi = 3.1415926535f;
+    int nramp = (int)(tramp / dt + 1.5f);
+    float q = hx / v1;
+    float tm = sqrtf(t1m * t1m + q * q);
+    int it = (int)((tm - ot) / dt + 1.5f);
+    for (int i = 1; i <= nt; i++) m[i - 1] = 1.f;
+    int beg = it + 1 > 1 ? it + 1 : 1, end = it + 1 + nramp < nt ? it + 1 + nramp : nt;
+    for (int i = beg; i <= end; i++) {
+        float c = cosf((float)(i - it - 1) * pi / (float)(2 * nramp));
+        m[i - 1] = c * c;
+    }
+    for (int i = (it + 1 + nramp > 1 ? it + 1 + nramp : 1); i <= nt; i++) m[i - 1] = 0.f;
+}
+
+void orc_f_mute_mask(int nshots, const sfd_shot *shots, float dt, float ot, const float *opt5, float *mask)
+{
+    /* opt5: v0, t0, v1, t1, tramp (sfd_mute_opts order) */
+    size_t off = 0;
+    for (int s = 0; s < nshots; s++) {
+        const sfd_shot *sh = &shots[s];
+        float *a = (float *)malloc(sizeof(float) * sh->nt), *b = (float *)malloc(sizeof(float) * sh->nt);
+        for (int r = 0; r < sh->ntr; r++) {
+            float hx = sh->rx[r] - sh->sx;
+            mute_top(opt5[0], opt5[1], opt5[4], dt, ot, hx, sh->nt, a);
+            mute_bottom(opt5[2], opt5[3], opt5[4], dt, ot, hx, sh->nt, b);
+            for (int i = 0; i < sh->nt; i++) mask[off + i] = a[i] * b[i];
+            off += (size_t)sh->nt;
+        }
+        free(a);
+        free(b);
+    }
+}

@@ -987,9 +987,54 @@ void FN(triangle2)(int n1, int n2, int rect1, int rect2, int doz, int dox, REAL 
     free(ss);
 }
 
+/* Norm.f90:12-79 fct_compute on one trace rd(nd).  *thresh is the caller's
+ * THRESH, overwritten with max|rd| when it is 0 (Huber, Cauchy, hyperbolic;
+ * :36-40,52-56,65-69).  The L2 sum is of FP64 products (dprod); the others sum
+ * their terms in REAL, in order, as SUM of a default-real array does. */
+double FN(fct_compute)(int ntype, const REAL *rd, int nd, REAL *thresh)
+{
+    if (ntype == 2 || ntype == 0) {
+        double f = 0;
+        for (int i = 0; i < nd; i++) f += (double)rd[i] * (double)rd[i];
+        return f / 2;
+    }
+    if (ntype != 1 && *thresh == (REAL)0) {
+        REAL m = 0;
+        for (int i = 0; i < nd; i++) m = FABS(rd[i]) > m ? FABS(rd[i]) : m;
+        *thresh = m;
+    }
+    const REAL t = *thresh;
+    REAL acc = 0;
+    for (int i = 0; i < nd; i++) {
+        const REAL r = rd[i], a = FABS(r);
+        REAL term;
+        if (ntype == 1) term = a;
+        else if (ntype == 12) term = a > t ? a - t / (REAL)2 : r * r / ((REAL)2 * t);
+        else if (ntype == 3) { const REAL q = r / t; term = LOG((REAL)1 + q * q) / (REAL)2; }
+        else term = SQRT(t * t + r * r) - t;
+        acc = acc + term;
+    }
+    return (double)acc;
+}
+
+/* Norm.f90:81-141 gdt_compute: rd(nd) becomes the adjoint source. */
+void FN(gdt_compute)(int ntype, REAL *rd, int nd, REAL t)
+{
+    for (int i = 0; i < nd; i++) {
+        const REAL r = rd[i];
+        if (ntype == 1) rd[i] = r < 0 ? (REAL)-1 : (r > 0 ? (REAL)1 : (REAL)0);
+        else if (ntype == 12) {   /* max(-1, min(1, q)); a NaN q stays NaN, as the reference's MIN/MAX */
+            const REAL q = r / t;
+            rd[i] = q > (REAL)1 ? (REAL)1 : (q < (REAL)-1 ? (REAL)-1 : q);
+        }
+        else if (ntype == 3) rd[i] = r / (t * t + r * r);
+        else if (ntype == 4) rd[i] = (r / t) / SQRT((REAL)1 + r * r / (t * t));
+    }
+}
+
 /* compute_function_gradient.f90:208-350 compute_fct_gdt (single thread, shots
- * summed in order), with Compute_OF_RES_ADJ (Objective_...:17-40), L2 norm
- * (Norm.f90:28) and AMOD/AGRAD (to_disk_to_memory_AMOD.f90:27-82,
+ * summed in order), with Compute_OF_RES_ADJ (Objective_...:17-40) and its
+ * norms (Norm.f90:12-141: o->norm, o->norm_thresh) and AMOD/AGRAD (to_disk_to_memory_AMOD.f90:27-82,
  * to_disk_to_memory_AGRAD.f90:27-78).  dobs/resid: per shot (nt,ntr) blocks
  * concatenated; mute same layout (NULL = 1); vpmask (nz*nx, NULL = 1).
  * All shots share nt (invparam%n1).  grad/illu_out are (nz,nx). */
@@ -1022,20 +1067,20 @@ int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_s
         P.tmin = 1; P.tmax = s->nt; P.tstep = 1;
         CAT(propagate_, SFX)(&P);
         CAT(free_traces_, SFX)(P.sou, 1);
-        /* Compute_OF_RES_ADJ */
+        /* Compute_OF_RES_ADJ: one THRESH for the shot's traces, in order */
+        REAL thresh = (REAL)o->norm_thresh;
         for (int r = 0; r < s->ntr; r++) {
             REAL *d = P.dat[r].trace;
             const REAL *ob = dobs + off + (size_t)r * s->nt;
             const REAL *mu = mute ? mute + off + (size_t)r * s->nt : NULL;
-            double fr = 0;
             for (int it = 0; it < s->nt; it++) {
                 REAL w = mu ? mu[it] : (REAL)1;
                 d[it] = w * (ob[it] - d[it]);
-                fr += (double)d[it] * (double)d[it];
             }
-            f = f + fr / 2;
+            f = f + FN(fct_compute)(o->norm, d, s->nt, &thresh);
             if (resid) memcpy(resid + off + (size_t)r * s->nt, d, sizeof(REAL) * s->nt);
             for (int it = 0; it < s->nt; it++) d[it] = (mu ? mu[it] : (REAL)1) * d[it];
+            FN(gdt_compute)(o->norm, d, s->nt, thresh);
         }
         /* AGRAD */
         FN(compute_taper)(tap, P.nxw);

@@ -18,7 +18,14 @@ SFD_EUNSUPPORTED = -4
 SFD_ESTATE = -5
 SFD_EUNSTABLE = -6
 SFD_ENOTFINITE = -7
+SFD_EIO = -8
+SFD_EALIASED = -9
 
+SFD_NORM_L2 = 2
+SFD_NORM_L1 = 1
+SFD_NORM_HUBER = 12
+SFD_NORM_CAUCHY = 3
+SFD_NORM_HYPERBOLIC = 4
 SFD_SNAP_FP32 = 0
 SFD_SNAP_FP16S = 1
 SFD_SNAP_FP16G = 2
@@ -47,6 +54,7 @@ class sfd_grid(C.Structure):
         ("ntaper", C.c_int32), ("lsinc", C.c_int32), ("snapi", C.c_int32),
         ("aperture_x", C.c_float),
         ("rec_type", C.c_int32), ("shot_type", C.c_int32), ("surf_type", C.c_int32),
+        ("fmax", C.c_float),
     ]
 
 
@@ -66,7 +74,12 @@ class sfd_fwi_opts(C.Structure):
         ("rect_z", C.c_int32), ("rect_x", C.c_int32),
         ("do_z", C.c_int32), ("do_x", C.c_int32),
         ("snap_mode", C.c_int32),
+        ("norm", C.c_int32), ("norm_thresh", C.c_float),
     ]
+
+
+class sfd_mute_opts(C.Structure):
+    _fields_ = [("v0", C.c_float), ("t0", C.c_float), ("v1", C.c_float), ("t1", C.c_float), ("tramp", C.c_float)]
 
 
 class sfd_lbfgs_opts(C.Structure):
@@ -115,7 +128,7 @@ SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_forward_noalloc", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
-    "sfd_born_adjoint_exact", "sfd_lsrtm",
+    "sfd_born_adjoint_exact", "sfd_lsrtm", "sfd_mute_mask",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
     "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgsoc_step", "sfd_lbfgsoc_step_files", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
@@ -170,6 +183,7 @@ def load():
     lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
     lib.sfd_lbfgsoc_step.argtypes = [C.c_char_p, C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p,
                                      C.c_double, C.c_void_p, P(C.c_int), P(C.c_int)]
+    lib.sfd_mute_mask.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_float, P(sfd_mute_opts), C.c_void_p]
     lib.sfd_lbfgsoc_step_files.argtypes = [C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p, C.c_double,
                                            C.c_void_p, P(C.c_int), P(C.c_int)]
     lib.sfd_lbfgs_run.argtypes = [P(sfd_lbfgs_opts), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,

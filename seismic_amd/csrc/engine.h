@@ -159,6 +159,7 @@ struct Engine {
     void born(const sfd_shot *shots, int ns, const float *refl, float *traces_out);
     void adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind);
     void born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *mbar);
+    void mute_mask(const sfd_shot *shots, int ns, float ot, const sfd_mute_opts &o, float *mask_out);
     void fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
                       const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
 

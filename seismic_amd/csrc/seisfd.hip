@@ -130,20 +130,34 @@ static void check_grid(const sfd_grid &g) {
     if (g.ntaper < 10) throw EngineError(SFD_EINVAL, "ntaper must be >= 10 (Higdon bands and injection must not overlap)");
     if (g.lsinc != 7) throw EngineError(SFD_EUNSUPPORTED, "only lsinc=7 is implemented");
     if (g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
+    if (!(g.fmax >= 0.f) || !std::isfinite(g.fmax)) throw EngineError(SFD_EINVAL, "fmax must be >= 0 (0: 30 Hz)");
     if (g.rec_type != 0 || g.shot_type != 0 || g.surf_type != 0)
         throw EngineError(SFD_EUNSUPPORTED, "ghost sources/receivers and free surface are not implemented");
 }
 
-// ReadSouVelRho.f90:603-675 vel_check (2D: Courant <= 0.45)
+// ReadSouVelRho.f90:603-752 vel_check, 2D: the Courant stop (:629-675,
+// vmax dt / min(dz,dx) > 0.45) and the sampling stop (:688-752: fewer than
+// 2 cells of max(dz,dx) per shortest wavelength vmin / fmax).
 static void vel_check(const sfd_grid &g, const float *v, size_t n) {
-    float vmax = v[0];
-    for (size_t i = 1; i < n; i++) vmax = std::max(vmax, v[i]);
+    float vmax = v[0], vmin = v[0];
+    for (size_t i = 1; i < n; i++) {
+        vmax = std::max(vmax, v[i]);
+        vmin = std::min(vmin, v[i]);
+    }
     const float stab = vmax * g.dt / std::min(g.dz, g.dx);
+    char b[200];
     if (stab > 0.45f) {
-        char b[160];
         snprintf(b, sizeof b, "Courant number %.6g > 0.45: unstable (largest stable dt %.6g)", stab,
                  0.45f / stab * g.dt);
         throw EngineError(SFD_EUNSTABLE, b);
+    }
+    const float fmax = g.fmax > 0.f ? g.fmax : 30.f;
+    const float samplerate = vmin / fmax / std::max(g.dz, g.dx);
+    if (samplerate < 2.0f) {
+        snprintf(b, sizeof b, "wavefield not adequately sampled: %.6g cells per shortest wavelength (vmin %.6g, "
+                 "fmax %.6g Hz); lower fmax to %.6g or refine the grid", samplerate, vmin, fmax,
+                 samplerate / 3.0f * fmax);
+        throw EngineError(SFD_EALIASED, b);
     }
 }
 
@@ -1798,6 +1812,11 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     if ((opt->do_z && opt->rect_z < 1) || (opt->do_x && opt->rect_x < 1)) throw EngineError(SFD_EINVAL, "bad smoothing length");
     if ((opt->do_z && opt->rect_z > g.nz) || (opt->do_x && opt->rect_x > g.nx))
         throw EngineError(SFD_EINVAL, "smoothing box longer than the model (Smoothing.f90:202 boxconv)");
+    const int norm = opt->norm == 0 ? SFD_NORM_L2 : opt->norm;
+    if (norm != SFD_NORM_L2 && norm != SFD_NORM_L1 && norm != SFD_NORM_HUBER && norm != SFD_NORM_CAUCHY &&
+        norm != SFD_NORM_HYPERBOLIC)
+        throw EngineError(SFD_EINVAL, "unknown norm (SFD_NORM_*)");
+    if (!std::isfinite(opt->norm_thresh)) throw EngineError(SFD_EINVAL, "norm_thresh is not finite");
     long ndata = 0;
     int ntraces = 0;
     for (int s = 0; s < ns; s++) {
@@ -1824,7 +1843,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     Layout L;
     const size_t o_dobs = L.take(dobs ? ndata * 4 : 16), o_mute = L.take(mute ? ndata * 4 : 16), o_res = L.take(ndata * 4);
     const size_t o_refs = L.take(sizeof(TraceRef) * ntraces), o_adj = L.take(sizeof(float *) * ns);
-    const size_t o_ft = L.take(sizeof(double) * ntraces);
+    const size_t o_ft = L.take(sizeof(double) * ntraces), o_tmax = L.take(sizeof(float) * ntraces),
+                 o_thr = L.take(sizeof(float) * ntraces), o_tbeg = L.take(sizeof(int) * (ns + 1));
     const size_t o_grad = L.take(N * 4), o_illu = L.take(N * 4), o_tmp = L.take(N * 4);
     const size_t o_mask = L.take(vpmask ? N * 4 : 16);
     const size_t o_bmax = L.take(rblocks * 4), o_bsum = L.take(rblocks * 8);
@@ -1877,8 +1897,11 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
         }
         std::vector<float *> adjp(nw);
         for (int s = 0; s < nw; s++) adjp[s] = (float *)B.shots_h[s].inj_traces;
+        std::vector<int> tbeg(nw + 1, 0);   // the wave's traces of shot s: [tbeg[s], tbeg[s+1])
+        for (int s = 0; s < nw; s++) tbeg[s + 1] = tbeg[s] + ws[s].ntr;
         HIPCHK(hipMemcpyAsync(fb + o_refs, refs.data(), sizeof(TraceRef) * wtr, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(fb + o_adj, adjp.data(), sizeof(float *) * nw, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(fb + o_tbeg, tbeg.data(), sizeof(int) * (nw + 1), hipMemcpyHostToDevice, stream));
         setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
 
         // AMOD: forward modeling with nearest extraction and snapshots
@@ -1891,10 +1914,18 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
         A.snap_global = need.snap_global;
         HIPCHK(hipEventRecord(ev0, stream));
         run_loop(B, A, false, max_nt(plans));
-        // residual, FP64 misfit per trace, adjoint source into the injection buffers
-        hipLaunchKernelGGL(residual_kernel, dim3(wtr), dim3(256), 0, stream, (const TraceRef *)(fb + o_refs), B.shots,
-                           d_dobs, mute ? (const float *)(fb + o_mute) : nullptr, (float *)(fb + o_res),
-                           (float *const *)(fb + o_adj), (double *)(fb + o_ft) + toff);
+        // residual, misfit per trace and adjoint source into the injection buffers
+        const TraceRef *d_refs = (const TraceRef *)(fb + o_refs);
+        float *const *d_adj = (float *const *)(fb + o_adj);
+        float *d_tmax = (float *)(fb + o_tmax) + toff, *d_thr = (float *)(fb + o_thr) + toff;
+        hipLaunchKernelGGL(residual_kernel, dim3(wtr), dim3(256), 0, stream, d_refs, B.shots, d_dobs,
+                           mute ? (const float *)(fb + o_mute) : nullptr, (float *)(fb + o_res), d_adj, d_tmax);
+        hipLaunchKernelGGL(norm_thresh_kernel, dim3((nw + 63) / 64), dim3(64), 0, stream, nw,
+                           (const int *)(fb + o_tbeg), (const float *)d_tmax, norm, opt->norm_thresh, d_thr);
+        hipLaunchKernelGGL(misfit_kernel, dim3((wtr + 63) / 64), dim3(64), 0, stream, wtr, d_refs,
+                           (const float *)(fb + o_res), (const float *)d_thr, norm, (double *)(fb + o_ft) + toff);
+        if (norm != SFD_NORM_L2)
+            hipLaunchKernelGGL(adjsrc_kernel, dim3(wtr), dim3(256), 0, stream, d_refs, d_adj, (const float *)d_thr, norm);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ev1, stream));
         // AGRAD: reverse time, forward stencil, simple injection, FWI imaging condition
@@ -1950,7 +1981,8 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
         ss = ss + bsum[k];
     }
     const double rn = sqrt(ss / (double)N);
-    if (!std::isfinite(f) || !std::isfinite(rn)) throw EngineError(SFD_ENOTFINITE, "misfit or illumination is not finite");
+    if (!std::isfinite(f) || !std::isfinite(rn))
+        throw EngineError(SFD_ENOTFINITE, "misfit, gradient or illumination is not finite");
     const float *d_mask = vpmask ? (const float *)(fb + o_mask) : nullptr;
     hipLaunchKernelGGL(fwi_normalize_kernel, dim3(rblocks), dim3(256), 0, stream, d_grad, d_illu, N, mx, rn, d_mask);
     float *pp = (float *)(fb + o_pp), *qq = (float *)(fb + o_qq);
@@ -1973,6 +2005,65 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     *misfit = f;
     timing[0] = fwd_ms + bwd_ms; timing[1] = fwd_ms; timing[2] = bwd_ms;
     timing[3] = setup_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
+}
+
+// MuteParam_compute_mask (Mute.f90:50-98): the per-trace onsets and the two
+// ramp tables in the reference's FP32 expressions (:178-186, :204-212), then
+// the mask on the device.
+void Engine::mute_mask(const sfd_shot *shots, int ns, float ot, const sfd_mute_opts &o, float *mask_out) {
+    if (ns < 1 || !shots || !mask_out) throw EngineError(SFD_EINVAL, "no shots or no output");
+    if (!(o.v0 > 0.f) || !(o.v1 > 0.f) || !(o.tramp >= 0.f) || !std::isfinite(o.t0) || !std::isfinite(o.t1) ||
+        !std::isfinite(ot))
+        throw EngineError(SFD_EINVAL, "mute: v0, v1 > 0, tramp >= 0 and finite times required");
+    const float dt = g.dt, pi = 3.1415926535f;
+    const float nr_f = o.tramp / dt + 1.5f;
+    if (!(nr_f < 1e6f)) throw EngineError(SFD_EINVAL, "mute: tramp / dt too large");
+    const int nramp = (int)nr_f;
+    std::vector<float> rtop(nramp + 1), rbot(nramp + 1);
+    for (int k = 0; k <= nramp; k++) {
+        const float a = (float)k * pi / (float)(2 * nramp);
+        const float sn = sinf(a), cs = cosf(a);
+        rtop[k] = sn * sn;
+        rbot[k] = cs * cs;
+    }
+    std::vector<TraceRef> refs;
+    std::vector<int2> onset;
+    long off = 0;
+    for (int s = 0; s < ns; s++) {
+        const sfd_shot &sh = shots[s];
+        if (sh.nt < 1 || sh.ntr < 0 || (sh.ntr > 0 && !sh.rx)) throw EngineError(SFD_EINVAL, "bad shot");
+        for (int r = 0; r < sh.ntr; r++) {
+            const float hx = sh.rx[r] - sh.sx, hy = 0.f;
+            auto onset_of = [&](float t_init, float v) {
+                const float q = (hx + hy) / v;
+                const float tm = sqrtf(t_init * t_init + q * q);
+                const float x = (tm - ot) / dt + 1.5f;
+                if (!(x > -2e9f && x < 2e9f)) throw EngineError(SFD_EINVAL, "mute onset out of range");
+                return (int)x;   // conversion to INTEGER truncates
+            };
+            refs.push_back({s, r, sh.nt, off});
+            onset.push_back(make_int2(onset_of(o.t0, o.v0), onset_of(o.t1, o.v1)));
+            off += sh.nt;
+        }
+    }
+    const int ntr = (int)refs.size();
+    if (ntr == 0) return;
+    Layout L;
+    const size_t o_refs = L.take(sizeof(TraceRef) * ntr), o_on = L.take(sizeof(int2) * ntr),
+                 o_rt = L.take(sizeof(float) * (nramp + 1)), o_rb = L.take(sizeof(float) * (nramp + 1)),
+                 o_m = L.take(sizeof(float) * off);
+    tmp_mem.reserve(L.off);
+    char *b = (char *)tmp_mem.p;
+    HIPCHK(hipMemcpyAsync(b + o_refs, refs.data(), sizeof(TraceRef) * ntr, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(b + o_on, onset.data(), sizeof(int2) * ntr, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(b + o_rt, rtop.data(), sizeof(float) * (nramp + 1), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(b + o_rb, rbot.data(), sizeof(float) * (nramp + 1), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(mute_kernel, dim3(ntr), dim3(256), 0, stream, (const TraceRef *)(b + o_refs),
+                       (const int2 *)(b + o_on), nramp, (const float *)(b + o_rt), (const float *)(b + o_rb),
+                       (float *)(b + o_m));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(mask_out, b + o_m, sizeof(float) * off, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
 }
 
 void ShotPlan::take_backward(const ShotPlan &o) {
@@ -2137,6 +2228,12 @@ int sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const floa
     return guarded(h, [&](Engine &e) {
         e.fwi_gradient(shots, nshots, dobs, mute, vpmask, opt, misfit, grad, illum_out, resid_out);
     });
+}
+
+int sfd_mute_mask(sfd_handle h, const sfd_shot *shots, int nshots, float ot, const sfd_mute_opts *opt,
+                  float *mask_out) {
+    if (!opt) return SFD_EINVAL;
+    return guarded(h, [&](Engine &e) { e.mute_mask(shots, nshots, ot, *opt, mask_out); });
 }
 
 // FWI inversion: Inversion.f90 l_bfgs with compute_fct_gdt as fctgdt

@@ -24,14 +24,16 @@ class SeisFDError(RuntimeError):
         self.code = code
 
 
-def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture_x=0.0):
-    """GeneralParam subset (ReadParam.f90:18-36 defaults: ntaper=20, snapi=4, lsinc=7)."""
+def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture_x=0.0, fmax=30.0):
+    """GeneralParam subset (ReadParam.f90:18-36 defaults: ntaper=20, snapi=4,
+    lsinc=7, fmax=30 Hz)."""
     g = sfd_grid()
     g.nz, g.nx = int(nz), int(nx)
     g.oz, g.ox, g.dz, g.dx, g.dt = oz, ox, dz, dx, dt
     g.ntaper, g.lsinc, g.snapi = ntaper, lsinc, snapi
     g.aperture_x = aperture_x
     g.rec_type = g.shot_type = g.surf_type = 0
+    g.fmax = fmax
     return g
 
 
@@ -181,16 +183,26 @@ class Engine:
         self._chk(self.lib.sfd_born_adjoint_exact(self.h, self._shots(shots), len(shots), _ptr(db), _ptr(mb)))
         return mb.reshape(self.g.nx, self.g.nz).T.copy()
 
+    def mute_mask(self, shots, ot=0.0, v0=1500.0, t0=0.0, v1=10000.0, t1=40.0, tramp=0.5):
+        """MuteParam_compute_mask (Mute.f90:50-98; defaults of Init_MuteParam):
+        one (ntr, nt) mask per shot."""
+        o = abi.sfd_mute_opts(v0, t0, v1, t1, tramp)
+        out = np.zeros(sum(s.nt * s.ntr for s in shots), np.float32)
+        self._chk(self.lib.sfd_mute_mask(self.h, self._shots(shots), len(shots), float(ot), C.byref(o), _ptr(out)))
+        return self._split(out, shots)
+
     # compute_fct_gdt
     def fwi_gradient(self, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
-                     do_x=0, snap_mode=abi.SFD_SNAP_FP32):
-        """dobs None: the copy kept in HBM by set_traces."""
+                     do_x=0, snap_mode=abi.SFD_SNAP_FP32, norm=abi.SFD_NORM_L2, norm_thresh=0.0):
+        """dobs None: the copy kept in HBM by set_traces.  norm / norm_thresh:
+        data_nrm_type / data_threshold (SFD_NORM_*)."""
         nz, nx = self.g.nz, self.g.nx
         ob = None if dobs is None else self._cat(dobs, shots)
         mu = None if mute is None else self._cat(mute, shots)
         vm = None if vpmask is None else _model_buf(vpmask, nz, nx)
         o = sfd_fwi_opts()
         o.illupow, o.rect_z, o.rect_x, o.do_z, o.do_x, o.snap_mode = illupow, rect_z, rect_x, do_z, do_x, snap_mode
+        o.norm, o.norm_thresh = norm, norm_thresh
         f = C.c_double()
         grad = np.zeros(nz * nx, np.float32)
         illu = np.zeros(nz * nx, np.float32)
@@ -201,7 +213,7 @@ class Engine:
 
     def fwi_lbfgs(self, shots, dobs, x0, niter=10, neval=None, vpmin=1500.0, vpmax=4500.0, bound=2,
                   freeze_soft=True, m=5, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
-                  do_x=0, snap_mode=abi.SFD_SNAP_FP32, precond=None):
+                  do_x=0, snap_mode=abi.SFD_SNAP_FP32, precond=None, norm=abi.SFD_NORM_L2, norm_thresh=0.0):
         """FWI by Inversion.f90 l_bfgs with compute_fct_gdt as the evaluation
         (defaults of Inversion_types.f90:48-57).  precond: None for plain
         L-BFGS, or a dict of sfd_precond_opts fields (enable is implied) for
@@ -212,6 +224,7 @@ class Engine:
         vm = None if vpmask is None else _model_buf(vpmask, nz, nx)
         fo = sfd_fwi_opts()
         fo.illupow, fo.rect_z, fo.rect_x, fo.do_z, fo.do_x, fo.snap_mode = illupow, rect_z, rect_x, do_z, do_x, snap_mode
+        fo.norm, fo.norm_thresh = norm, norm_thresh
         lo = lbfgs_opts(niter, neval, m, bound, freeze_soft, vpmin, vpmax)
         pc = precond_opts(**(precond or {})) if precond is not None else None
         x = _model_buf(x0, nz, nx).copy()

@@ -11,8 +11,11 @@ module seisfd_mod
   implicit none
 
   integer(c_int), parameter :: SFD_OK = 0, SFD_EINVAL = -1, SFD_ENOMEM = -2, SFD_EHIP = -3, &
-       SFD_EUNSUPPORTED = -4, SFD_ESTATE = -5, SFD_EUNSTABLE = -6, SFD_ENOTFINITE = -7, SFD_EIO = -8
-  integer(c_int), parameter :: SFD_SNAP_FP32 = 0, SFD_SNAP_FP16S = 1
+       SFD_EUNSUPPORTED = -4, SFD_ESTATE = -5, SFD_EUNSTABLE = -6, SFD_ENOTFINITE = -7, SFD_EIO = -8, &
+       SFD_EALIASED = -9
+  integer(c_int), parameter :: SFD_SNAP_FP32 = 0, SFD_SNAP_FP16S = 1, SFD_SNAP_FP16G = 2
+  integer(c_int), parameter :: SFD_NORM_L2 = 2, SFD_NORM_L1 = 1, SFD_NORM_HUBER = 12, SFD_NORM_CAUCHY = 3, &
+       SFD_NORM_HYPERBOLIC = 4
   integer(c_int), parameter :: SFD_EXTRACT_SINC_V2 = 0, SFD_EXTRACT_NEAREST = 1, &
        SFD_EXTRACT_SINC_LSRTM = 2, SFD_EXTRACT_SINC_UNIT = 3
 
@@ -22,6 +25,7 @@ module seisfd_mod
      integer(c_int32_t) :: ntaper, lsinc, snapi
      real(c_float)      :: aperture_x
      integer(c_int32_t) :: rec_type, shot_type, surf_type
+     real(c_float)      :: fmax = 0.   ! vel_check's fmax (Hz); 0: 30
   end type sfd_grid
 
   type, bind(C) :: sfd_shot
@@ -33,7 +37,13 @@ module seisfd_mod
   type, bind(C) :: sfd_fwi_opts
      real(c_float)      :: illupow
      integer(c_int32_t) :: rect_z, rect_x, do_z, do_x, snap_mode
+     integer(c_int32_t) :: norm = 0          ! SFD_NORM_* (0 = L2), data_nrm_type
+     real(c_float)      :: norm_thresh = 0.  ! data_threshold
   end type sfd_fwi_opts
+
+  type, bind(C) :: sfd_mute_opts
+     real(c_float) :: v0 = 1500., t0 = 0., v1 = 10000., t1 = 40., tramp = 0.5
+  end type sfd_mute_opts
 
   type, bind(C) :: sfd_lbfgs_opts
      integer(c_int) :: niter, neval, m, const_type, freeze_soft, eval0
@@ -150,6 +160,16 @@ module seisfd_mod
        real(c_float), intent(in) :: traces(*)
        real(c_float), intent(inout) :: image(*), illum(*)
      end function sfd_lsrtm
+
+     integer(c_int) function sfd_mute_mask(h, shots, nshots, ot, opt, mask) bind(C, name='sfd_mute_mask')
+       import :: c_int, c_ptr, c_float, sfd_shot, sfd_mute_opts
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), value :: ot
+       type(sfd_mute_opts), intent(in) :: opt
+       real(c_float), intent(inout) :: mask(*)
+     end function sfd_mute_mask
 
      integer(c_int) function sfd_fwi_gradient(h, shots, nshots, dobs, mute, vpmask, opt, misfit, grad, &
           illum, resid) bind(C, name='sfd_fwi_gradient')

@@ -224,8 +224,9 @@ def born_adjoint(g, vel, shot, dbar, prec="d"):
 
 
 def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1,
-                 do_z=0, do_x=0, prec="f"):
-    """compute_fct_gdt restatement.  dobs: list of (ntr, nt) arrays per shot."""
+                 do_z=0, do_x=0, prec="f", norm=2, norm_thresh=0.0):
+    """compute_fct_gdt restatement.  dobs: list of (ntr, nt) arrays per shot;
+    norm / norm_thresh: data_nrm_type / data_threshold (Norm.f90 codes)."""
     L = oracle()
     T = _dt(prec)
     v = colmajor(vel, T)
@@ -236,6 +237,7 @@ def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_
     vm = None if vpmask is None else colmajor(vpmask, T)
     o = sfd_fwi_opts()
     o.illupow, o.rect_z, o.rect_x, o.do_z, o.do_x, o.snap_mode = illupow, rect_z, rect_x, do_z, do_x, 0
+    o.norm, o.norm_thresh = norm, norm_thresh
     f = C.c_double()
     grad = np.zeros(g.nz * g.nx, dtype=T)
     illu = np.zeros(g.nz * g.nx, dtype=T)
@@ -249,6 +251,52 @@ def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_
     assert rc == 0
     return dict(f=f.value, grad=grad.reshape(g.nx, g.nz).T, illum=illu.reshape(g.nx, g.nz).T,
                 resid=res)
+
+
+def mute_mask(dt, shots, ot=0.0, v0=1500.0, t0=0.0, v1=10000.0, t1=40.0, tramp=0.5):
+    """Mute.f90 MuteParam_compute_mask restatement: one (ntr, nt) mask per shot."""
+    L = oracle()
+    cs = (sfd_shot * len(shots))(*[s.c() for s in shots])
+    opt = np.array([v0, t0, v1, t1, tramp], np.float32)
+    n = [len(s.rx) * s.nt for s in shots]
+    out = np.zeros(sum(n), np.float32)
+    L.orc_f_mute_mask.argtypes = [C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+    L.orc_f_mute_mask(len(shots), cs, dt, ot, ptr(opt), ptr(out))
+    res, o = [], 0
+    for s, k in zip(shots, n):
+        res.append(out[o:o + k].reshape(len(s.rx), s.nt))
+        o += k
+    return res
+
+
+def norm_fct(ntype, rd, thresh, lib="oracle"):
+    """fct_compute (Norm.f90:12-79) on one FP32 trace: the oracle's or, with
+    lib="ref", the reference's own; returns (f, thresh after the call)."""
+    r = np.ascontiguousarray(rd, np.float32)
+    th = C.c_float(thresh)
+    if lib == "ref":
+        fn = ref().ref_fct_compute
+    else:
+        fn = oracle().orc_f_fct_compute
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+    f = fn(ntype, ptr(r), r.size, C.byref(th))
+    return f, th.value
+
+
+def norm_gdt(ntype, rd, thresh, lib="oracle"):
+    """gdt_compute (Norm.f90:81-141) on a copy of one FP32 trace."""
+    r = np.ascontiguousarray(rd, np.float32).copy()
+    if lib == "ref":
+        th = C.c_float(thresh)
+        fn = ref().ref_gdt_compute
+        fn.argtypes = [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        fn(ntype, ptr(r), r.size, C.byref(th))
+    else:
+        fn = oracle().orc_f_gdt_compute
+        fn.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_float]
+        fn(ntype, ptr(r), r.size, thresh)
+    return r
 
 
 def mksinc(lsinc, shift, prec="f"):

@@ -28,9 +28,9 @@ def test_library_exports_all_symbols():
 
 
 def test_struct_sizes():
-    assert ctypes.sizeof(abi.sfd_grid) == 14 * 4
+    assert ctypes.sizeof(abi.sfd_grid) == 15 * 4
     assert ctypes.sizeof(abi.sfd_shot) == 8 + 8 + 3 * 8
-    assert ctypes.sizeof(abi.sfd_fwi_opts) == 6 * 4
+    assert ctypes.sizeof(abi.sfd_fwi_opts) == 8 * 4
     # int m, nparam; two pointers; int stp1_opt (+4 pad); double; int clip_type (+4 pad)
     assert ctypes.sizeof(abi.sfd_lbfgsoc_opts) == 48
 
