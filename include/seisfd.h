@@ -213,6 +213,29 @@ int  sfd_born_adjoint_exact(sfd_handle h, const sfd_shot *shots, int nshots,
 int  sfd_lsrtm(sfd_handle h, const sfd_shot *shots, int nshots,
                const float *traces_in, float *image, float *illum);
 
+/* Butterworth band-pass (BandPassParam, Bandpass.f90:13-47; the reference's
+ * defaults: fhi 20 Hz, flo 0, nphi 14, nplo 14, phase 0). */
+typedef struct sfd_bandpass_opts {
+    float   flo, fhi;        /* cut-offs (Hz); flo dt <= 1e-6 skips the low-cut,
+                                fhi dt >= 4.999999 the high-cut               */
+    int32_t nplo, nphi;      /* poles (halved at zero phase), <= 16 sections  */
+    int32_t phase;           /* 0 zero phase (forward and reversed), 1 minimum */
+} sfd_bandpass_opts;
+
+/*
+ * bandpass (Bandpass.f90:89-147, the filters of libs/libwave/bandpass.c:72-266)
+ * on ntr traces of nt samples at the grid's dt, in place, trace after trace
+ * (time fastest).  BandpassSouTraces (:50-88) runs it on every shot's
+ * wavelet and traces before an inversion.  Each trace is padded by 10
+ * samples each side with sin^2 tapers of its end values, low-cut, high-cut,
+ * and the nt samples after the front pad are kept.  (The reference's last
+ * statement assigns data(npad+1:nt) to an nt-sample trace, a shape mismatch
+ * that flang -O2 and compilers looping over the left-hand side read as
+ * data(npad+1:npad+nt); flang -O0 stops there.)  Neither cut requested is
+ * SFD_EINVAL (the reference's erexit).
+ */
+int  sfd_bandpass(sfd_handle h, const sfd_bandpass_opts *opt, int nt, int ntr, float *traces);
+
 /* Offset-dependent top and bottom mutes (MuteParam, Mute.f90:14-46; the
  * reference's defaults: v0 1500 m/s, t0 0 s, v1 10000 m/s, t1 40 s, tramp
  * 0.5 s, Init_MuteParam :30-34). */

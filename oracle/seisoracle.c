@@ -142,3 +142,148 @@ void orc_f_mute_mask(int nshots, const sfd_shot *shots, float dt, float ot, cons
         free(b);
     }
 }
+
+/* ---- Butterworth band-pass (libs/libwave/bandpass.c:72-266, called by
+ * Bandpass.f90:89-147).  The filters are cascades of first- and second-order
+ * sections applied sample by sample in FP32 with coefficients built in the
+ * reference's mix of float and double; phase 0 runs every section forward
+ * and then over the time-reversed record.  Restated here from the
+ * published recipe (Oppenheim and Schafer 1975; Hale's Butterworth
+ * sections) and pinned bit for bit to the reference's own bandpass.c,
+ * compiled from its source into oracle/_ref. */
+static const double bp_pi = 3.14159265358979323846;
+
+static int bp_sections(int *np, int phase)
+{
+    int nodd = *np % 2;
+    if (phase == 0) {
+        *np = nodd ? (*np + 1) / 2 : *np / 2;
+    }
+    return (*np + 1) / 2;
+}
+
+/* one pass of every section over x[2..n-1] (y[0], y[1] are never written) */
+static void bp_run(const float *c, int nsec, int n, float *x, float *y)
+{
+    for (int s = 0; s < nsec; s++) {
+        const float *b = c + 5 * s;
+        for (int i = 2; i < n; i++)
+            y[i] = b[0] * x[i] + b[1] * x[i - 1] + b[2] * x[i - 2] - b[3] * y[i - 1] - b[4] * y[i - 2];
+        for (int i = 0; i < n; i++) x[i] = y[i];
+    }
+}
+
+static void bp_apply(const float *c, int nsec, int phase, int n, float *data, float *nw, float *tmp)
+{
+    bp_run(c, nsec, n, data, nw);
+    if (phase == 0) {
+        for (int i = 2; i < n; i++) tmp[i] = data[n + 1 - i];
+        bp_run(c, nsec, n, tmp, nw);
+        for (int i = 2; i < n; i++) data[i] = tmp[n + 1 - i];
+    }
+}
+
+/* highcut_ (bandpass.c:72-140) coefficients: c[5*s + k], s < nsec */
+int orc_highcut_coefs(float fhi, int nphi, int phase, float *c)
+{
+    int np = nphi, nsec = bp_sections(&np, phase), nodd = np % 2;
+    float a = 2.0 * tan(bp_pi * fhi), aa = a * a, aap4 = aa + 4.0;
+    float dtheta = bp_pi / np, theta0 = nodd ? 0.0f : dtheta / 2.0;
+    if (nodd) {
+        c[0] = a / (a + 2.0);
+        c[1] = c[0];
+        c[2] = 0.0f;
+        c[3] = (a - 2.0) / (a + 2.0);
+        c[4] = 0.0f;
+    }
+    for (int j = nodd; j < nsec; j++) {
+        double cc = 4.0 * a * cos(theta0 + j * dtheta);
+        float *b = c + 5 * j;
+        b[0] = aa / (aap4 + cc);
+        b[1] = 2.0 * b[0];
+        b[2] = b[0];
+        b[3] = (2.0 * aa - 8.0) / (aap4 + cc);
+        b[4] = (aap4 - cc) / (aap4 + cc);
+    }
+    return nsec;
+}
+
+/* lowcut_ (bandpass.c:185-266): the highcut at half Nyquist moved to flo */
+int orc_lowcut_coefs(float flo, int nplo, int phase, float *c)
+{
+    int np = nplo, nsec = bp_sections(&np, phase), nodd = np % 2;
+    float fno2 = 0.25f;
+    float a = 2. * sin(bp_pi * fno2) / cos(bp_pi * fno2), aa = a * a, aap4 = aa + 4;
+    float e = -cos(bp_pi * (flo + fno2)) / cos(bp_pi * (flo - fno2)), ee = e * e;
+    float dtheta = bp_pi / np, theta0 = nodd ? 0 : dtheta / 2;
+    if (nodd) {
+        float b1 = a / (a + 2), b2 = (a - 2) / (a + 2), den = 1.0 - b2 * e;
+        c[0] = b1 * (1. - e) / den;
+        c[1] = -c[0];
+        c[2] = 0.f;
+        c[3] = (e - b2) / den;
+        c[4] = 0.f;
+    }
+    for (int j = nodd; j < nsec; j++) {
+        double cc = 4. * a * cos(theta0 + j * dtheta);
+        float b1 = aa / (aap4 + cc), b2 = (2. * aa - 8.) / (aap4 + cc), b3 = (aap4 - cc) / (aap4 + cc);
+        float den = 1. - b2 * e + b3 * ee;
+        float *d = c + 5 * j;
+        d[0] = b1 * (1. - e) * (1. - e) / den;
+        d[1] = -2. * d[0];
+        d[2] = d[0];
+        d[3] = (2. * e * (1. + b3) - b2 * (1. + ee)) / den;
+        d[4] = (ee - b2 * e + b3) / den;
+    }
+    return nsec;
+}
+
+/* highcut_ / lowcut_ on one array (the reference's calling convention) */
+void orc_highcut(float fhi, int nphi, int phase, int n, float *data, float *nw, float *tmp)
+{
+    float c[5 * 64];
+    if (nphi > 128) return;
+    int nsec = orc_highcut_coefs(fhi, nphi, phase, c);
+    bp_apply(c, nsec, phase, n, data, nw, tmp);
+}
+
+void orc_lowcut(float flo, int nplo, int phase, int n, float *data, float *nw, float *tmp)
+{
+    float c[5 * 64];
+    if (nplo > 128) return;
+    int nsec = orc_lowcut_coefs(flo, nplo, phase, c);
+    bp_apply(c, nsec, phase, n, data, nw, tmp);
+}
+
+/* Bandpass.f90:89-147 bandpass: each trace (nt samples, traces nt apart) is
+ * padded by npad = 10 samples each side with sin^2 tapers of its end values,
+ * low-cut when flo dt > 1e-6, high-cut when fhi dt < 4.999999, and the nt
+ * samples after the front pad are kept.  Returns -1 for the reference's
+ * erexit (neither cut requested). */
+int orc_f_bandpass(float flo, float fhi, int nplo, int nphi, int phase, float dt, int nt, int ntr, float *tr)
+{
+    const float pi = 3.1415926535f;
+    const float flo_t = flo * dt, fhi_t = fhi * dt;
+    const int npad = 10, n1 = nt + 2 * npad;
+    if (flo_t < 0.0001f && fhi_t > 0.4999f) return -1;
+    float *data = (float *)malloc(sizeof(float) * n1), *nw = (float *)malloc(sizeof(float) * n1),
+          *tmp = (float *)malloc(sizeof(float) * n1);
+    for (int k = 0; k < ntr; k++) {
+        float *in = tr + (size_t)k * nt;
+        for (int i = 0; i < n1; i++) data[i] = nw[i] = tmp[i] = 0.f;
+        for (int i = 0; i < nt; i++) data[npad + i] = in[i];
+        for (int j = 1; j <= npad; j++) {
+            float s1 = sinf((float)(j - 1) * pi / (float)(2 * (npad - 1)));
+            float s2 = sinf((float)(npad - j) * pi / (float)(2 * (npad - 1)));
+            data[j - 1] = in[0] * (s1 * s1);
+            data[n1 - npad + j - 1] = in[nt - 1] * (s2 * s2);
+        }
+        if (flo_t > 0.000001f) orc_lowcut(flo_t, nplo, phase, n1, data, nw, tmp);
+        if (fhi_t < 4.999999f) orc_highcut(fhi_t, nphi, phase, n1, data, nw, tmp);
+        for (int i = 0; i < nt; i++) in[i] = data[npad + i];
+    }
+    free(data);
+    free(nw);
+    free(tmp);
+    return 0;
+}

@@ -78,6 +78,11 @@ class sfd_fwi_opts(C.Structure):
     ]
 
 
+class sfd_bandpass_opts(C.Structure):
+    _fields_ = [("flo", C.c_float), ("fhi", C.c_float), ("nplo", C.c_int32), ("nphi", C.c_int32),
+                ("phase", C.c_int32)]
+
+
 class sfd_mute_opts(C.Structure):
     _fields_ = [("v0", C.c_float), ("t0", C.c_float), ("v1", C.c_float), ("t1", C.c_float), ("tramp", C.c_float)]
 
@@ -128,7 +133,7 @@ SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_forward", "sfd_forward_noalloc", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
-    "sfd_born_adjoint_exact", "sfd_lsrtm", "sfd_mute_mask",
+    "sfd_born_adjoint_exact", "sfd_lsrtm", "sfd_bandpass", "sfd_mute_mask",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
     "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgsoc_step", "sfd_lbfgsoc_step_files", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
@@ -183,6 +188,7 @@ def load():
     lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
     lib.sfd_lbfgsoc_step.argtypes = [C.c_char_p, C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p,
                                      C.c_double, C.c_void_p, P(C.c_int), P(C.c_int)]
+    lib.sfd_bandpass.argtypes = [C.c_void_p, P(sfd_bandpass_opts), C.c_int, C.c_int, C.c_void_p]
     lib.sfd_mute_mask.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_float, P(sfd_mute_opts), C.c_void_p]
     lib.sfd_lbfgsoc_step_files.argtypes = [C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p, C.c_double,
                                            C.c_void_p, P(C.c_int), P(C.c_int)]

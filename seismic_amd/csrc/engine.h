@@ -160,6 +160,7 @@ struct Engine {
     void adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind);
     void born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *mbar);
     void mute_mask(const sfd_shot *shots, int ns, float ot, const sfd_mute_opts &o, float *mask_out);
+    void bandpass(const sfd_bandpass_opts &o, int nt, int ntr, float *traces);
     void fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
                       const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
 

@@ -21,6 +21,7 @@
 #include "../../include/seisfd.h"
 #include "fd_kernels.h"
 #include "fwi_kernels.h"
+#include "bandpass_kernels.h"
 #include "engine.h"
 #include "fd_step2.h"
 #include "adjoint_kernels.h"
@@ -2066,6 +2067,123 @@ void Engine::mute_mask(const sfd_shot *shots, int ns, float ot, const sfd_mute_o
     HIPCHK(hipStreamSynchronize(stream));
 }
 
+// ---- Butterworth band-pass (Bandpass.f90:89-147 bandpass, bandpass.c) ----
+// Section coefficients in the reference's own mix of float and double
+// (bandpass.c:82-111 highcut_, :193-236 lowcut_), so the device recursion
+// sees the reference's values; returns the number of sections.
+static int bp_sections(int *np, int phase) {
+    const int nodd = *np % 2;
+    if (phase == 0) *np = nodd ? (*np + 1) / 2 : *np / 2;
+    return (*np + 1) / 2;
+}
+static const double kBpPi = 3.14159265358979323846;
+
+static int highcut_coefs(float fhi, int nphi, int phase, float *c) {
+    int np = nphi;
+    const int nsec = bp_sections(&np, phase), nodd = np % 2;
+    if (nsec > kBpMaxSections) return -1;
+    const float a = 2.0 * tan(kBpPi * fhi), aa = a * a, aap4 = aa + 4.0;
+    const float dtheta = kBpPi / np, theta0 = nodd ? 0.0f : dtheta / 2.0;
+    if (nodd) {
+        c[0] = a / (a + 2.0);
+        c[1] = c[0];
+        c[2] = 0.0f;
+        c[3] = (a - 2.0) / (a + 2.0);
+        c[4] = 0.0f;
+    }
+    for (int j = nodd; j < nsec; j++) {
+        const double cc = 4.0 * a * cos(theta0 + j * dtheta);
+        float *b = c + 5 * j;
+        b[0] = aa / (aap4 + cc);
+        b[1] = 2.0 * b[0];
+        b[2] = b[0];
+        b[3] = (2.0 * aa - 8.0) / (aap4 + cc);
+        b[4] = (aap4 - cc) / (aap4 + cc);
+    }
+    return nsec;
+}
+
+static int lowcut_coefs(float flo, int nplo, int phase, float *c) {
+    int np = nplo;
+    const int nsec = bp_sections(&np, phase), nodd = np % 2;
+    if (nsec > kBpMaxSections) return -1;
+    const float fno2 = 0.25f;
+    const float a = 2. * sin(kBpPi * fno2) / cos(kBpPi * fno2), aa = a * a, aap4 = aa + 4;
+    const float e = -cos(kBpPi * (flo + fno2)) / cos(kBpPi * (flo - fno2)), ee = e * e;
+    const float dtheta = kBpPi / np, theta0 = nodd ? 0 : dtheta / 2;
+    if (nodd) {
+        const float b1 = a / (a + 2), b2 = (a - 2) / (a + 2), den = 1.0 - b2 * e;
+        c[0] = b1 * (1. - e) / den;
+        c[1] = -c[0];
+        c[2] = 0.f;
+        c[3] = (e - b2) / den;
+        c[4] = 0.f;
+    }
+    for (int j = nodd; j < nsec; j++) {
+        const double cc = 4. * a * cos(theta0 + j * dtheta);
+        const float b1 = aa / (aap4 + cc), b2 = (2. * aa - 8.) / (aap4 + cc), b3 = (aap4 - cc) / (aap4 + cc);
+        const float den = 1. - b2 * e + b3 * ee;
+        float *d = c + 5 * j;
+        d[0] = b1 * (1. - e) * (1. - e) / den;
+        d[1] = -2. * d[0];
+        d[2] = d[0];
+        d[3] = (2. * e * (1. + b3) - b2 * (1. + ee)) / den;
+        d[4] = (ee - b2 * e + b3) / den;
+    }
+    return nsec;
+}
+
+void Engine::bandpass(const sfd_bandpass_opts &o, int nt, int ntr, float *traces) {
+    if (nt < 1 || ntr < 0 || (ntr > 0 && !traces)) throw EngineError(SFD_EINVAL, "bandpass: bad nt, ntr or traces");
+    if (o.nplo < 0 || o.nphi < 0 || !std::isfinite(o.flo) || !std::isfinite(o.fhi))
+        throw EngineError(SFD_EINVAL, "bandpass: negative pole count or non-finite cut-off");
+    const float flo_t = o.flo * g.dt, fhi_t = o.fhi * g.dt;
+    if (flo_t < 0.0001f && fhi_t > 0.4999f)   // Bandpass.f90:115 erexit
+        throw EngineError(SFD_EINVAL, "bandpass: specify flo or fhi");
+    std::vector<float> coef(2 * kBpMaxSections * 5, 0.f);
+    int nlo = 0, nhi = 0;
+    if (flo_t > 0.000001f) nlo = lowcut_coefs(flo_t, o.nplo, o.phase, coef.data());
+    if (fhi_t < 4.999999f) nhi = highcut_coefs(fhi_t, o.nphi, o.phase, coef.data() + kBpMaxSections * 5);
+    if (nlo < 0 || nhi < 0)
+        throw EngineError(SFD_EUNSUPPORTED, "bandpass: more than " + std::to_string(kBpMaxSections) +
+                                                " filter sections (poles > 16, or > 32 at zero phase)");
+    if (ntr == 0) return;
+    const float pi = 3.1415926535f;   // Bandpass.f90:100
+    float tlo[kBpPad], thi[kBpPad];
+    for (int j = 0; j < kBpPad; j++) {
+        const float s1 = sinf((float)j * pi / (float)(2 * (kBpPad - 1)));
+        const float s2 = sinf((float)(kBpPad - 1 - j) * pi / (float)(2 * (kBpPad - 1)));
+        tlo[j] = s1 * s1;
+        thi[j] = s2 * s2;
+    }
+    const int n1 = nt + 2 * kBpPad;
+    const int ch = std::min(ntr, 1 << 16);
+    const int ld = (ch + 63) & ~63;
+    Layout L;
+    const size_t o_c = L.take(coef.size() * 4), o_t = L.take(2 * kBpPad * 4), o_in = L.take((size_t)ch * nt * 4),
+                 o_w = L.take((size_t)n1 * ld * 4);
+    tmp_mem.reserve(L.off);
+    char *b = (char *)tmp_mem.p;
+    float *d_c = (float *)(b + o_c), *d_t = (float *)(b + o_t), *d_in = (float *)(b + o_in), *d_w = (float *)(b + o_w);
+    HIPCHK(hipMemcpyAsync(d_c, coef.data(), coef.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_t, tlo, kBpPad * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_t + kBpPad, thi, kBpPad * 4, hipMemcpyHostToDevice, stream));
+    for (int k0 = 0; k0 < ntr; k0 += ch) {
+        const int m = std::min(ch, ntr - k0);
+        float *h = traces + (size_t)k0 * nt;
+        HIPCHK(hipMemcpyAsync(d_in, h, (size_t)m * nt * 4, hipMemcpyHostToDevice, stream));
+        const dim3 tg((nt + 63) / 64, (m + 63) / 64);
+        hipLaunchKernelGGL(bp_pad_kernel, tg, dim3(256), 0, stream, (const float *)d_in, nt, m, ld,
+                           (const float *)d_t, (const float *)(d_t + kBpPad), d_w);
+        hipLaunchKernelGGL(bp_iir_kernel, dim3((m + 63) / 64), dim3(64), 0, stream, d_w, ld, m, n1,
+                           (const float *)d_c, nlo, nhi, o.phase);
+        hipLaunchKernelGGL(bp_unpad_kernel, tg, dim3(256), 0, stream, (const float *)d_w, nt, m, ld, d_in);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h, d_in, (size_t)m * nt * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+}
+
 void ShotPlan::take_backward(const ShotPlan &o) {
     injb_tile = o.inj_tile; injb_cell = o.inj_cell; injb_cptr = o.inj_cptr;
     injb_coef = o.inj_coef; injb_tptr = o.inj_tptr; injb_ttr = o.inj_ttr;
@@ -2228,6 +2346,11 @@ int sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const floa
     return guarded(h, [&](Engine &e) {
         e.fwi_gradient(shots, nshots, dobs, mute, vpmask, opt, misfit, grad, illum_out, resid_out);
     });
+}
+
+int sfd_bandpass(sfd_handle h, const sfd_bandpass_opts *opt, int nt, int ntr, float *traces) {
+    if (!opt) return SFD_EINVAL;
+    return guarded(h, [&](Engine &e) { e.bandpass(*opt, nt, ntr, traces); });
 }
 
 int sfd_mute_mask(sfd_handle h, const sfd_shot *shots, int nshots, float ot, const sfd_mute_opts *opt,

@@ -183,6 +183,14 @@ class Engine:
         self._chk(self.lib.sfd_born_adjoint_exact(self.h, self._shots(shots), len(shots), _ptr(db), _ptr(mb)))
         return mb.reshape(self.g.nx, self.g.nz).T.copy()
 
+    def bandpass(self, traces, flo=0.0, fhi=20.0, nplo=14, nphi=14, phase=0):
+        """Bandpass.f90 bandpass (defaults of Init_BandPassParam) on a (ntr, nt)
+        array of traces at the grid's dt; returns the filtered copy."""
+        tr = np.ascontiguousarray(np.atleast_2d(traces), np.float32).copy()
+        o = abi.sfd_bandpass_opts(flo, fhi, nplo, nphi, phase)
+        self._chk(self.lib.sfd_bandpass(self.h, C.byref(o), tr.shape[1], tr.shape[0], _ptr(tr)))
+        return tr
+
     def mute_mask(self, shots, ot=0.0, v0=1500.0, t0=0.0, v1=10000.0, t1=40.0, tramp=0.5):
         """MuteParam_compute_mask (Mute.f90:50-98; defaults of Init_MuteParam):
         one (ntr, nt) mask per shot."""

@@ -41,6 +41,11 @@ module seisfd_mod
      real(c_float)      :: norm_thresh = 0.  ! data_threshold
   end type sfd_fwi_opts
 
+  type, bind(C) :: sfd_bandpass_opts
+     real(c_float)      :: flo = 0., fhi = 20.
+     integer(c_int32_t) :: nplo = 14, nphi = 14, phase = 0
+  end type sfd_bandpass_opts
+
   type, bind(C) :: sfd_mute_opts
      real(c_float) :: v0 = 1500., t0 = 0., v1 = 10000., t1 = 40., tramp = 0.5
   end type sfd_mute_opts
@@ -160,6 +165,14 @@ module seisfd_mod
        real(c_float), intent(in) :: traces(*)
        real(c_float), intent(inout) :: image(*), illum(*)
      end function sfd_lsrtm
+
+     integer(c_int) function sfd_bandpass(h, opt, nt, ntr, traces) bind(C, name='sfd_bandpass')
+       import :: c_int, c_ptr, c_float, sfd_bandpass_opts
+       type(c_ptr), value :: h
+       type(sfd_bandpass_opts), intent(in) :: opt
+       integer(c_int), value :: nt, ntr
+       real(c_float), intent(inout) :: traces(*)
+     end function sfd_bandpass
 
      integer(c_int) function sfd_mute_mask(h, shots, nshots, ot, opt, mask) bind(C, name='sfd_mute_mask')
        import :: c_int, c_ptr, c_float, sfd_shot, sfd_mute_opts
