@@ -511,6 +511,42 @@ int  sfd_cnn_loss_grad(sfd_cnn_handle c, int nimg, const float *x, const float *
 int  sfd_cnn_train(sfd_cnn_handle c, int nimg, const float *x, const float *target, int steps,
                    float lr, double *loss_first, double *loss_last);
 
+/* ---- SEP files (the reference reads and writes every array through SEPlib:
+ *      from_aux / sreed / srite, libs/libwave/IO.f90:14-51,
+ *      ReadSouVelRho.f90:59-297).  Host-side only: these calls touch no GPU.
+ * A history file holds name=value assignments (the last one of a name wins;
+ * values may be quoted); `in` names the binary file, `esize` must be 4,
+ * `data_format` is "xdr_float" (big-endian, the default when absent) or
+ * "native_float". ------------------------------------------------------- */
+typedef struct sfd_sep_header {
+    int32_t ndim;            /* highest axis with n > 1 (>= 1)                 */
+    int64_t n[6];            /* n1..n6 (1 when absent)                         */
+    float   o[6], d[6];      /* o1..o6 (0), d1..d6 (1)                          */
+    int32_t esize;           /* bytes per element: 4                           */
+    int32_t xdr;             /* 1: big-endian data (xdr_float), 0: native       */
+    char    in[1024];        /* the binary file (resolved against the header's
+                                directory when not found as given)             */
+} sfd_sep_header;
+
+/* from_aux(tag, 'n1', …): the header of a history file. */
+int  sfd_sep_read_header(const char *hfile, sfd_sep_header *hdr);
+/* sreed: count floats from element offset of the file's data, converted to
+ * native byte order. */
+int  sfd_sep_read(const char *hfile, size_t offset, size_t count, float *data);
+/* srite + to_history: writes count = n1*…*n6 floats (native, to hdr->in or,
+ * when that is empty, to hfile followed by '@') and the history file. */
+int  sfd_sep_write(const char *hfile, const sfd_sep_header *hdr, const float *data, size_t count);
+/* readgathercoords (ReadSouVelRho.f90:161-241) with number_of_shots /
+ * number_of_traces (:828-890): the coordinate file holds n1 keys per trace
+ * and n2 traces; keys are the 1-based indexes keygx, keygy, keysx, keysy,
+ * keyselev, keygelev.  A shot is a run of consecutive traces with equal
+ * (sx, sy); its source (selev, sx) is read from its last trace, each trace's
+ * receiver is (gelev, gx).  Call with shot_first NULL to get *nshots only;
+ * then with arrays of cap_shots shots (shot_first, shot_ntr, sz, sx) and
+ * cap_traces traces (rz, rx); any of sz, sx, rz, rx may be NULL. */
+int  sfd_sep_gathers(const char *coordfile, const int keys[6], int *nshots, int *shot_first, int *shot_ntr,
+                     float *sz, float *sx, float *rz, float *rx, int cap_shots, int cap_traces);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,11 @@ class sfd_mute_opts(C.Structure):
     _fields_ = [("v0", C.c_float), ("t0", C.c_float), ("v1", C.c_float), ("t1", C.c_float), ("tramp", C.c_float)]
 
 
+class sfd_sep_header(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("n", C.c_int64 * 6), ("o", C.c_float * 6), ("d", C.c_float * 6),
+                ("esize", C.c_int32), ("xdr", C.c_int32), ("in_", C.c_char * 1024)]
+
+
 class sfd_lbfgs_opts(C.Structure):
     _fields_ = [
         ("niter", C.c_int), ("neval", C.c_int), ("m", C.c_int), ("const_type", C.c_int),
@@ -139,6 +144,7 @@ EXPORTS = [
     "sfd_lbfgsoc_step", "sfd_lbfgsoc_step_files", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
+    "sfd_sep_read_header", "sfd_sep_read", "sfd_sep_write", "sfd_sep_gathers",
 ]
 
 _lib = None
@@ -188,6 +194,11 @@ def load():
     lib.sfd_lbfgs_destroy.argtypes = [C.c_void_p]
     lib.sfd_lbfgsoc_step.argtypes = [C.c_char_p, C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p,
                                      C.c_double, C.c_void_p, P(C.c_int), P(C.c_int)]
+    lib.sfd_sep_read_header.argtypes = [C.c_char_p, P(sfd_sep_header)]
+    lib.sfd_sep_read.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, C.c_void_p]
+    lib.sfd_sep_write.argtypes = [C.c_char_p, P(sfd_sep_header), C.c_void_p, C.c_size_t]
+    lib.sfd_sep_gathers.argtypes = [C.c_char_p, P(C.c_int), P(C.c_int), C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     lib.sfd_bandpass.argtypes = [C.c_void_p, P(sfd_bandpass_opts), C.c_int, C.c_int, C.c_void_p]
     lib.sfd_mute_mask.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_float, P(sfd_mute_opts), C.c_void_p]
     lib.sfd_lbfgsoc_step_files.argtypes = [C.c_char_p, P(sfd_lbfgsoc_opts), C.c_int, C.c_void_p, C.c_double,

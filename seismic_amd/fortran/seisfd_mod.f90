@@ -50,6 +50,15 @@ module seisfd_mod
      real(c_float) :: v0 = 1500., t0 = 0., v1 = 10000., t1 = 40., tramp = 0.5
   end type sfd_mute_opts
 
+  ! SEP history-file header (sfd_sep_read_header / sfd_sep_write)
+  type, bind(C) :: sfd_sep_header
+     integer(c_int32_t) :: ndim
+     integer(c_int64_t) :: n(6)
+     real(c_float)      :: o(6), d(6)
+     integer(c_int32_t) :: esize, xdr
+     character(kind=c_char) :: in(1024)
+  end type sfd_sep_header
+
   type, bind(C) :: sfd_lbfgs_opts
      integer(c_int) :: niter, neval, m, const_type, freeze_soft, eval0
      real(c_double) :: xmin, xmax
@@ -165,6 +174,38 @@ module seisfd_mod
        real(c_float), intent(in) :: traces(*)
        real(c_float), intent(inout) :: image(*), illum(*)
      end function sfd_lsrtm
+
+     ! SEP files (host side): from_aux / sreed / srite of the reference
+     integer(c_int) function sfd_sep_read_header(hfile, hdr) bind(C, name='sfd_sep_read_header')
+       import :: c_int, c_char, sfd_sep_header
+       character(kind=c_char), intent(in) :: hfile(*)
+       type(sfd_sep_header), intent(out) :: hdr
+     end function sfd_sep_read_header
+
+     integer(c_int) function sfd_sep_read(hfile, offset, count, data) bind(C, name='sfd_sep_read')
+       import :: c_int, c_char, c_size_t, c_float
+       character(kind=c_char), intent(in) :: hfile(*)
+       integer(c_size_t), value :: offset, count
+       real(c_float), intent(out) :: data(*)
+     end function sfd_sep_read
+
+     integer(c_int) function sfd_sep_write(hfile, hdr, data, count) bind(C, name='sfd_sep_write')
+       import :: c_int, c_char, c_size_t, c_float, sfd_sep_header
+       character(kind=c_char), intent(in) :: hfile(*)
+       type(sfd_sep_header), intent(in) :: hdr
+       real(c_float), intent(in) :: data(*)
+       integer(c_size_t), value :: count
+     end function sfd_sep_write
+
+     integer(c_int) function sfd_sep_gathers(coordfile, keys, nshots, shot_first, shot_ntr, sz, sx, rz, rx, &
+          cap_shots, cap_traces) bind(C, name='sfd_sep_gathers')
+       import :: c_int, c_char, c_ptr
+       character(kind=c_char), intent(in) :: coordfile(*)
+       integer(c_int), intent(in) :: keys(6)
+       integer(c_int), intent(out) :: nshots
+       type(c_ptr), value :: shot_first, shot_ntr, sz, sx, rz, rx
+       integer(c_int), value :: cap_shots, cap_traces
+     end function sfd_sep_gathers
 
      integer(c_int) function sfd_bandpass(h, opt, nt, ntr, traces) bind(C, name='sfd_bandpass')
        import :: c_int, c_ptr, c_float, sfd_bandpass_opts
