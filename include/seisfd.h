@@ -60,9 +60,24 @@ typedef struct sfd_grid {
     int32_t lsinc;           /* sinc interpolator length (7)                  */
     int32_t snapi;           /* snapshot interval (steps)                     */
     float   aperture_x;      /* migration aperture (m), ExtractPadModel.f90:55 */
-    int32_t rec_type;        /* 0 only (no receiver ghost)                    */
-    int32_t shot_type;       /* 0 only (no source ghost)                      */
-    int32_t surf_type;       /* 0 only (absorbing top, Boundary.f90:1623)      */
+    int32_t rec_type;        /* != 0: receivers read u(r) - u(mirror of r)
+                                (Extraction.f90:46-48,88-90,548-556); the
+                                simple receiver injection of FWI's adjoint
+                                pass adds the mirror source (Injection.f90:583) */
+    int32_t shot_type;       /* != 0: sinc injections (sources, RTM/LSRTM
+                                receivers) add the mirror source
+                                (Injection.f90:432,447-450); > 0 also extends
+                                the FD rows above the model to ntaper+5
+                                (ExtractPadModel.f90:229)                     */
+    int32_t surf_type;       /* != 0: free surface at the elevation of
+                                sfd_set_elevation, set on u2 every step
+                                (Boundary.f90:1609-1658), snapshots zero above
+                                it (Extraction.f90:580-592); > 0 also drops
+                                the top Higdon band and keeps lsinc FD rows
+                                above the model (Boundary.f90:104-124,
+                                ExtractPadModel.f90:230).  The exact adjoints
+                                (sfd_adjoint_exact, sfd_born_adjoint_exact)
+                                need all three types 0.                       */
     float   fmax;            /* source's highest frequency (Hz) for vel_check's
                                 sampling stop; 0: the reference default 30
                                 (ReadParam.f90:18)                            */
@@ -127,6 +142,18 @@ int  sfd_device_info(char *name, int len, int *cu_count, size_t *hbm_bytes);
  * `vel2` used by the RTM backward pass, to_disk_to_memory_RTM.f90:438).
  */
 int  sfd_set_model(sfd_handle h, const float *vel, const float *vel_bwd);
+
+/*
+ * Free-surface elevation (m, depth axis) per model column, nx values; NULL
+ * (the default) is 0 everywhere, which is what the reference's drivers set
+ * (progs/rtm/Acoustic_RTM_sep.f90:118-119, 3DRTM.f90:74-75).  Replaces
+ * ModelSpace_elevation%elev and ModelSpace_elevation_parameters
+ * (libs/libwave/ModelSpaceTypes.f90:68-73,128-141): the surface cell
+ * ielev_z = nint((elev - oz)/dz) + 1 must lie in 1..nz.  FD columns outside a
+ * shot's window take the nearest window column's elevation.  Read only when
+ * surf_type != 0.
+ */
+int  sfd_set_elevation(sfd_handle h, const float *elev);
 
 /*
  * Forward modeling of nshots shots (Acoustic_modeling_sep.f90:80-88 for one

@@ -87,13 +87,20 @@ contains
   !            2 Extraction_array_LSRTM_sinc, 3 Extraction_array_sinc_afwi
   !   tdir   : +1 it=1..nt, -1 it=nt..1
   !   wave   : (nz,nxw,nsnap) captured u2 every snapi steps (Extraction_wavefield)
+  !   rec_type, shot_type, surf_type: GeneralParam's ghost / free-surface
+  !            switches; elevj: elev%elev over the FD columns nmin2..nmax2
+  !            (read when surf_type /= 0).  The FD bounds follow
+  !            read_window_vel's rule (ExtractPadModel.f90:228-233, which needs
+  !            SEPlib and is not compiled); vel_pad must have those bounds.
   subroutine ref_propagate(nz, nxw, ntaper, dz, dx, dt, oz, oxw, lsinc, snapi, vel_pad, &
        &                   stencil, inj, nsou, souz, soux, soutr, ext, ndat, datz, datx, dattr, &
-       &                   nt, tdir, nsnap, wave) bind(C, name='ref_propagate')
+       &                   nt, tdir, nsnap, wave, rec_type, shot_type, surf_type, elevj) &
+       &                   bind(C, name='ref_propagate')
     integer(c_int), value :: nz, nxw, ntaper, lsinc, snapi, stencil, inj, nsou, ext, ndat, nt, tdir, nsnap
+    integer(c_int), value :: rec_type, shot_type, surf_type
     real(c_float), value  :: dz, dx, dt, oz, oxw
     real(c_float)         :: vel_pad(*), souz(*), soux(*), soutr(nt, *), datz(*), datx(*), dattr(nt, *)
-    real(c_float)         :: wave(nz, nxw, *)
+    real(c_float)         :: wave(nz, nxw, *), elevj(*)
 
     type(GeneralParam)         :: genpar
     type(FDbounds)             :: bounds
@@ -122,9 +129,9 @@ contains
     genpar%ntaper = ntaper
     genpar%snapi = snapi
     genpar%ntsnap = nsnap
-    genpar%rec_type = 0
-    genpar%shot_type = 0
-    genpar%surf_type = 0
+    genpar%rec_type = rec_type
+    genpar%shot_type = shot_type
+    genpar%surf_type = surf_type
     genpar%coefpower = 2
     if (tdir > 0) then
        genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
@@ -132,7 +139,12 @@ contains
        genpar%tmin = nt; genpar%tmax = 1; genpar%tstep = -1
     end if
 
-    bounds%nmin1 = -ntaper + 1
+    if (shot_type > 0 .or. surf_type > 0) then
+       if (shot_type > 0) bounds%nmin1 = -(ntaper + lsinc/2 + 2) + 1
+       if (surf_type > 0) bounds%nmin1 = -lsinc + 1
+    else
+       bounds%nmin1 = -ntaper + 1
+    end if
     bounds%nmax1 = nz + ntaper
     bounds%nmin2 = -ntaper + 1
     bounds%nmax2 = nxw + ntaper
@@ -182,6 +194,13 @@ contains
     call Higdon(genpar%dt, model, bounds, hig)
     if (nsou > 0) call ModelSpace_compute_array_xyz_positions(genpar, sou(1:nsou))
     if (ndat > 0) call ModelSpace_compute_array_xyz_positions(genpar, dat(1:ndat))
+    ! Propagator.f90:146-150 allocates the elevation tables; the drivers fill elev
+    allocate(elev%elev(bounds%nmin2:bounds%nmax2, 1:1))
+    allocate(elev%ielev_z(bounds%nmin2:bounds%nmax2, 1:1))
+    allocate(elev%delev_z(bounds%nmin2:bounds%nmax2, 1:1))
+    do j = bounds%nmin2, bounds%nmax2
+       elev%elev(j, 1) = elevj(j - bounds%nmin2 + 1)
+    end do
     call ModelSpace_elevation_parameters(elev, bounds, genpar)
     wfld%counter = 0
 
@@ -231,6 +250,7 @@ contains
     end if
     call deallocateHigdonParam(hig)
     call deallocateUSpace(grid)
+    call deallocateModelSpace_elev(elev)
   end subroutine ref_propagate
 
 end module ref_harness

@@ -46,15 +46,43 @@ typedef struct {
 /* index of Fortran (i,j) in an array allocated (nmin1-4:nmax1+4, nmin2-4:nmax2+4) */
 #define ORC_IDX(b, i, j) ((size_t)((i) - ((b)->nmin1 - 4)) + (size_t)((j) - ((b)->nmin2 - 4)) * (size_t)(b)->NZ)
 
-/* read_window_vel bounds, ExtractPadModel.f90:228-239 (shot/surf type 0) */
-void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
+/* read_window_vel bounds, ExtractPadModel.f90:228-239: a ghost source keeps
+ * ntaper + lsinc/2 + 2 FD rows above the model, a free surface lsinc - 1
+ * (the surface wins when both are set). */
+void orc_bounds_t(ORC_BOUNDS *b, int nz, int nxw, int ntaper, int lsinc, int shot_type, int surf_type)
 {
-    b->nmin1 = -ntaper + 1;
+    if (shot_type > 0 || surf_type > 0) {
+        if (shot_type > 0) b->nmin1 = -(ntaper + lsinc / 2 + 2) + 1;
+        if (surf_type > 0) b->nmin1 = -lsinc + 1;
+    } else {
+        b->nmin1 = -ntaper + 1;
+    }
     b->nmax1 = nz + ntaper;
     b->nmin2 = -ntaper + 1;
     b->nmax2 = nxw + ntaper;
     b->NZ = b->nmax1 - b->nmin1 + 9;
     b->NX = b->nmax2 - b->nmin2 + 9;
+}
+
+void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
+{
+    orc_bounds_t(b, nz, nxw, ntaper, 7, 0, 0);
+}
+
+/* Free-surface elevation per model column (ModelSpace_elevation%elev, which
+ * the reference's drivers set to 0): a copy held for the oracle's calls;
+ * NULL or n = 0 restores 0 everywhere. */
+static float *orc_elev = NULL;
+static int orc_elev_n = 0;
+void orc_set_elevation(const float *e, int n)
+{
+    free(orc_elev);
+    orc_elev = NULL;
+    orc_elev_n = 0;
+    if (!e || n < 1) return;
+    orc_elev = (float *)malloc(sizeof(float) * (size_t)n);
+    memcpy(orc_elev, e, sizeof(float) * (size_t)n);
+    orc_elev_n = n;
 }
 
 #define REAL float

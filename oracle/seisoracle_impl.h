@@ -123,6 +123,8 @@ typedef struct {
     int icounter;
     int tmin, tmax, tstep;
     REAL *capture; int ncapture;  /* optional: u2 at imaging steps (checks) */
+    int rec_type, shot_type, surf_type;
+    int *ielev; REAL *delev;      /* per FD column nmin2..nmax2 (surf_type != 0) */
 } CAT(PROP_, SFX);
 #define PROP CAT(PROP_, SFX)
 
@@ -272,13 +274,15 @@ static void CAT(higdon_factor_, SFX)(PROP *P, const REAL *U1, const REAL *U2, RE
     AT(P, U3, i, j) = -f;
 }
 
-/* Boundary.f90:89-184 Boundary_3d, 2D, surf_type = 0 (nz0 = nmin1, nstencil = 3) */
+/* Boundary.f90:89-184 Boundary_3d, 2D (nstencil = 3): with surf_type > 0
+ * the top band is dropped and the side bands start at row nz0 = 1. */
 static void CAT(boundary_, SFX)(PROP *P, const REAL *u1, const REAL *u2, REAL *u3)
 {
     const ORC_BOUNDS *b = &P->b;
-    int i, j, nz0 = b->nmin1;
+    int i, j, nz0 = P->surf_type > 0 ? 1 : b->nmin1;
     for (j = b->nmin2; j <= b->nmax2; j++) {
-        for (i = nz0 + 3; i >= b->nmin1 - 4; i--) CAT(higdon_factor_, SFX)(P, u1, u2, u3, i, j, -1);
+        if (P->surf_type <= 0)
+            for (i = nz0 + 3; i >= b->nmin1 - 4; i--) CAT(higdon_factor_, SFX)(P, u1, u2, u3, i, j, -1);
         for (i = b->nmax1 - 3; i <= b->nmax1 + 4; i++) CAT(higdon_factor_, SFX)(P, u1, u2, u3, i, j, 1);
     }
     for (j = b->nmin2 + 3; j >= b->nmin2 - 4; j--)
@@ -287,12 +291,16 @@ static void CAT(boundary_, SFX)(PROP *P, const REAL *u1, const REAL *u2, REAL *u
         for (i = nz0; i <= b->nmax1; i++) CAT(higdon_factor_, SFX)(P, u1, u2, u3, i, j, 2);
 }
 
-/* Injection.f90:151-194 Injection_sinc -> :402-456 Injection_source_sinc_xz (shot_type 0),
- * :63-104 Injection_simple_noomp -> :561-589, :239-272 Injection_LSRTM_sinc */
+/* Injection.f90:151-194 Injection_sinc -> :402-456 Injection_source_sinc_xz,
+ * :63-104 Injection_simple_noomp -> :561-589, :239-272 Injection_LSRTM_sinc.
+ * Ghosts: the sinc form tests shot_type (:432), the simple form type_inject
+ * = rec_type when tmax is 1, else shot_type (:74-78); the mirror term is
+ * subtracted at row -(row) right after the direct one (:447-450, :583-586). */
 static void CAT(inject_, SFX)(PROP *P, REAL *u, int it)
 {
     int k, i, j, l = P->lsinc;
     int minx = (int)(-l * 0.5), maxx = (int)(l * 0.5);
+    int type_inject = P->tmax == 1 ? P->rec_type : P->shot_type;
     REAL sz[64], sx[64];
     for (k = 0; k < P->nsou; k++) {
         TR *s = &P->sou[k];
@@ -305,14 +313,20 @@ static void CAT(inject_, SFX)(PROP *P, REAL *u, int it)
         }
         if (P->inject == 2) {
             AT(P, u, s->icoord[0], s->icoord[1]) = AT(P, u, s->icoord[0], s->icoord[1]) + v2r * tr;
+            if (type_inject != 0)
+                AT(P, u, -s->icoord[0], s->icoord[1]) = AT(P, u, -s->icoord[0], s->icoord[1]) - v2r * tr;
             continue;
         }
         FN(mksinc)(sz, l, s->dcoord[0] / P->dz);
         FN(mksinc)(sx, l, s->dcoord[1] / P->dx);
         for (j = minx; j <= maxx; j++)
-            for (i = minx; i <= maxx; i++)
+            for (i = minx; i <= maxx; i++) {
                 AT(P, u, i + s->icoord[0], j + s->icoord[1]) =
                     AT(P, u, i + s->icoord[0], j + s->icoord[1]) + v2r * tr * sz[maxx + i] * sx[maxx + j];
+                if (P->shot_type != 0)
+                    AT(P, u, i - s->icoord[0], j + s->icoord[1]) =
+                        AT(P, u, i - s->icoord[0], j + s->icoord[1]) - v2r * tr * sz[maxx - i] * sx[maxx + j];
+            }
     }
 }
 
@@ -327,7 +341,11 @@ static void CAT(extract_, SFX)(PROP *P, const REAL *u, int it)
         TR *d = &P->dat[k];
         REAL g, acc;
         if (P->extract == SFD_EXTRACT_NEAREST) {
-            d->trace[it - 1] = d->trace[it - 1] + AT(P, u, d->icoord[0], d->icoord[1]);
+            if (P->rec_type != 0)   /* Extraction.f90:88-90 */
+                d->trace[it - 1] = d->trace[it - 1] + (AT(P, u, d->icoord[0], d->icoord[1]) -
+                                                       AT(P, u, -d->icoord[0], d->icoord[1]));
+            else
+                d->trace[it - 1] = d->trace[it - 1] + AT(P, u, d->icoord[0], d->icoord[1]);
             continue;
         }
         if (P->extract == SFD_EXTRACT_SINC_V2) {
@@ -341,14 +359,22 @@ static void CAT(extract_, SFX)(PROP *P, const REAL *u, int it)
         FN(mksinc)(sz, l, d->dcoord[0] / P->dz);
         FN(mksinc)(sx, l, d->dcoord[1] / P->dx);
         acc = d->trace[it - 1];
-        for (j = minx; j <= maxx; j++)
-            for (i = minx; i <= maxx; i++)
-                acc = acc + g * AT(P, u, i + d->icoord[0], j + d->icoord[1]) * sz[maxx + i] * sx[maxx + j];
+        if (P->rec_type != 0) {   /* Extraction.f90:548-556 */
+            for (j = minx; j <= maxx; j++)
+                for (i = minx; i <= maxx; i++)
+                    acc = acc + g * (AT(P, u, i + d->icoord[0], j + d->icoord[1]) -
+                                     AT(P, u, -i - d->icoord[0], j + d->icoord[1])) * sz[maxx + i] * sx[maxx + j];
+        } else {
+            for (j = minx; j <= maxx; j++)
+                for (i = minx; i <= maxx; i++)
+                    acc = acc + g * AT(P, u, i + d->icoord[0], j + d->icoord[1]) * sz[maxx + i] * sx[maxx + j];
+        }
         d->trace[it - 1] = acc;
     }
 }
 
-/* Extraction.f90:562-601 Extraction_wavefield (surf_type 0) */
+/* Extraction.f90:562-601 Extraction_wavefield: with a free surface the rows
+ * above ielev_z of each column are zero */
 static void CAT(extract_wave_, SFX)(PROP *P, const REAL *u, int it)
 {
     if (it % P->snapi != 0) return;
@@ -356,8 +382,38 @@ static void CAT(extract_wave_, SFX)(PROP *P, const REAL *u, int it)
     if (P->wcounter > P->ntsnap) return;
     REAL *w = P->wave + (size_t)(P->wcounter - 1) * P->nz * P->nxw;
 #pragma omp parallel for schedule(static)
-    for (int jj = 1; jj <= P->nxw; jj++)
-        for (int ii = 1; ii <= P->nz; ii++) w[(ii - 1) + (size_t)(jj - 1) * P->nz] = AT(P, u, ii, jj);
+    for (int jj = 1; jj <= P->nxw; jj++) {
+        const int i0 = P->surf_type != 0 ? P->ielev[jj - P->b.nmin2] : 1;
+        for (int ii = 1; ii <= P->nz; ii++)
+            w[(ii - 1) + (size_t)(jj - 1) * P->nz] = ii >= i0 ? AT(P, u, ii, jj) : (REAL)0;
+    }
+}
+
+/* Boundary.f90:1609-1658 Boundary_set_free_surface_grid on u2 */
+static void CAT(free_surface_, SFX)(PROP *P, REAL *u)
+{
+    const ORC_BOUNDS *b = &P->b;
+    const int h = P->lsinc / 2;
+    REAL sinc[64], ui[64];
+    if (P->surf_type == 0) return;
+    for (int j = b->nmin2; j <= b->nmax2; j++) {
+        const int ie = P->ielev[j - b->nmin2];
+        const REAL de = P->delev[j - b->nmin2];
+        for (int i = b->nmin1; i <= ie - 1; i++) AT(P, u, i, j) = 0;
+        if (de == 0) {
+            for (int i = 0; i <= 2; i++) AT(P, u, ie - 2 - i, j) = -AT(P, u, ie + i, j);
+        } else {
+            FN(mksinc)(sinc, P->lsinc, de / P->dz);
+            for (int i = -h; i <= h; i++) {
+                REAL f = 0;
+                for (int k = -h; k <= h; k++) f = f + AT(P, u, ie + k + i, j) * sinc[h + k];
+                ui[h + i] = f;
+            }
+            for (int i = 2; i <= h + 1; i++)
+                for (int k = -h; k <= i - 1; k++)
+                    AT(P, u, ie - i + k, j) = AT(P, u, ie - i + k, j) - ui[h + i - 2] * sinc[h + k];
+        }
+    }
 }
 
 /* Imaging.f90:228-275 (RTM), :322-370 (FWI), :698-745 (LSRTM), :747-788 (Born) */
@@ -373,7 +429,8 @@ static void CAT(imaging_, SFX)(PROP *P, REAL *u, int it)
 #pragma omp parallel for schedule(static)
         for (int j = 1; j <= P->nxw; j++) {
             REAL t = P->taperx[j - 1];
-            for (int i = 1; i <= P->nz; i++) {
+            const int i0 = P->surf_type != 0 ? P->ielev[j - P->b.nmin2] : 1;   /* Imaging.f90:762-770 */
+            for (int i = i0; i <= P->nz; i++) {
                 size_t q = (i - 1) + (size_t)(j - 1) * P->nz;
                 AT(P, u, i, j) = AT(P, u, i, j) + (REAL)2 * t * P->born_m[q] * S[q] / AT(P, P->vel, i, j);
             }
@@ -396,7 +453,8 @@ static void CAT(imaging_, SFX)(PROP *P, REAL *u, int it)
     for (int jj = 1; jj <= P->nxw; jj++) {
         const int j = jj;
         REAL t = P->taperx[j - 1];
-        for (int i = 1; i <= P->nz; i++) {
+        const int i0 = P->surf_type != 0 ? P->ielev[j - P->b.nmin2] : 1;   /* Imaging.f90:247-256 */
+        for (int i = i0; i <= P->nz; i++) {
             size_t q = (i - 1) + (size_t)(j - 1) * P->nz;
             REAL uu = AT(P, u, i, j);
             if (P->imaging == 1) {
@@ -437,6 +495,7 @@ static void CAT(propagate_, SFX)(PROP *P)
         if (P->extract >= 0 && P->ndat) CAT(extract_, SFX)(P, buf[3], it);
         if (P->wave) CAT(extract_wave_, SFX)(P, buf[3], it);
         if (P->imaging && P->imaging != 4 && P->tstep == -1) CAT(imaging_, SFX)(P, buf[3], it);
+        CAT(free_surface_, SFX)(P, buf[3]);
         if (P->stencil == 0) CAT(fd_forward_, SFX)(P, buf[3], buf[4]);
         else CAT(fd_adjoint_, SFX)(P, buf[3], buf[4]);
         if (P->inject && P->nsou) CAT(inject_, SFX)(P, buf[4], it);
@@ -518,11 +577,31 @@ static int CAT(setup_shot_, SFX)(PROP *P, const sfd_grid *g, const REAL *vel_ful
     P->oz = g->oz; P->dz = g->dz; P->dx = g->dx; P->dt = g->dt;
     P->lsinc = g->lsinc; P->snapi = g->snapi;
     P->ntsnap = s->nt / g->snapi;
-    orc_bounds(&P->b, g->nz, P->nxw, g->ntaper);
+    orc_bounds_t(&P->b, g->nz, P->nxw, g->ntaper, g->lsinc, g->shot_type, g->surf_type);
     P->vel = (REAL *)calloc((size_t)P->b.NZ * P->b.NX, sizeof(REAL));
     if (!P->vel) return -2;
     FN(window_pad)(g, vel_full, P->oxw, P->nxw, &P->b, P->vel);
     P->extract = -1;
+    P->rec_type = g->rec_type;
+    P->shot_type = g->shot_type;
+    P->surf_type = g->surf_type;
+    if (g->surf_type != 0) {
+        /* ModelSpace_elevation_parameters (ModelSpaceTypes.f90:128-141): find_i_d
+         * of each FD column's elevation; columns off the window take the
+         * nearest window column's, and that column the nearest model column's */
+        const int n2 = P->b.nmax2 - P->b.nmin2 + 1;
+        P->ielev = (int *)calloc((size_t)n2, sizeof(int));
+        P->delev = (REAL *)calloc((size_t)n2, sizeof(REAL));
+        for (int j = P->b.nmin2; j <= P->b.nmax2; j++) {
+            int jw = j < 1 ? 1 : (j > P->nxw ? P->nxw : j);
+            REAL cxx = (REAL)(jw - 1) * (REAL)g->dx + P->oxw;
+            int ix = NINT((cxx - (REAL)g->ox) / (REAL)g->dx) + 1;
+            ix = ix < 1 ? 1 : (ix > g->nx ? g->nx : ix);
+            REAL e = (orc_elev && ix <= orc_elev_n) ? (REAL)orc_elev[ix - 1] : (REAL)0;
+            CAT(find_i_d_, SFX)(&P->ielev[j - P->b.nmin2], &P->delev[j - P->b.nmin2], e, P->oz, P->dz);
+            if (P->ielev[j - P->b.nmin2] < 1 || P->ielev[j - P->b.nmin2] > g->nz) return -1;
+        }
+    }
     return 0;
 }
 
@@ -579,7 +658,7 @@ int FN(modeling)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const RE
         memcpy(traces + (size_t)r * s->nt, P.dat[r].trace, sizeof(REAL) * s->nt);
     CAT(free_traces_, SFX)(P.sou, 1);
     CAT(free_traces_, SFX)(P.dat, s->ntr);
-    free(P.vel);
+    free(P.vel); free(P.ielev); free(P.delev);
     return 0;
 }
 
@@ -743,13 +822,14 @@ static int CAT(adjoint_core_, SFX)(PROP P, const sfd_shot *s, const REAL *dbar, 
     free(P.gz); free(P.gx);
     CAT(free_traces_, SFX)(P.sou, 1);
     CAT(free_traces_, SFX)(P.dat, s->ntr);
-    free(P.vel);
+    free(P.vel); free(P.ielev); free(P.delev);
     return 0;
 }
 
 int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *dbar, REAL *sbar,
                          int extract_kind)
 {
+    if (g->rec_type || g->shot_type || g->surf_type) return -4;   /* no free surface or ghosts */
     PROP P;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
     return CAT(adjoint_core_, SFX)(P, s, dbar, sbar, extract_kind, NULL, NULL, NULL);
@@ -765,6 +845,7 @@ int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, 
 int FN(born_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *wavelet,
                      const REAL *dbar, REAL *mbar)
 {
+    if (g->rec_type || g->shot_type || g->surf_type) return -4;   /* no free surface or ghosts */
     PROP P;
     float zz = s->sz, xx = s->sx;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -844,7 +925,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
     if (win_image) memcpy(win_image, img, n * sizeof(REAL));
     if (win_illum) memcpy(win_illum, ill, n * sizeof(REAL));
     CAT(free_traces_, SFX)(P.sou, s->ntr);
-    free(snaps); free(img); free(ill); free(tap); free(P.vel);
+    free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev);
     return 0;
 }
 
@@ -883,7 +964,7 @@ int FN(born)(const sfd_grid *g, const REAL *vel, const REAL *refl, const sfd_sho
     CAT(propagate_, SFX)(&P);
     for (int r = 0; r < s->ntr; r++) memcpy(traces + (size_t)r * s->nt, P.dat[r].trace, sizeof(REAL) * s->nt);
     CAT(free_traces_, SFX)(P.dat, s->ntr);
-    free(snaps); free(m); free(mw); free(tap); free(P.vel);
+    free(snaps); free(m); free(mw); free(tap); free(P.vel); free(P.ielev); free(P.delev);
     return 0;
 }
 
@@ -930,7 +1011,7 @@ int FN(test_agrad_capture)(const sfd_grid *g, const REAL *vel, const sfd_shot *s
     P.tmin = s->nt; P.tmax = 1; P.tstep = -1;
     CAT(propagate_, SFX)(&P);
     CAT(free_traces_, SFX)(P.sou, s->ntr);
-    free(snaps); free(img); free(ill); free(tap); free(P.vel);
+    free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev);
     return 0;
 }
 
@@ -1037,7 +1118,8 @@ void FN(gdt_compute)(int ntype, REAL *rd, int nd, REAL t)
  * norms (Norm.f90:12-141: o->norm, o->norm_thresh) and AMOD/AGRAD (to_disk_to_memory_AMOD.f90:27-82,
  * to_disk_to_memory_AGRAD.f90:27-78).  dobs/resid: per shot (nt,ntr) blocks
  * concatenated; mute same layout (NULL = 1); vpmask (nz*nx, NULL = 1).
- * All shots share nt (invparam%n1).  grad/illu_out are (nz,nx). */
+ * Shots may differ in nt (ragged batch; the misfit scaling sums each
+ * shot's 2 nt ntr).  grad/illu_out are (nz,nx). */
 int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_shot *shots,
                      const REAL *wavelet, const REAL *dobs, const REAL *mute, const REAL *vpmask,
                      const sfd_fwi_opts *o, double *fout, REAL *grad, REAL *illu_out, REAL *resid)
@@ -1095,14 +1177,19 @@ int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_s
         FN(copy_image)(g, P.oxw, P.nxw, img, grad);
         FN(copy_image)(g, P.oxw, P.nxw, ill, illu);
         CAT(free_traces_, SFX)(P.sou, s->ntr);
-        free(snaps); free(img); free(ill); free(tap); free(P.vel);
+        free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev);
         off += (size_t)s->nt * s->ntr;
         ntot += s->ntr;
     }
     /* :332-343 */
     if (o->illupow != 1.0f)
         for (size_t q = 0; q < N; q++) illu[q] = POW(illu[q], (REAL)o->illupow);
-    double scaling = (double)(2 * n1 * ntot);
+    /* f / (2 n1 ntr) of :332-336 with the reference's one nt; a ragged batch
+     * (shots of their own nt) sums 2 nt ntr over its shots, which is the same
+     * number when every nt is n1 */
+    double scaling = 0;
+    for (int is = 0; is < nshots; is++) scaling += 2.0 * (double)shots[is].nt * (double)shots[is].ntr;
+    (void)n1; (void)ntot;
     f = f / scaling;
     REAL mx = illu[0];
     double ss = 0;

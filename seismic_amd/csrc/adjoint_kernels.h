@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k7_born_kernel(AdjStep A, int snap_idx, f
     const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (idx >= (long)S.nz * S.nxw) return;
     const int wz = (int)(idx % S.nz), wx = (int)(idx / S.nz);
-    const int iz = wz + S.i0, ix = wx + S.i0;
+    const int iz = wz + S.iz0, ix = wx + S.i0;
     if (iz < 4 || iz > S.NZ - 5 || ix < 4 || ix > S.NX - 5) return;
     const size_t q = (size_t)ix * S.LDZ + S.zoff + iz;
     const float Sv = ((const float *)S.snap)[(size_t)(snap_idx - 1) * S.nz * S.nxw + idx];

@@ -32,6 +32,10 @@ struct ShotPlan {
     float oxw = 0;
     int nxw = 0, nt = 0, ntr = 0, ntsnap = 0;
     int NZ = 0, NX = 0, LDZ = 0, zoff = 0, zsh = 0, xsh = 0, ntz = 0, ntx = 0, tile0 = 0, i0 = 0;
+    int ntop = 0, iz0 = 0;         // FD rows above model row 1 (1 - nmin1) and the padded row of row 1
+    std::vector<int> fs_row;       // free surface: per padded column, the padded row of ielev_z
+    std::vector<float> fs_w;       // per padded column: sinc weights of delev_z / dz and an off-grid flag
+    int snap_mask = 0;             // some window column's surface lies below model row 1
     int n_inj_traces = 1;
     std::vector<int> colsrc, colmap_raw;
     std::vector<float> taper;
@@ -110,6 +114,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     std::vector<float> vel_h, velb_h;
+    std::vector<float> elev_h;            // free-surface elevation per model column (empty: 0 everywhere)
     uint64_t model_version = 0;           // bumped by every set_model (plans read vel_h / velb_h)
     std::vector<ShotPlan> rtm_plans;      // host plan behind fwd_batch when its key is an RTM's
     float *d_vel = nullptr, *d_velb = nullptr;
@@ -171,6 +176,9 @@ struct Engine {
     void plan_shot(const sfd_shot &s, ShotPlan &p) const;
     void plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz, const float *sx,
                         int kind) const;
+    bool surface() const { return g.surf_type != 0; }
+    bool ghosts_or_surface() const { return g.surf_type != 0 || g.shot_type != 0 || g.rec_type != 0; }
+    void set_elevation(const float *elev);
     void plan_extraction(ShotPlan &p, const std::vector<float> &vh, const sfd_shot &s, int kind) const;
     void build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeeds &need);
     void reset_batch(Batch &B);

@@ -74,7 +74,8 @@ struct ShotDev {
     int NZ, NX, LDZ, zoff;  // padded dims, leading dim, storage offset of row 0
     int zsh, xsh;           // tile-origin shifts (tile k starts at k*T - sh)
     int ntz, ntx, tile0;    // tile grid and first logical tile
-    int nz, nxw, i0;        // window interior and its first padded index (4+ntaper)
+    int nz, nxw, i0, iz0;   // window interior, its first padded column (4+ntaper) and row (4+ntop;
+                            // ntop = 1-nmin1 FD rows lie above model row 1)
     int nt, ntsnap;
     // injection: per tile [inj_tile[t], inj_tile[t+1]) cells; per cell a
     // contribution range [inj_cptr[c], inj_cptr[c+1])
@@ -117,6 +118,18 @@ struct ShotDev {
     const float *born_m;    // nz*nxw reflectivity
     const float *taper;     // nxw
     float dt2snap;          // (dt*snapi)^2
+    // free surface and ghosts (Boundary.f90:1609-1658, Injection.f90:447-450,
+    // Extraction.f90:548-556): padded row M - r mirrors row r (M = 2 iz0 - 2,
+    // Fortran row -r of row r); fs_row[ix] is the padded row of the surface
+    // cell ielev_z of FD column ix, fs_w[8 ix ..] its sinc weights and an
+    // off-grid flag (w[7]); null when surf_type is 0
+    int mirror;             // M
+    int ext_ghost;          // rec_type != 0: receivers read u(r) - u(M - r)
+    int no_top;             // surf_type > 0: no top Higdon band (Boundary.f90:123)
+    int hrow0;              // first padded row of the side Higdon bands (nz0, Boundary.f90:104-108)
+    int snap_mask;          // a window column's surface lies below model row 1: snapshots zero above it
+    const int *fs_row;
+    const float *fs_w;
 };
 
 struct StepArgs {
@@ -197,14 +210,23 @@ __device__ __forceinline__ void extract_tile(const ShotDev &S, int t, int it, co
         const int iz = z0 + (pos & 0xffff), ix = x0 + (pos >> 16);
         GLB float *tr = gp(S.traces) + (size_t)r * S.nt + (it - 1);
         if (S.ext_kind == EXT_NEAREST) {
-            *tr = *tr + u2[addr(S, iz, ix)];
+            if (S.ext_ghost) *tr = *tr + (u2[addr(S, iz, ix)] - u2[addr(S, S.mirror - iz, ix)]);
+            else *tr = *tr + u2[addr(S, iz, ix)];
         } else {
-            // trace += g*u*wz*wx, j outer, i inner
+            // trace += g*u*wz*wx, j outer, i inner; with a receiver ghost
+            // trace += g*(u(r) - u(mirror of r))*wz*wx (Extraction.f90:548-556)
             const GLB float *w = gp(S.ext_w) + 16 * k;
             const float g = w[0];
             float acc = *tr;
-            for (int j = -3; j <= 3; j++)
-                for (int i = -3; i <= 3; i++) acc = acc + g * u2[addr(S, iz + i, ix + j)] * w[4 + i] * w[11 + j];
+            if (S.ext_ghost) {
+                for (int j = -3; j <= 3; j++)
+                    for (int i = -3; i <= 3; i++)
+                        acc = acc + g * (u2[addr(S, iz + i, ix + j)] - u2[addr(S, S.mirror - iz - i, ix + j)]) *
+                                        w[4 + i] * w[11 + j];
+            } else {
+                for (int j = -3; j <= 3; j++)
+                    for (int i = -3; i <= 3; i++) acc = acc + g * u2[addr(S, iz + i, ix + j)] * w[4 + i] * w[11 + j];
+            }
             *tr = acc;
         }
     }
@@ -234,7 +256,7 @@ template <bool FWI>
 __device__ __forceinline__ void snap_image_point(const ShotDev &S, const StepArgs &A, int it, int iz, int ix,
                                                  float u2c, float vel, int sb, float inv16, float Sv, float img0,
                                                  float ill0, float tp) {
-    const int wz = iz - S.i0, wx = ix - S.i0;
+    const int wz = iz - S.iz0, wx = ix - S.i0;
     const size_t wq = (size_t)wx * S.nz + wz;
     const size_t npix = (size_t)S.nz * S.nxw;
     const int idx = it / A.snapi;
@@ -281,7 +303,7 @@ __device__ __forceinline__ void image_cells(const StepArgs &A, const ShotDev &S,
         return;   // this shot has finished (ragged batch)
     }
     if (its % A.snapi != 0) return;
-    const int idx = it / A.snapi, iz = wz + S.i0;
+    const int idx = it / A.snapi, iz = wz + S.iz0;
     const size_t npix = (size_t)S.nz * S.nxw;
     const GLB float *u2 = gp(S.u[A.rot]);
     const bool vel_needed = FWI || A.imaging == IMG_LSRTM;
@@ -325,6 +347,67 @@ __global__ void __launch_bounds__(256) image_kernel(StepArgs A) {
                                     (int)threadIdx.x);
 }
 
+// Boundary_set_free_surface_grid (Boundary.f90:1609-1658) on u2 of every
+// running shot, after extraction, snapshot and imaging have read it and
+// before the stencil does: one thread per FD column (grid: column groups,
+// the running shots in order of decreasing record length).  Rows nmin1 ..
+// ielev-1 are zeroed; an on-grid surface then odd-mirrors rows ielev ..
+// ielev+2 into ielev-2 .. ielev-4; an off-grid one subtracts the mirror image
+// of the sinc-interpolated column, in the reference's order.  Only rows above
+// the surface change, and only from rows at and below it.
+__global__ void __launch_bounds__(256) free_surface_kernel(StepArgs A) {
+    const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.shot_order)[blockIdx.y]];
+    const int ix = 4 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (ix > S.NX - 5) return;
+    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    if (!A.nbase && (its < 1 || its > S.nt)) return;
+    GLB float *u = gp(S.u[A.rot]) + (size_t)ix * S.LDZ + S.zoff;
+    const int E = gp(S.fs_row)[ix];
+    for (int r = 4; r < E; r++) u[r] = 0.f;
+    const GLB float *w = gp(S.fs_w) + 8 * ix;
+    if (w[7] == 0.f) {
+#pragma unroll
+        for (int i = 0; i <= 2; i++) u[E - 2 - i] = -u[E + i];
+    } else {
+        float wl[7], ui[7];
+#pragma unroll
+        for (int l = 0; l < 7; l++) wl[l] = w[l];
+#pragma unroll
+        for (int i = -3; i <= 3; i++) {
+            float f = 0.f;
+#pragma unroll
+            for (int l = -3; l <= 3; l++) f = f + u[E + l + i] * wl[3 + l];
+            ui[3 + i] = f;
+        }
+#pragma unroll
+        for (int i = 2; i <= 4; i++)
+#pragma unroll
+            for (int l = -3; l <= i - 1; l++) u[E - i + l] = u[E - i + l] - ui[i + 1] * wl[3 + l];
+    }
+}
+
+// Receiver extraction on u2 before the free surface changes it (ExtractData
+// precedes Boundary_set_free_surface_grid, Propagator.f90:176-198): launches
+// with a free surface extract here, one block per logical tile of width TX,
+// and the step kernel of the same step skips extraction.
+__global__ void __launch_bounds__(256) extract_kernel(StepArgs A, int TX) {
+    const int lb = blockIdx.x;
+    if (lb >= A.nblocks) return;
+    const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.blk_shot)[lb]];
+    const int t = lb - S.tile0;
+    if (!S.ext_tile || t < S.ext_lo || t >= S.ext_hi) return;
+    if (gp(S.ext_tile)[t + 1] <= gp(S.ext_tile)[t]) return;
+    int it = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    if (A.nbase) {
+        const int b = *gp(A.nbase);
+        it = A.dir > 0 ? it + b : it - b;
+    } else if (it < 1 || it > S.nt) {
+        return;
+    }
+    const int tzi = t % S.ntz, txi = t / S.ntz;
+    extract_tile(S, t, it, gp((const float *)S.u[A.rot]), tzi * 64 - S.zsh, txi * TX - S.xsh);
+}
+
 // Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
 // the grid edge.  s_b holds the tile's u3, s_u2 / s_u1 its u2 / u1, column
 // stride TP = TZ + 1 (a thread walking a column and the lanes of a row then
@@ -353,7 +436,7 @@ __device__ __forceinline__ void higdon_load_g(const ShotDev &S, int z0, int x0, 
     if ((left || right) && tid < 2 * TZ) {
         const bool doleft = tid < TZ;
         const int i = z0 + tid % TZ;
-        if ((doleft ? left : right) && i >= 4 && i <= NZ - 5) {
+        if ((doleft ? left : right) && i >= S.hrow0 && i <= NZ - 5) {
             const GLB float *g = gp(S.hx) + 16 * (i - 4) + (doleft ? 0 : 8);
 #pragma unroll
             for (int k = 0; k < 8; k++) hg.x[k] = g[k];
@@ -408,7 +491,7 @@ __device__ __forceinline__ void higdon_tile(const ShotDev &S, const HigdonG &hg,
         if (tid < 2 * TZ) {
             const bool doleft = tid < TZ;
             const int i = z0 + tid % TZ;
-            if ((doleft ? left : right) && i >= 4 && i <= NZ - 5) {
+            if ((doleft ? left : right) && i >= S.hrow0 && i <= NZ - 5) {
                 if (doleft) higdon_line(hg.x, s_b, s_u2, s_u1, L(i, 9), -TP);          // columns 9, 8 | 7 .. 0
                 else higdon_line(hg.x, s_b, s_u2, s_u1, L(i, NX - 10), TP);            // columns NX-10, NX-9 | ..
             }
@@ -612,7 +695,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // ---- snapshot steps of a forward pass that keeps snapshots (mode 1;
     // reverse-pass imaging runs in the imaging blocks of mode 3)
     const bool snap_step = MODE == 1 && (its % A.snapi) == 0 && A.store_snap;
-    const bool interior_z = iz >= S.i0 && iz < S.i0 + S.nz;
+    const bool interior_z = iz >= S.iz0 && iz < S.iz0 + S.nz;
 
     // ---- injection tiles (workgroup-uniform branch): the trace samples and
     // the tile's contributions staged in LDS, so a contribution costs one
@@ -655,7 +738,9 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
                 for (int p = 4 * q; p < 4 * q + 4; p++) {
                     const int ix = xs + p;
-                    if (interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) m = fmaxf(m, fabsf(wu[p + 4]));
+                    if (interior_z && ix >= S.i0 && ix < S.i0 + S.nxw &&
+                        !(S.snap_mask && iz < gp(S.fs_row)[ix]))
+                        m = fmaxf(m, fabsf(wu[p + 4]));
                 }
                 for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
                 // blocks whose largest value is below 1e-30 store zeros (1/m would overflow FP32)
@@ -698,17 +783,19 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // non-temporal: the snapshots are read back only in the reverse pass, so
     // they should not push the wavefield levels out of the caches.
     if (snap_step && interior_z) {
-        const size_t sbase = (size_t)(it / A.snapi - 1) * S.nz * S.nxw + (iz - S.i0);
+        const size_t sbase = (size_t)(it / A.snapi - 1) * S.nz * S.nxw + (iz - S.iz0);
 #pragma unroll
         for (int p = 0; p < PX; p++) {
             const int ix = xs + p;
             if (ix < S.i0 || ix >= S.i0 + S.nxw) continue;
             const size_t q = sbase + (size_t)(ix - S.i0) * S.nz;
+            // with a free surface the snapshot is zero above it (Extraction.f90:580-592)
+            const float su = S.snap_mask && iz < gp(S.fs_row)[ix] ? 0.f : wu[p + 4];
             if (A.snap_fp16)
-                __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(wu[p + 4] * inv16[p / 4])),
+                __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(su * inv16[p / 4])),
                                             gp((unsigned short *)S.snap) + q);
             else
-                __builtin_nontemporal_store(wu[p + 4], gp((float *)S.snap) + q);
+                __builtin_nontemporal_store(su, gp((float *)S.snap) + q);
         }
     }
 
@@ -793,7 +880,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         float v = val[p];
         // Born secondary source: Imaging.f90:779-783, forward time only
         if (IMG && A.imaging == IMG_BORN && (its % A.snapi) == 0 && interior_z && ix >= S.i0 && ix < S.i0 + S.nxw) {
-            const int wz = iz - S.i0, wx = ix - S.i0;
+            const int wz = iz - S.iz0, wx = ix - S.i0;
             const size_t wq = (size_t)wx * S.nz + wz;
             const float Sv = load_snap(S, A.snap_fp16, it / A.snapi, (size_t)S.nz * S.nxw, wq, snap_block(S, iz, ix), true);
             v = v + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
@@ -802,7 +889,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         u3v[p] = (20.f * wu[p + 4] - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * v) * A.div11;
     }
 
-    const bool top = z0 <= 0, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
+    const bool top = z0 <= 0 && !S.no_top, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
     if (top || bot || left || right) {
         HigdonG hg;
         higdon_load_g<TZ, TX>(S, z0, x0, top, bot, left, right, hg);
@@ -877,13 +964,12 @@ __global__ void copy_calib_kernel(const float4 *__restrict__ src, float4 *__rest
 // (mod_window_pad ExtractPadModel.f90:509-530 + model_pad ReadSouVelRho.f90:545-596).
 // colsrc[j-1] = model column (0-based) feeding window column j.
 __global__ void window_pad_kernel(const float *__restrict__ big, int nz, const int *__restrict__ colsrc,
-                                  int nxw, int ntaper, int NZ, int NX, int LDZ, int zoff,
+                                  int nxw, int ntaper, int ntop, int NZ, int NX, int LDZ, int zoff,
                                   float *__restrict__ vel, float *__restrict__ v2) {
     const int iz = blockIdx.x * blockDim.x + threadIdx.x;
     const int ix = blockIdx.y;
     if (iz >= NZ || ix >= NX) return;
-    const int i0 = 4 + ntaper;
-    int wz = iz - i0, wx = ix - i0;
+    int wz = iz - (4 + ntop), wx = ix - (4 + ntaper);
     wz = wz < 0 ? 0 : (wz >= nz ? nz - 1 : wz);
     wx = wx < 0 ? 0 : (wx >= nxw ? nxw - 1 : wx);
     const float v = (float)0 + big[(size_t)colsrc[wx] * nz + wz];
@@ -962,8 +1048,10 @@ __global__ void __launch_bounds__(256) snap_max_kernel(StepArgs A) {
     if (snap_step_of(A, S, &idx)) {
         const GLB float *u2 = gp(S.u[A.rot]);
         for (int wx = blockIdx.x; wx < S.nxw; wx += gridDim.x) {
-            const size_t c = (size_t)(wx + S.i0) * S.LDZ + S.zoff + S.i0;
-            for (int wz = threadIdx.x; wz < S.nz; wz += blockDim.x) m = fmaxf(m, fabsf(u2[c + wz]));
+            const size_t c = (size_t)(wx + S.i0) * S.LDZ + S.zoff + S.iz0;
+            const int w0 = S.snap_mask ? gp(S.fs_row)[wx + S.i0] - S.iz0 : 0;   // zero above a free surface
+            for (int wz = threadIdx.x; wz < S.nz; wz += blockDim.x)
+                if (wz >= w0) m = fmaxf(m, fabsf(u2[c + wz]));
         }
     }
     __shared__ float red[256];

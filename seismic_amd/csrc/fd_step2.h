@@ -80,7 +80,7 @@ __device__ __forceinline__ void snap_image_tile(const ShotDev &S, const StepArgs
         for (int l = tid; l < B2Z * B2X; l += kThreads2) {
             const int iz = z0 + l % B2Z, ix = x0 + l / B2Z;
             float m = 0.f;
-            if (iz >= S.i0 && iz < S.i0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw) m = fabsf(u(l % B2Z, l / B2Z));
+            if (iz >= S.iz0 && iz < S.iz0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw) m = fabsf(u(l % B2Z, l / B2Z));
             for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
             if ((tid & 63) == 0) s_red[l / B2Z] = m;
         }
@@ -97,7 +97,7 @@ __device__ __forceinline__ void snap_image_tile(const ShotDev &S, const StepArgs
     const size_t npix = (size_t)S.nz * S.nxw;
     for (int l = tid; l < B2Z * B2X; l += kThreads2) {
         const int iz = z0 + l % B2Z, ix = x0 + l / B2Z;
-        if (!(iz >= S.i0 && iz < S.i0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw)) continue;
+        if (!(iz >= S.iz0 && iz < S.iz0 + S.nz && ix >= S.i0 && ix < S.i0 + S.nxw)) continue;
         float Sv = 0.f, img0 = 0.f, ill0 = 0.f, tp = 0.f, vel = 0.f;
         const int sb = snap_block(S, iz, ix);
         float inv16 = 0.f;
@@ -106,7 +106,7 @@ __device__ __forceinline__ void snap_image_tile(const ShotDev &S, const StepArgs
             inv16 = m > 0.f ? 1.0f / m : 0.f;
         }
         if (img) {
-            const size_t wq = (size_t)(ix - S.i0) * S.nz + (iz - S.i0);
+            const size_t wq = (size_t)(ix - S.i0) * S.nz + (iz - S.iz0);
             Sv = load_snap(S, A.snap_fp16, it / A.snapi, npix, wq, sb, true);
             img0 = gp(S.image)[wq];
             ill0 = gp(S.illum)[wq];

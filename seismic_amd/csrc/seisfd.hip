@@ -132,8 +132,16 @@ static void check_grid(const sfd_grid &g) {
     if (g.lsinc != 7) throw EngineError(SFD_EUNSUPPORTED, "only lsinc=7 is implemented");
     if (g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
     if (!(g.fmax >= 0.f) || !std::isfinite(g.fmax)) throw EngineError(SFD_EINVAL, "fmax must be >= 0 (0: 30 Hz)");
-    if (g.rec_type != 0 || g.shot_type != 0 || g.surf_type != 0)
-        throw EngineError(SFD_EUNSUPPORTED, "ghost sources/receivers and free surface are not implemented");
+}
+
+// Rows of the FD region above model row 1 (1 - nmin1): read_window_vel's
+// bounds (ExtractPadModel.f90:228-233) — a ghost source needs room for its
+// mirror image above the surface, a free surface keeps lsinc rows (nmin1 =
+// 1 - lsinc).
+static int top_rows(const sfd_grid &g) {
+    if (g.surf_type > 0) return g.lsinc;
+    if (g.shot_type > 0) return g.ntaper + g.lsinc / 2 + 2;
+    return g.ntaper;
 }
 
 // ReadSouVelRho.f90:603-752 vel_check, 2D: the Courant stop (:629-675,
@@ -203,6 +211,19 @@ void Engine::set_model(const float *vel, const float *vel_bwd) {
     have_model = true;
 }
 
+// Free-surface elevation per model column (the reference's elev%elev, which
+// its drivers set to 0: progs/rtm/Acoustic_RTM_sep.f90:118-119); NULL: 0.
+void Engine::set_elevation(const float *elev) {
+    if (!elev) {
+        elev_h.clear();
+    } else {
+        for (int i = 0; i < g.nx; i++)
+            if (!std::isfinite(elev[i])) throw EngineError(SFD_EINVAL, "elevation must be finite");
+        elev_h.assign(elev, elev + g.nx);
+    }
+    model_version++;   // plans hold the surface rows
+}
+
 // Padded velocity at a Fortran window cell (icz, icx) (1-based, may lie in
 // the pad): the edge-replicated value model_pad would hold there.
 float Engine::padded_vel(const std::vector<float> &vh, const ShotPlan &p, int icz, int icx) const {
@@ -218,9 +239,11 @@ void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
     p.nt = s.nt;
     p.ntr = s.ntr;
     p.ntsnap = s.nt / g.snapi;
-    p.NZ = g.nz + 2 * g.ntaper + 8;
+    p.ntop = top_rows(g);
+    p.NZ = g.nz + p.ntop + g.ntaper + 8;
     p.NX = p.nxw + 2 * g.ntaper + 8;
     p.i0 = 4 + g.ntaper;
+    p.iz0 = 4 + p.ntop;
     const int rz_ = (p.NZ - 1) % tz;
     p.zsh = rz_ >= 9 ? 0 : 9 - rz_;
     {
@@ -232,7 +255,7 @@ void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
         // its recursion reads), the bottom tile the last 10 rows.
         float zmin = s.sz;
         for (int r = 0; r < s.ntr; r++) zmin = std::min(zmin, s.rz[r]);
-        const int rmin = p.i0 + (int)std::floor((zmin - g.oz) / g.dz) - 4;
+        const int rmin = p.iz0 + (int)std::floor((zmin - g.oz) / g.dz) - 4;
         const int want = tz - rmin;   // the smallest shift that puts row rmin in tile row 1
         if (want > p.zsh && want <= tz - 10 && (p.NZ - 1 + want) % tz >= 9 &&
             (p.NZ + want + tz - 1) / tz == (p.NZ + p.zsh + tz - 1) / tz)
@@ -255,6 +278,33 @@ void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
     }
     p.taper.resize(p.nxw);
     host_taper(p.taper.data(), p.nxw);
+    // Free surface (ModelSpace_elevation_parameters, ModelSpaceTypes.f90:128-141):
+    // per FD column, ielev_z / delev_z of its elevation by find_i_d against
+    // the depth axis, and the sinc of delev_z / dz that an off-grid surface
+    // interpolates with (Boundary.f90:1636).  FD columns outside the window
+    // take the elevation of the nearest window column.
+    p.fs_row.clear();
+    p.fs_w.clear();
+    p.snap_mask = 0;
+    if (g.surf_type != 0) {
+        p.fs_row.assign(p.NX, 0);
+        p.fs_w.assign((size_t)8 * p.NX, 0.f);
+        for (int ix = 4; ix <= p.NX - 5; ix++) {
+            const int wx = std::min(std::max(ix - p.i0, 0), p.nxw - 1);
+            const float e = elev_h.empty() ? 0.f : elev_h[p.colsrc[wx]];
+            int ie;
+            float de;
+            find_i_d(&ie, &de, e, g.oz, g.dz);
+            if (ie < 1 || ie > g.nz)
+                throw EngineError(SFD_EINVAL, "free-surface elevation must lie within the model depth range");
+            p.fs_row[ix] = ie - 1 + p.iz0;
+            if (de != 0.f) {
+                sfd_host_mksinc(&p.fs_w[(size_t)8 * ix], 7, de / g.dz);
+                p.fs_w[(size_t)8 * ix + 7] = 1.f;
+            }
+            if (ie > 1 && ix >= p.i0 && ix < p.i0 + p.nxw) p.snap_mask = 1;
+        }
+    }
 }
 
 // Tile of a padded cell, and its local cell index.
@@ -375,16 +425,28 @@ void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc,
                             int kind) const {
     struct C { int iz, ix, tr; float a, wz, wx; };
     std::vector<C> cs;
-    cs.reserve((size_t)nsrc * 49);
+    // Ghosts: the sinc injections test shot_type whichever traces they inject
+    // (Injection.f90:432); the simple one type_inject, which is rec_type in
+    // the reverse-time passes that use it (:74-78, :583).  A ghost subtracts
+    // the same term at the mirror cell M - (iz - i) with the weight of -i,
+    // right after the direct term: a negated gain gives that product exactly.
+    if ((kind == 4 || kind == 5) && ghosts_or_surface())
+        throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
+    const bool ghost = (kind == 2) ? g.rec_type != 0 : g.shot_type != 0;
+    const int M = 2 * p.iz0 - 2;
+    cs.reserve((size_t)nsrc * (ghost ? 98 : 49));
     float wzs[7], wxs[7];
     for (int k = 0; k < nsrc; k++) {
         int icz, icx;
         float dcz, dcx;
         find_i_d(&icz, &dcz, sz_[k], g.oz, g.dz);
         find_i_d(&icx, &dcx, sx_[k], p.oxw, g.dx);
-        const int iz = icz - 1 + p.i0, ix = icx - 1 + p.i0;
-        if (iz - 3 < 4 || iz + 3 > p.NZ - 5 || ix - 3 < 4 || ix + 3 > p.NX - 5)
+        const int iz = icz - 1 + p.iz0, ix = icx - 1 + p.i0;
+        const int hz = (kind == 2 || kind == 5) ? 0 : 3;
+        if (iz - hz < 4 || iz + hz > p.NZ - 5 || ix - hz < 4 || ix + hz > p.NX - 5)
             throw EngineError(SFD_EINVAL, "source/receiver footprint leaves the FD region");
+        if (ghost && (M - iz - hz < 4 || M - iz + hz > p.NZ - 5))
+            throw EngineError(SFD_EINVAL, "ghost source footprint leaves the FD region");
         float a;
         if (kind == 3) {
             a = 1.f / (g.dz * g.dx);
@@ -396,12 +458,16 @@ void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc,
         }
         if (kind == 2 || kind == 5) {
             cs.push_back({iz, ix, k, a, 1.f, 1.f});
+            if (ghost) cs.push_back({M - iz, ix, k, -a, 1.f, 1.f});
             continue;
         }
         sfd_host_mksinc(wzs, 7, dcz / g.dz);
         sfd_host_mksinc(wxs, 7, dcx / g.dx);
         for (int j = -3; j <= 3; j++)
-            for (int i = -3; i <= 3; i++) cs.push_back({iz + i, ix + j, k, a, wzs[3 + i], wxs[3 + j]});
+            for (int i = -3; i <= 3; i++) {
+                cs.push_back({iz + i, ix + j, k, a, wzs[3 + i], wxs[3 + j]});
+                if (ghost) cs.push_back({M - (iz - i), ix + j, k, -a, wzs[3 - i], wxs[3 + j]});
+            }
     }
     // group by cell, keeping trace order within a cell (stable sort)
     const int ntile = p.ntz * p.ntx;
@@ -470,9 +536,11 @@ void Engine::plan_extraction(ShotPlan &p, const std::vector<float> &vh, const sf
         float dcz, dcx;
         find_i_d(&icz, &dcz, s.rz[r], g.oz, g.dz);
         find_i_d(&icx, &dcx, s.rx[r], p.oxw, g.dx);
-        const int iz = icz - 1 + p.i0, ix = icx - 1 + p.i0;
+        const int iz = icz - 1 + p.iz0, ix = icx - 1 + p.i0;
         if (iz - 3 < 0 || iz + 3 >= p.NZ || ix - 3 < 0 || ix + 3 >= p.NX)
             throw EngineError(SFD_EINVAL, "receiver outside the padded grid");
+        if (g.rec_type != 0 && (2 * p.iz0 - 2 - iz - 3 < 0 || 2 * p.iz0 - 2 - iz + 3 >= p.NZ))
+            throw EngineError(SFD_EINVAL, "receiver ghost outside the padded grid");
         R e;
         memset(&e, 0, sizeof e);
         cell_tile(p, tz, tx, iz, ix, &e.t, &e.loc);
@@ -532,6 +600,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t inj2[6], injb2[6];
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
         size_t snap, snap_bytes, snap_scale, image, illum, born_m;
+        size_t fs_row, fs_w;
     };
     std::vector<Off> off(ns);
     const bool tb = tz == B2Z && tx == B2X;
@@ -553,6 +622,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         o.hx = L.take((size_t)16 * (p.NZ - 8) * sizeof(float));
         o.colsrc = put(L, up, upn, p.colsrc);
         o.taper = put(L, up, upn, p.taper);
+        o.fs_row = put(L, up, upn, p.fs_row);
+        o.fs_w = put(L, up, upn, p.fs_w);
         o.inj_tile = put(L, up, upn, p.inj_tile);
         o.inj_cell = put(L, up, upn, p.inj_cell);
         o.inj_cptr = put(L, up, upn, p.inj_cptr);
@@ -682,7 +753,14 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.hx = (const float *)(base + o.hx);
         S.NZ = p.NZ; S.NX = p.NX; S.LDZ = p.LDZ; S.zoff = p.zoff;
         S.zsh = p.zsh; S.xsh = p.xsh; S.ntz = p.ntz; S.ntx = p.ntx; S.tile0 = p.tile0;
-        S.nz = g.nz; S.nxw = p.nxw; S.i0 = p.i0; S.nt = p.nt; S.ntsnap = p.ntsnap;
+        S.nz = g.nz; S.nxw = p.nxw; S.i0 = p.i0; S.iz0 = p.iz0; S.nt = p.nt; S.ntsnap = p.ntsnap;
+        S.mirror = 2 * p.iz0 - 2;
+        S.ext_ghost = g.rec_type != 0;
+        S.no_top = g.surf_type > 0;
+        S.hrow0 = g.surf_type > 0 ? p.iz0 : 4;   // Boundary_3d's nz0 = 1 with a free surface
+        S.snap_mask = p.snap_mask;
+        S.fs_row = p.fs_row.empty() ? nullptr : (const int *)(base + o.fs_row);
+        S.fs_w = p.fs_w.empty() ? nullptr : (const float *)(base + o.fs_w);
         S.inj_tile = (const int *)(base + o.inj_tile);
         S.inj_cell = (const int *)(base + o.inj_cell);
         S.inj_cptr = (const int *)(base + o.inj_cptr);
@@ -789,7 +867,7 @@ void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float 
         ShotDev &S = B.shots_h[s];
         dim3 blk(256), grd((p.NZ + 255) / 256, p.NX);
         hipLaunchKernelGGL(window_pad_kernel, grd, blk, 0, stream, d_model, g.nz, B.colsrc_d[s], p.nxw, g.ntaper,
-                           p.NZ, p.NX, p.LDZ, p.zoff, (float *)S.vel, (float *)S.v2);
+                           p.ntop, p.NZ, p.NX, p.LDZ, p.zoff, (float *)S.vel, (float *)S.v2);
         const int nk = std::max(p.NX, p.NZ);
         hipLaunchKernelGGL(higdon_tables_kernel, dim3((nk + 255) / 256), dim3(256), 0, stream, S.vel, p.NZ, p.NX,
                            p.LDZ, p.zoff, g.dt, g.dz, g.dx, (float *)S.hz, (float *)S.hx);
@@ -904,6 +982,20 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
             grid += (groups + per_block - 1) / per_block;
         }
     }
+    if (surface()) {
+        // Propagator.f90:176-198: ExtractData reads u2 before
+        // Boundary_set_free_surface_grid changes it, so the receivers are
+        // read first, then the surface is set; the step kernel skips
+        // extraction.  (Snapshots and imaging read only rows at and below the
+        // surface, which the surface kernel does not change.)
+        if (A.extract) {
+            hipLaunchKernelGGL(extract_kernel, dim3(A.nblocks), dim3(256), 0, stream, A, tx);
+            A.extract = 0;
+        }
+        int nxmax = 1;
+        for (int k = 0; k < nact; k++) nxmax = std::max(nxmax, B.shots_h[B.order[k]].NX - 8);
+        hipLaunchKernelGGL(free_surface_kernel, dim3((nxmax + 255) / 256, nact), dim3(256), 0, stream, A);
+    }
     if (mode == 1 && A.store_snap && A.snap_fp16 && A.snap_global) {
         // one FP16 scale per snapshot: max |u2| over each window before the step stores it
         hipLaunchKernelGGL(snap_max_kernel, dim3(kSnapParts, nact), dim3(256), 0, stream, A);
@@ -922,7 +1014,8 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 // The two-step kernel applies with 64x32 tiles, region plans within its trace
 // table, and no Born source term.
 bool Engine::use_step2(const StepArgs &A) const {
-    return time_block == 2 && tb_ok && tz == B2Z && tx == B2X && A.imaging != IMG_BORN && !A.snap_global;
+    return time_block == 2 && tb_ok && tz == B2Z && tx == B2X && A.imaging != IMG_BORN && !A.snap_global &&
+           !ghosts_or_surface();
 }
 
 void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
@@ -1213,7 +1306,7 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
     Cand pick = cands[0];
     tune_log.clear();
     for (const Cand &cd : cands) {
-        if (cd.tb == 2 && steps2 < 2) continue;
+        if (cd.tb == 2 && (steps2 < 2 || ghosts_or_surface())) continue;
         const int steps = cd.tb == 2 ? steps2 : steps1;
         tz = cd.tz; tx = cd.tx; strip_px = cd.px; time_block = cd.tb;
         std::vector<ShotPlan> plans(ns);
@@ -1273,7 +1366,7 @@ size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
     float oxw;
     int nxw;
     shot_window(g, s, &oxw, &nxw);
-    const size_t NZ = (size_t)g.nz + 2 * g.ntaper + 8, NX = (size_t)nxw + 2 * g.ntaper + 8;
+    const size_t NZ = (size_t)g.nz + top_rows(g) + g.ntaper + 8, NX = (size_t)nxw + 2 * g.ntaper + 8;
     const size_t LDZ = (NZ + 64 + 31) / 32 * 32;
     const size_t ntile = (NZ + 64) / 64 * ((NX + 32) / 32 + 1);
     const size_t npix = (size_t)g.nz * nxw, ntr = (size_t)std::max(s.ntr, 1), nt = (size_t)s.nt;
@@ -1443,6 +1536,7 @@ void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_
 // window's mbar at every snapshot step; the host adds the window into the
 // model through the transpose of born()'s clamped column gather.
 void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *mbar) {
+    if (ghosts_or_surface()) throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
     check_shots(shots, ns, SFD_SNAP_FP32);
     if (!dbar || !mbar) throw EngineError(SFD_EINVAL, "NULL dbar/mbar");
     if (!(tz == 64 && (tx == 32 || tx == 64))) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
@@ -1520,6 +1614,7 @@ void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar
     }
 }
 void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind) {
+    if (ghosts_or_surface()) throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
     check_shots(shots, ns);
     if (!dbar || !sbar) throw EngineError(SFD_EINVAL, "NULL dbar/sbar");
     if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract_kind");
@@ -1544,7 +1639,7 @@ void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, flo
         float dcz, dcx;
         find_i_d(&icz, &dcz, sh.sz, g.oz, g.dz);
         find_i_d(&icx, &dcx, sh.sx, p.oxw, g.dx);
-        const int iz0 = icz - 1 + p.i0, ix0 = icx - 1 + p.i0;
+        const int iz0 = icz - 1 + p.iz0, ix0 = icx - 1 + p.i0;
         const float v = padded_vel(vel_h, p, icz, icx), v2s = v * v;
         float wz[7], wx[7];
         sfd_host_mksinc(wz, 7, dcz / g.dz);
@@ -2451,6 +2546,10 @@ int sfd_copy_calibration(size_t nbytes, int reps, double *avg_ms) {
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return rc;
+}
+
+int sfd_set_elevation(sfd_handle h, const float *elev) {
+    return guarded(h, [&](Engine &e) { e.set_elevation(elev); });
 }
 
 int sfd_set_option(sfd_handle h, int opt, int value) {

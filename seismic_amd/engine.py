@@ -24,15 +24,16 @@ class SeisFDError(RuntimeError):
         self.code = code
 
 
-def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture_x=0.0, fmax=30.0):
+def make_grid(nz, nx, dz, dx, dt, ntaper=20, lsinc=7, snapi=4, oz=0.0, ox=0.0, aperture_x=0.0, fmax=30.0,
+              rec_type=0, shot_type=0, surf_type=0):
     """GeneralParam subset (ReadParam.f90:18-36 defaults: ntaper=20, snapi=4,
-    lsinc=7, fmax=30 Hz)."""
+    lsinc=7, fmax=30 Hz, rec_type = shot_type = surf_type = 0)."""
     g = sfd_grid()
     g.nz, g.nx = int(nz), int(nx)
     g.oz, g.ox, g.dz, g.dx, g.dt = oz, ox, dz, dx, dt
     g.ntaper, g.lsinc, g.snapi = ntaper, lsinc, snapi
     g.aperture_x = aperture_x
-    g.rec_type = g.shot_type = g.surf_type = 0
+    g.rec_type, g.shot_type, g.surf_type = int(rec_type), int(shot_type), int(surf_type)
     g.fmax = fmax
     return g
 
@@ -110,6 +111,14 @@ class Engine:
         self._vel = _model_buf(vel, self.g.nz, self.g.nx)
         self._velb = None if vel_bwd is None else _model_buf(vel_bwd, self.g.nz, self.g.nx)
         self._chk(self.lib.sfd_set_model(self.h, _ptr(self._vel), _ptr(self._velb)))
+
+    def set_elevation(self, elev=None):
+        """Free-surface elevation per model column (ModelSpace_elevation%elev);
+        None: 0 everywhere, as the reference's drivers set it."""
+        self._elev = None if elev is None else _f32c(elev)
+        if self._elev is not None and self._elev.size != self.g.nx:
+            raise ValueError("elevation needs nx = %d values" % self.g.nx)
+        self._chk(self.lib.sfd_set_elevation(self.h, _ptr(self._elev)))
 
     def set_tile(self, tz, tx):
         self._chk(self.lib.sfd_set_tile(self.h, tz, tx))

@@ -109,17 +109,27 @@ def shot_window(g, shot):
     return oxw.value, nxw.value
 
 
-def bounds(nz, nxw, ntaper):
+def bounds(nz, nxw, ntaper, lsinc=7, shot_type=0, surf_type=0):
     b = ORC_BOUNDS()
-    oracle().orc_bounds(C.byref(b), nz, nxw, ntaper)
+    oracle().orc_bounds_t(C.byref(b), nz, nxw, ntaper, lsinc, shot_type, surf_type)
     return b
+
+
+def set_elevation(elev):
+    """The oracle's free-surface elevation per model column (None: 0)."""
+    L = oracle()
+    if elev is None:
+        L.orc_set_elevation(None, 0)
+        return
+    e = np.ascontiguousarray(elev, dtype=np.float32)
+    L.orc_set_elevation(ptr(e), len(e))
 
 
 def padded_vel(g, vel, shot, prec="f"):
     """Window + edge-replication pad of the full model for one shot."""
     L = oracle()
     oxw, nxw = shot_window(g, shot)
-    b = bounds(g.nz, nxw, g.ntaper)
+    b = bounds(g.nz, nxw, g.ntaper, g.lsinc, g.shot_type, g.surf_type)
     v = colmajor(vel, _dt(prec))
     out = np.zeros(b.NZ * b.NX, dtype=_dt(prec))
     fn = getattr(L, "orc_%s_window_pad" % prec)
@@ -362,11 +372,16 @@ def ref_taper(n):
 
 
 def ref_propagate(g, vel_pad, oxw, nxw, stencil, inj, sou_z, sou_x, sou_tr, ext, dat_z, dat_x,
-                  nt, tdir, nsnap):
-    """Runs the reference kernels in Propagator.f90 order.  vel_pad: [NZ, NX]."""
+                  nt, tdir, nsnap, elev_fd=None):
+    """Runs the reference kernels in Propagator.f90 order.  vel_pad: [NZ, NX].
+    g's rec/shot/surf types are passed on; elev_fd: elevation per FD column
+    nmin2..nmax2 (None: 0)."""
     fn = ref().ref_propagate
     fn.argtypes = ([C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 2 + [C.c_void_p] + [C.c_int] * 3
-                   + [C.c_void_p] * 3 + [C.c_int] * 2 + [C.c_void_p] * 3 + [C.c_int] * 3 + [C.c_void_p])
+                   + [C.c_void_p] * 3 + [C.c_int] * 2 + [C.c_void_p] * 3 + [C.c_int] * 3 + [C.c_void_p]
+                   + [C.c_int] * 3 + [C.c_void_p])
+    ev = np.zeros(nxw + 2 * g.ntaper, np.float32) if elev_fd is None else \
+        np.ascontiguousarray(elev_fd, dtype=np.float32)
     vp = np.asfortranarray(vel_pad, dtype=np.float32)
     sz = np.ascontiguousarray(sou_z, dtype=np.float32)
     sx = np.ascontiguousarray(sou_x, dtype=np.float32)
@@ -378,7 +393,7 @@ def ref_propagate(g, vel_pad, oxw, nxw, stencil, inj, sou_z, sou_x, sou_tr, ext,
     wave = np.zeros((max(nsnap, 1), nxw, g.nz), dtype=np.float32)
     fn(g.nz, nxw, g.ntaper, g.dz, g.dx, g.dt, g.oz, oxw, g.lsinc, g.snapi, vp.ctypes.data_as(C.c_void_p),
        stencil, inj, len(sz), ptr(sz), ptr(sx), ptr(st), ext, len(dz_), ptr(dz_), ptr(dx_), ptr(dtr),
-       nt, tdir, nsnap, ptr(wave))
+       nt, tdir, nsnap, ptr(wave), g.rec_type, g.shot_type, g.surf_type, ptr(ev))
     return dtr[:len(dz_)], wave[:nsnap]
 
 
