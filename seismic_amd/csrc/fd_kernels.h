@@ -73,6 +73,7 @@ struct ShotDev {
     const float *hx;        // Higdon x-band coefficients: 16 per row iz in [4, NZ-5]
     int NZ, NX, LDZ, zoff;  // padded dims, leading dim, storage offset of row 0
     int zsh, xsh;           // tile-origin shifts (tile k starts at k*T - sh)
+    int tz;                 // tile rows (64, or 56 for the two-step kernel's tiling)
     int ntz, ntx, tile0;    // tile grid and first logical tile
     int nz, nxw, i0, iz0;   // window interior, its first padded column (4+ntaper) and row (4+ntop;
                             // ntop = 1-nmin1 FD rows lie above model row 1)
@@ -186,7 +187,8 @@ __device__ __forceinline__ int logical_block(const StepArgs &A) {
 // FP16 scale block of padded cell (iz, ix): 64 rows of a tile row by 4
 // columns, numbered down the tile rows first.
 __device__ __forceinline__ int snap_block(const ShotDev &S, int iz, int ix) {
-    return ((iz + S.zsh) >> 6) + S.ntz * ((ix + S.xsh) >> 2);
+    const int r = iz + S.zsh;
+    return (S.tz == 64 ? r >> 6 : r / 56) + S.ntz * ((ix + S.xsh) >> 2);
 }
 __device__ __forceinline__ float load_snap(const ShotDev &S, int fp16, int idx, size_t npix, size_t wq, int sb,
                                            bool stream) {
@@ -405,7 +407,7 @@ __global__ void __launch_bounds__(256) extract_kernel(StepArgs A, int TX) {
         return;
     }
     const int tzi = t % S.ntz, txi = t / S.ntz;
-    extract_tile(S, t, it, gp((const float *)S.u[A.rot]), tzi * 64 - S.zsh, txi * TX - S.xsh);
+    extract_tile(S, t, it, gp((const float *)S.u[A.rot]), tzi * S.tz - S.zsh, txi * TX - S.xsh);
 }
 
 // Higdon absorbing boundary (Boundary.f90:119-153) on a tile that touches
@@ -528,13 +530,14 @@ struct StepShape {
 // spend; 3 is mode 0 plus imaging blocks after the step tiles (imaging
 // steps of reverse passes of one shot, image_cells; batches launch mode 0
 // and image_kernel).
-template <int PX, bool ADJ, int MODE>
+template <int PX, bool ADJ, int MODE, int TZ>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
     constexpr bool IMG = MODE == 2;
-    constexpr int TZ = 64, NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
+    constexpr int NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
     static_assert(NT >= 2 * TX && NT >= 2 * TZ, "one thread per Higdon line");
-    __shared__ float s_col[NW][72];     // per-wavefront z-stencil column
+    static_assert(TZ == 64 || TZ == 56, "tiles of 64 or 56 rows (a wavefront's lanes)");
+    __shared__ float s_col[NW][TZ + 8]; // per-wavefront z-stencil column
     // Injection tables are read only inside the column loop and the edge
     // buffers are written only after it (behind a barrier on injection
     // tiles), so the two phases share one LDS block.  Keeping the block small
@@ -584,11 +587,12 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int iz = z0 + lane;
     const int xs = x0 + wv * PX;                      // first column of this wavefront's strip
-    const bool zarr = iz >= 0 && iz < NZ;
-    const bool zin = iz >= 4 && iz <= NZ - 5;
+    const bool lane_ok = lane < TZ;                   // 56-row tiles leave lanes 56..63 idle
+    const bool zarr = lane_ok && iz >= 0 && iz < NZ;
+    const bool zin = lane_ok && iz >= 4 && iz <= NZ - 5;
     const size_t zo = (size_t)(zoff + iz);
     // halo row served by lanes 0..7: 4 above, 4 below the strip
-    const int hz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;
+    const int hz = lane < 4 ? z0 - 4 + lane : z0 + TZ - 4 + lane;
     const bool hok = lane < 8 && hz >= 0 && hz < NZ;
 
     // ---- receivers centred in this tile (reads u2 only; done before the
@@ -631,7 +635,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             }
             const GLB int *mp = gp(S.inj_map) + imap;
 #pragma unroll
-            for (int p = 0; p < PX; p++) me[p] = mp[lane + TZ * (wv * PX + p)];
+            for (int p = 0; p < PX; p++) me[p] = lane_ok ? mp[lane + TZ * (wv * PX + p)] : 0;
             // The tile's contributions reach LDS in chunks of kMaxTileCoef
             // (one chunk except on the densest receiver tiles).  The first
             // chunk goes straight from memory into LDS (no registers), so
@@ -695,7 +699,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     // ---- snapshot steps of a forward pass that keeps snapshots (mode 1;
     // reverse-pass imaging runs in the imaging blocks of mode 3)
     const bool snap_step = MODE == 1 && (its % A.snapi) == 0 && A.store_snap;
-    const bool interior_z = iz >= S.iz0 && iz < S.iz0 + S.nz;
+    const bool interior_z = lane_ok && iz >= S.iz0 && iz < S.iz0 + S.nz;
 
     // ---- injection tiles (workgroup-uniform branch): the trace samples and
     // the tile's contributions staged in LDS, so a contribution costs one
@@ -706,7 +710,8 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     if (has_inj) {
         if constexpr (IMG) {
 #pragma unroll
-            for (int p = 0; p < PX; p++) s_p.inj.cmap[lane + TZ * (wv * PX + p)] = me[p];
+            for (int p = 0; p < PX; p++)
+                if (lane_ok) s_p.inj.cmap[lane + TZ * (wv * PX + p)] = me[p];
         }
 #pragma unroll
         for (int r = 0; r < kTrPerThread; r++) {
@@ -752,6 +757,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     }
     const float *c = A.c;
     float *col = s_col[wv];
+    const int cl = lane_ok ? lane : TZ - 1;   // idle lanes read a valid slot
 
     // Three passes over the strip's PX cells of a lane: stencil (and the
     // snapshot / imaging point), then the injections of all PX cells
@@ -763,12 +769,12 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         const int ix = xs + p;
         const float u2c = wu[p + 4];
         // z-stencil column through wavefront-private LDS (all 64 lanes take part)
-        col[lane + 4] = ADJ ? wv2[p + 4] : u2c;
-        if (lane < 8) col[lane < 4 ? lane : 64 + lane] = ADJ ? hv[p] : hu[p];
+        if (lane_ok) col[lane + 4] = ADJ ? wv2[p + 4] : u2c;
+        if (lane < 8) col[lane < 4 ? lane : TZ + lane] = ADJ ? hv[p] : hu[p];
         __builtin_amdgcn_wave_barrier();
-        const float tz = c[0] * (col[lane + 4]) + c[1] * (col[lane + 5] + col[lane + 3]) +
-                         c[2] * (col[lane + 6] + col[lane + 2]) + c[3] * (col[lane + 7] + col[lane + 1]) +
-                         c[4] * (col[lane + 8] + col[lane]);
+        const float tz = c[0] * (col[cl + 4]) + c[1] * (col[cl + 5] + col[cl + 3]) +
+                         c[2] * (col[cl + 6] + col[cl + 2]) + c[3] * (col[cl + 7] + col[cl + 1]) +
+                         c[4] * (col[cl + 8] + col[cl]);
         __builtin_amdgcn_wave_barrier();
         val[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
@@ -805,7 +811,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         int kb[PX], kn[PX];
 #pragma unroll
         for (int p = 0; p < PX; p++) {
-            const int e = IMG ? s_p.inj.cmap[lane + TZ * (wv * PX + p)] : me[p];
+            const int e = IMG ? (lane_ok ? s_p.inj.cmap[lane + TZ * (wv * PX + p)] : 0) : me[p];
             kb[p] = e >> 8;
             kn[p] = e & 0xff;
         }
@@ -899,6 +905,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
         for (int p = 0; p < PX; p++) {
             const int l = lane + TP * (wv * PX + p);
+            if (!lane_ok) continue;
             s_b[l] = u3v[p];
             s_p.edge.u2[l] = wu[p + 4];
             s_p.edge.u1[l] = r1[p];
@@ -906,7 +913,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         __syncthreads();
         higdon_tile<TZ, TX>(S, hg, z0, x0, top, bot, left, right, s_b, s_p.edge.u2, s_p.edge.u1);
 #pragma unroll
-        for (int p = 0; p < PX; p++) u3v[p] = s_b[lane + TP * (wv * PX + p)];
+        for (int p = 0; p < PX; p++) u3v[p] = s_b[cl + TP * (wv * PX + p)];
     }
 
     // ---- write u3 on every non-corner cell of the strip (FD region from the
@@ -926,7 +933,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // Kernel entry: SINGLE batches read the shot descriptor from the kernel
 // arguments (scalar loads, no dependent table lookups); batched launches
 // find their shot through the logical-tile table.
-template <int PX, bool ADJ, bool SINGLE, int MODE>
+template <int PX, bool ADJ, bool SINGLE, int MODE, int TZ>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
 __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
     // mode 3: the lean step plus imaging blocks after the step tiles
@@ -948,9 +955,9 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : Ste
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
-        fd_step_body<PX, ADJ, BODY>(A, A.s0, lb);
+        fd_step_body<PX, ADJ, BODY, TZ>(A, A.s0, lb);
     } else {
-        fd_step_body<PX, ADJ, BODY>(A, A.shots[gp(A.blk_shot)[lb]], lb);
+        fd_step_body<PX, ADJ, BODY, TZ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

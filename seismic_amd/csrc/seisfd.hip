@@ -752,7 +752,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.hz = (const float *)(base + o.hz);
         S.hx = (const float *)(base + o.hx);
         S.NZ = p.NZ; S.NX = p.NX; S.LDZ = p.LDZ; S.zoff = p.zoff;
-        S.zsh = p.zsh; S.xsh = p.xsh; S.ntz = p.ntz; S.ntx = p.ntx; S.tile0 = p.tile0;
+        S.zsh = p.zsh; S.xsh = p.xsh; S.ntz = p.ntz; S.ntx = p.ntx; S.tile0 = p.tile0; S.tz = tz;
         S.nz = g.nz; S.nxw = p.nxw; S.i0 = p.i0; S.iz0 = p.iz0; S.nt = p.nt; S.ntsnap = p.ntsnap;
         S.mirror = 2 * p.iz0 - 2;
         S.ext_ghost = g.rec_type != 0;
@@ -909,27 +909,34 @@ StepArgs Engine::base_args(const Batch &B) const {
 
 typedef void (*StepFn)(StepArgs);
 
-// Tiles are 64 rows x TX columns, one wavefront per 64 x PX strip.
-static int pick_px(int TZ, int TX) { return TZ != 64 ? 0 : (TX == 32 ? 4 : (TX == 64 ? 16 : 0)); }
-template <int PX, int MODE>
+// Tiles are 64 (or 56) rows x TX columns, one wavefront per TZ x PX strip;
+// 56-row tiles are the two-step kernel's tiling (a wavefront then spans a
+// tile and its 4-row rings), with 32 columns and 4-column strips.
+static int pick_px(int TZ, int TX) {
+    if (TZ == 56) return TX == 32 ? 4 : 0;
+    return TZ != 64 ? 0 : (TX == 32 ? 4 : (TX == 64 ? 16 : 0));
+}
+template <int PX, int MODE, int TZ>
 static StepFn step_fn(bool adj, bool single) {
-    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, MODE> : (StepFn)fd_step_kernel<PX, true, false, MODE>)
-               : (single ? (StepFn)fd_step_kernel<PX, false, true, MODE> : (StepFn)fd_step_kernel<PX, false, false, MODE>);
+    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, MODE, TZ>
+                         : (StepFn)fd_step_kernel<PX, true, false, MODE, TZ>)
+               : (single ? (StepFn)fd_step_kernel<PX, false, true, MODE, TZ>
+                         : (StepFn)fd_step_kernel<PX, false, false, MODE, TZ>);
 }
 // Mode 3 (step tiles + imaging blocks) is launched for one shot only.
-template <int PX>
+template <int PX, int TZ>
 static StepFn step_fn_image(bool adj, bool single) {
     if (!single) return nullptr;
-    return adj ? (StepFn)fd_step_kernel<PX, true, true, 3> : (StepFn)fd_step_kernel<PX, false, true, 3>;
+    return adj ? (StepFn)fd_step_kernel<PX, true, true, 3, TZ> : (StepFn)fd_step_kernel<PX, false, true, 3, TZ>;
 }
 static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int px_override = 0) {
     const int want = px_override ? px_override : pick_px(TZ, TX);
-#define K(px)                                                                                            \
-    if (TZ == 64 && TX == StepShape<px>::TX && want == px)                                               \
-        return mode == 3   ? step_fn_image<px>(adj, single)                                              \
-               : mode == 2 ? step_fn<px, 2>(adj, single)                                                 \
-                           : (mode == 1 ? step_fn<px, 1>(adj, single) : step_fn<px, 0>(adj, single));
-    K(4) K(8) K(16)
+#define K(px, tz)                                                                                        \
+    if (TZ == tz && TX == StepShape<px>::TX && want == px)                                               \
+        return mode == 3   ? step_fn_image<px, tz>(adj, single)                                          \
+               : mode == 2 ? step_fn<px, 2, tz>(adj, single)                                             \
+                           : (mode == 1 ? step_fn<px, 1, tz>(adj, single) : step_fn<px, 0, tz>(adj, single));
+    K(4, 64) K(8, 64) K(16, 64) K(4, 56)
 #undef K
     return nullptr;
 }
@@ -970,7 +977,8 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     const int px = strip_px ? strip_px : pick_px(tz, tx);
     const int threads = 64 * ((px >= 8 ? 4 * px : 32) / px);
     if (image) {
-        if (tz != 64 || (tx & (tx - 1))) throw EngineError(SFD_EUNSUPPORTED, "imaging needs 64-row, 2^k-column tiles");
+        if ((tz != 64 && tz != 56) || (tx & (tx - 1)))
+            throw EngineError(SFD_EUNSUPPORTED, "imaging needs 64- or 56-row, 2^k-column tiles");
         int nxw = 1;
         for (int s = 0; s < B.ns; s++) nxw = std::max(nxw, B.shots_h[s].nxw);
         const int cols = A.single ? kImgColsSingle : kImgColsBatch;
@@ -1297,7 +1305,7 @@ void Engine::check_shots(const sfd_shot *shots, int nshots, int snaps, bool may_
 // is more than 2% faster.  Runs on the first call after sfd_set_tile(h, 0, 0).
 void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
     struct Cand { int tz, tx, px, tb; };
-    static const Cand cands[] = {{64, 32, 4, 1}, {64, 32, 8, 1}, {64, 64, 16, 1}, {64, 32, 4, 2}};
+    static const Cand cands[] = {{64, 32, 4, 1}, {64, 32, 8, 1}, {64, 64, 16, 1}, {B2Z, B2X, 4, 2}};
     int nmin = shots[0].nt;
     for (int s = 1; s < ns; s++) nmin = std::min(nmin, shots[s].nt);
     const int steps1 = std::max(1, std::min(32, nmin));         // one-step launches timed
@@ -1368,7 +1376,7 @@ size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
     shot_window(g, s, &oxw, &nxw);
     const size_t NZ = (size_t)g.nz + top_rows(g) + g.ntaper + 8, NX = (size_t)nxw + 2 * g.ntaper + 8;
     const size_t LDZ = (NZ + 64 + 31) / 32 * 32;
-    const size_t ntile = (NZ + 64) / 64 * ((NX + 32) / 32 + 1);
+    const size_t ntile = (NZ + tz) / tz * ((NX + tx) / tx + 1);
     const size_t npix = (size_t)g.nz * nxw, ntr = (size_t)std::max(s.ntr, 1), nt = (size_t)s.nt;
     const size_t ntsnap = (size_t)std::max(s.nt / g.snapi, 1);
     size_t b = (kLevels + 2) * LDZ * NX * 4;                 // levels, v^2, vel

@@ -1,6 +1,7 @@
-"""Two steps per launch (SFD_OPT_TIME_BLOCK = 2): every result must be
-bitwise equal to the one-step kernel's (same arithmetic per cell, same
-order), which is itself bitwise equal to the oracle."""
+"""Two steps per launch (SFD_OPT_TIME_BLOCK = 2, on its 56 x 32 tiling):
+every result must be bitwise equal to the one-step kernel's on the same
+tiling (same arithmetic per cell, same order, same FP16 scale blocks), which
+is itself bitwise equal to the oracle."""
 import numpy as np
 import pytest
 
@@ -15,6 +16,7 @@ pytestmark = pytest.mark.gpu
 def engine(nz, nx, ntaper, vel, tb, dz=10.0, dt=1e-3, snapi=4):
     e = sa.Engine(sa.make_grid(nz, nx, dz, dz, dt, ntaper=ntaper, snapi=snapi))
     e.set_model(vel)
+    e.set_tile(56, 32)
     e.set_option(sa.abi.SFD_OPT_TIME_BLOCK, tb)
     return e
 
