@@ -92,15 +92,20 @@ contains
   !            (read when surf_type /= 0).  The FD bounds follow
   !            read_window_vel's rule (ExtractPadModel.f90:228-233, which needs
   !            SEPlib and is not compiled); vel_pad must have those bounds.
+  !   with_rho: withRho, rho_pad the padded density (as vel_pad): Interpolate,
+  !            allocate_sxx2d_szz2d_delp2d, FD_acoustic_rho_init_coefs with
+  !            coefpower 1 and FD_2D_derivatives_acoustic_forward_grid as the
+  !            stencil whatever `stencil` says, as the RTM and modeling
+  !            drivers call them (to_disk_to_memory_RTM.f90:389-397,454-462)
   subroutine ref_propagate(nz, nxw, ntaper, dz, dx, dt, oz, oxw, lsinc, snapi, vel_pad, &
        &                   stencil, inj, nsou, souz, soux, soutr, ext, ndat, datz, datx, dattr, &
-       &                   nt, tdir, nsnap, wave, rec_type, shot_type, surf_type, elevj) &
+       &                   nt, tdir, nsnap, wave, rec_type, shot_type, surf_type, elevj, with_rho, rho_pad) &
        &                   bind(C, name='ref_propagate')
     integer(c_int), value :: nz, nxw, ntaper, lsinc, snapi, stencil, inj, nsou, ext, ndat, nt, tdir, nsnap
-    integer(c_int), value :: rec_type, shot_type, surf_type
+    integer(c_int), value :: rec_type, shot_type, surf_type, with_rho
     real(c_float), value  :: dz, dx, dt, oz, oxw
     real(c_float)         :: vel_pad(*), souz(*), soux(*), soutr(nt, *), datz(*), datx(*), dattr(nt, *)
-    real(c_float)         :: wave(nz, nxw, *), elevj(*)
+    real(c_float)         :: wave(nz, nxw, *), elevj(*), rho_pad(*)
 
     type(GeneralParam)         :: genpar
     type(FDbounds)             :: bounds
@@ -115,7 +120,7 @@ contains
     integer :: i, j, it, n1, n2
 
     genpar%twoD = .true.
-    genpar%withRho = .false.
+    genpar%withRho = with_rho /= 0
     genpar%Born = .false.
     genpar%LSRTM = .false.
     genpar%verbose = .false.
@@ -133,6 +138,7 @@ contains
     genpar%shot_type = shot_type
     genpar%surf_type = surf_type
     genpar%coefpower = 2
+    if (with_rho /= 0) genpar%coefpower = 1
     if (tdir > 0) then
        genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
     else
@@ -166,6 +172,17 @@ contains
        end do
     end do
 
+    if (with_rho /= 0) then
+       allocate(model%rho(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, 1:1))
+       allocate(model%rho2(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, 1:1))
+       do j = 1, n2
+          do i = 1, n1
+             model%rho(bounds%nmin1-5+i, bounds%nmin2-5+j, 1) = rho_pad(i + (j-1)*n1)
+          end do
+       end do
+       call Interpolate(model, bounds, genpar)
+    end if
+
     allocate(sou(max(nsou,1)), dat(max(ndat,1)))
     do i = 1, nsou
        sou(i)%coord = (/ souz(i), soux(i), 0.0 /)
@@ -188,7 +205,12 @@ contains
     allocate(hig%gx(16, bounds%nmin1:bounds%nmax1, bounds%nmin3:bounds%nmax3))
     allocate(hig%gy(16, bounds%nmin1:bounds%nmax1, bounds%nmin2:bounds%nmax2))
     allocate(hig%gz(16, bounds%nmin2:bounds%nmax2, bounds%nmin3:bounds%nmax3))
-    call FD_acoustic_init_coefs(fdcoefs)
+    if (with_rho /= 0) then
+       call FD_acoustic_rho_init_coefs(fdcoefs)
+       call allocate_sxx2d_szz2d_delp2d(bounds, model)
+    else
+       call FD_acoustic_init_coefs(fdcoefs)
+    end if
     call FD_types_assign_scaled_coefs(fdcoefs, scaled, genpar)
     call FD_derivatives_coef_init(scaled)
     call Higdon(genpar%dt, model, bounds, hig)
@@ -220,7 +242,9 @@ contains
        end if
        if (nsnap > 0) call Extraction_wavefield(bounds, model, elev, grid%u2, genpar, it, dat=wfld)
        call Boundary_set_free_surface_grid(bounds, model, elev, grid, genpar)
-       if (stencil == 0) then
+       if (with_rho /= 0) then
+          call FD_2D_derivatives_acoustic_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       else if (stencil == 0) then
           call FD_2nd_2D_derivatives_scalar_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
        else
           call FD_2nd_2D_derivatives_scalar_adjoint_grid(genpar, bounds, grid%u2, grid%u3, model)
@@ -251,6 +275,40 @@ contains
     call deallocateHigdonParam(hig)
     call deallocateUSpace(grid)
     call deallocateModelSpace_elev(elev)
+    if (with_rho /= 0) call deallocate_sxx2d_szz2d_delp2d()
   end subroutine ref_propagate
+
+  ! Interpolate.f90:44-130 on a padded density (as vel_pad): rho2 out
+  subroutine ref_interpolate_rho(nz, nxw, ntaper, rho_pad, rho2_pad) bind(C, name='ref_interpolate_rho')
+    integer(c_int), value :: nz, nxw, ntaper
+    real(c_float)         :: rho_pad(*), rho2_pad(*)
+    type(GeneralParam)    :: genpar
+    type(FDbounds)        :: bounds
+    type(ModelSpace)      :: model
+    integer :: i, j, n1, n2
+    genpar%nbound = 0
+    bounds%nmin1 = -ntaper + 1
+    bounds%nmax1 = nz + ntaper
+    bounds%nmin2 = -ntaper + 1
+    bounds%nmax2 = nxw + ntaper
+    bounds%nmin3 = 1
+    bounds%nmax3 = 1
+    n1 = bounds%nmax1 - bounds%nmin1 + 9
+    n2 = bounds%nmax2 - bounds%nmin2 + 9
+    allocate(model%rho(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, 1:1))
+    allocate(model%rho2(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, 1:1))
+    do j = 1, n2
+       do i = 1, n1
+          model%rho(bounds%nmin1-5+i, bounds%nmin2-5+j, 1) = rho_pad(i + (j-1)*n1)
+       end do
+    end do
+    call Interpolate(model, bounds, genpar)
+    do j = 1, n2
+       do i = 1, n1
+          rho2_pad(i + (j-1)*n1) = model%rho2(bounds%nmin1-5+i, bounds%nmin2-5+j, 1)
+       end do
+    end do
+    deallocate(model%rho, model%rho2)
+  end subroutine ref_interpolate_rho
 
 end module ref_harness

@@ -69,6 +69,22 @@ void orc_bounds(ORC_BOUNDS *b, int nz, int nxw, int ntaper)
     orc_bounds_t(b, nz, nxw, ntaper, 7, 0, 0);
 }
 
+/* Density per model cell (nz*nx, column-major; the reference's mod%rho with
+ * withRho): a copy held for the oracle's calls; NULL or n = 0 restores
+ * constant density (withRho false). */
+static float *orc_rho = NULL;
+static size_t orc_rho_n = 0;
+void orc_set_density(const float *r, size_t n)
+{
+    free(orc_rho);
+    orc_rho = NULL;
+    orc_rho_n = 0;
+    if (!r || n < 1) return;
+    orc_rho = (float *)malloc(sizeof(float) * n);
+    memcpy(orc_rho, r, sizeof(float) * n);
+    orc_rho_n = n;
+}
+
 /* Free-surface elevation per model column (ModelSpace_elevation%elev, which
  * the reference's drivers set to 0): a copy held for the oracle's calls;
  * NULL or n = 0 restores 0 everywhere. */

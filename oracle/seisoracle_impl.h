@@ -108,7 +108,8 @@ typedef struct {
     REAL *vel;               /* padded vel, NZ*NX                        */
     REAL c[10];              /* c0z..c4z, c0x..c4x (FDtypes.f90:367-387) */
     REAL *gz, *gx;           /* Higdon tables                            */
-    int stencil;             /* 0 forward v^2 lap(u), 1 adjoint-style lap(v^2 u) */
+    int stencil;             /* 0 forward v^2 lap(u), 1 adjoint-style lap(v^2 u) (with_rho: the
+                                variable-density operator either way, as the drivers call it) */
     int inject;              /* 0 none, 1 sinc v^2, 2 simple v^2, 3 sinc 1/(dz dx) */
     int nsou; TR *sou;
     int extract;             /* -1 none, else SFD_EXTRACT_*              */
@@ -125,14 +126,29 @@ typedef struct {
     REAL *capture; int ncapture;  /* optional: u2 at imaging steps (checks) */
     int rec_type, shot_type, surf_type;
     int *ielev; REAL *delev;      /* per FD column nmin2..nmax2 (surf_type != 0) */
+    int with_rho;                 /* variable density (genpar%withRho) */
+    REAL *rho, *delp;             /* padded rho and 1/rho2 (NZ*NX) */
+    REAL *s2dzz, *s2dxx;          /* staggered gradients (NZ*NX), persistent over the steps */
 } CAT(PROP_, SFX);
 #define PROP CAT(PROP_, SFX)
 
 #define AT(P, u, i, j) (u)[ORC_IDX(&(P)->b, i, j)]
 
-/* FDtypes.f90:50-80 scaled coefficients, FDcoefs.f90:25-36 values */
+/* FDtypes.f90:50-80 scaled coefficients, FDcoefs.f90:25-36 values; with
+ * variable density FDcoefs.f90:12-23 (staggered first derivative) scaled by
+ * 1/delta (coefpower 1, ReadParam.f90:49-53) */
 static void CAT(init_coefs_, SFX)(PROP *P)
 {
+    if (P->with_rho) {
+        REAL r[5] = {(REAL)0., (REAL)1225. / (REAL)1024., -((REAL)1225. / ((REAL)1024. * (REAL)15.)),
+                     (REAL)1225. / ((REAL)1024. * (REAL)125.), -((REAL)1225. / ((REAL)1024. * (REAL)1715.))};
+        REAL dzi = (REAL)1. / P->dz, dxi = (REAL)1. / P->dx;
+        for (int k = 0; k < 5; k++) {
+            P->c[k] = r[k] * dzi;
+            P->c[5 + k] = r[k] * dxi;
+        }
+        return;
+    }
     REAL c0 = (REAL)-205. / (REAL)72., c1 = (REAL)8. / (REAL)5, c2 = (REAL)-1. / (REAL)5.,
          c3 = (REAL)8. / (REAL)315., c4 = (REAL)-1. / (REAL)560.;
     REAL dzi = (REAL)1. / P->dz, dxi = (REAL)1. / P->dx;
@@ -223,6 +239,121 @@ static void CAT(fd_adjoint_, SFX)(PROP *P, const REAL *u2, REAL *u3)
 #undef V2
 }
 
+/* FDderivatives.f90:588-657 FD_2D_derivatives_acoustic_forward_grid: staggered
+ * gradients times 1/rho2 on the FD region, first-order gradients times the
+ * boundary's 1/rho2 in the operator padding (3 rows / columns; the corners
+ * and the outermost row / column are never computed and keep their initial
+ * zero), then the staggered divergence times v^2 rho. */
+static void CAT(fd_rho_, SFX)(PROP *P, const REAL *u2, REAL *u3)
+{
+    const ORC_BOUNDS *b = &P->b;
+    const REAL *c = P->c;
+    REAL *sz = P->s2dzz, *sx = P->s2dxx;
+    const REAL dzi = (REAL)1. / P->dz, dxi = (REAL)1. / P->dx;
+#define D(i, j) AT(P, P->delp, i, j)
+#define U(i, j) AT(P, u2, i, j)
+#pragma omp parallel for schedule(static)
+    for (int j = b->nmin2; j <= b->nmax2; j++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
+            AT(P, sz, i, j) = (c[1] * (U(i + 1, j) - U(i, j)) + c[2] * (U(i + 2, j) - U(i - 1, j)) +
+                               c[3] * (U(i + 3, j) - U(i - 2, j)) + c[4] * (U(i + 4, j) - U(i - 3, j))) * D(i, j);
+            AT(P, sx, i, j) = (c[6] * (U(i, j + 1) - U(i, j)) + c[7] * (U(i, j + 2) - U(i, j - 1)) +
+                               c[8] * (U(i, j + 3) - U(i, j - 2)) + c[9] * (U(i, j + 4) - U(i, j - 3))) * D(i, j);
+        }
+    for (int j = b->nmin2; j <= b->nmax2; j++) {
+        for (int i = b->nmin1 - 1; i >= b->nmin1 - 3; i--) {
+            AT(P, sz, i, j) = dzi * (U(i + 1, j) - U(i, j)) * D(b->nmin1, j);
+            AT(P, sx, i, j) = dxi * (U(i, j + 1) - U(i, j)) * D(b->nmin1, j);
+        }
+        for (int i = b->nmax1 + 1; i <= b->nmax1 + 3; i++) {
+            AT(P, sz, i, j) = dzi * (U(i + 1, j) - U(i, j)) * D(b->nmax1, j);
+            AT(P, sx, i, j) = dxi * (U(i, j + 1) - U(i, j)) * D(b->nmax1, j);
+        }
+    }
+    for (int j = b->nmin2 - 1; j >= b->nmin2 - 3; j--)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
+            AT(P, sz, i, j) = dzi * (U(i + 1, j) - U(i, j)) * D(i, b->nmin2);
+            AT(P, sx, i, j) = dxi * (U(i, j + 1) - U(i, j)) * D(i, b->nmin2);
+        }
+    for (int j = b->nmax2 + 1; j <= b->nmax2 + 3; j++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
+            AT(P, sz, i, j) = dzi * (U(i + 1, j) - U(i, j)) * D(i, b->nmax2);
+            AT(P, sx, i, j) = dxi * (U(i, j + 1) - U(i, j)) * D(i, b->nmax2);
+        }
+#undef U
+#undef D
+#pragma omp parallel for schedule(static)
+    for (int j = b->nmin2; j <= b->nmax2; j++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
+            REAL tz = c[1] * (AT(P, sz, i, j) - AT(P, sz, i - 1, j)) + c[2] * (AT(P, sz, i + 1, j) - AT(P, sz, i - 2, j)) +
+                      c[3] * (AT(P, sz, i + 2, j) - AT(P, sz, i - 3, j)) + c[4] * (AT(P, sz, i + 3, j) - AT(P, sz, i - 4, j));
+            REAL tx = c[6] * (AT(P, sx, i, j) - AT(P, sx, i, j - 1)) + c[7] * (AT(P, sx, i, j + 1) - AT(P, sx, i, j - 2)) +
+                      c[8] * (AT(P, sx, i, j + 2) - AT(P, sx, i, j - 3)) + c[9] * (AT(P, sx, i, j + 3) - AT(P, sx, i, j - 4));
+            REAL v = AT(P, P->vel, i, j);
+            AT(P, u3, i, j) = v * v * AT(P, P->rho, i, j) * (tx + tz);
+        }
+}
+
+/* Interpolate.f90:44-130 Interpolate, 2D: rho2 = rho sinc-interpolated half a
+ * cell along z, then along x (9-point sinc at shift 0.5, ends extended by
+ * their edge values); delp = 1/rho2 (allocate_sxx2d_szz2d_delp2d,
+ * FDderivatives.f90:574-582). */
+static void CAT(rho_half_, SFX)(PROP *P)
+{
+    const ORC_BOUNDS *b = &P->b;
+    const int L = 9, h = L / 2;
+    REAL sinc[9];
+    FN(mksinc)(sinc, L, (REAL)0.5);
+    const int i0 = b->nmin1 - 4, i1 = b->nmax1 + 4, j0 = b->nmin2 - 4, j1 = b->nmax2 + 4;
+    const int nmax = (i1 - i0 > j1 - j0 ? i1 - i0 : j1 - j0) + 1 + 2 * L;
+    REAL *tr = (REAL *)malloc(sizeof(REAL) * (size_t)nmax), *r2 = P->delp;
+    for (int j = j0; j <= j1; j++) {
+        REAL *t = tr + L - i0;   /* t[i] for i in i0-L .. i1+L */
+        for (int i = i0; i <= i1; i++) t[i] = AT(P, P->rho, i, j);
+        for (int i = i0 - L; i <= i0 - 1; i++) t[i] = t[i0];
+        for (int i = i1 + 1; i <= i1 + L; i++) t[i] = t[i1];
+        for (int i = i0; i <= i1; i++) {
+            REAL f = 0;
+            for (int l = -h; l <= h; l++) f = f + sinc[h + l] * t[i + l];
+            AT(P, r2, i, j) = f;
+        }
+    }
+    for (int i = i0; i <= i1; i++) {
+        REAL *t = tr + L - j0;
+        for (int j = j0; j <= j1; j++) t[j] = AT(P, r2, i, j);
+        for (int j = j0 - L; j <= j0 - 1; j++) t[j] = t[j0];
+        for (int j = j1 + 1; j <= j1 + L; j++) t[j] = t[j1];
+        for (int j = j0; j <= j1; j++) {
+            REAL f = 0;
+            for (int l = -h; l <= h; l++) f = f + sinc[h + l] * t[j + l];
+            AT(P, r2, i, j) = f;
+        }
+    }
+    free(tr);
+    for (size_t q = 0; q < (size_t)b->NZ * b->NX; q++) r2[q] = (REAL)1. / r2[q];
+}
+
+/* readsou with withRho (ReadSouVelRho.f90:328-349): the source shifted half a
+ * time step forward by the lsinc-point sinc, renormalised where the
+ * interpolator runs off the trace */
+static void CAT(half_step_, SFX)(REAL *tr, int nt, int lsinc)
+{
+    REAL sinc[64], *o = (REAL *)malloc(sizeof(REAL) * (size_t)nt);
+    const int h = lsinc / 2;
+    FN(mksinc)(sinc, lsinc, (REAL)0.5);
+    memcpy(o, tr, sizeof(REAL) * (size_t)nt);
+    for (int it = 1; it <= nt; it++) {
+        REAL fscale = 1, fdum = 0;
+        for (int l = -h; l <= h; l++) {
+            if (it + l >= 1 && it + l <= nt) fdum = fdum + sinc[h + l] * o[it + l - 1];
+            else fscale = fscale - sinc[h + l];
+        }
+        tr[it - 1] = fdum / fscale;
+    }
+    free(o);
+}
+void FN(half_step)(REAL *tr, int nt, int lsinc) { CAT(half_step_, SFX)(tr, nt, lsinc); }
+
 /* FDderivatives.f90:960-982 FD_2nd_time_derivative_grid */
 static void CAT(time_update_, SFX)(PROP *P, const REAL *u_1, const REAL *u0, const REAL *u1,
                                    const REAL *u2, REAL *u3)
@@ -310,6 +441,7 @@ static void CAT(inject_, SFX)(PROP *P, REAL *u, int it)
         } else {
             REAL v = AT(P, P->vel, s->icoord[0], s->icoord[1]);
             v2r = v * v;
+            if (P->with_rho) v2r = v2r * AT(P, P->rho, s->icoord[0], s->icoord[1]);   /* Injection.f90:172-176 */
         }
         if (P->inject == 2) {
             AT(P, u, s->icoord[0], s->icoord[1]) = AT(P, u, s->icoord[0], s->icoord[1]) + v2r * tr;
@@ -480,6 +612,10 @@ static void CAT(propagate_, SFX)(PROP *P)
     int k, it;
     for (k = 0; k < 5; k++) buf[k] = (REAL *)calloc(n, sizeof(REAL));
     /* buf[0]=u_1, buf[1]=u0, buf[2]=u1, buf[3]=u2, buf[4]=u3 */
+    if (P->with_rho) {   /* allocate_sxx2d_szz2d_delp2d: cells never written stay 0 */
+        P->s2dzz = (REAL *)calloc(n, sizeof(REAL));
+        P->s2dxx = (REAL *)calloc(n, sizeof(REAL));
+    }
     CAT(init_coefs_, SFX)(P);
     CAT(higdon_tables_, SFX)(P);
     for (k = 0; k < P->nsou; k++)
@@ -496,7 +632,8 @@ static void CAT(propagate_, SFX)(PROP *P)
         if (P->wave) CAT(extract_wave_, SFX)(P, buf[3], it);
         if (P->imaging && P->imaging != 4 && P->tstep == -1) CAT(imaging_, SFX)(P, buf[3], it);
         CAT(free_surface_, SFX)(P, buf[3]);
-        if (P->stencil == 0) CAT(fd_forward_, SFX)(P, buf[3], buf[4]);
+        if (P->with_rho) CAT(fd_rho_, SFX)(P, buf[3], buf[4]);
+        else if (P->stencil == 0) CAT(fd_forward_, SFX)(P, buf[3], buf[4]);
         else CAT(fd_adjoint_, SFX)(P, buf[3], buf[4]);
         if (P->inject && P->nsou) CAT(inject_, SFX)(P, buf[4], it);
         if (P->imaging == 4 && P->tstep == 1) CAT(imaging_, SFX)(P, buf[4], it);
@@ -508,6 +645,8 @@ static void CAT(propagate_, SFX)(PROP *P)
     for (k = 0; k < 5; k++) free(buf[k]);
     free(P->gz); free(P->gx);
     P->gz = P->gx = NULL;
+    free(P->s2dzz); free(P->s2dxx);
+    P->s2dzz = P->s2dxx = NULL;
 }
 
 /* ------------------------------------------------------------------ */
@@ -602,6 +741,19 @@ static int CAT(setup_shot_, SFX)(PROP *P, const sfd_grid *g, const REAL *vel_ful
             if (P->ielev[j - P->b.nmin2] < 1 || P->ielev[j - P->b.nmin2] > g->nz) return -1;
         }
     }
+    if (orc_rho && orc_rho_n == (size_t)g->nz * g->nx) {
+        /* read_window_vel's rho: windowed and padded like vel, then Interpolate
+         * (ExtractPadModel.f90:259-270) */
+        const size_t nb = (size_t)P->b.NZ * P->b.NX, nm = (size_t)g->nz * g->nx;
+        REAL *big = (REAL *)malloc(sizeof(REAL) * nm);
+        for (size_t q = 0; q < nm; q++) big[q] = (REAL)orc_rho[q];
+        P->with_rho = 1;
+        P->rho = (REAL *)calloc(nb, sizeof(REAL));
+        P->delp = (REAL *)calloc(nb, sizeof(REAL));
+        FN(window_pad)(g, big, P->oxw, P->nxw, &P->b, P->rho);
+        free(big);
+        CAT(rho_half_, SFX)(P);
+    }
     return 0;
 }
 
@@ -648,6 +800,7 @@ int FN(modeling)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const RE
     P.nsou = 1;
     P.sou = CAT(make_traces_, SFX)(1, &zz, &xx, s->nt);
     for (int it = 0; it < s->nt; it++) P.sou[0].trace[it] = wavelet[it];
+    if (P.with_rho) CAT(half_step_, SFX)(P.sou[0].trace, s->nt, P.lsinc);
     P.extract = extract_kind;
     P.ndat = s->ntr;
     P.dat = CAT(make_traces_, SFX)(s->ntr, s->rz, s->rx, s->nt);
@@ -658,7 +811,7 @@ int FN(modeling)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const RE
         memcpy(traces + (size_t)r * s->nt, P.dat[r].trace, sizeof(REAL) * s->nt);
     CAT(free_traces_, SFX)(P.sou, 1);
     CAT(free_traces_, SFX)(P.dat, s->ntr);
-    free(P.vel); free(P.ielev); free(P.delev);
+    free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
     return 0;
 }
 
@@ -822,13 +975,14 @@ static int CAT(adjoint_core_, SFX)(PROP P, const sfd_shot *s, const REAL *dbar, 
     free(P.gz); free(P.gx);
     CAT(free_traces_, SFX)(P.sou, 1);
     CAT(free_traces_, SFX)(P.dat, s->ntr);
-    free(P.vel); free(P.ielev); free(P.delev);
+    free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
     return 0;
 }
 
 int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *dbar, REAL *sbar,
                          int extract_kind)
 {
+    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
     if (g->rec_type || g->shot_type || g->surf_type) return -4;   /* no free surface or ghosts */
     PROP P;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -845,6 +999,7 @@ int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, 
 int FN(born_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *wavelet,
                      const REAL *dbar, REAL *mbar)
 {
+    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
     if (g->rec_type || g->shot_type || g->surf_type) return -4;   /* no free surface or ghosts */
     PROP P;
     float zz = s->sz, xx = s->sx;
@@ -888,6 +1043,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
             const REAL *wavelet, const REAL *traces_in, REAL *image, REAL *illum, int lsrtm,
             REAL *capture, REAL *win_image, REAL *win_illum)
 {
+    if (orc_rho && lsrtm) return -4;   /* variable density: forward modeling and RTM only */
     PROP P;
     float zz = s->sz, xx = s->sx;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -901,6 +1057,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
     P.sou = CAT(make_traces_, SFX)(1, &zz, &xx, s->nt);
     for (int it = 0; it < s->nt; it++) P.sou[0].trace[it] = wavelet[it];
     if (lsrtm) CAT(second_diff_, SFX)(P.sou[0].trace, s->nt, P.dt);
+    if (P.with_rho) CAT(half_step_, SFX)(P.sou[0].trace, s->nt, P.lsinc);
     P.wave = snaps;
     P.tmin = 1; P.tmax = s->nt; P.tstep = 1;
     CAT(propagate_, SFX)(&P);
@@ -925,7 +1082,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
     if (win_image) memcpy(win_image, img, n * sizeof(REAL));
     if (win_illum) memcpy(win_illum, ill, n * sizeof(REAL));
     CAT(free_traces_, SFX)(P.sou, s->ntr);
-    free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev);
+    free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
     return 0;
 }
 
@@ -935,6 +1092,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
 int FN(born)(const sfd_grid *g, const REAL *vel, const REAL *refl, const sfd_shot *s,
              const REAL *wavelet, REAL *traces)
 {
+    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
     PROP P;
     float zz = s->sz, xx = s->sx;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -964,7 +1122,7 @@ int FN(born)(const sfd_grid *g, const REAL *vel, const REAL *refl, const sfd_sho
     CAT(propagate_, SFX)(&P);
     for (int r = 0; r < s->ntr; r++) memcpy(traces + (size_t)r * s->nt, P.dat[r].trace, sizeof(REAL) * s->nt);
     CAT(free_traces_, SFX)(P.dat, s->ntr);
-    free(snaps); free(m); free(mw); free(tap); free(P.vel); free(P.ielev); free(P.delev);
+    free(snaps); free(m); free(mw); free(tap); free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
     return 0;
 }
 
@@ -1011,7 +1169,7 @@ int FN(test_agrad_capture)(const sfd_grid *g, const REAL *vel, const sfd_shot *s
     P.tmin = s->nt; P.tmax = 1; P.tstep = -1;
     CAT(propagate_, SFX)(&P);
     CAT(free_traces_, SFX)(P.sou, s->ntr);
-    free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev);
+    free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
     return 0;
 }
 
@@ -1124,6 +1282,7 @@ int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_s
                      const REAL *wavelet, const REAL *dobs, const REAL *mute, const REAL *vpmask,
                      const sfd_fwi_opts *o, double *fout, REAL *grad, REAL *illu_out, REAL *resid)
 {
+    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
     size_t N = (size_t)g->nz * g->nx, off = 0;
     REAL *illu = (REAL *)calloc(N, sizeof(REAL));
     double f = 0;
@@ -1177,7 +1336,7 @@ int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_s
         FN(copy_image)(g, P.oxw, P.nxw, img, grad);
         FN(copy_image)(g, P.oxw, P.nxw, ill, illu);
         CAT(free_traces_, SFX)(P.sou, s->ntr);
-        free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev);
+        free(snaps); free(img); free(ill); free(tap); free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
         off += (size_t)s->nt * s->ntr;
         ntot += s->ntr;
     }

@@ -372,14 +372,20 @@ def ref_taper(n):
 
 
 def ref_propagate(g, vel_pad, oxw, nxw, stencil, inj, sou_z, sou_x, sou_tr, ext, dat_z, dat_x,
-                  nt, tdir, nsnap, elev_fd=None):
+                  nt, tdir, nsnap, elev_fd=None, rho_pad=None):
     """Runs the reference kernels in Propagator.f90 order.  vel_pad: [NZ, NX].
     g's rec/shot/surf types are passed on; elev_fd: elevation per FD column
-    nmin2..nmax2 (None: 0)."""
+    nmin2..nmax2 (None: 0); rho_pad: padded density [NZ, NX] (withRho).
+
+    With a density the reference's FD_2D_derivatives_acoustic_forward_grid
+    reads a row and a column of its gradient arrays that nothing writes (they
+    only reach cells the Higdon bands overwrite, and the halo beyond them);
+    glibc's M_PERTURB makes those fresh allocations zero for the call, the
+    value the oracle takes."""
     fn = ref().ref_propagate
     fn.argtypes = ([C.c_int] * 3 + [C.c_float] * 5 + [C.c_int] * 2 + [C.c_void_p] + [C.c_int] * 3
                    + [C.c_void_p] * 3 + [C.c_int] * 2 + [C.c_void_p] * 3 + [C.c_int] * 3 + [C.c_void_p]
-                   + [C.c_int] * 3 + [C.c_void_p])
+                   + [C.c_int] * 3 + [C.c_void_p] + [C.c_int] + [C.c_void_p])
     ev = np.zeros(nxw + 2 * g.ntaper, np.float32) if elev_fd is None else \
         np.ascontiguousarray(elev_fd, dtype=np.float32)
     vp = np.asfortranarray(vel_pad, dtype=np.float32)
@@ -391,10 +397,51 @@ def ref_propagate(g, vel_pad, oxw, nxw, stencil, inj, sou_z, sou_x, sou_tr, ext,
     dx_ = np.ascontiguousarray(dat_x, dtype=np.float32)
     dtr = np.zeros((max(len(dz_), 1), nt), dtype=np.float32)
     wave = np.zeros((max(nsnap, 1), nxw, g.nz), dtype=np.float32)
-    fn(g.nz, nxw, g.ntaper, g.dz, g.dx, g.dt, g.oz, oxw, g.lsinc, g.snapi, vp.ctypes.data_as(C.c_void_p),
-       stencil, inj, len(sz), ptr(sz), ptr(sx), ptr(st), ext, len(dz_), ptr(dz_), ptr(dx_), ptr(dtr),
-       nt, tdir, nsnap, ptr(wave), g.rec_type, g.shot_type, g.surf_type, ptr(ev))
+    rp = np.zeros(1, np.float32) if rho_pad is None else np.asfortranarray(rho_pad, dtype=np.float32)
+    libc = C.CDLL(None)
+    if rho_pad is not None:
+        libc.mallopt(-6, 255)   # M_PERTURB: fresh allocations filled with zero bytes
+    try:
+        fn(g.nz, nxw, g.ntaper, g.dz, g.dx, g.dt, g.oz, oxw, g.lsinc, g.snapi, vp.ctypes.data_as(C.c_void_p),
+           stencil, inj, len(sz), ptr(sz), ptr(sx), ptr(st), ext, len(dz_), ptr(dz_), ptr(dx_), ptr(dtr),
+           nt, tdir, nsnap, ptr(wave), g.rec_type, g.shot_type, g.surf_type, ptr(ev),
+           int(rho_pad is not None), rp.ctypes.data_as(C.c_void_p))
+    finally:
+        if rho_pad is not None:
+            libc.mallopt(-6, 0)
     return dtr[:len(dz_)], wave[:nsnap]
+
+
+def ref_interpolate_rho(g, rho_pad, nxw):
+    """The reference's Interpolate on a padded density: rho2 [NZ, NX]."""
+    fn = ref().ref_interpolate_rho
+    fn.argtypes = [C.c_int] * 3 + [C.c_void_p] * 2
+    rp = np.asfortranarray(rho_pad, dtype=np.float32)
+    out = np.zeros_like(rp, order="F")
+    fn(g.nz, nxw, g.ntaper, rp.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def set_density(rho):
+    """The oracle's density (nz, nx) (None: constant density)."""
+    L = oracle()
+    L.orc_set_density.argtypes = [C.c_void_p, C.c_size_t]
+    if rho is None:
+        L.orc_set_density(None, 0)
+        return
+    r = colmajor(rho, np.float32)
+    L.orc_set_density(ptr(r), r.size)
+
+
+def half_step(trace, lsinc=7, prec="f"):
+    """readsou's half-time-step source shift (withRho)."""
+    L = oracle()
+    T = _dt(prec)
+    t = np.ascontiguousarray(trace, dtype=T).copy()
+    fn = getattr(L, "orc_%s_half_step" % prec)
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    fn(ptr(t), len(t), lsinc)
+    return t
 
 
 # ---- L-BFGS oracle ----
