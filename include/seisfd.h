@@ -144,6 +144,23 @@ int  sfd_device_info(char *name, int len, int *cu_count, size_t *hbm_bytes);
 int  sfd_set_model(sfd_handle h, const float *vel, const float *vel_bwd);
 
 /*
+ * Variable density (genpar%withRho, the model's rho file read by
+ * read_window_vel, libs/libwave/ExtractPadModel.f90:259-270): rho(nz,nx) > 0,
+ * column-major; NULL (the default) is constant density.  With a density the
+ * engine propagates with FD_2D_derivatives_acoustic_forward_grid
+ * (FDderivatives.f90:588-657: staggered gradients times 1/rho2, rho2 the
+ * density sinc-interpolated to the half cells by Interpolate, Interpolate.f90:
+ * 44-130, divergence times v^2 rho, coefficients FD_acoustic_rho_init_coefs
+ * over delta), injects with gain v^2 rho (Injection.f90:172-176) and shifts
+ * each source wavelet half a time step forward as readsou does
+ * (ReadSouVelRho.f90:328-349), in sfd_forward and sfd_rtm (both RTM passes use
+ * that operator, to_disk_to_memory_RTM.f90:389-397,454-462).  sfd_lsrtm,
+ * sfd_born, sfd_fwi_gradient and the exact adjoints return SFD_EUNSUPPORTED
+ * while a density is set; tiles must be 64 x 32 with 4-column strips.
+ */
+int  sfd_set_density(sfd_handle h, const float *rho);
+
+/*
  * Free-surface elevation (m, depth axis) per model column, nx values; NULL
  * (the default) is 0 everywhere, which is what the reference's drivers set
  * (progs/rtm/Acoustic_RTM_sep.f90:118-119, 3DRTM.f90:74-75).  Replaces

@@ -137,7 +137,7 @@ SFD_FCTGDT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.
 # Every symbol declared in include/seisfd.h, in declaration order.
 EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
-    "sfd_set_model", "sfd_set_elevation", "sfd_forward", "sfd_forward_noalloc", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
+    "sfd_set_model", "sfd_set_density", "sfd_set_elevation", "sfd_forward", "sfd_forward_noalloc", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
     "sfd_born_adjoint_exact", "sfd_lsrtm", "sfd_bandpass", "sfd_mute_mask",
     "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
     "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
@@ -167,6 +167,7 @@ def load():
     lib.sfd_device_info.argtypes = [C.c_char_p, C.c_int, P(C.c_int), P(C.c_size_t)]
     lib.sfd_set_model.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.sfd_set_elevation.argtypes = [C.c_void_p, C.c_void_p]
+    lib.sfd_set_density.argtypes = [C.c_void_p, C.c_void_p]
     lib.sfd_forward.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_int]
     lib.sfd_forward_noalloc.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_int]
     lib.sfd_rtm.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,

@@ -115,6 +115,9 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     std::vector<float> vel_h, velb_h;
     std::vector<float> elev_h;            // free-surface elevation per model column (empty: 0 everywhere)
+    std::vector<float> rho_h;             // density per model cell (empty: constant density)
+    float *d_rho = nullptr;
+    std::vector<std::vector<float>> src_shifted;   // variable density: half-step shifted source wavelets
     uint64_t model_version = 0;           // bumped by every set_model (plans read vel_h / velb_h)
     std::vector<ShotPlan> rtm_plans;      // host plan behind fwd_batch when its key is an RTM's
     float *d_vel = nullptr, *d_velb = nullptr;
@@ -177,6 +180,10 @@ struct Engine {
     void plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc, const float *sz, const float *sx,
                         int kind) const;
     bool surface() const { return g.surf_type != 0; }
+    bool has_rho() const { return !rho_h.empty(); }
+    void set_density(const float *rho);
+    const float *source_wavelet(const sfd_shot &sh, int k);
+    float padded_rho(const ShotPlan &p, int icz, int icx) const;
     bool ghosts_or_surface() const { return g.surf_type != 0 || g.shot_type != 0 || g.rec_type != 0; }
     void set_elevation(const float *elev);
     void plan_extraction(ShotPlan &p, const std::vector<float> &vh, const sfd_shot &s, int kind) const;

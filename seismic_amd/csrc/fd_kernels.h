@@ -131,6 +131,11 @@ struct ShotDev {
     int snap_mask;          // a window column's surface lies below model row 1: snapshots zero above it
     const int *fs_row;
     const float *fs_w;
+    // variable density (withRho): v2 holds v^2 rho; delp = 1/rho2 (rho at the
+    // half cells); sz / sx the staggered gradients of u2 times delp, written by
+    // rho_grad_kernel each step (cells it never writes stay zero)
+    const float *delp;
+    float *sz, *sx;
 };
 
 struct StepArgs {
@@ -388,6 +393,91 @@ __global__ void __launch_bounds__(256) free_surface_kernel(StepArgs A) {
     }
 }
 
+// Variable density, each step before the stencil (FD_2D_derivatives_acoustic_
+// forward_grid, FDderivatives.f90:588-640): the staggered gradients of u2
+// times delp = 1/rho2 on the FD region (8th order), first-order gradients
+// times the boundary's delp on the three operator-padding rows and columns
+// next to it (FD columns / rows only).  One thread per padded cell (grid:
+// row groups, columns, running shots); the outermost row and column and the
+// corners are never written (zero).  c[1..4] / c[6..9]: the staggered
+// coefficients over dz / dx.
+__global__ void __launch_bounds__(256) rho_grad_kernel(StepArgs A, float dzi, float dxi) {
+    const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.shot_order)[blockIdx.z]];
+    const int iz = (int)(blockIdx.x * blockDim.x + threadIdx.x), ix = blockIdx.y;
+    const int NZ = S.NZ, NX = S.NX;
+    if (iz >= NZ || ix >= NX) return;
+    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    if (!A.nbase && (its < 1 || its > S.nt)) return;
+    const bool zin = iz >= 4 && iz <= NZ - 5, xin = ix >= 4 && ix <= NX - 5;
+    const bool zpad = (iz >= 1 && iz <= 3) || (iz >= NZ - 4 && iz <= NZ - 2);
+    const bool xpad = (ix >= 1 && ix <= 3) || (ix >= NX - 4 && ix <= NX - 2);
+    int dz, dx;   // the cell whose delp scales a padding gradient
+    if (zin && xin) {
+        dz = iz; dx = ix;
+    } else if (zpad && xin) {
+        dz = iz <= 3 ? 4 : NZ - 5; dx = ix;
+    } else if (xpad && zin) {
+        dz = iz; dx = ix <= 3 ? 4 : NX - 5;
+    } else {
+        return;
+    }
+    const GLB float *u = gp(S.u[A.rot]);
+    const int LDZ = S.LDZ, zoff = S.zoff;
+    auto U = [&](int i, int j) -> float { return ldw(u, (unsigned)j * (unsigned)LDZ + (unsigned)(zoff + i)); };
+    const float d = ldw(gp(S.delp), (unsigned)dx * (unsigned)LDZ + (unsigned)(zoff + dz));
+    const float *c = A.c;
+    float gz, gx;
+    if (zin && xin) {
+        gz = (c[1] * (U(iz + 1, ix) - U(iz, ix)) + c[2] * (U(iz + 2, ix) - U(iz - 1, ix)) +
+              c[3] * (U(iz + 3, ix) - U(iz - 2, ix)) + c[4] * (U(iz + 4, ix) - U(iz - 3, ix))) * d;
+        gx = (c[6] * (U(iz, ix + 1) - U(iz, ix)) + c[7] * (U(iz, ix + 2) - U(iz, ix - 1)) +
+              c[8] * (U(iz, ix + 3) - U(iz, ix - 2)) + c[9] * (U(iz, ix + 4) - U(iz, ix - 3))) * d;
+    } else {
+        gz = dzi * (U(iz + 1, ix) - U(iz, ix)) * d;
+        gx = dxi * (U(iz, ix + 1) - U(iz, ix)) * d;
+    }
+    const unsigned q = (unsigned)ix * (unsigned)LDZ + (unsigned)(zoff + iz);
+    stw(gp(S.sz), q, gz);
+    stw(gp(S.sx), q, gx);
+}
+
+// Window + edge pad of the full-model density for one shot (as vel):
+// rho into out.  Then rho_half_kernel (twice: along z into tmp, along x into
+// delp) puts it on the half cells with the 9-point sinc at shift 0.5, the
+// padded array's ends extended by their edge values (Interpolate.f90:44-102),
+// and the x pass stores 1/rho2 (allocate_sxx2d_szz2d_delp2d, FDderivatives.f90:580).
+__global__ void rho_pad_kernel(const float *__restrict__ big, int nz, const int *__restrict__ colsrc, int nxw,
+                               int ntaper, int ntop, int NZ, int NX, int LDZ, int zoff, float *__restrict__ out) {
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = blockIdx.y;
+    if (iz >= NZ || ix >= NX) return;
+    int wz = iz - (4 + ntop), wx = ix - (4 + ntaper);
+    wz = wz < 0 ? 0 : (wz >= nz ? nz - 1 : wz);
+    wx = wx < 0 ? 0 : (wx >= nxw ? nxw - 1 : wx);
+    out[(size_t)ix * LDZ + zoff + iz] = (float)0 + big[(size_t)colsrc[wx] * nz + wz];
+}
+struct Sinc9 { float w[9]; };
+__global__ void rho_half_kernel(const float *__restrict__ in, float *__restrict__ out, int NZ, int NX, int LDZ,
+                                int zoff, Sinc9 s, int along_x) {
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = blockIdx.y;
+    if (iz >= NZ || ix >= NX) return;
+    float f = 0.f;
+#pragma unroll
+    for (int l = -4; l <= 4; l++) {
+        int i = along_x ? iz : iz + l, j = along_x ? ix + l : ix;
+        i = i < 0 ? 0 : (i >= NZ ? NZ - 1 : i);
+        j = j < 0 ? 0 : (j >= NX ? NX - 1 : j);
+        f = f + s.w[4 + l] * in[(size_t)j * LDZ + zoff + i];
+    }
+    out[(size_t)ix * LDZ + zoff + iz] = along_x ? 1.f / f : f;
+}
+// v2 <- v2 * rho (the variable-density gain v^2 rho, FDderivatives.f90:651-652)
+__global__ void v2_rho_kernel(float *__restrict__ v2, const float *__restrict__ rho, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        v2[q] = v2[q] * rho[q];
+}
+
 // Receiver extraction on u2 before the free surface changes it (ExtractData
 // precedes Boundary_set_free_surface_grid, Propagator.f90:176-198): launches
 // with a free surface extract here, one block per logical tile of width TX,
@@ -530,8 +620,9 @@ struct StepShape {
 // spend; 3 is mode 0 plus imaging blocks after the step tiles (imaging
 // steps of reverse passes of one shot, image_cells; batches launch mode 0
 // and image_kernel).
-template <int PX, bool ADJ, int MODE, int TZ>
+template <int PX, bool ADJ, int MODE, int TZ, bool RHO>
 __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S, const int lb) {
+    static_assert(!(RHO && (ADJ || MODE == 2)), "variable density: forward-style stencil, no Born term");
     constexpr bool IMG = MODE == 2;
     constexpr int NW = StepShape<PX>::NW, TX = StepShape<PX>::TX, NWIN = PX + 8;
     constexpr int NT = StepShape<PX>::THREADS;
@@ -661,13 +752,17 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const int izc = iz < 0 ? 0 : (iz >= NZ ? NZ - 1 : iz);
     const int hzc = hz < 0 ? 0 : (hz >= NZ ? NZ - 1 : hz);
     const unsigned zoc = (unsigned)(zoff + izc), hzo = (unsigned)(zoff + hzc);
-    float wu[NWIN], wv2[ADJ ? NWIN : 1], hu[PX], hv[ADJ ? PX : 1];
+    // Variable density: the x-window holds the x-gradient sx and the z column
+    // (own rows and halo) the z-gradient sz; u2 of the own cells is read apart.
+    const GLB float *__restrict__ wsrc = RHO ? gp((const float *)S.sx) : u2;
+    const GLB float *__restrict__ hsrc = RHO ? gp((const float *)S.sz) : u2;
+    float wu[NWIN], wv2[ADJ ? NWIN : 1], hu[PX], hv[ADJ ? PX : 1], zc[RHO ? PX : 1], uo_r[RHO ? PX : 1];
     float r1[PX], r0[PX], rm[PX], rv[PX];
 #pragma unroll
     for (int k = 0; k < NWIN; k++) {
         int ix = xs - 4 + k;
         ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
-        wu[k] = ldw(u2, (unsigned)ix * (unsigned)LDZ + zoc);
+        wu[k] = ldw(wsrc, (unsigned)ix * (unsigned)LDZ + zoc);
         if (ADJ) wv2[k] = ldw(v2p, (unsigned)ix * (unsigned)LDZ + zoc);
     }
 #pragma unroll
@@ -675,7 +770,11 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         int ix = xs + p;
         ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
         const unsigned cb = (unsigned)ix * (unsigned)LDZ;
-        hu[p] = ldw(u2, cb + hzo);
+        hu[p] = ldw(hsrc, cb + hzo);
+        if (RHO) {
+            zc[p] = ldw(hsrc, cb + zoc);
+            uo_r[p] = ldw(u2, cb + zoc);
+        }
         if (ADJ) hv[p] = ldw(v2p, cb + hzo);
         r1[p] = ldw(u1, cb + zoc);
         r0[p] = ldw(u0, cb + zoc);
@@ -695,6 +794,10 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     }
 
     const int it = (MODE == 1 || MODE == 2) ? time_index() : its;   // snapshot and Born steps
+    // u2 of the lane's own cells
+    float uo[PX];
+#pragma unroll
+    for (int p = 0; p < PX; p++) uo[p] = RHO ? uo_r[p] : wu[p + 4];
 
     // ---- snapshot steps of a forward pass that keeps snapshots (mode 1;
     // reverse-pass imaging runs in the imaging blocks of mode 3)
@@ -745,7 +848,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
                     const int ix = xs + p;
                     if (interior_z && ix >= S.i0 && ix < S.i0 + S.nxw &&
                         !(S.snap_mask && iz < gp(S.fs_row)[ix]))
-                        m = fmaxf(m, fabsf(wu[p + 4]));
+                        m = fmaxf(m, fabsf(uo[p]));
                 }
                 for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
                 // blocks whose largest value is below 1e-30 store zeros (1/m would overflow FP32)
@@ -767,21 +870,29 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
     for (int p = 0; p < PX; p++) {
         const int ix = xs + p;
-        const float u2c = wu[p + 4];
+        const float u2c = RHO ? zc[p] : wu[p + 4];
         // z-stencil column through wavefront-private LDS (all 64 lanes take part)
         if (lane_ok) col[lane + 4] = ADJ ? wv2[p + 4] : u2c;
         if (lane < 8) col[lane < 4 ? lane : TZ + lane] = ADJ ? hv[p] : hu[p];
         __builtin_amdgcn_wave_barrier();
-        const float tz = c[0] * (col[cl + 4]) + c[1] * (col[cl + 5] + col[cl + 3]) +
-                         c[2] * (col[cl + 6] + col[cl + 2]) + c[3] * (col[cl + 7] + col[cl + 1]) +
-                         c[4] * (col[cl + 8] + col[cl]);
+        float tz;
+        if (RHO)   // staggered divergence of sz (FDderivatives.f90:641-644)
+            tz = c[1] * (col[cl + 4] - col[cl + 3]) + c[2] * (col[cl + 5] - col[cl + 2]) +
+                 c[3] * (col[cl + 6] - col[cl + 1]) + c[4] * (col[cl + 7] - col[cl]);
+        else
+            tz = c[0] * (col[cl + 4]) + c[1] * (col[cl + 5] + col[cl + 3]) + c[2] * (col[cl + 6] + col[cl + 2]) +
+                 c[3] * (col[cl + 7] + col[cl + 1]) + c[4] * (col[cl + 8] + col[cl]);
         __builtin_amdgcn_wave_barrier();
         val[p] = 0.f;
         if (!(zin && ix >= 4 && ix <= NX - 5)) continue;
-        // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint)
+        // FD_scheme: FDderivatives.f90:728-738 (forward) / :877-887 (adjoint) / :645-652 (density)
 #define WX(k) (ADJ ? wv2[p + 4 + (k)] : wu[p + 4 + (k)])
-        const float tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) +
-                         c[8] * (WX(3) + WX(-3)) + c[9] * (WX(4) + WX(-4));
+        float tx;
+        if (RHO)
+            tx = c[6] * (WX(0) - WX(-1)) + c[7] * (WX(1) - WX(-2)) + c[8] * (WX(2) - WX(-3)) + c[9] * (WX(3) - WX(-4));
+        else
+            tx = c[5] * (WX(0)) + c[6] * (WX(1) + WX(-1)) + c[7] * (WX(2) + WX(-2)) + c[8] * (WX(3) + WX(-3)) +
+                 c[9] * (WX(4) + WX(-4));
 #undef WX
         val[p] = ADJ ? (tx + tz) : rv[p] * (tx + tz);
     }
@@ -796,7 +907,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             if (ix < S.i0 || ix >= S.i0 + S.nxw) continue;
             const size_t q = sbase + (size_t)(ix - S.i0) * S.nz;
             // with a free surface the snapshot is zero above it (Extraction.f90:580-592)
-            const float su = S.snap_mask && iz < gp(S.fs_row)[ix] ? 0.f : wu[p + 4];
+            const float su = S.snap_mask && iz < gp(S.fs_row)[ix] ? 0.f : uo[p];
             if (A.snap_fp16)
                 __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(su * inv16[p / 4])),
                                             gp((unsigned short *)S.snap) + q);
@@ -892,7 +1003,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             v = v + 2.f * gp(S.taper)[wx] * gp(S.born_m)[wq] * Sv / gp(S.vel)[(size_t)ix * LDZ + zo];
         }
         // TimeDer: FDderivatives.f90:974-976
-        u3v[p] = (20.f * wu[p + 4] - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * v) * A.div11;
+        u3v[p] = (20.f * uo[p] - 6.f * r1[p] - 4.f * r0[p] + rm[p] + A.c12dt2 * v) * A.div11;
     }
 
     const bool top = z0 <= 0 && !S.no_top, bot = z0 + TZ >= NZ, left = x0 <= 0, right = x0 + TX >= NX;
@@ -907,7 +1018,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             const int l = lane + TP * (wv * PX + p);
             if (!lane_ok) continue;
             s_b[l] = u3v[p];
-            s_p.edge.u2[l] = wu[p + 4];
+            s_p.edge.u2[l] = uo[p];
             s_p.edge.u1[l] = r1[p];
         }
         __syncthreads();
@@ -933,7 +1044,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // Kernel entry: SINGLE batches read the shot descriptor from the kernel
 // arguments (scalar loads, no dependent table lookups); batched launches
 // find their shot through the logical-tile table.
-template <int PX, bool ADJ, bool SINGLE, int MODE, int TZ>
+template <int PX, bool ADJ, bool SINGLE, int MODE, int TZ, bool RHO>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
 __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
     // mode 3: the lean step plus imaging blocks after the step tiles
@@ -955,9 +1066,9 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : Ste
     const int lb = logical_block(A);
     if (lb >= A.nblocks) return;
     if constexpr (SINGLE) {
-        fd_step_body<PX, ADJ, BODY, TZ>(A, A.s0, lb);
+        fd_step_body<PX, ADJ, BODY, TZ, RHO>(A, A.s0, lb);
     } else {
-        fd_step_body<PX, ADJ, BODY, TZ>(A, A.shots[gp(A.blk_shot)[lb]], lb);
+        fd_step_body<PX, ADJ, BODY, TZ, RHO>(A, A.shots[gp(A.blk_shot)[lb]], lb);
     }
 }
 

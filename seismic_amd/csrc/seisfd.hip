@@ -186,6 +186,7 @@ Engine::~Engine() {
     if (ev2) (void)hipEventDestroy(ev2);
     if (d_vel) (void)hipFree(d_vel);
     if (d_velb) (void)hipFree(d_velb);
+    if (d_rho) (void)hipFree(d_rho);
 }
 
 void Engine::set_model(const float *vel, const float *vel_bwd) {
@@ -222,6 +223,53 @@ void Engine::set_elevation(const float *elev) {
         elev_h.assign(elev, elev + g.nx);
     }
     model_version++;   // plans hold the surface rows
+}
+
+// Variable density (genpar%withRho with the model's rho, ReadSouVelRho.f90:
+// 504-516): rho per model cell, NULL for constant density.
+void Engine::set_density(const float *rho) {
+    const size_t n = (size_t)g.nz * g.nx;
+    if (!rho) {
+        rho_h.clear();
+        if (d_rho) (void)hipFree(d_rho);
+        d_rho = nullptr;
+    } else {
+        for (size_t i = 0; i < n; i++)
+            if (!(rho[i] > 0.f) || !std::isfinite(rho[i])) throw EngineError(SFD_EINVAL, "density must be finite and > 0");
+        rho_h.assign(rho, rho + n);
+        if (!d_rho) HIPCHK(hipMalloc(&d_rho, n * sizeof(float)));
+        HIPCHK(hipMemcpy(d_rho, rho, n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    model_version++;
+}
+
+// The source as the propagation injects it: with variable density readsou
+// shifts it half a time step forward with the lsinc-point sinc, renormalised
+// where the interpolator runs off the trace (ReadSouVelRho.f90:328-349).
+const float *Engine::source_wavelet(const sfd_shot &sh, int k) {
+    if (!has_rho()) return sh.wavelet;
+    if ((int)src_shifted.size() <= k) src_shifted.resize(k + 1);
+    std::vector<float> &o = src_shifted[k];
+    o.assign(sh.nt, 0.f);
+    float w[64];
+    const int h = g.lsinc / 2;
+    sfd_host_mksinc(w, g.lsinc, 0.5f);
+    for (int it = 1; it <= sh.nt; it++) {
+        float fscale = 1.f, fdum = 0.f;
+        for (int l = -h; l <= h; l++) {
+            if (it + l >= 1 && it + l <= sh.nt) fdum = fdum + w[h + l] * sh.wavelet[it + l - 1];
+            else fscale = fscale - w[h + l];
+        }
+        o[it - 1] = fdum / fscale;
+    }
+    return o.data();
+}
+
+// Padded density at a Fortran window cell, as padded_vel.
+float Engine::padded_rho(const ShotPlan &p, int icz, int icx) const {
+    int wz = std::min(std::max(icz, 1), g.nz) - 1;
+    int wx = std::min(std::max(icx, 1), p.nxw) - 1;
+    return rho_h[(size_t)p.colsrc[wx] * g.nz + wz];
 }
 
 // Padded velocity at a Fortran window cell (icz, icx) (1-based, may lie in
@@ -455,6 +503,7 @@ void Engine::plan_injection(ShotPlan &p, const std::vector<float> &vh, int nsrc,
         } else {
             const float v = padded_vel(vh, p, icz, icx);
             a = v * v;
+            if (has_rho()) a = a * padded_rho(p, icz, icx);   // Injection.f90:172-176
         }
         if (kind == 2 || kind == 5) {
             cs.push_back({iz, ix, k, a, 1.f, 1.f});
@@ -601,6 +650,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
         size_t snap, snap_bytes, snap_scale, image, illum, born_m;
         size_t fs_row, fs_w;
+        size_t sz, sx, delp;
     };
     std::vector<Off> off(ns);
     const bool tb = tz == B2Z && tx == B2X;
@@ -618,6 +668,12 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         for (int k = 0; k < kLevels; k++) o.u[k] = L.take(nf * sizeof(float));
         o.v2 = L.take(nf * sizeof(float));
         o.vel = L.take(nf * sizeof(float));
+        o.sz = o.sx = o.delp = 0;
+        if (has_rho()) {
+            o.sz = L.take(nf * sizeof(float));
+            o.sx = L.take(nf * sizeof(float));
+            o.delp = L.take(nf * sizeof(float));
+        }
         o.hz = L.take((size_t)16 * (p.NX - 8) * sizeof(float));
         o.hx = L.take((size_t)16 * (p.NZ - 8) * sizeof(float));
         o.colsrc = put(L, up, upn, p.colsrc);
@@ -761,6 +817,9 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.snap_mask = p.snap_mask;
         S.fs_row = p.fs_row.empty() ? nullptr : (const int *)(base + o.fs_row);
         S.fs_w = p.fs_w.empty() ? nullptr : (const float *)(base + o.fs_w);
+        S.sz = has_rho() ? (float *)(base + o.sz) : nullptr;
+        S.sx = has_rho() ? (float *)(base + o.sx) : nullptr;
+        S.delp = has_rho() ? (const float *)(base + o.delp) : nullptr;
         S.inj_tile = (const int *)(base + o.inj_tile);
         S.inj_cell = (const int *)(base + o.inj_cell);
         S.inj_cptr = (const int *)(base + o.inj_cptr);
@@ -871,6 +930,22 @@ void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float 
         const int nk = std::max(p.NX, p.NZ);
         hipLaunchKernelGGL(higdon_tables_kernel, dim3((nk + 255) / 256), dim3(256), 0, stream, S.vel, p.NZ, p.NX,
                            p.LDZ, p.zoff, g.dt, g.dz, g.dx, (float *)S.hz, (float *)S.hx);
+        if (has_rho()) {
+            // padded rho (in sx), rho2 along z (in sz), 1/rho2 along x (delp),
+            // v^2 rho; then the gradient arrays start at zero
+            Sinc9 s9;
+            sfd_host_mksinc(s9.w, 9, 0.5f);
+            const size_t nf = (size_t)p.LDZ * p.NX;
+            hipLaunchKernelGGL(rho_pad_kernel, grd, blk, 0, stream, (const float *)d_rho, g.nz, B.colsrc_d[s], p.nxw,
+                               g.ntaper, p.ntop, p.NZ, p.NX, p.LDZ, p.zoff, S.sx);
+            hipLaunchKernelGGL(rho_half_kernel, grd, blk, 0, stream, (const float *)S.sx, S.sz, p.NZ, p.NX, p.LDZ,
+                               p.zoff, s9, 0);
+            hipLaunchKernelGGL(rho_half_kernel, grd, blk, 0, stream, (const float *)S.sz, (float *)S.delp, p.NZ, p.NX,
+                               p.LDZ, p.zoff, s9, 1);
+            hipLaunchKernelGGL(v2_rho_kernel, dim3(2048), dim3(256), 0, stream, (float *)S.v2, (const float *)S.sx, nf);
+            hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, (char *)S.sz, nf * sizeof(float));
+            hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, (char *)S.sx, nf * sizeof(float));
+        }
     }
     HIPCHK(hipGetLastError());
 }
@@ -892,14 +967,18 @@ StepArgs Engine::base_args(const Batch &B) const {
     A.nblocks = B.nblocks;
     A.xcd_remap = xcd_remap;
     A.snapi = g.snapi;
-    // FDcoefs.f90:25-36 scaled as FDtypes.f90:367-387 (coefpower 2)
+    // FDcoefs.f90:25-36 scaled as FDtypes.f90:367-387 (coefpower 2); with
+    // variable density the staggered first-derivative coefficients
+    // (FDcoefs.f90:12-23) over delta (coefpower 1, ReadParam.f90:49-53)
     const float c0 = -205.f / 72.f, c1 = 8.f / 5, c2 = -1.f / 5.f, c3 = 8.f / 315.f, c4 = -1.f / 560.f;
     const float dzi = 1.f / g.dz, dxi = 1.f / g.dx;
     const float dz2 = dzi * dzi, dx2 = dxi * dxi;
     const float cc[5] = {c0, c1, c2, c3, c4};
+    const float cr[5] = {0.f, 1225.f / 1024.f, -(1225.f / (1024.f * 15.f)), 1225.f / (1024.f * 125.f),
+                         -(1225.f / (1024.f * 1715.f))};
     for (int k = 0; k < 5; k++) {
-        A.c[k] = cc[k] * dz2;
-        A.c[5 + k] = cc[k] * dx2;
+        A.c[k] = has_rho() ? cr[k] * dzi : cc[k] * dz2;
+        A.c[5 + k] = has_rho() ? cr[k] * dxi : cc[k] * dx2;
     }
     const float dt2 = g.dt * g.dt;
     A.c12dt2 = 12.f * dt2;
@@ -918,18 +997,32 @@ static int pick_px(int TZ, int TX) {
 }
 template <int PX, int MODE, int TZ>
 static StepFn step_fn(bool adj, bool single) {
-    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, MODE, TZ>
-                         : (StepFn)fd_step_kernel<PX, true, false, MODE, TZ>)
-               : (single ? (StepFn)fd_step_kernel<PX, false, true, MODE, TZ>
-                         : (StepFn)fd_step_kernel<PX, false, false, MODE, TZ>);
+    return adj ? (single ? (StepFn)fd_step_kernel<PX, true, true, MODE, TZ, false>
+                         : (StepFn)fd_step_kernel<PX, true, false, MODE, TZ, false>)
+               : (single ? (StepFn)fd_step_kernel<PX, false, true, MODE, TZ, false>
+                         : (StepFn)fd_step_kernel<PX, false, false, MODE, TZ, false>);
 }
 // Mode 3 (step tiles + imaging blocks) is launched for one shot only.
 template <int PX, int TZ>
 static StepFn step_fn_image(bool adj, bool single) {
     if (!single) return nullptr;
-    return adj ? (StepFn)fd_step_kernel<PX, true, true, 3, TZ> : (StepFn)fd_step_kernel<PX, false, true, 3, TZ>;
+    return adj ? (StepFn)fd_step_kernel<PX, true, true, 3, TZ, false>
+               : (StepFn)fd_step_kernel<PX, false, true, 3, TZ, false>;
 }
-static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int px_override = 0) {
+// Variable density: 64 x 32 tiles, 4-column strips, the one stencil for both
+// time directions (the drivers pass FD_2D_derivatives_acoustic_forward_grid
+// to both RTM passes), no Born term.
+static StepFn pick_rho_kernel(int TZ, int TX, bool single, int mode, int px_override) {
+    if (TZ != 64 || TX != 32 || (px_override && px_override != 4)) return nullptr;
+    switch (mode) {
+    case 0: return single ? (StepFn)fd_step_kernel<4, false, true, 0, 64, true> : (StepFn)fd_step_kernel<4, false, false, 0, 64, true>;
+    case 1: return single ? (StepFn)fd_step_kernel<4, false, true, 1, 64, true> : (StepFn)fd_step_kernel<4, false, false, 1, 64, true>;
+    case 3: return single ? (StepFn)fd_step_kernel<4, false, true, 3, 64, true> : nullptr;
+    default: return nullptr;
+    }
+}
+static StepFn pick_kernel(int TZ, int TX, bool adj, bool single, int mode, int px_override = 0, bool rho = false) {
+    if (rho) return pick_rho_kernel(TZ, TX, single, mode, px_override);
     const int want = px_override ? px_override : pick_px(TZ, TX);
 #define K(px, tz)                                                                                        \
     if (TZ == tz && TX == StepShape<px>::TX && want == px)                                               \
@@ -966,8 +1059,9 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
     const int mode = snap_launch(B, A);
     // reverse-pass imaging: one shot, mode 3; a batch, mode 0 + image_kernel
     const bool image = mode == 2 && A.imaging != IMG_BORN;
-    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? (A.single ? 3 : 0) : mode, strip_px);
-    if (!f) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
+    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? (A.single ? 3 : 0) : mode, strip_px, has_rho());
+    if (!f) throw EngineError(SFD_EUNSUPPORTED, has_rho() ? "variable density runs on 64x32 tiles with 4-column strips"
+                                                          : "tile shape not compiled");
     // the shots still running at step n: a prefix of the tile layout
     const int nact = B.active_shots(A.n);
     A.nblocks = B.tiles_upto[nact];
@@ -1004,6 +1098,16 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         for (int k = 0; k < nact; k++) nxmax = std::max(nxmax, B.shots_h[B.order[k]].NX - 8);
         hipLaunchKernelGGL(free_surface_kernel, dim3((nxmax + 255) / 256, nact), dim3(256), 0, stream, A);
     }
+    if (has_rho()) {
+        // the staggered gradients of u2 (after the surface is set) for the stencil
+        int nzmax = 1, nxmax = 1;
+        for (int k = 0; k < nact; k++) {
+            nzmax = std::max(nzmax, B.shots_h[B.order[k]].NZ);
+            nxmax = std::max(nxmax, B.shots_h[B.order[k]].NX);
+        }
+        hipLaunchKernelGGL(rho_grad_kernel, dim3((nzmax + 255) / 256, nxmax, nact), dim3(256), 0, stream, A,
+                           1.f / g.dz, 1.f / g.dx);
+    }
     if (mode == 1 && A.store_snap && A.snap_fp16 && A.snap_global) {
         // one FP16 scale per snapshot: max |u2| over each window before the step stores it
         hipLaunchKernelGGL(snap_max_kernel, dim3(kSnapParts, nact), dim3(256), 0, stream, A);
@@ -1023,7 +1127,7 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
 // table, and no Born source term.
 bool Engine::use_step2(const StepArgs &A) const {
     return time_block == 2 && tb_ok && tz == B2Z && tx == B2X && A.imaging != IMG_BORN && !A.snap_global &&
-           !ghosts_or_surface();
+           !ghosts_or_surface() && !has_rho();
 }
 
 void Engine::launch_step2(const Batch &B, StepArgs A, bool adj) {
@@ -1119,7 +1223,9 @@ void Engine::run_loop(const Batch &B, StepArgs A, bool adj, int nsteps) {
     if (A.inject) traces_time_major(B);
     const int dir = A.dir > 0 ? 0 : 1;
     // the step kernel must exist before any capture starts
-    if (!pick_kernel(tz, tx, adj, B.ns == 1, 0, strip_px)) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
+    if (!pick_kernel(tz, tx, adj, B.ns == 1, 0, strip_px, has_rho()))
+        throw EngineError(SFD_EUNSUPPORTED, has_rho() ? "variable density runs on 64x32 tiles with 4-column strips"
+                                                      : "tile shape not compiled");
     int period = kLevels;
     while (period % g.snapi) period += kLevels;
     const int K = std::max(period, (std::max(1, graph_chunk) + period - 1) / period * period);
@@ -1315,6 +1421,7 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
     tune_log.clear();
     for (const Cand &cd : cands) {
         if (cd.tb == 2 && (steps2 < 2 || ghosts_or_surface())) continue;
+        if (has_rho() && !(cd.tz == 64 && cd.tx == 32 && cd.px == 4 && cd.tb == 1)) continue;
         const int steps = cd.tb == 2 ? steps2 : steps1;
         tz = cd.tz; tx = cd.tx; strip_px = cd.px; time_block = cd.tb;
         std::vector<ShotPlan> plans(ns);
@@ -1379,7 +1486,7 @@ size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
     const size_t ntile = (NZ + tz) / tz * ((NX + tx) / tx + 1);
     const size_t npix = (size_t)g.nz * nxw, ntr = (size_t)std::max(s.ntr, 1), nt = (size_t)s.nt;
     const size_t ntsnap = (size_t)std::max(s.nt / g.snapi, 1);
-    size_t b = (kLevels + 2) * LDZ * NX * 4;                 // levels, v^2, vel
+    size_t b = (kLevels + 2 + (has_rho() ? 3 : 0)) * LDZ * NX * 4;   // levels, v^2, vel (+ gradients, 1/rho2)
     b += 16 * (NX + NZ) * 4 + 2 * NX * 4;                     // Higdon tables, window columns, taper
     b += 3 * nt * ntr * 4;                                    // injected (two layouts) and extracted traces
     b += ntr * 49 * 16 * 4 + ntr * 16 * 4 + ntile * 64;       // contributions (both sets, two-step regions)
@@ -1458,7 +1565,9 @@ void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extra
             plan_extraction(plans[s], vel_h, sh, extract_kind);
         }
         build_batch(fwd_batch, plans, need);
-        for (int s = 0; s < nw; s++) upload_inj_traces(fwd_batch, s, shots[s0 + s].wavelet, shots[s0 + s].nt);
+        for (int s = 0; s < nw; s++)
+            upload_inj_traces(fwd_batch, s, source_wavelet(shots[s0 + s], s), shots[s0 + s].nt);
+        HIPCHK(hipStreamSynchronize(stream));   // the shifted copies are rewritten by the next wave
         prepare_models(fwd_batch, plans, d_vel);
         StepArgs A = base_args(fwd_batch);
         A.dir = 1;
@@ -1545,6 +1654,7 @@ void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_
 // model through the transpose of born()'s clamped column gather.
 void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *mbar) {
     if (ghosts_or_surface()) throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
     check_shots(shots, ns, SFD_SNAP_FP32);
     if (!dbar || !mbar) throw EngineError(SFD_EINVAL, "NULL dbar/mbar");
     if (!(tz == 64 && (tx == 32 || tx == 64))) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
@@ -1623,6 +1733,7 @@ void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar
 }
 void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind) {
     if (ghosts_or_surface()) throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
     check_shots(shots, ns);
     if (!dbar || !sbar) throw EngineError(SFD_EINVAL, "NULL dbar/sbar");
     if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract_kind");
@@ -1718,6 +1829,7 @@ void Engine::set_traces(const float *tr, size_t n) {
 
 void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
                  bool lsrtm) {
+    if (lsrtm && has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
     check_shots(shots, ns, snap_mode);
     reset_kstats();
     if (!traces_in) {
@@ -1772,7 +1884,8 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
         std::vector<ShotPlan> &plans = rtm_plans;
         // forward source (second-differenced for LSRTM, ReadSouVelRho.f90:318-326)
         for (int s = 0; s < nw; s++) {
-            std::vector<float> w(ws[s].wavelet, ws[s].wavelet + ws[s].nt);
+            const float *w0 = source_wavelet(ws[s], s);
+            std::vector<float> w(w0, w0 + ws[s].nt);
             if (lsrtm) second_difference(w, g.dt);
             upload_inj_traces(B, s, w.data(), w.size());
             HIPCHK(hipStreamSynchronize(stream));   // w is freed at the end of this iteration
@@ -1835,6 +1948,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
 
 // ---- Born modeling (to_disk_to_memory_BORN.f90:123-220) ----------------------
 void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *traces_out) {
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
     check_shots(shots, ns, SFD_SNAP_FP32);
     reset_kstats();
     if (!refl || !traces_out) throw EngineError(SFD_EINVAL, "NULL refl/traces_out");
@@ -1908,6 +2022,7 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
 // post-processing chain runs once on the sums.
 void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
                           const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out) {
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
     check_shots(shots, ns, opt ? opt->snap_mode : -1);
     reset_kstats();
     if (!opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL opt/misfit/grad");
@@ -2554,6 +2669,10 @@ int sfd_copy_calibration(size_t nbytes, int reps, double *avg_ms) {
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return rc;
+}
+
+int sfd_set_density(sfd_handle h, const float *rho) {
+    return guarded(h, [&](Engine &e) { e.set_density(rho); });
 }
 
 int sfd_set_elevation(sfd_handle h, const float *elev) {

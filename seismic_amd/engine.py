@@ -112,6 +112,11 @@ class Engine:
         self._velb = None if vel_bwd is None else _model_buf(vel_bwd, self.g.nz, self.g.nx)
         self._chk(self.lib.sfd_set_model(self.h, _ptr(self._vel), _ptr(self._velb)))
 
+    def set_density(self, rho=None):
+        """Variable density rho(nz, nx) (genpar%withRho); None: constant density."""
+        self._rho = None if rho is None else _model_buf(rho, self.g.nz, self.g.nx)
+        self._chk(self.lib.sfd_set_density(self.h, _ptr(self._rho)))
+
     def set_elevation(self, elev=None):
         """Free-surface elevation per model column (ModelSpace_elevation%elev);
         None: 0 everywhere, as the reference's drivers set it."""

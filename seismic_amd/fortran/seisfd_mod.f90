@@ -111,6 +111,12 @@ module seisfd_mod
        type(c_ptr), value :: vel_bwd
      end function sfd_set_model
 
+     integer(c_int) function sfd_set_density(h, rho) bind(C, name='sfd_set_density')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       type(c_ptr), value :: rho
+     end function sfd_set_density
+
      integer(c_int) function sfd_set_elevation(h, elev) bind(C, name='sfd_set_elevation')
        import :: c_int, c_ptr
        type(c_ptr), value :: h
