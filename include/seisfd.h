@@ -555,6 +555,47 @@ int  sfd_cnn_loss_grad(sfd_cnn_handle c, int nimg, const float *x, const float *
 int  sfd_cnn_train(sfd_cnn_handle c, int nimg, const float *x, const float *target, int steps,
                    float lr, double *loss_first, double *loss_last);
 
+/* ---- 3D forward modeling (genpar%twoD false: Acoustic_modeling_sep.f90:
+ *      112-121; constant density, nbound 4).  A handle of its own: model
+ *      vel(nz,nx,ny) column-major (z fastest, then x, then y). -------------- */
+typedef struct sfd_grid3d {
+    int32_t nz, nx, ny;      /* model samples: depth, x, y                     */
+    float   oz, ox, oy;      /* origin (m)                                     */
+    float   dz, dx, dy;      /* spacing (m)                                    */
+    float   dt;              /* time step (s)                                  */
+    int32_t ntaper;          /* absorbing pad per side (cells), >= 10          */
+    int32_t lsinc;           /* sinc interpolator length (7)                   */
+    float   aperture_x, aperture_y;   /* migration apertures (m), ExtractPadModel.f90:55-58 */
+    float   fmax;            /* vel_check's frequency (Hz); 0: 30              */
+} sfd_grid3d;
+
+/* One 3D shot: coord(1..3) = z, x, y of the source and of each receiver. */
+typedef struct sfd_shot3d {
+    float        sz, sx, sy;
+    int32_t      nt, ntr;
+    const float *wavelet;
+    const float *rz, *rx, *ry;
+} sfd_shot3d;
+
+typedef struct sfd_ctx3d *sfd_handle3d;
+int  sfd_create3d(sfd_handle3d *h, const sfd_grid3d *g);
+int  sfd_destroy3d(sfd_handle3d h);
+const char *sfd_last_error3d(sfd_handle3d h);
+/* readvel + vel_check in 3D (Courant stop 0.35, ReadSouVelRho.f90:629-675). */
+int  sfd_set_model3d(sfd_handle3d h, const float *vel);
+/* Forward modeling of nshots 3D shots, one after another on the device: per
+ * shot the window of its patch (extract_coord_source_receiver_patch with the
+ * x and y apertures), FD_2nd_3D_derivatives_scalar_forward_grid
+ * (FDderivatives.f90:97-131), Injection_sinc of the source
+ * (Injection.f90:274-346), Boundary_3d's bands in the reference's order
+ * (Boundary.f90:89-184) and extraction of extract_kind
+ * (SFD_EXTRACT_*: Extraction.f90:71-162,217-246 -> :276-337).  As in the
+ * reference, the 3D sinc injection and extraction weight the y axis with the
+ * x sinc (Injection.f90:314, Extraction.f90:315), and SFD_EXTRACT_SINC_V2
+ * scales by v, not v^2, in 3D (Extraction.f90:128).  traces: per shot
+ * (nt, ntr) time fastest, concatenated. */
+int  sfd_forward3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, float *traces_out, int extract_kind);
+
 /* ---- SEP files (the reference reads and writes every array through SEPlib:
  *      from_aux / sreed / srite, libs/libwave/IO.f90:14-51,
  *      ReadSouVelRho.f90:59-297).  Host-side only: these calls touch no GPU.

@@ -278,6 +278,134 @@ contains
     if (with_rho /= 0) call deallocate_sxx2d_szz2d_delp2d()
   end subroutine ref_propagate
 
+  ! One 3D forward modeling run in Propagator.f90 order (genpar%twoD false,
+  ! nbound 4): ExtractData of kind ext (as ref_propagate), the 3D stencil
+  ! FD_2nd_3D_derivatives_scalar_forward_grid, Injection_sinc of the sources,
+  ! the time update, Boundary_3d.  vel_pad: the padded model with the bounds
+  ! 1-ntaper .. n+ntaper per axis and the 4-cell halos.  OpenMP runs on one
+  ! thread here: the reference's 3D sinc extraction adds to one trace sample
+  ! from every thread of its parallel loop (Extraction.f90:309-319), which is
+  ! ordered only on one thread.
+  subroutine ref_propagate3d(nz, nxw, nyw, ntaper, dz, dx, dy, dt, oz, oxw, oyw, lsinc, vel_pad, &
+       &                     nsou, souz, soux, souy, soutr, ext, ndat, datz, datx, daty, dattr, nt) &
+       &                     bind(C, name='ref_propagate3d')
+    use omp_lib
+    integer(c_int), value :: nz, nxw, nyw, ntaper, lsinc, nsou, ext, ndat, nt
+    real(c_float), value  :: dz, dx, dy, dt, oz, oxw, oyw
+    real(c_float)         :: vel_pad(*), souz(*), soux(*), souy(*), soutr(nt, *), datz(*), datx(*), daty(*)
+    real(c_float)         :: dattr(nt, *)
+
+    type(GeneralParam)         :: genpar
+    type(FDbounds)             :: bounds
+    type(ModelSpace)           :: model
+    type(ModelSpace_elevation) :: elev
+    type(HigdonParam)          :: hig
+    type(UnscaledFDcoefs)      :: fdcoefs
+    type(ScaledFDcoefs), target :: scaled
+    type(USpace)               :: grid
+    type(TraceSpace), allocatable :: sou(:), dat(:)
+    integer :: i, j, k, it, n1, n2, n3, nthr
+
+    genpar%twoD = .false.
+    genpar%withRho = .false.
+    genpar%Born = .false.
+    genpar%LSRTM = .false.
+    genpar%verbose = .false.
+    genpar%optim = .true.
+    genpar%dt = dt
+    genpar%nt = nt
+    genpar%delta = (/ dz, dx, dy /)
+    genpar%omodel = (/ oz, oxw, oyw /)
+    genpar%lsinc = lsinc
+    genpar%nbound = 4
+    genpar%ntaper = ntaper
+    genpar%snapi = 1
+    genpar%ntsnap = 0
+    genpar%rec_type = 0
+    genpar%shot_type = 0
+    genpar%surf_type = 0
+    genpar%coefpower = 2
+    genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
+
+    bounds%nmin1 = -ntaper + 1
+    bounds%nmax1 = nz + ntaper
+    bounds%nmin2 = -ntaper + 1
+    bounds%nmax2 = nxw + ntaper
+    bounds%nmin3 = -ntaper + 1
+    bounds%nmax3 = nyw + ntaper
+
+    model%nz = nz
+    model%nxw = nxw
+    model%nyw = nyw
+    model%dz = dz
+    model%dx = dx
+    model%dy = dy
+    n1 = bounds%nmax1 - bounds%nmin1 + 9
+    n2 = bounds%nmax2 - bounds%nmin2 + 9
+    n3 = bounds%nmax3 - bounds%nmin3 + 9
+    allocate(model%vel(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, bounds%nmin3-4:bounds%nmax3+4))
+    do k = 1, n3
+       do j = 1, n2
+          do i = 1, n1
+             model%vel(bounds%nmin1-5+i, bounds%nmin2-5+j, bounds%nmin3-5+k) = vel_pad(i + (j-1)*n1 + (k-1)*n1*n2)
+          end do
+       end do
+    end do
+
+    allocate(sou(max(nsou,1)), dat(max(ndat,1)))
+    do i = 1, nsou
+       sou(i)%coord = (/ souz(i), soux(i), souy(i) /)
+       allocate(sou(i)%trace(nt, 1))
+       sou(i)%trace(:, 1) = soutr(:, i)
+    end do
+    do i = 1, ndat
+       dat(i)%coord = (/ datz(i), datx(i), daty(i) /)
+       allocate(dat(i)%trace(nt, 1))
+       dat(i)%trace(:, 1) = 0.
+    end do
+
+    nthr = omp_get_max_threads()
+    call omp_set_num_threads(1)
+    call allocateUSpace(grid, genpar, bounds)
+    call zeroUSpace(grid)
+    allocate(hig%gx(16, bounds%nmin1:bounds%nmax1, bounds%nmin3:bounds%nmax3))
+    allocate(hig%gy(16, bounds%nmin1:bounds%nmax1, bounds%nmin2:bounds%nmax2))
+    allocate(hig%gz(16, bounds%nmin2:bounds%nmax2, bounds%nmin3:bounds%nmax3))
+    call FD_acoustic_init_coefs(fdcoefs)
+    call FD_types_assign_scaled_coefs(fdcoefs, scaled, genpar)
+    call FD_derivatives_coef_init(scaled)
+    call Higdon(genpar%dt, model, bounds, hig)
+    if (nsou > 0) call ModelSpace_compute_array_xyz_positions(genpar, sou(1:nsou))
+    if (ndat > 0) call ModelSpace_compute_array_xyz_positions(genpar, dat(1:ndat))
+
+    do it = genpar%tmin, genpar%tmax, genpar%tstep
+       if (ndat > 0) then
+          select case (ext)
+          case (0)
+             call Extraction_array_sinc(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          case (1)
+             call Extraction_array_simple_afwi_noomp(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          case (2)
+             call Extraction_array_LSRTM_sinc(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          case (3)
+             call Extraction_array_sinc_afwi(bounds, model, dat(1:ndat), grid%u2, genpar, it)
+          end select
+       end if
+       call FD_2nd_3D_derivatives_scalar_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       if (nsou > 0) call Injection_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+       call FD_2nd_time_derivative_grid(genpar, bounds, grid)
+       call Boundary_3d(genpar, bounds, grid, model, hig)
+       call FDswaptime_pointer(grid)
+    end do
+    call omp_set_num_threads(nthr)
+
+    do i = 1, ndat
+       dattr(:, i) = dat(i)%trace(:, 1)
+    end do
+    call deallocateHigdonParam(hig)
+    call deallocateUSpace(grid)
+  end subroutine ref_propagate3d
+
   ! Interpolate.f90:44-130 on a padded density (as vel_pad): rho2 out
   subroutine ref_interpolate_rho(nz, nxw, ntaper, rho_pad, rho2_pad) bind(C, name='ref_interpolate_rho')
     integer(c_int), value :: nz, nxw, ntaper

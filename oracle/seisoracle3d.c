@@ -1,0 +1,315 @@
+/*
+ * seisoracle3d.c — CPU oracle for 3D forward modeling (FP32).
+ *
+ * TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker, never by the
+ * product.  It restates the reference's 3D constant-density forward modeling
+ * (Acoustic_modeling_sep.f90:112-121 with genpar%twoD false, nbound 4):
+ *   window and padding  ExtractPadModel.f90:19-85 (patch with apertures in x
+ *                       and y), :183-186 (nxw, nyw), :509-530 mod_window_pad,
+ *                       ReadSouVelRho.f90:545-596 model_pad
+ *   Higdon tables       Boundary.f90:17-63 (gz per (x,y), gx per (z,y), gy per (z,x))
+ *   stencil             FDderivatives.f90:97-131 FD_2nd_3D_derivatives_scalar_forward_grid
+ *   injection           Injection.f90:151-194 Injection_sinc -> :274-346
+ *                       Injection_source_sinc_xyz (y weights from the x sinc,
+ *                       sinc(:,2), as the reference does)
+ *   extraction          Extraction.f90:109-134 (sinc, gain v, not v^2, in 3D),
+ *                       :71-106 (nearest), :217-246 (LSRTM, 1/(dz dx dy)),
+ *                       :137-162 (sinc, gain 1) -> :276-337 Extraction_1trace_sinc_xyz
+ *                       (y weights from the x sinc), summed k, j, i in order
+ *   time update         FDderivatives.f90:960-982
+ *   Higdon bands        Boundary.f90:89-184 Boundary_3d (surf_type 0)
+ * Pinned bitwise against the compiled reference kernels composed in
+ * Propagator.f90 order (tests/test_oracle_3d.py).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "seisfd.h"
+
+void orc_f_mksinc(float *sinc, int lsinc, float shift);
+void orc_f_higdon_coeff(float nu, float beta1, float beta2, float weight, float *c);
+
+typedef struct {
+    int nmin1, nmax1, nmin2, nmax2, nmin3, nmax3;
+    int NZ, NX, NY;
+} B3;
+
+#define IX3(b, i, j, k)                                                                                       \
+    ((size_t)((i) - ((b)->nmin1 - 4)) + (size_t)((j) - ((b)->nmin2 - 4)) * (size_t)(b)->NZ +                 \
+     (size_t)((k) - ((b)->nmin3 - 4)) * (size_t)(b)->NZ * (size_t)(b)->NX)
+
+static int nint3(float x) { return (int)roundf(x); }
+
+static void find_i_d3(int *ic, float *dc, float c, float o, float d)
+{
+    *ic = nint3((c - o) / d) + 1;
+    *dc = c - (o + (float)(*ic - 1) * d);
+}
+
+/* extract_coord_source_receiver_patch (x and y) and nxw / nyw */
+void orc_window3d(const sfd_grid3d *g, const sfd_shot3d *s, float *oxw, int *nxw, float *oyw, int *nyw)
+{
+    float sx = s->sx, sy = s->sy, gx1 = s->rx[0], gxn = s->rx[0], gy1 = s->ry[0], gyn = s->ry[0];
+    for (int i = 1; i < s->ntr; i++) {
+        gx1 = gx1 < s->rx[i] ? gx1 : s->rx[i];
+        gxn = gxn > s->rx[i] ? gxn : s->rx[i];
+        gy1 = gy1 < s->ry[i] ? gy1 : s->ry[i];
+        gyn = gyn > s->ry[i] ? gyn : s->ry[i];
+    }
+    float minx = sx < gx1 ? sx : gx1, miny = sy < gy1 ? sy : gy1;
+    float maxx = sx > gxn ? sx : gxn, maxy = sy > gyn ? sy : gyn;
+    float mx1 = (gx1 + sx) / 2, mxn = (gxn + sx) / 2, my1 = (gy1 + sy) / 2, myn = (gyn + sy) / 2;
+    float cx = mx1 + fabsf(mxn - mx1) / 2, cy = my1 + fabsf(myn - my1) / 2;
+    *oxw = minx < cx - g->aperture_x / 2 ? minx : cx - g->aperture_x / 2;
+    float endx = maxx > cx + g->aperture_x / 2 ? maxx : cx + g->aperture_x / 2;
+    *oyw = miny < cy - g->aperture_y / 2 ? miny : cy - g->aperture_y / 2;
+    float endy = maxy > cy + g->aperture_y / 2 ? maxy : cy + g->aperture_y / 2;
+    *nxw = nint3((endx - *oxw) / g->dx) + 1;
+    *nyw = nint3((endy - *oyw) / g->dy) + 1;
+}
+
+void orc_bounds3d(B3 *b, int nz, int nxw, int nyw, int ntaper)
+{
+    b->nmin1 = 1 - ntaper; b->nmax1 = nz + ntaper;
+    b->nmin2 = 1 - ntaper; b->nmax2 = nxw + ntaper;
+    b->nmin3 = 1 - ntaper; b->nmax3 = nyw + ntaper;
+    b->NZ = b->nmax1 - b->nmin1 + 9;
+    b->NX = b->nmax2 - b->nmin2 + 9;
+    b->NY = b->nmax3 - b->nmin3 + 9;
+}
+
+/* mod_window_pad (nearest model column and line, clamped) then model_pad
+ * (edge replication): padded(i,j,k) = the window's value at the clamped
+ * (i,j,k) */
+void orc_window_pad3d(const sfd_grid3d *g, const float *big, float oxw, int nxw, float oyw, int nyw, const B3 *b,
+                      float *out)
+{
+    for (int k = b->nmin3 - 4; k <= b->nmax3 + 4; k++) {
+        int kw = k < 1 ? 1 : (k > nyw ? nyw : k);
+        float cyy = (float)(kw - 1) * g->dy + oyw;
+        int iy = nint3((cyy - g->oy) / g->dy) + 1;
+        iy = iy < 1 ? 1 : (iy > g->ny ? g->ny : iy);
+        for (int j = b->nmin2 - 4; j <= b->nmax2 + 4; j++) {
+            int jw = j < 1 ? 1 : (j > nxw ? nxw : j);
+            float cxx = (float)(jw - 1) * g->dx + oxw;
+            int ix = nint3((cxx - g->ox) / g->dx) + 1;
+            ix = ix < 1 ? 1 : (ix > g->nx ? g->nx : ix);
+            for (int i = b->nmin1 - 4; i <= b->nmax1 + 4; i++) {
+                int iw = i < 1 ? 1 : (i > g->nz ? g->nz : i);
+                out[IX3(b, i, j, k)] =
+                    (float)0 + big[(size_t)(iw - 1) + (size_t)(ix - 1) * g->nz + (size_t)(iy - 1) * g->nz * g->nx];
+            }
+        }
+    }
+}
+
+typedef struct {
+    B3 b;
+    float dz, dx, dy, dt;
+    float *vel;
+    float *gz, *gx, *gy;   /* Higdon tables: 16 per (j,k), (i,k), (i,j) of the FD region */
+    float c[15];           /* c0z..c4z, c0x..c4x, c0y..c4y */
+} P3;
+
+#define A3(P, u, i, j, k) (u)[IX3(&(P)->b, i, j, k)]
+
+static void higdon3(P3 *P)
+{
+    const B3 *b = &P->b;
+    const int n1 = b->nmax1 - b->nmin1 + 1, n2 = b->nmax2 - b->nmin2 + 1, n3 = b->nmax3 - b->nmin3 + 1;
+    const float dxi = 1.f / P->dx, dyi = 1.f / P->dy, dzi = 1.f / P->dz;
+    P->gz = (float *)calloc((size_t)16 * n2 * n3, sizeof(float));
+    P->gx = (float *)calloc((size_t)16 * n1 * n3, sizeof(float));
+    P->gy = (float *)calloc((size_t)16 * n1 * n2, sizeof(float));
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++) {
+            float *g = P->gz + 16 * ((size_t)(j - b->nmin2) + (size_t)(k - b->nmin3) * n2);
+            orc_f_higdon_coeff(A3(P, P->vel, b->nmin1, j, k) * P->dt * dzi, 1.0f, 0.7f, 0.5f, g);
+            orc_f_higdon_coeff(A3(P, P->vel, b->nmax1, j, k) * P->dt * dzi, 1.0f, 0.7f, 0.5f, g + 8);
+        }
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
+            float *g = P->gx + 16 * ((size_t)(i - b->nmin1) + (size_t)(k - b->nmin3) * n1);
+            orc_f_higdon_coeff(A3(P, P->vel, i, b->nmin2, k) * P->dt * dxi, 1.0f, 0.7f, 0.5f, g);
+            orc_f_higdon_coeff(A3(P, P->vel, i, b->nmax2, k) * P->dt * dxi, 1.0f, 0.7f, 0.5f, g + 8);
+        }
+    for (int j = b->nmin2; j <= b->nmax2; j++)
+        for (int i = b->nmin1; i <= b->nmax1; i++) {
+            float *g = P->gy + 16 * ((size_t)(i - b->nmin1) + (size_t)(j - b->nmin2) * n1);
+            orc_f_higdon_coeff(A3(P, P->vel, i, j, b->nmin3) * P->dt * dyi, 1.0f, 0.7f, 0.5f, g);
+            orc_f_higdon_coeff(A3(P, P->vel, i, j, b->nmax3) * P->dt * dyi, 1.0f, 0.7f, 0.5f, g + 8);
+        }
+}
+
+static void stencil3(P3 *P, const float *u2, float *u3)
+{
+    const B3 *b = &P->b;
+    const float *c = P->c;
+#pragma omp parallel for schedule(static)
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) {
+#define U(ii, jj, kk) A3(P, u2, ii, jj, kk)
+                float tz = c[0] * (U(i, j, k)) + c[1] * (U(i + 1, j, k) + U(i - 1, j, k)) +
+                           c[2] * (U(i + 2, j, k) + U(i - 2, j, k)) + c[3] * (U(i + 3, j, k) + U(i - 3, j, k)) +
+                           c[4] * (U(i + 4, j, k) + U(i - 4, j, k));
+                float tx = c[5] * (U(i, j, k)) + c[6] * (U(i, j + 1, k) + U(i, j - 1, k)) +
+                           c[7] * (U(i, j + 2, k) + U(i, j - 2, k)) + c[8] * (U(i, j + 3, k) + U(i, j - 3, k)) +
+                           c[9] * (U(i, j + 4, k) + U(i, j - 4, k));
+                float ty = c[10] * (U(i, j, k)) + c[11] * (U(i, j, k + 1) + U(i, j, k - 1)) +
+                           c[12] * (U(i, j, k + 2) + U(i, j, k - 2)) + c[13] * (U(i, j, k + 3) + U(i, j, k - 3)) +
+                           c[14] * (U(i, j, k + 4) + U(i, j, k - 4));
+#undef U
+                float v = A3(P, P->vel, i, j, k);
+                A3(P, u3, i, j, k) = v * v * (tx + tz + ty);
+            }
+}
+
+static void update3(P3 *P, const float *um, const float *u0, const float *u1, const float *u2, float *u3)
+{
+    const B3 *b = &P->b;
+    const float div11 = 1.f / 11.f, dt2 = P->dt * P->dt;
+#pragma omp parallel for schedule(static)
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) {
+                size_t q = IX3(b, i, j, k);
+                u3[q] = (20.f * u2[q] - 6.f * u1[q] - 4.f * u0[q] + um[q] + 12.f * dt2 * u3[q]) * div11;
+            }
+}
+
+/* Higdon_factor with (u0, u1, u2) = (grid%u1, grid%u2, grid%u3) */
+static void hfac3(P3 *P, const float *U1, const float *U2, float *U3, int i, int j, int k, int axis)
+{
+    const B3 *b = &P->b;
+    const int n1 = b->nmax1 - b->nmin1 + 1, n2 = b->nmax2 - b->nmin2 + 1;
+    const float *g;
+    int di = 0, dj = 0, dk = 0;
+    switch (axis) {
+    case -1: g = P->gz + 16 * ((size_t)(j - b->nmin2) + (size_t)(k - b->nmin3) * n2); di = 1; break;
+    case 1: g = P->gz + 16 * ((size_t)(j - b->nmin2) + (size_t)(k - b->nmin3) * n2) + 8; di = -1; break;
+    case -2: g = P->gx + 16 * ((size_t)(i - b->nmin1) + (size_t)(k - b->nmin3) * n1); dj = 1; break;
+    case 2: g = P->gx + 16 * ((size_t)(i - b->nmin1) + (size_t)(k - b->nmin3) * n1) + 8; dj = -1; break;
+    case -3: g = P->gy + 16 * ((size_t)(i - b->nmin1) + (size_t)(j - b->nmin2) * n1); dk = 1; break;
+    default: g = P->gy + 16 * ((size_t)(i - b->nmin1) + (size_t)(j - b->nmin2) * n1) + 8; dk = -1; break;
+    }
+    float f = g[0] * A3(P, U3, i + di, j + dj, k + dk) + g[1] * A3(P, U3, i + 2 * di, j + 2 * dj, k + 2 * dk) +
+              g[2] * A3(P, U2, i, j, k) + g[3] * A3(P, U2, i + di, j + dj, k + dk) +
+              g[4] * A3(P, U2, i + 2 * di, j + 2 * dj, k + 2 * dk) + g[5] * A3(P, U1, i, j, k) +
+              g[6] * A3(P, U1, i + di, j + dj, k + dk) + g[7] * A3(P, U1, i + 2 * di, j + 2 * dj, k + 2 * dk);
+    A3(P, U3, i, j, k) = -f;
+}
+
+static void boundary3(P3 *P, const float *u1, const float *u2, float *u3)
+{
+    const B3 *b = &P->b;
+    const int nz0 = b->nmin1;
+    for (int k = b->nmin3; k <= b->nmax3; k++) {
+        for (int j = b->nmin2; j <= b->nmax2; j++) {
+            for (int i = nz0 + 3; i >= b->nmin1 - 4; i--) hfac3(P, u1, u2, u3, i, j, k, -1);
+            for (int i = b->nmax1 - 3; i <= b->nmax1 + 4; i++) hfac3(P, u1, u2, u3, i, j, k, 1);
+        }
+        for (int j = b->nmin2 + 3; j >= b->nmin2 - 4; j--)
+            for (int i = nz0; i <= b->nmax1; i++) hfac3(P, u1, u2, u3, i, j, k, -2);
+        for (int j = b->nmax2 - 3; j <= b->nmax2 + 4; j++)
+            for (int i = nz0; i <= b->nmax1; i++) hfac3(P, u1, u2, u3, i, j, k, 2);
+    }
+    for (int j = b->nmin2; j <= b->nmax2; j++) {
+        for (int k = b->nmin3 + 3; k >= b->nmin3 - 4; k--)
+            for (int i = nz0; i <= b->nmax1; i++) hfac3(P, u1, u2, u3, i, j, k, -3);
+        for (int k = b->nmax3 - 3; k <= b->nmax3 + 4; k++)
+            for (int i = nz0; i <= b->nmax1; i++) hfac3(P, u1, u2, u3, i, j, k, 3);
+    }
+}
+
+/* 3D forward modeling of one shot: traces (nt, ntr) time fastest. */
+int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3d *s, float *traces, int extract_kind)
+{
+    P3 P;
+    memset(&P, 0, sizeof P);
+    float oxw, oyw;
+    int nxw, nyw;
+    orc_window3d(g, s, &oxw, &nxw, &oyw, &nyw);
+    orc_bounds3d(&P.b, g->nz, nxw, nyw, g->ntaper);
+    const B3 *b = &P.b;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY;
+    P.dz = g->dz; P.dx = g->dx; P.dy = g->dy; P.dt = g->dt;
+    P.vel = (float *)calloc(n, sizeof(float));
+    float *buf[5];
+    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+    if (!P.vel || !buf[4]) return -2;
+    orc_window_pad3d(g, vel_full, oxw, nxw, oyw, nyw, b, P.vel);
+    {
+        /* FDcoefs.f90:25-36 over delta^2 per axis */
+        const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
+        const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
+        for (int k = 0; k < 5; k++) {
+            P.c[k] = cc[k] * (dzi * dzi);
+            P.c[5 + k] = cc[k] * (dxi * dxi);
+            P.c[10 + k] = cc[k] * (dyi * dyi);
+        }
+    }
+    higdon3(&P);
+    const int l = g->lsinc, h = (int)(l * 0.5);
+    /* source */
+    int sic[3];
+    float sdc[3], sw[3][64];
+    find_i_d3(&sic[0], &sdc[0], s->sz, g->oz, g->dz);
+    find_i_d3(&sic[1], &sdc[1], s->sx, oxw, g->dx);
+    find_i_d3(&sic[2], &sdc[2], s->sy, oyw, g->dy);
+    orc_f_mksinc(sw[0], l, sdc[0] / g->dz);
+    orc_f_mksinc(sw[1], l, sdc[1] / g->dx);
+    orc_f_mksinc(sw[2], l, sdc[2] / g->dy);
+    /* receivers */
+    int *ric = (int *)malloc(sizeof(int) * 3 * (size_t)s->ntr);
+    float *rw = (float *)malloc(sizeof(float) * 2 * 64 * (size_t)s->ntr);
+    for (int r = 0; r < s->ntr; r++) {
+        float dcz, dcx, dcy;
+        find_i_d3(&ric[3 * r], &dcz, s->rz[r], g->oz, g->dz);
+        find_i_d3(&ric[3 * r + 1], &dcx, s->rx[r], oxw, g->dx);
+        find_i_d3(&ric[3 * r + 2], &dcy, s->ry[r], oyw, g->dy);
+        orc_f_mksinc(rw + 128 * r, l, dcz / g->dz);
+        orc_f_mksinc(rw + 128 * r + 64, l, dcx / g->dx);
+        (void)dcy;   /* the y weights are the x sinc (Extraction.f90:315) */
+    }
+    memset(traces, 0, sizeof(float) * (size_t)s->nt * s->ntr);
+    for (int it = 1; it <= s->nt; it++) {
+        float *u2 = buf[3], *u3 = buf[4];
+        /* ExtractData */
+        for (int r = 0; r < s->ntr; r++) {
+            const int *ic = ric + 3 * r;
+            float *tr = traces + (size_t)r * s->nt + (it - 1);
+            if (extract_kind == SFD_EXTRACT_NEAREST) {
+                *tr = *tr + A3(&P, u2, ic[0], ic[1], ic[2]);
+                continue;
+            }
+            float gain = extract_kind == SFD_EXTRACT_SINC_V2 ? A3(&P, P.vel, ic[0], ic[1], ic[2])
+                         : extract_kind == SFD_EXTRACT_SINC_LSRTM ? 1.f / (g->dz * g->dx * g->dy)
+                                                                  : 1.f;
+            const float *wz = rw + 128 * r, *wx = wz + 64;
+            float acc = *tr;
+            for (int k = -h; k <= h; k++)
+                for (int j = -h; j <= h; j++)
+                    for (int i = -h; i <= h; i++)
+                        acc = acc + gain * A3(&P, u2, i + ic[0], j + ic[1], k + ic[2]) * wz[h + i] * wx[h + j] * wx[h + k];
+            *tr = acc;
+        }
+        stencil3(&P, u2, u3);
+        /* Injection_sinc, v2r = vel^2 */
+        {
+            float v = A3(&P, P.vel, sic[0], sic[1], sic[2]), v2r = v * v, tr = s->wavelet[it - 1];
+            for (int k = -h; k <= h; k++)
+                for (int j = -h; j <= h; j++)
+                    for (int i = -h; i <= h; i++)
+                        A3(&P, u3, i + sic[0], j + sic[1], k + sic[2]) =
+                            A3(&P, u3, i + sic[0], j + sic[1], k + sic[2]) + v2r * tr * sw[0][h + i] * sw[1][h + j] * sw[1][h + k];
+        }
+        update3(&P, buf[0], buf[1], buf[2], buf[3], buf[4]);
+        boundary3(&P, buf[2], buf[3], buf[4]);
+        float *t = buf[0];
+        buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+    }
+    for (int q = 0; q < 5; q++) free(buf[q]);
+    free(P.vel); free(P.gz); free(P.gx); free(P.gy); free(ric); free(rw);
+    return 0;
+}

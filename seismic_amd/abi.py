@@ -68,6 +68,20 @@ class sfd_shot(C.Structure):
     ]
 
 
+class sfd_grid3d(C.Structure):
+    _fields_ = [("nz", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("oz", C.c_float), ("ox", C.c_float), ("oy", C.c_float),
+                ("dz", C.c_float), ("dx", C.c_float), ("dy", C.c_float), ("dt", C.c_float),
+                ("ntaper", C.c_int32), ("lsinc", C.c_int32),
+                ("aperture_x", C.c_float), ("aperture_y", C.c_float), ("fmax", C.c_float)]
+
+
+class sfd_shot3d(C.Structure):
+    _fields_ = [("sz", C.c_float), ("sx", C.c_float), ("sy", C.c_float), ("nt", C.c_int32), ("ntr", C.c_int32),
+                ("wavelet", C.POINTER(C.c_float)), ("rz", C.POINTER(C.c_float)), ("rx", C.POINTER(C.c_float)),
+                ("ry", C.POINTER(C.c_float))]
+
+
 class sfd_fwi_opts(C.Structure):
     _fields_ = [
         ("illupow", C.c_float),
