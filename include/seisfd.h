@@ -580,6 +580,7 @@ typedef struct sfd_shot3d {
 typedef struct sfd_ctx3d *sfd_handle3d;
 int  sfd_create3d(sfd_handle3d *h, const sfd_grid3d *g);
 int  sfd_destroy3d(sfd_handle3d h);
+/* h NULL: the message of the last failed sfd_create3d. */
 const char *sfd_last_error3d(sfd_handle3d h);
 /* readvel + vel_check in 3D (Courant stop 0.35, ReadSouVelRho.f90:629-675). */
 int  sfd_set_model3d(sfd_handle3d h, const float *vel);

@@ -9,4 +9,4 @@ from .abi import (SFD_SNAP_FP32, SFD_SNAP_FP16S, SFD_SNAP_FP16G, SFD_EXTRACT_SIN
                   SFD_EXTRACT_SINC_LSRTM, SFD_EXTRACT_SINC_UNIT, SFD_NORM_L2, SFD_NORM_L1, SFD_NORM_HUBER,
                   SFD_NORM_CAUCHY, SFD_NORM_HYPERBOLIC)
 from .engine import (Engine, Shot, LBFGS, LBFGSOC, CNN, lbfgs_run, lbfgsoc_step_files, lbfgsoc_rotate_files,  # noqa: F401
-                     make_grid, device_info, SeisFDError)
+                     make_grid, device_info, SeisFDError, Engine3D, Shot3D, make_grid3d)

@@ -1,0 +1,484 @@
+// seisfd3d.hip — 3D constant-density forward modeling on gfx950 (the
+// reference's propagator_acoustic with genpar%twoD false: Acoustic_modeling_sep
+// .f90:112-121, nbound 4).
+//
+// One shot at a time: the shot's window and its padded model live in HBM,
+// five rotating wavefield levels.  Each time step runs, in the reference's
+// order (Propagator.f90:168-243): receiver extraction on u2, the 25-point
+// stencil u3 = v^2 ((lap_x + lap_z) + lap_y), the source's 7x7x7 sinc
+// injection, the 5-level time update, and Boundary_3d's Higdon bands — z
+// bands per (x, y) line, then x bands per (z, y) line, then y bands per (z, x)
+// line, each line a sequential recursion outwards (Boundary.f90:89-184).  The
+// stencil and update are one thread per cell with z the fastest (coalesced)
+// index; the 3D path is a correctness path (seven launches a step), not tuned.
+// Compiled with -ffp-contract=off: the reference's FP32 rounding bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/seisfd.h"
+#include "common.h"
+
+void sfd_host_mksinc(float *sinc, int lsinc, float shift);
+
+namespace sfd3 {
+
+struct Geo {
+    int NZ, NX, NY;   // padded dims (FD region + 4-cell halo per side)
+    int LDZ;          // leading dimension (NZ rounded up to 32)
+    size_t plane;     // LDZ * NX
+    __host__ __device__ size_t at(int iz, int ix, int iy) const {
+        return (size_t)iy * plane + (size_t)ix * LDZ + (size_t)iz;
+    }
+};
+
+// Boundary.f90:66-87 Higdon_coeff
+__device__ __forceinline__ void higdon_coeff3(float nu, float *c) {
+    const float beta1 = 1.0f, beta2 = 0.7f, weight = 0.5f;
+    float fdum = (beta1 + nu) * (1.f - weight);
+    const float qx = (weight * (beta1 + nu) - nu) / fdum;
+    const float qt = (weight * (beta1 + nu) - beta1) / fdum;
+    const float qxt = weight / (weight - 1.f);
+    fdum = (beta2 + nu) * (1.f - weight);
+    const float rx = (weight * (beta2 + nu) - nu) / fdum;
+    const float rt = (weight * (beta2 + nu) - beta2) / fdum;
+    const float rxt = weight / (weight - 1.f);
+    c[0] = qx + rx;
+    c[1] = qx * rx;
+    c[2] = qt + rt;
+    c[3] = qx * rt + qt * rx + qxt + rxt;
+    c[4] = qx * rxt + rx * qxt;
+    c[5] = qt * rt;
+    c[6] = qt * rxt + rt * qxt;
+    c[7] = qxt * rxt;
+}
+
+// mod_window_pad + model_pad (3D): the window's value at the clamped cell;
+// v2 = v * v.  colsrc / rowsrc: model column / line (0-based) of each window
+// column / line.
+__global__ void pad3_kernel(const float *__restrict__ big, int nz, int nx, const int *__restrict__ colsrc, int nxw,
+                            const int *__restrict__ linesrc, int nyw, int ntaper, Geo G, float *__restrict__ vel,
+                            float *__restrict__ v2) {
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x, ix = blockIdx.y, iy = blockIdx.z;
+    if (iz >= G.NZ) return;
+    int wz = iz - 4 - ntaper, wx = ix - 4 - ntaper, wy = iy - 4 - ntaper;
+    wz = wz < 0 ? 0 : (wz >= nz ? nz - 1 : wz);
+    wx = wx < 0 ? 0 : (wx >= nxw ? nxw - 1 : wx);
+    wy = wy < 0 ? 0 : (wy >= nyw ? nyw - 1 : wy);
+    const float v = (float)0 + big[(size_t)wz + (size_t)colsrc[wx] * nz + (size_t)linesrc[wy] * nz * nx];
+    const size_t q = G.at(iz, ix, iy);
+    vel[q] = v;
+    v2[q] = v * v;
+}
+
+// Boundary.f90:36-60 Higdon tables: gz per (x, y) of the FD region from the
+// top / bottom FD rows, gx per (z, y) from the left / right FD columns, gy
+// per (z, x) from the front / back FD lines.
+__global__ void higdon3_kernel(const float *__restrict__ vel, Geo G, float dt, float dz, float dx, float dy,
+                               float *__restrict__ gz, float *__restrict__ gx, float *__restrict__ gy) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y, which = blockIdx.z;
+    const int fz = G.NZ - 8, fx = G.NX - 8, fy = G.NY - 8;
+    const float dzi = 1.f / dz, dxi = 1.f / dx, dyi = 1.f / dy;
+    if (which == 0) {   // a = x, b = y
+        if (a >= fx || b >= fy) return;
+        float *g = gz + 16 * ((size_t)a + (size_t)b * fx);
+        higdon_coeff3(vel[G.at(4, a + 4, b + 4)] * dt * dzi, g);
+        higdon_coeff3(vel[G.at(G.NZ - 5, a + 4, b + 4)] * dt * dzi, g + 8);
+    } else if (which == 1) {   // a = z, b = y
+        if (a >= fz || b >= fy) return;
+        float *g = gx + 16 * ((size_t)a + (size_t)b * fz);
+        higdon_coeff3(vel[G.at(a + 4, 4, b + 4)] * dt * dxi, g);
+        higdon_coeff3(vel[G.at(a + 4, G.NX - 5, b + 4)] * dt * dxi, g + 8);
+    } else {   // a = z, b = x
+        if (a >= fz || b >= fx) return;
+        float *g = gy + 16 * ((size_t)a + (size_t)b * fz);
+        higdon_coeff3(vel[G.at(a + 4, b + 4, 4)] * dt * dyi, g);
+        higdon_coeff3(vel[G.at(a + 4, b + 4, G.NY - 5)] * dt * dyi, g + 8);
+    }
+}
+
+struct Coef3 { float c[15]; };
+
+// FDderivatives.f90:97-131 on the FD region: u3 = v^2 ((tx + tz) + ty)
+__global__ void __launch_bounds__(256) stencil3_kernel(const float *__restrict__ u2, const float *__restrict__ v2,
+                                                       float *__restrict__ u3, Geo G, Coef3 C) {
+    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
+    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
+    const size_t q = G.at(iz, ix, iy);
+    const size_t sx = G.LDZ, sy = G.plane;
+    const float *c = C.c;
+#define U(o) u2[q + (o)]
+    const float tz = c[0] * (U(0)) + c[1] * (U(1) + U(-1)) + c[2] * (U(2) + U(-2)) + c[3] * (U(3) + U(-3)) +
+                     c[4] * (U(4) + U(-4));
+    const float tx = c[5] * (U(0)) + c[6] * (U(sx) + U(-(long)sx)) + c[7] * (U(2 * sx) + U(-2 * (long)sx)) +
+                     c[8] * (U(3 * sx) + U(-3 * (long)sx)) + c[9] * (U(4 * sx) + U(-4 * (long)sx));
+    const float ty = c[10] * (U(0)) + c[11] * (U(sy) + U(-(long)sy)) + c[12] * (U(2 * sy) + U(-2 * (long)sy)) +
+                     c[13] * (U(3 * sy) + U(-3 * (long)sy)) + c[14] * (U(4 * sy) + U(-4 * (long)sy));
+#undef U
+    u3[q] = v2[q] * (tx + tz + ty);
+}
+
+// Injection_source_sinc_xyz (Injection.f90:274-346, shot_type 0): one thread
+// per footprint cell (every cell gets one term), u3 += v2r s wz wx wy with the
+// y weights from the x sinc, as the reference computes them.
+struct Inj3 { int iz, ix, iy; float v2r; float wz[7], wx[7]; };
+__global__ void inject3_kernel(float *__restrict__ u3, Geo G, Inj3 I, const float *__restrict__ wavelet, int it) {
+    const int t = threadIdx.x;
+    if (t >= 343) return;
+    const int i = t % 7 - 3, j = (t / 7) % 7 - 3, k = t / 49 - 3;
+    const size_t q = G.at(I.iz + i, I.ix + j, I.iy + k);
+    u3[q] = u3[q] + I.v2r * wavelet[it - 1] * I.wz[3 + i] * I.wx[3 + j] * I.wx[3 + k];
+}
+
+// FDderivatives.f90:960-982 on the FD region
+__global__ void __launch_bounds__(256) update3_kernel(const float *__restrict__ um, const float *__restrict__ u0,
+                                                      const float *__restrict__ u1, const float *__restrict__ u2,
+                                                      float *__restrict__ u3, Geo G, float c12dt2, float div11) {
+    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
+    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
+    const size_t q = G.at(iz, ix, iy);
+    u3[q] = (20.f * u2[q] - 6.f * u1[q] - 4.f * u0[q] + um[q] + c12dt2 * u3[q]) * div11;
+}
+
+// Higdon_factor (Boundary.f90:366-447) at cell q along a line of stride s
+// (s < 0: the band lies at the low end and the recursion reads q - s, q - 2s)
+__device__ __forceinline__ void hfac(const float *g, float *U3, const float *U2, const float *U1, size_t q, long s) {
+    const float f = g[0] * U3[q + s] + g[1] * U3[q + 2 * s] + g[2] * U2[q] + g[3] * U2[q + s] +
+                    g[4] * U2[q + 2 * s] + g[5] * U1[q] + g[6] * U1[q + s] + g[7] * U1[q + 2 * s];
+    U3[q] = -f;
+}
+
+// z bands of every (x, y) line of the FD region: rows nmin1+3 .. nmin1-4
+// (padded 7 .. 0), then nmax1-3 .. nmax1+4
+__global__ void zband3_kernel(float *__restrict__ U3, const float *__restrict__ U2, const float *__restrict__ U1,
+                              const float *__restrict__ gz, Geo G) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;   // FD x, FD y
+    if (a >= G.NX - 8) return;
+    const float *g = gz + 16 * ((size_t)a + (size_t)b * (G.NX - 8));
+    for (int i = 7; i >= 0; i--) hfac(g, U3, U2, U1, G.at(i, a + 4, b + 4), 1);
+    for (int i = G.NZ - 8; i <= G.NZ - 1; i++) hfac(g + 8, U3, U2, U1, G.at(i, a + 4, b + 4), -1);
+}
+// x bands of every (z, y) line with z in the FD rows and y in the FD lines
+__global__ void xband3_kernel(float *__restrict__ U3, const float *__restrict__ U2, const float *__restrict__ U1,
+                              const float *__restrict__ gx, Geo G) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;   // FD z, FD y
+    if (a >= G.NZ - 8) return;
+    const float *g = gx + 16 * ((size_t)a + (size_t)b * (G.NZ - 8));
+    const long s = (long)G.LDZ;
+    for (int j = 7; j >= 0; j--) hfac(g, U3, U2, U1, G.at(a + 4, j, b + 4), s);
+    for (int j = G.NX - 8; j <= G.NX - 1; j++) hfac(g + 8, U3, U2, U1, G.at(a + 4, j, b + 4), -s);
+}
+// y bands of every (z, x) line with z in the FD rows and x in the FD columns
+__global__ void yband3_kernel(float *__restrict__ U3, const float *__restrict__ U2, const float *__restrict__ U1,
+                              const float *__restrict__ gy, Geo G) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;   // FD z, FD x
+    if (a >= G.NZ - 8) return;
+    const float *g = gy + 16 * ((size_t)a + (size_t)b * (G.NZ - 8));
+    const long s = (long)G.plane;
+    for (int k = 7; k >= 0; k--) hfac(g, U3, U2, U1, G.at(a + 4, b + 4, k), s);
+    for (int k = G.NY - 8; k <= G.NY - 1; k++) hfac(g + 8, U3, U2, U1, G.at(a + 4, b + 4, k), -s);
+}
+
+// Receivers (Extraction.f90:71-162,217-246 -> :276-337), one thread each:
+// nearest, or the 7x7x7 sinc summed k, j, i in order with the y weights from
+// the x sinc; gain per receiver (v for SINC_V2 in 3D, 1/(dz dx dy), 1).
+struct Rec3 { int iz, ix, iy; float gain; float wz[7], wx[7]; };
+__global__ void extract3_kernel(const float *__restrict__ u2, Geo G, const Rec3 *__restrict__ R, int ntr,
+                                int nearest, float *__restrict__ traces, int nt, int it) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ntr) return;
+    const Rec3 &e = R[r];
+    float *tr = traces + (size_t)r * nt + (it - 1);
+    if (nearest) {
+        *tr = *tr + u2[G.at(e.iz, e.ix, e.iy)];
+        return;
+    }
+    float acc = *tr;
+    for (int k = -3; k <= 3; k++)
+        for (int j = -3; j <= 3; j++)
+            for (int i = -3; i <= 3; i++)
+                acc = acc + e.gain * u2[G.at(e.iz + i, e.ix + j, e.iy + k)] * e.wz[3 + i] * e.wx[3 + j] * e.wx[3 + k];
+    *tr = acc;
+}
+
+}  // namespace sfd3
+
+using namespace sfd3;
+
+struct sfd_ctx3d {
+    sfd_grid3d g;
+    hipStream_t st = nullptr;
+    std::vector<float> vel_h;
+    float *d_vel = nullptr;
+    std::string err;
+    void *mem = nullptr;
+    size_t cap = 0;
+    ~sfd_ctx3d() {
+        if (mem) (void)hipFree(mem);
+        if (d_vel) (void)hipFree(d_vel);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    char *reserve(size_t n) {
+        if (n > cap) {
+            if (mem) (void)hipFree(mem);
+            mem = nullptr;
+            cap = 0;
+            if (hipMalloc(&mem, n) != hipSuccess) throw EngineError(SFD_ENOMEM, "device allocation failed (3D shot)");
+            cap = n;
+        }
+        return (char *)mem;
+    }
+};
+
+static int nint3(float x) { return (int)roundf(x); }
+static void find_i_d3(int *ic, float *dc, float c, float o, float d) {
+    *ic = nint3((c - o) / d) + 1;
+    *dc = c - (o + (float)(*ic - 1) * d);
+}
+
+// extract_coord_source_receiver_patch (ExtractPadModel.f90:19-85) and
+// nxw / nyw (:183-186)
+static void window3(const sfd_grid3d &g, const sfd_shot3d &s, float *oxw, int *nxw, float *oyw, int *nyw) {
+    float gx1 = s.rx[0], gxn = s.rx[0], gy1 = s.ry[0], gyn = s.ry[0];
+    for (int i = 1; i < s.ntr; i++) {
+        gx1 = std::min(gx1, s.rx[i]);
+        gxn = std::max(gxn, s.rx[i]);
+        gy1 = std::min(gy1, s.ry[i]);
+        gyn = std::max(gyn, s.ry[i]);
+    }
+    const float minx = std::min(s.sx, gx1), miny = std::min(s.sy, gy1);
+    const float maxx = std::max(s.sx, gxn), maxy = std::max(s.sy, gyn);
+    const float mx1 = (gx1 + s.sx) / 2, mxn = (gxn + s.sx) / 2, my1 = (gy1 + s.sy) / 2, myn = (gyn + s.sy) / 2;
+    const float cx = mx1 + fabsf(mxn - mx1) / 2, cy = my1 + fabsf(myn - my1) / 2;
+    *oxw = std::min(minx, cx - g.aperture_x / 2);
+    const float endx = std::max(maxx, cx + g.aperture_x / 2);
+    *oyw = std::min(miny, cy - g.aperture_y / 2);
+    const float endy = std::max(maxy, cy + g.aperture_y / 2);
+    *nxw = nint3((endx - *oxw) / g.dx) + 1;
+    *nyw = nint3((endy - *oyw) / g.dy) + 1;
+}
+
+static void check3(const sfd_grid3d &g) {
+    if (g.nz < 2 || g.nx < 1 || g.ny < 1 || !(g.dz > 0) || !(g.dx > 0) || !(g.dy > 0) || !(g.dt > 0))
+        throw EngineError(SFD_EINVAL, "bad 3D grid dimensions or spacing");
+    if (g.ntaper < 10) throw EngineError(SFD_EINVAL, "ntaper must be >= 10");
+    if (g.lsinc != 7) throw EngineError(SFD_EUNSUPPORTED, "only lsinc=7 is implemented");
+    if (!(g.fmax >= 0.f) || !std::isfinite(g.fmax)) throw EngineError(SFD_EINVAL, "fmax must be >= 0 (0: 30 Hz)");
+}
+
+// ReadSouVelRho.f90:603-752 vel_check in 3D: Courant stop 0.35, sampling stop
+static void vel_check3(const sfd_grid3d &g, const float *v, size_t n) {
+    float vmax = v[0], vmin = v[0];
+    for (size_t i = 1; i < n; i++) {
+        vmax = std::max(vmax, v[i]);
+        vmin = std::min(vmin, v[i]);
+    }
+    const float stab = vmax * g.dt / std::min(g.dz, std::min(g.dx, g.dy));
+    char b[200];
+    if (stab > 0.35f) {
+        snprintf(b, sizeof b, "Courant number %.6g > 0.35: unstable in 3D", stab);
+        throw EngineError(SFD_EUNSTABLE, b);
+    }
+    const float fmax = g.fmax > 0.f ? g.fmax : 30.f;
+    const float samplerate = vmin / fmax / std::max(g.dz, std::max(g.dx, g.dy));
+    if (samplerate < 2.0f) {
+        snprintf(b, sizeof b, "wavefield not adequately sampled: %.6g cells per shortest wavelength", samplerate);
+        throw EngineError(SFD_EALIASED, b);
+    }
+}
+
+static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int kind) {
+    const sfd_grid3d &g = h.g;
+    if (s.nt < 1 || s.ntr < 1 || !s.wavelet || !s.rz || !s.rx || !s.ry)
+        throw EngineError(SFD_EINVAL, "3D shot needs nt >= 1, >= 1 receiver and its arrays");
+    float oxw, oyw;
+    int nxw, nyw;
+    window3(g, s, &oxw, &nxw, &oyw, &nyw);
+    if (nxw < 1 || nyw < 1) throw EngineError(SFD_EINVAL, "empty shot window");
+    Geo G;
+    G.NZ = g.nz + 2 * g.ntaper + 8;
+    G.NX = nxw + 2 * g.ntaper + 8;
+    G.NY = nyw + 2 * g.ntaper + 8;
+    G.LDZ = (G.NZ + 31) / 32 * 32;
+    G.plane = (size_t)G.LDZ * G.NX;
+    const size_t nf = G.plane * G.NY;
+    const int i0 = 4 + g.ntaper;   // padded index of window cell 1 on every axis
+    // window columns / lines -> model columns / lines (mod_window_pad, nearest, clamped)
+    std::vector<int> colsrc(nxw), linesrc(nyw);
+    for (int j = 1; j <= nxw; j++) {
+        int ix = nint3(((float)(j - 1) * g.dx + oxw - g.ox) / g.dx) + 1;
+        colsrc[j - 1] = std::min(std::max(ix, 1), g.nx) - 1;
+    }
+    for (int k = 1; k <= nyw; k++) {
+        int iy = nint3(((float)(k - 1) * g.dy + oyw - g.oy) / g.dy) + 1;
+        linesrc[k - 1] = std::min(std::max(iy, 1), g.ny) - 1;
+    }
+    auto vel_at = [&](int icz, int icx, int icy) {   // padded vel at a window cell (1-based, clamped)
+        const int wz = std::min(std::max(icz, 1), g.nz) - 1, wx = std::min(std::max(icx, 1), nxw) - 1,
+                  wy = std::min(std::max(icy, 1), nyw) - 1;
+        return h.vel_h[(size_t)wz + (size_t)colsrc[wx] * g.nz + (size_t)linesrc[wy] * g.nz * g.nx];
+    };
+    // source (Injection_sinc: v2r = vel^2 at the source cell)
+    Inj3 I;
+    {
+        int ic[3];
+        float dc[3];
+        find_i_d3(&ic[0], &dc[0], s.sz, g.oz, g.dz);
+        find_i_d3(&ic[1], &dc[1], s.sx, oxw, g.dx);
+        find_i_d3(&ic[2], &dc[2], s.sy, oyw, g.dy);
+        I.iz = ic[0] - 1 + i0;
+        I.ix = ic[1] - 1 + i0;
+        I.iy = ic[2] - 1 + i0;
+        if (I.iz - 3 < 4 || I.iz + 3 > G.NZ - 5 || I.ix - 3 < 4 || I.ix + 3 > G.NX - 5 || I.iy - 3 < 4 ||
+            I.iy + 3 > G.NY - 5)
+            throw EngineError(SFD_EINVAL, "source footprint leaves the FD region");
+        const float v = vel_at(ic[0], ic[1], ic[2]);
+        I.v2r = v * v;
+        sfd_host_mksinc(I.wz, 7, dc[0] / g.dz);
+        sfd_host_mksinc(I.wx, 7, dc[1] / g.dx);
+    }
+    // receivers
+    std::vector<Rec3> R(s.ntr);
+    for (int r = 0; r < s.ntr; r++) {
+        int ic[3];
+        float dc[3];
+        find_i_d3(&ic[0], &dc[0], s.rz[r], g.oz, g.dz);
+        find_i_d3(&ic[1], &dc[1], s.rx[r], oxw, g.dx);
+        find_i_d3(&ic[2], &dc[2], s.ry[r], oyw, g.dy);
+        Rec3 &e = R[r];
+        e.iz = ic[0] - 1 + i0;
+        e.ix = ic[1] - 1 + i0;
+        e.iy = ic[2] - 1 + i0;
+        if (e.iz - 3 < 0 || e.iz + 3 >= G.NZ || e.ix - 3 < 0 || e.ix + 3 >= G.NX || e.iy - 3 < 0 ||
+            e.iy + 3 >= G.NY)
+            throw EngineError(SFD_EINVAL, "receiver outside the padded grid");
+        e.gain = kind == SFD_EXTRACT_SINC_V2 ? vel_at(ic[0], ic[1], ic[2])
+                 : kind == SFD_EXTRACT_SINC_LSRTM ? 1.f / (g.dz * g.dx * g.dy)
+                                                  : 1.f;
+        sfd_host_mksinc(e.wz, 7, dc[0] / g.dz);
+        sfd_host_mksinc(e.wx, 7, dc[1] / g.dx);
+    }
+    // device memory: 5 levels, vel, v^2, Higdon tables, window maps, receivers, traces, wavelet
+    const int fz = G.NZ - 8, fx = G.NX - 8, fy = G.NY - 8;
+    Layout L;
+    size_t ou[5];
+    for (int k = 0; k < 5; k++) ou[k] = L.take(nf * 4);
+    const size_t ovel = L.take(nf * 4), ov2 = L.take(nf * 4);
+    const size_t ogz = L.take((size_t)16 * fx * fy * 4), ogx = L.take((size_t)16 * fz * fy * 4),
+                 ogy = L.take((size_t)16 * fz * fx * 4);
+    const size_t ocol = L.take(nxw * 4), olin = L.take(nyw * 4), orec = L.take(sizeof(Rec3) * s.ntr);
+    const size_t otr = L.take((size_t)s.nt * s.ntr * 4), owav = L.take((size_t)s.nt * 4);
+    char *base = h.reserve(L.off);
+    float *u[5];
+    for (int k = 0; k < 5; k++) {
+        u[k] = (float *)(base + ou[k]);
+        HIPCHK(hipMemsetAsync(u[k], 0, nf * 4, h.st));
+    }
+    float *vel = (float *)(base + ovel), *v2 = (float *)(base + ov2);
+    float *gz = (float *)(base + ogz), *gx = (float *)(base + ogx), *gy = (float *)(base + ogy);
+    float *trc = (float *)(base + otr), *wav = (float *)(base + owav);
+    HIPCHK(hipMemsetAsync(trc, 0, (size_t)s.nt * s.ntr * 4, h.st));
+    HIPCHK(hipMemcpyAsync(base + ocol, colsrc.data(), nxw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(base + olin, linesrc.data(), nyw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(base + orec, R.data(), sizeof(Rec3) * s.ntr, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(wav, s.wavelet, (size_t)s.nt * 4, hipMemcpyHostToDevice, h.st));
+    hipLaunchKernelGGL(pad3_kernel, dim3((G.NZ + 255) / 256, G.NX, G.NY), dim3(256), 0, h.st, (const float *)h.d_vel,
+                       g.nz, g.nx, (const int *)(base + ocol), nxw, (const int *)(base + olin), nyw, g.ntaper, G, vel,
+                       v2);
+    const int mx = std::max(fx, fz);
+    hipLaunchKernelGGL(higdon3_kernel, dim3((mx + 255) / 256, std::max(fy, fx), 3), dim3(256), 0, h.st,
+                       (const float *)vel, G, g.dt, g.dz, g.dx, g.dy, gz, gx, gy);
+    Coef3 C;
+    {
+        const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
+        const float dzi = 1.f / g.dz, dxi = 1.f / g.dx, dyi = 1.f / g.dy;
+        for (int k = 0; k < 5; k++) {
+            C.c[k] = cc[k] * (dzi * dzi);
+            C.c[5 + k] = cc[k] * (dxi * dxi);
+            C.c[10 + k] = cc[k] * (dyi * dyi);
+        }
+    }
+    const float c12dt2 = 12.f * (g.dt * g.dt), div11 = 1.f / 11.f;
+    const dim3 sg((fz + 63) / 64, (fx + 3) / 4, fy), sb(256);
+    const int nearest = kind == SFD_EXTRACT_NEAREST;
+    // buf[0] = u_-1, buf[1] = u0, buf[2] = u1, buf[3] = u2, buf[4] = u3 (FDswaptimes.f90:14-24)
+    float *buf[5] = {u[0], u[1], u[2], u[3], u[4]};
+    for (int it = 1; it <= s.nt; it++) {
+        hipLaunchKernelGGL(extract3_kernel, dim3((s.ntr + 63) / 64), dim3(64), 0, h.st, (const float *)buf[3], G,
+                           (const Rec3 *)(base + orec), s.ntr, nearest, trc, s.nt, it);
+        hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)v2, buf[4], G, C);
+        hipLaunchKernelGGL(inject3_kernel, dim3(1), dim3(384), 0, h.st, buf[4], G, I, (const float *)wav, it);
+        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
+                           (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
+        hipLaunchKernelGGL(zband3_kernel, dim3((fx + 63) / 64, fy), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
+                           (const float *)buf[2], (const float *)gz, G);
+        hipLaunchKernelGGL(xband3_kernel, dim3((fz + 63) / 64, fy), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
+                           (const float *)buf[2], (const float *)gx, G);
+        hipLaunchKernelGGL(yband3_kernel, dim3((fz + 63) / 64, fx), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
+                           (const float *)buf[2], (const float *)gy, G);
+        float *t = buf[0];
+        buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(traces_out, trc, (size_t)s.nt * s.ntr * 4, hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipStreamSynchronize(h.st));
+}
+
+static std::string g_create_err3d;   // message of the last failed sfd_create3d
+
+extern "C" {
+
+int sfd_create3d(sfd_handle3d *h, const sfd_grid3d *g) {
+    if (!h || !g) return SFD_EINVAL;
+    *h = nullptr;
+    return abi_call(&g_create_err3d, [&]() {
+        check3(*g);
+        std::unique_ptr<sfd_ctx3d> c(new sfd_ctx3d());
+        c->g = *g;
+        HIPCHK(hipStreamCreate(&c->st));
+        *h = c.release();
+    });
+}
+
+int sfd_destroy3d(sfd_handle3d h) {
+    if (!h) return SFD_EINVAL;
+    delete h;
+    return SFD_OK;
+}
+
+const char *sfd_last_error3d(sfd_handle3d h) { return h ? h->err.c_str() : g_create_err3d.c_str(); }
+
+int sfd_set_model3d(sfd_handle3d h, const float *vel) {
+    if (!h || !vel) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        const size_t n = (size_t)h->g.nz * h->g.nx * h->g.ny;
+        for (size_t i = 0; i < n; i++)
+            if (!(vel[i] > 0.f) || !std::isfinite(vel[i])) throw EngineError(SFD_EINVAL, "velocity must be finite and > 0");
+        vel_check3(h->g, vel, n);
+        h->vel_h.assign(vel, vel + n);
+        if (!h->d_vel) HIPCHK(hipMalloc(&h->d_vel, n * sizeof(float)));
+        HIPCHK(hipMemcpy(h->d_vel, vel, n * sizeof(float), hipMemcpyHostToDevice));
+    });
+}
+
+int sfd_forward3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, float *traces_out, int extract_kind) {
+    if (!h || !shots || nshots < 1 || !traces_out) return SFD_EINVAL;
+    if (extract_kind < 0 || extract_kind > 3) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        if (h->vel_h.empty()) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model3d first");
+        size_t off = 0;
+        for (int s = 0; s < nshots; s++) {
+            forward3(*h, shots[s], traces_out + off, extract_kind);
+            off += (size_t)shots[s].nt * shots[s].ntr;
+        }
+    });
+}
+
+}  // extern "C"

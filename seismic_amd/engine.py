@@ -298,6 +298,80 @@ class Engine:
         return dict(total_ms=ms[0], fwd_ms=ms[1], bwd_ms=ms[2], setup_ms=ms[3], pts=pts[0])
 
 
+def make_grid3d(nz, nx, ny, dz, dx, dy, dt, ntaper=20, lsinc=7, oz=0.0, ox=0.0, oy=0.0, aperture_x=0.0,
+                aperture_y=0.0, fmax=30.0):
+    """sfd_grid3d: model (nz, nx, ny) at (oz, ox, oy) with spacing (dz, dx, dy)."""
+    g = abi.sfd_grid3d()
+    g.nz, g.nx, g.ny = nz, nx, ny
+    g.oz, g.ox, g.oy, g.dz, g.dx, g.dy, g.dt = oz, ox, oy, dz, dx, dy, dt
+    g.ntaper, g.lsinc = ntaper, lsinc
+    g.aperture_x, g.aperture_y, g.fmax = aperture_x, aperture_y, fmax
+    return g
+
+
+class Shot3D:
+    """One 3D shot: source (sz, sx, sy), wavelet[nt], receivers (rz, rx, ry)[ntr]."""
+
+    def __init__(self, sz, sx, sy, wavelet, rz, rx, ry):
+        self.sz, self.sx, self.sy = float(sz), float(sx), float(sy)
+        self.wavelet = _f32c(wavelet)
+        self.rz, self.rx, self.ry = _f32c(rz), _f32c(rx), _f32c(ry)
+        if not (self.rz.size == self.rx.size == self.ry.size):
+            raise ValueError("rz, rx and ry differ in length")
+        self.nt, self.ntr = int(self.wavelet.size), int(self.rz.size)
+
+    def c(self):
+        s = abi.sfd_shot3d()
+        s.sz, s.sx, s.sy, s.nt, s.ntr = self.sz, self.sx, self.sy, self.nt, self.ntr
+        fp = C.POINTER(C.c_float)
+        s.wavelet = self.wavelet.ctypes.data_as(fp)
+        s.rz, s.rx, s.ry = (a.ctypes.data_as(fp) for a in (self.rz, self.rx, self.ry))
+        return s
+
+
+class Engine3D:
+    """One 3D seisfd handle: constant-density forward modeling (genpar%twoD false)."""
+
+    def __init__(self, grid):
+        self.lib = abi.load()
+        self.g = grid
+        h = C.c_void_p()
+        rc = self.lib.sfd_create3d(C.byref(h), C.byref(grid))
+        if rc != 0:
+            raise SeisFDError(rc, self.lib.sfd_last_error3d(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sfd_destroy3d(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SeisFDError(rc, self.lib.sfd_last_error3d(self.h).decode())
+
+    def set_model(self, vel):
+        """vel (nz, nx, ny)."""
+        v = np.asarray(vel, dtype=np.float32)
+        if v.shape != (self.g.nz, self.g.nx, self.g.ny):
+            raise ValueError("model shape %s != (%d, %d, %d)" % (v.shape, self.g.nz, self.g.nx, self.g.ny))
+        self._vel = np.ascontiguousarray(v.transpose(2, 1, 0))
+        self._chk(self.lib.sfd_set_model3d(self.h, _ptr(self._vel)))
+
+    def forward(self, shots, extract_kind=abi.SFD_EXTRACT_SINC_V2):
+        """Traces per shot, (ntr, nt) each."""
+        arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])
+        out = np.zeros(sum(s.nt * s.ntr for s in shots), dtype=np.float32)
+        self._chk(self.lib.sfd_forward3d(self.h, arr, len(shots), _ptr(out), extract_kind))
+        return Engine._split(out, shots)
+
+
 def device_info():
     lib = abi.load()
     name = C.create_string_buffer(256)

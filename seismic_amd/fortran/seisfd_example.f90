@@ -51,6 +51,8 @@ program seisfd_example
   type(sfd_lbfgs_report) :: rep
   type(sfd_fwi_opts) :: fo
   type(sfd_lbfgsoc_opts) :: oco
+  type(sfd_grid3d) :: g3
+  type(sfd_shot3d) :: s3
   type(c_ptr) :: h
   character(len=512) :: outdir
   integer :: i, j, rc, u
@@ -59,8 +61,8 @@ program seisfd_example
   call get_command_argument(1, outdir)
   if (trim(outdir) == '--sizes') then
      ! struct sizes as Fortran sees them (checked against C by the tests)
-     write(*, '(6i6)') c_sizeof(g), c_sizeof(shot(1)), c_sizeof(lo), c_sizeof(rep), c_sizeof(fo), &
-          c_sizeof(oco)
+     write(*, '(8i6)') c_sizeof(g), c_sizeof(shot(1)), c_sizeof(lo), c_sizeof(rep), c_sizeof(fo), &
+          c_sizeof(oco), c_sizeof(g3), c_sizeof(s3)
      stop
   end if
   if (trim(outdir) == '--lbfgsoc') then

@@ -34,6 +34,20 @@ module seisfd_mod
      type(c_ptr)        :: wavelet, rz, rx
   end type sfd_shot
 
+  ! 3D forward modeling (sfd_create3d ...): model vel(nz, nx, ny)
+  type, bind(C) :: sfd_grid3d
+     integer(c_int32_t) :: nz, nx, ny
+     real(c_float)      :: oz, ox, oy, dz, dx, dy, dt
+     integer(c_int32_t) :: ntaper, lsinc
+     real(c_float)      :: aperture_x, aperture_y, fmax
+  end type sfd_grid3d
+
+  type, bind(C) :: sfd_shot3d
+     real(c_float)      :: sz, sx, sy
+     integer(c_int32_t) :: nt, ntr
+     type(c_ptr)        :: wavelet, rz, rx, ry
+  end type sfd_shot3d
+
   type, bind(C) :: sfd_fwi_opts
      real(c_float)      :: illupow
      integer(c_int32_t) :: rect_z, rect_x, do_z, do_x, snap_mode
@@ -122,6 +136,36 @@ module seisfd_mod
        type(c_ptr), value :: h
        type(c_ptr), value :: elev
      end function sfd_set_elevation
+
+     integer(c_int) function sfd_create3d(h, g) bind(C, name='sfd_create3d')
+       import :: c_int, c_ptr, sfd_grid3d
+       type(c_ptr), intent(out) :: h
+       type(sfd_grid3d), intent(in) :: g
+     end function sfd_create3d
+
+     integer(c_int) function sfd_destroy3d(h) bind(C, name='sfd_destroy3d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function sfd_destroy3d
+
+     type(c_ptr) function sfd_last_error3d(h) bind(C, name='sfd_last_error3d')
+       import :: c_ptr
+       type(c_ptr), value :: h
+     end function sfd_last_error3d
+
+     integer(c_int) function sfd_set_model3d(h, vel) bind(C, name='sfd_set_model3d')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: h
+       real(c_float), intent(in) :: vel(*)
+     end function sfd_set_model3d
+
+     integer(c_int) function sfd_forward3d(h, shots, nshots, traces, extract_kind) bind(C, name='sfd_forward3d')
+       import :: c_int, c_ptr, c_float, sfd_shot3d
+       type(c_ptr), value :: h
+       type(sfd_shot3d), intent(in) :: shots(*)
+       integer(c_int), value :: nshots, extract_kind
+       real(c_float), intent(inout) :: traces(*)
+     end function sfd_forward3d
 
      integer(c_int) function sfd_forward(h, shots, nshots, traces, extract_kind) bind(C, name='sfd_forward')
        import :: c_int, c_ptr, c_float, sfd_shot

@@ -33,6 +33,8 @@ def test_struct_sizes():
     assert ctypes.sizeof(abi.sfd_fwi_opts) == 8 * 4
     # int m, nparam; two pointers; int stp1_opt (+4 pad); double; int clip_type (+4 pad)
     assert ctypes.sizeof(abi.sfd_lbfgsoc_opts) == 48
+    assert ctypes.sizeof(abi.sfd_grid3d) == 15 * 4
+    assert ctypes.sizeof(abi.sfd_shot3d) == 12 + 8 + 4 + 4 * 8   # 3 floats, 2 ints, pad, 4 pointers
 
 
 def test_fortran_binding_struct_sizes():
@@ -44,5 +46,5 @@ def test_fortran_binding_struct_sizes():
         pytest.skip("Fortran example not built")
     out = subprocess.run([exe, "--sizes"], capture_output=True, text=True, timeout=60).stdout.split()
     want = [ctypes.sizeof(t) for t in (abi.sfd_grid, abi.sfd_shot, abi.sfd_lbfgs_opts, abi.sfd_lbfgs_report,
-                                  abi.sfd_fwi_opts, abi.sfd_lbfgsoc_opts)]
+                                  abi.sfd_fwi_opts, abi.sfd_lbfgsoc_opts, abi.sfd_grid3d, abi.sfd_shot3d)]
     assert [int(v) for v in out] == want
