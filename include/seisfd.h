@@ -584,6 +584,15 @@ int  sfd_destroy3d(sfd_handle3d h);
 const char *sfd_last_error3d(sfd_handle3d h);
 /* readvel + vel_check in 3D (Courant stop 0.35, ReadSouVelRho.f90:629-675). */
 int  sfd_set_model3d(sfd_handle3d h, const float *vel);
+/* genpar%withRho in 3D (ExtractPadModel.f90:259-270, Acoustic_modeling_sep.f90:
+ * 113-121): rho(nz, nx, ny) as vel, or NULL for constant density.  With a
+ * density, sfd_forward3d windows and pads rho as vel, takes 1/rho2 from the
+ * 3D Interpolate (Interpolate.f90:44-130), runs
+ * FD_3D_derivatives_acoustic_forward_grid (FDderivatives.f90:263-382) with
+ * FD_acoustic_rho_init_coefs over delta, injects with the v^2 rho gain
+ * (Injection.f90:181-187) and shifts each source wavelet half a time step
+ * as readsou does (ReadSouVelRho.f90:328-347). */
+int  sfd_set_density3d(sfd_handle3d h, const float *rho);
 /* Forward modeling of nshots 3D shots, one after another on the device: per
  * shot the window of its patch (extract_coord_source_receiver_patch with the
  * x and y apertures), FD_2nd_3D_derivatives_scalar_forward_grid

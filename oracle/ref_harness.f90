@@ -282,18 +282,22 @@ contains
   ! nbound 4): ExtractData of kind ext (as ref_propagate), the 3D stencil
   ! FD_2nd_3D_derivatives_scalar_forward_grid, Injection_sinc of the sources,
   ! the time update, Boundary_3d.  vel_pad: the padded model with the bounds
-  ! 1-ntaper .. n+ntaper per axis and the 4-cell halos.  OpenMP runs on one
+  ! 1-ntaper .. n+ntaper per axis and the 4-cell halos.  with_rho: withRho,
+  ! rho_pad the padded density (as vel_pad): Interpolate,
+  ! allocate_sxx3d_syy3d_szz3d_delp3d, FD_acoustic_rho_init_coefs with
+  ! coefpower 1 and FD_3D_derivatives_acoustic_forward_grid, as
+  ! Acoustic_modeling_sep.f90:113-121 calls them.  OpenMP runs on one
   ! thread here: the reference's 3D sinc extraction adds to one trace sample
   ! from every thread of its parallel loop (Extraction.f90:309-319), which is
   ! ordered only on one thread.
   subroutine ref_propagate3d(nz, nxw, nyw, ntaper, dz, dx, dy, dt, oz, oxw, oyw, lsinc, vel_pad, &
-       &                     nsou, souz, soux, souy, soutr, ext, ndat, datz, datx, daty, dattr, nt) &
-       &                     bind(C, name='ref_propagate3d')
+       &                     nsou, souz, soux, souy, soutr, ext, ndat, datz, datx, daty, dattr, nt, &
+       &                     with_rho, rho_pad) bind(C, name='ref_propagate3d')
     use omp_lib
-    integer(c_int), value :: nz, nxw, nyw, ntaper, lsinc, nsou, ext, ndat, nt
+    integer(c_int), value :: nz, nxw, nyw, ntaper, lsinc, nsou, ext, ndat, nt, with_rho
     real(c_float), value  :: dz, dx, dy, dt, oz, oxw, oyw
     real(c_float)         :: vel_pad(*), souz(*), soux(*), souy(*), soutr(nt, *), datz(*), datx(*), daty(*)
-    real(c_float)         :: dattr(nt, *)
+    real(c_float)         :: dattr(nt, *), rho_pad(*)
 
     type(GeneralParam)         :: genpar
     type(FDbounds)             :: bounds
@@ -307,7 +311,7 @@ contains
     integer :: i, j, k, it, n1, n2, n3, nthr
 
     genpar%twoD = .false.
-    genpar%withRho = .false.
+    genpar%withRho = with_rho /= 0
     genpar%Born = .false.
     genpar%LSRTM = .false.
     genpar%verbose = .false.
@@ -325,6 +329,7 @@ contains
     genpar%shot_type = 0
     genpar%surf_type = 0
     genpar%coefpower = 2
+    if (with_rho /= 0) genpar%coefpower = 1
     genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
 
     bounds%nmin1 = -ntaper + 1
@@ -351,6 +356,18 @@ contains
           end do
        end do
     end do
+    if (with_rho /= 0) then
+       allocate(model%rho(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, bounds%nmin3-4:bounds%nmax3+4))
+       allocate(model%rho2(bounds%nmin1-4:bounds%nmax1+4, bounds%nmin2-4:bounds%nmax2+4, bounds%nmin3-4:bounds%nmax3+4))
+       do k = 1, n3
+          do j = 1, n2
+             do i = 1, n1
+                model%rho(bounds%nmin1-5+i, bounds%nmin2-5+j, bounds%nmin3-5+k) = rho_pad(i + (j-1)*n1 + (k-1)*n1*n2)
+             end do
+          end do
+       end do
+       call Interpolate(model, bounds, genpar)
+    end if
 
     allocate(sou(max(nsou,1)), dat(max(ndat,1)))
     do i = 1, nsou
@@ -371,7 +388,12 @@ contains
     allocate(hig%gx(16, bounds%nmin1:bounds%nmax1, bounds%nmin3:bounds%nmax3))
     allocate(hig%gy(16, bounds%nmin1:bounds%nmax1, bounds%nmin2:bounds%nmax2))
     allocate(hig%gz(16, bounds%nmin2:bounds%nmax2, bounds%nmin3:bounds%nmax3))
-    call FD_acoustic_init_coefs(fdcoefs)
+    if (with_rho /= 0) then
+       call FD_acoustic_rho_init_coefs(fdcoefs)
+       call allocate_sxx3d_syy3d_szz3d_delp3d(bounds, model)
+    else
+       call FD_acoustic_init_coefs(fdcoefs)
+    end if
     call FD_types_assign_scaled_coefs(fdcoefs, scaled, genpar)
     call FD_derivatives_coef_init(scaled)
     call Higdon(genpar%dt, model, bounds, hig)
@@ -391,7 +413,11 @@ contains
              call Extraction_array_sinc_afwi(bounds, model, dat(1:ndat), grid%u2, genpar, it)
           end select
        end if
-       call FD_2nd_3D_derivatives_scalar_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       if (with_rho /= 0) then
+          call FD_3D_derivatives_acoustic_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       else
+          call FD_2nd_3D_derivatives_scalar_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       end if
        if (nsou > 0) call Injection_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
        call FD_2nd_time_derivative_grid(genpar, bounds, grid)
        call Boundary_3d(genpar, bounds, grid, model, hig)
@@ -404,6 +430,7 @@ contains
     end do
     call deallocateHigdonParam(hig)
     call deallocateUSpace(grid)
+    if (with_rho /= 0) call deallocate_sxx3d_syy3d_szz3d_delp3d()
   end subroutine ref_propagate3d
 
   ! Interpolate.f90:44-130 on a padded density (as vel_pad): rho2 out

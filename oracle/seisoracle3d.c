@@ -109,6 +109,8 @@ typedef struct {
     float *vel;
     float *gz, *gx, *gy;   /* Higdon tables: 16 per (j,k), (i,k), (i,j) of the FD region */
     float c[15];           /* c0z..c4z, c0x..c4x, c0y..c4y */
+    float *rho, *delp;     /* variable density: padded rho and 1/rho2 (NULL: constant density) */
+    float *sz, *sx, *sy;   /* staggered gradients s3dzz, s3dxx, s3dyy (cells never written stay 0) */
 } P3;
 
 #define A3(P, u, i, j, k) (u)[IX3(&(P)->b, i, j, k)]
@@ -163,6 +165,112 @@ static void stencil3(P3 *P, const float *u2, float *u3)
                 float v = A3(P, P->vel, i, j, k);
                 A3(P, u3, i, j, k) = v * v * (tx + tz + ty);
             }
+}
+
+/* FDderivatives.f90:263-382 FD_3D_derivatives_acoustic_forward_grid:
+ * staggered gradients times 1/rho2 on the FD region; first-order gradients
+ * times the boundary's 1/rho2 in the 3 operator-padding layers of each face
+ * (top/bottom for (x, y) in the FD region, left/right for (z, y), front/back
+ * for (z, x)); the outermost layer and the edges and corners are never
+ * written and keep their initial zero; then the staggered divergence times
+ * v^2 rho, summed x, y, z. */
+static void stencil3_rho(P3 *P, const float *u2, float *u3)
+{
+    const B3 *b = &P->b;
+    const float *c = P->c;
+    const float dzi = 1.f / P->dz, dxi = 1.f / P->dx, dyi = 1.f / P->dy;
+    float *sz = P->sz, *sx = P->sx, *sy = P->sy;
+#define U(ii, jj, kk) A3(P, u2, ii, jj, kk)
+#define D(ii, jj, kk) A3(P, P->delp, ii, jj, kk)
+#define FO(i, j, k, dd)                                                                             \
+    do {                                                                                            \
+        A3(P, sz, i, j, k) = dzi * (U(i + 1, j, k) - U(i, j, k)) * (dd);                            \
+        A3(P, sx, i, j, k) = dxi * (U(i, j + 1, k) - U(i, j, k)) * (dd);                            \
+        A3(P, sy, i, j, k) = dyi * (U(i, j, k + 1) - U(i, j, k)) * (dd);                            \
+    } while (0)
+#pragma omp parallel for schedule(static)
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) {
+                A3(P, sz, i, j, k) = (c[1] * (U(i + 1, j, k) - U(i, j, k)) + c[2] * (U(i + 2, j, k) - U(i - 1, j, k)) +
+                                      c[3] * (U(i + 3, j, k) - U(i - 2, j, k)) + c[4] * (U(i + 4, j, k) - U(i - 3, j, k))) *
+                                     D(i, j, k);
+                A3(P, sx, i, j, k) = (c[6] * (U(i, j + 1, k) - U(i, j, k)) + c[7] * (U(i, j + 2, k) - U(i, j - 1, k)) +
+                                      c[8] * (U(i, j + 3, k) - U(i, j - 2, k)) + c[9] * (U(i, j + 4, k) - U(i, j - 3, k))) *
+                                     D(i, j, k);
+                A3(P, sy, i, j, k) = (c[11] * (U(i, j, k + 1) - U(i, j, k)) + c[12] * (U(i, j, k + 2) - U(i, j, k - 1)) +
+                                      c[13] * (U(i, j, k + 3) - U(i, j, k - 2)) + c[14] * (U(i, j, k + 4) - U(i, j, k - 3))) *
+                                     D(i, j, k);
+            }
+    for (int k = b->nmin3; k <= b->nmax3; k++) {
+        for (int j = b->nmin2; j <= b->nmax2; j++) {
+            for (int i = b->nmin1 - 1; i >= b->nmin1 - 3; i--) FO(i, j, k, D(b->nmin1, j, k));
+            for (int i = b->nmax1 + 1; i <= b->nmax1 + 3; i++) FO(i, j, k, D(b->nmax1, j, k));
+        }
+        for (int j = b->nmin2 - 1; j >= b->nmin2 - 3; j--)
+            for (int i = b->nmin1; i <= b->nmax1; i++) FO(i, j, k, D(i, b->nmin2, k));
+        for (int j = b->nmax2 + 1; j <= b->nmax2 + 3; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) FO(i, j, k, D(i, b->nmax2, k));
+    }
+    for (int k = b->nmin3 - 1; k >= b->nmin3 - 3; k--)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) FO(i, j, k, D(i, j, b->nmin3));
+    for (int k = b->nmax3 + 1; k <= b->nmax3 + 3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) FO(i, j, k, D(i, j, b->nmax3));
+#undef FO
+#undef D
+#undef U
+#define S(a, ii, jj, kk) A3(P, a, ii, jj, kk)
+#pragma omp parallel for schedule(static)
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) {
+                float tz = c[1] * (S(sz, i, j, k) - S(sz, i - 1, j, k)) + c[2] * (S(sz, i + 1, j, k) - S(sz, i - 2, j, k)) +
+                           c[3] * (S(sz, i + 2, j, k) - S(sz, i - 3, j, k)) + c[4] * (S(sz, i + 3, j, k) - S(sz, i - 4, j, k));
+                float tx = c[6] * (S(sx, i, j, k) - S(sx, i, j - 1, k)) + c[7] * (S(sx, i, j + 1, k) - S(sx, i, j - 2, k)) +
+                           c[8] * (S(sx, i, j + 2, k) - S(sx, i, j - 3, k)) + c[9] * (S(sx, i, j + 3, k) - S(sx, i, j - 4, k));
+                float ty = c[11] * (S(sy, i, j, k) - S(sy, i, j, k - 1)) + c[12] * (S(sy, i, j, k + 1) - S(sy, i, j, k - 2)) +
+                           c[13] * (S(sy, i, j, k + 2) - S(sy, i, j, k - 3)) + c[14] * (S(sy, i, j, k + 3) - S(sy, i, j, k - 4));
+                float v = A3(P, P->vel, i, j, k);
+                A3(P, u3, i, j, k) = v * v * A3(P, P->rho, i, j, k) * (tx + ty + tz);
+            }
+#undef S
+}
+
+/* Interpolate.f90:44-130 in 3D (nbound 4): rho2 = rho sinc-interpolated half
+ * a cell along z, then x, then y (9-point sinc at shift 0.5, each line
+ * extended by its end values); delp = 1/rho2 (allocate_sxx3d_syy3d_szz3d_delp3d,
+ * FDderivatives.f90:385-395). */
+void orc_rho_half3d(const B3 *b, const float *rho, float *delp)
+{
+    const int L = 9, h = L / 2;
+    float sinc[9];
+    orc_f_mksinc(sinc, L, 0.5f);
+    const int n[3] = {b->NZ, b->NX, b->NY};
+    const size_t st[3] = {1, (size_t)b->NZ, (size_t)b->NZ * b->NX};
+    const size_t ntot = (size_t)b->NZ * b->NX * b->NY;
+    float *cur = (float *)malloc(sizeof(float) * ntot), *t = (float *)malloc(sizeof(float) * (size_t)(b->NZ + b->NX + b->NY + 2 * L));
+    memcpy(cur, rho, sizeof(float) * ntot);
+    for (int ax = 0; ax < 3; ax++) {
+        const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3, m = n[ax];
+        for (int p = 0; p < n[a2]; p++)
+            for (int q = 0; q < n[a1]; q++) {
+                const size_t base = (size_t)p * st[a2] + (size_t)q * st[a1];
+                float *tt = t + L;   /* tt[i] for i in -L .. m-1+L */
+                for (int i = 0; i < m; i++) tt[i] = cur[base + (size_t)i * st[ax]];
+                for (int i = -L; i < 0; i++) tt[i] = tt[0];
+                for (int i = m; i < m + L; i++) tt[i] = tt[m - 1];
+                for (int i = 0; i < m; i++) {
+                    float f = 0;
+                    for (int l = -h; l <= h; l++) f = f + sinc[h + l] * tt[i + l];
+                    cur[base + (size_t)i * st[ax]] = f;
+                }
+            }
+    }
+    for (size_t q = 0; q < ntot; q++) delp[q] = 1.f / cur[q];
+    free(cur);
+    free(t);
 }
 
 static void update3(P3 *P, const float *um, const float *u0, const float *u1, const float *u2, float *u3)
@@ -222,8 +330,16 @@ static void boundary3(P3 *P, const float *u1, const float *u2, float *u3)
     }
 }
 
-/* 3D forward modeling of one shot: traces (nt, ntr) time fastest. */
-int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3d *s, float *traces, int extract_kind)
+void orc_f_half_step(float *tr, int nt, int lsinc);
+
+/* 3D forward modeling of one shot: traces (nt, ntr) time fastest.  rho_full
+ * (nz, nx, ny) or NULL: with a density, read_window_vel's rho (windowed and
+ * padded like vel, then Interpolate), coefficients over delta
+ * (FD_acoustic_rho_init_coefs, coefpower 1), the density stencil, the v^2 rho
+ * injection gain (Injection.f90:181-187) and readsou's half-step source shift
+ * (ReadSouVelRho.f90:328-347). */
+int orc_f_modeling3d_rho(const sfd_grid3d *g, const float *vel_full, const float *rho_full, const sfd_shot3d *s,
+                         float *traces, int extract_kind)
 {
     P3 P;
     memset(&P, 0, sizeof P);
@@ -239,7 +355,28 @@ int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3
     for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
     if (!P.vel || !buf[4]) return -2;
     orc_window_pad3d(g, vel_full, oxw, nxw, oyw, nyw, b, P.vel);
-    {
+    float *wav = (float *)malloc(sizeof(float) * (size_t)s->nt);
+    memcpy(wav, s->wavelet, sizeof(float) * (size_t)s->nt);
+    if (rho_full) {
+        P.rho = (float *)calloc(n, sizeof(float));
+        P.delp = (float *)calloc(n, sizeof(float));
+        P.sz = (float *)calloc(n, sizeof(float));
+        P.sx = (float *)calloc(n, sizeof(float));
+        P.sy = (float *)calloc(n, sizeof(float));
+        if (!P.sy) return -2;
+        orc_window_pad3d(g, rho_full, oxw, nxw, oyw, nyw, b, P.rho);
+        orc_rho_half3d(b, P.rho, P.delp);
+        orc_f_half_step(wav, s->nt, g->lsinc);
+        /* FDcoefs.f90:12-23 over delta per axis (coefpower 1) */
+        const float r[5] = {0.f, 1225.f / 1024.f, -(1225.f / (1024.f * 15.f)), 1225.f / (1024.f * 125.f),
+                            -(1225.f / (1024.f * 1715.f))};
+        const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
+        for (int k = 0; k < 5; k++) {
+            P.c[k] = r[k] * dzi;
+            P.c[5 + k] = r[k] * dxi;
+            P.c[10 + k] = r[k] * dyi;
+        }
+    } else {
         /* FDcoefs.f90:25-36 over delta^2 per axis */
         const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
         const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
@@ -294,10 +431,12 @@ int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3
                         acc = acc + gain * A3(&P, u2, i + ic[0], j + ic[1], k + ic[2]) * wz[h + i] * wx[h + j] * wx[h + k];
             *tr = acc;
         }
-        stencil3(&P, u2, u3);
-        /* Injection_sinc, v2r = vel^2 */
+        if (P.rho) stencil3_rho(&P, u2, u3);
+        else stencil3(&P, u2, u3);
+        /* Injection_sinc, v2r = vel^2 (times rho with a density) */
         {
-            float v = A3(&P, P.vel, sic[0], sic[1], sic[2]), v2r = v * v, tr = s->wavelet[it - 1];
+            float v = A3(&P, P.vel, sic[0], sic[1], sic[2]), v2r = v * v, tr = wav[it - 1];
+            if (P.rho) v2r = v2r * A3(&P, P.rho, sic[0], sic[1], sic[2]);
             for (int k = -h; k <= h; k++)
                 for (int j = -h; j <= h; j++)
                     for (int i = -h; i <= h; i++)
@@ -310,6 +449,12 @@ int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3
         buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
     }
     for (int q = 0; q < 5; q++) free(buf[q]);
-    free(P.vel); free(P.gz); free(P.gx); free(P.gy); free(ric); free(rw);
+    free(P.vel); free(P.gz); free(P.gx); free(P.gy); free(ric); free(rw); free(wav);
+    free(P.rho); free(P.delp); free(P.sz); free(P.sx); free(P.sy);
     return 0;
+}
+
+int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3d *s, float *traces, int extract_kind)
+{
+    return orc_f_modeling3d_rho(g, vel_full, NULL, s, traces, extract_kind);
 }

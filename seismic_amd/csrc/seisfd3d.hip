@@ -37,6 +37,8 @@ struct Geo {
     }
 };
 
+struct Coef3 { float c[15]; };   // c0..c4 per axis: z, x, y
+
 // Boundary.f90:66-87 Higdon_coeff
 __device__ __forceinline__ void higdon_coeff3(float nu, float *c) {
     const float beta1 = 1.0f, beta2 = 0.7f, weight = 0.5f;
@@ -73,7 +75,90 @@ __global__ void pad3_kernel(const float *__restrict__ big, int nz, int nx, const
     const float v = (float)0 + big[(size_t)wz + (size_t)colsrc[wx] * nz + (size_t)linesrc[wy] * nz * nx];
     const size_t q = G.at(iz, ix, iy);
     vel[q] = v;
-    v2[q] = v * v;
+    if (v2) v2[q] = v * v;
+}
+
+// Interpolate.f90:44-130 in 3D, one pass: out = the 9-point sinc (shift 0.5)
+// along one axis of the padded box, each line extended by its end values
+// (clamped indices), summed l = -4..4; the last pass stores 1 / rho2
+// (allocate_sxx3d_syy3d_szz3d_delp3d, FDderivatives.f90:393).
+struct Sinc9 { float w[9]; };
+__global__ void rho_half3_kernel(const float *__restrict__ in, float *__restrict__ out, Geo G, Sinc9 S, int axis,
+                                 int invert) {
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x, ix = blockIdx.y, iy = blockIdx.z;
+    if (iz >= G.NZ) return;
+    const int n = axis == 0 ? G.NZ : axis == 1 ? G.NX : G.NY;
+    const int c = axis == 0 ? iz : axis == 1 ? ix : iy;
+    float f = 0.f;
+    for (int l = -4; l <= 4; l++) {
+        int m = c + l;
+        m = m < 0 ? 0 : (m >= n ? n - 1 : m);
+        const float t = axis == 0 ? in[G.at(m, ix, iy)] : axis == 1 ? in[G.at(iz, m, iy)] : in[G.at(iz, ix, m)];
+        f = f + S.w[4 + l] * t;
+    }
+    out[G.at(iz, ix, iy)] = invert ? 1.f / f : f;
+}
+
+// FDderivatives.f90:263-382, first half: the staggered gradients s3dzz,
+// s3dxx, s3dyy times 1/rho2 on the FD region, and first-order gradients
+// times the face's 1/rho2 in the three operator-padding layers of each face
+// (top/bottom over FD (x, y), left/right over FD (z, y), front/back over FD
+// (z, x)).  The outermost layer, edges and corners are never written (they
+// keep the zero of the shot's start).  One thread per cell.
+__global__ void __launch_bounds__(256) rho_grad3_kernel(const float *__restrict__ u2, const float *__restrict__ delp,
+                                                        float *__restrict__ sz, float *__restrict__ sx,
+                                                        float *__restrict__ sy, Geo G, Coef3 C, float dzi, float dxi,
+                                                        float dyi) {
+    const int iz = blockIdx.x * 64 + (threadIdx.x & 63), ix = blockIdx.y * 4 + (threadIdx.x >> 6),
+              iy = blockIdx.z;
+    if (iz >= G.NZ || ix >= G.NX) return;
+    const bool fz = iz >= 4 && iz <= G.NZ - 5, fx = ix >= 4 && ix <= G.NX - 5, fy = iy >= 4 && iy <= G.NY - 5;
+    const bool pz = (iz >= 1 && iz <= 3) || (iz >= G.NZ - 4 && iz <= G.NZ - 2);
+    const bool px = (ix >= 1 && ix <= 3) || (ix >= G.NX - 4 && ix <= G.NX - 2);
+    const bool py = (iy >= 1 && iy <= 3) || (iy >= G.NY - 4 && iy <= G.NY - 2);
+    const size_t q = G.at(iz, ix, iy), X = G.LDZ, Y = G.plane;
+    const float *c = C.c;
+#define U(o) u2[q + (o)]
+    if (fz && fx && fy) {
+        const float d = delp[q];
+        sz[q] = (c[1] * (U(1) - U(0)) + c[2] * (U(2) - U(-1)) + c[3] * (U(3) - U(-2)) + c[4] * (U(4) - U(-3))) * d;
+        sx[q] = (c[6] * (U(X) - U(0)) + c[7] * (U(2 * X) - U(-(long)X)) + c[8] * (U(3 * X) - U(-2 * (long)X)) +
+                 c[9] * (U(4 * X) - U(-3 * (long)X))) * d;
+        sy[q] = (c[11] * (U(Y) - U(0)) + c[12] * (U(2 * Y) - U(-(long)Y)) + c[13] * (U(3 * Y) - U(-2 * (long)Y)) +
+                 c[14] * (U(4 * Y) - U(-3 * (long)Y))) * d;
+        return;
+    }
+    size_t e;   // the face cell whose 1/rho2 the padding takes
+    if (pz && fx && fy) e = G.at(iz < 4 ? 4 : G.NZ - 5, ix, iy);
+    else if (fz && px && fy) e = G.at(iz, ix < 4 ? 4 : G.NX - 5, iy);
+    else if (fz && fx && py) e = G.at(iz, ix, iy < 4 ? 4 : G.NY - 5);
+    else return;
+    const float d = delp[e];
+    sz[q] = dzi * (U(1) - U(0)) * d;
+    sx[q] = dxi * (U(X) - U(0)) * d;
+    sy[q] = dyi * (U(Y) - U(0)) * d;
+#undef U
+}
+
+// FDderivatives.f90:263-382, second half: u3 = v^2 rho ((tx + ty) + tz) with
+// the staggered divergence of the gradients, on the FD region.
+__global__ void __launch_bounds__(256) stencil3_rho_kernel(const float *__restrict__ sz, const float *__restrict__ sx,
+                                                           const float *__restrict__ sy, const float *__restrict__ v2,
+                                                           const float *__restrict__ rho, float *__restrict__ u3,
+                                                           Geo G, Coef3 C) {
+    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
+    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
+    const size_t q = G.at(iz, ix, iy);
+    const long X = (long)G.LDZ, Y = (long)G.plane;
+    const float *c = C.c;
+    const float tz = c[1] * (sz[q] - sz[q - 1]) + c[2] * (sz[q + 1] - sz[q - 2]) + c[3] * (sz[q + 2] - sz[q - 3]) +
+                     c[4] * (sz[q + 3] - sz[q - 4]);
+    const float tx = c[6] * (sx[q] - sx[q - X]) + c[7] * (sx[q + X] - sx[q - 2 * X]) +
+                     c[8] * (sx[q + 2 * X] - sx[q - 3 * X]) + c[9] * (sx[q + 3 * X] - sx[q - 4 * X]);
+    const float ty = c[11] * (sy[q] - sy[q - Y]) + c[12] * (sy[q + Y] - sy[q - 2 * Y]) +
+                     c[13] * (sy[q + 2 * Y] - sy[q - 3 * Y]) + c[14] * (sy[q + 3 * Y] - sy[q - 4 * Y]);
+    u3[q] = v2[q] * rho[q] * (tx + ty + tz);
 }
 
 // Boundary.f90:36-60 Higdon tables: gz per (x, y) of the FD region from the
@@ -102,7 +187,6 @@ __global__ void higdon3_kernel(const float *__restrict__ vel, Geo G, float dt, f
     }
 }
 
-struct Coef3 { float c[15]; };
 
 // FDderivatives.f90:97-131 on the FD region: u3 = v^2 ((tx + tz) + ty)
 __global__ void __launch_bounds__(256) stencil3_kernel(const float *__restrict__ u2, const float *__restrict__ v2,
@@ -215,14 +299,15 @@ using namespace sfd3;
 struct sfd_ctx3d {
     sfd_grid3d g;
     hipStream_t st = nullptr;
-    std::vector<float> vel_h;
-    float *d_vel = nullptr;
+    std::vector<float> vel_h, rho_h;   // rho_h empty: constant density
+    float *d_vel = nullptr, *d_rho = nullptr;
     std::string err;
     void *mem = nullptr;
     size_t cap = 0;
     ~sfd_ctx3d() {
         if (mem) (void)hipFree(mem);
         if (d_vel) (void)hipFree(d_vel);
+        if (d_rho) (void)hipFree(d_rho);
         if (st) (void)hipStreamDestroy(st);
     }
     char *reserve(size_t n) {
@@ -320,11 +405,14 @@ static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int k
         int iy = nint3(((float)(k - 1) * g.dy + oyw - g.oy) / g.dy) + 1;
         linesrc[k - 1] = std::min(std::max(iy, 1), g.ny) - 1;
     }
-    auto vel_at = [&](int icz, int icx, int icy) {   // padded vel at a window cell (1-based, clamped)
+    const bool rho = !h.rho_h.empty();
+    auto model_at = [&](const std::vector<float> &m, int icz, int icx, int icy) {
+        // padded model value at a window cell (1-based, clamped as model_pad replicates edges)
         const int wz = std::min(std::max(icz, 1), g.nz) - 1, wx = std::min(std::max(icx, 1), nxw) - 1,
                   wy = std::min(std::max(icy, 1), nyw) - 1;
-        return h.vel_h[(size_t)wz + (size_t)colsrc[wx] * g.nz + (size_t)linesrc[wy] * g.nz * g.nx];
+        return m[(size_t)wz + (size_t)colsrc[wx] * g.nz + (size_t)linesrc[wy] * g.nz * g.nx];
     };
+    auto vel_at = [&](int icz, int icx, int icy) { return model_at(h.vel_h, icz, icx, icy); };
     // source (Injection_sinc: v2r = vel^2 at the source cell)
     Inj3 I;
     {
@@ -340,7 +428,8 @@ static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int k
             I.iy + 3 > G.NY - 5)
             throw EngineError(SFD_EINVAL, "source footprint leaves the FD region");
         const float v = vel_at(ic[0], ic[1], ic[2]);
-        I.v2r = v * v;
+        I.v2r = v * v;   // Injection.f90:181-187: vel^2, times rho with a density
+        if (rho) I.v2r = I.v2r * model_at(h.rho_h, ic[0], ic[1], ic[2]);
         sfd_host_mksinc(I.wz, 7, dc[0] / g.dz);
         sfd_host_mksinc(I.wx, 7, dc[1] / g.dx);
     }
@@ -375,6 +464,13 @@ static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int k
                  ogy = L.take((size_t)16 * fz * fx * 4);
     const size_t ocol = L.take(nxw * 4), olin = L.take(nyw * 4), orec = L.take(sizeof(Rec3) * s.ntr);
     const size_t otr = L.take((size_t)s.nt * s.ntr * 4), owav = L.take((size_t)s.nt * 4);
+    size_t orho = 0, odelp = 0, otmp = 0, osg[3] = {0, 0, 0};
+    if (rho) {
+        orho = L.take(nf * 4);
+        odelp = L.take(nf * 4);
+        otmp = L.take(nf * 4);
+        for (int k = 0; k < 3; k++) osg[k] = L.take(nf * 4);
+    }
     char *base = h.reserve(L.off);
     float *u[5];
     for (int k = 0; k < 5; k++) {
@@ -388,17 +484,60 @@ static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int k
     HIPCHK(hipMemcpyAsync(base + ocol, colsrc.data(), nxw * 4, hipMemcpyHostToDevice, h.st));
     HIPCHK(hipMemcpyAsync(base + olin, linesrc.data(), nyw * 4, hipMemcpyHostToDevice, h.st));
     HIPCHK(hipMemcpyAsync(base + orec, R.data(), sizeof(Rec3) * s.ntr, hipMemcpyHostToDevice, h.st));
-    HIPCHK(hipMemcpyAsync(wav, s.wavelet, (size_t)s.nt * 4, hipMemcpyHostToDevice, h.st));
-    hipLaunchKernelGGL(pad3_kernel, dim3((G.NZ + 255) / 256, G.NX, G.NY), dim3(256), 0, h.st, (const float *)h.d_vel,
-                       g.nz, g.nx, (const int *)(base + ocol), nxw, (const int *)(base + olin), nyw, g.ntaper, G, vel,
-                       v2);
+    // readsou with withRho (ReadSouVelRho.f90:328-347): the source shifted half a time step
+    std::vector<float> wshift;
+    if (rho) {
+        wshift.assign(s.nt, 0.f);
+        float w[64];
+        const int hl = g.lsinc / 2;
+        sfd_host_mksinc(w, g.lsinc, 0.5f);
+        for (int it = 1; it <= s.nt; it++) {
+            float fscale = 1.f, fdum = 0.f;
+            for (int l = -hl; l <= hl; l++) {
+                if (it + l >= 1 && it + l <= s.nt) fdum = fdum + w[hl + l] * s.wavelet[it + l - 1];
+                else fscale = fscale - w[hl + l];
+            }
+            wshift[it - 1] = fdum / fscale;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(wav, rho ? wshift.data() : s.wavelet, (size_t)s.nt * 4, hipMemcpyHostToDevice, h.st));
+    const dim3 pg((G.NZ + 255) / 256, G.NX, G.NY);
+    hipLaunchKernelGGL(pad3_kernel, pg, dim3(256), 0, h.st, (const float *)h.d_vel, g.nz, g.nx,
+                       (const int *)(base + ocol), nxw, (const int *)(base + olin), nyw, g.ntaper, G, vel, v2);
+    float *rhop = nullptr, *delp = nullptr, *sgz = nullptr, *sgx = nullptr, *sgy = nullptr;
+    if (rho) {
+        rhop = (float *)(base + orho);
+        delp = (float *)(base + odelp);
+        float *tmp = (float *)(base + otmp);
+        sgz = (float *)(base + osg[0]);
+        sgx = (float *)(base + osg[1]);
+        sgy = (float *)(base + osg[2]);
+        for (int k = 0; k < 3; k++) HIPCHK(hipMemsetAsync(base + osg[k], 0, nf * 4, h.st));
+        hipLaunchKernelGGL(pad3_kernel, pg, dim3(256), 0, h.st, (const float *)h.d_rho, g.nz, g.nx,
+                           (const int *)(base + ocol), nxw, (const int *)(base + olin), nyw, g.ntaper, G, rhop,
+                           (float *)nullptr);
+        Sinc9 S9;
+        sfd_host_mksinc(S9.w, 9, 0.5f);
+        // Interpolate: z pass rho -> delp, x pass delp -> tmp, y pass tmp -> delp (stored as 1 / rho2)
+        hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)rhop, delp, G, S9, 0, 0);
+        hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)delp, tmp, G, S9, 1, 0);
+        hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)tmp, delp, G, S9, 2, 1);
+    }
     const int mx = std::max(fx, fz);
     hipLaunchKernelGGL(higdon3_kernel, dim3((mx + 255) / 256, std::max(fy, fx), 3), dim3(256), 0, h.st,
                        (const float *)vel, G, g.dt, g.dz, g.dx, g.dy, gz, gx, gy);
     Coef3 C;
-    {
+    const float dzi = 1.f / g.dz, dxi = 1.f / g.dx, dyi = 1.f / g.dy;
+    if (rho) {   // FDcoefs.f90:12-23 over delta (coefpower 1)
+        const float r[5] = {0.f, 1225.f / 1024.f, -(1225.f / (1024.f * 15.f)), 1225.f / (1024.f * 125.f),
+                            -(1225.f / (1024.f * 1715.f))};
+        for (int k = 0; k < 5; k++) {
+            C.c[k] = r[k] * dzi;
+            C.c[5 + k] = r[k] * dxi;
+            C.c[10 + k] = r[k] * dyi;
+        }
+    } else {   // FDcoefs.f90:25-36 over delta^2
         const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
-        const float dzi = 1.f / g.dz, dxi = 1.f / g.dx, dyi = 1.f / g.dy;
         for (int k = 0; k < 5; k++) {
             C.c[k] = cc[k] * (dzi * dzi);
             C.c[5 + k] = cc[k] * (dxi * dxi);
@@ -413,7 +552,15 @@ static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int k
     for (int it = 1; it <= s.nt; it++) {
         hipLaunchKernelGGL(extract3_kernel, dim3((s.ntr + 63) / 64), dim3(64), 0, h.st, (const float *)buf[3], G,
                            (const Rec3 *)(base + orec), s.ntr, nearest, trc, s.nt, it);
-        hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)v2, buf[4], G, C);
+        if (rho) {
+            hipLaunchKernelGGL(rho_grad3_kernel, dim3((G.NZ + 63) / 64, (G.NX + 3) / 4, G.NY), sb, 0, h.st,
+                               (const float *)buf[3], (const float *)delp, sgz, sgx, sgy, G, C, dzi, dxi, dyi);
+            hipLaunchKernelGGL(stencil3_rho_kernel, sg, sb, 0, h.st, (const float *)sgz, (const float *)sgx,
+                               (const float *)sgy, (const float *)v2, (const float *)rhop, buf[4], G, C);
+        } else {
+            hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)v2, buf[4], G,
+                               C);
+        }
         hipLaunchKernelGGL(inject3_kernel, dim3(1), dim3(384), 0, h.st, buf[4], G, I, (const float *)wav, it);
         hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
                            (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
@@ -465,6 +612,22 @@ int sfd_set_model3d(sfd_handle3d h, const float *vel) {
         h->vel_h.assign(vel, vel + n);
         if (!h->d_vel) HIPCHK(hipMalloc(&h->d_vel, n * sizeof(float)));
         HIPCHK(hipMemcpy(h->d_vel, vel, n * sizeof(float), hipMemcpyHostToDevice));
+    });
+}
+
+int sfd_set_density3d(sfd_handle3d h, const float *rho) {
+    if (!h) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        if (!rho) {
+            h->rho_h.clear();
+            return;
+        }
+        const size_t n = (size_t)h->g.nz * h->g.nx * h->g.ny;
+        for (size_t i = 0; i < n; i++)
+            if (!(rho[i] > 0.f) || !std::isfinite(rho[i])) throw EngineError(SFD_EINVAL, "density must be finite and > 0");
+        if (!h->d_rho) HIPCHK(hipMalloc(&h->d_rho, n * sizeof(float)));
+        HIPCHK(hipMemcpy(h->d_rho, rho, n * sizeof(float), hipMemcpyHostToDevice));
+        h->rho_h.assign(rho, rho + n);
     });
 }
 

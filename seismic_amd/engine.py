@@ -364,6 +364,17 @@ class Engine3D:
         self._vel = np.ascontiguousarray(v.transpose(2, 1, 0))
         self._chk(self.lib.sfd_set_model3d(self.h, _ptr(self._vel)))
 
+    def set_density(self, rho=None):
+        """Variable density rho (nz, nx, ny) (genpar%withRho); None: constant density."""
+        if rho is None:
+            self._rho = None
+        else:
+            r = np.asarray(rho, dtype=np.float32)
+            if r.shape != (self.g.nz, self.g.nx, self.g.ny):
+                raise ValueError("density shape %s != (%d, %d, %d)" % (r.shape, self.g.nz, self.g.nx, self.g.ny))
+            self._rho = np.ascontiguousarray(r.transpose(2, 1, 0))
+        self._chk(self.lib.sfd_set_density3d(self.h, _ptr(self._rho)))
+
     def forward(self, shots, extract_kind=abi.SFD_EXTRACT_SINC_V2):
         """Traces per shot, (ntr, nt) each."""
         arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])

@@ -159,6 +159,12 @@ module seisfd_mod
        real(c_float), intent(in) :: vel(*)
      end function sfd_set_model3d
 
+     integer(c_int) function sfd_set_density3d(h, rho) bind(C, name='sfd_set_density3d')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       type(c_ptr), value :: rho     ! c_loc of rho(nz, nx, ny), or c_null_ptr
+     end function sfd_set_density3d
+
      integer(c_int) function sfd_forward3d(h, shots, nshots, traces, extract_kind) bind(C, name='sfd_forward3d')
        import :: c_int, c_ptr, c_float, sfd_shot3d
        type(c_ptr), value :: h

@@ -726,27 +726,50 @@ def padded_vel3d(g, vel, shot):
     return out, (oxw, nxw, oyw, nyw), b
 
 
-def modeling3d(g, vel, shot, extract_kind=0):
+def modeling3d(g, vel, shot, extract_kind=0, rho=None):
+    """rho (nz, nx, ny) or None: variable density (withRho)."""
     L = oracle()
     tr = np.zeros((shot.ntr, shot.nt), np.float32)
     s = shot.c()
-    L.orc_f_modeling3d.argtypes = [C.c_void_p] * 4 + [C.c_int]
-    rc = L.orc_f_modeling3d(C.byref(g), ptr(model3d_buf(vel)), C.byref(s), ptr(tr), extract_kind)
+    L.orc_f_modeling3d_rho.argtypes = [C.c_void_p] * 5 + [C.c_int]
+    rb = None if rho is None else model3d_buf(rho)
+    rc = L.orc_f_modeling3d_rho(C.byref(g), ptr(model3d_buf(vel)), None if rb is None else ptr(rb), C.byref(s),
+                                ptr(tr), extract_kind)
     assert rc == 0
     return tr
 
 
-def ref_propagate3d(g, vel_pad, win, shot, ext):
+def rho_half3d(g, rho_pad, b):
+    """orc_rho_half3d: 1 / (rho sinc-interpolated half a cell along z, x, y)."""
+    L = oracle()
+    out = np.zeros_like(rho_pad)
+    L.orc_rho_half3d.argtypes = [C.c_void_p] * 3
+    L.orc_rho_half3d(C.byref(b), ptr(rho_pad), ptr(out))
+    return out
+
+
+def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None):
+    """rho_pad: padded density (withRho); the wavelet is passed as given
+    (readsou's half-step shift is the caller's, as in the drivers)."""
     oxw, nxw, oyw, nyw = win
     fn = ref().ref_propagate3d
     fn.argtypes = ([C.c_int] * 4 + [C.c_float] * 7 + [C.c_int] + [C.c_void_p] + [C.c_int] + [C.c_void_p] * 4
-                   + [C.c_int] * 2 + [C.c_void_p] * 4 + [C.c_int])
+                   + [C.c_int] * 2 + [C.c_void_p] * 4 + [C.c_int] * 2 + [C.c_void_p])
     sz = np.array([shot.sz], np.float32)
     sx = np.array([shot.sx], np.float32)
     sy = np.array([shot.sy], np.float32)
-    st = np.ascontiguousarray(shot.wavelet[None, :], np.float32)
+    w = shot.wavelet if wavelet is None else wavelet
+    st = np.ascontiguousarray(np.asarray(w, np.float32)[None, :])
     dtr = np.zeros((shot.ntr, shot.nt), np.float32)
-    fn(g.nz, nxw, nyw, g.ntaper, g.dz, g.dx, g.dy, g.dt, g.oz, oxw, oyw, g.lsinc, ptr(vel_pad),
-       1, ptr(sz), ptr(sx), ptr(sy), ptr(st), ext, shot.ntr, ptr(shot.rz), ptr(shot.rx), ptr(shot.ry), ptr(dtr),
-       shot.nt)
+    rp = np.zeros(1, np.float32) if rho_pad is None else np.ascontiguousarray(rho_pad, np.float32)
+    libc = C.CDLL(None)
+    if rho_pad is not None:
+        libc.mallopt(-6, 255)   # M_PERTURB: the never-written gradient cells start at zero, as in the oracle
+    try:
+        fn(g.nz, nxw, nyw, g.ntaper, g.dz, g.dx, g.dy, g.dt, g.oz, oxw, oyw, g.lsinc, ptr(vel_pad),
+           1, ptr(sz), ptr(sx), ptr(sy), ptr(st), ext, shot.ntr, ptr(shot.rz), ptr(shot.rx), ptr(shot.ry), ptr(dtr),
+           shot.nt, 0 if rho_pad is None else 1, ptr(rp))
+    finally:
+        if rho_pad is not None:
+            libc.mallopt(-6, 0)
     return dtr

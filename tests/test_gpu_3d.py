@@ -120,3 +120,69 @@ def test_vel_check3d_and_errors():
         e.forward([shot_pair(-150.0, 121.2, 99.9, ricker(30.0, 20, 1e-3), RZ, RX, RY)[0]])
     with pytest.raises(sa.SeisFDError):
         e.forward([shot_pair(44.4, 121.2, 99.9, ricker(30.0, 20, 1e-3), RZ, RX, RY)[0]], extract_kind=7)
+
+
+def rho3d(nz, nx, ny, seed=11):
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[:, None, None] / nz
+    x = np.arange(nx)[None, :, None] / nx
+    y = np.arange(ny)[None, None, :] / ny
+    r = 1.8 + 0.5 * z + 0.3 * np.sin(2 * np.pi * (y + rng.uniform())) + 0.2 * np.cos(2 * np.pi * (x + rng.uniform()))
+    return r.astype(np.float32)
+
+
+@pytest.mark.parametrize("ext", [0, 1, 2, 3])
+def test_forward3d_density_bitwise(ext):
+    """Variable density (withRho) in 3D: 3D Interpolate of rho, the staggered
+    density stencil with its padding rules, the v^2 rho gain and the
+    half-step source shift, against the oracle (pinned bitwise to the
+    compiled FD_3D_derivatives_acoustic_forward_grid)."""
+    say("3d density ext", ext)
+    gg, og, vel = case(nt=140)
+    rho = rho3d(og.nz, og.nx, og.ny)
+    w = ricker(30.0, 140, 1e-3)
+    gs, os_ = shot_pair(44.4, 121.2, 99.9, w, RZ, RX, RY)
+    e = sa.Engine3D(gg)
+    e.set_model(vel)
+    e.set_density(rho)
+    got = e.forward([gs], extract_kind=ext)[0]
+    want = O.modeling3d(og, vel, os_, extract_kind=ext, rho=rho)
+    assert np.abs(want).max() > 0
+    assert np.array_equal(got, want), rel(got, want)
+    e.set_density(None)   # back to constant density on the same handle
+    got0 = e.forward([gs], extract_kind=ext)[0]
+    assert np.array_equal(got0, O.modeling3d(og, vel, os_, extract_kind=ext))
+
+
+def test_forward3d_density_several_shots():
+    nz, nx, ny = 22, 28, 20
+    gg = sa.make_grid3d(nz, nx, ny, 8.0, 10.0, 12.0, 8e-4, ntaper=10, aperture_x=30.0, aperture_y=20.0, fmax=0.0)
+    og = O.make_grid3d(nz, nx, ny, 8.0, 10.0, 12.0, 8e-4, ntaper=10, apx=30.0, apy=20.0)
+    vel = model3d(nz, nx, ny, seed=4)
+    rho = rho3d(nz, nx, ny, seed=5)
+    e = sa.Engine3D(gg)
+    e.set_model(vel)
+    e.set_density(rho)
+    rng = np.random.default_rng(9)
+    pairs = []
+    for k, (nt, ntr) in enumerate([(100, 5), (130, 9)]):
+        w = ricker(25.0 + 5 * k, nt, 8e-4)
+        pairs.append(shot_pair(rng.uniform(40, 110), rng.uniform(60, 210), rng.uniform(50, 160), w,
+                               rng.uniform(10.0, 140.0, ntr), rng.uniform(20.0, 260.0, ntr),
+                               rng.uniform(15.0, 200.0, ntr)))
+    got = e.forward([p[0] for p in pairs], extract_kind=0)
+    for k, (_, o1) in enumerate(pairs):
+        want = O.modeling3d(og, vel, o1, extract_kind=0, rho=rho)
+        assert np.array_equal(got[k], want), (k, rel(got[k], want))
+
+
+def test_density3d_errors():
+    gg, og, vel = case()
+    e = sa.Engine3D(gg)
+    bad = np.ones((og.nz, og.nx, og.ny), np.float32)
+    bad[3, 4, 5] = 0.0
+    with pytest.raises(sa.SeisFDError) as ei:
+        e.set_density(bad)
+    assert ei.value.code == -1
+    with pytest.raises(ValueError):
+        e.set_density(np.ones((og.nz, og.nx), np.float32))
