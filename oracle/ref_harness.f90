@@ -466,4 +466,19 @@ contains
     deallocate(model%rho, model%rho2)
   end subroutine ref_interpolate_rho
 
+  ! The reference's integer powers of reals as this compiler lowers them
+  ! (Imaging.f90:365,470-472: vel**3, rho**2, vel**2): out(:,i) = x**3, x**2,
+  ! x**3*y, y**2*x**2
+  subroutine ref_powers(n, x, y, out) bind(C, name='ref_powers')
+    integer(c_int), value :: n
+    real(c_float)         :: x(n), y(n), out(4, n)
+    integer :: i
+    do i = 1, n
+       out(1, i) = x(i)**3
+       out(2, i) = x(i)**2
+       out(3, i) = x(i)**3*y(i)
+       out(4, i) = y(i)**2*x(i)**2
+    end do
+  end subroutine ref_powers
+
 end module ref_harness

@@ -115,8 +115,9 @@ typedef struct {
     int extract;             /* -1 none, else SFD_EXTRACT_*              */
     int ndat; TR *dat;
     REAL *wave; int wcounter;          /* snapshots nz*nxw*ntsnap, NULL = none */
-    int imaging;             /* 0 none, 1 RTM, 2 FWI, 3 LSRTM, 4 Born injection */
+    int imaging;             /* 0 none, 1 RTM, 2 FWI, 3 LSRTM, 4 Born injection, 5 FWI for vp and rho */
     REAL *image, *illum;     /* nz*nxw window accumulators               */
+    REAL *image2;            /* imaging 5: the rho gradient (imagesmall_nparam(:,:,:,2)) */
     const REAL *born_m;      /* nz*nxw reflectivity (Born)               */
     const REAL *snaps;       /* forward snapshots read by the imaging   */
     REAL *taperx;
@@ -591,6 +592,23 @@ static void CAT(imaging_, SFX)(PROP *P, REAL *u, int it)
             REAL uu = AT(P, u, i, j);
             if (P->imaging == 1) {
                 P->image[q] = P->image[q] + uu * S[q] * t;
+            } else if (P->imaging == 5) {
+                /* Imaging.f90:424-512 Imaging_condition_AFWI_RHOVP_from_memory in
+                 * 2D (dk = 0): one-sided gradients of u and of the snapshot,
+                 * ip = min(nz, i+1), jp = min(j+1, nxw), im = ip-1, jm = jp-1;
+                 * the y term is u and S minus themselves, a zero */
+                const REAL dzi = (REAL)1. / P->dz, dxi = (REAL)1. / P->dx, dti2 = (REAL)1. / P->dt2snap;
+                const int ip = i + 1 < P->nz ? i + 1 : P->nz, jp = j + 1 < P->nxw ? j + 1 : P->nxw;
+                const int im = ip - 1, jm = jp - 1;
+#define WS(a, b) S[((a) - 1) + (size_t)((b) - 1) * P->nz]
+                REAL tmp = dzi * (AT(P, u, ip, jm) - AT(P, u, im, jm)) * dzi * (WS(ip, jm) - WS(im, jm));
+                tmp = tmp + dxi * (AT(P, u, im, jp) - AT(P, u, im, jm)) * dxi * (WS(im, jp) - WS(im, jm));
+                tmp = tmp + (REAL)0;
+#undef WS
+                const REAL tmp1 = uu * dti2 * (Sm[q] - (REAL)2 * S[q] + Sp[q]);
+                const REAL v = AT(P, P->vel, i, j), r = AT(P, P->rho, i, j);
+                P->image[q] = P->image[q] + (REAL)2 * tmp1 * t / (v * v * v * r);
+                P->image2[q] = P->image2[q] + tmp * t / (r * r) + tmp1 * t / (r * r * (v * v));
             } else if (P->imaging == 2) {
                 REAL v = AT(P, P->vel, i, j);
                 REAL tmp = (Sm[q] - (REAL)2 * S[q] + Sp[q]) / (P->dt2snap * (v * v * v));
@@ -1278,6 +1296,90 @@ void FN(gdt_compute)(int ntype, REAL *rd, int nd, REAL t)
  * concatenated; mute same layout (NULL = 1); vpmask (nz*nx, NULL = 1).
  * Shots may differ in nt (ragged batch; the misfit scaling sums each
  * shot's 2 nt ntr).  grad/illu_out are (nz,nx). */
+/* progs/afwi/3DAFWI.f90 one-shot gradient, AFWI_to_memory
+ * (to_disk_to_memory_AFWI.f90:229-415), 2D, per shot: forward modeling with
+ * Injection_sinc (readsou's half-step shift with a density) and
+ * Extraction_array_sinc_afwi (gain 1) storing snapshots; Compute_OF_RES_3D
+ * (Objective_Function_Residual_Adjsource.f90:42-67: one THRESH, in trace
+ * order; the misfit starts at 0 where the driver leaves it unset); reverse
+ * time with the adjoint-style stencil (the density operator with a density),
+ * Injection_sinc of the residual and Imaging_condition_AFWI_from_memory, or
+ * Imaging_condition_AFWI_RHOVP_from_memory with a density; then
+ * mod_copy_gradient_to_disk (ExtractPadModel.f90:592-660): each window added
+ * into the model, gradients summed over the shots, and with a density and
+ * vprho 1 the Vp/Ip change of variables on the model's vel and rho.  grad:
+ * (nz,nx) for vp, then (nz,nx) for rho (or Ip) with a density. */
+int FN(afwi)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_shot *shots, const REAL *wavelet,
+             const REAL *dobs, const REAL *mute, int norm, REAL norm_thresh, int vprho, double *fout, REAL *grad,
+             REAL *illum)
+{
+    const size_t N = (size_t)g->nz * g->nx;
+    const int np = orc_rho && orc_rho_n == N ? 2 : 1;
+    size_t off = 0;
+    double f = 0;
+    for (size_t q = 0; q < N * np; q++) grad[q] = 0;
+    for (size_t q = 0; q < N; q++) illum[q] = 0;
+    for (int is = 0; is < nshots; is++) {
+        const sfd_shot *s = &shots[is];
+        PROP P;
+        float zz = s->sz, xx = s->sx;
+        if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
+        if (P.with_rho && P.nxw < 2) return -1;   /* the one-sided x gradient needs two window columns */
+        size_t n = (size_t)P.nz * P.nxw;
+        REAL *snaps = (REAL *)calloc(n * (P.ntsnap > 0 ? P.ntsnap : 1), sizeof(REAL));
+        REAL *img = (REAL *)calloc(n, sizeof(REAL)), *img2 = (REAL *)calloc(n, sizeof(REAL));
+        REAL *ill = (REAL *)calloc(n, sizeof(REAL));
+        REAL *tap = (REAL *)malloc(sizeof(REAL) * P.nxw);
+        if (!snaps || !img || !img2 || !ill || !tap) return -2;
+        P.stencil = 0; P.inject = 1; P.nsou = 1;
+        P.sou = CAT(make_traces_, SFX)(1, &zz, &xx, s->nt);
+        for (int it = 0; it < s->nt; it++) P.sou[0].trace[it] = wavelet[it];
+        if (P.with_rho) CAT(half_step_, SFX)(P.sou[0].trace, s->nt, P.lsinc);
+        P.extract = SFD_EXTRACT_SINC_UNIT;
+        P.ndat = s->ntr;
+        P.dat = CAT(make_traces_, SFX)(s->ntr, s->rz, s->rx, s->nt);
+        P.wave = snaps;
+        P.tmin = 1; P.tmax = s->nt; P.tstep = 1;
+        CAT(propagate_, SFX)(&P);
+        CAT(free_traces_, SFX)(P.sou, 1);
+        REAL thresh = norm_thresh;
+        for (int r = 0; r < s->ntr; r++) {
+            REAL *d = P.dat[r].trace;
+            const REAL *ob = dobs + off + (size_t)r * s->nt;
+            const REAL *mu = mute ? mute + off + (size_t)r * s->nt : NULL;
+            for (int it = 0; it < s->nt; it++) d[it] = (mu ? mu[it] : (REAL)1) * (ob[it] - d[it]);
+            f = f + FN(fct_compute)(norm, d, s->nt, &thresh);
+            for (int it = 0; it < s->nt; it++) d[it] = (mu ? mu[it] : (REAL)1) * d[it];
+            FN(gdt_compute)(norm, d, s->nt, thresh);
+        }
+        FN(compute_taper)(tap, P.nxw);
+        P.wave = NULL; P.extract = -1;
+        P.stencil = 1; P.inject = 1;
+        P.nsou = s->ntr; P.sou = P.dat; P.ndat = 0; P.dat = NULL;
+        P.imaging = P.with_rho ? 5 : 2;
+        P.image = img; P.image2 = img2; P.illum = ill; P.snaps = snaps; P.taperx = tap;
+        P.dt2snap = (P.dt * (REAL)P.snapi) * (P.dt * (REAL)P.snapi);
+        P.icounter = 0;
+        P.tmin = s->nt; P.tmax = 1; P.tstep = -1;
+        CAT(propagate_, SFX)(&P);
+        FN(copy_image)(g, P.oxw, P.nxw, img, grad);
+        if (np == 2) FN(copy_image)(g, P.oxw, P.nxw, img2, grad + N);
+        FN(copy_image)(g, P.oxw, P.nxw, ill, illum);
+        CAT(free_traces_, SFX)(P.sou, s->ntr);
+        free(snaps); free(img); free(img2); free(ill); free(tap);
+        free(P.vel); free(P.ielev); free(P.delev); free(P.rho); free(P.delp);
+        off += (size_t)s->nt * s->ntr;
+    }
+    if (np == 2 && vprho == 1)   /* ExtractPadModel.f90:643-646 on the model's vel and rho */
+        for (size_t q = 0; q < N; q++) {
+            const REAL v = vel[q], r = (REAL)orc_rho[q];
+            grad[q] = grad[q] - (r / v) * grad[N + q];
+            grad[N + q] = ((REAL)1 / v) * grad[N + q];
+        }
+    *fout = f;
+    return 0;
+}
+
 int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_shot *shots,
                      const REAL *wavelet, const REAL *dobs, const REAL *mute, const REAL *vpmask,
                      const sfd_fwi_opts *o, double *fout, REAL *grad, REAL *illu_out, REAL *resid)

@@ -263,6 +263,31 @@ def fwi_gradient(g, vel, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_
                 resid=res)
 
 
+def afwi(g, vel, shots, dobs, mute=None, norm=2, norm_thresh=0.0, vprho=0, prec="f"):
+    """3DAFWI.x's gradient (AFWI_to_memory + mod_copy_gradient_to_disk) in 2D,
+    summed over the shots; with the oracle's density (set_density) two
+    gradients (vp, then rho or Ip).  Returns f, grad [nparam, nz, nx], illum."""
+    L = oracle()
+    T = _dt(prec)
+    v = colmajor(vel, T)
+    cs = (sfd_shot * len(shots))(*[s.c() for s in shots])
+    ob = np.ascontiguousarray(np.concatenate([np.asarray(d, dtype=T).ravel() for d in dobs]))
+    mu = None if mute is None else np.ascontiguousarray(
+        np.concatenate([np.asarray(m, dtype=T).ravel() for m in mute]))
+    f = C.c_double()
+    grad = np.zeros(2 * g.nz * g.nx, dtype=T)
+    illu = np.zeros(g.nz * g.nx, dtype=T)
+    w = np.ascontiguousarray(shots[0].wavelet, dtype=T)
+    fn = getattr(L, "orc_%s_afwi" % prec)
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                   C.c_double if prec == "d" else C.c_float, C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+    rc = fn(C.byref(g), ptr(v), len(shots), cs, ptr(w), ptr(ob), None if mu is None else ptr(mu), norm,
+            norm_thresh, vprho, C.byref(f), ptr(grad), ptr(illu))
+    assert rc == 0, rc
+    gr = grad.reshape(2, g.nx, g.nz).transpose(0, 2, 1)
+    return dict(f=f.value, grad=gr, illum=illu.reshape(g.nx, g.nz).T)
+
+
 def mute_mask(dt, shots, ot=0.0, v0=1500.0, t0=0.0, v1=10000.0, t1=40.0, tramp=0.5):
     """Mute.f90 MuteParam_compute_mask restatement: one (ntr, nt) mask per shot."""
     L = oracle()
