@@ -318,6 +318,40 @@ int  sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots,
                       const sfd_fwi_opts *opt, double *misfit, float *grad,
                       float *illum_out, float *resid_out);
 
+/* Options of sfd_afwi_gradient (the InversionParam fields 3DAFWI reads). */
+typedef struct sfd_afwi_opts {
+    int32_t snap_mode;       /* SFD_SNAP_*                                    */
+    int32_t norm;            /* SFD_NORM_* (0 = L2), data_nrm_type             */
+    float   norm_thresh;     /* data_threshold (0: per shot from the first trace
+                                whose residual is not all zero, as in
+                                sfd_fwi_gradient)                              */
+    int32_t vprho_param;     /* with a density: 0 gradients for (vp, rho), 1 for
+                                (vp, Ip = rho vp) (InversionParam%vprho_param,
+                                ExtractPadModel.f90:643-646)                   */
+} sfd_afwi_opts;
+
+/*
+ * The one-shot gradient of 3DAFWI.x (progs/afwi/3DAFWI.f90), AFWI_to_memory
+ * (progs/afwi/to_disk_to_memory_AFWI.f90:229-415), for a batch of shots in
+ * 2D: per shot forward modeling with Injection_sinc (readsou's half-step
+ * shift with a density) and Extraction_array_sinc_afwi (gain 1) storing
+ * snapshots; Compute_OF_RES_3D (Objective_Function_Residual_Adjsource.f90:
+ * 42-67) under opt->norm; reverse time with the adjoint-style stencil
+ * (the density operator with a density), Injection_sinc of the residual and
+ * Imaging_condition_AFWI_from_memory (Imaging.f90:371-422), or with a density
+ * Imaging_condition_AFWI_RHOVP_from_memory (:424-512); each window added into
+ * the model (mod_copy_gradient_to_disk, ExtractPadModel.f90:592-660).
+ * Outputs, summed over the shots (3DAFWI writes one shot's): misfit the sum
+ * of the traces' norms (no scaling); grad (nz,nx) for vp, followed with a
+ * density by (nz,nx) for rho (vprho_param 0) or Ip (1, the change of
+ * variables on the model's vel and rho); illum (nz,nx), may be NULL.  dobs
+ * may be NULL (the copy kept by sfd_set_traces); mute may be NULL.  With a
+ * density every shot's window needs two columns (the imaging's one-sided x
+ * difference).
+ */
+int  sfd_afwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs, const float *mute,
+                       const sfd_afwi_opts *opt, double *misfit, float *grad, float *illum);
+
 /*
  * Per-phase timing of the last call (the reference's verbose timers,
  * Propagator.f90:165-258): ms[0]=total GPU time of the propagation kernels,

@@ -1000,7 +1000,7 @@ static int CAT(adjoint_core_, SFX)(PROP P, const sfd_shot *s, const REAL *dbar, 
 int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *dbar, REAL *sbar,
                          int extract_kind)
 {
-    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
+    if (orc_rho) return -4;   /* variable density: forward modeling, RTM and afwi only */
     if (g->rec_type || g->shot_type || g->surf_type) return -4;   /* no free surface or ghosts */
     PROP P;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -1017,7 +1017,7 @@ int FN(modeling_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, 
 int FN(born_adjoint)(const sfd_grid *g, const REAL *vel, const sfd_shot *s, const REAL *wavelet,
                      const REAL *dbar, REAL *mbar)
 {
-    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
+    if (orc_rho) return -4;   /* variable density: forward modeling, RTM and afwi only */
     if (g->rec_type || g->shot_type || g->surf_type) return -4;   /* no free surface or ghosts */
     PROP P;
     float zz = s->sz, xx = s->sx;
@@ -1061,7 +1061,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
             const REAL *wavelet, const REAL *traces_in, REAL *image, REAL *illum, int lsrtm,
             REAL *capture, REAL *win_image, REAL *win_illum)
 {
-    if (orc_rho && lsrtm) return -4;   /* variable density: forward modeling and RTM only */
+    if (orc_rho && lsrtm) return -4;   /* variable density: forward modeling, RTM and afwi only */
     PROP P;
     float zz = s->sz, xx = s->sx;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -1110,7 +1110,7 @@ int FN(rtm)(const sfd_grid *g, const REAL *vel, const REAL *vel_bwd, const sfd_s
 int FN(born)(const sfd_grid *g, const REAL *vel, const REAL *refl, const sfd_shot *s,
              const REAL *wavelet, REAL *traces)
 {
-    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
+    if (orc_rho) return -4;   /* variable density: forward modeling, RTM and afwi only */
     PROP P;
     float zz = s->sz, xx = s->sx;
     if (CAT(setup_shot_, SFX)(&P, g, vel, s)) return -2;
@@ -1384,7 +1384,7 @@ int FN(fwi_gradient)(const sfd_grid *g, const REAL *vel, int nshots, const sfd_s
                      const REAL *wavelet, const REAL *dobs, const REAL *mute, const REAL *vpmask,
                      const sfd_fwi_opts *o, double *fout, REAL *grad, REAL *illu_out, REAL *resid)
 {
-    if (orc_rho) return -4;   /* variable density: forward modeling and RTM only */
+    if (orc_rho) return -4;   /* variable density: forward modeling, RTM and afwi only */
     size_t N = (size_t)g->nz * g->nx, off = 0;
     REAL *illu = (REAL *)calloc(N, sizeof(REAL));
     double f = 0;

@@ -92,6 +92,11 @@ class sfd_fwi_opts(C.Structure):
     ]
 
 
+class sfd_afwi_opts(C.Structure):
+    _fields_ = [("snap_mode", C.c_int32), ("norm", C.c_int32), ("norm_thresh", C.c_float),
+                ("vprho_param", C.c_int32)]
+
+
 class sfd_bandpass_opts(C.Structure):
     _fields_ = [("flo", C.c_float), ("fhi", C.c_float), ("nplo", C.c_int32), ("nphi", C.c_int32),
                 ("phase", C.c_int32)]
@@ -153,7 +158,7 @@ EXPORTS = [
     "sfd_create", "sfd_destroy", "sfd_last_error", "sfd_device_info",
     "sfd_set_model", "sfd_set_density", "sfd_set_elevation", "sfd_forward", "sfd_forward_noalloc", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
     "sfd_born_adjoint_exact", "sfd_lsrtm", "sfd_bandpass", "sfd_mute_mask",
-    "sfd_fwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
+    "sfd_fwi_gradient", "sfd_afwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
     "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgsoc_step", "sfd_lbfgsoc_step_files", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
@@ -203,6 +208,8 @@ def load():
     lib.sfd_fwi_gradient.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p,
                                      C.c_void_p, P(sfd_fwi_opts), P(C.c_double), C.c_void_p,
                                      C.c_void_p, C.c_void_p]
+    lib.sfd_afwi_gradient.argtypes = [C.c_void_p, P(sfd_shot), C.c_int, C.c_void_p, C.c_void_p, P(sfd_afwi_opts),
+                                      P(C.c_double), C.c_void_p, C.c_void_p]
     lib.sfd_last_timing.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.sfd_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.sfd_get_tile.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]

@@ -75,6 +75,7 @@ struct BatchNeeds {
     bool snap_global = false;   // FP16 with one scale per snapshot (SFD_SNAP_FP16G)
     bool imaging = false;
     bool born = false;
+    bool rhovp = false;         // a second image (the rho gradient of the v-rho imaging condition)
 };
 
 struct Batch {
@@ -171,6 +172,8 @@ struct Engine {
     void bandpass(const sfd_bandpass_opts &o, int nt, int ntr, float *traces);
     void fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
                       const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out);
+    void afwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute,
+                       const sfd_afwi_opts *opt, double *misfit, float *grad, float *illum_out);
 
     static bool tile_supported(int TZ, int TX);
     void check_shots(const sfd_shot *shots, int nshots, int snaps = -1, bool may_tune = true);
@@ -203,7 +206,8 @@ struct Engine {
     double count_pts(const std::vector<ShotPlan> &plans) const;
     void upload_inj_traces(Batch &B, int s, const float *tr, size_t count);
     void traces_time_major(const Batch &B);
-    void accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill);
+    void accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill,
+                           float *d_img2 = nullptr);
     size_t shot_bytes(const sfd_shot &s, const BatchNeeds &need) const;
     std::vector<std::pair<int, int>> plan_waves(const sfd_shot *shots, int ns, const BatchNeeds &need);
 };

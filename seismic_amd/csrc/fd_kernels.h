@@ -60,7 +60,7 @@ __device__ __forceinline__ void stw(GLB float *base, unsigned e, float v) {
 }
 
 // Imaging modes (Imaging.f90)
-enum { IMG_NONE = 0, IMG_RTM = 1, IMG_FWI = 2, IMG_LSRTM = 3, IMG_BORN = 4 };
+enum { IMG_NONE = 0, IMG_RTM = 1, IMG_FWI = 2, IMG_LSRTM = 3, IMG_BORN = 4, IMG_RHOVP = 5 };
 // Extraction modes (Extraction.f90), -1 none
 enum { EXT_SINC_V2 = 0, EXT_NEAREST = 1, EXT_SINC_LSRTM = 2, EXT_SINC_UNIT = 3 };
 
@@ -136,6 +136,8 @@ struct ShotDev {
     // rho_grad_kernel each step (cells it never writes stay zero)
     const float *delp;
     float *sz, *sx;
+    const float *rho;       // variable density: rho, padded (the v-rho imaging condition reads it)
+    float *image2;          // IMG_RHOVP: the rho gradient's window accumulator (nz*nxw)
 };
 
 struct StepArgs {
@@ -352,6 +354,61 @@ template <bool FWI>
 __global__ void __launch_bounds__(256) image_kernel(StepArgs A) {
     image_cells<FWI, kImgColsBatch>(A, A.shots[gp(A.shot_order)[blockIdx.z]], (int)blockIdx.x, (int)blockIdx.y,
                                     (int)threadIdx.x);
+}
+
+// Imaging_condition_AFWI_RHOVP_from_memory (Imaging.f90:424-512) in 2D at
+// one step of a reverse pass with variable density: one thread per window
+// cell (grid: window columns, 256-row chunks, the running shots in order of
+// decreasing record length), launched before the free surface and the step
+// change u2.  With ip = min(nz, i+1), jp = min(j+1, nxw), im = ip-1,
+// jm = jp-1 (window cells, 1-based):
+//   tmp  = dzi (u(ip,jm) - u(im,jm)) dzi (S(ip,jm) - S(im,jm))
+//        + dxi (u(im,jp) - u(im,jm)) dxi (S(im,jp) - S(im,jm))  (+ the 2D y term, 0)
+//   tmp1 = u(i,j) / (dt snapi)^2 (S- - 2 S + S+)
+//   image  += 2 tmp1 t / (v^3 rho),  image2 += tmp t / rho^2 + tmp1 t / (rho^2 v^2),
+//   illum  += S S t
+// in the reference's operation order; rows above the free surface are not
+// imaged (:455-458).
+__global__ void __launch_bounds__(256) image_rhovp_kernel(StepArgs A, float dzi, float dxi) {
+    const ShotDev &S = A.shots[gp(A.shot_order)[blockIdx.z]];
+    const int wx = blockIdx.x, wz = blockIdx.y * 256 + threadIdx.x;
+    if (wx >= S.nxw || wz >= S.nz) return;
+    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
+    int it = its;
+    if (A.nbase) {
+        const int b = *gp(A.nbase);
+        it = A.dir > 0 ? its + b : its - b;
+    } else if (it < 1 || it > S.nt) {
+        return;   // this shot has finished (ragged batch)
+    }
+    if (its % A.snapi != 0) return;
+    const int iz = wz + S.iz0, ix = wx + S.i0;
+    if (S.fs_row && iz < gp(S.fs_row)[ix]) return;
+    const int idx = it / A.snapi;
+    const size_t npix = (size_t)S.nz * S.nxw;
+    const int i = wz + 1, j = wx + 1;
+    const int ip = i + 1 < S.nz ? i + 1 : S.nz, jp = j + 1 < S.nxw ? j + 1 : S.nxw;
+    const int im = ip - 1, jm = jp - 1;
+    const GLB float *u2 = gp(S.u[A.rot]);
+    auto U = [&](int a, int b) { return u2[(size_t)(b - 1 + S.i0) * S.LDZ + S.zoff + (a - 1 + S.iz0)]; };
+    auto W = [&](int k, int a, int b) {
+        return load_snap(S, A.snap_fp16, k, npix, (size_t)(b - 1) * S.nz + (a - 1),
+                         snap_block(S, a - 1 + S.iz0, b - 1 + S.i0), false);
+    };
+    const float uu = U(i, j);
+    const float Sv = W(idx, i, j);
+    const float Sm = W(idx - 1 > 1 ? idx - 1 : 1, i, j), Sp = W(idx + 1 < S.ntsnap ? idx + 1 : S.ntsnap, i, j);
+    const float wmm = W(idx, im, jm);
+    float tmp = dzi * (U(ip, jm) - U(im, jm)) * dzi * (W(idx, ip, jm) - wmm);
+    tmp = tmp + dxi * (U(im, jp) - U(im, jm)) * dxi * (W(idx, im, jp) - wmm);
+    tmp = tmp + 0.f;
+    const float tmp1 = uu * (1.f / S.dt2snap) * (Sm - 2.f * Sv + Sp);
+    const size_t a = (size_t)ix * S.LDZ + S.zoff + iz, q = (size_t)wx * S.nz + wz;
+    const float v = gp(S.vel)[a], r = gp(S.rho)[a], t = gp(S.taper)[wx];
+    GLB float *img = gp(S.image), *img2 = gp(S.image2), *ill = gp(S.illum);
+    img[q] = img[q] + 2.f * tmp1 * t / (v * v * v * r);
+    img2[q] = img2[q] + tmp * t / (r * r) + tmp1 * t / (r * r * (v * v));
+    ill[q] = ill[q] + Sv * Sv * t;
 }
 
 // Boundary_set_free_surface_grid (Boundary.f90:1609-1658) on u2 of every

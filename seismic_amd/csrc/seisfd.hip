@@ -650,7 +650,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
         size_t snap, snap_bytes, snap_scale, image, illum, born_m;
         size_t fs_row, fs_w;
-        size_t sz, sx, delp;
+        size_t sz, sx, delp, rho, image2;
     };
     std::vector<Off> off(ns);
     const bool tb = tz == B2Z && tx == B2X;
@@ -668,11 +668,12 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         for (int k = 0; k < kLevels; k++) o.u[k] = L.take(nf * sizeof(float));
         o.v2 = L.take(nf * sizeof(float));
         o.vel = L.take(nf * sizeof(float));
-        o.sz = o.sx = o.delp = 0;
+        o.sz = o.sx = o.delp = o.rho = 0;
         if (has_rho()) {
             o.sz = L.take(nf * sizeof(float));
             o.sx = L.take(nf * sizeof(float));
             o.delp = L.take(nf * sizeof(float));
+            o.rho = L.take(nf * sizeof(float));
         }
         o.hz = L.take((size_t)16 * (p.NX - 8) * sizeof(float));
         o.hx = L.take((size_t)16 * (p.NZ - 8) * sizeof(float));
@@ -729,6 +730,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
                                              : 16);
         o.image = L.take(need.imaging ? npix * sizeof(float) : 16);
         o.illum = L.take(need.imaging ? npix * sizeof(float) : 16);
+        o.image2 = L.take(need.rhovp ? npix * sizeof(float) : 16);   // zeroed with the illumination's range
         o.born_m = L.take(need.born ? npix * sizeof(float) : 16);
     }
     // tiles shot by shot, longest record first (stable in shot order)
@@ -820,6 +822,8 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.sz = has_rho() ? (float *)(base + o.sz) : nullptr;
         S.sx = has_rho() ? (float *)(base + o.sx) : nullptr;
         S.delp = has_rho() ? (const float *)(base + o.delp) : nullptr;
+        S.rho = has_rho() ? (const float *)(base + o.rho) : nullptr;
+        S.image2 = need.rhovp ? (float *)(base + o.image2) : nullptr;
         S.inj_tile = (const int *)(base + o.inj_tile);
         S.inj_cell = (const int *)(base + o.inj_cell);
         S.inj_cptr = (const int *)(base + o.inj_cptr);
@@ -931,18 +935,17 @@ void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float 
         hipLaunchKernelGGL(higdon_tables_kernel, dim3((nk + 255) / 256), dim3(256), 0, stream, S.vel, p.NZ, p.NX,
                            p.LDZ, p.zoff, g.dt, g.dz, g.dx, (float *)S.hz, (float *)S.hx);
         if (has_rho()) {
-            // padded rho (in sx), rho2 along z (in sz), 1/rho2 along x (delp),
-            // v^2 rho; then the gradient arrays start at zero
+            // padded rho, rho2 along z (in sz), 1/rho2 along x (delp), v^2
+            // rho; then the gradient arrays start at zero
             Sinc9 s9;
             sfd_host_mksinc(s9.w, 9, 0.5f);
             const size_t nf = (size_t)p.LDZ * p.NX;
             hipLaunchKernelGGL(rho_pad_kernel, grd, blk, 0, stream, (const float *)d_rho, g.nz, B.colsrc_d[s], p.nxw,
-                               g.ntaper, p.ntop, p.NZ, p.NX, p.LDZ, p.zoff, S.sx);
-            hipLaunchKernelGGL(rho_half_kernel, grd, blk, 0, stream, (const float *)S.sx, S.sz, p.NZ, p.NX, p.LDZ,
-                               p.zoff, s9, 0);
+                               g.ntaper, p.ntop, p.NZ, p.NX, p.LDZ, p.zoff, (float *)S.rho);
+            hipLaunchKernelGGL(rho_half_kernel, grd, blk, 0, stream, S.rho, S.sz, p.NZ, p.NX, p.LDZ, p.zoff, s9, 0);
             hipLaunchKernelGGL(rho_half_kernel, grd, blk, 0, stream, (const float *)S.sz, (float *)S.delp, p.NZ, p.NX,
                                p.LDZ, p.zoff, s9, 1);
-            hipLaunchKernelGGL(v2_rho_kernel, dim3(2048), dim3(256), 0, stream, (float *)S.v2, (const float *)S.sx, nf);
+            hipLaunchKernelGGL(v2_rho_kernel, dim3(2048), dim3(256), 0, stream, (float *)S.v2, S.rho, nf);
             hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, (char *)S.sz, nf * sizeof(float));
             hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, (char *)S.sx, nf * sizeof(float));
         }
@@ -1057,9 +1060,20 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         A.s0 = B.shots_h[0];
     }
     const int mode = snap_launch(B, A);
+    // the v-rho imaging condition: its own kernel on u2, before the surface
+    // and the step change it; the step then runs lean
+    const bool rhovp = mode == 2 && A.imaging == IMG_RHOVP;
+    if (rhovp) {
+        const int nact0 = B.active_shots(A.n);
+        int nxw = 1;
+        for (int k = 0; k < nact0; k++) nxw = std::max(nxw, B.shots_h[B.order[k]].nxw);
+        hipLaunchKernelGGL(image_rhovp_kernel, dim3(nxw, (g.nz + 255) / 256, nact0), dim3(256), 0, stream, A,
+                           1.f / g.dz, 1.f / g.dx);
+    }
     // reverse-pass imaging: one shot, mode 3; a batch, mode 0 + image_kernel
-    const bool image = mode == 2 && A.imaging != IMG_BORN;
-    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? (A.single ? 3 : 0) : mode, strip_px, has_rho());
+    const bool image = mode == 2 && A.imaging != IMG_BORN && !rhovp;
+    StepFn f = pick_kernel(tz, tx, adj, A.single != 0, image ? (A.single ? 3 : 0) : rhovp ? 0 : mode, strip_px,
+                           has_rho());
     if (!f) throw EngineError(SFD_EUNSUPPORTED, has_rho() ? "variable density runs on 64x32 tiles with 4-column strips"
                                                           : "tile shape not compiled");
     // the shots still running at step n: a prefix of the tile layout
@@ -1199,6 +1213,7 @@ double Engine::step_bytes(const Batch &B, const StepArgs &A, int n, bool two) co
             const double npix = (double)S.nz * (double)S.nxw;
             if (A.store_snap) b += (A.snap_global ? sb + 4.0 : sb) * npix;   // + the scale's read of u2
             if (A.imaging == IMG_BORN) b += (sb + 8.0) * npix;   // snapshot, reflectivity, velocity
+            else if (A.imaging == IMG_RHOVP) b += (3.0 * sb + 36.0) * npix;   // + rho, u2, image2
             else if (A.imaging != IMG_NONE) b += ((A.imaging == IMG_FWI ? 3.0 : 1.0) * sb + 16.0) * npix;
         }
     }
@@ -1486,13 +1501,14 @@ size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
     const size_t ntile = (NZ + tz) / tz * ((NX + tx) / tx + 1);
     const size_t npix = (size_t)g.nz * nxw, ntr = (size_t)std::max(s.ntr, 1), nt = (size_t)s.nt;
     const size_t ntsnap = (size_t)std::max(s.nt / g.snapi, 1);
-    size_t b = (kLevels + 2 + (has_rho() ? 3 : 0)) * LDZ * NX * 4;   // levels, v^2, vel (+ gradients, 1/rho2)
+    size_t b = (kLevels + 2 + (has_rho() ? 4 : 0)) * LDZ * NX * 4;   // levels, v^2, vel (+ gradients, 1/rho2, rho)
     b += 16 * (NX + NZ) * 4 + 2 * NX * 4;                     // Higdon tables, window columns, taper
     b += 3 * nt * ntr * 4;                                    // injected (two layouts) and extracted traces
     b += ntr * 49 * 16 * 4 + ntr * 16 * 4 + ntile * 64;       // contributions (both sets, two-step regions)
     b += (ntr + 4) * 2 * 64 * 32 * 4;                         // dense cell maps of the injecting tiles
     if (need.snapshots) b += ntsnap * npix * (need.snap_fp16 ? 2 : 4) + (need.snap_fp16 ? ntsnap * ntile * 16 * 4 : 0);
     if (need.imaging) b += 2 * npix * 4;
+    if (need.rhovp) b += npix * 4;
     if (need.born) b += npix * 4;
     return b + (b >> 4) + (1 << 20);
 }
@@ -1594,7 +1610,7 @@ void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extra
 
 // Add every shot's window image into device full-model buffers (mod_copy_image,
 // nearest index, shots in order); the buffers are zeroed by the caller.
-void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill) {
+void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_img, float *d_ill, float *d_img2) {
     std::vector<std::vector<std::pair<int, int>>> inv(g.nx);
     for (int s = 0; s < B.ns; s++)
         for (int j = 0; j < plans[s].nxw; j++) {
@@ -1613,7 +1629,8 @@ void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_
     Layout L;
     const size_t o_ptr = L.take(ptr.size() * 4), o_shot = L.take(std::max<size_t>(ishot.size(), 1) * 4),
                  o_col = L.take(std::max<size_t>(icol.size(), 1) * 4);
-    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_ci2 = L.take(sizeof(CopyImg) * B.ns);
+    const size_t o_ci = L.take(sizeof(CopyImg) * B.ns), o_ci2 = L.take(sizeof(CopyImg) * B.ns),
+                 o_ci3 = L.take(sizeof(CopyImg) * B.ns);
     tmp_mem.reserve(L.off);
     char *base = (char *)tmp_mem.p;
     HIPCHK(hipMemcpyAsync(base + o_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, stream));
@@ -1621,13 +1638,15 @@ void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_
         HIPCHK(hipMemcpyAsync(base + o_shot, ishot.data(), ishot.size() * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(base + o_col, icol.data(), icol.size() * 4, hipMemcpyHostToDevice, stream));
     }
-    std::vector<CopyImg> ci(B.ns), ci2(B.ns);
+    std::vector<CopyImg> ci(B.ns), ci2(B.ns), ci3(B.ns);
     for (int s = 0; s < B.ns; s++) {
         ci[s] = {B.shots_h[s].image, nullptr, plans[s].nxw};
         ci2[s] = {B.shots_h[s].illum, nullptr, plans[s].nxw};
+        ci3[s] = {B.shots_h[s].image2, nullptr, plans[s].nxw};
     }
     HIPCHK(hipMemcpyAsync(base + o_ci, ci.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(base + o_ci2, ci2.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(base + o_ci3, ci3.data(), sizeof(CopyImg) * B.ns, hipMemcpyHostToDevice, stream));
     const dim3 grid((g.nz + 255) / 256, g.nx);
     if (d_img)
         hipLaunchKernelGGL(copy_image_kernel, grid, dim3(256), 0, stream, (const CopyImg *)(base + o_ci), g.nz, g.nx,
@@ -1637,6 +1656,10 @@ void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_
         hipLaunchKernelGGL(copy_image_kernel, grid, dim3(256), 0, stream, (const CopyImg *)(base + o_ci2), g.nz, g.nx,
                            (const int *)(base + o_ptr), (const int *)(base + o_shot), (const int *)(base + o_col),
                            d_ill);
+    if (d_img2)
+        hipLaunchKernelGGL(copy_image_kernel, grid, dim3(256), 0, stream, (const CopyImg *)(base + o_ci3), g.nz, g.nx,
+                           (const int *)(base + o_ptr), (const int *)(base + o_shot), (const int *)(base + o_col),
+                           d_img2);
     HIPCHK(hipGetLastError());
     // the host tables above must outlive their async transfers
     HIPCHK(hipStreamSynchronize(stream));
@@ -1654,7 +1677,7 @@ void Engine::accumulate_images(Batch &B, std::vector<ShotPlan> &plans, float *d_
 // model through the transpose of born()'s clamped column gather.
 void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *mbar) {
     if (ghosts_or_surface()) throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
-    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling, RTM and sfd_afwi_gradient");
     check_shots(shots, ns, SFD_SNAP_FP32);
     if (!dbar || !mbar) throw EngineError(SFD_EINVAL, "NULL dbar/mbar");
     if (!(tz == 64 && (tx == 32 || tx == 64))) throw EngineError(SFD_EUNSUPPORTED, "tile shape not compiled");
@@ -1733,7 +1756,7 @@ void Engine::born_adjoint_exact(const sfd_shot *shots, int ns, const float *dbar
 }
 void Engine::adjoint_exact(const sfd_shot *shots, int ns, const float *dbar, float *sbar, int extract_kind) {
     if (ghosts_or_surface()) throw EngineError(SFD_EUNSUPPORTED, "the exact adjoints have no free surface or ghosts");
-    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling, RTM and sfd_afwi_gradient");
     check_shots(shots, ns);
     if (!dbar || !sbar) throw EngineError(SFD_EINVAL, "NULL dbar/sbar");
     if (extract_kind < 0 || extract_kind > 3) throw EngineError(SFD_EINVAL, "bad extract_kind");
@@ -1829,7 +1852,7 @@ void Engine::set_traces(const float *tr, size_t n) {
 
 void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *image, float *illum, int snap_mode,
                  bool lsrtm) {
-    if (lsrtm && has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
+    if (lsrtm && has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling, RTM and sfd_afwi_gradient");
     check_shots(shots, ns, snap_mode);
     reset_kstats();
     if (!traces_in) {
@@ -1948,7 +1971,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
 
 // ---- Born modeling (to_disk_to_memory_BORN.f90:123-220) ----------------------
 void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *traces_out) {
-    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling, RTM and sfd_afwi_gradient");
     check_shots(shots, ns, SFD_SNAP_FP32);
     reset_kstats();
     if (!refl || !traces_out) throw EngineError(SFD_EINVAL, "NULL refl/traces_out");
@@ -2022,7 +2045,7 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
 // post-processing chain runs once on the sums.
 void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute, const float *vpmask,
                           const sfd_fwi_opts *opt, double *misfit, float *grad, float *illum_out, float *resid_out) {
-    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling and RTM only");
+    if (has_rho()) throw EngineError(SFD_EUNSUPPORTED, "variable density is built for forward modeling, RTM and sfd_afwi_gradient");
     check_shots(shots, ns, opt ? opt->snap_mode : -1);
     reset_kstats();
     if (!opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL opt/misfit/grad");
@@ -2221,6 +2244,177 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
     if (illum_out) HIPCHK(hipMemcpyAsync(illum_out, d_illu, N * 4, hipMemcpyDeviceToHost, stream));
     if (resid_out) HIPCHK(hipMemcpyAsync(resid_out, fb + o_res, ndata * 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    *misfit = f;
+    timing[0] = fwd_ms + bwd_ms; timing[1] = fwd_ms; timing[2] = bwd_ms;
+    timing[3] = setup_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
+}
+
+// ---- 3DAFWI one-shot gradient (to_disk_to_memory_AFWI.f90:229-415) ---------
+// The batch machinery of fwi_gradient with 3DAFWI's plugins: sinc extraction
+// of gain 1 (kind 3) in the forward pass, the residual (same kernels:
+// Compute_OF_RES_3D is Compute_OF_RES_ADJ without the residual copy), and a
+// reverse pass with the adjoint-style stencil, Injection_sinc of the residual
+// and FWI imaging, or with a density the density operator and the v-rho
+// imaging condition (image_rhovp_kernel) into a second image.
+void Engine::afwi_gradient(const sfd_shot *shots, int ns, const float *dobs, const float *mute,
+                           const sfd_afwi_opts *opt, double *misfit, float *grad, float *illum_out) {
+    if (!opt || !misfit || !grad) throw EngineError(SFD_EINVAL, "NULL opt/misfit/grad");
+    check_shots(shots, ns, opt->snap_mode);
+    reset_kstats();
+    if (opt->snap_mode != SFD_SNAP_FP32 && opt->snap_mode != SFD_SNAP_FP16S && opt->snap_mode != SFD_SNAP_FP16G)
+        throw EngineError(SFD_EINVAL, "bad snap_mode");
+    const int norm = opt->norm == 0 ? SFD_NORM_L2 : opt->norm;
+    if (norm != SFD_NORM_L2 && norm != SFD_NORM_L1 && norm != SFD_NORM_HUBER && norm != SFD_NORM_CAUCHY &&
+        norm != SFD_NORM_HYPERBOLIC)
+        throw EngineError(SFD_EINVAL, "unknown norm (SFD_NORM_*)");
+    if (!std::isfinite(opt->norm_thresh)) throw EngineError(SFD_EINVAL, "norm_thresh is not finite");
+    if (opt->vprho_param != 0 && opt->vprho_param != 1) throw EngineError(SFD_EINVAL, "vprho_param must be 0 or 1");
+    const bool rho = has_rho();
+    if (rho)
+        for (int s = 0; s < ns; s++) {
+            float oxw;
+            int nxw;
+            shot_window(g, shots[s], &oxw, &nxw);
+            if (nxw < 2)
+                throw EngineError(SFD_EINVAL, "the v-rho imaging condition needs a window of two or more columns");
+        }
+    long ndata = 0;
+    int ntraces = 0;
+    for (int s = 0; s < ns; s++) {
+        ndata += (long)shots[s].nt * shots[s].ntr;
+        ntraces += shots[s].ntr;
+    }
+    if (!dobs) {
+        if (resident_n == 0) throw EngineError(SFD_EINVAL, "dobs is NULL and no traces are resident");
+        if ((size_t)ndata != resident_n) throw EngineError(SFD_EINVAL, "resident traces do not match the shots' nt * ntr");
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    BatchNeeds need;
+    need.backward_injection = true;
+    need.snapshots = true;
+    need.snap_fp16 = opt->snap_mode == SFD_SNAP_FP16S || opt->snap_mode == SFD_SNAP_FP16G;
+    need.snap_global = opt->snap_mode == SFD_SNAP_FP16G;
+    need.imaging = true;
+    need.rhovp = rho;
+    Batch &B = fwd_batch;
+    const auto waves = plan_waves(shots, ns, need);
+    const size_t N = (size_t)g.nz * g.nx;
+    const int np = rho ? 2 : 1;
+    Layout L;
+    const size_t o_dobs = L.take(dobs ? ndata * 4 : 16), o_mute = L.take(mute ? ndata * 4 : 16), o_res = L.take(ndata * 4);
+    const size_t o_refs = L.take(sizeof(TraceRef) * ntraces), o_adj = L.take(sizeof(float *) * ns);
+    const size_t o_ft = L.take(sizeof(double) * ntraces), o_tmax = L.take(sizeof(float) * ntraces),
+                 o_thr = L.take(sizeof(float) * ntraces), o_tbeg = L.take(sizeof(int) * (ns + 1));
+    const size_t o_grad = L.take(np * N * 4), o_illu = L.take(N * 4);
+    fwi_mem.reserve(L.off);
+    char *fb = (char *)fwi_mem.p;
+    const float *d_dobs = dobs ? (const float *)(fb + o_dobs) : (const float *)resident.p;
+    if (dobs) HIPCHK(hipMemcpyAsync(fb + o_dobs, dobs, ndata * 4, hipMemcpyHostToDevice, stream));
+    if (mute) HIPCHK(hipMemcpyAsync(fb + o_mute, mute, ndata * 4, hipMemcpyHostToDevice, stream));
+    float *d_grad = (float *)(fb + o_grad), *d_illu = (float *)(fb + o_illu);
+    HIPCHK(hipMemsetAsync(d_grad, 0, np * N * 4, stream));
+    HIPCHK(hipMemsetAsync(d_illu, 0, N * 4, stream));
+    double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    double fwd_ms = 0, bwd_ms = 0;
+    pts = 0;
+    long doff = 0;
+    int toff = 0;
+    for (const auto &wv : waves) {
+        const int s0 = wv.first, nw = wv.second - wv.first;
+        const sfd_shot *ws = shots + s0;
+        auto ta = std::chrono::steady_clock::now();
+        std::vector<ShotPlan> plans(nw);
+        for (int s = 0; s < nw; s++) {
+            ShotPlan &p = plans[s];
+            plan_shot(ws[s], p);
+            plan_injection(p, vel_h, 1, &ws[s].sz, &ws[s].sx, 1);         // Injection_sinc
+            plan_extraction(p, vel_h, ws[s], EXT_SINC_UNIT);               // Extraction_array_sinc_afwi
+            ShotPlan tmp = p;
+            plan_injection(tmp, vel_h, ws[s].ntr, ws[s].rz, ws[s].rx, 1);   // Injection_sinc of the residual
+            p.take_backward(tmp);
+            p.n_inj_traces = std::max(1, ws[s].ntr);
+        }
+        build_batch(B, plans, need);
+        for (int s = 0; s < nw; s++) upload_inj_traces(B, s, source_wavelet(ws[s], s), ws[s].nt);
+        HIPCHK(hipStreamSynchronize(stream));   // the shifted wavelets are reused by the next wave
+        prepare_models(B, plans, d_vel);
+        std::vector<TraceRef> refs;
+        int wtr = 0;
+        {
+            long o = doff;
+            for (int s = 0; s < nw; s++)
+                for (int r = 0; r < ws[s].ntr; r++) {
+                    refs.push_back({s, r, ws[s].nt, o});
+                    o += ws[s].nt;
+                    wtr++;
+                }
+        }
+        std::vector<float *> adjp(nw);
+        for (int s = 0; s < nw; s++) adjp[s] = (float *)B.shots_h[s].inj_traces;
+        std::vector<int> tbeg(nw + 1, 0);
+        for (int s = 0; s < nw; s++) tbeg[s + 1] = tbeg[s] + ws[s].ntr;
+        HIPCHK(hipMemcpyAsync(fb + o_refs, refs.data(), sizeof(TraceRef) * wtr, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(fb + o_adj, adjp.data(), sizeof(float *) * nw, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(fb + o_tbeg, tbeg.data(), sizeof(int) * (nw + 1), hipMemcpyHostToDevice, stream));
+        setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+
+        StepArgs A = base_args(B);
+        A.dir = 1;
+        A.inject = 1;
+        A.extract = 1;
+        A.store_snap = 1;
+        A.snap_fp16 = need.snap_fp16;
+        A.snap_global = need.snap_global;
+        HIPCHK(hipEventRecord(ev0, stream));
+        run_loop(B, A, false, max_nt(plans));
+        const TraceRef *d_refs = (const TraceRef *)(fb + o_refs);
+        float *const *d_adj = (float *const *)(fb + o_adj);
+        float *d_tmax = (float *)(fb + o_tmax) + toff, *d_thr = (float *)(fb + o_thr) + toff;
+        hipLaunchKernelGGL(residual_kernel, dim3(wtr), dim3(256), 0, stream, d_refs, B.shots, d_dobs,
+                           mute ? (const float *)(fb + o_mute) : nullptr, (float *)(fb + o_res), d_adj, d_tmax);
+        hipLaunchKernelGGL(norm_thresh_kernel, dim3((nw + 63) / 64), dim3(64), 0, stream, nw,
+                           (const int *)(fb + o_tbeg), (const float *)d_tmax, norm, opt->norm_thresh, d_thr);
+        hipLaunchKernelGGL(misfit_kernel, dim3((wtr + 63) / 64), dim3(64), 0, stream, wtr, d_refs,
+                           (const float *)(fb + o_res), (const float *)d_thr, norm, (double *)(fb + o_ft) + toff);
+        if (norm != SFD_NORM_L2)
+            hipLaunchKernelGGL(adjsrc_kernel, dim3(wtr), dim3(256), 0, stream, d_refs, d_adj, (const float *)d_thr, norm);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev1, stream));
+        zero_wavefields(B, plans);
+        for (int s = 0; s < nw; s++) B.injb[s].apply(B.shots_h[s]);
+        upload_shots(B);
+        StepArgs Ab = base_args(B);
+        Ab.dir = -1;
+        Ab.inject = 1;
+        Ab.snap_fp16 = need.snap_fp16;
+        Ab.imaging = rho ? IMG_RHOVP : IMG_FWI;
+        run_loop(B, Ab, true, max_nt(plans));
+        HIPCHK(hipEventRecord(ev2, stream));
+        accumulate_images(B, plans, d_grad, d_illu, rho ? d_grad + N : nullptr);
+        float ms1 = 0, ms2 = 0;
+        HIPCHK(hipEventElapsedTime(&ms1, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&ms2, ev1, ev2));
+        fwd_ms += ms1;
+        bwd_ms += ms2;
+        pts += 2 * count_pts(plans);
+        for (int s = 0; s < nw; s++) doff += (long)ws[s].nt * ws[s].ntr;
+        toff += wtr;
+    }
+    auto tp = std::chrono::steady_clock::now();
+    std::vector<double> ft(ntraces);
+    HIPCHK(hipMemcpyAsync(ft.data(), fb + o_ft, sizeof(double) * ntraces, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(grad, d_grad, np * N * 4, hipMemcpyDeviceToHost, stream));
+    if (illum_out) HIPCHK(hipMemcpyAsync(illum_out, d_illu, N * 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double f = 0;
+    for (int k = 0; k < ntraces; k++) f = f + ft[k];
+    if (!std::isfinite(f)) throw EngineError(SFD_ENOTFINITE, "misfit is not finite");
+    if (rho && opt->vprho_param == 1)   // ExtractPadModel.f90:643-646 on the model's vel and rho
+        for (size_t q = 0; q < N; q++) {
+            const float v = vel_h[q], r = rho_h[q];
+            grad[q] = grad[q] - (r / v) * grad[N + q];
+            grad[N + q] = (1.f / v) * grad[N + q];
+        }
     *misfit = f;
     timing[0] = fwd_ms + bwd_ms; timing[1] = fwd_ms; timing[2] = bwd_ms;
     timing[3] = setup_ms + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
@@ -2564,6 +2758,11 @@ int sfd_fwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const floa
     return guarded(h, [&](Engine &e) {
         e.fwi_gradient(shots, nshots, dobs, mute, vpmask, opt, misfit, grad, illum_out, resid_out);
     });
+}
+
+int sfd_afwi_gradient(sfd_handle h, const sfd_shot *shots, int nshots, const float *dobs, const float *mute,
+                      const sfd_afwi_opts *opt, double *misfit, float *grad, float *illum) {
+    return guarded(h, [&](Engine &e) { e.afwi_gradient(shots, nshots, dobs, mute, opt, misfit, grad, illum); });
 }
 
 int sfd_bandpass(sfd_handle h, const sfd_bandpass_opts *opt, int nt, int ntr, float *traces) {

@@ -256,6 +256,24 @@ class Engine:
                                          C.byref(rep)))
         return x.reshape(nx, nz).T.copy(), _report_dict(rep, keep)
 
+    def afwi_gradient(self, shots, dobs=None, mute=None, snap_mode=abi.SFD_SNAP_FP32, norm=0, norm_thresh=0.0,
+                      vprho_param=0):
+        """3DAFWI.x's gradient summed over the shots: dict(f, grad [nparam, nz, nx]
+        (vp; with a density also rho or Ip), illum [nz, nx]).  dobs None: the
+        traces kept by set_traces."""
+        o = abi.sfd_afwi_opts()
+        o.snap_mode, o.norm, o.norm_thresh, o.vprho_param = snap_mode, norm, norm_thresh, vprho_param
+        ob = None if dobs is None else self._cat(dobs, shots)
+        mu = None if mute is None else self._cat(mute, shots)
+        npar = 2 if getattr(self, "_rho", None) is not None else 1
+        f = C.c_double()
+        grad = np.zeros(npar * self.g.nz * self.g.nx, np.float32)
+        illum = np.zeros(self.g.nz * self.g.nx, np.float32)
+        self._chk(self.lib.sfd_afwi_gradient(self.h, self._shots(shots), len(shots), _ptr(ob), _ptr(mu), C.byref(o),
+                                             C.byref(f), _ptr(grad), _ptr(illum)))
+        return dict(f=f.value, grad=grad.reshape(npar, self.g.nx, self.g.nz).transpose(0, 2, 1),
+                    illum=illum.reshape(self.g.nx, self.g.nz).T)
+
     def adjoint_exact(self, shots, dbar, extract_kind=abi.SFD_EXTRACT_SINC_V2):
         """F^T dbar for each shot (F: wavelet -> traces of forward()); returns a list of nt arrays."""
         db = self._cat(dbar, shots)

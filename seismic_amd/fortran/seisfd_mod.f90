@@ -55,6 +55,12 @@ module seisfd_mod
      real(c_float)      :: norm_thresh = 0.  ! data_threshold
   end type sfd_fwi_opts
 
+  type, bind(C) :: sfd_afwi_opts
+     integer(c_int32_t) :: snap_mode = 0, norm = 0
+     real(c_float)      :: norm_thresh = 0.
+     integer(c_int32_t) :: vprho_param = 0
+  end type sfd_afwi_opts
+
   type, bind(C) :: sfd_bandpass_opts
      real(c_float)      :: flo = 0., fhi = 20.
      integer(c_int32_t) :: nplo = 14, nphi = 14, phase = 0
@@ -172,6 +178,19 @@ module seisfd_mod
        integer(c_int), value :: nshots, extract_kind
        real(c_float), intent(inout) :: traces(*)
      end function sfd_forward3d
+
+     integer(c_int) function sfd_afwi_gradient(h, shots, nshots, dobs, mute, opt, misfit, grad, illum) &
+          bind(C, name='sfd_afwi_gradient')
+       import :: c_int, c_ptr, c_float, c_double, sfd_shot, sfd_afwi_opts
+       type(c_ptr), value :: h
+       type(sfd_shot), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       type(c_ptr), value :: dobs, mute          ! c_loc of the arrays, or c_null_ptr
+       type(sfd_afwi_opts), intent(in) :: opt
+       real(c_double), intent(out) :: misfit
+       real(c_float), intent(inout) :: grad(*)   ! (nz,nx) vp [, (nz,nx) rho or Ip]
+       type(c_ptr), value :: illum
+     end function sfd_afwi_gradient
 
      integer(c_int) function sfd_forward(h, shots, nshots, traces, extract_kind) bind(C, name='sfd_forward')
        import :: c_int, c_ptr, c_float, sfd_shot
