@@ -316,6 +316,20 @@ def main():
                 "algorithmic_bytes_per_launch": b["fwd"]["algorithmic_bytes_per_launch"],
                 "avg_launch_us": b["fwd"]["avg_launch_us"], "launches": b["fwd"]["launches"],
                 "reverse_time_pass": b["bwd"]}
+    if roof:
+        # the streaming ceiling at this size: a plain 5-read / 1-write kernel
+        # over the step kernel's algorithmic bytes per launch
+        import ctypes
+        npts = int(b["fwd"]["algorithmic_bytes_per_launch"] // 24)
+        ms = ctypes.c_double()
+        rc = sa.abi.load().sfd_stream_calibration(npts, 20, ctypes.byref(ms))
+        if rc == 0 and ms.value > 0:
+            ceil = 24.0 * npts / (ms.value * 1e-3) / 1e9
+            roof["stream_ceiling"] = {"kernel": "stream5_calib_kernel (5 FP32 reads, 1 write per point)",
+                                      "points": npts, "avg_launch_us": round(ms.value * 1e3, 3),
+                                      "achieved": round(ceil, 1), "frac": round(ceil / HBM_PEAK_GBS, 4),
+                                      "step_kernel_frac_of_ceiling": round(b["fwd"]["achieved"] / ceil, 4)}
+        log("stream ceiling:", roof.get("stream_ceiling"))
     w = b["w"]
     shots = b["shots"]
     extra = {}

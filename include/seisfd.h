@@ -418,6 +418,15 @@ int  sfd_kernel_totals(sfd_handle h, double *us, double *bytes, double *pts, int
  */
 int  sfd_copy_calibration(size_t nbytes, int reps, double *avg_ms);
 
+/*
+ * Measurement helper (no reference counterpart): the streaming ceiling of a
+ * step.  Runs a kernel that reads five FP32 arrays of npoints values and
+ * writes a sixth (24 bytes per point, the step kernel's algorithmic traffic
+ * and nothing else) `reps` times after one untimed launch; returns the
+ * average kernel time.  Allocates and frees 24 * npoints bytes.
+ */
+int  sfd_stream_calibration(size_t npoints, int reps, double *avg_ms);
+
 /* ---- L-BFGS (libs/libinv/lbfgs.f:49-558 LBFGS + MCSRCH + MCSTEP), reverse
  *      communication exactly as the reference's IFLAG protocol.  Vectors
  *      live on the device; the two-loop recursion and the line-search dot

@@ -159,7 +159,7 @@ EXPORTS = [
     "sfd_set_model", "sfd_set_density", "sfd_set_elevation", "sfd_forward", "sfd_forward_noalloc", "sfd_rtm", "sfd_set_traces", "sfd_born", "sfd_adjoint_exact",
     "sfd_born_adjoint_exact", "sfd_lsrtm", "sfd_bandpass", "sfd_mute_mask",
     "sfd_fwi_gradient", "sfd_afwi_gradient", "sfd_last_timing", "sfd_set_tile", "sfd_get_tile", "sfd_set_option", "sfd_last_waves", "sfd_kernel_stats",
-    "sfd_kernel_totals", "sfd_copy_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
+    "sfd_kernel_totals", "sfd_copy_calibration", "sfd_stream_calibration", "sfd_lbfgs_create", "sfd_lbfgs_step", "sfd_lbfgs_destroy",
     "sfd_lbfgsoc_step", "sfd_lbfgsoc_step_files", "sfd_lbfgs_run", "sfd_fwi_lbfgs",
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
@@ -218,6 +218,7 @@ def load():
     lib.sfd_last_waves.argtypes = [C.c_void_p, P(C.c_int)]
     lib.sfd_kernel_totals.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_int)]
     lib.sfd_copy_calibration.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]
+    lib.sfd_stream_calibration.argtypes = [C.c_size_t, C.c_int, P(C.c_double)]
     lib.sfd_lbfgs_create.argtypes = [P(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     lib.sfd_lbfgs_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,

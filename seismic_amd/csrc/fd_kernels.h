@@ -1130,6 +1130,17 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : Ste
 }
 
 // Plain streaming copy used to calibrate hardware byte counters.
+// The streaming ceiling of a 5-read / 1-write step at a given size: one point
+// per thread, FP32 loads of five arrays and one store, no stencil (the step
+// kernel's algorithmic traffic with nothing else).
+__global__ void __launch_bounds__(256) stream5_calib_kernel(const float *__restrict__ a, const float *__restrict__ b,
+                                                            const float *__restrict__ c, const float *__restrict__ d,
+                                                            const float *__restrict__ e, float *__restrict__ o,
+                                                            size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (20.f * a[i] - 6.f * b[i] - 4.f * c[i] + d[i]) + e[i];
+}
+
 __global__ void copy_calib_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n4) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];

@@ -2870,6 +2870,36 @@ int sfd_copy_calibration(size_t nbytes, int reps, double *avg_ms) {
     return rc;
 }
 
+int sfd_stream_calibration(size_t npoints, int reps, double *avg_ms) {
+    if (npoints < 1 || reps < 1 || !avg_ms) return SFD_EINVAL;
+    float *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SFD_OK;
+    for (int k = 0; k < 6 && rc == SFD_OK; k++)
+        if (hipMalloc(&p[k], npoints * 4) != hipSuccess || hipMemset(p[k], 0, npoints * 4) != hipSuccess)
+            rc = SFD_ENOMEM;
+    if (rc == SFD_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = SFD_EHIP;
+    if (rc == SFD_OK) {
+        const dim3 grid((unsigned)((npoints + 255) / 256));
+        for (int r = 0; r <= reps; r++) {   // the first launch warms up, untimed
+            if (r == 1) (void)hipEventRecord(e0, 0);
+            hipLaunchKernelGGL(stream5_calib_kernel, grid, dim3(256), 0, 0, (const float *)p[0], (const float *)p[1],
+                               (const float *)p[2], (const float *)p[3], (const float *)p[4], p[5], npoints);
+        }
+        (void)hipEventRecord(e1, 0);
+        float ms = 0;
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = SFD_EHIP;
+        *avg_ms = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (int k = 0; k < 6; k++)
+        if (p[k]) (void)hipFree(p[k]);
+    return rc;
+}
+
 int sfd_set_density(sfd_handle h, const float *rho) {
     return guarded(h, [&](Engine &e) { e.set_density(rho); });
 }
