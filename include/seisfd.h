@@ -610,6 +610,7 @@ typedef struct sfd_grid3d {
     int32_t lsinc;           /* sinc interpolator length (7)                   */
     float   aperture_x, aperture_y;   /* migration apertures (m), ExtractPadModel.f90:55-58 */
     float   fmax;            /* vel_check's frequency (Hz); 0: 30              */
+    int32_t snapi;           /* RTM snapshot interval (steps)                  */
 } sfd_grid3d;
 
 /* One 3D shot: coord(1..3) = z, x, y of the source and of each receiver. */
@@ -625,8 +626,10 @@ int  sfd_create3d(sfd_handle3d *h, const sfd_grid3d *g);
 int  sfd_destroy3d(sfd_handle3d h);
 /* h NULL: the message of the last failed sfd_create3d. */
 const char *sfd_last_error3d(sfd_handle3d h);
-/* readvel + vel_check in 3D (Courant stop 0.35, ReadSouVelRho.f90:629-675). */
-int  sfd_set_model3d(sfd_handle3d h, const float *vel);
+/* readvel + vel_check in 3D (Courant stop 0.35, ReadSouVelRho.f90:629-675);
+ * vel_bwd (may be NULL): RTM's backward-pass model (mod%vel2,
+ * to_disk_to_memory_RTM.f90:425). */
+int  sfd_set_model3d(sfd_handle3d h, const float *vel, const float *vel_bwd);
 /* genpar%withRho in 3D (ExtractPadModel.f90:259-270, Acoustic_modeling_sep.f90:
  * 113-121): rho(nz, nx, ny) as vel, or NULL for constant density.  With a
  * density, sfd_forward3d windows and pads rho as vel, takes 1/rho2 from the
@@ -648,6 +651,19 @@ int  sfd_set_density3d(sfd_handle3d h, const float *rho);
  * scales by v, not v^2, in 3D (Extraction.f90:128).  traces: per shot
  * (nt, ntr) time fastest, concatenated. */
 int  sfd_forward3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, float *traces_out, int extract_kind);
+/* 3DRTM.x in 3D (RTM_to_memory, to_disk_to_memory_RTM.f90:358-492, twoD
+ * false): per shot forward modeling storing u2 on the window every snapi
+ * steps (Extraction_wavefield, Extraction.f90:562-601), then reverse time
+ * with the same forward stencil (the 3D branch calls
+ * FD_2nd_3D_derivatives_scalar_forward_grid, or the density operator), the
+ * backward model when set, Injection_sinc of every receiver's trace in trace
+ * order and Imaging_condition_sourceonly_from_memory (Imaging.f90:228-275:
+ * image += u S taperx taper_y, illum += S S taperx tapery); the window image
+ * is added into image / illum (nz,nx,ny) as mod_copy_image does
+ * (ExtractPadModel.f90:662-695, x and y by nearest index).  traces_in: per
+ * shot (nt, ntr) time fastest, concatenated. */
+int  sfd_rtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *traces_in, float *image,
+               float *illum);
 
 /* ---- SEP files (the reference reads and writes every array through SEPlib:
  *      from_aux / sreed / srite, libs/libwave/IO.f90:14-51,

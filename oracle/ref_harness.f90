@@ -292,12 +292,12 @@ contains
   ! ordered only on one thread.
   subroutine ref_propagate3d(nz, nxw, nyw, ntaper, dz, dx, dy, dt, oz, oxw, oyw, lsinc, vel_pad, &
        &                     nsou, souz, soux, souy, soutr, ext, ndat, datz, datx, daty, dattr, nt, &
-       &                     with_rho, rho_pad) bind(C, name='ref_propagate3d')
+       &                     with_rho, rho_pad, tdir, snapi, nsnap, wave) bind(C, name='ref_propagate3d')
     use omp_lib
-    integer(c_int), value :: nz, nxw, nyw, ntaper, lsinc, nsou, ext, ndat, nt, with_rho
+    integer(c_int), value :: nz, nxw, nyw, ntaper, lsinc, nsou, ext, ndat, nt, with_rho, tdir, snapi, nsnap
     real(c_float), value  :: dz, dx, dy, dt, oz, oxw, oyw
     real(c_float)         :: vel_pad(*), souz(*), soux(*), souy(*), soutr(nt, *), datz(*), datx(*), daty(*)
-    real(c_float)         :: dattr(nt, *), rho_pad(*)
+    real(c_float)         :: dattr(nt, *), rho_pad(*), wave(nz, nxw, nyw, *)
 
     type(GeneralParam)         :: genpar
     type(FDbounds)             :: bounds
@@ -307,6 +307,7 @@ contains
     type(UnscaledFDcoefs)      :: fdcoefs
     type(ScaledFDcoefs), target :: scaled
     type(USpace)               :: grid
+    type(WaveSpace)            :: wfld
     type(TraceSpace), allocatable :: sou(:), dat(:)
     integer :: i, j, k, it, n1, n2, n3, nthr
 
@@ -323,14 +324,18 @@ contains
     genpar%lsinc = lsinc
     genpar%nbound = 4
     genpar%ntaper = ntaper
-    genpar%snapi = 1
-    genpar%ntsnap = 0
+    genpar%snapi = snapi
+    genpar%ntsnap = nsnap
     genpar%rec_type = 0
     genpar%shot_type = 0
     genpar%surf_type = 0
     genpar%coefpower = 2
     if (with_rho /= 0) genpar%coefpower = 1
-    genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
+    if (tdir > 0) then
+       genpar%tmin = 1; genpar%tmax = nt; genpar%tstep = 1
+    else
+       genpar%tmin = nt; genpar%tmax = 1; genpar%tstep = -1
+    end if
 
     bounds%nmin1 = -ntaper + 1
     bounds%nmax1 = nz + ntaper
@@ -381,6 +386,14 @@ contains
        dat(i)%trace(:, 1) = 0.
     end do
 
+    if (nsnap > 0) then
+       allocate(wfld%wave(nz, nxw, nyw, nsnap, 1))
+       wfld%wave = 0.
+    end if
+    wfld%counter = 0
+    allocate(elev%ielev_z(bounds%nmin2:bounds%nmax2, bounds%nmin3:bounds%nmax3))
+    elev%ielev_z = 1
+
     nthr = omp_get_max_threads()
     call omp_set_num_threads(1)
     call allocateUSpace(grid, genpar, bounds)
@@ -413,6 +426,7 @@ contains
              call Extraction_array_sinc_afwi(bounds, model, dat(1:ndat), grid%u2, genpar, it)
           end select
        end if
+       if (nsnap > 0) call Extraction_wavefield(bounds, model, elev, grid%u2, genpar, it, dat=wfld)
        if (with_rho /= 0) then
           call FD_3D_derivatives_acoustic_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
        else
@@ -428,6 +442,11 @@ contains
     do i = 1, ndat
        dattr(:, i) = dat(i)%trace(:, 1)
     end do
+    if (nsnap > 0) then
+       do i = 1, nsnap
+          wave(:, :, :, i) = wfld%wave(:, :, :, i, 1)
+       end do
+    end if
     call deallocateHigdonParam(hig)
     call deallocateUSpace(grid)
     if (with_rho /= 0) call deallocate_sxx3d_syy3d_szz3d_delp3d()

@@ -338,55 +338,72 @@ void orc_f_half_step(float *tr, int nt, int lsinc);
  * (FD_acoustic_rho_init_coefs, coefpower 1), the density stencil, the v^2 rho
  * injection gain (Injection.f90:181-187) and readsou's half-step source shift
  * (ReadSouVelRho.f90:328-347). */
-int orc_f_modeling3d_rho(const sfd_grid3d *g, const float *vel_full, const float *rho_full, const sfd_shot3d *s,
-                         float *traces, int extract_kind)
+/* The shot's padded model (window, vel, with a density rho and 1/rho2),
+ * coefficients and Higdon tables. */
+static int setup3(P3 *P, const sfd_grid3d *g, const float *vel_full, const float *rho_full, const sfd_shot3d *s,
+                  float *oxw, int *nxw, float *oyw, int *nyw)
 {
-    P3 P;
-    memset(&P, 0, sizeof P);
-    float oxw, oyw;
-    int nxw, nyw;
-    orc_window3d(g, s, &oxw, &nxw, &oyw, &nyw);
-    orc_bounds3d(&P.b, g->nz, nxw, nyw, g->ntaper);
-    const B3 *b = &P.b;
+    memset(P, 0, sizeof *P);
+    orc_window3d(g, s, oxw, nxw, oyw, nyw);
+    orc_bounds3d(&P->b, g->nz, *nxw, *nyw, g->ntaper);
+    const B3 *b = &P->b;
     const size_t n = (size_t)b->NZ * b->NX * b->NY;
-    P.dz = g->dz; P.dx = g->dx; P.dy = g->dy; P.dt = g->dt;
-    P.vel = (float *)calloc(n, sizeof(float));
-    float *buf[5];
-    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
-    if (!P.vel || !buf[4]) return -2;
-    orc_window_pad3d(g, vel_full, oxw, nxw, oyw, nyw, b, P.vel);
-    float *wav = (float *)malloc(sizeof(float) * (size_t)s->nt);
-    memcpy(wav, s->wavelet, sizeof(float) * (size_t)s->nt);
+    P->dz = g->dz; P->dx = g->dx; P->dy = g->dy; P->dt = g->dt;
+    P->vel = (float *)calloc(n, sizeof(float));
+    if (!P->vel) return -2;
+    orc_window_pad3d(g, vel_full, *oxw, *nxw, *oyw, *nyw, b, P->vel);
+    const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
     if (rho_full) {
-        P.rho = (float *)calloc(n, sizeof(float));
-        P.delp = (float *)calloc(n, sizeof(float));
-        P.sz = (float *)calloc(n, sizeof(float));
-        P.sx = (float *)calloc(n, sizeof(float));
-        P.sy = (float *)calloc(n, sizeof(float));
-        if (!P.sy) return -2;
-        orc_window_pad3d(g, rho_full, oxw, nxw, oyw, nyw, b, P.rho);
-        orc_rho_half3d(b, P.rho, P.delp);
-        orc_f_half_step(wav, s->nt, g->lsinc);
+        P->rho = (float *)calloc(n, sizeof(float));
+        P->delp = (float *)calloc(n, sizeof(float));
+        P->sz = (float *)calloc(n, sizeof(float));
+        P->sx = (float *)calloc(n, sizeof(float));
+        P->sy = (float *)calloc(n, sizeof(float));
+        if (!P->sy) return -2;
+        orc_window_pad3d(g, rho_full, *oxw, *nxw, *oyw, *nyw, b, P->rho);
+        orc_rho_half3d(b, P->rho, P->delp);
         /* FDcoefs.f90:12-23 over delta per axis (coefpower 1) */
         const float r[5] = {0.f, 1225.f / 1024.f, -(1225.f / (1024.f * 15.f)), 1225.f / (1024.f * 125.f),
                             -(1225.f / (1024.f * 1715.f))};
-        const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
         for (int k = 0; k < 5; k++) {
-            P.c[k] = r[k] * dzi;
-            P.c[5 + k] = r[k] * dxi;
-            P.c[10 + k] = r[k] * dyi;
+            P->c[k] = r[k] * dzi;
+            P->c[5 + k] = r[k] * dxi;
+            P->c[10 + k] = r[k] * dyi;
         }
     } else {
         /* FDcoefs.f90:25-36 over delta^2 per axis */
         const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
-        const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
         for (int k = 0; k < 5; k++) {
-            P.c[k] = cc[k] * (dzi * dzi);
-            P.c[5 + k] = cc[k] * (dxi * dxi);
-            P.c[10 + k] = cc[k] * (dyi * dyi);
+            P->c[k] = cc[k] * (dzi * dzi);
+            P->c[5 + k] = cc[k] * (dxi * dxi);
+            P->c[10 + k] = cc[k] * (dyi * dyi);
         }
     }
-    higdon3(&P);
+    higdon3(P);
+    return 0;
+}
+
+static void free3(P3 *P)
+{
+    free(P->vel); free(P->gz); free(P->gx); free(P->gy);
+    free(P->rho); free(P->delp); free(P->sz); free(P->sx); free(P->sy);
+}
+
+int orc_f_modeling3d_rho(const sfd_grid3d *g, const float *vel_full, const float *rho_full, const sfd_shot3d *s,
+                         float *traces, int extract_kind)
+{
+    P3 P;
+    float oxw, oyw;
+    int nxw, nyw;
+    if (setup3(&P, g, vel_full, rho_full, s, &oxw, &nxw, &oyw, &nyw)) return -2;
+    const B3 *b = &P.b;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY;
+    float *buf[5];
+    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+    if (!buf[4]) return -2;
+    float *wav = (float *)malloc(sizeof(float) * (size_t)s->nt);
+    memcpy(wav, s->wavelet, sizeof(float) * (size_t)s->nt);
+    if (rho_full) orc_f_half_step(wav, s->nt, g->lsinc);
     const int l = g->lsinc, h = (int)(l * 0.5);
     /* source */
     int sic[3];
@@ -449,12 +466,184 @@ int orc_f_modeling3d_rho(const sfd_grid3d *g, const float *vel_full, const float
         buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
     }
     for (int q = 0; q < 5; q++) free(buf[q]);
-    free(P.vel); free(P.gz); free(P.gx); free(P.gy); free(ric); free(rw); free(wav);
-    free(P.rho); free(P.delp); free(P.sz); free(P.sx); free(P.sy);
+    free(ric); free(rw); free(wav);
+    free3(&P);
     return 0;
 }
 
 int orc_f_modeling3d(const sfd_grid3d *g, const float *vel_full, const sfd_shot3d *s, float *traces, int extract_kind)
 {
     return orc_f_modeling3d_rho(g, vel_full, NULL, s, traces, extract_kind);
+}
+
+void orc_f_compute_taper(float *t, int n);
+
+/* Injection_sinc (Injection.f90:151-194 -> :274-346) of ntr traces at their
+ * positions, in trace order (the harness runs the reference's OpenMP loop
+ * over traces on one thread): u += (((v^2 [rho]) s wz) wx) wy, y weights from
+ * the x sinc. */
+typedef struct { int ic[3]; float wz[64], wx[64]; } Pt3;
+
+static void inject3(P3 *P, float *u, const Pt3 *pt, int ntr, const float *tr, int nt, int it, int h)
+{
+    for (int r = 0; r < ntr; r++) {
+        const int *ic = pt[r].ic;
+        float v = A3(P, P->vel, ic[0], ic[1], ic[2]), v2r = v * v;
+        if (P->rho) v2r = v2r * A3(P, P->rho, ic[0], ic[1], ic[2]);
+        const float s = tr[(size_t)r * nt + (it - 1)];
+        for (int k = -h; k <= h; k++)
+            for (int j = -h; j <= h; j++)
+                for (int i = -h; i <= h; i++)
+                    A3(P, u, i + ic[0], j + ic[1], k + ic[2]) =
+                        A3(P, u, i + ic[0], j + ic[1], k + ic[2]) + v2r * s * pt[r].wz[h + i] * pt[r].wx[h + j] *
+                                                                        pt[r].wx[h + k];
+    }
+}
+
+static void points3(const sfd_grid3d *g, float oxw, float oyw, int n, const float *z, const float *x, const float *y,
+                    Pt3 *pt)
+{
+    for (int r = 0; r < n; r++) {
+        float dc[3];
+        find_i_d3(&pt[r].ic[0], &dc[0], z[r], g->oz, g->dz);
+        find_i_d3(&pt[r].ic[1], &dc[1], x[r], oxw, g->dx);
+        find_i_d3(&pt[r].ic[2], &dc[2], y[r], oyw, g->dy);
+        orc_f_mksinc(pt[r].wz, g->lsinc, dc[0] / g->dz);
+        orc_f_mksinc(pt[r].wx, g->lsinc, dc[1] / g->dx);
+    }
+}
+
+/* One time step after extraction / imaging: stencil, injection, update,
+ * Higdon, rotation. */
+static void step3(P3 *P, float **buf, const Pt3 *pt, int npt, const float *tr, int nt, int it, int h)
+{
+    if (P->rho) stencil3_rho(P, buf[3], buf[4]);
+    else stencil3(P, buf[3], buf[4]);
+    inject3(P, buf[4], pt, npt, tr, nt, it, h);
+    update3(P, buf[0], buf[1], buf[2], buf[3], buf[4]);
+    boundary3(P, buf[2], buf[3], buf[4]);
+    float *t = buf[0];
+    buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+}
+
+/* progs/rtm/3DRTM.f90 -> RTM_to_memory (to_disk_to_memory_RTM.f90:358-492)
+ * with genpar%twoD false: the forward pass stores u2 on the window every
+ * snapi steps (Extraction_wavefield, Extraction.f90:562-601), the backward
+ * pass runs the same forward stencil (the 3D branch's
+ * FD_2nd_3D_derivatives_scalar_forward_grid, or the density operator) on
+ * vel_bwd when given (mod%vel2, :425) with its own Higdon tables, injects the
+ * receivers' traces by Injection_sinc and images with
+ * Imaging_condition_sourceonly_from_memory (Imaging.f90:228-275) before the
+ * stencil of every snapi-th step; compute_taper (Taper.f90:14-28) in x and
+ * y; mod_copy_image (ExtractPadModel.f90:662-695) adds the window into
+ * image / illum (nz, nx, ny).  Optional outputs: the window image and
+ * illumination, the snapshots and the backward u2 at each imaging step
+ * (nz*nxw*nyw each, in the order they are taken). */
+int orc_f_rtm3d(const sfd_grid3d *g, const float *vel_full, const float *vel_bwd, const float *rho_full,
+                const sfd_shot3d *s, const float *traces_in, float *image, float *illum, float *win_image,
+                float *win_illum, float *snaps_out, float *capture)
+{
+    P3 P;
+    float oxw, oyw;
+    int nxw, nyw;
+    if (g->snapi < 1) return -1;
+    if (setup3(&P, g, vel_full, rho_full, s, &oxw, &nxw, &oyw, &nyw)) return -2;
+    const B3 *b = &P.b;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY, nwin = (size_t)g->nz * nxw * nyw;
+    const int ntsnap = s->nt / g->snapi, h = (int)(g->lsinc * 0.5);
+    float *buf[5], *snaps = (float *)calloc(nwin * (ntsnap > 0 ? ntsnap : 1), sizeof(float));
+    float *img = (float *)calloc(nwin, sizeof(float)), *ill = (float *)calloc(nwin, sizeof(float));
+    float *tx = (float *)malloc(sizeof(float) * nxw), *ty = (float *)malloc(sizeof(float) * nyw);
+    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+    if (!buf[4] || !snaps || !img || !ill || !tx || !ty) return -2;
+    float *wav = (float *)malloc(sizeof(float) * (size_t)s->nt);
+    memcpy(wav, s->wavelet, sizeof(float) * (size_t)s->nt);
+    if (rho_full) orc_f_half_step(wav, s->nt, g->lsinc);
+    Pt3 src, *rec = (Pt3 *)malloc(sizeof(Pt3) * (size_t)(s->ntr > 0 ? s->ntr : 1));
+    points3(g, oxw, oyw, 1, &s->sz, &s->sx, &s->sy, &src);
+    points3(g, oxw, oyw, s->ntr, s->rz, s->rx, s->ry, rec);
+#define WIN(u, i, j, k) A3(&P, u, i, j, k)
+    int counter = 0;
+    for (int it = 1; it <= s->nt; it++) {
+        if (it % g->snapi == 0) {   /* Extraction_wavefield */
+            float *S = snaps + (size_t)counter * nwin;
+            counter++;
+            for (int k = 1; k <= nyw; k++)
+                for (int j = 1; j <= nxw; j++)
+                    for (int i = 1; i <= g->nz; i++)
+                        S[(i - 1) + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw] = WIN(buf[3], i, j, k);
+        }
+        step3(&P, buf, &src, 1, wav, s->nt, it, h);
+    }
+    /* backward: a new propagator call (zeroed levels and gradient arrays) */
+    for (int q = 0; q < 5; q++) memset(buf[q], 0, n * sizeof(float));
+    if (P.sz) {
+        memset(P.sz, 0, n * sizeof(float));
+        memset(P.sx, 0, n * sizeof(float));
+        memset(P.sy, 0, n * sizeof(float));
+    }
+    if (vel_bwd) {
+        orc_window_pad3d(g, vel_bwd, oxw, nxw, oyw, nyw, b, P.vel);
+        free(P.gz); free(P.gx); free(P.gy);
+        higdon3(&P);
+    }
+    orc_f_compute_taper(tx, nxw);
+    for (int k = 0; k < nyw; k++) ty[k] = 1.f;
+    if (nyw > 1) orc_f_compute_taper(ty, nyw);
+    counter = 0;
+    for (int it = s->nt; it >= 1; it--) {
+        if (it % g->snapi == 0) {   /* Imaging_condition_sourceonly_from_memory */
+            counter++;
+            const int index = ntsnap - counter + 1;
+            const float *S = snaps + (size_t)(index - 1) * nwin;
+            if (capture) {
+                float *c = capture + (size_t)(counter - 1) * nwin;
+                for (int k = 1; k <= nyw; k++)
+                    for (int j = 1; j <= nxw; j++)
+                        for (int i = 1; i <= g->nz; i++)
+                            c[(i - 1) + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw] = WIN(buf[3], i, j, k);
+            }
+            for (int k = 1; k <= nyw; k++)
+                for (int j = 1; j <= nxw; j++) {
+                    const float t = tx[j - 1] * ty[k - 1];
+                    for (int i = 1; i <= g->nz; i++) {
+                        const size_t q = (i - 1) + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw;
+                        img[q] = img[q] + WIN(buf[3], i, j, k) * S[q] * t;
+                        ill[q] = ill[q] + S[q] * S[q] * t;
+                    }
+                }
+        }
+        step3(&P, buf, rec, s->ntr, traces_in, s->nt, it, h);
+    }
+#undef WIN
+    /* mod_copy_image: a zeroed model-size sum of the window, then added */
+    const size_t N = (size_t)g->nz * g->nx * g->ny;
+    for (int pass = 0; pass < 2; pass++) {
+        float *dst = pass == 0 ? image : illum;
+        const float *src_w = pass == 0 ? img : ill;
+        if (!dst) continue;
+        float *tmp = (float *)calloc(N, sizeof(float));
+        for (int k = 1; k <= nyw; k++) {
+            const float cy = (float)(k - 1) * g->dy + oyw;
+            const int iy = nint3((cy - g->oy) / g->dy) + 1;
+            if (iy < 1 || iy > g->ny) continue;
+            for (int j = 1; j <= nxw; j++) {
+                const float cx = (float)(j - 1) * g->dx + oxw;
+                const int ix = nint3((cx - g->ox) / g->dx) + 1;
+                if (ix < 1 || ix > g->nx) continue;
+                float *d = tmp + (size_t)(ix - 1) * g->nz + (size_t)(iy - 1) * g->nz * g->nx;
+                const float *w = src_w + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw;
+                for (int i = 0; i < g->nz; i++) d[i] = d[i] + w[i];
+            }
+        }
+        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + tmp[q];
+        free(tmp);
+    }
+    if (win_image) memcpy(win_image, img, nwin * sizeof(float));
+    if (win_illum) memcpy(win_illum, ill, nwin * sizeof(float));
+    if (snaps_out) memcpy(snaps_out, snaps, nwin * (size_t)ntsnap * sizeof(float));
+    for (int q = 0; q < 5; q++) free(buf[q]);
+    free(snaps); free(img); free(ill); free(tx); free(ty); free(wav); free(rec);
+    free3(&P);
+    return 0;
 }

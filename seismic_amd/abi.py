@@ -73,7 +73,7 @@ class sfd_grid3d(C.Structure):
                 ("oz", C.c_float), ("ox", C.c_float), ("oy", C.c_float),
                 ("dz", C.c_float), ("dx", C.c_float), ("dy", C.c_float), ("dt", C.c_float),
                 ("ntaper", C.c_int32), ("lsinc", C.c_int32),
-                ("aperture_x", C.c_float), ("aperture_y", C.c_float), ("fmax", C.c_float)]
+                ("aperture_x", C.c_float), ("aperture_y", C.c_float), ("fmax", C.c_float), ("snapi", C.c_int32)]
 
 
 class sfd_shot3d(C.Structure):
@@ -164,7 +164,7 @@ EXPORTS = [
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
     "sfd_sep_read_header", "sfd_sep_read", "sfd_sep_write", "sfd_sep_gathers",
-    "sfd_create3d", "sfd_destroy3d", "sfd_last_error3d", "sfd_set_model3d", "sfd_set_density3d", "sfd_forward3d",
+    "sfd_create3d", "sfd_destroy3d", "sfd_last_error3d", "sfd_set_model3d", "sfd_set_density3d", "sfd_forward3d", "sfd_rtm3d",
 ]
 
 _lib = None
@@ -185,7 +185,8 @@ def load():
     lib.sfd_destroy3d.argtypes = [C.c_void_p]
     lib.sfd_last_error3d.argtypes = [C.c_void_p]
     lib.sfd_last_error3d.restype = C.c_char_p
-    lib.sfd_set_model3d.argtypes = [C.c_void_p, C.c_void_p]
+    lib.sfd_set_model3d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sfd_rtm3d.argtypes = [C.c_void_p, P(sfd_shot3d), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.sfd_set_density3d.argtypes = [C.c_void_p, C.c_void_p]
     lib.sfd_forward3d.argtypes = [C.c_void_p, P(sfd_shot3d), C.c_int, C.c_void_p, C.c_int]
     lib.sfd_destroy.argtypes = [C.c_void_p]

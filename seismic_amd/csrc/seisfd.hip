@@ -62,8 +62,8 @@ void sfd_host_mksinc(float *sinc, int lsinc, float shift) {
     for (int n = 1; n <= lsinc; n++) sinc[n - 1] = sinc[n - 1] / fsum;
 }
 
-// Taper.f90:14-44 compute_taper / cos_sin_taper
-static void host_taper(float *t, int n) {
+// Taper.f90:14-44 compute_taper / cos_sin_taper (also the 3D engine's, per axis)
+void sfd_host_taper(float *t, int n) {
     const float pi = 3.14159265359f;
     const int nt = std::min(n / 2, 10);
     const int m = std::min(nt, n);
@@ -325,7 +325,7 @@ void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
         p.colsrc[j - 1] = ix - 1;
     }
     p.taper.resize(p.nxw);
-    host_taper(p.taper.data(), p.nxw);
+    sfd_host_taper(p.taper.data(), p.nxw);
     // Free surface (ModelSpace_elevation_parameters, ModelSpaceTypes.f90:128-141):
     // per FD column, ielev_z / delev_z of its elevation by find_i_d against
     // the depth axis, and the sinc of delev_z / dz that an off-grid surface

@@ -25,6 +25,7 @@
 #include "common.h"
 
 void sfd_host_mksinc(float *sinc, int lsinc, float shift);
+void sfd_host_taper(float *t, int n);
 
 namespace sfd3 {
 
@@ -208,16 +209,47 @@ __global__ void __launch_bounds__(256) stencil3_kernel(const float *__restrict__
     u3[q] = v2[q] * (tx + tz + ty);
 }
 
-// Injection_source_sinc_xyz (Injection.f90:274-346, shot_type 0): one thread
-// per footprint cell (every cell gets one term), u3 += v2r s wz wx wy with the
-// y weights from the x sinc, as the reference computes them.
-struct Inj3 { int iz, ix, iy; float v2r; float wz[7], wx[7]; };
-__global__ void inject3_kernel(float *__restrict__ u3, Geo G, Inj3 I, const float *__restrict__ wavelet, int it) {
-    const int t = threadIdx.x;
-    if (t >= 343) return;
-    const int i = t % 7 - 3, j = (t / 7) % 7 - 3, k = t / 49 - 3;
-    const size_t q = G.at(I.iz + i, I.ix + j, I.iy + k);
-    u3[q] = u3[q] + I.v2r * wavelet[it - 1] * I.wz[3 + i] * I.wx[3 + j] * I.wx[3 + k];
+// Injection_sinc -> Injection_source_sinc_xyz (Injection.f90:151-194,
+// :274-346, shot_type 0) of one or more traces: one thread per footprint cell
+// of the plan, which lists the cell's contributions in trace order (the
+// reference adds trace after trace); each is u3 += (((v2r s) wz) wx) wy, v2r
+// the trace's v^2 (times rho) and the y weight the x sinc, as the reference
+// computes them.
+struct InjC { int tr; float wz, wx, wy; };
+__global__ void inject3_kernel(float *__restrict__ u3, const size_t *__restrict__ cell, const int *__restrict__ cptr,
+                               const InjC *__restrict__ con, const float *__restrict__ v2r,
+                               const float *__restrict__ traces, int nt, int ncell, int it) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const size_t q = cell[c];
+    float u = u3[q];
+    for (int k = cptr[c]; k < cptr[c + 1]; k++) {
+        const InjC e = con[k];
+        u = u + v2r[e.tr] * traces[(size_t)e.tr * nt + (it - 1)] * e.wz * e.wx * e.wy;
+    }
+    u3[q] = u;
+}
+
+// Extraction_wavefield (Extraction.f90:562-601): u2 on the window (rows 1..nz,
+// window columns and lines) into snapshot slot s.
+__global__ void snap3_kernel(const float *__restrict__ u2, Geo G, int nz, int nxw, int i0,
+                             float *__restrict__ snap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nz) return;
+    snap[(size_t)i + (size_t)j * nz + (size_t)k * nz * nxw] = u2[G.at(i + i0, j + i0, k + i0)];
+}
+
+// Imaging_condition_sourceonly_from_memory (Imaging.f90:228-275): image +=
+// u S t, illum += S S t with t = taperx(j) tapery(k), per window cell.
+__global__ void image3_kernel(const float *__restrict__ u2, Geo G, int nz, int nxw, int i0,
+                              const float *__restrict__ snap, const float *__restrict__ tx,
+                              const float *__restrict__ ty, float *__restrict__ img, float *__restrict__ ill) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nz) return;
+    const size_t q = (size_t)i + (size_t)j * nz + (size_t)k * nz * nxw;
+    const float t = tx[j] * ty[k], S = snap[q];
+    img[q] = img[q] + u2[G.at(i + i0, j + i0, k + i0)] * S * t;
+    ill[q] = ill[q] + S * S * t;
 }
 
 // FDderivatives.f90:960-982 on the FD region
@@ -299,14 +331,15 @@ using namespace sfd3;
 struct sfd_ctx3d {
     sfd_grid3d g;
     hipStream_t st = nullptr;
-    std::vector<float> vel_h, rho_h;   // rho_h empty: constant density
-    float *d_vel = nullptr, *d_rho = nullptr;
+    std::vector<float> vel_h, velb_h, rho_h;   // velb_h: RTM's backward model; rho_h empty: constant density
+    float *d_vel = nullptr, *d_velb = nullptr, *d_rho = nullptr;
     std::string err;
     void *mem = nullptr;
     size_t cap = 0;
     ~sfd_ctx3d() {
         if (mem) (void)hipFree(mem);
         if (d_vel) (void)hipFree(d_vel);
+        if (d_velb) (void)hipFree(d_velb);
         if (d_rho) (void)hipFree(d_rho);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -379,203 +412,433 @@ static void vel_check3(const sfd_grid3d &g, const float *v, size_t n) {
     }
 }
 
-static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int kind) {
-    const sfd_grid3d &g = h.g;
+// One shot's run on the device: window, padded geometry, buffers, tables.
+struct Run3 {
+    const sfd_grid3d *g = nullptr;
+    Geo G;
+    float oxw = 0, oyw = 0;
+    int nxw = 0, nyw = 0, i0 = 0, fz = 0, fx = 0, fy = 0;
+    bool rho = false;
+    std::vector<int> colsrc, linesrc;
+    char *base = nullptr;
+    float *u[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *vel = nullptr, *v2 = nullptr, *gz = nullptr, *gx = nullptr, *gy = nullptr;
+    float *rhop = nullptr, *delp = nullptr, *tmp = nullptr, *sg[3] = {nullptr, nullptr, nullptr};
+    const int *dcol = nullptr, *dlin = nullptr;
+    Coef3 C;
+    size_t nf = 0;
+    float dzi = 0, dxi = 0, dyi = 0;
+    // padded model value at a window cell (1-based, clamped as model_pad replicates edges)
+    float at(const std::vector<float> &m, int icz, int icx, int icy) const {
+        const int wz = std::min(std::max(icz, 1), g->nz) - 1, wx = std::min(std::max(icx, 1), nxw) - 1,
+                  wy = std::min(std::max(icy, 1), nyw) - 1;
+        return m[(size_t)wz + (size_t)colsrc[wx] * g->nz + (size_t)linesrc[wy] * g->nz * g->nx];
+    }
+};
+
+// A host injection plan: footprint cells and their contributions in trace order.
+struct InjPlan {
+    std::vector<size_t> cell;
+    std::vector<int> cptr;
+    std::vector<InjC> con;
+    std::vector<float> v2r;
+};
+
+// Injection_sinc's footprints (Injection.f90:181-187, :274-346): the 7x7x7
+// cells of every trace, with v2r = vel^2 (times rho) at the trace's cell of
+// the model `vm`; cells must lie in the FD region.
+static void plan_inj3(const Run3 &R, const std::vector<float> &vm, const std::vector<float> &rh, int n,
+                      const float *z, const float *x, const float *y, InjPlan &P) {
+    const sfd_grid3d &g = *R.g;
+    std::vector<std::pair<size_t, InjC>> all;
+    all.reserve((size_t)n * 343);
+    P.v2r.assign(std::max(n, 1), 0.f);
+    for (int r = 0; r < n; r++) {
+        int ic[3];
+        float dc[3], wz[7], wx[7];
+        find_i_d3(&ic[0], &dc[0], z[r], g.oz, g.dz);
+        find_i_d3(&ic[1], &dc[1], x[r], R.oxw, g.dx);
+        find_i_d3(&ic[2], &dc[2], y[r], R.oyw, g.dy);
+        const int pz = ic[0] - 1 + R.i0, px = ic[1] - 1 + R.i0, py = ic[2] - 1 + R.i0;
+        if (pz - 3 < 4 || pz + 3 > R.G.NZ - 5 || px - 3 < 4 || px + 3 > R.G.NX - 5 || py - 3 < 4 ||
+            py + 3 > R.G.NY - 5)
+            throw EngineError(SFD_EINVAL, "injection footprint leaves the FD region");
+        const float v = R.at(vm, ic[0], ic[1], ic[2]);
+        P.v2r[r] = v * v;
+        if (R.rho) P.v2r[r] = P.v2r[r] * R.at(rh, ic[0], ic[1], ic[2]);
+        sfd_host_mksinc(wz, 7, dc[0] / g.dz);
+        sfd_host_mksinc(wx, 7, dc[1] / g.dx);
+        for (int k = -3; k <= 3; k++)
+            for (int j = -3; j <= 3; j++)
+                for (int i = -3; i <= 3; i++)
+                    all.push_back({R.G.at(pz + i, px + j, py + k), InjC{r, wz[3 + i], wx[3 + j], wx[3 + k]}});
+    }
+    std::stable_sort(all.begin(), all.end(),
+                     [](const std::pair<size_t, InjC> &a, const std::pair<size_t, InjC> &b) { return a.first < b.first; });
+    P.cell.clear();
+    P.cptr.assign(1, 0);
+    P.con.clear();
+    for (size_t k = 0; k < all.size(); k++) {
+        if (k == 0 || all[k].first != all[k - 1].first) {
+            if (k) P.cptr.push_back((int)P.con.size());
+            P.cell.push_back(all[k].first);
+        }
+        P.con.push_back(all[k].second);
+    }
+    P.cptr.push_back((int)P.con.size());
+}
+
+static void check_shot3(const sfd_shot3d &s) {
     if (s.nt < 1 || s.ntr < 1 || !s.wavelet || !s.rz || !s.rx || !s.ry)
         throw EngineError(SFD_EINVAL, "3D shot needs nt >= 1, >= 1 receiver and its arrays");
-    float oxw, oyw;
-    int nxw, nyw;
-    window3(g, s, &oxw, &nxw, &oyw, &nyw);
-    if (nxw < 1 || nyw < 1) throw EngineError(SFD_EINVAL, "empty shot window");
-    Geo G;
+}
+
+// The shot's window and padded geometry, the window -> model maps and the
+// stencil coefficients (no device work).
+static void geom3(sfd_ctx3d &h, const sfd_shot3d &s, Run3 &R) {
+    const sfd_grid3d &g = h.g;
+    R.g = &g;
+    window3(g, s, &R.oxw, &R.nxw, &R.oyw, &R.nyw);
+    if (R.nxw < 1 || R.nyw < 1) throw EngineError(SFD_EINVAL, "empty shot window");
+    Geo &G = R.G;
     G.NZ = g.nz + 2 * g.ntaper + 8;
-    G.NX = nxw + 2 * g.ntaper + 8;
-    G.NY = nyw + 2 * g.ntaper + 8;
+    G.NX = R.nxw + 2 * g.ntaper + 8;
+    G.NY = R.nyw + 2 * g.ntaper + 8;
     G.LDZ = (G.NZ + 31) / 32 * 32;
     G.plane = (size_t)G.LDZ * G.NX;
-    const size_t nf = G.plane * G.NY;
-    const int i0 = 4 + g.ntaper;   // padded index of window cell 1 on every axis
+    R.nf = G.plane * G.NY;
+    R.i0 = 4 + g.ntaper;   // padded index of window cell 1 on every axis
+    R.fz = G.NZ - 8; R.fx = G.NX - 8; R.fy = G.NY - 8;
+    R.rho = !h.rho_h.empty();
     // window columns / lines -> model columns / lines (mod_window_pad, nearest, clamped)
-    std::vector<int> colsrc(nxw), linesrc(nyw);
-    for (int j = 1; j <= nxw; j++) {
-        int ix = nint3(((float)(j - 1) * g.dx + oxw - g.ox) / g.dx) + 1;
-        colsrc[j - 1] = std::min(std::max(ix, 1), g.nx) - 1;
+    R.colsrc.assign(R.nxw, 0);
+    R.linesrc.assign(R.nyw, 0);
+    for (int j = 1; j <= R.nxw; j++) {
+        const int ix = nint3(((float)(j - 1) * g.dx + R.oxw - g.ox) / g.dx) + 1;
+        R.colsrc[j - 1] = std::min(std::max(ix, 1), g.nx) - 1;
     }
-    for (int k = 1; k <= nyw; k++) {
-        int iy = nint3(((float)(k - 1) * g.dy + oyw - g.oy) / g.dy) + 1;
-        linesrc[k - 1] = std::min(std::max(iy, 1), g.ny) - 1;
+    for (int k = 1; k <= R.nyw; k++) {
+        const int iy = nint3(((float)(k - 1) * g.dy + R.oyw - g.oy) / g.dy) + 1;
+        R.linesrc[k - 1] = std::min(std::max(iy, 1), g.ny) - 1;
     }
-    const bool rho = !h.rho_h.empty();
-    auto model_at = [&](const std::vector<float> &m, int icz, int icx, int icy) {
-        // padded model value at a window cell (1-based, clamped as model_pad replicates edges)
-        const int wz = std::min(std::max(icz, 1), g.nz) - 1, wx = std::min(std::max(icx, 1), nxw) - 1,
-                  wy = std::min(std::max(icy, 1), nyw) - 1;
-        return m[(size_t)wz + (size_t)colsrc[wx] * g.nz + (size_t)linesrc[wy] * g.nz * g.nx];
-    };
-    auto vel_at = [&](int icz, int icx, int icy) { return model_at(h.vel_h, icz, icx, icy); };
-    // source (Injection_sinc: v2r = vel^2 at the source cell)
-    Inj3 I;
-    {
-        int ic[3];
-        float dc[3];
-        find_i_d3(&ic[0], &dc[0], s.sz, g.oz, g.dz);
-        find_i_d3(&ic[1], &dc[1], s.sx, oxw, g.dx);
-        find_i_d3(&ic[2], &dc[2], s.sy, oyw, g.dy);
-        I.iz = ic[0] - 1 + i0;
-        I.ix = ic[1] - 1 + i0;
-        I.iy = ic[2] - 1 + i0;
-        if (I.iz - 3 < 4 || I.iz + 3 > G.NZ - 5 || I.ix - 3 < 4 || I.ix + 3 > G.NX - 5 || I.iy - 3 < 4 ||
-            I.iy + 3 > G.NY - 5)
-            throw EngineError(SFD_EINVAL, "source footprint leaves the FD region");
-        const float v = vel_at(ic[0], ic[1], ic[2]);
-        I.v2r = v * v;   // Injection.f90:181-187: vel^2, times rho with a density
-        if (rho) I.v2r = I.v2r * model_at(h.rho_h, ic[0], ic[1], ic[2]);
-        sfd_host_mksinc(I.wz, 7, dc[0] / g.dz);
-        sfd_host_mksinc(I.wx, 7, dc[1] / g.dx);
+    R.dzi = 1.f / g.dz;
+    R.dxi = 1.f / g.dx;
+    R.dyi = 1.f / g.dy;
+    if (R.rho) {   // FDcoefs.f90:12-23 over delta (coefpower 1)
+        const float r[5] = {0.f, 1225.f / 1024.f, -(1225.f / (1024.f * 15.f)), 1225.f / (1024.f * 125.f),
+                            -(1225.f / (1024.f * 1715.f))};
+        for (int k = 0; k < 5; k++) {
+            R.C.c[k] = r[k] * R.dzi;
+            R.C.c[5 + k] = r[k] * R.dxi;
+            R.C.c[10 + k] = r[k] * R.dyi;
+        }
+    } else {   // FDcoefs.f90:25-36 over delta^2
+        const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
+        for (int k = 0; k < 5; k++) {
+            R.C.c[k] = cc[k] * (R.dzi * R.dzi);
+            R.C.c[5 + k] = cc[k] * (R.dxi * R.dxi);
+            R.C.c[10 + k] = cc[k] * (R.dyi * R.dyi);
+        }
     }
-    // receivers
-    std::vector<Rec3> R(s.ntr);
+}
+
+// Lay out the shot's device memory: five levels, models, Higdon tables, the
+// window maps, the density arrays, then `extra` bytes at *extra_off (the
+// caller's tables and outputs).
+static void alloc3(sfd_ctx3d &h, Run3 &R, size_t extra, size_t *extra_off) {
+    Layout L;
+    size_t ou[5];
+    for (int k = 0; k < 5; k++) ou[k] = L.take(R.nf * 4);
+    const size_t ovel = L.take(R.nf * 4), ov2 = L.take(R.nf * 4);
+    const size_t ogz = L.take((size_t)16 * R.fx * R.fy * 4), ogx = L.take((size_t)16 * R.fz * R.fy * 4),
+                 ogy = L.take((size_t)16 * R.fz * R.fx * 4);
+    const size_t ocol = L.take(R.nxw * 4), olin = L.take(R.nyw * 4);
+    size_t orho = 0, odelp = 0, otmp = 0, osg[3] = {0, 0, 0};
+    if (R.rho) {
+        orho = L.take(R.nf * 4);
+        odelp = L.take(R.nf * 4);
+        otmp = L.take(R.nf * 4);
+        for (int k = 0; k < 3; k++) osg[k] = L.take(R.nf * 4);
+    }
+    *extra_off = L.take(extra);
+    R.base = h.reserve(L.off);
+    for (int k = 0; k < 5; k++) R.u[k] = (float *)(R.base + ou[k]);
+    R.vel = (float *)(R.base + ovel);
+    R.v2 = (float *)(R.base + ov2);
+    R.gz = (float *)(R.base + ogz);
+    R.gx = (float *)(R.base + ogx);
+    R.gy = (float *)(R.base + ogy);
+    R.dcol = (const int *)(R.base + ocol);
+    R.dlin = (const int *)(R.base + olin);
+    HIPCHK(hipMemcpyAsync(R.base + ocol, R.colsrc.data(), R.nxw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(R.base + olin, R.linesrc.data(), R.nyw * 4, hipMemcpyHostToDevice, h.st));
+    if (R.rho) {
+        R.rhop = (float *)(R.base + orho);
+        R.delp = (float *)(R.base + odelp);
+        R.tmp = (float *)(R.base + otmp);
+        for (int k = 0; k < 3; k++) R.sg[k] = (float *)(R.base + osg[k]);
+    }
+}
+
+// The pass's model: padded vel and v^2, Higdon tables (Propagator.f90:152-154
+// computes them at every call from the model it is given).
+static void models3(sfd_ctx3d &h, Run3 &R, const float *d_model) {
+    const sfd_grid3d &g = h.g;
+    const Geo &G = R.G;
+    const dim3 pg((G.NZ + 255) / 256, G.NX, G.NY);
+    hipLaunchKernelGGL(pad3_kernel, pg, dim3(256), 0, h.st, d_model, g.nz, g.nx, R.dcol, R.nxw, R.dlin, R.nyw,
+                       g.ntaper, G, R.vel, R.v2);
+    const int mx = std::max(R.fx, R.fz);
+    hipLaunchKernelGGL(higdon3_kernel, dim3((mx + 255) / 256, std::max(R.fy, R.fx), 3), dim3(256), 0, h.st,
+                       (const float *)R.vel, G, g.dt, g.dz, g.dx, g.dy, R.gz, R.gx, R.gy);
+    HIPCHK(hipGetLastError());
+}
+
+// A new propagator call: zeroed levels (and gradient arrays); with a density
+// the padded rho and 1/rho2 (Interpolate: z pass rho -> delp, x pass delp ->
+// tmp, y pass tmp -> delp, stored as 1/rho2).
+static void start_pass3(sfd_ctx3d &h, Run3 &R, bool density_tables) {
+    for (int k = 0; k < 5; k++) HIPCHK(hipMemsetAsync(R.u[k], 0, R.nf * 4, h.st));
+    if (!R.rho) return;
+    for (int k = 0; k < 3; k++) HIPCHK(hipMemsetAsync(R.sg[k], 0, R.nf * 4, h.st));
+    if (!density_tables) return;
+    const sfd_grid3d &g = h.g;
+    const Geo &G = R.G;
+    const dim3 pg((G.NZ + 255) / 256, G.NX, G.NY);
+    hipLaunchKernelGGL(pad3_kernel, pg, dim3(256), 0, h.st, (const float *)h.d_rho, g.nz, g.nx, R.dcol, R.nxw, R.dlin,
+                       R.nyw, g.ntaper, G, R.rhop, (float *)nullptr);
+    Sinc9 S9;
+    sfd_host_mksinc(S9.w, 9, 0.5f);
+    hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)R.rhop, R.delp, G, S9, 0, 0);
+    hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)R.delp, R.tmp, G, S9, 1, 0);
+    hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)R.tmp, R.delp, G, S9, 2, 1);
+    HIPCHK(hipGetLastError());
+}
+
+// Device copy of an injection plan in `mem` (laid out by the caller).
+struct InjDev {
+    const size_t *cell = nullptr;
+    const int *cptr = nullptr;
+    const InjC *con = nullptr;
+    const float *v2r = nullptr, *traces = nullptr;
+    int ncell = 0, nt = 0;
+};
+static size_t inj_bytes(const InjPlan &P, size_t ntrace_samples) {
+    Layout L;
+    L.take(P.cell.size() * sizeof(size_t));
+    L.take(P.cptr.size() * 4);
+    L.take(P.con.size() * sizeof(InjC));
+    L.take(P.v2r.size() * 4);
+    L.take(ntrace_samples * 4);
+    return L.off + 256;
+}
+static InjDev upload_inj(sfd_ctx3d &h, char *mem, const InjPlan &P, const float *traces, int ntraces, int nt) {
+    Layout L;
+    const size_t o0 = L.take(P.cell.size() * sizeof(size_t)), o1 = L.take(P.cptr.size() * 4),
+                 o2 = L.take(P.con.size() * sizeof(InjC)), o3 = L.take(P.v2r.size() * 4),
+                 o4 = L.take((size_t)ntraces * nt * 4);
+    HIPCHK(hipMemcpyAsync(mem + o0, P.cell.data(), P.cell.size() * sizeof(size_t), hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(mem + o1, P.cptr.data(), P.cptr.size() * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(mem + o2, P.con.data(), P.con.size() * sizeof(InjC), hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(mem + o3, P.v2r.data(), P.v2r.size() * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(mem + o4, traces, (size_t)ntraces * nt * 4, hipMemcpyHostToDevice, h.st));
+    InjDev D;
+    D.cell = (const size_t *)(mem + o0);
+    D.cptr = (const int *)(mem + o1);
+    D.con = (const InjC *)(mem + o2);
+    D.v2r = (const float *)(mem + o3);
+    D.traces = (const float *)(mem + o4);
+    D.ncell = (int)P.cell.size();
+    D.nt = nt;
+    return D;
+}
+
+// Stencil, injection, time update and the Higdon bands of one step, then the
+// level rotation (Propagator.f90:199-243).
+static void step3(sfd_ctx3d &h, Run3 &R, float **buf, const InjDev &I, int it) {
+    const Geo &G = R.G;
+    const dim3 sg((R.fz + 63) / 64, (R.fx + 3) / 4, R.fy), sb(256);
+    if (R.rho) {
+        hipLaunchKernelGGL(rho_grad3_kernel, dim3((G.NZ + 63) / 64, (G.NX + 3) / 4, G.NY), sb, 0, h.st,
+                           (const float *)buf[3], (const float *)R.delp, R.sg[0], R.sg[1], R.sg[2], G, R.C, R.dzi,
+                           R.dxi, R.dyi);
+        hipLaunchKernelGGL(stencil3_rho_kernel, sg, sb, 0, h.st, (const float *)R.sg[0], (const float *)R.sg[1],
+                           (const float *)R.sg[2], (const float *)R.v2, (const float *)R.rhop, buf[4], G, R.C);
+    } else {
+        hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)R.v2, buf[4], G,
+                           R.C);
+    }
+    if (I.ncell > 0)
+        hipLaunchKernelGGL(inject3_kernel, dim3((I.ncell + 255) / 256), dim3(256), 0, h.st, buf[4], I.cell, I.cptr,
+                           I.con, I.v2r, I.traces, I.nt, I.ncell, it);
+    const float c12dt2 = 12.f * (h.g.dt * h.g.dt), div11 = 1.f / 11.f;
+    hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
+                       (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
+    hipLaunchKernelGGL(zband3_kernel, dim3((R.fx + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
+                       (const float *)buf[2], (const float *)R.gz, G);
+    hipLaunchKernelGGL(xband3_kernel, dim3((R.fz + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
+                       (const float *)buf[2], (const float *)R.gx, G);
+    hipLaunchKernelGGL(yband3_kernel, dim3((R.fz + 63) / 64, R.fx), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
+                       (const float *)buf[2], (const float *)R.gy, G);
+    float *t = buf[0];
+    buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+}
+
+// readsou with withRho (ReadSouVelRho.f90:328-347): the source shifted half a time step.
+static std::vector<float> source3(const sfd_ctx3d &h, const sfd_shot3d &s) {
+    std::vector<float> w(s.wavelet, s.wavelet + s.nt);
+    if (h.rho_h.empty()) return w;
+    float sinc[64];
+    const int hl = h.g.lsinc / 2;
+    sfd_host_mksinc(sinc, h.g.lsinc, 0.5f);
+    for (int it = 1; it <= s.nt; it++) {
+        float fscale = 1.f, fdum = 0.f;
+        for (int l = -hl; l <= hl; l++) {
+            if (it + l >= 1 && it + l <= s.nt) fdum = fdum + sinc[hl + l] * s.wavelet[it + l - 1];
+            else fscale = fscale - sinc[hl + l];
+        }
+        w[it - 1] = fdum / fscale;
+    }
+    return w;
+}
+
+static void forward3(sfd_ctx3d &h, const sfd_shot3d &s, float *traces_out, int kind) {
+    const sfd_grid3d &g = h.g;
+    check_shot3(s);
+    Run3 R;
+    geom3(h, s, R);
+    InjPlan ip;
+    plan_inj3(R, h.vel_h, h.rho_h, 1, &s.sz, &s.sx, &s.sy, ip);
+    // receivers (Extraction.f90:71-162,217-246 -> :276-337)
+    std::vector<Rec3> rec(s.ntr);
+    const size_t trace_bytes = (size_t)s.nt * s.ntr * 4;
+    const size_t rec_bytes = sizeof(Rec3) * s.ntr;
     for (int r = 0; r < s.ntr; r++) {
         int ic[3];
         float dc[3];
         find_i_d3(&ic[0], &dc[0], s.rz[r], g.oz, g.dz);
-        find_i_d3(&ic[1], &dc[1], s.rx[r], oxw, g.dx);
-        find_i_d3(&ic[2], &dc[2], s.ry[r], oyw, g.dy);
-        Rec3 &e = R[r];
-        e.iz = ic[0] - 1 + i0;
-        e.ix = ic[1] - 1 + i0;
-        e.iy = ic[2] - 1 + i0;
-        if (e.iz - 3 < 0 || e.iz + 3 >= G.NZ || e.ix - 3 < 0 || e.ix + 3 >= G.NX || e.iy - 3 < 0 ||
-            e.iy + 3 >= G.NY)
+        find_i_d3(&ic[1], &dc[1], s.rx[r], R.oxw, g.dx);
+        find_i_d3(&ic[2], &dc[2], s.ry[r], R.oyw, g.dy);
+        Rec3 &e = rec[r];
+        e.iz = ic[0] - 1 + R.i0;
+        e.ix = ic[1] - 1 + R.i0;
+        e.iy = ic[2] - 1 + R.i0;
+        if (e.iz - 3 < 0 || e.iz + 3 >= R.G.NZ || e.ix - 3 < 0 || e.ix + 3 >= R.G.NX || e.iy - 3 < 0 ||
+            e.iy + 3 >= R.G.NY)
             throw EngineError(SFD_EINVAL, "receiver outside the padded grid");
-        e.gain = kind == SFD_EXTRACT_SINC_V2 ? vel_at(ic[0], ic[1], ic[2])
+        e.gain = kind == SFD_EXTRACT_SINC_V2 ? R.at(h.vel_h, ic[0], ic[1], ic[2])
                  : kind == SFD_EXTRACT_SINC_LSRTM ? 1.f / (g.dz * g.dx * g.dy)
                                                   : 1.f;
         sfd_host_mksinc(e.wz, 7, dc[0] / g.dz);
         sfd_host_mksinc(e.wx, 7, dc[1] / g.dx);
     }
-    // device memory: 5 levels, vel, v^2, Higdon tables, window maps, receivers, traces, wavelet
-    const int fz = G.NZ - 8, fx = G.NX - 8, fy = G.NY - 8;
-    Layout L;
-    size_t ou[5];
-    for (int k = 0; k < 5; k++) ou[k] = L.take(nf * 4);
-    const size_t ovel = L.take(nf * 4), ov2 = L.take(nf * 4);
-    const size_t ogz = L.take((size_t)16 * fx * fy * 4), ogx = L.take((size_t)16 * fz * fy * 4),
-                 ogy = L.take((size_t)16 * fz * fx * 4);
-    const size_t ocol = L.take(nxw * 4), olin = L.take(nyw * 4), orec = L.take(sizeof(Rec3) * s.ntr);
-    const size_t otr = L.take((size_t)s.nt * s.ntr * 4), owav = L.take((size_t)s.nt * 4);
-    size_t orho = 0, odelp = 0, otmp = 0, osg[3] = {0, 0, 0};
-    if (rho) {
-        orho = L.take(nf * 4);
-        odelp = L.take(nf * 4);
-        otmp = L.take(nf * 4);
-        for (int k = 0; k < 3; k++) osg[k] = L.take(nf * 4);
-    }
-    char *base = h.reserve(L.off);
-    float *u[5];
-    for (int k = 0; k < 5; k++) {
-        u[k] = (float *)(base + ou[k]);
-        HIPCHK(hipMemsetAsync(u[k], 0, nf * 4, h.st));
-    }
-    float *vel = (float *)(base + ovel), *v2 = (float *)(base + ov2);
-    float *gz = (float *)(base + ogz), *gx = (float *)(base + ogx), *gy = (float *)(base + ogy);
-    float *trc = (float *)(base + otr), *wav = (float *)(base + owav);
-    HIPCHK(hipMemsetAsync(trc, 0, (size_t)s.nt * s.ntr * 4, h.st));
-    HIPCHK(hipMemcpyAsync(base + ocol, colsrc.data(), nxw * 4, hipMemcpyHostToDevice, h.st));
-    HIPCHK(hipMemcpyAsync(base + olin, linesrc.data(), nyw * 4, hipMemcpyHostToDevice, h.st));
-    HIPCHK(hipMemcpyAsync(base + orec, R.data(), sizeof(Rec3) * s.ntr, hipMemcpyHostToDevice, h.st));
-    // readsou with withRho (ReadSouVelRho.f90:328-347): the source shifted half a time step
-    std::vector<float> wshift;
-    if (rho) {
-        wshift.assign(s.nt, 0.f);
-        float w[64];
-        const int hl = g.lsinc / 2;
-        sfd_host_mksinc(w, g.lsinc, 0.5f);
-        for (int it = 1; it <= s.nt; it++) {
-            float fscale = 1.f, fdum = 0.f;
-            for (int l = -hl; l <= hl; l++) {
-                if (it + l >= 1 && it + l <= s.nt) fdum = fdum + w[hl + l] * s.wavelet[it + l - 1];
-                else fscale = fscale - w[hl + l];
-            }
-            wshift[it - 1] = fdum / fscale;
-        }
-    }
-    HIPCHK(hipMemcpyAsync(wav, rho ? wshift.data() : s.wavelet, (size_t)s.nt * 4, hipMemcpyHostToDevice, h.st));
-    const dim3 pg((G.NZ + 255) / 256, G.NX, G.NY);
-    hipLaunchKernelGGL(pad3_kernel, pg, dim3(256), 0, h.st, (const float *)h.d_vel, g.nz, g.nx,
-                       (const int *)(base + ocol), nxw, (const int *)(base + olin), nyw, g.ntaper, G, vel, v2);
-    float *rhop = nullptr, *delp = nullptr, *sgz = nullptr, *sgx = nullptr, *sgy = nullptr;
-    if (rho) {
-        rhop = (float *)(base + orho);
-        delp = (float *)(base + odelp);
-        float *tmp = (float *)(base + otmp);
-        sgz = (float *)(base + osg[0]);
-        sgx = (float *)(base + osg[1]);
-        sgy = (float *)(base + osg[2]);
-        for (int k = 0; k < 3; k++) HIPCHK(hipMemsetAsync(base + osg[k], 0, nf * 4, h.st));
-        hipLaunchKernelGGL(pad3_kernel, pg, dim3(256), 0, h.st, (const float *)h.d_rho, g.nz, g.nx,
-                           (const int *)(base + ocol), nxw, (const int *)(base + olin), nyw, g.ntaper, G, rhop,
-                           (float *)nullptr);
-        Sinc9 S9;
-        sfd_host_mksinc(S9.w, 9, 0.5f);
-        // Interpolate: z pass rho -> delp, x pass delp -> tmp, y pass tmp -> delp (stored as 1 / rho2)
-        hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)rhop, delp, G, S9, 0, 0);
-        hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)delp, tmp, G, S9, 1, 0);
-        hipLaunchKernelGGL(rho_half3_kernel, pg, dim3(256), 0, h.st, (const float *)tmp, delp, G, S9, 2, 1);
-    }
-    const int mx = std::max(fx, fz);
-    hipLaunchKernelGGL(higdon3_kernel, dim3((mx + 255) / 256, std::max(fy, fx), 3), dim3(256), 0, h.st,
-                       (const float *)vel, G, g.dt, g.dz, g.dx, g.dy, gz, gx, gy);
-    Coef3 C;
-    const float dzi = 1.f / g.dz, dxi = 1.f / g.dx, dyi = 1.f / g.dy;
-    if (rho) {   // FDcoefs.f90:12-23 over delta (coefpower 1)
-        const float r[5] = {0.f, 1225.f / 1024.f, -(1225.f / (1024.f * 15.f)), 1225.f / (1024.f * 125.f),
-                            -(1225.f / (1024.f * 1715.f))};
-        for (int k = 0; k < 5; k++) {
-            C.c[k] = r[k] * dzi;
-            C.c[5 + k] = r[k] * dxi;
-            C.c[10 + k] = r[k] * dyi;
-        }
-    } else {   // FDcoefs.f90:25-36 over delta^2
-        const float cc[5] = {-205.f / 72.f, 8.f / 5, -1.f / 5.f, 8.f / 315.f, -1.f / 560.f};
-        for (int k = 0; k < 5; k++) {
-            C.c[k] = cc[k] * (dzi * dzi);
-            C.c[5 + k] = cc[k] * (dxi * dxi);
-            C.c[10 + k] = cc[k] * (dyi * dyi);
-        }
-    }
-    const float c12dt2 = 12.f * (g.dt * g.dt), div11 = 1.f / 11.f;
-    const dim3 sg((fz + 63) / 64, (fx + 3) / 4, fy), sb(256);
+    const size_t extra = trace_bytes + 256 + rec_bytes + 256 + inj_bytes(ip, s.nt);
+    size_t extra_off = 0;
+    alloc3(h, R, extra, &extra_off);
+    char *ex = R.base + extra_off;
+    float *trc = (float *)ex;
+    Rec3 *drec = (Rec3 *)(ex + ((trace_bytes + 255) & ~(size_t)255));
+    char *injmem = (char *)drec + ((rec_bytes + 255) & ~(size_t)255);
+    HIPCHK(hipMemsetAsync(trc, 0, trace_bytes, h.st));
+    HIPCHK(hipMemcpyAsync(drec, rec.data(), rec_bytes, hipMemcpyHostToDevice, h.st));
+    const std::vector<float> wav = source3(h, s);
+    const InjDev I = upload_inj(h, injmem, ip, wav.data(), 1, s.nt);
+    models3(h, R, h.d_vel);
+    start_pass3(h, R, true);
     const int nearest = kind == SFD_EXTRACT_NEAREST;
-    // buf[0] = u_-1, buf[1] = u0, buf[2] = u1, buf[3] = u2, buf[4] = u3 (FDswaptimes.f90:14-24)
-    float *buf[5] = {u[0], u[1], u[2], u[3], u[4]};
+    float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};   // u_-1, u0, u1, u2, u3 (FDswaptimes.f90:14-24)
     for (int it = 1; it <= s.nt; it++) {
-        hipLaunchKernelGGL(extract3_kernel, dim3((s.ntr + 63) / 64), dim3(64), 0, h.st, (const float *)buf[3], G,
-                           (const Rec3 *)(base + orec), s.ntr, nearest, trc, s.nt, it);
-        if (rho) {
-            hipLaunchKernelGGL(rho_grad3_kernel, dim3((G.NZ + 63) / 64, (G.NX + 3) / 4, G.NY), sb, 0, h.st,
-                               (const float *)buf[3], (const float *)delp, sgz, sgx, sgy, G, C, dzi, dxi, dyi);
-            hipLaunchKernelGGL(stencil3_rho_kernel, sg, sb, 0, h.st, (const float *)sgz, (const float *)sgx,
-                               (const float *)sgy, (const float *)v2, (const float *)rhop, buf[4], G, C);
-        } else {
-            hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)v2, buf[4], G,
-                               C);
-        }
-        hipLaunchKernelGGL(inject3_kernel, dim3(1), dim3(384), 0, h.st, buf[4], G, I, (const float *)wav, it);
-        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
-                           (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
-        hipLaunchKernelGGL(zband3_kernel, dim3((fx + 63) / 64, fy), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
-                           (const float *)buf[2], (const float *)gz, G);
-        hipLaunchKernelGGL(xband3_kernel, dim3((fz + 63) / 64, fy), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
-                           (const float *)buf[2], (const float *)gx, G);
-        hipLaunchKernelGGL(yband3_kernel, dim3((fz + 63) / 64, fx), dim3(64), 0, h.st, buf[4], (const float *)buf[3],
-                           (const float *)buf[2], (const float *)gy, G);
-        float *t = buf[0];
-        buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+        hipLaunchKernelGGL(extract3_kernel, dim3((s.ntr + 63) / 64), dim3(64), 0, h.st, (const float *)buf[3], R.G,
+                           (const Rec3 *)drec, s.ntr, nearest, trc, s.nt, it);
+        step3(h, R, buf, I, it);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(traces_out, trc, (size_t)s.nt * s.ntr * 4, hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipMemcpyAsync(traces_out, trc, trace_bytes, hipMemcpyDeviceToHost, h.st));
     HIPCHK(hipStreamSynchronize(h.st));
+}
+
+// RTM_to_memory with genpar%twoD false (to_disk_to_memory_RTM.f90:358-492).
+static void rtm3(sfd_ctx3d &h, const sfd_shot3d &s, const float *traces_in, float *image, float *illum) {
+    const sfd_grid3d &g = h.g;
+    check_shot3(s);
+    if (g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
+    Run3 R;
+    geom3(h, s, R);
+    InjPlan src, recp;
+    plan_inj3(R, h.vel_h, h.rho_h, 1, &s.sz, &s.sx, &s.sy, src);
+    plan_inj3(R, h.velb_h.empty() ? h.vel_h : h.velb_h, h.rho_h, s.ntr, s.rz, s.rx, s.ry, recp);
+    const int ntsnap = s.nt / g.snapi;
+    const size_t nwin = (size_t)g.nz * R.nxw * R.nyw;
+    Layout E;
+    const size_t o_snap = E.take(nwin * std::max(ntsnap, 1) * 4), o_img = E.take(nwin * 4), o_ill = E.take(nwin * 4),
+                 o_tx = E.take(R.nxw * 4), o_ty = E.take(R.nyw * 4), o_src = E.take(inj_bytes(src, s.nt)),
+                 o_rec = E.take(inj_bytes(recp, (size_t)s.ntr * s.nt));
+    size_t extra_off = 0;
+    alloc3(h, R, E.off, &extra_off);
+    char *ex = R.base + extra_off;
+    float *snaps = (float *)(ex + o_snap), *img = (float *)(ex + o_img), *ill = (float *)(ex + o_ill);
+    HIPCHK(hipMemsetAsync(img, 0, nwin * 4, h.st));
+    HIPCHK(hipMemsetAsync(ill, 0, nwin * 4, h.st));
+    std::vector<float> tx(R.nxw), ty(R.nyw, 1.f);
+    sfd_host_taper(tx.data(), R.nxw);
+    if (R.nyw > 1) sfd_host_taper(ty.data(), R.nyw);
+    HIPCHK(hipMemcpyAsync(ex + o_tx, tx.data(), R.nxw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(ex + o_ty, ty.data(), R.nyw * 4, hipMemcpyHostToDevice, h.st));
+    const std::vector<float> wav = source3(h, s);
+    const InjDev Is = upload_inj(h, ex + o_src, src, wav.data(), 1, s.nt);
+    const InjDev Ir = upload_inj(h, ex + o_rec, recp, traces_in, s.ntr, s.nt);
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw);
+    // forward: snapshots every snapi steps
+    models3(h, R, h.d_vel);
+    start_pass3(h, R, true);
+    float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
+    int counter = 0;
+    for (int it = 1; it <= s.nt; it++) {
+        if (it % g.snapi == 0)
+            hipLaunchKernelGGL(snap3_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3], R.G, g.nz, R.nxw, R.i0,
+                               snaps + (size_t)(counter++) * nwin);
+        step3(h, R, buf, Is, it);
+    }
+    // backward: a new propagator call on the backward model, receivers as sources
+    if (h.d_velb) models3(h, R, h.d_velb);
+    start_pass3(h, R, false);
+    for (int k = 0; k < 5; k++) buf[k] = R.u[k];
+    counter = 0;
+    for (int it = s.nt; it >= 1; it--) {
+        if (it % g.snapi == 0) {
+            counter++;
+            const int index = ntsnap - counter + 1;
+            hipLaunchKernelGGL(image3_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3], R.G, g.nz, R.nxw, R.i0,
+                               (const float *)(snaps + (size_t)(index - 1) * nwin), (const float *)(ex + o_tx),
+                               (const float *)(ex + o_ty), img, ill);
+        }
+        step3(h, R, buf, Ir, it);
+    }
+    HIPCHK(hipGetLastError());
+    // mod_copy_image (ExtractPadModel.f90:662-695): a zeroed model-size sum of the window, then added
+    std::vector<float> wimg(nwin), will(nwin);
+    HIPCHK(hipMemcpyAsync(wimg.data(), img, nwin * 4, hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipMemcpyAsync(will.data(), ill, nwin * 4, hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipStreamSynchronize(h.st));
+    const size_t N = (size_t)g.nz * g.nx * g.ny;
+    for (int pass = 0; pass < 2; pass++) {
+        float *dst = pass == 0 ? image : illum;
+        const std::vector<float> &w = pass == 0 ? wimg : will;
+        if (!dst) continue;
+        std::vector<float> tmp(N, 0.f);
+        for (int k = 1; k <= R.nyw; k++) {
+            const int iy = nint3(((float)(k - 1) * g.dy + R.oyw - g.oy) / g.dy) + 1;
+            if (iy < 1 || iy > g.ny) continue;
+            for (int j = 1; j <= R.nxw; j++) {
+                const int ix = nint3(((float)(j - 1) * g.dx + R.oxw - g.ox) / g.dx) + 1;
+                if (ix < 1 || ix > g.nx) continue;
+                float *d = tmp.data() + (size_t)(ix - 1) * g.nz + (size_t)(iy - 1) * g.nz * g.nx;
+                const float *ww = w.data() + (size_t)(j - 1) * g.nz + (size_t)(k - 1) * g.nz * R.nxw;
+                for (int i = 0; i < g.nz; i++) d[i] = d[i] + ww[i];
+            }
+        }
+        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + tmp[q];
+    }
 }
 
 static std::string g_create_err3d;   // message of the last failed sfd_create3d
@@ -602,16 +865,29 @@ int sfd_destroy3d(sfd_handle3d h) {
 
 const char *sfd_last_error3d(sfd_handle3d h) { return h ? h->err.c_str() : g_create_err3d.c_str(); }
 
-int sfd_set_model3d(sfd_handle3d h, const float *vel) {
+int sfd_set_model3d(sfd_handle3d h, const float *vel, const float *vel_bwd) {
     if (!h || !vel) return SFD_EINVAL;
     return abi_call(&h->err, [&]() {
         const size_t n = (size_t)h->g.nz * h->g.nx * h->g.ny;
-        for (size_t i = 0; i < n; i++)
-            if (!(vel[i] > 0.f) || !std::isfinite(vel[i])) throw EngineError(SFD_EINVAL, "velocity must be finite and > 0");
-        vel_check3(h->g, vel, n);
+        for (int m = 0; m < 2; m++) {
+            const float *v = m == 0 ? vel : vel_bwd;
+            if (!v) continue;
+            for (size_t i = 0; i < n; i++)
+                if (!(v[i] > 0.f) || !std::isfinite(v[i])) throw EngineError(SFD_EINVAL, "velocity must be finite and > 0");
+            vel_check3(h->g, v, n);
+        }
         h->vel_h.assign(vel, vel + n);
         if (!h->d_vel) HIPCHK(hipMalloc(&h->d_vel, n * sizeof(float)));
         HIPCHK(hipMemcpy(h->d_vel, vel, n * sizeof(float), hipMemcpyHostToDevice));
+        if (vel_bwd) {
+            h->velb_h.assign(vel_bwd, vel_bwd + n);
+            if (!h->d_velb) HIPCHK(hipMalloc(&h->d_velb, n * sizeof(float)));
+            HIPCHK(hipMemcpy(h->d_velb, vel_bwd, n * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            h->velb_h.clear();
+            if (h->d_velb) (void)hipFree(h->d_velb);
+            h->d_velb = nullptr;
+        }
     });
 }
 
@@ -639,6 +915,19 @@ int sfd_forward3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, float *tr
         size_t off = 0;
         for (int s = 0; s < nshots; s++) {
             forward3(*h, shots[s], traces_out + off, extract_kind);
+            off += (size_t)shots[s].nt * shots[s].ntr;
+        }
+    });
+}
+
+int sfd_rtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *traces_in, float *image,
+              float *illum) {
+    if (!h || !shots || nshots < 1 || !traces_in || (!image && !illum)) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        if (h->vel_h.empty()) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model3d first");
+        size_t off = 0;
+        for (int s = 0; s < nshots; s++) {
+            rtm3(*h, shots[s], traces_in + off, image, illum);
             off += (size_t)shots[s].nt * shots[s].ntr;
         }
     });

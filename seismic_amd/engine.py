@@ -317,13 +317,14 @@ class Engine:
 
 
 def make_grid3d(nz, nx, ny, dz, dx, dy, dt, ntaper=20, lsinc=7, oz=0.0, ox=0.0, oy=0.0, aperture_x=0.0,
-                aperture_y=0.0, fmax=30.0):
+                aperture_y=0.0, fmax=30.0, snapi=1):
     """sfd_grid3d: model (nz, nx, ny) at (oz, ox, oy) with spacing (dz, dx, dy)."""
     g = abi.sfd_grid3d()
     g.nz, g.nx, g.ny = nz, nx, ny
     g.oz, g.ox, g.oy, g.dz, g.dx, g.dy, g.dt = oz, ox, oy, dz, dx, dy, dt
     g.ntaper, g.lsinc = ntaper, lsinc
     g.aperture_x, g.aperture_y, g.fmax = aperture_x, aperture_y, fmax
+    g.snapi = snapi
     return g
 
 
@@ -374,13 +375,29 @@ class Engine3D:
         if rc != 0:
             raise SeisFDError(rc, self.lib.sfd_last_error3d(self.h).decode())
 
-    def set_model(self, vel):
-        """vel (nz, nx, ny)."""
-        v = np.asarray(vel, dtype=np.float32)
+    def _model(self, m, what):
+        v = np.asarray(m, dtype=np.float32)
         if v.shape != (self.g.nz, self.g.nx, self.g.ny):
-            raise ValueError("model shape %s != (%d, %d, %d)" % (v.shape, self.g.nz, self.g.nx, self.g.ny))
-        self._vel = np.ascontiguousarray(v.transpose(2, 1, 0))
-        self._chk(self.lib.sfd_set_model3d(self.h, _ptr(self._vel)))
+            raise ValueError("%s shape %s != (%d, %d, %d)" % (what, v.shape, self.g.nz, self.g.nx, self.g.ny))
+        return np.ascontiguousarray(v.transpose(2, 1, 0))
+
+    def set_model(self, vel, vel_bwd=None):
+        """vel (nz, nx, ny); vel_bwd: RTM's backward-pass model (mod%vel2)."""
+        self._vel = self._model(vel, "model")
+        self._velb = None if vel_bwd is None else self._model(vel_bwd, "backward model")
+        self._chk(self.lib.sfd_set_model3d(self.h, _ptr(self._vel), _ptr(self._velb)))
+
+    def rtm(self, shots, traces, image=None, illum=None):
+        """3D RTM; traces per shot (ntr, nt); returns (image, illum) (nz, nx, ny):
+        new arrays holding image / illum (when given; they are not modified)
+        plus this call's images, as the library accumulates."""
+        n = (self.g.nz, self.g.nx, self.g.ny)
+        img = np.zeros(n[::-1], np.float32) if image is None else self._model(image, "image").copy()
+        ill = np.zeros(n[::-1], np.float32) if illum is None else self._model(illum, "illum").copy()
+        arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])
+        tr = Engine._cat(traces, shots)
+        self._chk(self.lib.sfd_rtm3d(self.h, arr, len(shots), _ptr(tr), _ptr(img), _ptr(ill)))
+        return img.transpose(2, 1, 0), ill.transpose(2, 1, 0)
 
     def set_density(self, rho=None):
         """Variable density rho (nz, nx, ny) (genpar%withRho); None: constant density."""

@@ -40,6 +40,7 @@ module seisfd_mod
      real(c_float)      :: oz, ox, oy, dz, dx, dy, dt
      integer(c_int32_t) :: ntaper, lsinc
      real(c_float)      :: aperture_x, aperture_y, fmax
+     integer(c_int32_t) :: snapi = 1
   end type sfd_grid3d
 
   type, bind(C) :: sfd_shot3d
@@ -159,11 +160,21 @@ module seisfd_mod
        type(c_ptr), value :: h
      end function sfd_last_error3d
 
-     integer(c_int) function sfd_set_model3d(h, vel) bind(C, name='sfd_set_model3d')
+     integer(c_int) function sfd_set_model3d(h, vel, vel_bwd) bind(C, name='sfd_set_model3d')
        import :: c_int, c_ptr, c_float
        type(c_ptr), value :: h
        real(c_float), intent(in) :: vel(*)
+       type(c_ptr), value :: vel_bwd   ! c_loc of the backward model, or c_null_ptr
      end function sfd_set_model3d
+
+     integer(c_int) function sfd_rtm3d(h, shots, nshots, traces, image, illum) bind(C, name='sfd_rtm3d')
+       import :: c_int, c_ptr, c_float, sfd_shot3d
+       type(c_ptr), value :: h
+       type(sfd_shot3d), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: traces(*)
+       real(c_float), intent(inout) :: image(*), illum(*)   ! (nz, nx, ny), accumulated
+     end function sfd_rtm3d
 
      integer(c_int) function sfd_set_density3d(h, rho) bind(C, name='sfd_set_density3d')
        import :: c_int, c_ptr

@@ -699,8 +699,9 @@ class ORC_B3(C.Structure):
                 ("nmin3", C.c_int), ("nmax3", C.c_int), ("NZ", C.c_int), ("NX", C.c_int), ("NY", C.c_int)]
 
 
-def make_grid3d(nz, nx, ny, dz, dx, dy, dt, ntaper=12, lsinc=7, oz=0.0, ox=0.0, oy=0.0, apx=0.0, apy=0.0):
+def make_grid3d(nz, nx, ny, dz, dx, dy, dt, ntaper=12, lsinc=7, oz=0.0, ox=0.0, oy=0.0, apx=0.0, apy=0.0, snapi=1):
     g = sfd_grid3d()
+    g.snapi = snapi
     g.nz, g.nx, g.ny = nz, nx, ny
     g.oz, g.ox, g.oy, g.dz, g.dx, g.dy, g.dt = oz, ox, oy, dz, dx, dy, dt
     g.ntaper, g.lsinc = ntaper, lsinc
@@ -773,28 +774,64 @@ def rho_half3d(g, rho_pad, b):
     return out
 
 
-def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None):
+def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None, sources=None, tdir=1, snapi=1,
+                    nsnap=0):
     """rho_pad: padded density (withRho); the wavelet is passed as given
-    (readsou's half-step shift is the caller's, as in the drivers)."""
+    (readsou's half-step shift is the caller's, as in the drivers).
+    sources: (z, x, y, traces (n, nt)) injected instead of the shot's source
+    (RTM's receivers); tdir -1 runs reverse time; nsnap > 0 also returns u2
+    on the window at every snapi-th step (Extraction_wavefield), in the order
+    taken: (traces, waves [nsnap, nyw, nxw, nz])."""
     oxw, nxw, oyw, nyw = win
     fn = ref().ref_propagate3d
     fn.argtypes = ([C.c_int] * 4 + [C.c_float] * 7 + [C.c_int] + [C.c_void_p] + [C.c_int] + [C.c_void_p] * 4
-                   + [C.c_int] * 2 + [C.c_void_p] * 4 + [C.c_int] * 2 + [C.c_void_p])
-    sz = np.array([shot.sz], np.float32)
-    sx = np.array([shot.sx], np.float32)
-    sy = np.array([shot.sy], np.float32)
-    w = shot.wavelet if wavelet is None else wavelet
-    st = np.ascontiguousarray(np.asarray(w, np.float32)[None, :])
+                   + [C.c_int] * 2 + [C.c_void_p] * 4 + [C.c_int] * 2 + [C.c_void_p] + [C.c_int] * 3 + [C.c_void_p])
+    if sources is None:
+        sz = np.array([shot.sz], np.float32)
+        sx = np.array([shot.sx], np.float32)
+        sy = np.array([shot.sy], np.float32)
+        w = shot.wavelet if wavelet is None else wavelet
+        st = np.ascontiguousarray(np.asarray(w, np.float32)[None, :])
+    else:
+        sz, sx, sy = (np.ascontiguousarray(a, np.float32) for a in sources[:3])
+        st = np.ascontiguousarray(sources[3], np.float32)
+    nsou = len(sz)
     dtr = np.zeros((shot.ntr, shot.nt), np.float32)
+    wave = np.zeros((max(nsnap, 1), nyw, nxw, g.nz), np.float32)
     rp = np.zeros(1, np.float32) if rho_pad is None else np.ascontiguousarray(rho_pad, np.float32)
     libc = C.CDLL(None)
     if rho_pad is not None:
         libc.mallopt(-6, 255)   # M_PERTURB: the never-written gradient cells start at zero, as in the oracle
     try:
         fn(g.nz, nxw, nyw, g.ntaper, g.dz, g.dx, g.dy, g.dt, g.oz, oxw, oyw, g.lsinc, ptr(vel_pad),
-           1, ptr(sz), ptr(sx), ptr(sy), ptr(st), ext, shot.ntr, ptr(shot.rz), ptr(shot.rx), ptr(shot.ry), ptr(dtr),
-           shot.nt, 0 if rho_pad is None else 1, ptr(rp))
+           nsou, ptr(sz), ptr(sx), ptr(sy), ptr(st), ext, shot.ntr, ptr(shot.rz), ptr(shot.rx), ptr(shot.ry), ptr(dtr),
+           shot.nt, 0 if rho_pad is None else 1, ptr(rp), tdir, snapi, nsnap, ptr(wave))
     finally:
         if rho_pad is not None:
             libc.mallopt(-6, 0)
-    return dtr
+    return (dtr, wave[:nsnap]) if nsnap > 0 else dtr
+
+
+def rtm3d(g, vel, shot, traces_in, vel_bwd=None, rho=None):
+    """orc_f_rtm3d: dict(image, illum (nz, nx, ny), win_image, win_illum
+    [nyw, nxw, nz], snaps, capture [ntsnap, nyw, nxw, nz])."""
+    L = oracle()
+    oxw, nxw, oyw, nyw = window3d(g, shot)
+    ns = shot.nt // g.snapi
+    img = np.zeros((g.ny, g.nx, g.nz), np.float32)
+    ill = np.zeros_like(img)
+    wi = np.zeros((nyw, nxw, g.nz), np.float32)
+    wl = np.zeros_like(wi)
+    sn = np.zeros((max(ns, 1), nyw, nxw, g.nz), np.float32)
+    cp = np.zeros_like(sn)
+    s = shot.c()
+    tr = np.ascontiguousarray(traces_in, np.float32)
+    L.orc_f_rtm3d.argtypes = [C.c_void_p] * 12
+    vb = None if vel_bwd is None else model3d_buf(vel_bwd)
+    rb = None if rho is None else model3d_buf(rho)
+    rc = L.orc_f_rtm3d(C.byref(g), ptr(model3d_buf(vel)), None if vb is None else ptr(vb),
+                       None if rb is None else ptr(rb), C.byref(s), ptr(tr), ptr(img), ptr(ill), ptr(wi), ptr(wl),
+                       ptr(sn), ptr(cp))
+    assert rc == 0, rc
+    t = lambda a: a.transpose(2, 1, 0)
+    return dict(image=t(img), illum=t(ill), win_image=wi, win_illum=wl, snaps=sn[:ns], capture=cp[:ns])

@@ -186,3 +186,56 @@ def test_density3d_errors():
     assert ei.value.code == -1
     with pytest.raises(ValueError):
         e.set_density(np.ones((og.nz, og.nx), np.float32))
+
+
+# ---- 3D RTM ----
+
+def rtm_setup(density=False, vel_bwd=False, snapi=2):
+    nz, nx, ny = 20, 26, 22
+    gg = sa.make_grid3d(nz, nx, ny, 10.0, 10.0, 10.0, 1e-3, ntaper=10, aperture_x=60.0, aperture_y=40.0, fmax=0.0,
+                        snapi=snapi)
+    og = O.make_grid3d(nz, nx, ny, 10.0, 10.0, 10.0, 1e-3, ntaper=10, apx=60.0, apy=40.0, snapi=snapi)
+    vel = model3d(nz, nx, ny, 1)
+    rho = rho3d(nz, nx, ny) if density else None
+    vb = (vel * 1.03).astype(np.float32) if vel_bwd else None
+    w = ricker(30.0, 120, 1e-3)
+    rx = np.array([33.0, 97.5, 141.0, 200.2, 77.7, 120.0, 120.5], np.float32)
+    ry = np.array([51.0, 88.8, 120.1, 60.0, 150.3, 99.0, 101.0], np.float32)   # the last two share cells
+    rz = np.array([23.0, 31.0, 47.3, 28.0, 55.5, 40.0, 41.0], np.float32)
+    gs, os_ = shot_pair(44.4, 121.2, 99.9, w, rz, rx, ry)
+    tr = O.modeling3d(og, (vel * 0.97).astype(np.float32), os_, extract_kind=0, rho=rho)
+    e = sa.Engine3D(gg)
+    e.set_model(vel, vb)
+    if density:
+        e.set_density(rho)
+    return e, og, vel, vb, rho, gs, os_, tr
+
+
+@pytest.mark.parametrize("density,vel_bwd,snapi", [(False, False, 2), (False, True, 3), (True, False, 2)])
+def test_rtm3d_bitwise(density, vel_bwd, snapi):
+    say("3d rtm density=%s vel_bwd=%s snapi=%d" % (density, vel_bwd, snapi))
+    e, og, vel, vb, rho, gs, os_, tr = rtm_setup(density, vel_bwd, snapi)
+    img, ill = e.rtm([gs], [tr])
+    want = O.rtm3d(og, vel, os_, tr, vel_bwd=vb, rho=rho)
+    assert np.abs(want["image"]).max() > 0
+    assert np.array_equal(img, want["image"]), rel(img, want["image"])
+    assert np.array_equal(ill, want["illum"]), rel(ill, want["illum"])
+    # a second call adds to the given arrays
+    img2, ill2 = e.rtm([gs], [tr], image=img, illum=ill)
+    assert np.array_equal(img2, img + want["image"])
+
+
+def test_rtm3d_two_shots():
+    e, og, vel, vb, rho, gs, os_, tr = rtm_setup()
+    w = ricker(30.0, 100, 1e-3)
+    rx = np.array([60.0, 150.0, 210.0], np.float32)
+    ry = np.array([70.0, 80.0, 130.0], np.float32)
+    rz = np.array([25.0, 25.0, 25.0], np.float32)
+    g2, o2 = shot_pair(30.0, 90.0, 110.0, w, rz, rx, ry)
+    tr2 = O.modeling3d(og, vel, o2, extract_kind=0)
+    img, ill = e.rtm([gs, g2], [tr, tr2])
+    a = O.rtm3d(og, vel, os_, tr)
+    b = O.rtm3d(og, vel, o2, tr2)
+    # the second shot adds its window into the first's image (float32, in shot order)
+    assert np.array_equal(img, (a["image"] + b["image"]).astype(np.float32))
+    assert np.array_equal(ill, (a["illum"] + b["illum"]).astype(np.float32))
