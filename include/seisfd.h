@@ -664,6 +664,16 @@ int  sfd_forward3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, float *t
  * shot (nt, ntr) time fastest, concatenated. */
 int  sfd_rtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *traces_in, float *image,
                float *illum);
+/* 3DAFWI.x's gradient in 3D (AFWI_to_memory, to_disk_to_memory_AFWI.f90:
+ * 229-415, twoD false): as sfd_afwi_gradient, with the 3D branch's forward
+ * stencil in the reverse pass (:394-403) and, with sfd_set_density3d, the
+ * v-rho imaging condition with its y term (Imaging.f90:424-512, dk = 1
+ * unless the window has one line).  dobs / mute: per shot (nt, ntr)
+ * concatenated (mute may be NULL); grad: (nz,nx,ny) vp [, (nz,nx,ny) rho or
+ * Ip], overwritten with the sum over the shots; illum may be NULL.  FP32
+ * snapshots only (opt->snap_mode SFD_SNAP_FP32). */
+int  sfd_afwi_gradient3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *dobs, const float *mute,
+                         const sfd_afwi_opts *opt, double *misfit, float *grad, float *illum);
 
 /* ---- SEP files (the reference reads and writes every array through SEPlib:
  *      from_aux / sreed / srite, libs/libwave/IO.f90:14-51,

@@ -647,3 +647,173 @@ int orc_f_rtm3d(const sfd_grid3d *g, const float *vel_full, const float *vel_bwd
     free3(&P);
     return 0;
 }
+
+double orc_f_fct_compute(int ntype, const float *rd, int nd, float *thresh);
+void orc_f_gdt_compute(int ntype, float *rd, int nd, float t);
+
+/* progs/afwi/3DAFWI.f90's gradient, AFWI_to_memory (to_disk_to_memory_AFWI.f90:
+ * 229-415) with genpar%twoD false, summed over nshots shots: per shot forward
+ * modeling (Injection_sinc; readsou's half-step shift with a density) with
+ * Extraction_array_sinc_afwi (gain 1, Extraction.f90:137-162 -> :276-337)
+ * storing snapshots; Compute_OF_RES_3D (Objective_Function_Residual_Adjsource.
+ * f90:42-67; one THRESH per shot, traces in order; the misfit starts at 0);
+ * reverse time with the 3D branch's forward stencil (or the density
+ * operator), Injection_sinc of the residual and
+ * Imaging_condition_AFWI_from_memory (Imaging.f90:371-422), or with a density
+ * Imaging_condition_AFWI_RHOVP_from_memory (:424-512, dk = 1 unless nyw = 1);
+ * each window added into the model (mod_copy_gradient_to_disk,
+ * ExtractPadModel.f90:592-660) and, with a density and vprho 1, the Vp/Ip
+ * change of variables on the model's vel and rho.  dobs, mute: per shot
+ * (nt, ntr) time fastest, concatenated (mute may be NULL); grad: (nz,nx,ny)
+ * for vp, then for rho (or Ip) with a density. */
+int orc_f_afwi3d(const sfd_grid3d *g, const float *vel_full, const float *rho_full, int nshots,
+                 const sfd_shot3d *shots, const float *dobs, const float *mute, int norm, float norm_thresh, int vprho,
+                 double *fout, float *grad, float *illum)
+{
+    if (g->snapi < 1) return -1;
+    const size_t N = (size_t)g->nz * g->nx * g->ny;
+    const int np = rho_full ? 2 : 1;
+    for (size_t q = 0; q < N * np; q++) grad[q] = 0;
+    for (size_t q = 0; q < N; q++) illum[q] = 0;
+    double f = 0;
+    size_t off = 0;
+    for (int is = 0; is < nshots; is++) {
+        const sfd_shot3d *s = &shots[is];
+        P3 P;
+        float oxw, oyw;
+        int nxw, nyw;
+        if (setup3(&P, g, vel_full, rho_full, s, &oxw, &nxw, &oyw, &nyw)) return -2;
+        if (nxw < 2 || g->nz < 2) return -1;   /* the one-sided differences need two cells */
+        const B3 *b = &P.b;
+        const size_t n = (size_t)b->NZ * b->NX * b->NY, nwin = (size_t)g->nz * nxw * nyw;
+        const int ntsnap = s->nt / g->snapi, h = (int)(g->lsinc * 0.5);
+        float *buf[5], *snaps = (float *)calloc(nwin * (ntsnap > 0 ? ntsnap : 1), sizeof(float));
+        float *img = (float *)calloc(nwin, sizeof(float)), *img2 = (float *)calloc(nwin, sizeof(float));
+        float *ill = (float *)calloc(nwin, sizeof(float));
+        float *tx = (float *)malloc(sizeof(float) * nxw), *ty = (float *)malloc(sizeof(float) * nyw);
+        float *dmod = (float *)calloc((size_t)s->nt * s->ntr, sizeof(float));
+        for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+        if (!buf[4] || !snaps || !img || !img2 || !ill || !tx || !ty || !dmod) return -2;
+        float *wav = (float *)malloc(sizeof(float) * (size_t)s->nt);
+        memcpy(wav, s->wavelet, sizeof(float) * (size_t)s->nt);
+        if (rho_full) orc_f_half_step(wav, s->nt, g->lsinc);
+        Pt3 src, *rec = (Pt3 *)malloc(sizeof(Pt3) * (size_t)s->ntr);
+        points3(g, oxw, oyw, 1, &s->sz, &s->sx, &s->sy, &src);
+        points3(g, oxw, oyw, s->ntr, s->rz, s->rx, s->ry, rec);
+#define WIN(u, i, j, k) A3(&P, u, i, j, k)
+#define WQ(i, j, k) ((size_t)((i) - 1) + (size_t)((j) - 1) * g->nz + (size_t)((k) - 1) * g->nz * nxw)
+        int counter = 0;
+        for (int it = 1; it <= s->nt; it++) {
+            for (int r = 0; r < s->ntr; r++) {   /* Extraction_array_sinc_afwi */
+                const int *ic = rec[r].ic;
+                float *tr = dmod + (size_t)r * s->nt + (it - 1), acc = *tr;
+                for (int k = -h; k <= h; k++)
+                    for (int j = -h; j <= h; j++)
+                        for (int i = -h; i <= h; i++)
+                            acc = acc + 1.f * A3(&P, buf[3], i + ic[0], j + ic[1], k + ic[2]) * rec[r].wz[h + i] *
+                                            rec[r].wx[h + j] * rec[r].wx[h + k];
+                *tr = acc;
+            }
+            if (it % g->snapi == 0) {
+                float *S = snaps + (size_t)(counter++) * nwin;
+                for (int k = 1; k <= nyw; k++)
+                    for (int j = 1; j <= nxw; j++)
+                        for (int i = 1; i <= g->nz; i++) S[WQ(i, j, k)] = WIN(buf[3], i, j, k);
+            }
+            step3(&P, buf, &src, 1, wav, s->nt, it, h);
+        }
+        /* Compute_OF_RES_3D */
+        float thresh = norm_thresh;
+        for (int r = 0; r < s->ntr; r++) {
+            float *d = dmod + (size_t)r * s->nt;
+            const float *ob = dobs + off + (size_t)r * s->nt;
+            const float *mu = mute ? mute + off + (size_t)r * s->nt : NULL;
+            for (int it = 0; it < s->nt; it++) d[it] = (mu ? mu[it] : 1.f) * (ob[it] - d[it]);
+            f = f + orc_f_fct_compute(norm, d, s->nt, &thresh);
+            for (int it = 0; it < s->nt; it++) d[it] = (mu ? mu[it] : 1.f) * d[it];
+            orc_f_gdt_compute(norm, d, s->nt, thresh);
+        }
+        /* backward */
+        for (int q = 0; q < 5; q++) memset(buf[q], 0, n * sizeof(float));
+        if (P.sz) {
+            memset(P.sz, 0, n * sizeof(float));
+            memset(P.sx, 0, n * sizeof(float));
+            memset(P.sy, 0, n * sizeof(float));
+        }
+        orc_f_compute_taper(tx, nxw);
+        for (int k = 0; k < nyw; k++) ty[k] = 1.f;
+        if (nyw > 1) orc_f_compute_taper(ty, nyw);
+        const float dt2 = (g->dt * (float)g->snapi) * (g->dt * (float)g->snapi), dti2 = 1.f / dt2;
+        const float dzi = 1.f / g->dz, dxi = 1.f / g->dx, dyi = 1.f / g->dy;
+        const int dk = nyw == 1 ? 0 : 1;
+        counter = 0;
+        for (int it = s->nt; it >= 1; it--) {
+            if (it % g->snapi == 0) {
+                counter++;
+                const int index = ntsnap - counter + 1;
+                const int im1 = index - 1 > 1 ? index - 1 : 1, ip1 = index + 1 < ntsnap ? index + 1 : ntsnap;
+                const float *S = snaps + (size_t)(index - 1) * nwin, *Sm = snaps + (size_t)(im1 - 1) * nwin,
+                            *Sp = snaps + (size_t)(ip1 - 1) * nwin;
+                for (int k = 1; k <= nyw; k++)
+                    for (int j = 1; j <= nxw; j++) {
+                        const float t = tx[j - 1] * ty[k - 1];
+                        for (int i = 1; i <= g->nz; i++) {
+                            const size_t q = WQ(i, j, k);
+                            const float u = WIN(buf[3], i, j, k), v = WIN(P.vel, i, j, k);
+                            if (!rho_full) {
+                                const float tmp = (Sm[q] - 2.f * S[q] + Sp[q]) / (dt2 * (v * v * v));
+                                img[q] = img[q] + u * tmp * t;
+                            } else {
+                                const int ip = i + 1 < g->nz ? i + 1 : g->nz, jp = j + 1 < nxw ? j + 1 : nxw;
+                                const int kp = k + dk < nyw ? k + dk : nyw;
+                                const int im = ip - 1, jm = jp - 1, km = kp - dk;
+                                float tmp = dzi * (WIN(buf[3], ip, jm, km) - WIN(buf[3], im, jm, km)) * dzi *
+                                            (S[WQ(ip, jm, km)] - S[WQ(im, jm, km)]);
+                                tmp = tmp + dxi * (WIN(buf[3], im, jp, km) - WIN(buf[3], im, jm, km)) * dxi *
+                                                (S[WQ(im, jp, km)] - S[WQ(im, jm, km)]);
+                                tmp = tmp + dyi * (WIN(buf[3], im, jm, kp) - WIN(buf[3], im, jm, km)) * dyi *
+                                                (S[WQ(im, jm, kp)] - S[WQ(im, jm, km)]);
+                                const float tmp1 = u * dti2 * (Sm[q] - 2.f * S[q] + Sp[q]);
+                                const float r = WIN(P.rho, i, j, k);
+                                img[q] = img[q] + 2.f * tmp1 * t / (v * v * v * r);
+                                img2[q] = img2[q] + tmp * t / (r * r) + tmp1 * t / (r * r * (v * v));
+                            }
+                            ill[q] = ill[q] + S[q] * S[q] * t;
+                        }
+                    }
+            }
+            step3(&P, buf, rec, s->ntr, dmod, s->nt, it, h);
+        }
+#undef WQ
+#undef WIN
+        /* mod_copy_gradient_to_disk: the window added at the nearest model columns and lines */
+        for (int pass = 0; pass < 3; pass++) {
+            float *dst = pass == 0 ? grad : pass == 1 ? grad + N : illum;
+            const float *w = pass == 0 ? img : pass == 1 ? img2 : ill;
+            if (pass == 1 && np == 1) continue;
+            for (int k = 1; k <= nyw; k++) {
+                const int iy = nint3(((float)(k - 1) * g->dy + oyw - g->oy) / g->dy) + 1;
+                if (iy < 1 || iy > g->ny) continue;
+                for (int j = 1; j <= nxw; j++) {
+                    const int ix = nint3(((float)(j - 1) * g->dx + oxw - g->ox) / g->dx) + 1;
+                    if (ix < 1 || ix > g->nx) continue;
+                    float *d = dst + (size_t)(ix - 1) * g->nz + (size_t)(iy - 1) * g->nz * g->nx;
+                    const float *ww = w + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw;
+                    for (int i = 0; i < g->nz; i++) d[i] = d[i] + ww[i];
+                }
+            }
+        }
+        for (int q = 0; q < 5; q++) free(buf[q]);
+        free(snaps); free(img); free(img2); free(ill); free(tx); free(ty); free(dmod); free(wav); free(rec);
+        free3(&P);
+        off += (size_t)s->nt * s->ntr;
+    }
+    if (np == 2 && vprho == 1)   /* ExtractPadModel.f90:643-646 on the model's vel and rho */
+        for (size_t q = 0; q < N; q++) {
+            const float v = vel_full[q], r = rho_full[q];
+            grad[q] = grad[q] - (r / v) * grad[N + q];
+            grad[N + q] = (1.f / v) * grad[N + q];
+        }
+    *fout = f;
+    return 0;
+}

@@ -263,6 +263,52 @@ __global__ void __launch_bounds__(256) update3_kernel(const float *__restrict__ 
     u3[q] = (20.f * u2[q] - 6.f * u1[q] - 4.f * u0[q] + um[q] + c12dt2 * u3[q]) * div11;
 }
 
+// Imaging_condition_AFWI_from_memory (Imaging.f90:371-422) in 3D: image +=
+// u ((S- - 2 S) + S+) / (dt2 v^3) t, illum += S S t, per window cell.
+__global__ void afwi3_image_kernel(const float *__restrict__ u2, const float *__restrict__ vel, Geo G, int nz, int nxw,
+                                   int i0, const float *__restrict__ S, const float *__restrict__ Sm,
+                                   const float *__restrict__ Sp, const float *__restrict__ tx,
+                                   const float *__restrict__ ty, float dt2, float *__restrict__ img,
+                                   float *__restrict__ ill) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nz) return;
+    const size_t q = (size_t)i + (size_t)j * nz + (size_t)k * nz * nxw, a = G.at(i + i0, j + i0, k + i0);
+    const float t = tx[j] * ty[k], v = vel[a];
+    const float tmp = (Sm[q] - 2.f * S[q] + Sp[q]) / (dt2 * (v * v * v));
+    img[q] = img[q] + u2[a] * tmp * t;
+    ill[q] = ill[q] + S[q] * S[q] * t;
+}
+
+// Imaging_condition_AFWI_RHOVP_from_memory (Imaging.f90:424-512) in 3D: with
+// ip = min(nz, i+1), jp = min(j+1, nxw), kp = min(k+dk, nyw) (dk = 0 when
+// nyw = 1), im = ip-1, jm = jp-1, km = kp-dk:
+//   tmp  = dzi du_z dzi dS_z + dxi du_x dxi dS_x + dyi du_y dyi dS_y (one-sided)
+//   tmp1 = u / dt2 (S- - 2 S + S+)
+//   image += 2 tmp1 t / (v^3 rho), image2 += tmp t / rho^2 + tmp1 t / (rho^2 v^2)
+__global__ void rhovp3_image_kernel(const float *__restrict__ u2, const float *__restrict__ vel,
+                                    const float *__restrict__ rho, Geo G, int nz, int nxw, int nyw, int i0,
+                                    const float *__restrict__ S, const float *__restrict__ Sm,
+                                    const float *__restrict__ Sp, const float *__restrict__ tx,
+                                    const float *__restrict__ ty, float dt2, float dzi, float dxi, float dyi,
+                                    float *__restrict__ img, float *__restrict__ img2, float *__restrict__ ill) {
+    const int i0w = blockIdx.x * blockDim.x + threadIdx.x, j0w = blockIdx.y, k0w = blockIdx.z;
+    if (i0w >= nz) return;
+    const int i = i0w + 1, j = j0w + 1, k = k0w + 1, dk = nyw == 1 ? 0 : 1;
+    const int ip = i + 1 < nz ? i + 1 : nz, jp = j + 1 < nxw ? j + 1 : nxw, kp = k + dk < nyw ? k + dk : nyw;
+    const int im = ip - 1, jm = jp - 1, km = kp - dk;
+    auto U = [&](int a, int b, int c) { return u2[G.at(a - 1 + i0, b - 1 + i0, c - 1 + i0)]; };
+    auto W = [&](int a, int b, int c) { return S[(size_t)(a - 1) + (size_t)(b - 1) * nz + (size_t)(c - 1) * nz * nxw]; };
+    float tmp = dzi * (U(ip, jm, km) - U(im, jm, km)) * dzi * (W(ip, jm, km) - W(im, jm, km));
+    tmp = tmp + dxi * (U(im, jp, km) - U(im, jm, km)) * dxi * (W(im, jp, km) - W(im, jm, km));
+    tmp = tmp + dyi * (U(im, jm, kp) - U(im, jm, km)) * dyi * (W(im, jm, kp) - W(im, jm, km));
+    const size_t q = (size_t)i0w + (size_t)j0w * nz + (size_t)k0w * nz * nxw, a = G.at(i0w + i0, j0w + i0, k0w + i0);
+    const float tmp1 = u2[a] * (1.f / dt2) * (Sm[q] - 2.f * S[q] + Sp[q]);
+    const float t = tx[j0w] * ty[k0w], v = vel[a], r = rho[a];
+    img[q] = img[q] + 2.f * tmp1 * t / (v * v * v * r);
+    img2[q] = img2[q] + tmp * t / (r * r) + tmp1 * t / (r * r * (v * v));
+    ill[q] = ill[q] + S[q] * S[q] * t;
+}
+
 // Higdon_factor (Boundary.f90:366-447) at cell q along a line of stride s
 // (s < 0: the band lies at the low end and the recursion reads q - s, q - 2s)
 __device__ __forceinline__ void hfac(const float *g, float *U3, const float *U2, const float *U1, size_t q, long s) {
@@ -841,6 +887,177 @@ static void rtm3(sfd_ctx3d &h, const sfd_shot3d &s, const float *traces_in, floa
     }
 }
 
+// Compute_OF_RES_3D (Objective_Function_Residual_Adjsource.f90:42-67) on one
+// shot's traces, in trace order, with fct_compute / gdt_compute
+// (libs/libinv/Norm.f90:12-141): r = w (dobs - dmod); misfit += the trace's
+// norm (L2: FP64 products over 2; the others FP32 terms summed in FP32 in
+// time order); adjoint source w r, through gdt for the non-L2 norms.  One
+// THRESH for the shot: data_threshold, or when 0 (and the norm uses it) the
+// max |r| of the first trace whose residual is not all zero.
+// f: the running misfit over every trace of the call, added to in trace order.
+static void residual3(int norm, float thresh0, int nt, int ntr, const float *dobs, const float *mute, float *d,
+                      double &f) {
+    const bool uses = norm == SFD_NORM_HUBER || norm == SFD_NORM_CAUCHY || norm == SFD_NORM_HYPERBOLIC;
+    float t = thresh0;
+    for (int r = 0; r < ntr; r++) {
+        float *x = d + (size_t)r * nt;
+        const float *ob = dobs + (size_t)r * nt, *mu = mute ? mute + (size_t)r * nt : nullptr;
+        float m = 0.f;
+        for (int it = 0; it < nt; it++) {
+            x[it] = (mu ? mu[it] : 1.0f) * (ob[it] - x[it]);
+            m = std::max(m, fabsf(x[it]));
+        }
+        if (uses && t == 0.f) t = m;
+        if (norm == SFD_NORM_L2) {
+            double a = 0.0;
+            for (int it = 0; it < nt; it++) a = a + (double)x[it] * (double)x[it];
+            f = f + a / 2;
+        } else {
+            float acc = 0.f;
+            for (int it = 0; it < nt; it++) {
+                const float v = x[it], av = fabsf(v);
+                float term;
+                if (norm == SFD_NORM_L1) term = av;
+                else if (norm == SFD_NORM_HUBER) term = av > t ? av - t / 2.0f : v * v / (2.0f * t);
+                else if (norm == SFD_NORM_CAUCHY) { const float qq = v / t; term = logf(1.0f + qq * qq) / 2.0f; }
+                else term = sqrtf(t * t + v * v) - t;
+                acc = acc + term;
+            }
+            f = f + (double)acc;
+        }
+        for (int it = 0; it < nt; it++) {
+            float y = (mu ? mu[it] : 1.0f) * x[it];
+            if (norm == SFD_NORM_L1) y = y < 0.f ? -1.f : (y > 0.f ? 1.f : 0.f);
+            else if (norm == SFD_NORM_HUBER) { const float qq = y / t; y = qq > 1.f ? 1.f : (qq < -1.f ? -1.f : qq); }
+            else if (norm == SFD_NORM_CAUCHY) y = y / (t * t + y * y);
+            else if (norm == SFD_NORM_HYPERBOLIC) y = (y / t) / sqrtf(1.0f + y * y / (t * t));
+            x[it] = y;
+        }
+    }
+}
+
+// Adds a window field (nz, nxw, nyw) into a model field (nz, nx, ny) at the
+// nearest model columns and lines (mod_copy_gradient_to_disk / mod_copy_image).
+static void copy_window3(const sfd_grid3d &g, const Run3 &R, const float *w, float *dst) {
+    for (int k = 1; k <= R.nyw; k++) {
+        const int iy = nint3(((float)(k - 1) * g.dy + R.oyw - g.oy) / g.dy) + 1;
+        if (iy < 1 || iy > g.ny) continue;
+        for (int j = 1; j <= R.nxw; j++) {
+            const int ix = nint3(((float)(j - 1) * g.dx + R.oxw - g.ox) / g.dx) + 1;
+            if (ix < 1 || ix > g.nx) continue;
+            float *d = dst + (size_t)(ix - 1) * g.nz + (size_t)(iy - 1) * g.nz * g.nx;
+            const float *ww = w + (size_t)(j - 1) * g.nz + (size_t)(k - 1) * g.nz * R.nxw;
+            for (int i = 0; i < g.nz; i++) d[i] = d[i] + ww[i];
+        }
+    }
+}
+
+// AFWI_to_memory with genpar%twoD false (to_disk_to_memory_AFWI.f90:229-415),
+// one shot: forward with gain-1 sinc extraction and snapshots, the residual
+// on the host, reverse time with the residual injected and the AFWI or v-rho
+// imaging; the windows are added into grad (vp, then rho) and illum.
+static void afwi3(sfd_ctx3d &h, const sfd_shot3d &s, const float *dobs, const float *mute, int norm, float thresh,
+                  float *grad, float *illum, double &f) {
+    const sfd_grid3d &g = h.g;
+    check_shot3(s);
+    Run3 R;
+    geom3(h, s, R);
+    if (R.nxw < 2) throw EngineError(SFD_EINVAL, "the imaging's one-sided x difference needs two window columns");
+    InjPlan src, recp;
+    plan_inj3(R, h.vel_h, h.rho_h, 1, &s.sz, &s.sx, &s.sy, src);
+    plan_inj3(R, h.vel_h, h.rho_h, s.ntr, s.rz, s.rx, s.ry, recp);
+    std::vector<Rec3> rec(s.ntr);
+    for (int r = 0; r < s.ntr; r++) {   // Extraction_array_sinc_afwi: gain 1
+        int ic[3];
+        float dc[3];
+        find_i_d3(&ic[0], &dc[0], s.rz[r], g.oz, g.dz);
+        find_i_d3(&ic[1], &dc[1], s.rx[r], R.oxw, g.dx);
+        find_i_d3(&ic[2], &dc[2], s.ry[r], R.oyw, g.dy);
+        Rec3 &e = rec[r];
+        e.iz = ic[0] - 1 + R.i0;
+        e.ix = ic[1] - 1 + R.i0;
+        e.iy = ic[2] - 1 + R.i0;
+        e.gain = 1.f;
+        sfd_host_mksinc(e.wz, 7, dc[0] / g.dz);
+        sfd_host_mksinc(e.wx, 7, dc[1] / g.dx);
+    }
+    const int ntsnap = s.nt / g.snapi;
+    const size_t nwin = (size_t)g.nz * R.nxw * R.nyw, ntr_nt = (size_t)s.nt * s.ntr;
+    Layout E;
+    const size_t o_snap = E.take(nwin * std::max(ntsnap, 1) * 4), o_img = E.take(nwin * 4),
+                 o_img2 = E.take(nwin * 4), o_ill = E.take(nwin * 4), o_tx = E.take(R.nxw * 4),
+                 o_ty = E.take(R.nyw * 4), o_trc = E.take(ntr_nt * 4), o_rec = E.take(sizeof(Rec3) * s.ntr),
+                 o_src = E.take(inj_bytes(src, s.nt)), o_rinj = E.take(inj_bytes(recp, ntr_nt));
+    size_t extra_off = 0;
+    alloc3(h, R, E.off, &extra_off);
+    char *ex = R.base + extra_off;
+    float *snaps = (float *)(ex + o_snap), *img = (float *)(ex + o_img), *img2 = (float *)(ex + o_img2),
+          *ill = (float *)(ex + o_ill), *trc = (float *)(ex + o_trc);
+    for (size_t o : {o_img, o_img2, o_ill}) HIPCHK(hipMemsetAsync(ex + o, 0, nwin * 4, h.st));
+    HIPCHK(hipMemsetAsync(trc, 0, ntr_nt * 4, h.st));
+    std::vector<float> tx(R.nxw), ty(R.nyw, 1.f);
+    sfd_host_taper(tx.data(), R.nxw);
+    if (R.nyw > 1) sfd_host_taper(ty.data(), R.nyw);
+    HIPCHK(hipMemcpyAsync(ex + o_tx, tx.data(), R.nxw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(ex + o_ty, ty.data(), R.nyw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(ex + o_rec, rec.data(), sizeof(Rec3) * s.ntr, hipMemcpyHostToDevice, h.st));
+    const std::vector<float> wav = source3(h, s);
+    const InjDev Is = upload_inj(h, ex + o_src, src, wav.data(), 1, s.nt);
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw);
+    models3(h, R, h.d_vel);
+    start_pass3(h, R, true);
+    float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
+    int counter = 0;
+    for (int it = 1; it <= s.nt; it++) {
+        hipLaunchKernelGGL(extract3_kernel, dim3((s.ntr + 63) / 64), dim3(64), 0, h.st, (const float *)buf[3], R.G,
+                           (const Rec3 *)(ex + o_rec), s.ntr, 0, trc, s.nt, it);
+        if (it % g.snapi == 0)
+            hipLaunchKernelGGL(snap3_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3], R.G, g.nz, R.nxw, R.i0,
+                               snaps + (size_t)(counter++) * nwin);
+        step3(h, R, buf, Is, it);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<float> d(ntr_nt);
+    HIPCHK(hipMemcpyAsync(d.data(), trc, ntr_nt * 4, hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipStreamSynchronize(h.st));
+    residual3(norm, thresh, s.nt, s.ntr, dobs, mute, d.data(), f);
+    const InjDev Ir = upload_inj(h, ex + o_rinj, recp, d.data(), s.ntr, s.nt);
+    start_pass3(h, R, false);
+    for (int k = 0; k < 5; k++) buf[k] = R.u[k];
+    const float dt2 = (g.dt * (float)g.snapi) * (g.dt * (float)g.snapi);
+    counter = 0;
+    for (int it = s.nt; it >= 1; it--) {
+        if (it % g.snapi == 0) {
+            counter++;
+            const int index = ntsnap - counter + 1;
+            const int im1 = index - 1 > 1 ? index - 1 : 1, ip1 = index + 1 < ntsnap ? index + 1 : ntsnap;
+            const float *S = snaps + (size_t)(index - 1) * nwin, *Sm = snaps + (size_t)(im1 - 1) * nwin,
+                        *Sp = snaps + (size_t)(ip1 - 1) * nwin;
+            if (R.rho)
+                hipLaunchKernelGGL(rhovp3_image_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3],
+                                   (const float *)R.vel, (const float *)R.rhop, R.G, g.nz, R.nxw, R.nyw, R.i0, S, Sm,
+                                   Sp, (const float *)(ex + o_tx), (const float *)(ex + o_ty), dt2, R.dzi, R.dxi,
+                                   R.dyi, img, img2, ill);
+            else
+                hipLaunchKernelGGL(afwi3_image_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3],
+                                   (const float *)R.vel, R.G, g.nz, R.nxw, R.i0, S, Sm, Sp, (const float *)(ex + o_tx),
+                                   (const float *)(ex + o_ty), dt2, img, ill);
+        }
+        step3(h, R, buf, Ir, it);
+    }
+    HIPCHK(hipGetLastError());
+    const size_t N = (size_t)g.nz * g.nx * g.ny;
+    std::vector<float> w(nwin);
+    for (int pass = 0; pass < 3; pass++) {
+        if (pass == 1 && !R.rho) continue;
+        if (pass == 2 && !illum) continue;
+        HIPCHK(hipMemcpyAsync(w.data(), pass == 0 ? img : pass == 1 ? img2 : ill, nwin * 4, hipMemcpyDeviceToHost,
+                              h.st));
+        HIPCHK(hipStreamSynchronize(h.st));
+        copy_window3(g, R, w.data(), pass == 0 ? grad : pass == 1 ? grad + N : illum);
+    }
+}
+
 static std::string g_create_err3d;   // message of the last failed sfd_create3d
 
 extern "C" {
@@ -930,6 +1147,41 @@ int sfd_rtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *
             rtm3(*h, shots[s], traces_in + off, image, illum);
             off += (size_t)shots[s].nt * shots[s].ntr;
         }
+    });
+}
+
+int sfd_afwi_gradient3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *dobs, const float *mute,
+                        const sfd_afwi_opts *opt, double *misfit, float *grad, float *illum) {
+    if (!h || !shots || nshots < 1 || !dobs || !opt || !misfit || !grad) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        const sfd_grid3d &g = h->g;
+        if (h->vel_h.empty()) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model3d first");
+        if (g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
+        if (opt->snap_mode != SFD_SNAP_FP32) throw EngineError(SFD_EUNSUPPORTED, "the 3D path keeps FP32 snapshots");
+        const int norm = opt->norm == 0 ? SFD_NORM_L2 : opt->norm;
+        if (norm != SFD_NORM_L2 && norm != SFD_NORM_L1 && norm != SFD_NORM_HUBER && norm != SFD_NORM_CAUCHY &&
+            norm != SFD_NORM_HYPERBOLIC)
+            throw EngineError(SFD_EINVAL, "unknown norm (SFD_NORM_*)");
+        if (!std::isfinite(opt->norm_thresh)) throw EngineError(SFD_EINVAL, "norm_thresh is not finite");
+        if (opt->vprho_param != 0 && opt->vprho_param != 1) throw EngineError(SFD_EINVAL, "vprho_param must be 0 or 1");
+        const size_t N = (size_t)g.nz * g.nx * g.ny;
+        const bool rho = !h->rho_h.empty();
+        std::fill(grad, grad + N * (rho ? 2 : 1), 0.f);
+        if (illum) std::fill(illum, illum + N, 0.f);
+        double f = 0;
+        size_t off = 0;
+        for (int s = 0; s < nshots; s++) {
+            afwi3(*h, shots[s], dobs + off, mute ? mute + off : nullptr, norm, opt->norm_thresh, grad, illum, f);
+            off += (size_t)shots[s].nt * shots[s].ntr;
+        }
+        if (!std::isfinite(f)) throw EngineError(SFD_ENOTFINITE, "misfit is not finite");
+        if (rho && opt->vprho_param == 1)   // ExtractPadModel.f90:643-646 on the model's vel and rho
+            for (size_t q = 0; q < N; q++) {
+                const float v = h->vel_h[q], r = h->rho_h[q];
+                grad[q] = grad[q] - (r / v) * grad[N + q];
+                grad[N + q] = (1.f / v) * grad[N + q];
+            }
+        *misfit = f;
     });
 }
 

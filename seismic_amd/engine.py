@@ -410,6 +410,24 @@ class Engine3D:
             self._rho = np.ascontiguousarray(r.transpose(2, 1, 0))
         self._chk(self.lib.sfd_set_density3d(self.h, _ptr(self._rho)))
 
+    def afwi_gradient(self, shots, dobs, mute=None, norm=0, norm_thresh=0.0, vprho_param=0):
+        """3DAFWI.x's gradient summed over the shots: dict(f, grad [nparam, nz,
+        nx, ny] (vp; with a density also rho or Ip), illum [nz, nx, ny])."""
+        o = abi.sfd_afwi_opts()
+        o.snap_mode, o.norm, o.norm_thresh, o.vprho_param = abi.SFD_SNAP_FP32, norm, norm_thresh, vprho_param
+        arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])
+        ob = Engine._cat(dobs, shots)
+        mu = None if mute is None else Engine._cat(mute, shots)
+        npar = 2 if getattr(self, "_rho", None) is not None else 1
+        N = self.g.nz * self.g.nx * self.g.ny
+        grad = np.zeros(npar * N, np.float32)
+        illum = np.zeros(N, np.float32)
+        f = C.c_double()
+        self._chk(self.lib.sfd_afwi_gradient3d(self.h, arr, len(shots), _ptr(ob), _ptr(mu), C.byref(o), C.byref(f),
+                                               _ptr(grad), _ptr(illum)))
+        t = lambda a: a.reshape(self.g.ny, self.g.nx, self.g.nz).transpose(2, 1, 0)
+        return dict(f=f.value, grad=np.stack([t(grad[k * N:(k + 1) * N]) for k in range(npar)]), illum=t(illum))
+
     def forward(self, shots, extract_kind=abi.SFD_EXTRACT_SINC_V2):
         """Traces per shot, (ntr, nt) each."""
         arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])

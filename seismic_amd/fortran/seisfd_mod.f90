@@ -176,6 +176,20 @@ module seisfd_mod
        real(c_float), intent(inout) :: image(*), illum(*)   ! (nz, nx, ny), accumulated
      end function sfd_rtm3d
 
+     integer(c_int) function sfd_afwi_gradient3d(h, shots, nshots, dobs, mute, opt, misfit, grad, illum) &
+          bind(C, name='sfd_afwi_gradient3d')
+       import :: c_int, c_ptr, c_float, c_double, sfd_shot3d, sfd_afwi_opts
+       type(c_ptr), value :: h
+       type(sfd_shot3d), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: dobs(*)
+       type(c_ptr), value :: mute                ! c_loc of the mute, or c_null_ptr
+       type(sfd_afwi_opts), intent(in) :: opt
+       real(c_double), intent(out) :: misfit
+       real(c_float), intent(inout) :: grad(*)   ! (nz,nx,ny) vp [, (nz,nx,ny) rho or Ip]
+       type(c_ptr), value :: illum
+     end function sfd_afwi_gradient3d
+
      integer(c_int) function sfd_set_density3d(h, rho) bind(C, name='sfd_set_density3d')
        import :: c_int, c_ptr
        type(c_ptr), value :: h

@@ -812,6 +812,27 @@ def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None, sour
     return (dtr, wave[:nsnap]) if nsnap > 0 else dtr
 
 
+def afwi3d(g, vel, shots, dobs, rho=None, mute=None, norm=2, norm_thresh=0.0, vprho=0):
+    """orc_f_afwi3d: dict(f, grad [nparam, nz, nx, ny], illum [nz, nx, ny])."""
+    L = oracle()
+    cs = (sfd_shot3d * len(shots))(*[s.c() for s in shots])
+    ob = np.ascontiguousarray(np.concatenate([np.asarray(d, np.float32).ravel() for d in dobs]))
+    mu = None if mute is None else np.ascontiguousarray(np.concatenate([np.asarray(m, np.float32).ravel()
+                                                                        for m in mute]))
+    N = g.nz * g.nx * g.ny
+    grad = np.zeros(2 * N, np.float32)
+    ill = np.zeros(N, np.float32)
+    f = C.c_double()
+    rb = None if rho is None else model3d_buf(rho)
+    L.orc_f_afwi3d.argtypes = [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 + [C.c_int, C.c_float, C.c_int,
+                                                                                   C.POINTER(C.c_double)] + [C.c_void_p] * 2
+    rc = L.orc_f_afwi3d(C.byref(g), ptr(model3d_buf(vel)), None if rb is None else ptr(rb), len(shots), cs, ptr(ob),
+                        None if mu is None else ptr(mu), norm, norm_thresh, vprho, C.byref(f), ptr(grad), ptr(ill))
+    assert rc == 0, rc
+    t = lambda a: a.reshape(g.ny, g.nx, g.nz).transpose(2, 1, 0)
+    return dict(f=f.value, grad=np.stack([t(grad[:N]), t(grad[N:])]), illum=t(ill))
+
+
 def rtm3d(g, vel, shot, traces_in, vel_bwd=None, rho=None):
     """orc_f_rtm3d: dict(image, illum (nz, nx, ny), win_image, win_illum
     [nyw, nxw, nz], snaps, capture [ntsnap, nyw, nxw, nz])."""

@@ -239,3 +239,77 @@ def test_rtm3d_two_shots():
     # the second shot adds its window into the first's image (float32, in shot order)
     assert np.array_equal(img, (a["image"] + b["image"]).astype(np.float32))
     assert np.array_equal(ill, (a["illum"] + b["illum"]).astype(np.float32))
+
+
+# ---- 3DAFWI's gradient in 3D ----
+
+def afwi_setup(density=False, snapi=2, seed=1):
+    nz, nx, ny = 20, 26, 22
+    gg = sa.make_grid3d(nz, nx, ny, 10.0, 10.0, 10.0, 1e-3, ntaper=10, aperture_x=60.0, aperture_y=40.0, fmax=0.0,
+                        snapi=snapi)
+    og = O.make_grid3d(nz, nx, ny, 10.0, 10.0, 10.0, 1e-3, ntaper=10, apx=60.0, apy=40.0, snapi=snapi)
+    vel = model3d(nz, nx, ny, seed)
+    rho = rho3d(nz, nx, ny) if density else None
+    e = sa.Engine3D(gg)
+    e.set_model(vel)
+    if density:
+        e.set_density(rho)
+    return e, og, vel, rho
+
+
+def afwi_shots(og, vel, rho):
+    rng = np.random.default_rng(3)
+    pairs = []
+    for k, (nt, ntr) in enumerate([(130, 6), (100, 4)]):
+        w = ricker(28.0, 130, 1e-3)[:nt]
+        pairs.append(shot_pair(rng.uniform(40, 80), rng.uniform(90, 160), rng.uniform(80, 130), w,
+                               rng.uniform(20.0, 60.0, ntr), rng.uniform(40.0, 210.0, ntr),
+                               rng.uniform(40.0, 170.0, ntr)))
+    dobs = [O.modeling3d(og, (vel * 0.96).astype(np.float32), p[1], extract_kind=3,
+                         rho=None if rho is None else (rho * 1.05).astype(np.float32)) for p in pairs]
+    return pairs, dobs
+
+
+@pytest.mark.parametrize("density,vprho", [(False, 0), (True, 0), (True, 1)])
+def test_afwi3d_bitwise(density, vprho):
+    say("3d afwi density=%s vprho=%d" % (density, vprho))
+    e, og, vel, rho = afwi_setup(density)
+    pairs, dobs = afwi_shots(og, vel, rho)
+    got = e.afwi_gradient([p[0] for p in pairs], dobs, vprho_param=vprho)
+    want = O.afwi3d(og, vel, [p[1] for p in pairs], dobs, rho=rho, vprho=vprho)
+    assert got["f"] == want["f"]
+    for k in range(2 if density else 1):
+        assert np.abs(want["grad"][k]).max() > 0
+        assert np.array_equal(got["grad"][k], want["grad"][k]), (k, rel(got["grad"][k], want["grad"][k]))
+    assert np.array_equal(got["illum"], want["illum"])
+
+
+@pytest.mark.parametrize("norm", [1, 3, 12, 4])
+def test_afwi3d_norms_and_mute(norm):
+    e, og, vel, rho = afwi_setup(True, snapi=3, seed=2)
+    pairs, dobs = afwi_shots(og, vel, rho)
+    rng = np.random.default_rng(8)
+    mute = [rng.uniform(0.2, 1.0, d.shape).astype(np.float32) for d in dobs]
+    got = e.afwi_gradient([p[0] for p in pairs], dobs, mute=mute, norm=norm)
+    want = O.afwi3d(og, vel, [p[1] for p in pairs], dobs, rho=rho, mute=mute, norm=norm)
+    say("norm %d f %.9e / %.9e" % (norm, got["f"], want["f"]))
+    assert abs(got["f"] - want["f"]) <= 1e-7 * abs(want["f"])
+    for k in range(2):
+        assert np.array_equal(got["grad"][k], want["grad"][k]), (k, rel(got["grad"][k], want["grad"][k]))
+
+
+def test_afwi3d_errors():
+    e, og, vel, rho = afwi_setup()
+    pairs, dobs = afwi_shots(og, vel, rho)
+    o = sa.abi.sfd_afwi_opts()
+    o.snap_mode = sa.SFD_SNAP_FP16S
+    import ctypes as C
+    arr = (sa.abi.sfd_shot3d * 1)(pairs[0][0].c())
+    d = np.ascontiguousarray(dobs[0], np.float32)
+    grad = np.zeros(og.nz * og.nx * og.ny, np.float32)
+    f = C.c_double()
+    rc = e.lib.sfd_afwi_gradient3d(e.h, arr, 1, d.ctypes.data_as(C.c_void_p), None, C.byref(o), C.byref(f),
+                                   grad.ctypes.data_as(C.c_void_p), None)
+    assert rc == -4
+    with pytest.raises(sa.SeisFDError):
+        e.afwi_gradient([pairs[0][0]], [dobs[0]], vprho_param=3)

@@ -243,3 +243,96 @@ def test_rtm3d_image_is_the_zero_lag_sum():
     # deterministic: a second call gives the same image
     r2 = O.rtm3d(g, vel, shot, tr)
     assert np.array_equal(r2["image"], r["image"])
+
+
+# ---- 3DAFWI's gradient in 3D ----
+
+def bump3(shape, c, s):
+    zz, xx, yy = np.mgrid[0:shape[0], 0:shape[1], 0:shape[2]]
+    return np.exp(-((zz - c[0]) ** 2 + (xx - c[1]) ** 2 + (yy - c[2]) ** 2) / (2 * s ** 2))
+
+
+def afwi3d_case(snapi=1, density=False, fpeak=25.0):
+    nz, nx, ny = 24, 30, 26
+    g = O.make_grid3d(nz, nx, ny, 10.0, 10.0, 10.0, 1e-3, ntaper=10, apx=300.0, apy=300.0, snapi=snapi)
+    w = ricker(fpeak, 220, 1e-3)
+    rx, ry = np.meshgrid(np.arange(3, nx - 3, 3) * 10.0, np.arange(3, ny - 3, 3) * 10.0)
+    rx, ry = rx.ravel().astype(np.float32), ry.ravel().astype(np.float32)
+    shots = [O.Shot3D(20.0, 150.0, 130.0, w, np.full_like(rx, 20.0), rx, ry)]
+    v0 = np.full((nz, nx, ny), 2000.0, np.float32)
+    vt = v0.copy()
+    vt[12:18, 8:22, 8:18] += 150.0
+    rho0 = rt = None
+    if density:
+        rho0 = np.full((nz, nx, ny), 2.0, np.float32)
+        rt = rho0.copy()
+        rt[15:20, 6:20, 8:20] += 0.3
+    dobs = [O.modeling3d(g, vt, s, extract_kind=3, rho=rt) for s in shots]
+    return g, v0, rho0, shots, dobs
+
+
+@pytest.mark.parametrize("snapi", [1, 2])
+def test_afwi3d_gradient_constant_density(snapi):
+    """dJ / <grad, dv> = -2 snapi in a homogeneous model (the 2D pin's
+    relation; FP32 here, so 2%)."""
+    g, v0, _, shots, dobs = afwi3d_case(snapi)
+    r = O.afwi3d(g, v0, shots, dobs)
+    J = lambda v: O.afwi3d(g, v, shots, dobs)["f"]
+    for c in ((14, 12, 12), (15, 18, 14), (18, 15, 10)):
+        dv = bump3(v0.shape, c, 2.5).astype(np.float32)
+        D = (J(v0 + dv) - J(v0 - dv)) / 2
+        q = D / float(np.sum(r["grad"][0].astype(np.float64) * dv))
+        assert abs(q + 2 * snapi) < 0.02 * 2 * snapi, (c, q)
+
+
+def test_afwi3d_rhovp_vp_gradient():
+    """With a density the vp gradient carries d(v^2)/dv's 2: ratio -snapi."""
+    g, v0, rho0, shots, dobs = afwi3d_case(1, density=True, fpeak=15.0)
+    r = O.afwi3d(g, v0, shots, dobs, rho=rho0)
+    J = lambda v: O.afwi3d(g, v, shots, dobs, rho=rho0)["f"]
+    for c in ((14, 12, 12), (18, 15, 10)):
+        dv = bump3(v0.shape, c, 2.5).astype(np.float32)
+        D = (J(v0 + dv) - J(v0 - dv)) / 2
+        q = D / float(np.sum(r["grad"][0].astype(np.float64) * dv))
+        assert abs(q + 1.0) < 0.06, (c, q)
+
+
+def test_afwi3d_rhovp_y_term_mirrors_x_term():
+    """A model varying along x, and the same model turned to vary along y with
+    the geometry turned with it (dx = dy, positions on the grid so the x-sinc
+    y weights are the delta): both gradients come out turned, to FP32
+    rounding.  The 2D pin fixes the x and z terms of the v-rho condition; this
+    fixes its y term (dyi, kp/km) and the 3D bookkeeping to match them."""
+    nz, n = 18, 22
+    g = O.make_grid3d(nz, n, n, 10.0, 10.0, 10.0, 1e-3, ntaper=10, apx=300.0, apy=300.0, snapi=2)
+    x = np.arange(n) / n
+    v1 = (1900.0 + 300.0 * np.sin(2 * np.pi * x)).astype(np.float32)
+    r1 = (2.0 + 0.5 * np.cos(2 * np.pi * x)).astype(np.float32)
+    vx = np.ascontiguousarray(np.broadcast_to(v1[None, :, None], (nz, n, n))).astype(np.float32)
+    rx_ = np.ascontiguousarray(np.broadcast_to(r1[None, :, None], (nz, n, n))).astype(np.float32)
+    vy = np.ascontiguousarray(vx.transpose(0, 2, 1))
+    ry_ = np.ascontiguousarray(rx_.transpose(0, 2, 1))
+    w = ricker(20.0, 160, 1e-3)
+    ra = np.array([40.0, 90.0, 160.0, 120.0], np.float32)
+    rb = np.array([60.0, 140.0, 80.0, 110.0], np.float32)   # window centres on the grid in x and y
+    rz = np.full(4, 30.0, np.float32)
+    sx = O.Shot3D(20.0, 100.0, 120.0, w, rz, ra, rb)
+    sy = O.Shot3D(20.0, 120.0, 100.0, w, rz, rb, ra)
+    dx = [O.modeling3d(g, (vx * 0.95).astype(np.float32), sx, extract_kind=3, rho=rx_)]
+    dy = [O.modeling3d(g, (vy * 0.95).astype(np.float32), sy, extract_kind=3, rho=ry_)]
+    gx = O.afwi3d(g, vx, [sx], dx, rho=rx_)
+    gy = O.afwi3d(g, vy, [sy], dy, rho=ry_)
+    assert abs(gy["f"] - gx["f"]) < 1e-5 * abs(gx["f"])
+    for k in range(2):
+        a, b = gx["grad"][k], gy["grad"][k].transpose(0, 2, 1)
+        assert np.abs(a).max() > 0
+        assert rel(b, a) < 1e-4, (k, rel(b, a))
+
+
+def test_afwi3d_vprho_change_of_variables():
+    g, v0, rho0, shots, dobs = afwi3d_case(1, density=True, fpeak=15.0)
+    rho0 = (rho0 + 0.1 * np.sin(np.arange(v0.size).reshape(v0.shape) * 0.01)).astype(np.float32)
+    a = O.afwi3d(g, v0, shots, dobs, rho=rho0, vprho=0)
+    b = O.afwi3d(g, v0, shots, dobs, rho=rho0, vprho=1)
+    assert np.array_equal(b["grad"][1], ((1.0 / v0).astype(np.float32) * a["grad"][1]).astype(np.float32))
+    assert np.array_equal(b["grad"][0], (a["grad"][0] - (rho0 / v0).astype(np.float32) * a["grad"][1]).astype(np.float32))
