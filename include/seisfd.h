@@ -664,6 +664,23 @@ int  sfd_forward3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, float *t
  * shot (nt, ntr) time fastest, concatenated. */
 int  sfd_rtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *traces_in, float *image,
                float *illum);
+/* 3DRTM.x LSRTM=1 in 3D (LSRTM_to_memory, to_disk_to_memory_RTM.f90:494-579,
+ * twoD false): the forward pass with the second-differenced source
+ * (ReadSouVelRho.f90:318-326) storing snapshots, then reverse time on the
+ * backward model when set with the adjoint-style 3D stencil
+ * (FDderivatives.f90:780-819), Injection_LSRTM_sinc of the traces (weight
+ * 1/(dz dx dy), Injection.f90:239-272) and
+ * Imaging_condition_LSRTM_sourceonly_from_memory (Imaging.f90:698-745);
+ * image / illum (nz,nx,ny) accumulate as in sfd_rtm3d.  Constant density. */
+int  sfd_lsrtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *traces_in, float *image,
+                 float *illum);
+/* 3DBORNMOD.x in 3D (BORN_to_memory, to_disk_to_memory_BORN.f90:123-220,
+ * twoD false): the same forward pass, then a second forward pass on the
+ * backward model when set with no source, Injection_Born (Imaging.f90:747-788,
+ * 2 taper m S / v added to u3 every snapi steps) and
+ * Extraction_array_LSRTM_sinc.  refl (nz,nx,ny) is windowed like vel.
+ * traces_out: per shot (nt, ntr), concatenated.  Constant density. */
+int  sfd_born3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *refl, float *traces_out);
 /* 3DAFWI.x's gradient in 3D (AFWI_to_memory, to_disk_to_memory_AFWI.f90:
  * 229-415, twoD false): as sfd_afwi_gradient, with the 3D branch's forward
  * stencil in the reverse pass (:394-403) and, with sfd_set_density3d, the

@@ -292,9 +292,10 @@ contains
   ! ordered only on one thread.
   subroutine ref_propagate3d(nz, nxw, nyw, ntaper, dz, dx, dy, dt, oz, oxw, oyw, lsinc, vel_pad, &
        &                     nsou, souz, soux, souy, soutr, ext, ndat, datz, datx, daty, dattr, nt, &
-       &                     with_rho, rho_pad, tdir, snapi, nsnap, wave) bind(C, name='ref_propagate3d')
+       &                     with_rho, rho_pad, tdir, snapi, nsnap, wave, stencil, inj) bind(C, name='ref_propagate3d')
     use omp_lib
     integer(c_int), value :: nz, nxw, nyw, ntaper, lsinc, nsou, ext, ndat, nt, with_rho, tdir, snapi, nsnap
+    integer(c_int), value :: stencil, inj
     real(c_float), value  :: dz, dx, dy, dt, oz, oxw, oyw
     real(c_float)         :: vel_pad(*), souz(*), soux(*), souy(*), soutr(nt, *), datz(*), datx(*), daty(*)
     real(c_float)         :: dattr(nt, *), rho_pad(*), wave(nz, nxw, nyw, *)
@@ -429,10 +430,18 @@ contains
        if (nsnap > 0) call Extraction_wavefield(bounds, model, elev, grid%u2, genpar, it, dat=wfld)
        if (with_rho /= 0) then
           call FD_3D_derivatives_acoustic_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
-       else
+       else if (stencil == 0) then
           call FD_2nd_3D_derivatives_scalar_forward_grid(genpar, bounds, grid%u2, grid%u3, model)
+       else
+          call FD_2nd_3D_derivatives_scalar_adjoint_grid(genpar, bounds, grid%u2, grid%u3, model)
        end if
-       if (nsou > 0) call Injection_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+       if (nsou > 0) then
+          if (inj == 3) then
+             call Injection_LSRTM_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+          else
+             call Injection_sinc(bounds, model, sou(1:nsou), grid%u3, genpar, it)
+          end if
+       end if
        call FD_2nd_time_derivative_grid(genpar, bounds, grid)
        call Boundary_3d(genpar, bounds, grid, model, hig)
        call FDswaptime_pointer(grid)

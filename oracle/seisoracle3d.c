@@ -817,3 +817,254 @@ int orc_f_afwi3d(const sfd_grid3d *g, const float *vel_full, const float *rho_fu
     *fout = f;
     return 0;
 }
+
+/* FDderivatives.f90:780-819 FD_2nd_3D_derivatives_scalar_adjoint_grid:
+ * tmp = vel^2 u2 over the whole padded array, then u3 = (tx + tz) + ty of the
+ * 25-point stencil of tmp on the FD region (no v^2 factor after). */
+static void stencil3_adj(P3 *P, const float *u2, float *u3)
+{
+    const B3 *b = &P->b;
+    const float *c = P->c;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY;
+    float *tmp = (float *)malloc(sizeof(float) * n);
+    for (size_t q = 0; q < n; q++) tmp[q] = P->vel[q] * P->vel[q] * u2[q];
+#pragma omp parallel for schedule(static)
+    for (int k = b->nmin3; k <= b->nmax3; k++)
+        for (int j = b->nmin2; j <= b->nmax2; j++)
+            for (int i = b->nmin1; i <= b->nmax1; i++) {
+#define T(ii, jj, kk) A3(P, tmp, ii, jj, kk)
+                float tz = c[0] * T(i, j, k) + c[1] * (T(i + 1, j, k) + T(i - 1, j, k)) +
+                           c[2] * (T(i + 2, j, k) + T(i - 2, j, k)) + c[3] * (T(i + 3, j, k) + T(i - 3, j, k)) +
+                           c[4] * (T(i + 4, j, k) + T(i - 4, j, k));
+                float tx = c[5] * T(i, j, k) + c[6] * (T(i, j + 1, k) + T(i, j - 1, k)) +
+                           c[7] * (T(i, j + 2, k) + T(i, j - 2, k)) + c[8] * (T(i, j + 3, k) + T(i, j - 3, k)) +
+                           c[9] * (T(i, j + 4, k) + T(i, j - 4, k));
+                float ty = c[10] * T(i, j, k) + c[11] * (T(i, j, k + 1) + T(i, j, k - 1)) +
+                           c[12] * (T(i, j, k + 2) + T(i, j, k - 2)) + c[13] * (T(i, j, k + 3) + T(i, j, k - 3)) +
+                           c[14] * (T(i, j, k + 4) + T(i, j, k - 4));
+#undef T
+                A3(P, u3, i, j, k) = tx + tz + ty;
+            }
+    free(tmp);
+}
+
+/* Injection_LSRTM_sinc (Injection.f90:239-272): the traces by the 3D sinc
+ * with the weight 1/(dz dx dy) in place of v^2, in trace order. */
+static void inject3_w(P3 *P, float *u, const Pt3 *pt, int ntr, const float *tr, int nt, int it, int h, float w)
+{
+    for (int r = 0; r < ntr; r++) {
+        const int *ic = pt[r].ic;
+        const float s = tr[(size_t)r * nt + (it - 1)];
+        for (int k = -h; k <= h; k++)
+            for (int j = -h; j <= h; j++)
+                for (int i = -h; i <= h; i++)
+                    A3(P, u, i + ic[0], j + ic[1], k + ic[2]) =
+                        A3(P, u, i + ic[0], j + ic[1], k + ic[2]) + w * s * pt[r].wz[h + i] * pt[r].wx[h + j] *
+                                                                        pt[r].wx[h + k];
+    }
+}
+
+static void rotate3(float **buf)
+{
+    float *t = buf[0];
+    buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+}
+
+/* readsou with Born / LSRTM (ReadSouVelRho.f90:318-326): the source's second
+ * time difference. */
+static void second_diff3(float *tr, int nt, float dt)
+{
+    float *o = (float *)malloc(sizeof(float) * (size_t)nt);
+    memcpy(o, tr, sizeof(float) * (size_t)nt);
+    for (int it = 1; it <= nt; it++) tr[it - 1] = 0;
+    for (int it = 2; it <= nt - 1; it++) tr[it - 1] = tr[it - 1] + (-2.f * o[it - 1] + o[it - 2] + o[it]) / (dt * dt);
+    free(o);
+}
+
+/* The forward pass shared by LSRTM and Born in 3D: the second-differenced
+ * source by Injection_sinc, the forward stencil, snapshots of u2 on the window
+ * every snapi steps. */
+static void fwd_snaps3(P3 *P, const sfd_grid3d *g, const sfd_shot3d *s, float oxw, float oyw, int nxw, int nyw,
+                       float *snaps)
+{
+    const B3 *b = &P->b;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY, nwin = (size_t)g->nz * nxw * nyw;
+    const int h = (int)(g->lsinc * 0.5);
+    float *buf[5];
+    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+    float *wav = (float *)malloc(sizeof(float) * (size_t)s->nt);
+    memcpy(wav, s->wavelet, sizeof(float) * (size_t)s->nt);
+    second_diff3(wav, s->nt, g->dt);
+    Pt3 src;
+    points3(g, oxw, oyw, 1, &s->sz, &s->sx, &s->sy, &src);
+    int counter = 0;
+    for (int it = 1; it <= s->nt; it++) {
+        if (it % g->snapi == 0) {
+            float *S = snaps + (size_t)(counter++) * nwin;
+            for (int k = 1; k <= nyw; k++)
+                for (int j = 1; j <= nxw; j++)
+                    for (int i = 1; i <= g->nz; i++)
+                        S[(i - 1) + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw] = A3(P, buf[3], i, j, k);
+        }
+        step3(P, buf, &src, 1, wav, s->nt, it, h);
+    }
+    for (int q = 0; q < 5; q++) free(buf[q]);
+    free(wav);
+}
+
+/* 3DRTM.x LSRTM=1 with genpar%twoD false, LSRTM_to_memory
+ * (to_disk_to_memory_RTM.f90:494-579): the forward pass of fwd_snaps3, then
+ * reverse time on vel_bwd when given with the adjoint-style 3D stencil,
+ * Injection_LSRTM_sinc of the receivers' traces and
+ * Imaging_condition_LSRTM_sourceonly_from_memory (Imaging.f90:698-745:
+ * image += 2 t u S / v, illum += S S t); the window added into image / illum
+ * as mod_copy_image.  Constant density only (the driver has no density
+ * branch).  capture (optional): the backward u2 at each imaging step. */
+int orc_f_lsrtm3d(const sfd_grid3d *g, const float *vel_full, const float *vel_bwd, const sfd_shot3d *s,
+                  const float *traces_in, float *image, float *illum, float *capture)
+{
+    P3 P;
+    float oxw, oyw;
+    int nxw, nyw;
+    if (g->snapi < 1) return -1;
+    if (setup3(&P, g, vel_full, NULL, s, &oxw, &nxw, &oyw, &nyw)) return -2;
+    const B3 *b = &P.b;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY, nwin = (size_t)g->nz * nxw * nyw;
+    const int ntsnap = s->nt / g->snapi, h = (int)(g->lsinc * 0.5);
+    float *snaps = (float *)calloc(nwin * (ntsnap > 0 ? ntsnap : 1), sizeof(float));
+    float *img = (float *)calloc(nwin, sizeof(float)), *ill = (float *)calloc(nwin, sizeof(float));
+    float *tx = (float *)malloc(sizeof(float) * nxw), *ty = (float *)malloc(sizeof(float) * nyw), *buf[5];
+    fwd_snaps3(&P, g, s, oxw, oyw, nxw, nyw, snaps);
+    if (vel_bwd) {
+        orc_window_pad3d(g, vel_bwd, oxw, nxw, oyw, nyw, b, P.vel);
+        free(P.gz); free(P.gx); free(P.gy);
+        higdon3(&P);
+    }
+    orc_f_compute_taper(tx, nxw);
+    for (int k = 0; k < nyw; k++) ty[k] = 1.f;
+    if (nyw > 1) orc_f_compute_taper(ty, nyw);
+    Pt3 *rec = (Pt3 *)malloc(sizeof(Pt3) * (size_t)s->ntr);
+    points3(g, oxw, oyw, s->ntr, s->rz, s->rx, s->ry, rec);
+    const float w = 1.f / (g->dz * g->dx * g->dy);
+    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+    int counter = 0;
+    for (int it = s->nt; it >= 1; it--) {
+        if (it % g->snapi == 0) {
+            counter++;
+            const int index = ntsnap - counter + 1;
+            const float *S = snaps + (size_t)(index - 1) * nwin;
+            float *cp = capture ? capture + (size_t)(counter - 1) * nwin : NULL;
+            for (int k = 1; k <= nyw; k++)
+                for (int j = 1; j <= nxw; j++) {
+                    const float t = tx[j - 1] * ty[k - 1];
+                    for (int i = 1; i <= g->nz; i++) {
+                        const size_t q = (i - 1) + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw;
+                        const float u = A3(&P, buf[3], i, j, k);
+                        if (cp) cp[q] = u;
+                        img[q] = img[q] + 2.f * t * u * S[q] / A3(&P, P.vel, i, j, k);
+                        ill[q] = ill[q] + S[q] * S[q] * t;
+                    }
+                }
+        }
+        stencil3_adj(&P, buf[3], buf[4]);
+        inject3_w(&P, buf[4], rec, s->ntr, traces_in, s->nt, it, h, w);
+        update3(&P, buf[0], buf[1], buf[2], buf[3], buf[4]);
+        boundary3(&P, buf[2], buf[3], buf[4]);
+        rotate3(buf);
+    }
+    const size_t N = (size_t)g->nz * g->nx * g->ny;
+    for (int pass = 0; pass < 2; pass++) {
+        float *dst = pass == 0 ? image : illum;
+        const float *src_w = pass == 0 ? img : ill;
+        if (!dst) continue;
+        float *tmp = (float *)calloc(N, sizeof(float));
+        for (int k = 1; k <= nyw; k++) {
+            const int iy = nint3(((float)(k - 1) * g->dy + oyw - g->oy) / g->dy) + 1;
+            if (iy < 1 || iy > g->ny) continue;
+            for (int j = 1; j <= nxw; j++) {
+                const int ix = nint3(((float)(j - 1) * g->dx + oxw - g->ox) / g->dx) + 1;
+                if (ix < 1 || ix > g->nx) continue;
+                float *d = tmp + (size_t)(ix - 1) * g->nz + (size_t)(iy - 1) * g->nz * g->nx;
+                const float *ww = src_w + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw;
+                for (int i = 0; i < g->nz; i++) d[i] = d[i] + ww[i];
+            }
+        }
+        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + tmp[q];
+        free(tmp);
+    }
+    for (int q = 0; q < 5; q++) free(buf[q]);
+    free(snaps); free(img); free(ill); free(tx); free(ty); free(rec);
+    free3(&P);
+    return 0;
+}
+
+/* 3DBORNMOD.x with genpar%twoD false, BORN_to_memory
+ * (to_disk_to_memory_BORN.f90:123-220): the forward pass of fwd_snaps3, then
+ * a second forward pass (vel_bwd when given, mod%vel2) with no source: at every
+ * snapi-th step Injection_Born (Imaging.f90:747-788) adds 2 t m S_counter / v
+ * to u3 after the stencil, and Extraction_array_LSRTM_sinc (gain
+ * 1/(dz dx dy)) records the receivers.  refl: (nz,nx,ny), windowed like vel
+ * (readref + mod_window_pad, nearest columns and lines).  traces: (nt, ntr). */
+int orc_f_born3d(const sfd_grid3d *g, const float *vel_full, const float *vel_bwd, const float *refl,
+                 const sfd_shot3d *s, float *traces)
+{
+    P3 P;
+    float oxw, oyw;
+    int nxw, nyw;
+    if (g->snapi < 1) return -1;
+    if (setup3(&P, g, vel_full, NULL, s, &oxw, &nxw, &oyw, &nyw)) return -2;
+    const B3 *b = &P.b;
+    const size_t n = (size_t)b->NZ * b->NX * b->NY, nwin = (size_t)g->nz * nxw * nyw;
+    const int ntsnap = s->nt / g->snapi, h = (int)(g->lsinc * 0.5);
+    float *snaps = (float *)calloc(nwin * (ntsnap > 0 ? ntsnap : 1), sizeof(float));
+    float *m = (float *)calloc(n, sizeof(float));
+    float *tx = (float *)malloc(sizeof(float) * nxw), *ty = (float *)malloc(sizeof(float) * nyw), *buf[5];
+    fwd_snaps3(&P, g, s, oxw, oyw, nxw, nyw, snaps);
+    orc_window_pad3d(g, refl, oxw, nxw, oyw, nyw, b, m);
+    if (vel_bwd) {
+        orc_window_pad3d(g, vel_bwd, oxw, nxw, oyw, nyw, b, P.vel);
+        free(P.gz); free(P.gx); free(P.gy);
+        higdon3(&P);
+    }
+    orc_f_compute_taper(tx, nxw);
+    for (int k = 0; k < nyw; k++) ty[k] = 1.f;
+    if (nyw > 1) orc_f_compute_taper(ty, nyw);
+    Pt3 *rec = (Pt3 *)malloc(sizeof(Pt3) * (size_t)s->ntr);
+    points3(g, oxw, oyw, s->ntr, s->rz, s->rx, s->ry, rec);
+    const float gain = 1.f / (g->dz * g->dx * g->dy);
+    for (int q = 0; q < 5; q++) buf[q] = (float *)calloc(n, sizeof(float));
+    memset(traces, 0, sizeof(float) * (size_t)s->nt * s->ntr);
+    int counter = 0;
+    for (int it = 1; it <= s->nt; it++) {
+        for (int r = 0; r < s->ntr; r++) {   /* Extraction_array_LSRTM_sinc */
+            const int *ic = rec[r].ic;
+            float *tr = traces + (size_t)r * s->nt + (it - 1), acc = *tr;
+            for (int k = -h; k <= h; k++)
+                for (int j = -h; j <= h; j++)
+                    for (int i = -h; i <= h; i++)
+                        acc = acc + gain * A3(&P, buf[3], i + ic[0], j + ic[1], k + ic[2]) * rec[r].wz[h + i] *
+                                        rec[r].wx[h + j] * rec[r].wx[h + k];
+            *tr = acc;
+        }
+        stencil3(&P, buf[3], buf[4]);
+        if (it % g->snapi == 0) {   /* Injection_Born */
+            const float *S = snaps + (size_t)(counter++) * nwin;
+            for (int k = 1; k <= nyw; k++)
+                for (int j = 1; j <= nxw; j++) {
+                    const float t = tx[j - 1] * ty[k - 1];
+                    for (int i = 1; i <= g->nz; i++) {
+                        const size_t q = (i - 1) + (size_t)(j - 1) * g->nz + (size_t)(k - 1) * g->nz * nxw;
+                        A3(&P, buf[4], i, j, k) =
+                            A3(&P, buf[4], i, j, k) + 2.f * t * A3(&P, m, i, j, k) * S[q] / A3(&P, P.vel, i, j, k);
+                    }
+                }
+        }
+        update3(&P, buf[0], buf[1], buf[2], buf[3], buf[4]);
+        boundary3(&P, buf[2], buf[3], buf[4]);
+        rotate3(buf);
+    }
+    for (int q = 0; q < 5; q++) free(buf[q]);
+    free(snaps); free(m); free(tx); free(ty); free(rec);
+    free3(&P);
+    return 0;
+}

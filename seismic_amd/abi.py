@@ -164,7 +164,8 @@ EXPORTS = [
     "sfd_cnn_create", "sfd_cnn_destroy", "sfd_cnn_get_params", "sfd_cnn_set_params", "sfd_cnn_forward",
     "sfd_cnn_loss_grad", "sfd_cnn_train",
     "sfd_sep_read_header", "sfd_sep_read", "sfd_sep_write", "sfd_sep_gathers",
-    "sfd_create3d", "sfd_destroy3d", "sfd_last_error3d", "sfd_set_model3d", "sfd_set_density3d", "sfd_forward3d", "sfd_rtm3d", "sfd_afwi_gradient3d",
+    "sfd_create3d", "sfd_destroy3d", "sfd_last_error3d", "sfd_set_model3d", "sfd_set_density3d", "sfd_forward3d", "sfd_rtm3d", "sfd_lsrtm3d", "sfd_born3d",
+    "sfd_afwi_gradient3d",
 ]
 
 _lib = None
@@ -187,6 +188,8 @@ def load():
     lib.sfd_last_error3d.restype = C.c_char_p
     lib.sfd_set_model3d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.sfd_rtm3d.argtypes = [C.c_void_p, P(sfd_shot3d), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sfd_lsrtm3d.argtypes = [C.c_void_p, P(sfd_shot3d), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sfd_born3d.argtypes = [C.c_void_p, P(sfd_shot3d), C.c_int, C.c_void_p, C.c_void_p]
     lib.sfd_afwi_gradient3d.argtypes = [C.c_void_p, P(sfd_shot3d), C.c_int, C.c_void_p, C.c_void_p,
                                         P(sfd_afwi_opts), P(C.c_double), C.c_void_p, C.c_void_p]
     lib.sfd_set_density3d.argtypes = [C.c_void_p, C.c_void_p]

@@ -26,6 +26,7 @@
 
 void sfd_host_mksinc(float *sinc, int lsinc, float shift);
 void sfd_host_taper(float *t, int n);
+void second_difference(std::vector<float> &w, float dt);
 
 namespace sfd3 {
 
@@ -307,6 +308,57 @@ __global__ void rhovp3_image_kernel(const float *__restrict__ u2, const float *_
     img[q] = img[q] + 2.f * tmp1 * t / (v * v * v * r);
     img2[q] = img2[q] + tmp * t / (r * r) + tmp1 * t / (r * r * (v * v));
     ill[q] = ill[q] + S[q] * S[q] * t;
+}
+
+// FDderivatives.f90:780-819 FD_2nd_3D_derivatives_scalar_adjoint_grid, first
+// half: tmp = vel^2 u2 over the whole padded array.
+__global__ void __launch_bounds__(256) v2u3_kernel(const float *__restrict__ u2, const float *__restrict__ v2,
+                                                  float *__restrict__ tmp, size_t n) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) tmp[q] = v2[q] * u2[q];
+}
+// second half: u3 = (tx + tz) + ty of the 25-point stencil of tmp, no v^2 after.
+__global__ void __launch_bounds__(256) stencil3_adj_kernel(const float *__restrict__ tmp, float *__restrict__ u3,
+                                                           Geo G, Coef3 C) {
+    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
+    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
+    const size_t q = G.at(iz, ix, iy);
+    const long sx = (long)G.LDZ, sy = (long)G.plane;
+    const float *c = C.c;
+#define T(o) tmp[q + (o)]
+    const float tz = c[0] * T(0) + c[1] * (T(1) + T(-1)) + c[2] * (T(2) + T(-2)) + c[3] * (T(3) + T(-3)) +
+                     c[4] * (T(4) + T(-4));
+    const float tx = c[5] * T(0) + c[6] * (T(sx) + T(-sx)) + c[7] * (T(2 * sx) + T(-2 * sx)) +
+                     c[8] * (T(3 * sx) + T(-3 * sx)) + c[9] * (T(4 * sx) + T(-4 * sx));
+    const float ty = c[10] * T(0) + c[11] * (T(sy) + T(-sy)) + c[12] * (T(2 * sy) + T(-2 * sy)) +
+                     c[13] * (T(3 * sy) + T(-3 * sy)) + c[14] * (T(4 * sy) + T(-4 * sy));
+#undef T
+    u3[q] = tx + tz + ty;
+}
+
+// Imaging_condition_LSRTM_sourceonly_from_memory (Imaging.f90:698-745) in 3D:
+// image += 2 t u S / v, illum += S S t.
+__global__ void lsrtm3_image_kernel(const float *__restrict__ u2, const float *__restrict__ vel, Geo G, int nz,
+                                    int nxw, int i0, const float *__restrict__ S, const float *__restrict__ tx,
+                                    const float *__restrict__ ty, float *__restrict__ img, float *__restrict__ ill) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nz) return;
+    const size_t q = (size_t)i + (size_t)j * nz + (size_t)k * nz * nxw, a = G.at(i + i0, j + i0, k + i0);
+    const float t = tx[j] * ty[k];
+    img[q] = img[q] + 2.f * t * u2[a] * S[q] / vel[a];
+    ill[q] = ill[q] + S[q] * S[q] * t;
+}
+
+// Injection_Born (Imaging.f90:747-788) in 3D on u3: u3 += 2 t m S / v.
+__global__ void born3_kernel(float *__restrict__ u3, const float *__restrict__ m, const float *__restrict__ vel, Geo G,
+                             int nz, int nxw, int i0, const float *__restrict__ S, const float *__restrict__ tx,
+                             const float *__restrict__ ty) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nz) return;
+    const size_t q = (size_t)i + (size_t)j * nz + (size_t)k * nz * nxw, a = G.at(i + i0, j + i0, k + i0);
+    const float t = tx[j] * ty[k];
+    u3[a] = u3[a] + 2.f * t * m[a] * S[q] / vel[a];
 }
 
 // Higdon_factor (Boundary.f90:366-447) at cell q along a line of stride s
@@ -1058,6 +1110,188 @@ static void afwi3(sfd_ctx3d &h, const sfd_shot3d &s, const float *dobs, const fl
     }
 }
 
+// The forward pass of LSRTM and Born in 3D (readsou with Born / LSRTM,
+// ReadSouVelRho.f90:318-326): the second-differenced source by Injection_sinc,
+// the forward stencil, u2 on the window every snapi steps into snaps.
+static void fwd_snaps3(sfd_ctx3d &h, Run3 &R, const sfd_shot3d &s, const InjPlan &src, char *srcmem, float *snaps) {
+    const sfd_grid3d &g = h.g;
+    const size_t nwin = (size_t)g.nz * R.nxw * R.nyw;
+    std::vector<float> w(s.wavelet, s.wavelet + s.nt);
+    second_difference(w, g.dt);
+    const InjDev Is = upload_inj(h, srcmem, src, w.data(), 1, s.nt);
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw);
+    models3(h, R, h.d_vel);
+    start_pass3(h, R, false);
+    float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
+    int counter = 0;
+    for (int it = 1; it <= s.nt; it++) {
+        if (it % g.snapi == 0)
+            hipLaunchKernelGGL(snap3_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3], R.G, g.nz, R.nxw, R.i0,
+                               snaps + (size_t)(counter++) * nwin);
+        step3(h, R, buf, Is, it);
+    }
+    HIPCHK(hipStreamSynchronize(h.st));   // w is released on return
+}
+
+static void tapers3(sfd_ctx3d &h, const Run3 &R, char *dtx, char *dty) {
+    std::vector<float> tx(R.nxw), ty(R.nyw, 1.f);
+    sfd_host_taper(tx.data(), R.nxw);
+    if (R.nyw > 1) sfd_host_taper(ty.data(), R.nyw);
+    HIPCHK(hipMemcpyAsync(dtx, tx.data(), R.nxw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipMemcpyAsync(dty, ty.data(), R.nyw * 4, hipMemcpyHostToDevice, h.st));
+    HIPCHK(hipStreamSynchronize(h.st));
+}
+
+// LSRTM_to_memory with genpar%twoD false (to_disk_to_memory_RTM.f90:494-579).
+static void lsrtm3(sfd_ctx3d &h, const sfd_shot3d &s, const float *traces_in, float *image, float *illum) {
+    const sfd_grid3d &g = h.g;
+    check_shot3(s);
+    Run3 R;
+    geom3(h, s, R);
+    InjPlan src, recp;
+    plan_inj3(R, h.vel_h, h.rho_h, 1, &s.sz, &s.sx, &s.sy, src);
+    plan_inj3(R, h.vel_h, h.rho_h, s.ntr, s.rz, s.rx, s.ry, recp);
+    const float wls = 1.f / (g.dz * g.dx * g.dy);   // Injection_LSRTM_sinc: 1/product(delta(1:3))
+    for (float &v : recp.v2r) v = wls;
+    const int ntsnap = s.nt / g.snapi;
+    const size_t nwin = (size_t)g.nz * R.nxw * R.nyw;
+    Layout E;
+    const size_t o_snap = E.take(nwin * std::max(ntsnap, 1) * 4), o_img = E.take(nwin * 4), o_ill = E.take(nwin * 4),
+                 o_tx = E.take(R.nxw * 4), o_ty = E.take(R.nyw * 4), o_tmp = E.take(R.G.plane * R.G.NY * 4),
+                 o_src = E.take(inj_bytes(src, s.nt)), o_rec = E.take(inj_bytes(recp, (size_t)s.ntr * s.nt));
+    size_t extra_off = 0;
+    alloc3(h, R, E.off, &extra_off);
+    char *ex = R.base + extra_off;
+    float *snaps = (float *)(ex + o_snap), *img = (float *)(ex + o_img), *ill = (float *)(ex + o_ill);
+    float *tmp = (float *)(ex + o_tmp);
+    HIPCHK(hipMemsetAsync(img, 0, nwin * 4, h.st));
+    HIPCHK(hipMemsetAsync(ill, 0, nwin * 4, h.st));
+    tapers3(h, R, ex + o_tx, ex + o_ty);
+    fwd_snaps3(h, R, s, src, ex + o_src, snaps);
+    const InjDev Ir = upload_inj(h, ex + o_rec, recp, traces_in, s.ntr, s.nt);
+    if (h.d_velb) models3(h, R, h.d_velb);
+    start_pass3(h, R, false);
+    float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw), sg((R.fz + 63) / 64, (R.fx + 3) / 4, R.fy), sb(256);
+    const float c12dt2 = 12.f * (g.dt * g.dt), div11 = 1.f / 11.f;
+    int counter = 0;
+    for (int it = s.nt; it >= 1; it--) {
+        if (it % g.snapi == 0) {
+            counter++;
+            const int index = ntsnap - counter + 1;
+            hipLaunchKernelGGL(lsrtm3_image_kernel, wg, dim3(256), 0, h.st, (const float *)buf[3],
+                               (const float *)R.vel, R.G, g.nz, R.nxw, R.i0,
+                               (const float *)(snaps + (size_t)(index - 1) * nwin), (const float *)(ex + o_tx),
+                               (const float *)(ex + o_ty), img, ill);
+        }
+        // the adjoint-style stencil (FDderivatives.f90:780-819), then injection, update, bands
+        hipLaunchKernelGGL(v2u3_kernel, dim3((unsigned)((R.nf + 255) / 256)), dim3(256), 0, h.st,
+                           (const float *)buf[3], (const float *)R.v2, tmp, R.nf);
+        hipLaunchKernelGGL(stencil3_adj_kernel, sg, sb, 0, h.st, (const float *)tmp, buf[4], R.G, R.C);
+        if (Ir.ncell > 0)
+            hipLaunchKernelGGL(inject3_kernel, dim3((Ir.ncell + 255) / 256), dim3(256), 0, h.st, buf[4], Ir.cell,
+                               Ir.cptr, Ir.con, Ir.v2r, Ir.traces, Ir.nt, Ir.ncell, it);
+        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
+                           (const float *)buf[2], (const float *)buf[3], buf[4], R.G, c12dt2, div11);
+        hipLaunchKernelGGL(zband3_kernel, dim3((R.fx + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4],
+                           (const float *)buf[3], (const float *)buf[2], (const float *)R.gz, R.G);
+        hipLaunchKernelGGL(xband3_kernel, dim3((R.fz + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4],
+                           (const float *)buf[3], (const float *)buf[2], (const float *)R.gx, R.G);
+        hipLaunchKernelGGL(yband3_kernel, dim3((R.fz + 63) / 64, R.fx), dim3(64), 0, h.st, buf[4],
+                           (const float *)buf[3], (const float *)buf[2], (const float *)R.gy, R.G);
+        float *t = buf[0];
+        buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+    }
+    HIPCHK(hipGetLastError());
+    // mod_copy_image: a zeroed model-size sum of the window, then added
+    const size_t N = (size_t)g.nz * g.nx * g.ny;
+    std::vector<float> w(nwin), tmpm;
+    for (int pass = 0; pass < 2; pass++) {
+        float *dst = pass == 0 ? image : illum;
+        if (!dst) continue;
+        HIPCHK(hipMemcpyAsync(w.data(), pass == 0 ? img : ill, nwin * 4, hipMemcpyDeviceToHost, h.st));
+        HIPCHK(hipStreamSynchronize(h.st));
+        tmpm.assign(N, 0.f);
+        copy_window3(g, R, w.data(), tmpm.data());
+        for (size_t q = 0; q < N; q++) dst[q] = dst[q] + tmpm[q];
+    }
+}
+
+// BORN_to_memory with genpar%twoD false (to_disk_to_memory_BORN.f90:123-220).
+static void born3(sfd_ctx3d &h, const sfd_shot3d &s, const float *d_refl, float *traces_out) {
+    const sfd_grid3d &g = h.g;
+    check_shot3(s);
+    Run3 R;
+    geom3(h, s, R);
+    InjPlan src;
+    plan_inj3(R, h.vel_h, h.rho_h, 1, &s.sz, &s.sx, &s.sy, src);
+    std::vector<Rec3> rec(s.ntr);
+    for (int r = 0; r < s.ntr; r++) {   // Extraction_array_LSRTM_sinc: gain 1/(dz dx dy)
+        int ic[3];
+        float dc[3];
+        find_i_d3(&ic[0], &dc[0], s.rz[r], g.oz, g.dz);
+        find_i_d3(&ic[1], &dc[1], s.rx[r], R.oxw, g.dx);
+        find_i_d3(&ic[2], &dc[2], s.ry[r], R.oyw, g.dy);
+        Rec3 &e = rec[r];
+        e.iz = ic[0] - 1 + R.i0;
+        e.ix = ic[1] - 1 + R.i0;
+        e.iy = ic[2] - 1 + R.i0;
+        if (e.iz - 3 < 0 || e.iz + 3 >= R.G.NZ || e.ix - 3 < 0 || e.ix + 3 >= R.G.NX || e.iy - 3 < 0 ||
+            e.iy + 3 >= R.G.NY)
+            throw EngineError(SFD_EINVAL, "receiver outside the padded grid");
+        e.gain = 1.f / (g.dz * g.dx * g.dy);
+        sfd_host_mksinc(e.wz, 7, dc[0] / g.dz);
+        sfd_host_mksinc(e.wx, 7, dc[1] / g.dx);
+    }
+    const int ntsnap = s.nt / g.snapi;
+    const size_t nwin = (size_t)g.nz * R.nxw * R.nyw, ntr_nt = (size_t)s.nt * s.ntr;
+    Layout E;
+    const size_t o_snap = E.take(nwin * std::max(ntsnap, 1) * 4), o_m = E.take(R.G.plane * R.G.NY * 4),
+                 o_tx = E.take(R.nxw * 4), o_ty = E.take(R.nyw * 4), o_trc = E.take(ntr_nt * 4),
+                 o_rec = E.take(sizeof(Rec3) * s.ntr), o_src = E.take(inj_bytes(src, s.nt));
+    size_t extra_off = 0;
+    alloc3(h, R, E.off, &extra_off);
+    char *ex = R.base + extra_off;
+    float *snaps = (float *)(ex + o_snap), *m = (float *)(ex + o_m), *trc = (float *)(ex + o_trc);
+    HIPCHK(hipMemsetAsync(trc, 0, ntr_nt * 4, h.st));
+    HIPCHK(hipMemcpyAsync(ex + o_rec, rec.data(), sizeof(Rec3) * s.ntr, hipMemcpyHostToDevice, h.st));
+    tapers3(h, R, ex + o_tx, ex + o_ty);
+    fwd_snaps3(h, R, s, src, ex + o_src, snaps);
+    // readref + mod_window_pad: the reflectivity windowed and padded like vel
+    const Geo &G = R.G;
+    hipLaunchKernelGGL(pad3_kernel, dim3((G.NZ + 255) / 256, G.NX, G.NY), dim3(256), 0, h.st, d_refl, g.nz, g.nx,
+                       R.dcol, R.nxw, R.dlin, R.nyw, g.ntaper, G, m, (float *)nullptr);
+    if (h.d_velb) models3(h, R, h.d_velb);
+    start_pass3(h, R, false);
+    float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw), sg((R.fz + 63) / 64, (R.fx + 3) / 4, R.fy), sb(256);
+    const float c12dt2 = 12.f * (g.dt * g.dt), div11 = 1.f / 11.f;
+    int counter = 0;
+    for (int it = 1; it <= s.nt; it++) {
+        hipLaunchKernelGGL(extract3_kernel, dim3((s.ntr + 63) / 64), dim3(64), 0, h.st, (const float *)buf[3], G,
+                           (const Rec3 *)(ex + o_rec), s.ntr, 0, trc, s.nt, it);
+        hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)R.v2, buf[4], G,
+                           R.C);
+        if (it % g.snapi == 0)
+            hipLaunchKernelGGL(born3_kernel, wg, dim3(256), 0, h.st, buf[4], (const float *)m, (const float *)R.vel, G,
+                               g.nz, R.nxw, R.i0, (const float *)(snaps + (size_t)(counter++) * nwin),
+                               (const float *)(ex + o_tx), (const float *)(ex + o_ty));
+        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
+                           (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
+        hipLaunchKernelGGL(zband3_kernel, dim3((R.fx + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4],
+                           (const float *)buf[3], (const float *)buf[2], (const float *)R.gz, G);
+        hipLaunchKernelGGL(xband3_kernel, dim3((R.fz + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4],
+                           (const float *)buf[3], (const float *)buf[2], (const float *)R.gx, G);
+        hipLaunchKernelGGL(yband3_kernel, dim3((R.fz + 63) / 64, R.fx), dim3(64), 0, h.st, buf[4],
+                           (const float *)buf[3], (const float *)buf[2], (const float *)R.gy, G);
+        float *t = buf[0];
+        buf[0] = buf[1]; buf[1] = buf[2]; buf[2] = buf[3]; buf[3] = buf[4]; buf[4] = t;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(traces_out, trc, ntr_nt * 4, hipMemcpyDeviceToHost, h.st));
+    HIPCHK(hipStreamSynchronize(h.st));
+}
+
 static std::string g_create_err3d;   // message of the last failed sfd_create3d
 
 extern "C" {
@@ -1145,6 +1379,40 @@ int sfd_rtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *
         size_t off = 0;
         for (int s = 0; s < nshots; s++) {
             rtm3(*h, shots[s], traces_in + off, image, illum);
+            off += (size_t)shots[s].nt * shots[s].ntr;
+        }
+    });
+}
+
+int sfd_lsrtm3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *traces_in, float *image,
+                float *illum) {
+    if (!h || !shots || nshots < 1 || !traces_in || (!image && !illum)) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        if (h->vel_h.empty()) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model3d first");
+        if (!h->rho_h.empty()) throw EngineError(SFD_EUNSUPPORTED, "LSRTM runs with constant density (LSRTM_to_memory)");
+        if (h->g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
+        size_t off = 0;
+        for (int s = 0; s < nshots; s++) {
+            lsrtm3(*h, shots[s], traces_in + off, image, illum);
+            off += (size_t)shots[s].nt * shots[s].ntr;
+        }
+    });
+}
+
+int sfd_born3d(sfd_handle3d h, const sfd_shot3d *shots, int nshots, const float *refl, float *traces_out) {
+    if (!h || !shots || nshots < 1 || !refl || !traces_out) return SFD_EINVAL;
+    return abi_call(&h->err, [&]() {
+        if (h->vel_h.empty()) throw EngineError(SFD_ESTATE, "no model: call sfd_set_model3d first");
+        if (!h->rho_h.empty()) throw EngineError(SFD_EUNSUPPORTED, "Born runs with constant density (BORN_to_memory)");
+        if (h->g.snapi < 1) throw EngineError(SFD_EINVAL, "snapi must be >= 1");
+        const size_t n = (size_t)h->g.nz * h->g.nx * h->g.ny;
+        float *d_refl = nullptr;
+        HIPCHK(hipMalloc(&d_refl, n * 4));
+        std::unique_ptr<float, decltype(&hipFree)> guard(d_refl, &hipFree);
+        HIPCHK(hipMemcpy(d_refl, refl, n * 4, hipMemcpyHostToDevice));
+        size_t off = 0;
+        for (int s = 0; s < nshots; s++) {
+            born3(*h, shots[s], d_refl, traces_out + off);
             off += (size_t)shots[s].nt * shots[s].ntr;
         }
     });

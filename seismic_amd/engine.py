@@ -387,17 +387,27 @@ class Engine3D:
         self._velb = None if vel_bwd is None else self._model(vel_bwd, "backward model")
         self._chk(self.lib.sfd_set_model3d(self.h, _ptr(self._vel), _ptr(self._velb)))
 
-    def rtm(self, shots, traces, image=None, illum=None):
-        """3D RTM; traces per shot (ntr, nt); returns (image, illum) (nz, nx, ny):
-        new arrays holding image / illum (when given; they are not modified)
-        plus this call's images, as the library accumulates."""
+    def rtm(self, shots, traces, image=None, illum=None, lsrtm=False):
+        """3D RTM (LSRTM with lsrtm=True); traces per shot (ntr, nt); returns
+        (image, illum) (nz, nx, ny): new arrays holding image / illum (when
+        given; they are not modified) plus this call's images, as the library
+        accumulates."""
         n = (self.g.nz, self.g.nx, self.g.ny)
         img = np.zeros(n[::-1], np.float32) if image is None else self._model(image, "image").copy()
         ill = np.zeros(n[::-1], np.float32) if illum is None else self._model(illum, "illum").copy()
         arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])
         tr = Engine._cat(traces, shots)
-        self._chk(self.lib.sfd_rtm3d(self.h, arr, len(shots), _ptr(tr), _ptr(img), _ptr(ill)))
+        fn = self.lib.sfd_lsrtm3d if lsrtm else self.lib.sfd_rtm3d
+        self._chk(fn(self.h, arr, len(shots), _ptr(tr), _ptr(img), _ptr(ill)))
         return img.transpose(2, 1, 0), ill.transpose(2, 1, 0)
+
+    def born(self, shots, refl):
+        """3D Born modeling of the reflectivity refl (nz, nx, ny): traces per shot (ntr, nt)."""
+        arr = (abi.sfd_shot3d * len(shots))(*[s.c() for s in shots])
+        m = self._model(refl, "reflectivity")
+        out = np.zeros(sum(s.nt * s.ntr for s in shots), np.float32)
+        self._chk(self.lib.sfd_born3d(self.h, arr, len(shots), _ptr(m), _ptr(out)))
+        return Engine._split(out, shots)
 
     def set_density(self, rho=None):
         """Variable density rho (nz, nx, ny) (genpar%withRho); None: constant density."""

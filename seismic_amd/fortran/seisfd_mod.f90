@@ -176,6 +176,24 @@ module seisfd_mod
        real(c_float), intent(inout) :: image(*), illum(*)   ! (nz, nx, ny), accumulated
      end function sfd_rtm3d
 
+     integer(c_int) function sfd_lsrtm3d(h, shots, nshots, traces, image, illum) bind(C, name='sfd_lsrtm3d')
+       import :: c_int, c_ptr, c_float, sfd_shot3d
+       type(c_ptr), value :: h
+       type(sfd_shot3d), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: traces(*)
+       real(c_float), intent(inout) :: image(*), illum(*)   ! (nz, nx, ny), accumulated
+     end function sfd_lsrtm3d
+
+     integer(c_int) function sfd_born3d(h, shots, nshots, refl, traces) bind(C, name='sfd_born3d')
+       import :: c_int, c_ptr, c_float, sfd_shot3d
+       type(c_ptr), value :: h
+       type(sfd_shot3d), intent(in) :: shots(*)
+       integer(c_int), value :: nshots
+       real(c_float), intent(in) :: refl(*)                ! (nz, nx, ny)
+       real(c_float), intent(inout) :: traces(*)
+     end function sfd_born3d
+
      integer(c_int) function sfd_afwi_gradient3d(h, shots, nshots, dobs, mute, opt, misfit, grad, illum) &
           bind(C, name='sfd_afwi_gradient3d')
        import :: c_int, c_ptr, c_float, c_double, sfd_shot3d, sfd_afwi_opts

@@ -775,7 +775,7 @@ def rho_half3d(g, rho_pad, b):
 
 
 def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None, sources=None, tdir=1, snapi=1,
-                    nsnap=0):
+                    nsnap=0, stencil=0, inj=1):
     """rho_pad: padded density (withRho); the wavelet is passed as given
     (readsou's half-step shift is the caller's, as in the drivers).
     sources: (z, x, y, traces (n, nt)) injected instead of the shot's source
@@ -785,7 +785,8 @@ def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None, sour
     oxw, nxw, oyw, nyw = win
     fn = ref().ref_propagate3d
     fn.argtypes = ([C.c_int] * 4 + [C.c_float] * 7 + [C.c_int] + [C.c_void_p] + [C.c_int] + [C.c_void_p] * 4
-                   + [C.c_int] * 2 + [C.c_void_p] * 4 + [C.c_int] * 2 + [C.c_void_p] + [C.c_int] * 3 + [C.c_void_p])
+                   + [C.c_int] * 2 + [C.c_void_p] * 4 + [C.c_int] * 2 + [C.c_void_p] + [C.c_int] * 3 + [C.c_void_p]
+                   + [C.c_int] * 2)
     if sources is None:
         sz = np.array([shot.sz], np.float32)
         sx = np.array([shot.sx], np.float32)
@@ -805,7 +806,7 @@ def ref_propagate3d(g, vel_pad, win, shot, ext, rho_pad=None, wavelet=None, sour
     try:
         fn(g.nz, nxw, nyw, g.ntaper, g.dz, g.dx, g.dy, g.dt, g.oz, oxw, oyw, g.lsinc, ptr(vel_pad),
            nsou, ptr(sz), ptr(sx), ptr(sy), ptr(st), ext, shot.ntr, ptr(shot.rz), ptr(shot.rx), ptr(shot.ry), ptr(dtr),
-           shot.nt, 0 if rho_pad is None else 1, ptr(rp), tdir, snapi, nsnap, ptr(wave))
+           shot.nt, 0 if rho_pad is None else 1, ptr(rp), tdir, snapi, nsnap, ptr(wave), stencil, inj)
     finally:
         if rho_pad is not None:
             libc.mallopt(-6, 0)
@@ -831,6 +832,37 @@ def afwi3d(g, vel, shots, dobs, rho=None, mute=None, norm=2, norm_thresh=0.0, vp
     assert rc == 0, rc
     t = lambda a: a.reshape(g.ny, g.nx, g.nz).transpose(2, 1, 0)
     return dict(f=f.value, grad=np.stack([t(grad[:N]), t(grad[N:])]), illum=t(ill))
+
+
+def lsrtm3d(g, vel, shot, traces_in, vel_bwd=None):
+    """orc_f_lsrtm3d: dict(image, illum (nz, nx, ny), capture [ntsnap, nyw, nxw, nz])."""
+    L = oracle()
+    oxw, nxw, oyw, nyw = window3d(g, shot)
+    ns = shot.nt // g.snapi
+    img = np.zeros((g.ny, g.nx, g.nz), np.float32)
+    ill = np.zeros_like(img)
+    cp = np.zeros((max(ns, 1), nyw, nxw, g.nz), np.float32)
+    s = shot.c()
+    L.orc_f_lsrtm3d.argtypes = [C.c_void_p] * 8
+    vb = None if vel_bwd is None else model3d_buf(vel_bwd)
+    rc = L.orc_f_lsrtm3d(C.byref(g), ptr(model3d_buf(vel)), None if vb is None else ptr(vb), C.byref(s),
+                         ptr(np.ascontiguousarray(traces_in, np.float32)), ptr(img), ptr(ill), ptr(cp))
+    assert rc == 0, rc
+    t = lambda a: a.transpose(2, 1, 0)
+    return dict(image=t(img), illum=t(ill), capture=cp[:ns])
+
+
+def born3d(g, vel, refl, shot, vel_bwd=None):
+    """orc_f_born3d: traces (ntr, nt)."""
+    L = oracle()
+    tr = np.zeros((shot.ntr, shot.nt), np.float32)
+    s = shot.c()
+    L.orc_f_born3d.argtypes = [C.c_void_p] * 6
+    vb = None if vel_bwd is None else model3d_buf(vel_bwd)
+    rc = L.orc_f_born3d(C.byref(g), ptr(model3d_buf(vel)), None if vb is None else ptr(vb), ptr(model3d_buf(refl)),
+                        C.byref(s), ptr(tr))
+    assert rc == 0, rc
+    return tr
 
 
 def rtm3d(g, vel, shot, traces_in, vel_bwd=None, rho=None):

@@ -313,3 +313,68 @@ def test_afwi3d_errors():
     assert rc == -4
     with pytest.raises(sa.SeisFDError):
         e.afwi_gradient([pairs[0][0]], [dobs[0]], vprho_param=3)
+
+
+# ---- 3D LSRTM and Born ----
+
+@pytest.mark.parametrize("vel_bwd,snapi", [(False, 2), (True, 3)])
+def test_lsrtm3d_bitwise(vel_bwd, snapi):
+    say("3d lsrtm vel_bwd=%s snapi=%d" % (vel_bwd, snapi))
+    e, og, vel, vb, rho, gs, os_, tr = rtm_setup(False, vel_bwd, snapi)
+    img, ill = e.rtm([gs], [tr], lsrtm=True)
+    want = O.lsrtm3d(og, vel, os_, tr, vel_bwd=vb)
+    assert np.abs(want["image"]).max() > 0
+    assert np.array_equal(img, want["image"]), rel(img, want["image"])
+    assert np.array_equal(ill, want["illum"]), rel(ill, want["illum"])
+    # LSRTM's adjoint-style stencil differs from RTM's forward one
+    plain = O.rtm3d(og, vel, os_, tr, vel_bwd=vb)
+    assert not np.allclose(img, plain["image"])
+
+
+def test_lsrtm3d_two_shots_accumulate():
+    e, og, vel, vb, rho, gs, os_, tr = rtm_setup()
+    w = ricker(30.0, 100, 1e-3)
+    g2, o2 = shot_pair(30.0, 90.0, 110.0, w, np.full(3, 25.0, np.float32),
+                       np.array([60.0, 150.0, 210.0], np.float32), np.array([70.0, 80.0, 130.0], np.float32))
+    tr2 = O.modeling3d(og, vel, o2, extract_kind=0)
+    a = O.lsrtm3d(og, vel, os_, tr)
+    b = O.lsrtm3d(og, vel, o2, tr2)
+    img0 = np.full_like(a["image"], 0.5)
+    img, ill = e.rtm([gs, g2], [tr, tr2], image=img0, lsrtm=True)
+    assert np.array_equal(img, ((img0 + a["image"]) + b["image"]).astype(np.float32))
+    assert np.array_equal(ill, (a["illum"] + b["illum"]).astype(np.float32))
+
+
+def refl3d(nz, nx, ny, seed=4):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((nz, nx, ny), np.float32)
+    m[8:11, :, :] = 0.1
+    m += (0.02 * rng.standard_normal((nz, nx, ny))).astype(np.float32)
+    return m
+
+
+@pytest.mark.parametrize("vel_bwd,snapi", [(False, 1), (True, 2), (False, 3)])
+def test_born3d_bitwise(vel_bwd, snapi):
+    say("3d born vel_bwd=%s snapi=%d" % (vel_bwd, snapi))
+    e, og, vel, vb, rho, gs, os_, tr = rtm_setup(False, vel_bwd, snapi)
+    m = refl3d(og.nz, og.nx, og.ny)
+    got = e.born([gs], m)[0]
+    want = O.born3d(og, vel, m, os_, vel_bwd=vb)
+    assert np.abs(want).max() > 0
+    assert np.array_equal(got, want), rel(got, want)
+
+
+def test_born3d_two_shots_and_density_rejected():
+    e, og, vel, vb, rho, gs, os_, tr = rtm_setup()
+    m = refl3d(og.nz, og.nx, og.ny, seed=5)
+    w = ricker(30.0, 100, 1e-3)
+    g2, o2 = shot_pair(30.0, 90.0, 110.0, w, np.full(3, 25.0, np.float32),
+                       np.array([60.0, 150.0, 210.0], np.float32), np.array([70.0, 80.0, 130.0], np.float32))
+    got = e.born([gs, g2], m)
+    assert np.array_equal(got[0], O.born3d(og, vel, m, os_))
+    assert np.array_equal(got[1], O.born3d(og, vel, m, o2))
+    e.set_density(rho3d(og.nz, og.nx, og.ny))
+    with pytest.raises(sa.SeisFDError):
+        e.born([gs], m)
+    with pytest.raises(sa.SeisFDError):
+        e.rtm([gs], [tr], lsrtm=True)
