@@ -558,6 +558,11 @@ typedef struct {
     int   illum_h0;        /* 1: wrap the operator in the illumination diagonal
                               W = illum^(-1/2) (compute_fct_gdt's illumination at
                               the newest evaluated model): H0 = g W N(W q)     */
+    int   coarsen;         /* 0 or 1: the network runs on the model grid; r > 1:
+                              on the r-times coarser grid (r x r block means R,
+                              bilinear prolongation P back): H0 = g W P N(R W q),
+                              trained on R g_k -> R s_k; with use_net 0 the
+                              smoother g W P R W q alone                        */
 } sfd_precond_opts;
 
 /* fctgdt of Inversion.f90:31-39 with the model passed explicitly: evaluate

@@ -143,7 +143,7 @@ class sfd_precond_opts(C.Structure):
     _fields_ = [
         ("enable", C.c_int), ("train_after", C.c_int), ("train_steps", C.c_int), ("lr", C.c_float),
         ("seed", C.c_int), ("max_evals", C.c_int), ("max_fallbacks", C.c_int), ("target", C.c_int),
-        ("use_net", C.c_int), ("illum_h0", C.c_int),
+        ("use_net", C.c_int), ("illum_h0", C.c_int), ("coarsen", C.c_int),
     ]
 
 

@@ -190,29 +190,39 @@ struct Cnn {
 // L-BFGS hooks: collect pairs, train once, serve H0, judge preconditioned steps.
 struct CnnPrecond : LbfgsHooks {
     sfd_precond_opts o;
+    const int nz, nx, r, nzc, nxc;   // model grid, coarsening factor, the network's grid
+    const long Pf;                   // model cells
     Cnn net;
     LbfgsDev *d = nullptr;
     std::vector<double *> gk, sk, yk;
     double *tmp = nullptr, *tmp2 = nullptr, *tmp3 = nullptr, *gmax = nullptr;
+    double *cq = nullptr, *cy = nullptr, *cin = nullptr, *ctg = nullptr;   // coarse-grid images (r > 1)
     float *illum = nullptr, *w = nullptr;
     bool trained = false, have_w = false;
     int pc_iterations = 0, fallbacks = 0;
     double train_seconds = 0, loss_first = 0, loss_last = 0;
 
-    CnnPrecond(int nz, int nx, const sfd_precond_opts &opt)
-        : o(opt), net(nz, nx, 2 * std::max(1, opt.train_after), opt.seed) {
-        HIPCHK(hipMalloc(&tmp, net.P * sizeof(double)));
-        HIPCHK(hipMalloc(&tmp2, net.P * sizeof(double)));
-        HIPCHK(hipMalloc(&tmp3, net.P * sizeof(double)));
-        HIPCHK(hipMalloc(&illum, net.P * sizeof(float)));
-        HIPCHK(hipMalloc(&w, net.P * sizeof(float)));
+    static int factor(const sfd_precond_opts &o) { return std::max(1, o.coarsen); }
+    CnnPrecond(int nz_, int nx_, const sfd_precond_opts &opt)
+        : o(opt), nz(nz_), nx(nx_), r(factor(opt)), nzc((nz_ + factor(opt) - 1) / factor(opt)),
+          nxc((nx_ + factor(opt) - 1) / factor(opt)), Pf((long)nz_ * nx_),
+          net(nzc, nxc, 2 * std::max(1, opt.train_after), opt.seed) {
+        if (o.coarsen < 0 || r > 64) throw EngineError(SFD_EINVAL, "coarsen must be 0..64");
+        HIPCHK(hipMalloc(&tmp, Pf * sizeof(double)));
+        HIPCHK(hipMalloc(&tmp2, Pf * sizeof(double)));
+        HIPCHK(hipMalloc(&tmp3, Pf * sizeof(double)));
+        HIPCHK(hipMalloc(&illum, Pf * sizeof(float)));
+        HIPCHK(hipMalloc(&w, Pf * sizeof(float)));
         HIPCHK(hipMalloc(&gmax, 2 * sizeof(double)));
+        if (r > 1)
+            for (double **p : {&cq, &cy, &cin, &ctg}) HIPCHK(hipMalloc(p, net.P * sizeof(double)));
     }
     ~CnnPrecond() override {
         for (double *p : gk) (void)hipFree(p);
         for (double *p : sk) (void)hipFree(p);
         for (double *p : yk) (void)hipFree(p);
-        for (void *p : {(void *)tmp, (void *)tmp2, (void *)tmp3, (void *)gmax, (void *)illum, (void *)w})
+        for (void *p : {(void *)tmp, (void *)tmp2, (void *)tmp3, (void *)gmax, (void *)illum, (void *)w, (void *)cq,
+                        (void *)cy, (void *)cin, (void *)ctg})
             if (p) (void)hipFree(p);
     }
     static void pair_thunk(void *ctx, const double *g, const double *s, const double *y, long n, hipStream_t st) {
@@ -232,20 +242,22 @@ struct CnnPrecond : LbfgsHooks {
     }
     static void h0_thunk(void *ctx, double *q, long n, hipStream_t st) { ((CnnPrecond *)ctx)->apply(q); }
 
+    // the coarse-grid smoother P R alone is usable before the network is trained
+    bool smoother_only() const { return r > 1 && !o.use_net; }
     void attach(LbfgsDev &dd) override {
-        if (dd.n != net.P) throw EngineError(SFD_EINVAL, "preconditioner image size differs from the model size");
+        if (dd.n != Pf) throw EngineError(SFD_EINVAL, "preconditioner image size differs from the model size");
         d = &dd;
         dd.h0 = &h0_thunk;
         dd.h0_ctx = this;
         dd.on_pair = &pair_thunk;
         dd.pair_ctx = this;
-        dd.h0_active = o.illum_h0 != 0;   // a network-free operator is usable from the start
+        dd.h0_active = o.illum_h0 != 0 || smoother_only();   // a network-free operator is usable from the start
     }
     bool wants_illumination() const override { return o.illum_h0 != 0; }
     // W = (illum + 1e-4 max illum)^(-1/2): the pseudo-Hessian diagonal of the
     // summed source illumination, stabilised as compute_fct_gdt does (:338)
     void set_illumination(const float *il, long n) override {
-        if (n != net.P) return;
+        if (n != Pf) return;
         float mx = 0.f;
         for (long i = 0; i < n; i++) mx = std::max(mx, il[i]);
         HIPCHK(hipMemcpyAsync(illum, il, n * sizeof(float), hipMemcpyHostToDevice, net.st));
@@ -255,45 +267,58 @@ struct CnnPrecond : LbfgsHooks {
         have_w = true;
     }
     bool net_on() const { return o.use_net && trained; }
-    // M(src) = W N(W src) * scale, with W = 1 without illumination and N = identity without the network
+    void restrict_to(const double *f, double *c) {
+        hipLaunchKernelGGL(restrict_kernel, stream_grid(net.P), dim3(256), 0, net.st, f, nz, nx, r, nzc, nxc, c);
+    }
+    // M(src) = W P N(R W src) * scale, with W = 1 without illumination, N =
+    // identity without the network and P = R = identity when r = 1
     void op(const double *src, double *dst, double scale) {
         const float *wp = (o.illum_h0 && have_w) ? w : nullptr;
-        const long P = net.P;
-        hipLaunchKernelGGL(scale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, src, 1.0, tmp3);
-        if (net_on()) {
+        hipLaunchKernelGGL(scale_w_kernel, stream_grid(Pf), dim3(256), 0, net.st, Pf, wp, src, 1.0, tmp3);
+        const double *mid = tmp3;
+        if (r > 1 && (net_on() || smoother_only())) {
+            restrict_to(tmp3, cq);
+            if (net_on()) net.apply_odd(cq, cy, 1.0);
+            hipLaunchKernelGGL(prolong_kernel, stream_grid(Pf), dim3(256), 0, net.st,
+                               (const double *)(net_on() ? cy : cq), nzc, nxc, r, nz, nx, dst);
+            mid = dst;
+        } else if (net_on()) {
             net.apply_odd(tmp3, dst, 1.0);
-            hipLaunchKernelGGL(scale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, (const double *)dst, scale,
-                               dst);
-        } else {
-            hipLaunchKernelGGL(scale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, (const double *)tmp3,
-                               scale, dst);
+            mid = dst;
         }
+        hipLaunchKernelGGL(scale_w_kernel, stream_grid(Pf), dim3(256), 0, net.st, Pf, wp, mid, scale, dst);
         HIPCHK(hipStreamSynchronize(net.st));
     }
     void train() {
         HIPCHK(hipStreamSynchronize(d->st));
         const auto t0 = std::chrono::steady_clock::now();
         const int K = (int)gk.size();
-        const long P = net.P;
         if (o.target == 1) {
             // s_k <- sum_{j >= k} s_j: the accepted update from x_k to the newest model
             for (int k = K - 2; k >= 0; k--)
-                hipLaunchKernelGGL(accumulate_kernel, stream_grid(P), dim3(256), 0, net.st, P,
+                hipLaunchKernelGGL(accumulate_kernel, stream_grid(Pf), dim3(256), 0, net.st, Pf,
                                    (const double *)sk[k + 1], sk[k]);
         }
         // target 2 (secant pairs): N maps y_k to s_k, in the coordinates of the
         // operator W N(W .): inputs W y_k, targets s_k / W
         const bool secant = o.target == 2;
         const float *wp = (o.illum_h0 && have_w) ? w : nullptr;
+        const long P = net.P;
         for (int k = 0; k < K; k++) {
             const double *in = gk[k], *tg = sk[k];
             if (secant) {
-                hipLaunchKernelGGL(scale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, (const double *)yk[k],
-                                   1.0, tmp);
-                hipLaunchKernelGGL(unscale_w_kernel, stream_grid(P), dim3(256), 0, net.st, P, wp, (const double *)sk[k],
-                                   tmp2);
+                hipLaunchKernelGGL(scale_w_kernel, stream_grid(Pf), dim3(256), 0, net.st, Pf, wp,
+                                   (const double *)yk[k], 1.0, tmp);
+                hipLaunchKernelGGL(unscale_w_kernel, stream_grid(Pf), dim3(256), 0, net.st, Pf, wp,
+                                   (const double *)sk[k], tmp2);
                 in = tmp;
                 tg = tmp2;
+            }
+            if (r > 1) {   // the network learns on the coarse grid
+                restrict_to(in, cin);
+                restrict_to(tg, ctg);
+                in = cin;
+                tg = ctg;
             }
             net.maxabs(in, P, gmax);
             net.maxabs(tg, P, gmax + 1);
@@ -342,7 +367,7 @@ struct CnnPrecond : LbfgsHooks {
             d->mul_diag(q);
             return;
         }
-        HIPCHK(hipMemcpyAsync(q, tmp2, net.P * sizeof(double), hipMemcpyDeviceToDevice, net.st));
+        HIPCHK(hipMemcpyAsync(q, tmp2, Pf * sizeof(double), hipMemcpyDeviceToDevice, net.st));
         HIPCHK(hipStreamSynchronize(net.st));
     }
     void report(sfd_lbfgs_report *rep) override {

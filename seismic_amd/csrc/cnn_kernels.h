@@ -302,6 +302,44 @@ __global__ void unscale_w_kernel(long n, const float *__restrict__ w, const doub
         y[i] = w ? (w[i] > 0.f ? x[i] / (double)w[i] : 0.0) : x[i];
 }
 
+// Coarse grid of the preconditioner (r x r cells of the (nz, nx) column-major
+// model per coarse cell): c = the mean of each block (partial blocks at the
+// bottom and right edges average the cells they hold).
+__global__ void restrict_kernel(const double *__restrict__ f, int nz, int nx, int r, int nzc, int nxc,
+                                double *__restrict__ c) {
+    const long nc = (long)nzc * nxc;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nc; q += (long)gridDim.x * blockDim.x) {
+        const int I = (int)(q % nzc), J = (int)(q / nzc);
+        const int i1 = min((I + 1) * r, nz), j1 = min((J + 1) * r, nx);
+        double s = 0.0;
+        for (int j = J * r; j < j1; j++)
+            for (int i = I * r; i < i1; i++) s += f[i + (long)j * nz];
+        c[q] = s / (double)((i1 - I * r) * (j1 - J * r));
+    }
+}
+// Back to the model grid: bilinear between coarse-cell centres ((I + 1/2) r -
+// 1/2 in model cells), constant beyond the outermost centres.
+__device__ __forceinline__ void lin1(int i, int r, int nc, int &a, int &b, double &w) {
+    double x = ((double)i + 0.5) / r - 0.5;
+    x = fmin(fmax(x, 0.0), (double)(nc - 1));
+    a = (int)x;
+    b = min(a + 1, nc - 1);
+    w = x - a;
+}
+__global__ void prolong_kernel(const double *__restrict__ c, int nzc, int nxc, int r, int nz, int nx,
+                               double *__restrict__ f) {
+    const long n = (long)nz * nx;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        int i0, i1, j0, j1;
+        double wi, wj;
+        lin1((int)(q % nz), r, nzc, i0, i1, wi);
+        lin1((int)(q / nz), r, nxc, j0, j1, wj);
+        const double a = (1.0 - wi) * c[i0 + (long)j0 * nzc] + wi * c[i1 + (long)j0 * nzc];
+        const double b = (1.0 - wi) * c[i0 + (long)j1 * nzc] + wi * c[i1 + (long)j1 * nzc];
+        f[q] = (1.0 - wj) * a + wj * b;
+    }
+}
+
 // y += x
 __global__ void accumulate_kernel(long n, const double *__restrict__ x, double *__restrict__ y) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)

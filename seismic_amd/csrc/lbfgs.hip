@@ -584,9 +584,11 @@ int lbfgs_drive(const sfd_lbfgs_opts &o, long n, float *x, const float *mask, co
         } catch (const EngineError &err) {
             // A trial model along a preconditioner's direction that the
             // evaluation rejects (non-finite or non-positive velocity, Courant
-            // limit) is a failed preconditioned search: the hook restarts it
-            // plain from the same model.  Any other failure propagates.
-            const bool model_range = err.code == SFD_EINVAL || err.code == SFD_EUNSTABLE;
+            // limit, the sampling stop of a too-low velocity, a non-finite
+            // wavefield) is a failed preconditioned search: the hook restarts
+            // it plain from the same model.  Any other failure propagates.
+            const bool model_range = err.code == SFD_EINVAL || err.code == SFD_EUNSTABLE ||
+                                     err.code == SFD_EALIASED || err.code == SFD_ENOTFINITE;
             if (!(hooks && d.used_h0 && model_range)) throw;
             iflag = hooks->rejected_trial(d);
             if (iflag != 1) throw;

@@ -465,9 +465,9 @@ def lbfgs_opts(niter=10, neval=None, m=5, bound=2, freeze_soft=True, vpmin=1500.
 
 
 def precond_opts(train_after=3, train_steps=300, lr=2e-3, seed=1, max_evals=4, max_fallbacks=3, enable=1,
-                 target=0, use_net=1, illum_h0=0):
+                 target=0, use_net=1, illum_h0=0, coarsen=0):
     o = abi.sfd_precond_opts()
-    o.target, o.use_net, o.illum_h0 = int(target), int(use_net), int(illum_h0)
+    o.target, o.use_net, o.illum_h0, o.coarsen = int(target), int(use_net), int(illum_h0), int(coarsen)
     o.enable, o.train_after, o.train_steps, o.lr = int(enable), int(train_after), int(train_steps), float(lr)
     o.seed, o.max_evals, o.max_fallbacks = int(seed), int(max_evals), int(max_fallbacks)
     return o

@@ -105,7 +105,7 @@ module seisfd_mod
   type, bind(C) :: sfd_precond_opts
      integer(c_int) :: enable, train_after, train_steps
      real(c_float)  :: lr
-     integer(c_int) :: seed, max_evals, max_fallbacks, target, use_net, illum_h0
+     integer(c_int) :: seed, max_evals, max_fallbacks, target, use_net, illum_h0, coarsen
   end type sfd_precond_opts
 
   interface

@@ -34,6 +34,7 @@ def test_struct_sizes():
     # int m, nparam; two pointers; int stp1_opt (+4 pad); double; int clip_type (+4 pad)
     assert ctypes.sizeof(abi.sfd_lbfgsoc_opts) == 48
     assert ctypes.sizeof(abi.sfd_grid3d) == 16 * 4
+    assert ctypes.sizeof(abi.sfd_precond_opts) == 11 * 4
     assert ctypes.sizeof(abi.sfd_shot3d) == 12 + 8 + 4 + 4 * 8   # 3 floats, 2 ints, pad, 4 pointers
 
 

@@ -145,3 +145,25 @@ def test_preconditioned_search_falls_back_to_plain():
     assert rc["evaluations"] == rp["evaluations"] and rc["iterations"] == rp["iterations"]
     assert np.array_equal(rc["f_eval"], rp["f_eval"])
     assert np.array_equal(xc, xp)
+
+
+@pytest.mark.parametrize("use_net", [1, 0])
+def test_coarse_grid_preconditioner(use_net):
+    """The network on the 4x coarser grid (H0 = g P N(R q), trained on R g_k ->
+    R s_k), or with use_net 0 the coarse-grid smoother P R alone, serving from
+    the first iteration: the run descends at every accepted step and the
+    operator is used."""
+    print("case: coarse-grid preconditioner use_net=%d" % use_net, flush=True)
+    e, shots, dobs, init = _fwi_case()
+    kw = dict(niter=8, neval=32, bound=0, illupow=0.0)
+    _, rc = e.fwi_lbfgs(shots, dobs, init, precond=dict(train_after=3, train_steps=100, coarsen=4,
+                                                         use_net=use_net), **kw)
+    fc = rc["f_eval"][rc["accepted"]]
+    print("iterations %d, with the operator %d, restarts %d, final/initial %.4e, loss %s" % (
+        rc["iterations"], rc["pc_iterations"], rc["pc_fallbacks"], fc[-1] / rc["f_eval"][0], rc["pc_loss"]),
+        flush=True)
+    assert rc["iterations"] >= 4 and fc[-1] < 0.5 * rc["f_eval"][0]
+    assert np.all(np.diff(fc) < 0)
+    assert rc["pc_iterations"] >= (2 if use_net else 4)
+    if use_net:
+        assert rc["pc_train_seconds"] > 0 and rc["pc_loss"][1] < rc["pc_loss"][0]
