@@ -214,6 +214,74 @@ def salt_leg(sa):
             "reverse": pass_roofline(kt, 1, "fd_step_kernel<4,false,false,0> + image_kernel<true>")}
 
 
+def density_leg(sa, nt=2000):
+    """Variable-density forward modeling (FD_2D_derivatives_acoustic_forward_grid)
+    of one shot on the 751 x 2301 faulted model at 4 m, nt 2000, 2301 surface
+    receivers, density from the velocity by Gardner's relation (310 v^0.25),
+    beside the same shot at constant density.  Algorithmic bytes per FD point
+    and step: 24 at constant density (u_-1, u0, u1, u2, v^2 read, u3 written);
+    48 with a density (rho_grad_kernel reads u2 and delp and writes the two
+    staggered gradients, 16; the step reads both gradients, u2, u_-1, u0, u1,
+    v^2 rho and writes u3, 32)."""
+    from seismic_amd import synthetic as syn
+    w = syn.rtm_751x2301_workload(nt=nt)
+    s = shots_of(sa, w)
+    e = engine_for(sa, w)
+    e.set_model(w["true"])
+    rho = (310.0 * np.power(w["true"].astype(np.float64), 0.25)).astype(np.float32)
+    out = {"workload": "forward-751x2301-nt%d-one-shot" % nt}
+    for name, r, bpp in (("constant_density", None, 24.0), ("variable_density", rho, 48.0)):
+        e.set_density(r)
+        e.forward(s)
+        t0 = time.perf_counter()
+        e.forward(s)
+        wall = time.perf_counter() - t0
+        kt = e.kernel_totals()
+        us, pts, steps = kt["us"][0], kt["pts"][0], kt["steps"][0]
+        ach = bpp * pts / (us * 1e-6) / 1e9
+        out[name] = {"Gpts_per_s": round(pts / wall / 1e9, 3), "us_per_step": round(us / max(steps, 1), 3),
+                     "algorithmic_bytes_per_point": bpp, "achieved_GBps": round(ach, 1),
+                     "frac": round(ach / HBM_PEAK_GBS, 4)}
+    e.close()
+    return out
+
+
+def forward3d_leg(sa, n=300, nt=400):
+    """3D forward modeling (genpar%twoD false) of an n^3 model at 10 m (FD
+    region (n+40)^3 with ntaper 20), velocity 1500 -> 3000 m/s with depth,
+    15 Hz Ricker at 20 m depth in the middle, 100 receivers along x at 20 m
+    depth, dt 1 ms, nt 400; constant density, then a density.  The step's
+    algorithmic bytes at constant density: 24 per FD point (the fused step
+    kernel reads u_-1, u0, u1, u2, v^2 and writes u3); achieved = those bytes
+    over the whole call's wall time divided by nt (bands, extraction and
+    set-up included)."""
+    from seismic_amd.synthetic import ricker
+    g = sa.make_grid3d(n, n, n, 10.0, 10.0, 10.0, 1e-3, ntaper=20, aperture_x=10.0 * (n - 1),
+                       aperture_y=10.0 * (n - 1), fmax=0.0)
+    z = np.arange(n, dtype=np.float32)[:, None, None] / n
+    vel = np.ascontiguousarray(np.broadcast_to(1500.0 + 1500.0 * z, (n, n, n)), np.float32)
+    e = sa.Engine3D(g)
+    e.set_model(vel)
+    rx = np.linspace(100.0, 10.0 * (n - 10), 100).astype(np.float32)
+    shot = sa.Shot3D(20.0, 5.0 * n, 5.0 * n, ricker(15.0, nt, 1e-3), np.full_like(rx, 20.0), rx,
+                     np.full_like(rx, 5.0 * n))
+    F = (n + 40) ** 3
+    out = {"workload": "forward3d-%d^3-nt%d" % (n, nt), "fd_points": F}
+    for name, rho in (("constant_density", None), ("variable_density", 2000.0 + 500.0 * np.broadcast_to(z, (n, n, n)))):
+        if rho is not None:
+            e.set_density(np.ascontiguousarray(rho, np.float32))
+        e.forward([shot])
+        t0 = time.perf_counter()
+        e.forward([shot])
+        wall = time.perf_counter() - t0
+        out[name] = {"Gpts_per_s": round(F * nt / wall / 1e9, 3), "us_per_step": round(wall / nt * 1e6, 1)}
+    ach = 24.0 * F * nt / (out["constant_density"]["us_per_step"] * nt * 1e-6) / 1e9
+    out["constant_density"].update({"algorithmic_bytes_per_point": 24.0, "achieved_GBps": round(ach, 1),
+                                    "frac": round(ach / HBM_PEAK_GBS, 4)})
+    e.close()
+    return out
+
+
 def cpu_baseline(w, budget_s=15.0):
     """The reference's own Fortran kernels (flang build, OpenMP over columns)
     composed in Propagator.f90 order on a bounded sample of the batched
@@ -272,6 +340,10 @@ def main():
     ap.add_argument("--no-fwi", action="store_true", help="skip the 201x601 FWI time-to-misfit leg")
     ap.add_argument("--no-rtm", action="store_true", help="skip the single-shot 751x2301 RTM leg")
     ap.add_argument("--no-salt", action="store_true", help="skip the 1911x5395 salt gradient leg")
+    ap.add_argument("--no-density", action="store_true", help="skip the 2D variable-density leg")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward leg")
+    ap.add_argument("--leg", choices=["rtm", "fwi", "salt", "density", "3d"],
+                    help="run only this leg and print its JSON (profiling)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the 512 MiB copy kernel (counter calibration) and exit")
@@ -294,6 +366,10 @@ def main():
         sys.exit(0 if rc == 0 else 1)
     info = sa.device_info()
     log("device", info["name"], "CUs", info["cus"])
+    if args.leg:
+        legs = {"rtm": rtm_leg, "fwi": fwi_leg, "salt": salt_leg, "density": density_leg, "3d": forward3d_leg}
+        print(json.dumps({args.leg: legs[args.leg](sa)}), flush=True)
+        return
     b = batched_headline(sa, args)
     value = b["pts"] / b["wall"] / 1e9
     tr = None
@@ -343,6 +419,12 @@ def main():
         if not args.no_salt:
             log("salt gradient leg")
             extra["salt"] = salt_leg(sa)
+        if not args.no_density:
+            log("variable-density leg")
+            extra["density751x2301"] = density_leg(sa)
+        if not args.no_3d:
+            log("3D forward leg")
+            extra["forward3d"] = forward3d_leg(sa)
     cpu = None
     if not args.no_cpu and not args.only_batched:
         log("cpu baseline")

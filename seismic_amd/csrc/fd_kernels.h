@@ -454,48 +454,65 @@ __global__ void __launch_bounds__(256) free_surface_kernel(StepArgs A) {
 // forward_grid, FDderivatives.f90:588-640): the staggered gradients of u2
 // times delp = 1/rho2 on the FD region (8th order), first-order gradients
 // times the boundary's delp on the three operator-padding rows and columns
-// next to it (FD columns / rows only).  One thread per padded cell (grid:
-// row groups, columns, running shots); the outermost row and column and the
-// corners are never written (zero).  c[1..4] / c[6..9]: the staggered
+// next to it (FD columns / rows only).  Grid: 64 x 32 tiles of the padded
+// grid, running shots; the outermost row and column and the corners are
+// never written (zero).  c[1..4] / c[6..9]: the staggered
 // coefficients over dz / dx.
 __global__ void __launch_bounds__(256) rho_grad_kernel(StepArgs A, float dzi, float dxi) {
+    // u2 of a 64 x 32 tile plus 4 cells on every side, staged in LDS once
+    // ([column][row]); each wavefront then takes 8 columns, lane = row.
+    constexpr int RZ = 64, RX = 32, LZ = RZ + 8, LX = RX + 8;
+    __shared__ float t[LX * LZ];
     const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.shot_order)[blockIdx.z]];
-    const int iz = (int)(blockIdx.x * blockDim.x + threadIdx.x), ix = blockIdx.y;
     const int NZ = S.NZ, NX = S.NX;
-    if (iz >= NZ || ix >= NX) return;
+    const int iz0 = (int)blockIdx.x * RZ, ix0 = (int)blockIdx.y * RX;
+    if (iz0 >= NZ || ix0 >= NX) return;
     const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
     if (!A.nbase && (its < 1 || its > S.nt)) return;
-    const bool zin = iz >= 4 && iz <= NZ - 5, xin = ix >= 4 && ix <= NX - 5;
-    const bool zpad = (iz >= 1 && iz <= 3) || (iz >= NZ - 4 && iz <= NZ - 2);
-    const bool xpad = (ix >= 1 && ix <= 3) || (ix >= NX - 4 && ix <= NX - 2);
-    int dz, dx;   // the cell whose delp scales a padding gradient
-    if (zin && xin) {
-        dz = iz; dx = ix;
-    } else if (zpad && xin) {
-        dz = iz <= 3 ? 4 : NZ - 5; dx = ix;
-    } else if (xpad && zin) {
-        dz = iz; dx = ix <= 3 ? 4 : NX - 5;
-    } else {
-        return;
-    }
     const GLB float *u = gp(S.u[A.rot]);
     const int LDZ = S.LDZ, zoff = S.zoff;
-    auto U = [&](int i, int j) -> float { return ldw(u, (unsigned)j * (unsigned)LDZ + (unsigned)(zoff + i)); };
-    const float d = ldw(gp(S.delp), (unsigned)dx * (unsigned)LDZ + (unsigned)(zoff + dz));
-    const float *c = A.c;
-    float gz, gx;
-    if (zin && xin) {
-        gz = (c[1] * (U(iz + 1, ix) - U(iz, ix)) + c[2] * (U(iz + 2, ix) - U(iz - 1, ix)) +
-              c[3] * (U(iz + 3, ix) - U(iz - 2, ix)) + c[4] * (U(iz + 4, ix) - U(iz - 3, ix))) * d;
-        gx = (c[6] * (U(iz, ix + 1) - U(iz, ix)) + c[7] * (U(iz, ix + 2) - U(iz, ix - 1)) +
-              c[8] * (U(iz, ix + 3) - U(iz, ix - 2)) + c[9] * (U(iz, ix + 4) - U(iz, ix - 3))) * d;
-    } else {
-        gz = dzi * (U(iz + 1, ix) - U(iz, ix)) * d;
-        gx = dxi * (U(iz, ix + 1) - U(iz, ix)) * d;
+    for (int k = threadIdx.x; k < LX * LZ; k += 256) {
+        const int r = k % LZ, c = k / LZ;
+        const int gz = min(max(iz0 - 4 + r, 0), NZ - 1), gx = min(max(ix0 - 4 + c, 0), NX - 1);
+        t[k] = ldw(u, (unsigned)gx * (unsigned)LDZ + (unsigned)(zoff + gz));
     }
-    const unsigned q = (unsigned)ix * (unsigned)LDZ + (unsigned)(zoff + iz);
-    stw(gp(S.sz), q, gz);
-    stw(gp(S.sx), q, gx);
+    __syncthreads();
+    const int iz = iz0 + (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    if (iz >= NZ) return;
+    const float *c = A.c;
+    const bool zin = iz >= 4 && iz <= NZ - 5;
+    const bool zpad = (iz >= 1 && iz <= 3) || (iz >= NZ - 4 && iz <= NZ - 2);
+    auto U = [&](int i, int j) -> float { return t[(j - ix0 + 4) * LZ + (i - iz0 + 4)]; };
+    for (int p = 0; p < RX / 4; p++) {
+        const int ix = ix0 + w * (RX / 4) + p;
+        if (ix >= NX) break;
+        const bool xin = ix >= 4 && ix <= NX - 5;
+        const bool xpad = (ix >= 1 && ix <= 3) || (ix >= NX - 4 && ix <= NX - 2);
+        int dz, dx;   // the cell whose delp scales a padding gradient
+        if (zin && xin) {
+            dz = iz; dx = ix;
+        } else if (zpad && xin) {
+            dz = iz <= 3 ? 4 : NZ - 5; dx = ix;
+        } else if (xpad && zin) {
+            dz = iz; dx = ix <= 3 ? 4 : NX - 5;
+        } else {
+            continue;
+        }
+        const float d = ldw(gp(S.delp), (unsigned)dx * (unsigned)LDZ + (unsigned)(zoff + dz));
+        float gz, gx;
+        if (zin && xin) {
+            gz = (c[1] * (U(iz + 1, ix) - U(iz, ix)) + c[2] * (U(iz + 2, ix) - U(iz - 1, ix)) +
+                  c[3] * (U(iz + 3, ix) - U(iz - 2, ix)) + c[4] * (U(iz + 4, ix) - U(iz - 3, ix))) * d;
+            gx = (c[6] * (U(iz, ix + 1) - U(iz, ix)) + c[7] * (U(iz, ix + 2) - U(iz, ix - 1)) +
+                  c[8] * (U(iz, ix + 3) - U(iz, ix - 2)) + c[9] * (U(iz, ix + 4) - U(iz, ix - 3))) * d;
+        } else {
+            gz = dzi * (U(iz + 1, ix) - U(iz, ix)) * d;
+            gx = dxi * (U(iz, ix + 1) - U(iz, ix)) * d;
+        }
+        const unsigned q = (unsigned)ix * (unsigned)LDZ + (unsigned)(zoff + iz);
+        stw(gp(S.sz), q, gz);
+        stw(gp(S.sx), q, gx);
+    }
 }
 
 // Window + edge pad of the full-model density for one shot (as vel):

@@ -1119,7 +1119,7 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
             nzmax = std::max(nzmax, B.shots_h[B.order[k]].NZ);
             nxmax = std::max(nxmax, B.shots_h[B.order[k]].NX);
         }
-        hipLaunchKernelGGL(rho_grad_kernel, dim3((nzmax + 255) / 256, nxmax, nact), dim3(256), 0, stream, A,
+        hipLaunchKernelGGL(rho_grad_kernel, dim3((nzmax + 63) / 64, (nxmax + 31) / 32, nact), dim3(256), 0, stream, A,
                            1.f / g.dz, 1.f / g.dx);
     }
     if (mode == 1 && A.store_snap && A.snap_fp16 && A.snap_global) {
