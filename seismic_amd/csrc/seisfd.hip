@@ -12,10 +12,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/seisfd.h"
@@ -278,6 +280,34 @@ float Engine::padded_vel(const std::vector<float> &vh, const ShotPlan &p, int ic
     int wz = std::min(std::max(icz, 1), g.nz) - 1;
     int wx = std::min(std::max(icx, 1), p.nxw) - 1;
     return vh[(size_t)p.colsrc[wx] * g.nz + wz];
+}
+
+// Runs f(s) for s in [0, n) on up to 16 host threads: the shots' plans are
+// independent (every planner is const and writes only its own plan).  The
+// first failure in shot order is rethrown after all threads have joined.
+template <class F>
+static void for_shots(int n, F f) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min(std::min(n, 16), hw > 0 ? hw : 1));
+    if (nt <= 1) {
+        for (int s = 0; s < n; s++) f(s);
+        return;
+    }
+    std::vector<std::exception_ptr> err(n);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([&, t]() {
+            for (int s = t; s < n; s += nt) {
+                try {
+                    f(s);
+                } catch (...) {
+                    err[s] = std::current_exception();
+                }
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 void Engine::plan_shot(const sfd_shot &s, ShotPlan &p) const {
@@ -1440,11 +1470,11 @@ void Engine::autotune(const sfd_shot *shots, int ns, int snaps) {
         const int steps = cd.tb == 2 ? steps2 : steps1;
         tz = cd.tz; tx = cd.tx; strip_px = cd.px; time_block = cd.tb;
         std::vector<ShotPlan> plans(ns);
-        for (int s = 0; s < ns; s++) {
+        for_shots(ns, [&](int s) {
             plan_shot(shots[s], plans[s]);
             plan_injection(plans[s], vel_h, 1, &shots[s].sz, &shots[s].sx, 1);
             plans[s].n_inj_traces = 1;
-        }
+        });
         BatchNeeds need;
         need.snapshots = snaps >= 0;
         need.snap_fp16 = snaps == SFD_SNAP_FP16S || snaps == SFD_SNAP_FP16G;
@@ -1573,13 +1603,13 @@ void Engine::forward(const sfd_shot *shots, int ns, float *traces_out, int extra
         const int s0 = wv.first, nw = wv.second - wv.first;
         auto ta = std::chrono::steady_clock::now();
         std::vector<ShotPlan> plans(nw);
-        for (int s = 0; s < nw; s++) {
+        for_shots(nw, [&](int s) {
             const sfd_shot &sh = shots[s0 + s];
             plan_shot(sh, plans[s]);
             plan_injection(plans[s], vel_h, 1, &sh.sz, &sh.sx, 1);
             plans[s].n_inj_traces = 1;
             plan_extraction(plans[s], vel_h, sh, extract_kind);
-        }
+        });
         build_batch(fwd_batch, plans, need);
         for (int s = 0; s < nw; s++)
             upload_inj_traces(fwd_batch, s, source_wavelet(shots[s0 + s], s), shots[s0 + s].nt);
@@ -1890,7 +1920,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
             reset_batch(B);
         } else {
             std::vector<ShotPlan> fresh(nw);
-            for (int s = 0; s < nw; s++) {
+            for_shots(nw, [&](int s) {
                 ShotPlan &p = fresh[s];
                 plan_shot(ws[s], p);
                 plan_injection(p, vel_h, 1, &ws[s].sz, &ws[s].sx, 1);
@@ -1899,7 +1929,7 @@ void Engine::rtm(const sfd_shot *shots, int ns, const float *traces_in, float *i
                 plan_injection(tmp, velb_h.empty() ? vel_h : velb_h, ws[s].ntr, ws[s].rz, ws[s].rx, lsrtm ? 3 : 1);
                 p.take_backward(tmp);
                 p.n_inj_traces = std::max(1, ws[s].ntr);
-            }
+            });
             rtm_plans = std::move(fresh);
             build_batch(B, rtm_plans, need);
             B.key = key;
@@ -1988,12 +2018,12 @@ void Engine::born(const sfd_shot *shots, int ns, const float *refl, float *trace
         const sfd_shot *ws = shots + s0;
         auto ta = std::chrono::steady_clock::now();
         std::vector<ShotPlan> plans(nw);
-        for (int s = 0; s < nw; s++) {
+        for_shots(nw, [&](int s) {
             plan_shot(ws[s], plans[s]);
             plan_injection(plans[s], vel_h, 1, &ws[s].sz, &ws[s].sx, 1);
             plans[s].n_inj_traces = 1;
             plan_extraction(plans[s], vel_h, ws[s], EXT_SINC_LSRTM);
-        }
+        });
         build_batch(B, plans, need);
         for (int s = 0; s < nw; s++) {
             std::vector<float> w(ws[s].wavelet, ws[s].wavelet + ws[s].nt);
@@ -2113,7 +2143,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
         const sfd_shot *ws = shots + s0;
         auto ta = std::chrono::steady_clock::now();
         std::vector<ShotPlan> plans(nw);
-        for (int s = 0; s < nw; s++) {
+        for_shots(nw, [&](int s) {
             ShotPlan &p = plans[s];
             plan_shot(ws[s], p);
             plan_injection(p, vel_h, 1, &ws[s].sz, &ws[s].sx, 1);        // Injection_sinc (AMOD)
@@ -2122,7 +2152,7 @@ void Engine::fwi_gradient(const sfd_shot *shots, int ns, const float *dobs, cons
             plan_injection(tmp, vel_h, ws[s].ntr, ws[s].rz, ws[s].rx, 2);  // Injection_simple (AGRAD)
             p.take_backward(tmp);
             p.n_inj_traces = std::max(1, ws[s].ntr);
-        }
+        });
         build_batch(B, plans, need);
         for (int s = 0; s < nw; s++) upload_inj_traces(B, s, ws[s].wavelet, ws[s].nt);
         prepare_models(B, plans, d_vel);
@@ -2324,7 +2354,7 @@ void Engine::afwi_gradient(const sfd_shot *shots, int ns, const float *dobs, con
         const sfd_shot *ws = shots + s0;
         auto ta = std::chrono::steady_clock::now();
         std::vector<ShotPlan> plans(nw);
-        for (int s = 0; s < nw; s++) {
+        for_shots(nw, [&](int s) {
             ShotPlan &p = plans[s];
             plan_shot(ws[s], p);
             plan_injection(p, vel_h, 1, &ws[s].sz, &ws[s].sx, 1);         // Injection_sinc
@@ -2333,7 +2363,7 @@ void Engine::afwi_gradient(const sfd_shot *shots, int ns, const float *dobs, con
             plan_injection(tmp, vel_h, ws[s].ntr, ws[s].rz, ws[s].rx, 1);   // Injection_sinc of the residual
             p.take_backward(tmp);
             p.n_inj_traces = std::max(1, ws[s].ntr);
-        }
+        });
         build_batch(B, plans, need);
         for (int s = 0; s < nw; s++) upload_inj_traces(B, s, source_wavelet(ws[s], s), ws[s].nt);
         HIPCHK(hipStreamSynchronize(stream));   // the shifted wavelets are reused by the next wave

@@ -215,9 +215,11 @@ class Engine:
 
     # compute_fct_gdt
     def fwi_gradient(self, shots, dobs, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
-                     do_x=0, snap_mode=abi.SFD_SNAP_FP32, norm=abi.SFD_NORM_L2, norm_thresh=0.0):
+                     do_x=0, snap_mode=abi.SFD_SNAP_FP32, norm=abi.SFD_NORM_L2, norm_thresh=0.0, residuals=True):
         """dobs None: the copy kept in HBM by set_traces.  norm / norm_thresh:
-        data_nrm_type / data_threshold (SFD_NORM_*)."""
+        data_nrm_type / data_threshold (SFD_NORM_*).  residuals False: the
+        residual gathers stay on the device (None is returned for them), as
+        in the evaluations of sfd_fwi_lbfgs."""
         nz, nx = self.g.nz, self.g.nx
         ob = None if dobs is None else self._cat(dobs, shots)
         mu = None if mute is None else self._cat(mute, shots)
@@ -228,10 +230,11 @@ class Engine:
         f = C.c_double()
         grad = np.zeros(nz * nx, np.float32)
         illu = np.zeros(nz * nx, np.float32)
-        res = np.zeros(sum(s.nt * s.ntr for s in shots), np.float32)
+        res = np.zeros(sum(s.nt * s.ntr for s in shots), np.float32) if residuals else None
         self._chk(self.lib.sfd_fwi_gradient(self.h, self._shots(shots), len(shots), _ptr(ob), _ptr(mu), _ptr(vm),
                                             C.byref(o), C.byref(f), _ptr(grad), _ptr(illu), _ptr(res)))
-        return f.value, grad.reshape(nx, nz).T, illu.reshape(nx, nz).T, self._split(res, shots)
+        return (f.value, grad.reshape(nx, nz).T, illu.reshape(nx, nz).T,
+                self._split(res, shots) if residuals else None)
 
     def fwi_lbfgs(self, shots, dobs, x0, niter=10, neval=None, vpmin=1500.0, vpmax=4500.0, bound=2,
                   freeze_soft=True, m=5, mute=None, vpmask=None, illupow=1.0, rect_z=1, rect_x=1, do_z=0,
