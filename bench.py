@@ -223,9 +223,9 @@ def density_leg(sa, nt=2000):
     receivers, density from the velocity by Gardner's relation (310 v^0.25),
     beside the same shot at constant density.  Algorithmic bytes per FD point
     and step: 24 at constant density (u_-1, u0, u1, u2, v^2 read, u3 written);
-    48 with a density (rho_grad_kernel reads u2 and delp and writes the two
-    staggered gradients, 16; the step reads both gradients, u2, u_-1, u0, u1,
-    v^2 rho and writes u3, 32)."""
+    28 with a density (the step kernel forms the staggered gradients from u2
+    and delp = 1/rho2 itself: u_-1, u0, u1, u2, delp, v^2 rho read, u3
+    written)."""
     from seismic_amd import synthetic as syn
     w = syn.rtm_751x2301_workload(nt=nt)
     s = shots_of(sa, w)
@@ -233,7 +233,7 @@ def density_leg(sa, nt=2000):
     e.set_model(w["true"])
     rho = (310.0 * np.power(w["true"].astype(np.float64), 0.25)).astype(np.float32)
     out = {"workload": "forward-751x2301-nt%d-one-shot" % nt}
-    for name, r, bpp in (("constant_density", None, 24.0), ("variable_density", rho, 48.0)):
+    for name, r, bpp in (("constant_density", None, 24.0), ("variable_density", rho, 28.0)):
         e.set_density(r)
         e.forward(s)
         t0 = time.perf_counter()

@@ -132,10 +132,10 @@ struct ShotDev {
     const int *fs_row;
     const float *fs_w;
     // variable density (withRho): v2 holds v^2 rho; delp = 1/rho2 (rho at the
-    // half cells); sz / sx the staggered gradients of u2 times delp, written by
-    // rho_grad_kernel each step (cells it never writes stay zero)
+    // half cells), which the step kernel's staggered gradients read; sz is
+    // the set-up's scratch (rho2 along z)
     const float *delp;
-    float *sz, *sx;
+    float *sz;
     const float *rho;       // variable density: rho, padded (the v-rho imaging condition reads it)
     float *image2;          // IMG_RHOVP: the rho gradient's window accumulator (nz*nxw)
 };
@@ -168,6 +168,7 @@ struct StepArgs {
     float c[10];            // c0z..c4z, c0x..c4x (FDtypes.f90:367-387)
     float c12dt2;           // 12*dt^2
     float div11;            // 1/11
+    float rdz, rdx;         // 1/dz, 1/dx: variable density's first-order padding gradients
 };
 
 __device__ __forceinline__ size_t addr(const ShotDev &S, int iz, int ix) {
@@ -450,70 +451,6 @@ __global__ void __launch_bounds__(256) free_surface_kernel(StepArgs A) {
     }
 }
 
-// Variable density, each step before the stencil (FD_2D_derivatives_acoustic_
-// forward_grid, FDderivatives.f90:588-640): the staggered gradients of u2
-// times delp = 1/rho2 on the FD region (8th order), first-order gradients
-// times the boundary's delp on the three operator-padding rows and columns
-// next to it (FD columns / rows only).  Grid: 64 x 32 tiles of the padded
-// grid, running shots; the outermost row and column and the corners are
-// never written (zero).  c[1..4] / c[6..9]: the staggered
-// coefficients over dz / dx.
-__global__ void __launch_bounds__(256) rho_grad_kernel(StepArgs A, float dzi, float dxi) {
-    // u2 of a 64 x 32 tile plus 4 cells on every side, staged in LDS once
-    // ([column][row]); each wavefront then takes 8 columns, lane = row.
-    constexpr int RZ = 64, RX = 32, LZ = RZ + 8, LX = RX + 8;
-    __shared__ float t[LX * LZ];
-    const ShotDev &S = A.single ? A.s0 : A.shots[gp(A.shot_order)[blockIdx.z]];
-    const int NZ = S.NZ, NX = S.NX;
-    const int iz0 = (int)blockIdx.x * RZ, ix0 = (int)blockIdx.y * RX;
-    if (iz0 >= NZ || ix0 >= NX) return;
-    const int its = A.dir > 0 ? A.n + 1 : S.nt - A.n;
-    if (!A.nbase && (its < 1 || its > S.nt)) return;
-    const GLB float *u = gp(S.u[A.rot]);
-    const int LDZ = S.LDZ, zoff = S.zoff;
-    for (int k = threadIdx.x; k < LX * LZ; k += 256) {
-        const int r = k % LZ, c = k / LZ;
-        const int gz = min(max(iz0 - 4 + r, 0), NZ - 1), gx = min(max(ix0 - 4 + c, 0), NX - 1);
-        t[k] = ldw(u, (unsigned)gx * (unsigned)LDZ + (unsigned)(zoff + gz));
-    }
-    __syncthreads();
-    const int iz = iz0 + (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    if (iz >= NZ) return;
-    const float *c = A.c;
-    const bool zin = iz >= 4 && iz <= NZ - 5;
-    const bool zpad = (iz >= 1 && iz <= 3) || (iz >= NZ - 4 && iz <= NZ - 2);
-    auto U = [&](int i, int j) -> float { return t[(j - ix0 + 4) * LZ + (i - iz0 + 4)]; };
-    for (int p = 0; p < RX / 4; p++) {
-        const int ix = ix0 + w * (RX / 4) + p;
-        if (ix >= NX) break;
-        const bool xin = ix >= 4 && ix <= NX - 5;
-        const bool xpad = (ix >= 1 && ix <= 3) || (ix >= NX - 4 && ix <= NX - 2);
-        int dz, dx;   // the cell whose delp scales a padding gradient
-        if (zin && xin) {
-            dz = iz; dx = ix;
-        } else if (zpad && xin) {
-            dz = iz <= 3 ? 4 : NZ - 5; dx = ix;
-        } else if (xpad && zin) {
-            dz = iz; dx = ix <= 3 ? 4 : NX - 5;
-        } else {
-            continue;
-        }
-        const float d = ldw(gp(S.delp), (unsigned)dx * (unsigned)LDZ + (unsigned)(zoff + dz));
-        float gz, gx;
-        if (zin && xin) {
-            gz = (c[1] * (U(iz + 1, ix) - U(iz, ix)) + c[2] * (U(iz + 2, ix) - U(iz - 1, ix)) +
-                  c[3] * (U(iz + 3, ix) - U(iz - 2, ix)) + c[4] * (U(iz + 4, ix) - U(iz - 3, ix))) * d;
-            gx = (c[6] * (U(iz, ix + 1) - U(iz, ix)) + c[7] * (U(iz, ix + 2) - U(iz, ix - 1)) +
-                  c[8] * (U(iz, ix + 3) - U(iz, ix - 2)) + c[9] * (U(iz, ix + 4) - U(iz, ix - 3))) * d;
-        } else {
-            gz = dzi * (U(iz + 1, ix) - U(iz, ix)) * d;
-            gx = dxi * (U(iz, ix + 1) - U(iz, ix)) * d;
-        }
-        const unsigned q = (unsigned)ix * (unsigned)LDZ + (unsigned)(zoff + iz);
-        stw(gp(S.sz), q, gz);
-        stw(gp(S.sx), q, gx);
-    }
-}
 
 // Window + edge pad of the full-model density for one shot (as vel):
 // rho into out.  Then rho_half_kernel (twice: along z into tmp, along x into
@@ -703,6 +640,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     static_assert(NT >= 2 * TX && NT >= 2 * TZ, "one thread per Higdon line");
     static_assert(TZ == 64 || TZ == 56, "tiles of 64 or 56 rows (a wavefront's lanes)");
     __shared__ float s_col[NW][TZ + 8]; // per-wavefront z-stencil column
+    __shared__ float s_ucol[NW][RHO ? TZ + 15 : 1];   // variable density: u2 of a column, rows z0-7 .. z0+TZ+7
     // Injection tables are read only inside the column loop and the edge
     // buffers are written only after it (behind a barrier on injection
     // tiles), so the two phases share one LDS block.  Keeping the block small
@@ -826,29 +764,66 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
     const int izc = iz < 0 ? 0 : (iz >= NZ ? NZ - 1 : iz);
     const int hzc = hz < 0 ? 0 : (hz >= NZ ? NZ - 1 : hz);
     const unsigned zoc = (unsigned)(zoff + izc), hzo = (unsigned)(zoff + hzc);
-    // Variable density: the x-window holds the x-gradient sx and the z column
-    // (own rows and halo) the z-gradient sz; u2 of the own cells is read apart.
-    const GLB float *__restrict__ wsrc = RHO ? gp((const float *)S.sx) : u2;
-    const GLB float *__restrict__ hsrc = RHO ? gp((const float *)S.sz) : u2;
+    // Variable density: the stencil's operands are the staggered gradients
+    // of u2 times delp = 1/rho2 (FDderivatives.f90:588-640), formed here from
+    // u2 and delp (8th order on the FD region, first order times the boundary
+    // row's or column's delp on the three padding rows and columns, zero on the
+    // outermost row and column): sx along the lane's
+    // row over the x-window (u2 of columns xs-7 .. xs+PX+7 in registers), sz
+    // down each own column through an LDS column of u2 (below).
+    constexpr int NU = RHO ? PX + 15 : 1;
     float wu[NWIN], wv2[ADJ ? NWIN : 1], hu[PX], hv[ADJ ? PX : 1], zc[RHO ? PX : 1], uo_r[RHO ? PX : 1];
+    float ux[NU], dxw[RHO ? NWIN : 1], uh[RHO ? PX : 1], dzo[RHO ? PX : 1], dzh[RHO ? PX : 1];
     float r1[PX], r0[PX], rm[PX], rv[PX];
+    if constexpr (RHO) {
+        const GLB float *__restrict__ dp = gp(S.delp);
+        // delp's row for a gradient at row r: the boundary row's on the padding rows
+        auto drow = [&](int r) {
+            const int q = r <= 3 ? 4 : (r >= NZ - 4 ? NZ - 5 : r);
+            return q < 0 ? 0 : (q >= NZ ? NZ - 1 : q);
+        };
+        const unsigned dzoc = (unsigned)(zoff + drow(iz)), dhzo = (unsigned)(zoff + drow(hz));
+        const int hr = lane < 7 ? z0 - 7 + lane : z0 + TZ + lane - 7;   // lanes 0..14: rows around the tile
+        const unsigned hro = (unsigned)(zoff + (hr < 0 ? 0 : (hr >= NZ ? NZ - 1 : hr)));
 #pragma unroll
-    for (int k = 0; k < NWIN; k++) {
-        int ix = xs - 4 + k;
-        ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
-        wu[k] = ldw(wsrc, (unsigned)ix * (unsigned)LDZ + zoc);
-        if (ADJ) wv2[k] = ldw(v2p, (unsigned)ix * (unsigned)LDZ + zoc);
+        for (int m = 0; m < NU; m++) {
+            int ix = xs - 7 + m;
+            ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
+            ux[m] = ldw(u2, (unsigned)ix * (unsigned)LDZ + zoc);
+        }
+#pragma unroll
+        for (int k = 0; k < NWIN; k++) {
+            // the padding columns take the boundary column's delp
+            int jd = xs - 4 + k;
+            jd = jd <= 3 ? 4 : (jd >= NX - 4 ? NX - 5 : jd);
+            jd = jd < 0 ? 0 : (jd >= NX ? NX - 1 : jd);
+            dxw[k] = ldw(dp, (unsigned)jd * (unsigned)LDZ + zoc);
+        }
+#pragma unroll
+        for (int p = 0; p < PX; p++) {
+            int ix = xs + p;
+            ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
+            const unsigned cb = (unsigned)ix * (unsigned)LDZ;
+            uh[p] = ldw(u2, cb + hro);
+            dzo[p] = ldw(dp, cb + dzoc);
+            dzh[p] = ldw(dp, cb + dhzo);
+            uo_r[p] = ux[p + 7];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NWIN; k++) {
+            int ix = xs - 4 + k;
+            ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
+            wu[k] = ldw(u2, (unsigned)ix * (unsigned)LDZ + zoc);
+            if (ADJ) wv2[k] = ldw(v2p, (unsigned)ix * (unsigned)LDZ + zoc);
+        }
     }
 #pragma unroll
     for (int p = 0; p < PX; p++) {
         int ix = xs + p;
         ix = ix < 0 ? 0 : (ix >= NX ? NX - 1 : ix);
         const unsigned cb = (unsigned)ix * (unsigned)LDZ;
-        hu[p] = ldw(hsrc, cb + hzo);
-        if (RHO) {
-            zc[p] = ldw(hsrc, cb + zoc);
-            uo_r[p] = ldw(u2, cb + zoc);
-        }
+        if (!RHO) hu[p] = ldw(u2, cb + hzo);
         if (ADJ) hv[p] = ldw(v2p, cb + hzo);
         r1[p] = ldw(u1, cb + zoc);
         r0[p] = ldw(u0, cb + zoc);
@@ -933,6 +908,24 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
         }
     }
     const float *c = A.c;
+    if constexpr (RHO) {
+        // sx on the lane's row over the x-window: 8th order on the FD columns,
+        // first order on the three padding columns each side, zero beyond
+        // (rows other than FD rows feed no result)
+#pragma unroll
+        for (int k = 0; k < NWIN; k++) {
+            const int j = xs - 4 + k;
+#define UXW(d) ux[k + 3 + (d)]
+            float gx = 0.f;
+            if (j >= 4 && j <= NX - 5)
+                gx = (c[6] * (UXW(1) - UXW(0)) + c[7] * (UXW(2) - UXW(-1)) + c[8] * (UXW(3) - UXW(-2)) +
+                      c[9] * (UXW(4) - UXW(-3))) * dxw[k];
+            else if (j >= 1 && j <= NX - 2)
+                gx = A.rdx * (UXW(1) - UXW(0)) * dxw[k];
+#undef UXW
+            wu[k] = gx;
+        }
+    }
     float *col = s_col[wv];
     const int cl = lane_ok ? lane : TZ - 1;   // idle lanes read a valid slot
 
@@ -944,6 +937,24 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #pragma unroll
     for (int p = 0; p < PX; p++) {
         const int ix = xs + p;
+        if constexpr (RHO) {
+            // sz of this column on the lane's row and (lanes 0..7) its halo row,
+            // from u2 rows z0-7 .. z0+TZ+7 staged in the wavefront's LDS column
+            float *uc = s_ucol[wv];
+            if (lane_ok) uc[lane + 7] = ux[p + 7];
+            if (lane < 15) uc[lane < 7 ? lane : TZ + lane] = uh[p];
+            __builtin_amdgcn_wave_barrier();
+            auto szat = [&](int i, int sl, float d) -> float {
+                if (i >= 4 && i <= NZ - 5)
+                    return (c[1] * (uc[sl + 1] - uc[sl]) + c[2] * (uc[sl + 2] - uc[sl - 1]) +
+                            c[3] * (uc[sl + 3] - uc[sl - 2]) + c[4] * (uc[sl + 4] - uc[sl - 3])) * d;
+                if (i >= 1 && i <= NZ - 2) return A.rdz * (uc[sl + 1] - uc[sl]) * d;
+                return 0.f;
+            };
+            zc[p] = lane_ok ? szat(iz, lane + 7, dzo[p]) : 0.f;
+            if (lane < 8) hu[p] = szat(hz, lane < 4 ? lane + 3 : TZ + lane + 3, dzh[p]);
+            __builtin_amdgcn_wave_barrier();
+        }
         const float u2c = RHO ? zc[p] : wu[p + 4];
         // z-stencil column through wavefront-private LDS (all 64 lanes take part)
         if (lane_ok) col[lane + 4] = ADJ ? wv2[p + 4] : u2c;
@@ -1120,7 +1131,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 // find their shot through the logical-tile table.
 template <int PX, bool ADJ, bool SINGLE, int MODE, int TZ, bool RHO>
 __global__ void __launch_bounds__(StepShape<PX>::THREADS)
-__attribute__((amdgpu_waves_per_eu(MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN))) fd_step_kernel(StepArgs A) {
+__attribute__((amdgpu_waves_per_eu(RHO ? 4 : (MODE == 2 ? StepShape<PX>::WAVES_PER_EU : StepShape<PX>::WAVES_LEAN)))) fd_step_kernel(StepArgs A) {
     // mode 3: the lean step plus imaging blocks after the step tiles
     constexpr int BODY = MODE == 3 ? 0 : MODE;
     if constexpr (MODE == 3) {

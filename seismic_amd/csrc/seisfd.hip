@@ -680,7 +680,7 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         size_t ext_tile, ext_rec, ext_pos, ext_w, traces;
         size_t snap, snap_bytes, snap_scale, image, illum, born_m;
         size_t fs_row, fs_w;
-        size_t sz, sx, delp, rho, image2;
+        size_t sz, delp, rho, image2;
     };
     std::vector<Off> off(ns);
     const bool tb = tz == B2Z && tx == B2X;
@@ -698,10 +698,9 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         for (int k = 0; k < kLevels; k++) o.u[k] = L.take(nf * sizeof(float));
         o.v2 = L.take(nf * sizeof(float));
         o.vel = L.take(nf * sizeof(float));
-        o.sz = o.sx = o.delp = o.rho = 0;
+        o.sz = o.delp = o.rho = 0;
         if (has_rho()) {
             o.sz = L.take(nf * sizeof(float));
-            o.sx = L.take(nf * sizeof(float));
             o.delp = L.take(nf * sizeof(float));
             o.rho = L.take(nf * sizeof(float));
         }
@@ -850,7 +849,6 @@ void Engine::build_batch(Batch &B, std::vector<ShotPlan> &plans, const BatchNeed
         S.fs_row = p.fs_row.empty() ? nullptr : (const int *)(base + o.fs_row);
         S.fs_w = p.fs_w.empty() ? nullptr : (const float *)(base + o.fs_w);
         S.sz = has_rho() ? (float *)(base + o.sz) : nullptr;
-        S.sx = has_rho() ? (float *)(base + o.sx) : nullptr;
         S.delp = has_rho() ? (const float *)(base + o.delp) : nullptr;
         S.rho = has_rho() ? (const float *)(base + o.rho) : nullptr;
         S.image2 = need.rhovp ? (float *)(base + o.image2) : nullptr;
@@ -965,8 +963,7 @@ void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float 
         hipLaunchKernelGGL(higdon_tables_kernel, dim3((nk + 255) / 256), dim3(256), 0, stream, S.vel, p.NZ, p.NX,
                            p.LDZ, p.zoff, g.dt, g.dz, g.dx, (float *)S.hz, (float *)S.hx);
         if (has_rho()) {
-            // padded rho, rho2 along z (in sz), 1/rho2 along x (delp), v^2
-            // rho; then the gradient arrays start at zero
+            // padded rho, rho2 along z (in sz), 1/rho2 along x (delp), v^2 rho
             Sinc9 s9;
             sfd_host_mksinc(s9.w, 9, 0.5f);
             const size_t nf = (size_t)p.LDZ * p.NX;
@@ -976,8 +973,6 @@ void Engine::prepare_models(Batch &B, std::vector<ShotPlan> &plans, const float 
             hipLaunchKernelGGL(rho_half_kernel, grd, blk, 0, stream, (const float *)S.sz, (float *)S.delp, p.NZ, p.NX,
                                p.LDZ, p.zoff, s9, 1);
             hipLaunchKernelGGL(v2_rho_kernel, dim3(2048), dim3(256), 0, stream, (float *)S.v2, S.rho, nf);
-            hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, (char *)S.sz, nf * sizeof(float));
-            hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, stream, (char *)S.sx, nf * sizeof(float));
         }
     }
     HIPCHK(hipGetLastError());
@@ -1013,6 +1008,8 @@ StepArgs Engine::base_args(const Batch &B) const {
         A.c[k] = has_rho() ? cr[k] * dzi : cc[k] * dz2;
         A.c[5 + k] = has_rho() ? cr[k] * dxi : cc[k] * dx2;
     }
+    A.rdz = dzi;
+    A.rdx = dxi;
     const float dt2 = g.dt * g.dt;
     A.c12dt2 = 12.f * dt2;
     A.div11 = 1.f / 11.f;
@@ -1141,16 +1138,6 @@ void Engine::launch_step(const Batch &B, StepArgs A, bool adj) {
         int nxmax = 1;
         for (int k = 0; k < nact; k++) nxmax = std::max(nxmax, B.shots_h[B.order[k]].NX - 8);
         hipLaunchKernelGGL(free_surface_kernel, dim3((nxmax + 255) / 256, nact), dim3(256), 0, stream, A);
-    }
-    if (has_rho()) {
-        // the staggered gradients of u2 (after the surface is set) for the stencil
-        int nzmax = 1, nxmax = 1;
-        for (int k = 0; k < nact; k++) {
-            nzmax = std::max(nzmax, B.shots_h[B.order[k]].NZ);
-            nxmax = std::max(nxmax, B.shots_h[B.order[k]].NX);
-        }
-        hipLaunchKernelGGL(rho_grad_kernel, dim3((nzmax + 63) / 64, (nxmax + 31) / 32, nact), dim3(256), 0, stream, A,
-                           1.f / g.dz, 1.f / g.dx);
     }
     if (mode == 1 && A.store_snap && A.snap_fp16 && A.snap_global) {
         // one FP16 scale per snapshot: max |u2| over each window before the step stores it
@@ -1531,7 +1518,7 @@ size_t Engine::shot_bytes(const sfd_shot &s, const BatchNeeds &need) const {
     const size_t ntile = (NZ + tz) / tz * ((NX + tx) / tx + 1);
     const size_t npix = (size_t)g.nz * nxw, ntr = (size_t)std::max(s.ntr, 1), nt = (size_t)s.nt;
     const size_t ntsnap = (size_t)std::max(s.nt / g.snapi, 1);
-    size_t b = (kLevels + 2 + (has_rho() ? 4 : 0)) * LDZ * NX * 4;   // levels, v^2, vel (+ gradients, 1/rho2, rho)
+    size_t b = (kLevels + 2 + (has_rho() ? 3 : 0)) * LDZ * NX * 4;   // levels, v^2, vel (+ set-up scratch, 1/rho2, rho)
     b += 16 * (NX + NZ) * 4 + 2 * NX * 4;                     // Higdon tables, window columns, taper
     b += 3 * nt * ntr * 4;                                    // injected (two layouts) and extracted traces
     b += ntr * 49 * 16 * 4 + ntr * 16 * 4 + ntile * 64;       // contributions (both sets, two-step regions)
