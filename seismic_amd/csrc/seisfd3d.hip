@@ -190,26 +190,6 @@ __global__ void higdon3_kernel(const float *__restrict__ vel, Geo G, float dt, f
 }
 
 
-// FDderivatives.f90:97-131 on the FD region: u3 = v^2 ((tx + tz) + ty)
-__global__ void __launch_bounds__(256) stencil3_kernel(const float *__restrict__ u2, const float *__restrict__ v2,
-                                                       float *__restrict__ u3, Geo G, Coef3 C) {
-    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
-    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
-    const size_t q = G.at(iz, ix, iy);
-    const size_t sx = G.LDZ, sy = G.plane;
-    const float *c = C.c;
-#define U(o) u2[q + (o)]
-    const float tz = c[0] * (U(0)) + c[1] * (U(1) + U(-1)) + c[2] * (U(2) + U(-2)) + c[3] * (U(3) + U(-3)) +
-                     c[4] * (U(4) + U(-4));
-    const float tx = c[5] * (U(0)) + c[6] * (U(sx) + U(-(long)sx)) + c[7] * (U(2 * sx) + U(-2 * (long)sx)) +
-                     c[8] * (U(3 * sx) + U(-3 * (long)sx)) + c[9] * (U(4 * sx) + U(-4 * (long)sx));
-    const float ty = c[10] * (U(0)) + c[11] * (U(sy) + U(-(long)sy)) + c[12] * (U(2 * sy) + U(-2 * (long)sy)) +
-                     c[13] * (U(3 * sy) + U(-3 * (long)sy)) + c[14] * (U(4 * sy) + U(-4 * (long)sy));
-#undef U
-    u3[q] = v2[q] * (tx + tz + ty);
-}
-
 // Injection_sinc -> Injection_source_sinc_xyz (Injection.f90:151-194,
 // :274-346, shot_type 0) of one or more traces: one thread per footprint cell
 // of the plan, which lists the cell's contributions in trace order (the
@@ -233,26 +213,34 @@ __global__ void inject3_kernel(float *__restrict__ u3, const size_t *__restrict_
 
 // One step of the constant-density scheme fused into one pass over the FD
 // region: the 25-point stencil u3 = v^2 (tx + tz + ty) (FDderivatives.f90:
-// 588-657 with the 3D forward grid), the injection of the traces whose
-// footprints hold the cell (Injection_sinc, in trace order, as inject3_kernel)
-// and the time update (FDderivatives.f90:960-982), with the arithmetic of
-// stencil3_kernel, inject3_kernel and update3_kernel in the same order, so the
+// 588-657 with the 3D forward grid) or, with F3_ADJ, the adjoint-style
+// stencil of v^2 u2 (FD_2nd_3D_derivatives_scalar_adjoint_grid,
+// FDderivatives.f90:780-819), the injection of the traces whose footprints
+// hold the cell (Injection_sinc or Injection_LSRTM_sinc, in trace order, as
+// inject3_kernel), with F3_BORN Injection_Born's 2 t m S / v on the window
+// (Imaging.f90:747-788), and the time update (FDderivatives.f90:960-982),
+// each with the arithmetic of the separate kernels in the same order, so the
 // result is the same to the bit.  A workgroup owns a 64 (z) x 4 CPT (x) tile
-// and walks yb planes along y: each lane keeps u2 of its row for its CPT
-// columns and the 4 planes either side in registers (the y-derivative) and
-// stages the plane in LDS with a 4-cell ring (the z- and x-derivatives); the
-// next plane's ring, centre and update operands are loaded while the current
-// plane is computed.  Cells inside the injection box look up their line in
-// colptr (the injection cells per (x, y) line of the padded grid).
+// and walks yb planes along y: each lane keeps the stencil's operand (u2, or
+// v^2 u2) of its row for its CPT columns and the 4 planes either side in
+// registers (the y-derivative) and stages the plane in LDS with a 4-cell
+// ring (the z- and x-derivatives); the next plane's ring, centre and update
+// operands are loaded while the current plane is computed.  Cells inside the
+// injection box look up their line in colptr (the injection cells per (x, y)
+// line of the padded grid).
 constexpr int F3_TZ = 64, F3_LZ = F3_TZ + 8;
 struct Box3 { int z0, z1, x0, x1, y0, y1; };   // padded cells holding injections (empty: z0 > z1)
-template <int F3_CPT>
+struct Born3 {                                  // Injection_Born's operands (F3_BORN)
+    const float *m, *vel, *S, *tx, *ty;         // padded reflectivity and velocity, window snapshot, tapers
+    int nz, nxw, nyw, i0;
+};
+template <int F3_CPT, bool F3_ADJ, bool F3_BORN>
 __global__ void __launch_bounds__(256) step3_fused_kernel(
     const float *__restrict__ um, const float *__restrict__ u0, const float *__restrict__ u1,
     const float *__restrict__ u2, float *__restrict__ u3, const float *__restrict__ v2, Geo G, Coef3 C,
     float c12dt2, float div11, int yb, Box3 box, const int *__restrict__ colptr, const size_t *__restrict__ cell,
     const int *__restrict__ cptr, const InjC *__restrict__ con, const float *__restrict__ v2r,
-    const float *__restrict__ traces, int nt, int it) {
+    const float *__restrict__ traces, int nt, int it, Born3 born) {
     constexpr int TX = 4 * F3_CPT, LX = TX + 8, NRING = 8 * TX + 8 * F3_LZ, RPT = (NRING + 255) / 256;
     __shared__ float sh[LX * F3_LZ];   // [column][row], rows iz0-4 .. iz0+67, columns ix0-4 .. ix0+TX+3
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -268,6 +256,8 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
         ok[c] = zok && ix <= G.NX - 5;
         ixc[c] = min(ix, G.NX - 1);
     }
+    // the stencil's operand at a cell: u2, or v^2 u2 for the adjoint-style stencil
+    auto opnd = [&](size_t a) -> float { return F3_ADJ ? v2[a] * u2[a] : u2[a]; };
     // this thread's ring cells: 8 rows x TX columns above and below, 8 columns x 72 rows beside
     int rl[RPT];
     size_t rg[RPT];
@@ -288,20 +278,21 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
         rg[r] = G.at(min(iz0 - 4 + row, G.NZ - 1), min(ix0 - 4 + col, G.NX - 1), 0);
     }
     // prologue: the queue, then plane iy0's centre, ring and update operands
+    // (vv: v^2 for the forward stencil's product, u2 itself for the adjoint's update)
     float q[F3_CPT][9], nq[F3_CPT], vv[F3_CPT], mm[F3_CPT], z0[F3_CPT], z1[F3_CPT], rv[RPT];
 #pragma unroll
     for (int c = 0; c < F3_CPT; c++) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) q[c][m] = u2[G.at(izl, ixc[c], iy0 - 4 + m)];
-        nq[c] = u2[G.at(izl, ixc[c], iy0 + 4)];
+        for (int m = 0; m < 8; m++) q[c][m] = opnd(G.at(izl, ixc[c], iy0 - 4 + m));
+        nq[c] = opnd(G.at(izl, ixc[c], iy0 + 4));
         const size_t a = G.at(izl, ixc[c], iy0);
-        vv[c] = v2[a];
+        vv[c] = F3_ADJ ? u2[a] : v2[a];
         mm[c] = um[a];
         z0[c] = u0[a];
         z1[c] = u1[a];
     }
 #pragma unroll
-    for (int r = 0; r < RPT; r++) rv[r] = rl[r] >= 0 ? u2[rg[r] + (size_t)iy0 * G.plane] : 0.f;
+    for (int r = 0; r < RPT; r++) rv[r] = rl[r] >= 0 ? opnd(rg[r] + (size_t)iy0 * G.plane) : 0.f;
     const float *c = C.c;
     const bool box_z = iz >= box.z0 && iz <= box.z1;
     for (int iy = iy0; iy < iy1; iy++) {
@@ -321,16 +312,16 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
         if (more) {
 #pragma unroll
             for (int k = 0; k < F3_CPT; k++) {
-                nq[k] = u2[G.at(izl, ixc[k], iy + 5)];
+                nq[k] = opnd(G.at(izl, ixc[k], iy + 5));
                 const size_t a = G.at(izl, ixc[k], iy + 1);
-                nv[k] = v2[a];
+                nv[k] = F3_ADJ ? u2[a] : v2[a];
                 nm[k] = um[a];
                 n0[k] = u0[a];
                 n1[k] = u1[a];
             }
 #pragma unroll
             for (int r = 0; r < RPT; r++)
-                if (rl[r] >= 0) rv[r] = u2[rg[r] + (size_t)(iy + 1) * G.plane];
+                if (rl[r] >= 0) rv[r] = opnd(rg[r] + (size_t)(iy + 1) * G.plane);
         }
 #pragma unroll
         for (int k = 0; k < F3_CPT; k++) {
@@ -347,7 +338,7 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
                 const float ty = c[10] * (q[k][4]) + c[11] * (q[k][5] + q[k][3]) + c[12] * (q[k][6] + q[k][2]) +
                                  c[13] * (q[k][7] + q[k][1]) + c[14] * (q[k][8] + q[k][0]);
                 const size_t a = G.at(iz, ixc[k], iy);
-                float u = vv[k] * (tx + tz + ty);
+                float u = F3_ADJ ? tx + tz + ty : vv[k] * (tx + tz + ty);
                 if (box_z && ixc[k] >= box.x0 && ixc[k] <= box.x1 && iy >= box.y0 && iy <= box.y1) {
                     const int l = iy * G.NX + ixc[k];
                     for (int e = colptr[l]; e < colptr[l + 1]; e++) {
@@ -358,7 +349,16 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
                         }
                     }
                 }
-                u3[a] = (20.f * q[k][4] - 6.f * z1[k] - 4.f * z0[k] + mm[k] + c12dt2 * u) * div11;
+                if constexpr (F3_BORN) {
+                    const int i = iz - born.i0, j = ixc[k] - born.i0, l = iy - born.i0;
+                    if (i >= 0 && i < born.nz && j >= 0 && j < born.nxw && l >= 0 && l < born.nyw) {
+                        const size_t wq = (size_t)i + (size_t)j * born.nz + (size_t)l * born.nz * born.nxw;
+                        const float t = born.tx[j] * born.ty[l];
+                        u = u + 2.f * t * born.m[a] * born.S[wq] / born.vel[a];
+                    }
+                }
+                const float u2c = F3_ADJ ? vv[k] : q[k][4];
+                u3[a] = (20.f * u2c - 6.f * z1[k] - 4.f * z0[k] + mm[k] + c12dt2 * u) * div11;
             }
 #pragma unroll
             for (int m = 0; m < 8; m++) q[k][m] = q[k][m + 1];
@@ -452,33 +452,6 @@ __global__ void rhovp3_image_kernel(const float *__restrict__ u2, const float *_
     ill[q] = ill[q] + S[q] * S[q] * t;
 }
 
-// FDderivatives.f90:780-819 FD_2nd_3D_derivatives_scalar_adjoint_grid, first
-// half: tmp = vel^2 u2 over the whole padded array.
-__global__ void __launch_bounds__(256) v2u3_kernel(const float *__restrict__ u2, const float *__restrict__ v2,
-                                                  float *__restrict__ tmp, size_t n) {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) tmp[q] = v2[q] * u2[q];
-}
-// second half: u3 = (tx + tz) + ty of the 25-point stencil of tmp, no v^2 after.
-__global__ void __launch_bounds__(256) stencil3_adj_kernel(const float *__restrict__ tmp, float *__restrict__ u3,
-                                                           Geo G, Coef3 C) {
-    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
-    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
-    const size_t q = G.at(iz, ix, iy);
-    const long sx = (long)G.LDZ, sy = (long)G.plane;
-    const float *c = C.c;
-#define T(o) tmp[q + (o)]
-    const float tz = c[0] * T(0) + c[1] * (T(1) + T(-1)) + c[2] * (T(2) + T(-2)) + c[3] * (T(3) + T(-3)) +
-                     c[4] * (T(4) + T(-4));
-    const float tx = c[5] * T(0) + c[6] * (T(sx) + T(-sx)) + c[7] * (T(2 * sx) + T(-2 * sx)) +
-                     c[8] * (T(3 * sx) + T(-3 * sx)) + c[9] * (T(4 * sx) + T(-4 * sx));
-    const float ty = c[10] * T(0) + c[11] * (T(sy) + T(-sy)) + c[12] * (T(2 * sy) + T(-2 * sy)) +
-                     c[13] * (T(3 * sy) + T(-3 * sy)) + c[14] * (T(4 * sy) + T(-4 * sy));
-#undef T
-    u3[q] = tx + tz + ty;
-}
-
 // Imaging_condition_LSRTM_sourceonly_from_memory (Imaging.f90:698-745) in 3D:
 // image += 2 t u S / v, illum += S S t.
 __global__ void lsrtm3_image_kernel(const float *__restrict__ u2, const float *__restrict__ vel, Geo G, int nz,
@@ -490,17 +463,6 @@ __global__ void lsrtm3_image_kernel(const float *__restrict__ u2, const float *_
     const float t = tx[j] * ty[k];
     img[q] = img[q] + 2.f * t * u2[a] * S[q] / vel[a];
     ill[q] = ill[q] + S[q] * S[q] * t;
-}
-
-// Injection_Born (Imaging.f90:747-788) in 3D on u3: u3 += 2 t m S / v.
-__global__ void born3_kernel(float *__restrict__ u3, const float *__restrict__ m, const float *__restrict__ vel, Geo G,
-                             int nz, int nxw, int i0, const float *__restrict__ S, const float *__restrict__ tx,
-                             const float *__restrict__ ty) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i >= nz) return;
-    const size_t q = (size_t)i + (size_t)j * nz + (size_t)k * nz * nxw, a = G.at(i + i0, j + i0, k + i0);
-    const float t = tx[j] * ty[k];
-    u3[a] = u3[a] + 2.f * t * m[a] * S[q] / vel[a];
 }
 
 // Higdon_factor (Boundary.f90:366-447) at cell q along a line of stride s
@@ -996,10 +958,11 @@ static void step3(sfd_ctx3d &h, Run3 &R, float **buf, const InjDev &I, int it) {
         // for 8 planes and 349 for 4 columns per lane)
         const int yb = kFusedPlanes;
         const dim3 fg((R.fz + F3_TZ - 1) / F3_TZ, (R.fx + 7) / 8, (R.fy + yb - 1) / yb);
-        hipLaunchKernelGGL(step3_fused_kernel<2>, fg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
-                           (const float *)buf[2], (const float *)buf[3], buf[4], (const float *)R.v2, G, R.C, c12dt2,
-                           div11, yb, I.ncell > 0 ? I.box : Box3{1, 0, 1, 0, 1, 0}, I.colptr, I.cell, I.cptr, I.con,
-                           I.v2r, I.traces, I.nt, it);
+        hipLaunchKernelGGL((step3_fused_kernel<2, false, false>), fg, sb, 0, h.st, (const float *)buf[0],
+                           (const float *)buf[1], (const float *)buf[2], (const float *)buf[3], buf[4],
+                           (const float *)R.v2, G, R.C, c12dt2, div11, yb,
+                           I.ncell > 0 ? I.box : Box3{1, 0, 1, 0, 1, 0}, I.colptr, I.cell, I.cptr, I.con, I.v2r,
+                           I.traces, I.nt, it, Born3{});
     }
     hipLaunchKernelGGL(zband3s_kernel, dim3((R.fx + 7) / 8, R.fy), dim3(256), 0, h.st, buf[4], (const float *)buf[3],
                        (const float *)buf[2], (const float *)R.gz, G);
@@ -1384,13 +1347,12 @@ static void lsrtm3(sfd_ctx3d &h, const sfd_shot3d &s, const float *traces_in, fl
     const size_t nwin = (size_t)g.nz * R.nxw * R.nyw;
     Layout E;
     const size_t o_snap = E.take(nwin * std::max(ntsnap, 1) * 4), o_img = E.take(nwin * 4), o_ill = E.take(nwin * 4),
-                 o_tx = E.take(R.nxw * 4), o_ty = E.take(R.nyw * 4), o_tmp = E.take(R.G.plane * R.G.NY * 4),
+                 o_tx = E.take(R.nxw * 4), o_ty = E.take(R.nyw * 4),
                  o_src = E.take(inj_bytes(src, s.nt)), o_rec = E.take(inj_bytes(recp, (size_t)s.ntr * s.nt));
     size_t extra_off = 0;
     alloc3(h, R, E.off, &extra_off);
     char *ex = R.base + extra_off;
     float *snaps = (float *)(ex + o_snap), *img = (float *)(ex + o_img), *ill = (float *)(ex + o_ill);
-    float *tmp = (float *)(ex + o_tmp);
     HIPCHK(hipMemsetAsync(img, 0, nwin * 4, h.st));
     HIPCHK(hipMemsetAsync(ill, 0, nwin * 4, h.st));
     tapers3(h, R, ex + o_tx, ex + o_ty);
@@ -1399,7 +1361,8 @@ static void lsrtm3(sfd_ctx3d &h, const sfd_shot3d &s, const float *traces_in, fl
     if (h.d_velb) models3(h, R, h.d_velb);
     start_pass3(h, R, false);
     float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
-    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw), sg((R.fz + 63) / 64, (R.fx + 3) / 4, R.fy), sb(256);
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw), sb(256);
+    const dim3 fg((R.fz + F3_TZ - 1) / F3_TZ, (R.fx + 7) / 8, (R.fy + kFusedPlanes - 1) / kFusedPlanes);
     const float c12dt2 = 12.f * (g.dt * g.dt), div11 = 1.f / 11.f;
     int counter = 0;
     for (int it = s.nt; it >= 1; it--) {
@@ -1412,14 +1375,11 @@ static void lsrtm3(sfd_ctx3d &h, const sfd_shot3d &s, const float *traces_in, fl
                                (const float *)(ex + o_ty), img, ill);
         }
         // the adjoint-style stencil (FDderivatives.f90:780-819), then injection, update, bands
-        hipLaunchKernelGGL(v2u3_kernel, dim3((unsigned)((R.nf + 255) / 256)), dim3(256), 0, h.st,
-                           (const float *)buf[3], (const float *)R.v2, tmp, R.nf);
-        hipLaunchKernelGGL(stencil3_adj_kernel, sg, sb, 0, h.st, (const float *)tmp, buf[4], R.G, R.C);
-        if (Ir.ncell > 0)
-            hipLaunchKernelGGL(inject3_kernel, dim3((Ir.ncell + 255) / 256), dim3(256), 0, h.st, buf[4], Ir.cell,
-                               Ir.cptr, Ir.con, Ir.v2r, Ir.traces, Ir.nt, Ir.ncell, it);
-        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
-                           (const float *)buf[2], (const float *)buf[3], buf[4], R.G, c12dt2, div11);
+        hipLaunchKernelGGL((step3_fused_kernel<2, true, false>), fg, sb, 0, h.st, (const float *)buf[0],
+                           (const float *)buf[1], (const float *)buf[2], (const float *)buf[3], buf[4],
+                           (const float *)R.v2, R.G, R.C, c12dt2, div11, kFusedPlanes,
+                           Ir.ncell > 0 ? Ir.box : Box3{1, 0, 1, 0, 1, 0}, Ir.colptr, Ir.cell, Ir.cptr, Ir.con,
+                           Ir.v2r, Ir.traces, Ir.nt, it, Born3{});
         hipLaunchKernelGGL(zband3s_kernel, dim3((R.fx + 7) / 8, R.fy), dim3(256), 0, h.st, buf[4],
                            (const float *)buf[3], (const float *)buf[2], (const float *)R.gz, R.G);
         hipLaunchKernelGGL(xband3_kernel, dim3((R.fz + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4],
@@ -1491,19 +1451,27 @@ static void born3(sfd_ctx3d &h, const sfd_shot3d &s, const float *d_refl, float 
     if (h.d_velb) models3(h, R, h.d_velb);
     start_pass3(h, R, false);
     float *buf[5] = {R.u[0], R.u[1], R.u[2], R.u[3], R.u[4]};
-    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw), sg((R.fz + 63) / 64, (R.fx + 3) / 4, R.fy), sb(256);
+    const dim3 wg((g.nz + 255) / 256, R.nxw, R.nyw), sb(256);
+    const dim3 fg((R.fz + F3_TZ - 1) / F3_TZ, (R.fx + 7) / 8, (R.fy + kFusedPlanes - 1) / kFusedPlanes);
     const float c12dt2 = 12.f * (g.dt * g.dt), div11 = 1.f / 11.f;
     int counter = 0;
     for (int it = 1; it <= s.nt; it++) {
         extract3(h.st, G, buf[3], (const Rec3 *)(ex + o_rec), s.ntr, 0, trc, s.nt, it);
-        hipLaunchKernelGGL(stencil3_kernel, sg, sb, 0, h.st, (const float *)buf[3], (const float *)R.v2, buf[4], G,
-                           R.C);
-        if (it % g.snapi == 0)
-            hipLaunchKernelGGL(born3_kernel, wg, dim3(256), 0, h.st, buf[4], (const float *)m, (const float *)R.vel, G,
-                               g.nz, R.nxw, R.i0, (const float *)(snaps + (size_t)(counter++) * nwin),
-                               (const float *)(ex + o_tx), (const float *)(ex + o_ty));
-        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
-                           (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
+        if (it % g.snapi == 0) {   // Injection_Born after the stencil, in the same pass
+            const Born3 bo{(const float *)m, (const float *)R.vel, (const float *)(snaps + (size_t)(counter++) * nwin),
+                           (const float *)(ex + o_tx), (const float *)(ex + o_ty), g.nz, R.nxw, R.nyw, R.i0};
+            hipLaunchKernelGGL((step3_fused_kernel<2, false, true>), fg, sb, 0, h.st, (const float *)buf[0],
+                               (const float *)buf[1], (const float *)buf[2], (const float *)buf[3], buf[4],
+                               (const float *)R.v2, G, R.C, c12dt2, div11, kFusedPlanes, Box3{1, 0, 1, 0, 1, 0},
+                               (const int *)nullptr, (const size_t *)nullptr, (const int *)nullptr,
+                               (const InjC *)nullptr, (const float *)nullptr, (const float *)nullptr, 0, it, bo);
+        } else {
+            hipLaunchKernelGGL((step3_fused_kernel<2, false, false>), fg, sb, 0, h.st, (const float *)buf[0],
+                               (const float *)buf[1], (const float *)buf[2], (const float *)buf[3], buf[4],
+                               (const float *)R.v2, G, R.C, c12dt2, div11, kFusedPlanes, Box3{1, 0, 1, 0, 1, 0},
+                               (const int *)nullptr, (const size_t *)nullptr, (const int *)nullptr,
+                               (const InjC *)nullptr, (const float *)nullptr, (const float *)nullptr, 0, it, Born3{});
+        }
         hipLaunchKernelGGL(zband3s_kernel, dim3((R.fx + 7) / 8, R.fy), dim3(256), 0, h.st, buf[4],
                            (const float *)buf[3], (const float *)buf[2], (const float *)R.gz, G);
         hipLaunchKernelGGL(xband3_kernel, dim3((R.fz + 63) / 64, R.fy), dim3(64), 0, h.st, buf[4],
