@@ -257,7 +257,8 @@ def forward3d_leg(sa, n=300, nt=400):
     algorithmic bytes at constant density: 24 per FD point (the fused step
     kernel reads u_-1, u0, u1, u2, v^2 and writes u3); achieved = those bytes
     over the whole call's wall time divided by nt (bands, extraction and
-    set-up included)."""
+    set-up included); then one RTM of the same shot with snapshots every 4
+    steps (both passes counted)."""
     from seismic_amd.synthetic import ricker
     g = sa.make_grid3d(n, n, n, 10.0, 10.0, 10.0, 1e-3, ntaper=20, aperture_x=10.0 * (n - 1),
                        aperture_y=10.0 * (n - 1), fmax=0.0)
@@ -281,6 +282,19 @@ def forward3d_leg(sa, n=300, nt=400):
     ach = 24.0 * F * nt / (out["constant_density"]["us_per_step"] * nt * 1e-6) / 1e9
     out["constant_density"].update({"algorithmic_bytes_per_point": 24.0, "achieved_GBps": round(ach, 1),
                                     "frac": round(ach / HBM_PEAK_GBS, 4)})
+    e.close()
+    # 3D RTM of the same shot (constant density, snapshots every 4 steps on
+    # the window): forward pass with snapshots, reverse pass with imaging
+    g4 = sa.make_grid3d(n, n, n, 10.0, 10.0, 10.0, 1e-3, ntaper=20, aperture_x=10.0 * (n - 1),
+                        aperture_y=10.0 * (n - 1), fmax=0.0, snapi=4)
+    e = sa.Engine3D(g4)
+    e.set_model(vel)
+    tr = e.forward([shot])
+    e.rtm([shot], tr)
+    t0 = time.perf_counter()
+    e.rtm([shot], tr)
+    wall = time.perf_counter() - t0
+    out["rtm_snapi4"] = {"Gpts_per_s": round(2 * F * nt / wall / 1e9, 3), "ms_per_rtm": round(wall * 1e3, 1)}
     e.close()
     return out
 

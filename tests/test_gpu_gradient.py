@@ -42,3 +42,26 @@ def test_device_gradient_matches_finite_differences(snap_mode, tol):
         P = float(np.sum(grad.astype(np.float64) * dv))
         print("bump (%d,%d): dJ = %.6e, <grad, dv> = %.6e, ratio %.6f" % (cz, cx, D, P, D / P), flush=True)
         assert abs(D / P - want) <= tol * abs(want)
+
+
+def test_gradient_without_residual_download_is_identical():
+    """residuals=False leaves the residual gathers on the device (as the
+    evaluations of sfd_fwi_lbfgs do): misfit, gradient and illumination are
+    the same to the bit (20 ragged shots, planned on several host threads)."""
+    print("case: gradient without residual download", flush=True)
+    nz, nx, nt = 40, 96, 300
+    e = sa.Engine(sa.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12, snapi=2))
+    v0 = smooth_random(nz, nx, seed=4, vmin=1800, vmax=2400).astype(np.float32)
+    w = ricker(15.0, nt, 1e-3)
+    shots = []
+    for k, sx in enumerate(np.linspace(100.0, 850.0, 20)):
+        rx = (np.arange(2 + k % 3, nx - 2, 3) * 10.0).astype(np.float32)
+        shots.append(sa.Shot(20.0, float(sx), w[: nt - 7 * k], np.full(len(rx), 20.0, np.float32), rx))
+    e.set_model((v0 * 1.03).astype(np.float32))
+    dobs = e.forward(shots, extract_kind=1)
+    e.set_model(v0)
+    f1, g1, i1, r1 = e.fwi_gradient(shots, dobs, snap_mode=1)
+    f2, g2, i2, r2 = e.fwi_gradient(shots, dobs, snap_mode=1, residuals=False)
+    assert r2 is None and len(r1) == len(shots)
+    assert f1 == f2 and np.array_equal(g1, g2) and np.array_equal(i1, i2)
+    assert np.abs(g1).max() > 0 and all(np.abs(r).max() > 0 for r in r1)
