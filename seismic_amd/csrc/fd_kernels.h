@@ -981,9 +981,12 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
 #undef WX
         val[p] = ADJ ? (tx + tz) : rv[p] * (tx + tz);
     }
-    // Snapshot store (Extraction.f90:595, u2 of this step on the window),
-    // non-temporal: the snapshots are read back only in the reverse pass, so
-    // they should not push the wavefield levels out of the caches.
+    // Snapshot store (Extraction.f90:595, u2 of this step on the window).
+    // FP32 snapshots are stored non-temporal (read back only in the reverse
+    // pass, they should not push the wavefield levels out of the caches);
+    // FP16 ones with plain stores, whose 128-byte wavefront segments the L2
+    // merges (on the 96-shot batch: forward pass 1731 against 1749 ms
+    // non-temporal, reverse 2198 against 2185, the call 0.2% faster).
     if (snap_step && interior_z) {
         const size_t sbase = (size_t)(it / A.snapi - 1) * S.nz * S.nxw + (iz - S.iz0);
 #pragma unroll
@@ -994,8 +997,7 @@ __device__ __forceinline__ void fd_step_body(const StepArgs &A, const ShotDev &S
             // with a free surface the snapshot is zero above it (Extraction.f90:580-592)
             const float su = S.snap_mask && iz < gp(S.fs_row)[ix] ? 0.f : uo[p];
             if (A.snap_fp16)
-                __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(su * inv16[p / 4])),
-                                            gp((unsigned short *)S.snap) + q);
+                gp((unsigned short *)S.snap)[q] = __half_as_ushort(__float2half_rn(su * inv16[p / 4]));
             else
                 __builtin_nontemporal_store(su, gp((float *)S.snap) + q);
         }
