@@ -244,8 +244,17 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
     constexpr int TX = 4 * F3_CPT, LX = TX + 8, NRING = 8 * TX + 8 * F3_LZ, RPT = (NRING + 255) / 256;
     __shared__ float sh[LX * F3_LZ];   // [column][row], rows iz0-4 .. iz0+67, columns ix0-4 .. ix0+TX+3
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int iz0 = 4 + (int)blockIdx.x * F3_TZ, ix0 = 4 + (int)blockIdx.y * TX;
-    const int iy0 = 4 + (int)blockIdx.z * yb, iy1 = min(iy0 + yb, G.NY - 4);
+    // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2:
+    // renumber them so every XCD takes a contiguous run of tiles (z fastest,
+    // then x, then y), whose rings and planes its L2 then shares.
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int nb = nbx * nby * (int)gridDim.z;
+    const int hw = (int)blockIdx.x + nbx * ((int)blockIdx.y + nby * (int)blockIdx.z);
+    const int xq = nb / 8, xr = nb % 8, xcd = hw % 8;
+    const int lb = xcd * xq + min(xcd, xr) + hw / 8;
+    const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
+    const int iz0 = 4 + bx * F3_TZ, ix0 = 4 + by * TX;
+    const int iy0 = 4 + bz * yb, iy1 = min(iy0 + yb, G.NY - 4);
     const int iz = iz0 + lane, izl = min(iz, G.NZ - 1);
     const bool zok = iz <= G.NZ - 5;
     int ixc[F3_CPT];
