@@ -142,27 +142,6 @@ __global__ void __launch_bounds__(256) rho_grad3_kernel(const float *__restrict_
 #undef U
 }
 
-// FDderivatives.f90:263-382, second half: u3 = v^2 rho ((tx + ty) + tz) with
-// the staggered divergence of the gradients, on the FD region.
-__global__ void __launch_bounds__(256) stencil3_rho_kernel(const float *__restrict__ sz, const float *__restrict__ sx,
-                                                           const float *__restrict__ sy, const float *__restrict__ v2,
-                                                           const float *__restrict__ rho, float *__restrict__ u3,
-                                                           Geo G, Coef3 C) {
-    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
-    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
-    const size_t q = G.at(iz, ix, iy);
-    const long X = (long)G.LDZ, Y = (long)G.plane;
-    const float *c = C.c;
-    const float tz = c[1] * (sz[q] - sz[q - 1]) + c[2] * (sz[q + 1] - sz[q - 2]) + c[3] * (sz[q + 2] - sz[q - 3]) +
-                     c[4] * (sz[q + 3] - sz[q - 4]);
-    const float tx = c[6] * (sx[q] - sx[q - X]) + c[7] * (sx[q + X] - sx[q - 2 * X]) +
-                     c[8] * (sx[q + 2 * X] - sx[q - 3 * X]) + c[9] * (sx[q + 3 * X] - sx[q - 4 * X]);
-    const float ty = c[11] * (sy[q] - sy[q - Y]) + c[12] * (sy[q + Y] - sy[q - 2 * Y]) +
-                     c[13] * (sy[q + 2 * Y] - sy[q - 3 * Y]) + c[14] * (sy[q + 3 * Y] - sy[q - 4 * Y]);
-    u3[q] = v2[q] * rho[q] * (tx + ty + tz);
-}
-
 // Boundary.f90:36-60 Higdon tables: gz per (x, y) of the FD region from the
 // top / bottom FD rows, gx per (z, y) from the left / right FD columns, gy
 // per (z, x) from the front / back FD lines.
@@ -189,38 +168,23 @@ __global__ void higdon3_kernel(const float *__restrict__ vel, Geo G, float dt, f
     }
 }
 
-
 // Injection_sinc -> Injection_source_sinc_xyz (Injection.f90:151-194,
-// :274-346, shot_type 0) of one or more traces: one thread per footprint cell
-// of the plan, which lists the cell's contributions in trace order (the
-// reference adds trace after trace); each is u3 += (((v2r s) wz) wx) wy, v2r
-// the trace's v^2 (times rho) and the y weight the x sinc, as the reference
-// computes them.
+// :274-346, shot_type 0) of one or more traces, as the fused step kernels
+// apply it: the plan lists each footprint cell's contributions in trace
+// order (the reference adds trace after trace); each is u3 += (((v2r s) wz)
+// wx) wy, v2r the trace's v^2 (times rho) and the y weight the x sinc, as the
+// reference computes them.
 struct InjC { int tr; float wz, wx, wy; };
-__global__ void inject3_kernel(float *__restrict__ u3, const size_t *__restrict__ cell, const int *__restrict__ cptr,
-                               const InjC *__restrict__ con, const float *__restrict__ v2r,
-                               const float *__restrict__ traces, int nt, int ncell, int it) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const size_t q = cell[c];
-    float u = u3[q];
-    for (int k = cptr[c]; k < cptr[c + 1]; k++) {
-        const InjC e = con[k];
-        u = u + v2r[e.tr] * traces[(size_t)e.tr * nt + (it - 1)] * e.wz * e.wx * e.wy;
-    }
-    u3[q] = u;
-}
 
 // One step of the constant-density scheme fused into one pass over the FD
 // region: the 25-point stencil u3 = v^2 (tx + tz + ty) (FDderivatives.f90:
 // 588-657 with the 3D forward grid) or, with F3_ADJ, the adjoint-style
 // stencil of v^2 u2 (FD_2nd_3D_derivatives_scalar_adjoint_grid,
 // FDderivatives.f90:780-819), the injection of the traces whose footprints
-// hold the cell (Injection_sinc or Injection_LSRTM_sinc, in trace order, as
-// inject3_kernel), with F3_BORN Injection_Born's 2 t m S / v on the window
+// hold the cell (Injection_sinc or Injection_LSRTM_sinc, in trace order),
+// with F3_BORN Injection_Born's 2 t m S / v on the window
 // (Imaging.f90:747-788), and the time update (FDderivatives.f90:960-982),
-// each with the arithmetic of the separate kernels in the same order, so the
-// result is the same to the bit.  A workgroup owns a 64 (z) x 4 CPT (x) tile
+// in the reference's order of operations.  A workgroup owns a 64 (z) x 4 CPT (x) tile
 // and walks yb planes along y: each lane keeps the stencil's operand (u2, or
 // v^2 u2) of its row for its CPT columns and the 4 planes either side in
 // registers (the y-derivative) and stages the plane in LDS with a 4-cell
@@ -382,6 +346,164 @@ __global__ void __launch_bounds__(256) step3_fused_kernel(
     }
 }
 
+// The variable-density step after rho_grad3_kernel, fused into one pass over
+// the FD region: the staggered divergence u3 = v^2 rho ((tx + ty) + tz) of
+// the gradients sz, sx, sy (FDderivatives.f90:263-382), the injection
+// (Injection.f90:151-194, in trace order) and the time update
+// (FDderivatives.f90:960-982), in the reference's order of operations.  Same walk as step3_fused_kernel: a 64 (z) x
+// 4 CPT (x) tile walks yb planes along y, sy of the planes iy-4 .. iy+3 in
+// registers, sz of the plane in LDS with 4 rows above and below, sx with 4
+// columns each side; the next plane's loads are issued before the current
+// plane is computed.
+template <int F3_CPT>
+__global__ void __launch_bounds__(256) step3_rho_fused_kernel(
+    const float *__restrict__ um, const float *__restrict__ u0, const float *__restrict__ u1,
+    const float *__restrict__ u2, float *__restrict__ u3, const float *__restrict__ sz, const float *__restrict__ sx,
+    const float *__restrict__ sy, const float *__restrict__ v2, const float *__restrict__ rho, Geo G, Coef3 C,
+    float c12dt2, float div11, int yb, Box3 box, const int *__restrict__ colptr, const size_t *__restrict__ cell,
+    const int *__restrict__ cptr, const InjC *__restrict__ con, const float *__restrict__ v2r,
+    const float *__restrict__ traces, int nt, int it) {
+    constexpr int TX = 4 * F3_CPT, ZL = F3_LZ, XL = TX + 8;
+    constexpr int NZR = 8 * TX, NXR = 8 * F3_TZ, NRING = NZR + NXR, RPT = (NRING + 255) / 256;
+    __shared__ float shz[TX * ZL];        // sz: [column][row], rows iz0-4 .. iz0+67
+    __shared__ float shx[XL * F3_TZ];     // sx: [column][row], columns ix0-4 .. ix0+TX+3
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int nb = nbx * nby * (int)gridDim.z;
+    const int hw = (int)blockIdx.x + nbx * ((int)blockIdx.y + nby * (int)blockIdx.z);
+    const int xq = nb / 8, xr = nb % 8, xcd = hw % 8;
+    const int lb = xcd * xq + min(xcd, xr) + hw / 8;
+    const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
+    const int iz0 = 4 + bx * F3_TZ, ix0 = 4 + by * TX;
+    const int iy0 = 4 + bz * yb, iy1 = min(iy0 + yb, G.NY - 4);
+    const int iz = iz0 + lane, izl = min(iz, G.NZ - 1);
+    const bool zok = iz <= G.NZ - 5;
+    int ixc[F3_CPT];
+    bool ok[F3_CPT];
+#pragma unroll
+    for (int c = 0; c < F3_CPT; c++) {
+        const int ix = ix0 + F3_CPT * w + c;
+        ok[c] = zok && ix <= G.NX - 5;
+        ixc[c] = min(ix, G.NX - 1);
+    }
+    // ring cells: sz 8 rows x TX columns above and below, sx 8 columns x 64 rows beside
+    int rl[RPT];
+    bool rz[RPT];
+    size_t rg[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        const int t = threadIdx.x + 256 * r;
+        int col = 0, row = 0;
+        rz[r] = t < NZR;
+        if (t < NZR) {
+            col = t / 8;
+            const int m = t % 8;
+            row = m < 4 ? m : F3_TZ + m;   // LDS rows 0..3 and 68..71
+            rl[r] = col * ZL + row;
+            rg[r] = G.at(min(iz0 - 4 + row, G.NZ - 1), min(ix0 + col, G.NX - 1), 0);
+        } else if (t < NRING) {
+            const int u = t - NZR, cc = u / F3_TZ;
+            col = cc < 4 ? cc : TX + cc;    // LDS columns 0..3 and TX+4..TX+7
+            row = u % F3_TZ;
+            rl[r] = col * F3_TZ + row;
+            rg[r] = G.at(min(iz0 + row, G.NZ - 1), min(ix0 - 4 + col, G.NX - 1), 0);
+        } else {
+            rl[r] = -1;
+            rg[r] = 0;
+        }
+    }
+    float q[F3_CPT][8], nq[F3_CPT], cz[F3_CPT], cx[F3_CPT], ncz[F3_CPT], ncx[F3_CPT];
+    float vr[F3_CPT], uu[F3_CPT], mm[F3_CPT], z0[F3_CPT], z1[F3_CPT], rv[RPT];
+#pragma unroll
+    for (int c = 0; c < F3_CPT; c++) {
+#pragma unroll
+        for (int m = 0; m < 7; m++) q[c][m] = sy[G.at(izl, ixc[c], iy0 - 4 + m)];
+        nq[c] = sy[G.at(izl, ixc[c], iy0 + 3)];
+        const size_t a = G.at(izl, ixc[c], iy0);
+        cz[c] = sz[a];
+        cx[c] = sx[a];
+        vr[c] = v2[a] * rho[a];
+        uu[c] = u2[a];
+        mm[c] = um[a];
+        z0[c] = u0[a];
+        z1[c] = u1[a];
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; r++)
+        rv[r] = rl[r] >= 0 ? (rz[r] ? sz : sx)[rg[r] + (size_t)iy0 * G.plane] : 0.f;
+    const float *c = C.c;
+    const bool box_z = iz >= box.z0 && iz <= box.z1;
+    for (int iy = iy0; iy < iy1; iy++) {
+#pragma unroll
+        for (int k = 0; k < F3_CPT; k++) {
+            q[k][7] = nq[k];
+            shz[(F3_CPT * w + k) * ZL + 4 + lane] = cz[k];
+            shx[(4 + F3_CPT * w + k) * F3_TZ + lane] = cx[k];
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; r++)
+            if (rl[r] >= 0) (rz[r] ? shz : shx)[rl[r]] = rv[r];
+        __syncthreads();
+        float nvr[F3_CPT], nuu[F3_CPT], nm[F3_CPT], n0[F3_CPT], n1[F3_CPT];
+        const bool more = iy + 1 < iy1;
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < F3_CPT; k++) {
+                nq[k] = sy[G.at(izl, ixc[k], iy + 4)];
+                const size_t a = G.at(izl, ixc[k], iy + 1);
+                ncz[k] = sz[a];
+                ncx[k] = sx[a];
+                nvr[k] = v2[a] * rho[a];
+                nuu[k] = u2[a];
+                nm[k] = um[a];
+                n0[k] = u0[a];
+                n1[k] = u1[a];
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; r++)
+                if (rl[r] >= 0) rv[r] = (rz[r] ? sz : sx)[rg[r] + (size_t)(iy + 1) * G.plane];
+        }
+#pragma unroll
+        for (int k = 0; k < F3_CPT; k++) {
+            if (ok[k]) {
+                const float *Z = shz + (F3_CPT * w + k) * ZL + 4 + lane;       // sz at row iz
+                const float *X = shx + (4 + F3_CPT * w + k) * F3_TZ + lane;    // sx at column ix
+                const float tz = c[1] * (Z[0] - Z[-1]) + c[2] * (Z[1] - Z[-2]) + c[3] * (Z[2] - Z[-3]) +
+                                 c[4] * (Z[3] - Z[-4]);
+                const float tx = c[6] * (X[0] - X[-F3_TZ]) + c[7] * (X[F3_TZ] - X[-2 * F3_TZ]) +
+                                 c[8] * (X[2 * F3_TZ] - X[-3 * F3_TZ]) + c[9] * (X[3 * F3_TZ] - X[-4 * F3_TZ]);
+                const float ty = c[11] * (q[k][4] - q[k][3]) + c[12] * (q[k][5] - q[k][2]) +
+                                 c[13] * (q[k][6] - q[k][1]) + c[14] * (q[k][7] - q[k][0]);
+                const size_t a = G.at(iz, ixc[k], iy);
+                float u = vr[k] * (tx + ty + tz);
+                if (box_z && ixc[k] >= box.x0 && ixc[k] <= box.x1 && iy >= box.y0 && iy <= box.y1) {
+                    const int l = iy * G.NX + ixc[k];
+                    for (int e = colptr[l]; e < colptr[l + 1]; e++) {
+                        if (cell[e] != a) continue;
+                        for (int m = cptr[e]; m < cptr[e + 1]; m++) {
+                            const InjC f = con[m];
+                            u = u + v2r[f.tr] * traces[(size_t)f.tr * nt + (it - 1)] * f.wz * f.wx * f.wy;
+                        }
+                    }
+                }
+                u3[a] = (20.f * uu[k] - 6.f * z1[k] - 4.f * z0[k] + mm[k] + c12dt2 * u) * div11;
+            }
+#pragma unroll
+            for (int m = 0; m < 7; m++) q[k][m] = q[k][m + 1];
+            if (more) {
+                cz[k] = ncz[k];
+                cx[k] = ncx[k];
+                vr[k] = nvr[k];
+                uu[k] = nuu[k];
+                mm[k] = nm[k];
+                z0[k] = n0[k];
+                z1[k] = n1[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Extraction_wavefield (Extraction.f90:562-601): u2 on the window (rows 1..nz,
 // window columns and lines) into snapshot slot s.
 __global__ void snap3_kernel(const float *__restrict__ u2, Geo G, int nz, int nxw, int i0,
@@ -402,17 +524,6 @@ __global__ void image3_kernel(const float *__restrict__ u2, Geo G, int nz, int n
     const float t = tx[j] * ty[k], S = snap[q];
     img[q] = img[q] + u2[G.at(i + i0, j + i0, k + i0)] * S * t;
     ill[q] = ill[q] + S * S * t;
-}
-
-// FDderivatives.f90:960-982 on the FD region
-__global__ void __launch_bounds__(256) update3_kernel(const float *__restrict__ um, const float *__restrict__ u0,
-                                                      const float *__restrict__ u1, const float *__restrict__ u2,
-                                                      float *__restrict__ u3, Geo G, float c12dt2, float div11) {
-    const int iz = 4 + (int)(blockIdx.x * 64 + (threadIdx.x & 63));
-    const int ix = 4 + (int)(blockIdx.y * 4 + (threadIdx.x >> 6)), iy = 4 + (int)blockIdx.z;
-    if (iz > G.NZ - 5 || ix > G.NX - 5) return;
-    const size_t q = G.at(iz, ix, iy);
-    u3[q] = (20.f * u2[q] - 6.f * u1[q] - 4.f * u0[q] + um[q] + c12dt2 * u3[q]) * div11;
 }
 
 // Imaging_condition_AFWI_from_memory (Imaging.f90:371-422) in 3D: image +=
@@ -948,19 +1059,19 @@ static InjDev upload_inj(sfd_ctx3d &h, char *mem, const InjPlan &P, const float 
 constexpr int kFusedPlanes = 4;   // y planes per workgroup of step3_fused_kernel
 static void step3(sfd_ctx3d &h, Run3 &R, float **buf, const InjDev &I, int it) {
     const Geo &G = R.G;
-    const dim3 sg((R.fz + 63) / 64, (R.fx + 3) / 4, R.fy), sb(256);
+    const dim3 sb(256);
     const float c12dt2 = 12.f * (h.g.dt * h.g.dt), div11 = 1.f / 11.f;
     if (R.rho) {
         hipLaunchKernelGGL(rho_grad3_kernel, dim3((G.NZ + 63) / 64, (G.NX + 3) / 4, G.NY), sb, 0, h.st,
                            (const float *)buf[3], (const float *)R.delp, R.sg[0], R.sg[1], R.sg[2], G, R.C, R.dzi,
                            R.dxi, R.dyi);
-        hipLaunchKernelGGL(stencil3_rho_kernel, sg, sb, 0, h.st, (const float *)R.sg[0], (const float *)R.sg[1],
-                           (const float *)R.sg[2], (const float *)R.v2, (const float *)R.rhop, buf[4], G, R.C);
-        if (I.ncell > 0)
-            hipLaunchKernelGGL(inject3_kernel, dim3((I.ncell + 255) / 256), dim3(256), 0, h.st, buf[4], I.cell,
-                               I.cptr, I.con, I.v2r, I.traces, I.nt, I.ncell, it);
-        hipLaunchKernelGGL(update3_kernel, sg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
-                           (const float *)buf[2], (const float *)buf[3], buf[4], G, c12dt2, div11);
+        const dim3 fg((R.fz + F3_TZ - 1) / F3_TZ, (R.fx + 7) / 8, (R.fy + kFusedPlanes - 1) / kFusedPlanes);
+        hipLaunchKernelGGL(step3_rho_fused_kernel<2>, fg, sb, 0, h.st, (const float *)buf[0], (const float *)buf[1],
+                           (const float *)buf[2], (const float *)buf[3], buf[4], (const float *)R.sg[0],
+                           (const float *)R.sg[1], (const float *)R.sg[2], (const float *)R.v2,
+                           (const float *)R.rhop, G, R.C, c12dt2, div11, kFusedPlanes,
+                           I.ncell > 0 ? I.box : Box3{1, 0, 1, 0, 1, 0}, I.colptr, I.cell, I.cptr, I.con, I.v2r,
+                           I.traces, I.nt, it);
     } else {
         // 2 columns per lane (5 waves per SIMD) and 4 planes per workgroup
         // measured fastest on a 340^3 FD region (337 us per step against 347
