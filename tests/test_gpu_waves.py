@@ -170,3 +170,27 @@ def test_strip_width_must_match_tile():
     ref = O.modeling(O.make_grid(nz, nx, 10.0, 10.0, 1e-3, ntaper=12), vel, O.Shot(20.0, 400.0, w, rz, rx),
                      extract_kind=0)
     assert np.array_equal(gpu, ref)
+
+
+def test_bad_shot_in_a_planned_batch_is_an_error_and_the_engine_recovers():
+    """Shots are planned on several host threads: a shot whose source
+    footprint leaves the FD region (here the eighth of twelve) makes the call
+    fail with SFD_EINVAL and the planner's message, and the engine then runs
+    the good shots exactly as a fresh engine does."""
+    say("case: bad shot in a 12-shot batch")
+    vel = smooth_random(40, 96, seed=6, vmin=1700, vmax=2500)
+    w = ricker(15.0, 250, 1e-3)
+    good = []
+    for k in range(12):
+        rx = (np.arange(2 + k % 3, 94, 3) * 10.0).astype(np.float32)
+        good.append(sa.Shot(20.0, 60.0 + 70.0 * k, w, np.full(len(rx), 20.0, np.float32), rx))
+    bad = list(good)
+    bad[7] = sa.Shot(-2000.0, good[7].sx, w, good[7].rz, good[7].rx)
+    e = engine(vel)
+    with pytest.raises(sa.SeisFDError) as ex:
+        e.forward(bad)
+    assert ex.value.code == abi.SFD_EINVAL and "footprint" in str(ex.value)
+    got = e.forward(good)
+    want = engine(vel).forward(good)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
