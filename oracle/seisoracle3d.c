@@ -918,7 +918,7 @@ static void fwd_snaps3(P3 *P, const sfd_grid3d *g, const sfd_shot3d *s, float ox
  * Injection_LSRTM_sinc of the receivers' traces and
  * Imaging_condition_LSRTM_sourceonly_from_memory (Imaging.f90:698-745:
  * image += 2 t u S / v, illum += S S t); the window added into image / illum
- * as mod_copy_image.  Constant density only (the driver has no density
+ * as mod_copy_image.  Constant density only (LSRTM_to_memory has no density
  * branch).  capture (optional): the backward u2 at each imaging step. */
 int orc_f_lsrtm3d(const sfd_grid3d *g, const float *vel_full, const float *vel_bwd, const sfd_shot3d *s,
                   const float *traces_in, float *image, float *illum, float *capture)
